@@ -1,0 +1,146 @@
+/*
+ * qjx.h -- C ABI of the MI355X quantum-jump trajectory engine for the driven,
+ * dissipative Bose-Hubbard dimer (os_d QJX, sys_id 0 / task_id 1 / prop_id 0).
+ *
+ * The reference runs this path as C++ strategy objects over raw arrays, one
+ * trajectory per OpenMP iteration (all citations are files of the reference
+ * repository, source/cpp/QJX/QJX/):
+ *   NewDelBehavior::init_sizes/init_hamiltonians/init_dissipators/init_hamiltonians_qj
+ *       (newdel.h:6-17, newdel.cpp:3-37, 450-498, 1454-1498, 2285-2388)  -> qjx_ctx_create
+ *   CoreBehavior::init_splits (core.h, core.cpp:6-22) + init_split_branches
+ *       (qj_proc.cpp:4-120)                                                -> qjx_ctx_create
+ *   ExpNewDelBehavior::init_data (newdel_exp.h:7-12, newdel_exp.cpp:98-117)-> qjx_ensemble_init
+ *   ExperimentBehavior::trans_process (experiment.h:8-13, experiment.cpp:411-446)
+ *                                                                          -> qjx_trans
+ *   PropagateBehavior::one_period (propagator.h:6-23, propagator.cpp:27-30)
+ *       over all trajectories, as in obser_process (experiment.cpp:481-494)-> qjx_run_periods
+ *   CoreBehavior::calc_chars_std(_start) (core.cpp:665-725)               -> qjx_chars
+ *   dump_adr_avg (dump.cpp:26-61)                                          -> qjx_chars(adr_avg)
+ *   ed->jump_times/diss_types/jump_norms/jump_etas (data.h, core.cpp:1065-1081)
+ *                                                                          -> qjx_get_jumps
+ *   main() + Processor::process (main.cpp:3-192, processor.cpp:50-75)     -> qjx_run_config
+ * The per-ensemble granularity replaces the per-trajectory calls: a GPU step
+ * advances every trajectory at once.
+ *
+ * Conventions: plain C types only; complex arrays are interleaved (re, im)
+ * doubles, row-major; every call returns QJX_OK or a negative code, and
+ * qjx_last_error() describes the last failure of the calling thread.  The
+ * reference prints and exits (utils.cpp:3-7) or throws (experiment.cpp:1250);
+ * this library returns the error instead.  One host thread per context.
+ */
+#ifndef QJX_H
+#define QJX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QJX_OK 0
+#define QJX_ERR_ARG (-1)    /* invalid argument / call order                        */
+#define QJX_ERR_HIP (-2)    /* HIP runtime failure                                  */
+#define QJX_ERR_NORM (-3)   /* ||psi||^2 > 1 after a step (reference: invalid_argument) */
+#define QJX_ERR_LOG (-4)    /* jump log overflow between two drains                 */
+#define QJX_ERR_IO (-5)     /* config / output file error                           */
+#define QJX_ERR_NOMEM (-6)  /* device memory exhausted                              */
+
+#define QJX_FP64 0          /* complex128 states and propagators, FP64 arithmetic   */
+#define QJX_FP32 1          /* complex64 states/propagators, FP32 GEMM; FP64 norms  */
+#define QJX_BF16X3 2        /* complex64 states, 3-term bf16 split GEMM; FP64 norms  */
+
+typedef struct qjx_ctx qjx_ctx;
+
+/* Dimer model; field names follow params.txt keys, with the reference meanings
+ * (dimer_prm_U enters as H_U = 2 (U/N) sum_j n_j (n_j - 1), newdel.cpp:459,484). */
+typedef struct qjx_model_desc {
+    int N;                  /* N                                                    */
+    double prm_E;           /* dimer_prm_E                                          */
+    double prm_U;           /* dimer_prm_U                                          */
+    double prm_J;           /* dimer_prm_J                                          */
+    double drv_ampl;        /* dimer_drv_ampl (square drive E +- A per half period) */
+    double drv_freq;        /* dimer_drv_freq (T = 2 pi / freq)                     */
+    double diss_gamma;      /* diss_gamma (rate gamma / N)                          */
+    int ex_deep;            /* ex_deep: levels of the bisection tree                */
+    int steps_per_period;   /* top-level steps per period, even; 4 = reference QJX  */
+    int precision;          /* QJX_FP64 / QJX_FP32 / QJX_BF16X3                     */
+} qjx_model_desc;
+
+typedef struct qjx_ensemble_desc {
+    int num_trajectories;   /* num_trajectories                                     */
+    int seed;               /* seed: stream of trajectory tr is leap-frog(seed + tr) */
+    int mns;                /* mns: leap-frog stride                                */
+    int start_type;         /* start_type: 0 = basis state, else uniform            */
+    int start_state;        /* start_state                                          */
+    int jump;               /* jump: 0 off, 1 times + types, 2 also norms + etas    */
+    int log_capacity;       /* device jump records per trajectory between drains (0 = default) */
+} qjx_ensemble_desc;
+
+typedef struct qjx_stats {
+    int64_t matvecs;        /* propagator applications (columns of all step GEMMs)  */
+    int64_t iterations;     /* grouped step iterations launched                     */
+    int64_t jumps;          /* jumps applied                                        */
+    double gemm_flops;      /* 8 d^2 per matvec (complex MAC = 8 real flop)         */
+} qjx_stats;
+
+const char *qjx_last_error(void);
+
+/* Build H, H_drv, L, H_eff,b and the propagator tree exp(-i H_eff,b dt_i),
+ * dt_0 = T / steps_per_period, dt_i = dt_{i-1} / 2, on the current HIP device. */
+int qjx_ctx_create(const qjx_model_desc *model, qjx_ctx **out);
+void qjx_ctx_destroy(qjx_ctx *ctx);
+
+/* Allocate the ensemble; states start at e_{start_state} (or uniform), RNG
+ * streams at leap-frog(seed + tr, mns) of MCG59 seeded 777. */
+int qjx_ensemble_init(qjx_ctx *ctx, const qjx_ensemble_desc *ens);
+
+/* Transient: draw eta_0 (redraw while 0), run num_tp_periods periods without
+ * logging, then set every trajectory's time to 0. */
+int qjx_trans(qjx_ctx *ctx, int num_tp_periods);
+
+/* Advance every trajectory by n periods; is_obs = 1 logs jumps (if jump > 0). */
+int qjx_run_periods(qjx_ctx *ctx, int n, int is_obs);
+
+/* Advance every trajectory by n top-level steps of T / steps_per_period (with
+ * all their bisection refinement); n periods = n * steps_per_period steps. */
+int qjx_run_steps(qjx_ctx *ctx, int n, int is_obs);
+
+/* Observables of every trajectory (arrays of num_trajectories doubles, any may be
+ * NULL); adr_avg (d doubles) is the ensemble mean of |psi_k|^2/||psi||^2;
+ * adr (num_trajectories x d) the per-trajectory values.  is_start = 1 also
+ * records mean_start (calc_chars_std_start). */
+int qjx_chars(qjx_ctx *ctx, int is_start, double *norm, double *mean, double *dispersion,
+              double *m2, double *energy, double *adr_avg, double *adr);
+
+/* Current states (num_trajectories x d interleaved complex), eta, time, logged
+ * jump counts and RNG states; any pointer may be NULL. */
+int qjx_get_state(qjx_ctx *ctx, double *psi, double *eta, double *t, int *njumps, uint64_t *rng);
+int qjx_set_state(qjx_ctx *ctx, const double *psi, const double *eta, const double *t,
+                  const uint64_t *rng);
+
+/* Jump log of trajectory tr accumulated on the host since qjx_ensemble_init.
+ * Copies at most cap records, returns the number of records (or < 0). */
+int qjx_get_jumps(qjx_ctx *ctx, int tr, double *times, int *types, double *norms, double *etas,
+                  int cap);
+
+/* Propagator of branch b, level lvl as dense d x d complex (FP64 copy of the
+ * operand the GEMM uses); set replaces it (e.g. to share one tree between runs). */
+int qjx_get_propagator(qjx_ctx *ctx, int b, int lvl, double *U);
+int qjx_set_propagator(qjx_ctx *ctx, int b, int lvl, const double *U);
+
+int qjx_get_stats(qjx_ctx *ctx, qjx_stats *out);
+/* Hilbert dimension of the context (N + 1). */
+int qjx_dim(const qjx_ctx *ctx);
+
+/* Drop-in runner: read config.txt / params.txt (reference keys, config.cpp:29-145),
+ * run the STD experiment and write the reference's output files into `path`
+ * (core.cpp:918-1084, dump.cpp:26-61).  precision selects the GEMM form;
+ * steps_per_period = 0 means the reference's 4. */
+int qjx_run_config(const char *config_path, const char *params_path, int precision,
+                   int steps_per_period);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

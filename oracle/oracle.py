@@ -1,0 +1,189 @@
+"""ctypes loader for the FP64 CPU oracle (oracle/qjx_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, never by the os_d_amd package.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(BUILD, "libqjx_oracle.so")
+SOURCES = ["qjx_oracle.cpp", "lindblad_oracle.cpp"]
+
+
+def build():
+    """Compile the oracle with g++ (OpenMP)."""
+    os.makedirs(BUILD, exist_ok=True)
+    srcs = [os.path.join(HERE, s) for s in SOURCES if os.path.exists(os.path.join(HERE, s))]
+    tmp = LIB + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fopenmp", "-fPIC", "-shared"] + srcs + ["-o", tmp]
+    subprocess.run(cmd, check=True, capture_output=True, text=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+class Model(ctypes.Structure):
+    _fields_ = [("N", ctypes.c_int), ("prm_E", ctypes.c_double), ("prm_U", ctypes.c_double),
+                ("prm_J", ctypes.c_double), ("drv_ampl", ctypes.c_double),
+                ("drv_freq", ctypes.c_double), ("gamma", ctypes.c_double), ("S", ctypes.c_int),
+                ("D", ctypes.c_int)]
+
+
+_lib = None
+_P = ctypes.POINTER
+_D = _P(ctypes.c_double)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        L.qjo_model_matrices.argtypes = [_P(Model), _D, _D, _D, _D, _D]
+        L.qjo_expm.argtypes = [ctypes.c_int, _D, _D]
+        L.qjo_mcg59_uniform.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _D]
+        L.qjo_create.restype = ctypes.c_void_p
+        L.qjo_create.argtypes = [_P(Model)] + [ctypes.c_int] * 8
+        L.qjo_destroy.argtypes = [ctypes.c_void_p]
+        L.qjo_error.restype = ctypes.c_char_p
+        L.qjo_error.argtypes = [ctypes.c_void_p]
+        L.qjo_set_propagator.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D]
+        L.qjo_get_propagator.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D]
+        L.qjo_trans.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.qjo_periods.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.qjo_periods_one.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.qjo_chars.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, _D, _D, _D]
+        L.qjo_get_state.argtypes = [ctypes.c_void_p, _D, _D, _D, _P(ctypes.c_int), _P(ctypes.c_uint64)]
+        L.qjo_set_state.argtypes = [ctypes.c_void_p, _D, _D, _D, _P(ctypes.c_uint64)]
+        L.qjo_get_jumps.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _P(ctypes.c_int), _D, _D,
+                                    ctypes.c_int]
+        L.qjo_num_matvecs.restype = ctypes.c_int64
+        L.qjo_num_matvecs.argtypes = [ctypes.c_void_p]
+        L.qjo_num_jumps.restype = ctypes.c_int64
+        L.qjo_num_jumps.argtypes = [ctypes.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _p(a, t=ctypes.c_double):
+    return None if a is None else a.ctypes.data_as(_P(t))
+
+
+def model(N, prm_E=1.0, prm_U=0.5, prm_J=1.0, drv_ampl=1.5, drv_freq=1.0, gamma=0.1, S=4, D=16):
+    return Model(N, prm_E, prm_U, prm_J, drv_ampl, drv_freq, gamma, S, D)
+
+
+def model_matrices(m):
+    d = m.N + 1
+    out = [np.zeros((d, d), dtype=np.complex128) for _ in range(5)]
+    lib().qjo_model_matrices(ctypes.byref(m), *[_p(x.view(np.float64)) for x in out])
+    return dict(zip(("H", "Hdrv", "L", "Hqj0", "Hqj1"), out))
+
+
+def expm(A):
+    A = np.ascontiguousarray(A, dtype=np.complex128)
+    E = np.zeros_like(A)
+    rc = lib().qjo_expm(A.shape[0], _p(A.view(np.float64)), _p(E.view(np.float64)))
+    assert rc == 0
+    return E
+
+
+def mcg59_uniform(k, mns, n):
+    out = np.zeros(n)
+    lib().qjo_mcg59_uniform(k, mns, n, _p(out))
+    return out
+
+
+class Ensemble:
+    """The oracle ensemble: one trajectory per OpenMP iteration, as the reference."""
+
+    def __init__(self, m, num_traj, seed=0, mns=1000000, start_type=0, start_state=0, jump=1,
+                 threads=0, build_propagators=True):
+        self.L = lib()
+        self.m = m
+        self.d = m.N + 1
+        self.M = num_traj
+        threads = threads or os.cpu_count() or 1
+        self.h = self.L.qjo_create(ctypes.byref(m), num_traj, seed, mns, start_type, start_state,
+                                   jump, threads, 1 if build_propagators else 0)
+        if not self.h:
+            raise ValueError("qjo_create rejected the arguments")
+
+    def close(self):
+        if self.h:
+            self.L.qjo_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ok(self, rc):
+        if rc != 0:
+            raise RuntimeError("oracle: " + self.L.qjo_error(self.h).decode())
+
+    def set_propagator(self, b, lvl, U):
+        U = np.ascontiguousarray(U, dtype=np.complex128)
+        self._ok(self.L.qjo_set_propagator(self.h, b, lvl, _p(U.view(np.float64))))
+
+    def get_propagator(self, b, lvl):
+        U = np.zeros((self.d, self.d), dtype=np.complex128)
+        self._ok(self.L.qjo_get_propagator(self.h, b, lvl, _p(U.view(np.float64))))
+        return U
+
+    def trans(self, n):
+        self._ok(self.L.qjo_trans(self.h, n))
+
+    def periods(self, n, is_obs=1):
+        self._ok(self.L.qjo_periods(self.h, n, is_obs))
+
+    def periods_one(self, tr, n, is_obs=1):
+        self._ok(self.L.qjo_periods_one(self.h, tr, n, is_obs))
+
+    def chars(self, is_start=0, adr=False):
+        M, d = self.M, self.d
+        out = {k: np.zeros(M) for k in ("norm", "mean", "dispersion", "m2", "energy")}
+        a = np.zeros((M, d)) if adr else None
+        self._ok(self.L.qjo_chars(self.h, is_start, _p(out["norm"]), _p(out["mean"]),
+                                  _p(out["dispersion"]), _p(out["m2"]), _p(out["energy"]), _p(a)))
+        if adr:
+            out["adr"] = a
+        return out
+
+    def get_state(self):
+        M, d = self.M, self.d
+        psi = np.zeros((M, d), dtype=np.complex128)
+        eta, t = np.zeros(M), np.zeros(M)
+        nj = np.zeros(M, dtype=np.int32)
+        rng = np.zeros(M, dtype=np.uint64)
+        self._ok(self.L.qjo_get_state(self.h, _p(psi.view(np.float64)), _p(eta), _p(t),
+                                      _p(nj, ctypes.c_int), _p(rng, ctypes.c_uint64)))
+        return {"psi": psi, "eta": eta, "t": t, "njumps": nj, "rng": rng}
+
+    def set_state(self, psi=None, eta=None, t=None, rng=None):
+        psi = None if psi is None else np.ascontiguousarray(psi, dtype=np.complex128)
+        self._ok(self.L.qjo_set_state(
+            self.h, None if psi is None else _p(psi.view(np.float64)),
+            _p(None if eta is None else np.ascontiguousarray(eta, dtype=np.float64)),
+            _p(None if t is None else np.ascontiguousarray(t, dtype=np.float64)),
+            _p(None if rng is None else np.ascontiguousarray(rng, dtype=np.uint64), ctypes.c_uint64)))
+
+    def get_jumps(self, tr):
+        n = self.L.qjo_get_jumps(self.h, tr, None, None, None, None, 0)
+        times, norms, etas = np.zeros(n), np.zeros(n), np.zeros(n)
+        types = np.zeros(n, dtype=np.int32)
+        self.L.qjo_get_jumps(self.h, tr, _p(times), _p(types, ctypes.c_int), _p(norms), _p(etas), n)
+        return {"times": times, "types": types, "norms": norms, "etas": etas}
+
+    def num_matvecs(self):
+        return self.L.qjo_num_matvecs(self.h)
+
+    def num_jumps(self):
+        return self.L.qjo_num_jumps(self.h)

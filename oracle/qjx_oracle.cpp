@@ -1,0 +1,645 @@
+/*
+ * qjx_oracle.cpp -- FP64 CPU restatement of os_d QJX's QJ trajectory path for the
+ * Bose-Hubbard dimer.  TEST INFRASTRUCTURE ONLY (see qjx_oracle.h): the checker for
+ * the HIP engine and the CPU baseline timed by bench.py.  Never linked by the product.
+ *
+ * Each function cites the reference code it restates (paths relative to the
+ * reference repository root, source/cpp/QJX/QJX/ unless stated otherwise).
+ */
+#include "qjx_oracle.h"
+
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef std::complex<double> cplx;
+
+static const double kPi = 3.1415926535897932384626433832795;  /* config.h PI */
+
+/* ------------------------------------------------------------------------- */
+/* Dense complex matrix helpers (row-major n x n).                           */
+/* ------------------------------------------------------------------------- */
+
+static void matmul(int n, const cplx *A, const cplx *B, cplx *C)
+{
+    /* C = A B, i-k-j order; parallel over rows. */
+#pragma omp parallel for schedule(static) if (n >= 128)
+    for (int i = 0; i < n; i++) {
+        cplx *c = C + (size_t)i * n;
+        for (int j = 0; j < n; j++) c[j] = 0.0;
+        for (int k = 0; k < n; k++) {
+            const cplx a = A[(size_t)i * n + k];
+            if (a == cplx(0.0, 0.0)) continue;
+            const cplx *b = B + (size_t)k * n;
+            for (int j = 0; j < n; j++) c[j] += a * b[j];
+        }
+    }
+}
+
+static double norm1(int n, const cplx *A)
+{
+    double best = 0.0;
+    for (int j = 0; j < n; j++) {
+        double s = 0.0;
+        for (int i = 0; i < n; i++) s += std::abs(A[(size_t)i * n + j]);
+        if (s > best) best = s;
+    }
+    return best;
+}
+
+/* Solve P X = Q in place (Q <- X) by LU with partial pivoting.  P is destroyed. */
+static int lu_solve(int n, cplx *P, cplx *Q)
+{
+    std::vector<int> piv(n);
+    for (int k = 0; k < n; k++) {
+        int p = k;
+        double best = std::abs(P[(size_t)k * n + k]);
+        for (int i = k + 1; i < n; i++) {
+            double v = std::abs(P[(size_t)i * n + k]);
+            if (v > best) { best = v; p = i; }
+        }
+        if (best == 0.0) return -1;
+        piv[k] = p;
+        if (p != k) {
+            for (int j = 0; j < n; j++) std::swap(P[(size_t)k * n + j], P[(size_t)p * n + j]);
+            for (int j = 0; j < n; j++) std::swap(Q[(size_t)k * n + j], Q[(size_t)p * n + j]);
+        }
+        const cplx inv = 1.0 / P[(size_t)k * n + k];
+#pragma omp parallel for schedule(static) if (n >= 128)
+        for (int i = k + 1; i < n; i++) {
+            cplx f = P[(size_t)i * n + k] * inv;
+            if (f == cplx(0.0, 0.0)) continue;
+            P[(size_t)i * n + k] = f;
+            for (int j = k + 1; j < n; j++) P[(size_t)i * n + j] -= f * P[(size_t)k * n + j];
+            for (int j = 0; j < n; j++) Q[(size_t)i * n + j] -= f * Q[(size_t)k * n + j];
+        }
+    }
+    for (int k = n - 1; k >= 0; k--) {
+        const cplx inv = 1.0 / P[(size_t)k * n + k];
+        for (int j = 0; j < n; j++) Q[(size_t)k * n + j] *= inv;
+#pragma omp parallel for schedule(static) if (n >= 128)
+        for (int i = 0; i < k; i++) {
+            cplx f = P[(size_t)i * n + k];
+            if (f == cplx(0.0, 0.0)) continue;
+            for (int j = 0; j < n; j++) Q[(size_t)i * n + j] -= f * Q[(size_t)k * n + j];
+        }
+    }
+    return 0;
+}
+
+/* Matrix exponential by diagonal Pade approximants with scaling and squaring,
+ * N. J. Higham, "The scaling and squaring method for the matrix exponential
+ * revisited", SIAM J. Matrix Anal. Appl. 26 (2005) 1179-1193, Algorithm 2.3:
+ * degree m in {3,5,7,9,13} picked from the 1-norm thresholds theta_m, and for
+ * m = 13 the matrix is scaled by 2^-s so that ||A/2^s||_1 <= theta_13.
+ * The reference calls Eigen's MatrixBase::exp(), which implements the same
+ * published algorithm (qj_proc.cpp:55, 98). */
+static int expm_pade(int n, const cplx *A, cplx *E)
+{
+    static const double b3[] = {120., 60., 12., 1.};
+    static const double b5[] = {30240., 15120., 3360., 420., 30., 1.};
+    static const double b7[] = {17297280., 8648640., 1995840., 277200., 25200., 1512., 56., 1.};
+    static const double b9[] = {17643225600., 8821612800., 2075673600., 302702400., 30270240.,
+                                2162160., 110880., 3960., 90., 1.};
+    static const double b13[] = {64764752532480000., 32382376266240000., 7771770303897600.,
+                                 1187353796428800., 129060195264000., 10559470521600.,
+                                 670442572800., 33522128640., 1323241920., 40840800.,
+                                 960960., 16380., 182., 1.};
+    const size_t nn = (size_t)n * n;
+    const double a1 = norm1(n, A);
+    int m;
+    const double *b;
+    int s = 0;
+    if (a1 <= 1.495585217958292e-2) { m = 3; b = b3; }
+    else if (a1 <= 2.539398330063230e-1) { m = 5; b = b5; }
+    else if (a1 <= 9.504178996162932e-1) { m = 7; b = b7; }
+    else if (a1 <= 2.097847961257068e0) { m = 9; b = b9; }
+    else {
+        m = 13; b = b13;
+        const double theta13 = 5.371920351148152e0;
+        if (a1 > theta13) s = (int)std::ceil(std::log2(a1 / theta13));
+        if (s < 0) s = 0;
+    }
+    std::vector<cplx> X(A, A + nn);
+    if (s > 0) {
+        const double f = std::ldexp(1.0, -s);
+        for (size_t i = 0; i < nn; i++) X[i] *= f;
+    }
+    std::vector<cplx> X2(nn), U(nn), V(nn), T(nn);
+    matmul(n, X.data(), X.data(), X2.data());
+    if (m < 13) {
+        /* U = X * sum_{odd j} b_j X^{j-1}, V = sum_{even j} b_j X^j, powers of X2. */
+        std::vector<cplx> P(nn), Uo(nn, 0.0);
+        for (size_t i = 0; i < nn; i++) V[i] = 0.0;
+        /* P = I */
+        for (size_t i = 0; i < nn; i++) P[i] = 0.0;
+        for (int i = 0; i < n; i++) P[(size_t)i * n + i] = 1.0;
+        for (int j = 0; 2 * j <= m; j++) {
+            for (size_t i = 0; i < nn; i++) {
+                V[i] += b[2 * j] * P[i];
+                if (2 * j + 1 <= m) Uo[i] += b[2 * j + 1] * P[i];
+            }
+            if (2 * (j + 1) <= m) {
+                matmul(n, P.data(), X2.data(), T.data());
+                P.swap(T);
+            }
+        }
+        matmul(n, X.data(), Uo.data(), U.data());
+    } else {
+        std::vector<cplx> X4(nn), X6(nn), W(nn);
+        matmul(n, X2.data(), X2.data(), X4.data());
+        matmul(n, X4.data(), X2.data(), X6.data());
+        /* W = X6 (b13 X6 + b11 X4 + b9 X2) + b7 X6 + b5 X4 + b3 X2 + b1 I */
+        for (size_t i = 0; i < nn; i++) T[i] = b[13] * X6[i] + b[11] * X4[i] + b[9] * X2[i];
+        matmul(n, X6.data(), T.data(), W.data());
+        for (size_t i = 0; i < nn; i++) W[i] += b[7] * X6[i] + b[5] * X4[i] + b[3] * X2[i];
+        for (int i = 0; i < n; i++) W[(size_t)i * n + i] += b[1];
+        matmul(n, X.data(), W.data(), U.data());
+        /* V = X6 (b12 X6 + b10 X4 + b8 X2) + b6 X6 + b4 X4 + b2 X2 + b0 I */
+        for (size_t i = 0; i < nn; i++) T[i] = b[12] * X6[i] + b[10] * X4[i] + b[8] * X2[i];
+        matmul(n, X6.data(), T.data(), V.data());
+        for (size_t i = 0; i < nn; i++) V[i] += b[6] * X6[i] + b[4] * X4[i] + b[2] * X2[i];
+        for (int i = 0; i < n; i++) V[(size_t)i * n + i] += b[0];
+    }
+    /* (V - U) E = (V + U) */
+    std::vector<cplx> P(nn);
+    for (size_t i = 0; i < nn; i++) {
+        P[i] = V[i] - U[i];
+        E[i] = V[i] + U[i];
+    }
+    if (lu_solve(n, P.data(), E) != 0) return -1;
+    for (int k = 0; k < s; k++) {
+        matmul(n, E, E, T.data());
+        std::memcpy(E, T.data(), nn * sizeof(cplx));
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Model (newdel.cpp:3-37, 450-498, 1454-1498, 2285-2388).                   */
+/* ------------------------------------------------------------------------- */
+
+struct Model {
+    int N = 0, d = 0;
+    double T = 0.0;
+    std::vector<cplx> H, Hdrv, L, Hqj[2];
+};
+
+static void build_model(const qjo_model *p, Model *m)
+{
+    const int N = p->N, d = N + 1;
+    m->N = N;
+    m->d = d;
+    m->T = 2.0 * kPi / p->drv_freq;                       /* newdel.cpp:31 */
+    const size_t dd = (size_t)d * d;
+    m->H.assign(dd, 0.0);
+    m->Hdrv.assign(dd, 0.0);
+    m->L.assign(dd, 0.0);
+    const double U = p->prm_U / double(d - 1);            /* newdel.cpp:459 */
+    for (int k = 0; k < d; k++) {                         /* newdel.cpp:480-486 */
+        m->H[(size_t)k * d + k] = 2.0 * U * double(k * (k - 1) + (d - (k + 1)) * (d - (k + 1) - 1));
+        m->Hdrv[(size_t)k * d + k] = double((d - (k + 1)) - k);
+        m->L[(size_t)k * d + k] = double(k - (d - (k + 1))); /* newdel.cpp:1482-1488 */
+    }
+    for (int k = 0; k < d - 1; k++) {                     /* newdel.cpp:488-495, 1490-1497 */
+        const double h = p->prm_J * std::sqrt(double((d - (k + 1)) * (k + 1)));
+        m->H[(size_t)(k + 1) * d + k] -= h;
+        m->H[(size_t)k * d + k + 1] -= h;
+        const double l = std::sqrt(double((d - (k + 1)) * (k + 1)));
+        m->L[(size_t)(k + 1) * d + k] -= l;
+        m->L[(size_t)k * d + k + 1] += l;
+    }
+    /* diss_part = gamma' L^dag L (newdel.cpp:2327-2342); H_part = H - i/2 diss_part;
+     * Hqj[b] = -i (H_part + E_b H_drv) (newdel.cpp:2346-2383). */
+    const double g = p->gamma / double(d - 1);
+    std::vector<cplx> LdL(dd, 0.0);
+    for (int i = 0; i < d; i++)
+        for (int j = 0; j < d; j++) {
+            cplx s = 0.0;
+            for (int k = 0; k < d; k++) s += std::conj(m->L[(size_t)k * d + i]) * m->L[(size_t)k * d + j];
+            LdL[(size_t)i * d + j] = g * s;
+        }
+    const double E[2] = {p->prm_E + p->drv_ampl, p->prm_E - p->drv_ampl};
+    for (int b = 0; b < 2; b++) {
+        m->Hqj[b].assign(dd, 0.0);
+        for (size_t idx = 0; idx < dd; idx++) {
+            const double hre = m->H[idx].real() + 0.5 * LdL[idx].imag();
+            const double him = 0.0 - 0.5 * LdL[idx].real();
+            m->Hqj[b][idx] = cplx(him + E[b] * 0.0, -(hre + E[b] * m->Hdrv[idx].real()));
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* RNG: MKL VSL MCG59, a = 13^13, m = 2^59, seed 777, leap-frog(k, mns)       */
+/* (newdel_exp.cpp:244-296).  Output n of stream k is x_{k + n mns} / 2^59 with */
+/* x_j = 777 a^j mod 2^59; the conversion is the correctly rounded double of x */
+/* times 2^-59 (pinned by MKL golden vectors).                                 */
+/* ------------------------------------------------------------------------- */
+
+static const uint64_t kMcgA = 302875106592253ULL;          /* 13^13 */
+static const uint64_t kMcgMask = (1ULL << 59) - 1ULL;
+
+static uint64_t mcg_pow(uint64_t a, uint64_t e)
+{
+    uint64_t r = 1;
+    while (e) {
+        if (e & 1) r = (r * a) & kMcgMask;
+        a = (a * a) & kMcgMask;
+        e >>= 1;
+    }
+    return r;
+}
+
+struct Rng {
+    uint64_t x, stride;
+    double next()
+    {
+        double u = (double)x * 0x1p-59;
+        x = (x * stride) & kMcgMask;
+        return u;
+    }
+};
+
+static Rng rng_stream(int k, int mns)
+{
+    Rng r;
+    r.x = (777ULL * mcg_pow(kMcgA, (uint64_t)k)) & kMcgMask;
+    r.stride = mcg_pow(kMcgA, (uint64_t)mns);
+    return r;
+}
+
+extern "C" void qjo_mcg59_uniform(int k, int mns, int n, double *out)
+{
+    Rng r = rng_stream(k, mns);
+    for (int i = 0; i < n; i++) out[i] = r.next();
+}
+
+/* ------------------------------------------------------------------------- */
+/* Ensemble.                                                                  */
+/* ------------------------------------------------------------------------- */
+
+struct qjo_ens {
+    qjo_model p;
+    Model m;
+    int M = 0, seed = 0, mns = 0, start_type = 0, start_state = 0, jump = 0, threads = 1;
+    std::vector<double> dt;                 /* dt[i] of level i                    */
+    std::vector<cplx> U[2];                 /* U[b][lvl * d*d ...]                  */
+    std::vector<cplx> psi, psi_aux;         /* phi_all, phi_all_aux (M x d)         */
+    std::vector<double> t, eta, mean_start;
+    std::vector<int> njumps;
+    std::vector<Rng> rng;
+    std::vector<std::vector<double>> jt, jn, je;
+    std::vector<std::vector<int>> jtype;
+    std::vector<int64_t> matvecs, jumps_done;
+    std::string err;
+    volatile int failed = 0;
+};
+
+static void set_err(qjo_ens *e, const std::string &s)
+{
+#pragma omp critical(qjo_err)
+    {
+        if (!e->failed) e->err = s;
+        e->failed = 1;
+    }
+}
+
+extern "C" int qjo_model_matrices(const qjo_model *p, double *H, double *Hdrv, double *L,
+                                  double *Hqj0, double *Hqj1)
+{
+    Model m;
+    build_model(p, &m);
+    const size_t bytes = (size_t)m.d * m.d * sizeof(cplx);
+    if (H) std::memcpy(H, m.H.data(), bytes);
+    if (Hdrv) std::memcpy(Hdrv, m.Hdrv.data(), bytes);
+    if (L) std::memcpy(L, m.L.data(), bytes);
+    if (Hqj0) std::memcpy(Hqj0, m.Hqj[0].data(), bytes);
+    if (Hqj1) std::memcpy(Hqj1, m.Hqj[1].data(), bytes);
+    return 0;
+}
+
+extern "C" int qjo_expm(int n, const double *A, double *E)
+{
+    return expm_pade(n, reinterpret_cast<const cplx *>(A), reinterpret_cast<cplx *>(E));
+}
+
+/* Split tree (core.cpp:9880-9926, qj_proc.cpp:4-120), generalised top level:
+ * level 0 has S/2 steps of dt_0 = T/S, level i > 0 has 2 steps of dt_{i-1}/2.
+ * With S = 4 this is exactly QJX (dt_i = T / 2^(i+2)). */
+static void build_tree(qjo_ens *e, int build)
+{
+    const int d = e->m.d, D = e->p.D;
+    e->dt.resize(D);
+    double dt = e->m.T / double(e->p.S);
+    for (int i = 0; i < D; i++) {
+        e->dt[i] = dt;
+        dt *= 0.5;
+    }
+    const size_t dd = (size_t)d * d;
+    for (int b = 0; b < 2; b++) {
+        e->U[b].assign(dd * D, 0.0);
+        if (!build) continue;
+        std::vector<cplx> A(dd);
+        for (int i = 0; i < D; i++) {
+            for (size_t idx = 0; idx < dd; idx++)
+                A[idx] = cplx(e->m.Hqj[b][idx].real() * e->dt[i], e->m.Hqj[b][idx].imag() * e->dt[i]);
+            expm_pade(d, A.data(), e->U[b].data() + dd * i);
+        }
+    }
+}
+
+extern "C" qjo_ens *qjo_create(const qjo_model *p, int num_traj, int seed, int mns,
+                               int start_type, int start_state, int jump, int num_threads,
+                               int build_propagators)
+{
+    if (!p || p->N < 1 || num_traj < 1 || p->S < 2 || (p->S % 2) || p->D < 1) return nullptr;
+    qjo_ens *e = new qjo_ens;
+    e->p = *p;
+    build_model(p, &e->m);
+    e->M = num_traj;
+    e->seed = seed;
+    e->mns = mns;
+    e->start_type = start_type;
+    e->start_state = start_state;
+    e->jump = jump;
+    e->threads = num_threads > 0 ? num_threads : 1;
+#ifdef _OPENMP
+    omp_set_num_threads(e->threads);
+#endif
+    build_tree(e, build_propagators);
+    const int d = e->m.d, M = num_traj;
+    /* init_basic_data (newdel_exp.cpp:332-375), init_start_state (656-679) */
+    e->psi.assign((size_t)M * d, 0.0);
+    e->psi_aux.assign((size_t)M * d, 0.0);
+    e->t.assign(M, 0.0);
+    e->eta.assign(M, 0.0);
+    e->mean_start.assign(M, 0.0);
+    e->njumps.assign(M, 0);
+    e->jt.assign(M, {});
+    e->jn.assign(M, {});
+    e->je.assign(M, {});
+    e->jtype.assign(M, {});
+    e->matvecs.assign(M, 0);
+    e->jumps_done.assign(M, 0);
+    e->rng.resize(M);
+    for (int tr = 0; tr < M; tr++) {
+        e->rng[tr] = rng_stream(seed + tr, mns);
+        cplx *ph = &e->psi[(size_t)tr * d];
+        if (start_type == 0) ph[start_state] = 1.0;
+        else for (int i = 0; i < d; i++) ph[i] = std::sqrt(1.0 / double(d));
+    }
+    return e;
+}
+
+extern "C" void qjo_destroy(qjo_ens *e) { delete e; }
+extern "C" const char *qjo_error(const qjo_ens *e) { return e ? e->err.c_str() : "null handle"; }
+
+extern "C" int qjo_set_propagator(qjo_ens *e, int b, int lvl, const double *U)
+{
+    if (b < 0 || b > 1 || lvl < 0 || lvl >= e->p.D) return -1;
+    const size_t dd = (size_t)e->m.d * e->m.d;
+    std::memcpy(e->U[b].data() + dd * lvl, U, dd * sizeof(cplx));
+    return 0;
+}
+
+extern "C" int qjo_get_propagator(const qjo_ens *e, int b, int lvl, double *U)
+{
+    if (b < 0 || b > 1 || lvl < 0 || lvl >= e->p.D) return -1;
+    const size_t dd = (size_t)e->m.d * e->m.d;
+    std::memcpy(U, e->U[b].data() + dd * lvl, dd * sizeof(cplx));
+    return 0;
+}
+
+/* norm_square (experiment.cpp:1263-1271) */
+static double norm_square(const cplx *x, int d)
+{
+    double s = 0.0;
+    for (int i = 0; i < d; i++) s += x[i].real() * x[i].real() + x[i].imag() * x[i].imag();
+    return s;
+}
+
+/* prop_step: row-major zgemv, res = U x (qj_proc.cpp:264-269) */
+static void prop_step(const cplx *x, const cplx *U, cplx *res, int d)
+{
+    for (int i = 0; i < d; i++) {
+        const cplx *row = U + (size_t)i * d;
+        double re = 0.0, im = 0.0;
+        for (int j = 0; j < d; j++) {
+            re += row[j].real() * x[j].real() - row[j].imag() * x[j].imag();
+            im += row[j].real() * x[j].imag() + row[j].imag() * x[j].real();
+        }
+        res[i] = cplx(re, im);
+    }
+}
+
+/* recovery (experiment.cpp:1273-1382), one dissipator (num_diss = 1, g = 1). */
+static void recovery(qjo_ens *e, int tr, int is_obs)
+{
+    const int d = e->m.d;
+    cplx *phi = &e->psi_aux[(size_t)tr * d];
+    double nrm = std::sqrt(norm_square(phi, d));
+    for (int i = 0; i < d; i++) phi[i] = cplx(phi[i].real() / nrm, phi[i].imag() / nrm);
+    const double ran = e->rng[tr].next();
+    std::vector<cplx> res(d);
+    prop_step(phi, e->m.L.data(), res.data(), d);         /* L_i phi            */
+    double gnorm = norm_square(res.data(), d) * 1.0;       /* g_i = 1            */
+    double tmp = gnorm;
+    double r = ran * tmp;
+    int index = 0;
+    (void)r;                                                /* k = 1: index 0     */
+    const double sc = std::sqrt(gnorm / 1.0);
+    for (int i = 0; i < d; i++) phi[i] = cplx(res[i].real() / sc, res[i].imag() / sc);
+    if (e->jump > 0 && is_obs == 1) e->jtype[tr].push_back(index);
+    e->jumps_done[tr] += 1;
+}
+
+/* one_period_branch (qj_proc.cpp:271-343) as an explicit recursion over levels. */
+static void level_run(qjo_ens *e, int tr, int b, int lvl, int steps, int is_obs)
+{
+    const int d = e->m.d, D = e->p.D;
+    const size_t dd = (size_t)d * d;
+    cplx *phi = &e->psi[(size_t)tr * d];
+    cplx *aux = &e->psi_aux[(size_t)tr * d];
+    const cplx *U = e->U[b].data() + dd * lvl;
+    const double dt = e->dt[lvl];
+    double &t = e->t[tr];
+    double &eta = e->eta[tr];
+    for (int c = 0; c < steps; c++) {
+        if (e->failed) return;
+        prop_step(phi, U, aux, d);
+        e->matvecs[tr] += 1;
+        const double n = norm_square(aux, d);
+        if (n > 1.0) {                                      /* experiment.cpp:1247-1251 */
+            char buf[160];
+            std::snprintf(buf, sizeof buf, "norm more than 1 (trajectory %d, norm %.17g)", tr, n);
+            set_err(e, buf);
+            return;
+        }
+        const bool crossed = !(n > eta);                   /* experiment.cpp:1253-1260 */
+        if (lvl == D - 1) {
+            if (crossed) {
+                if (e->jump > 0 && is_obs == 1) {
+                    e->jt[tr].push_back(t + dt);
+                    e->jn[tr].push_back(n);
+                    e->je[tr].push_back(eta);
+                    e->njumps[tr]++;
+                }
+                recovery(e, tr, is_obs);
+                eta = e->rng[tr].next();
+                while (eta == 0.0) eta = e->rng[tr].next();
+            }
+            std::memcpy(phi, aux, sizeof(cplx) * d);
+            t += dt;
+        } else {
+            if (crossed) {
+                level_run(e, tr, b, lvl + 1, 2, is_obs);
+                t -= dt;
+            } else {
+                std::memcpy(phi, aux, sizeof(cplx) * d);
+            }
+            t += dt;
+        }
+    }
+}
+
+/* ex_period (core.cpp:84-95): both branches, top level S/2 steps each. */
+static void one_period(qjo_ens *e, int tr, int is_obs)
+{
+    for (int b = 0; b < 2; b++) level_run(e, tr, b, 0, e->p.S / 2, is_obs);
+}
+
+extern "C" int qjo_periods_one(qjo_ens *e, int tr, int n, int is_obs)
+{
+    if (tr < 0 || tr >= e->M) return -1;
+    for (int p = 0; p < n && !e->failed; p++) one_period(e, tr, is_obs);
+    return e->failed ? -1 : 0;
+}
+
+extern "C" int qjo_periods(qjo_ens *e, int n, int is_obs)
+{
+    for (int p = 0; p < n && !e->failed; p++) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(e->threads)
+        for (int tr = 0; tr < e->M; tr++) one_period(e, tr, is_obs);
+    }
+    return e->failed ? -1 : 0;
+}
+
+extern "C" int qjo_trans(qjo_ens *e, int num_tp_periods)
+{
+    /* trans_process_single (experiment.cpp:3249-3271), resresh_times (2214-2219) */
+#pragma omp parallel for schedule(dynamic, 1) num_threads(e->threads)
+    for (int tr = 0; tr < e->M; tr++) {
+        double &eta = e->eta[tr];
+        eta = 0.0;
+        while (eta == 0.0) eta = e->rng[tr].next();
+        for (int p = 0; p < num_tp_periods && !e->failed; p++) one_period(e, tr, 0);
+    }
+    for (int tr = 0; tr < e->M; tr++) e->t[tr] = 0.0;
+    return e->failed ? -1 : 0;
+}
+
+/* calc_chars_std(_start) (core.cpp:665-725) with get_mean_simple, get_dispersion_simple,
+ * get_m2, get_energy (experiment.cpp:1395-1469). */
+extern "C" int qjo_chars(qjo_ens *e, int is_start, double *norm, double *mean, double *dispersion,
+                         double *m2, double *energy, double *adr_out)
+{
+    const int d = e->m.d;
+    const double E0 = e->p.prm_E;
+#pragma omp parallel for schedule(static) num_threads(e->threads)
+    for (int tr = 0; tr < e->M; tr++) {
+        const cplx *phi = &e->psi[(size_t)tr * d];
+        std::vector<double> adr(d);
+        const double nrm = norm_square(phi, d);
+        for (int k = 0; k < d; k++)
+            adr[k] = (phi[k].real() * phi[k].real() + phi[k].imag() * phi[k].imag()) * (1.0 / nrm);
+        double mn = 0.0;
+        for (int k = 0; k < d; k++) mn += double(k) * adr[k];
+        if (is_start) e->mean_start[tr] = mn;
+        const double disp = (mn - e->mean_start[tr]) * (mn - e->mean_start[tr]);
+        double mm2 = 0.0;
+        for (int k = 0; k < d; k++) mm2 += (double(k) - mn) * (double(k) - mn) * adr[k];
+        /* energy: <phi|H + E H_drv|phi> / ||phi||^2, dense loop as experiment.cpp:1441-1466 */
+        const double sn = std::sqrt(nrm);
+        double en = 0.0;
+        for (int i = 0; i < d; i++) {
+            double sre = 0.0, sim = 0.0;
+            for (int j = 0; j < d; j++) {
+                const size_t idx = (size_t)i * d + j;
+                const double hv = e->m.H[idx].real() + E0 * e->m.Hdrv[idx].real();
+                sre += hv * phi[j].real() / sn - 0.0 * phi[j].imag() / sn;
+                sim += hv * phi[j].imag() / sn + 0.0 * phi[j].real() / sn;
+            }
+            en += phi[i].real() / sn * sre + phi[i].imag() / sn * sim;
+        }
+        if (norm) norm[tr] = nrm;
+        if (mean) mean[tr] = mn;
+        if (dispersion) dispersion[tr] = disp;
+        if (m2) m2[tr] = mm2;
+        if (energy) energy[tr] = en;
+        if (adr_out) std::memcpy(adr_out + (size_t)tr * d, adr.data(), sizeof(double) * d);
+    }
+    return 0;
+}
+
+extern "C" int qjo_get_state(const qjo_ens *e, double *psi, double *eta, double *t, int *njumps,
+                             uint64_t *rng)
+{
+    const int M = e->M, d = e->m.d;
+    if (psi) std::memcpy(psi, e->psi.data(), sizeof(cplx) * (size_t)M * d);
+    for (int tr = 0; tr < M; tr++) {
+        if (eta) eta[tr] = e->eta[tr];
+        if (t) t[tr] = e->t[tr];
+        if (njumps) njumps[tr] = e->njumps[tr];
+        if (rng) rng[tr] = e->rng[tr].x;
+    }
+    return 0;
+}
+
+extern "C" int qjo_set_state(qjo_ens *e, const double *psi, const double *eta, const double *t,
+                             const uint64_t *rng)
+{
+    const int M = e->M, d = e->m.d;
+    if (psi) std::memcpy(e->psi.data(), psi, sizeof(cplx) * (size_t)M * d);
+    for (int tr = 0; tr < M; tr++) {
+        if (eta) e->eta[tr] = eta[tr];
+        if (t) e->t[tr] = t[tr];
+        if (rng) e->rng[tr].x = rng[tr];
+    }
+    return 0;
+}
+
+extern "C" int qjo_get_jumps(const qjo_ens *e, int tr, double *times, int *types, double *norms,
+                             double *etas, int cap)
+{
+    if (tr < 0 || tr >= e->M) return -1;
+    const int n = (int)e->jt[tr].size();
+    for (int i = 0; i < n && i < cap; i++) {
+        if (times) times[i] = e->jt[tr][i];
+        if (types) types[i] = e->jtype[tr][i];
+        if (norms) norms[i] = e->jn[tr][i];
+        if (etas) etas[i] = e->je[tr][i];
+    }
+    return n;
+}
+
+extern "C" int64_t qjo_num_matvecs(const qjo_ens *e)
+{
+    int64_t s = 0;
+    for (auto v : e->matvecs) s += v;
+    return s;
+}
+
+extern "C" int64_t qjo_num_jumps(const qjo_ens *e)
+{
+    int64_t s = 0;
+    for (auto v : e->jumps_done) s += v;
+    return s;
+}

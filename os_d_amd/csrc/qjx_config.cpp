@@ -1,0 +1,447 @@
+// qjx_config.cpp -- the drop-in runner: the reference's config.txt / params.txt in,
+// its STD-experiment output files out, with the propagation on the GPU engine.
+//
+// Restates, for sys_id 0 / task_id 1 / prop_id 0 (reference files under
+// source/cpp/QJX/QJX/):
+//   init_params / set_param (config.cpp:29-145), the single-run main
+//   (main.cpp:3-192) and Processor::process (processor.cpp:50-75),
+//   DimerOutputBehavior::suffix_param (outputter.cpp:3-44, 353-388),
+//   init_dump_periods (newdel_exp.cpp:408-455),
+//   StdExperimentBehaviour::trans_process / obser_process (experiment.cpp:411-542),
+//   DimerCoreBehaviour::dump_std / dump_std_evo (core.cpp:918-1084),
+//   dump_adr_single / dump_adr_avg (dump.cpp:3-61), dump_phi(_evo) (core.cpp:12567-12613),
+//   the text writers (utils.cpp:64-467).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/qjx.h"
+
+namespace {
+
+thread_local std::string g_cfg_err;
+
+struct RunCfg {
+    int sys_id = 0, task_id = 0, prop_id = 0, is_debug = 0, is_pp = 0;
+    std::string init_fn, path;
+    int num_threads = 1;
+    int num_trajectories = 1, seed = 0, mns = 1000000;
+    int num_tp_periods = 0, num_obs_periods = 10, ex_deep = 16, rk_ns = 1000;
+    std::map<std::string, double> params;
+};
+
+std::vector<std::string> split_tokens(const std::string &str, const std::string &delim)
+{
+    // config.cpp:3-27: empty tokens are dropped
+    std::vector<std::string> tokens;
+    size_t prev = 0, pos = 0;
+    do {
+        pos = str.find(delim, prev);
+        if (pos == std::string::npos) pos = str.length();
+        std::string token = str.substr(prev, pos - prev);
+        if (!token.empty()) tokens.push_back(token);
+        prev = pos + delim.length();
+    } while (pos < str.length() && prev < str.length());
+    return tokens;
+}
+
+void set_param(RunCfg &c, const std::string &k, const std::string &v)
+{
+    const int iv = std::atoi(v.c_str());
+    if (k == "sys_id") c.sys_id = iv;
+    else if (k == "task_id") c.task_id = iv;
+    else if (k == "prop_id") c.prop_id = iv;
+    else if (k == "is_debug") c.is_debug = iv;
+    else if (k == "is_pp") c.is_pp = iv;
+    else if (k == "init_fn") c.init_fn = v;
+    else if (k == "path") c.path = v;
+    else if (k == "num_threads") c.num_threads = iv;
+    else if (k == "num_trajectories") c.num_trajectories = iv;
+    else if (k == "seed") c.seed = iv;
+    else if (k == "mns") c.mns = iv;
+    else if (k == "num_tp_periods") c.num_tp_periods = iv;
+    else if (k == "num_obs_periods") c.num_obs_periods = iv;
+    else if (k == "ex_deep") c.ex_deep = iv;
+    else if (k == "rk_ns") c.rk_ns = iv;
+}
+
+bool read_files(RunCfg &c, const char *fn_config, const char *fn_param)
+{
+    std::string line;
+    std::ifstream cf(fn_config);
+    if (!cf.is_open()) {
+        g_cfg_err = std::string("unable to open config file ") + fn_config;
+        return false;
+    }
+    while (std::getline(cf, line)) {
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        std::vector<std::string> t = split_tokens(line, " ");
+        if (t.size() == 2) set_param(c, t[0], t[1]);
+    }
+    std::ifstream pf(fn_param);
+    if (!pf.is_open()) {
+        g_cfg_err = std::string("unable to open params file ") + fn_param;
+        return false;
+    }
+    std::string key;
+    double val = 0.0;
+    while (std::getline(pf, line)) {
+        std::istringstream iss(line);
+        if (iss >> key >> val) c.params[key] = val;
+    }
+    return true;
+}
+
+// The reference looks every key up with params.find(key)->second, which is
+// undefined for a missing key; here a missing key is an error naming it.
+struct Params {
+    const std::map<std::string, double> &m;
+    std::string missing;
+    double get(const std::string &k)
+    {
+        auto it = m.find(k);
+        if (it == m.end()) {
+            if (missing.empty()) missing = k;
+            return 0.0;
+        }
+        return it->second;
+    }
+};
+
+std::string suffix_param(const RunCfg &c, Params &p)
+{
+    const int prec = 4;   // Processor::process passes 4 (processor.cpp:52)
+    std::stringstream s;
+    s << "_setup(" << c.sys_id << "_" << c.task_id << "_" << c.prop_id << ")";
+    s << "_rnd(" << c.seed << "_" << c.mns << ")";
+    s << "_N(" << int(p.get("N")) << ")";
+    s << "_diss(" << int(p.get("diss_type")) << "_" << std::setprecision(prec) << std::fixed
+      << p.get("diss_gamma") << "_" << std::setprecision(prec) << std::fixed << p.get("diss_phase") << ")";
+    s << "_drv(" << int(p.get("dimer_drv_type")) << "_" << std::setprecision(prec) << std::fixed
+      << p.get("dimer_drv_ampl") << "_" << std::setprecision(prec) << std::fixed << p.get("dimer_drv_freq")
+      << "_" << std::setprecision(prec) << std::fixed << p.get("dimer_drv_phase") << ")";
+    s << "_prm(" << std::setprecision(prec) << std::fixed << p.get("dimer_prm_E") << "_"
+      << std::setprecision(prec) << std::fixed << p.get("dimer_prm_U") << "_" << std::setprecision(prec)
+      << std::fixed << p.get("dimer_prm_J") << ")";
+    s << "_start(" << int(p.get("start_type")) << "_" << int(p.get("start_state")) << ")";
+    s << ".txt";
+    return s.str();
+}
+
+std::vector<int> dump_periods(int dump_type, int dump_num, int num_obs_periods)
+{
+    std::vector<int> dp;
+    if (dump_type == 0) {
+        dp.assign(dump_num + 1, 0);
+        const int shift = num_obs_periods / dump_num;
+        for (int i = 0; i < dump_num; i++) dp[i + 1] = (i + 1) * shift;
+    } else if (dump_type == 1) {
+        dp.assign(dump_num + 2, 0);   // entry 0 is left unset by the reference; 0 here
+        const double begin_decade = std::log10(1.0);
+        const double end_decade = std::log10(double(num_obs_periods));
+        const double num_decades = end_decade - begin_decade;
+        const double num_decades_dump = double(dump_num) / num_decades;
+        for (int i = 0; i < dump_num + 1; i++) {
+            const int cur = int(std::pow(10.0, begin_decade) * std::pow(10.0, (1.0 / num_decades_dump) * double(i)));
+            dp[i + 1] = cur > dp[i] ? cur : dp[i] + 1;
+        }
+    }
+    return dp;
+}
+
+// utils.cpp writers: setprecision(16) scientific, one value per line.
+bool save_doubles(const std::string &fn, const double *v, size_t n, bool append)
+{
+    std::ofstream o(fn, append ? std::ios::app : std::ios::out);
+    if (!o.is_open()) return false;
+    o << std::setprecision(16) << std::scientific;
+    for (size_t i = 0; i < n; i++) o << v[i] << std::endl;
+    return true;
+}
+
+bool save_ints(const std::string &fn, const int *v, size_t n, bool append)
+{
+    std::ofstream o(fn, append ? std::ios::app : std::ios::out);
+    if (!o.is_open()) return false;
+    for (size_t i = 0; i < n; i++) o << v[i] << std::endl;
+    return true;
+}
+
+// save_2d_inv_double_data (utils.cpp:199-226): rows = dumps, columns = trajectories.
+bool save_2d_inv(const std::string &fn, const std::vector<double> &v, int rows, int cols)
+{
+    std::ofstream o(fn);
+    if (!o.is_open()) return false;
+    o << std::setprecision(16) << std::scientific;
+    for (int r = 0; r < rows; r++) {
+        for (int c = 0; c < cols; c++) o << v[(size_t)c * rows + r] << " ";
+        o << std::endl;
+    }
+    return true;
+}
+
+bool save_complex(const std::string &fn, const double *v, size_t n, bool append)
+{
+    std::ofstream o(fn, append ? std::ios::app : std::ios::out);
+    if (!o.is_open()) return false;
+    o << std::setprecision(16) << std::scientific;
+    for (size_t i = 0; i < n; i++) o << v[2 * i] << " " << v[2 * i + 1] << std::endl;
+    return true;
+}
+
+}  // namespace
+
+#define CFG_FAIL(code, msg)          \
+    do {                             \
+        g_cfg_err = (msg);           \
+        rc = (code);                 \
+        goto done;                   \
+    } while (0)
+
+#define IO_CHECK(expr)                                                  \
+    do {                                                                \
+        if (!(expr)) CFG_FAIL(QJX_ERR_IO, "unable to write output file"); \
+    } while (0)
+
+#define API_CHECK(expr)                                  \
+    do {                                                 \
+        int r_ = (expr);                                 \
+        if (r_ < 0) CFG_FAIL(r_, qjx_last_error());      \
+    } while (0)
+
+namespace qjx {
+void set_last_error(const std::string &msg);
+}
+
+extern "C" int qjx_run_config(const char *config_path, const char *params_path, int precision,
+                              int steps_per_period)
+{
+    int rc = QJX_OK;
+    RunCfg c;
+    qjx_ctx *ctx = nullptr;
+    std::string sfx;
+    int M = 0, d = 0, jump = 0, dump_obs = 0, dump_evo_sep = 0, dump_evo_avg = 0, dump_adr_sep = 0,
+        dump_adr_avg = 0, dump_phi = 0, dump_phi_evo = 0, jumps_lim = 0, dnt = 0;
+    std::vector<int> dps;
+    std::vector<double> norm_evo, mean_evo, disp_evo, m2_evo, energy_evo, adr, adr_avg, psi;
+    std::vector<double> norm, mean_start, mean, disp, m2, energy;
+    std::vector<int> nj;
+    auto t0 = std::chrono::steady_clock::now();
+
+    if (!config_path || !params_path) {
+        qjx::set_last_error("null path");
+        return QJX_ERR_ARG;
+    }
+    if (!read_files(c, config_path, params_path)) {
+        qjx::set_last_error(g_cfg_err);
+        return QJX_ERR_IO;
+    }
+    {
+        Params p{c.params, ""};
+        if (c.sys_id != 0 || c.task_id != 1 || c.prop_id != 0)
+            CFG_FAIL(QJX_ERR_ARG, "only sys_id 0 (dimer), task_id 1 (STD), prop_id 0 (QJ) are implemented");
+        sfx = suffix_param(c, p);
+        qjx_model_desc md{};
+        md.N = int(p.get("N"));
+        md.prm_E = p.get("dimer_prm_E");
+        md.prm_U = p.get("dimer_prm_U");
+        md.prm_J = p.get("dimer_prm_J");
+        md.drv_ampl = p.get("dimer_drv_ampl");
+        md.drv_freq = p.get("dimer_drv_freq");
+        md.diss_gamma = p.get("diss_gamma");
+        md.ex_deep = c.ex_deep;
+        md.steps_per_period = steps_per_period > 0 ? steps_per_period : 4;
+        md.precision = precision;
+        jump = int(p.get("jump"));
+        jumps_lim = int(p.get("jumps_counts"));
+        dump_obs = int(p.get("dump_obs"));
+        dump_evo_sep = int(p.get("dump_evo_sep"));
+        dump_evo_avg = int(p.get("dump_evo_avg"));
+        dump_adr_sep = int(p.get("dump_adr_sep"));
+        dump_adr_avg = int(p.get("dump_adr_avg"));
+        dump_phi = int(p.get("dump_phi"));
+        dump_phi_evo = int(p.get("dump_phi_evo"));
+        const int dump_type = int(p.get("dump_type"));
+        const int dump_num = int(p.get("dump_num"));
+        const int num_random_obs = int(p.get("num_random_obs"));
+        const int start_type = int(p.get("start_type"));
+        const int start_state = int(p.get("start_state"));
+        if (!p.missing.empty()) CFG_FAIL(QJX_ERR_ARG, "params.txt lacks key " + p.missing);
+        if (num_random_obs != 0)
+            CFG_FAIL(QJX_ERR_ARG, "num_random_obs > 0 (random observables) is not implemented; set it to 0");
+        if (dump_type != 0 && dump_type != 1) CFG_FAIL(QJX_ERR_ARG, "dump_type must be 0 or 1");
+        if (dump_num < 1 || (dump_type == 0 && c.num_obs_periods < dump_num))
+            CFG_FAIL(QJX_ERR_ARG, "dump_num must be in [1, num_obs_periods]");
+        dps = dump_periods(dump_type, dump_num, c.num_obs_periods);
+        dnt = (int)dps.size();
+
+        API_CHECK(qjx_ctx_create(&md, &ctx));
+        M = c.num_trajectories;
+        d = md.N + 1;
+        qjx_ensemble_desc ed{};
+        ed.num_trajectories = M;
+        ed.seed = c.seed;
+        ed.mns = c.mns;
+        ed.start_type = start_type;
+        ed.start_state = start_state;
+        ed.jump = jump;
+        // Logged jumps are drained after every period: the jump rate is at most
+        // gamma (N + 2), so 1.5 gamma (N + 2) T + 64 records per period suffice.
+        const double T = 2.0 * 3.1415926535897932384626433832795 / md.drv_freq;
+        ed.log_capacity = jump > 0 ? int(std::ceil(1.5 * md.diss_gamma * (md.N + 2) * T)) + 64 : 0;
+        API_CHECK(qjx_ensemble_init(ctx, &ed));
+
+        const std::string &path = c.path;
+        norm.assign(M, 0.0);
+        mean_start.assign(M, 0.0);
+        mean.assign(M, 0.0);
+        disp.assign(M, 0.0);
+        m2.assign(M, 0.0);
+        energy.assign(M, 0.0);
+        norm_evo.assign((size_t)M * dnt, 0.0);
+        mean_evo = disp_evo = m2_evo = energy_evo = norm_evo;
+        const bool need_adr_single = dump_evo_sep == 1 && dump_adr_sep == 1;
+        if (need_adr_single) adr.assign((size_t)M * d, 0.0);
+        adr_avg.assign(d, 0.0);
+        nj.assign(M, 0);
+
+        auto chars = [&](int is_start, int dump_id) -> int {
+            int r = qjx_chars(ctx, is_start, norm.data(), mean.data(), disp.data(), m2.data(), energy.data(),
+                              adr_avg.data(), need_adr_single ? adr.data() : nullptr);
+            if (r < 0) return r;
+            if (is_start) mean_start = mean;
+            for (int tr = 0; tr < M; tr++) {
+                const size_t i = (size_t)tr * dnt + dump_id;   // evo_chars_std (core.cpp:795-816)
+                norm_evo[i] = norm[tr];
+                mean_evo[i] = mean[tr];
+                disp_evo[i] = disp[tr];
+                m2_evo[i] = m2[tr];
+                energy_evo[i] = energy[tr];
+            }
+            return 0;
+        };
+        auto adr_single = [&](bool append) -> bool {
+            if (!need_adr_single) return true;
+            for (int tr = 0; tr < M; tr++)
+                if (!save_doubles(path + "adr_" + std::to_string(tr) + sfx, &adr[(size_t)tr * d], d, append))
+                    return false;
+            return true;
+        };
+        auto adr_avg_dump = [&](bool append) -> bool {
+            if (dump_adr_avg != 1) return true;
+            return save_doubles(path + "adr_avg" + sfx, adr_avg.data(), d, append);
+        };
+        auto phi_evo = [&](bool append) -> int {
+            if (dump_phi_evo != 1) return 1;
+            psi.assign((size_t)M * d * 2, 0.0);
+            if (qjx_get_state(ctx, psi.data(), nullptr, nullptr, nullptr, nullptr) < 0) return -1;
+            for (int tr = 0; tr < M; tr++)
+                if (!save_complex(path + "phi_evo_" + std::to_string(tr) + sfx, &psi[(size_t)tr * d * 2], d, append))
+                    return 0;
+            return 1;
+        };
+
+        // trans_process (experiment.cpp:411-446)
+        API_CHECK(qjx_trans(ctx, c.num_tp_periods));
+        API_CHECK(chars(1, 0));
+        IO_CHECK(adr_single(false));
+        if (dump_evo_avg == 1) IO_CHECK(adr_avg_dump(false));
+
+        // obser_process (experiment.cpp:448-542)
+        {
+            int pr = phi_evo(false);
+            if (pr < 0) CFG_FAIL(QJX_ERR_HIP, qjx_last_error());
+            IO_CHECK(pr);
+        }
+        bool all_finished = false;
+        for (int dump_id = 1; dump_id < dnt; dump_id++) {
+            if (c.is_pp == 1) std::cout << "dump_id: " << dump_id << std::endl;
+            const int n = dps[dump_id] - dps[dump_id - 1];
+            const bool per_period = jump > 0 || jumps_lim > 0;
+            if (per_period) {
+                for (int k = 0; k < n; k++) {
+                    API_CHECK(qjx_run_periods(ctx, 1, 1));
+                    if (jumps_lim > 0) {
+                        API_CHECK(qjx_get_state(ctx, nullptr, nullptr, nullptr, nj.data(), nullptr));
+                        all_finished = std::all_of(nj.begin(), nj.end(), [&](int v) { return v >= jumps_lim; });
+                        if (all_finished) break;
+                    }
+                }
+            } else if (n > 0) {
+                API_CHECK(qjx_run_periods(ctx, n, 1));
+            }
+            API_CHECK(chars(0, dump_id));
+            IO_CHECK(adr_single(true));
+            {
+                int pr = phi_evo(true);
+                if (pr < 0) CFG_FAIL(QJX_ERR_HIP, qjx_last_error());
+                IO_CHECK(pr);
+            }
+            if (dump_evo_avg == 1) IO_CHECK(adr_avg_dump(true));
+            if (all_finished && jumps_lim > 0) break;
+        }
+
+        // dump_std (core.cpp:918-957)
+        if (dump_obs == 1) {
+            IO_CHECK(save_doubles(path + "norm" + sfx, norm.data(), M, false));
+            IO_CHECK(save_doubles(path + "mean_start" + sfx, mean_start.data(), M, false));
+            IO_CHECK(save_doubles(path + "mean" + sfx, mean.data(), M, false));
+            IO_CHECK(save_doubles(path + "dispersion" + sfx, disp.data(), M, false));
+            IO_CHECK(save_doubles(path + "m2" + sfx, m2.data(), M, false));
+            IO_CHECK(save_doubles(path + "energy" + sfx, energy.data(), M, false));
+        }
+        // dump_phi (core.cpp:12567-12589)
+        if (dump_phi == 1) {
+            psi.assign((size_t)M * d * 2, 0.0);
+            API_CHECK(qjx_get_state(ctx, psi.data(), nullptr, nullptr, nullptr, nullptr));
+            for (int tr = 0; tr < M; tr++)
+                IO_CHECK(save_complex(path + "phi_" + std::to_string(tr) + sfx, &psi[(size_t)tr * d * 2], d, false));
+        }
+        {
+            int pr = phi_evo(true);
+            if (pr < 0) CFG_FAIL(QJX_ERR_HIP, qjx_last_error());
+            IO_CHECK(pr);
+        }
+        // dump_std_evo (core.cpp:1008-1084)
+        if (dump_evo_sep == 1 && dump_obs == 1) {
+            IO_CHECK(save_ints(path + "periods" + sfx, dps.data(), dnt, false));
+            IO_CHECK(save_2d_inv(path + "norm_evo" + sfx, norm_evo, dnt, M));
+            IO_CHECK(save_2d_inv(path + "mean_evo" + sfx, mean_evo, dnt, M));
+            IO_CHECK(save_2d_inv(path + "dispersion_evo" + sfx, disp_evo, dnt, M));
+            IO_CHECK(save_2d_inv(path + "m2_evo" + sfx, m2_evo, dnt, M));
+            IO_CHECK(save_2d_inv(path + "energy_evo" + sfx, energy_evo, dnt, M));
+            if (jump > 0) {
+                for (int tr = 0; tr < M; tr++) {
+                    const int n = qjx_get_jumps(ctx, tr, nullptr, nullptr, nullptr, nullptr, 0);
+                    if (n < 0) CFG_FAIL(n, qjx_last_error());
+                    std::vector<double> t(n), nr(n), et(n);
+                    std::vector<int> ty(n);
+                    API_CHECK(qjx_get_jumps(ctx, tr, t.data(), ty.data(), nr.data(), et.data(), n));
+                    IO_CHECK(save_doubles(path + "jump_times_" + std::to_string(tr) + sfx, t.data(), n, false));
+                    IO_CHECK(save_ints(path + "diss_types_" + std::to_string(tr) + sfx, ty.data(), n, false));
+                    if (jump > 1) {
+                        IO_CHECK(save_doubles(path + "jump_norms_" + std::to_string(tr) + sfx, nr.data(), n, false));
+                        IO_CHECK(save_doubles(path + "jump_etas_" + std::to_string(tr) + sfx, et.data(), n, false));
+                    }
+                }
+            }
+        }
+        if (dump_evo_avg == 0) IO_CHECK(adr_avg_dump(false));
+    }
+done:
+    if (ctx) qjx_ctx_destroy(ctx);
+    if (rc != QJX_OK) qjx::set_last_error(g_cfg_err);
+    if (rc == QJX_OK) {
+        const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::cout << "Elapsed time: " << el << std::endl;
+    }
+    return rc;
+}

@@ -1,0 +1,710 @@
+// qjx_engine.cpp -- host side of the MI355X trajectory engine behind include/qjx.h.
+//
+// Owns the device memory of one dimer model (propagator tree) and one ensemble,
+// and drives the grouped step iterations until every trajectory has finished
+// the requested periods.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/qjx.h"
+#include "qjx_internal.h"
+#include "qjx_kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(e_ == hipErrorOutOfMemory ? QJX_ERR_NOMEM : QJX_ERR_HIP,         \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));              \
+    } while (0)
+
+const double kPi = 3.1415926535897932384626433832795;   // reference config.h PI
+const uint64_t kMcgA = 302875106592253ULL;              // 13^13
+
+uint64_t mcg_pow_host(uint64_t a, uint64_t e)
+{
+    const uint64_t mask = (1ULL << 59) - 1ULL;
+    uint64_t r = 1;
+    while (e) {
+        if (e & 1ULL) r = (r * a) & mask;
+        a = (a * a) & mask;
+        e >>= 1;
+    }
+    return r;
+}
+
+template <typename T>
+void dev_free(T *&p)
+{
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+}  // namespace
+
+struct qjx_ctx {
+    qjx_model_desc model{};
+    int d = 0, dp = 0, L = 0, D = 0, S = 0, G = 0, R = 0;
+    int prec = 0;
+    size_t esize = 8;
+    double T = 0.0;
+    std::vector<double> dt;
+    hipStream_t stream = nullptr;
+
+    // model (host)
+    std::vector<double> Hdiag, Hoff, Hdrv, Ldiag, Loff, HdiagE;
+    std::vector<double> Hqj[2];    // dense interleaved complex d x d
+
+    // device model
+    void *props = nullptr;         // 2*D * L*L elements
+    double *d_dt = nullptr, *d_Ldiag = nullptr, *d_Loff = nullptr;
+    double *d_HdiagE = nullptr, *d_Hoff = nullptr;
+
+    // ensemble
+    int M = 0;
+    qjx_ensemble_desc ens{};
+    uint64_t stride = 0;
+    void *slot[2] = {nullptr, nullptr};
+    qjx::TrajState *st = nullptr;
+    int *lists[2] = {nullptr, nullptr};
+    int *cnts[2] = {nullptr, nullptr};
+    double *normpart = nullptr;
+    int *err = nullptr;
+    unsigned long long *counters = nullptr;
+    qjx::JumpLog log{};
+    double *mean_start = nullptr, *obs = nullptr, *adr = nullptr, *adr_avg = nullptr;
+    double *xfer = nullptr;        // export/import buffer (M x d complex)
+    int *h_cnt = nullptr;          // pinned
+    bool have_ens = false;
+    int parity = 0;
+
+    // host jump logs
+    std::vector<std::vector<double>> jt, jn, je;
+    std::vector<std::vector<int>> jty;
+
+    int64_t iterations = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Model (newdel.cpp:3-37, 450-498, 1454-1498, 2285-2388)
+// ---------------------------------------------------------------------------
+static void build_model(qjx_ctx *c)
+{
+    const qjx_model_desc &m = c->model;
+    const int d = c->d;
+    const double U = m.prm_U / double(d - 1);
+    c->Hdiag.assign(d, 0.0);
+    c->Hdrv.assign(d, 0.0);
+    c->Ldiag.assign(d, 0.0);
+    c->Hoff.assign(std::max(d - 1, 1), 0.0);
+    c->Loff.assign(std::max(d - 1, 1), 0.0);
+    for (int k = 0; k < d; k++) {
+        c->Hdiag[k] = 2.0 * U * double(k * (k - 1) + (d - (k + 1)) * (d - (k + 1) - 1));
+        c->Hdrv[k] = double((d - (k + 1)) - k);
+        c->Ldiag[k] = double(k - (d - (k + 1)));
+    }
+    for (int k = 0; k + 1 < d; k++) {
+        c->Hoff[k] = 0.0 - m.prm_J * std::sqrt(double((d - (k + 1)) * (k + 1)));
+        c->Loff[k] = std::sqrt(double((d - (k + 1)) * (k + 1)));
+    }
+    // Observables use H + E * H_drv with the static E (experiment.cpp:1446).
+    c->HdiagE.assign(d, 0.0);
+    for (int k = 0; k < d; k++) c->HdiagE[k] = c->Hdiag[k] + m.prm_E * c->Hdrv[k];
+
+    // Dense L (row-major): L_kk, L_{k+1,k} = -Loff[k], L_{k,k+1} = +Loff[k].
+    auto Lel = [&](int i, int j) -> double {
+        if (i == j) return c->Ldiag[i];
+        if (i == j + 1) return -c->Loff[j];
+        if (j == i + 1) return c->Loff[i];
+        return 0.0;
+    };
+    const double g = m.diss_gamma / double(d - 1);
+    const double E[2] = {m.prm_E + m.drv_ampl, m.prm_E - m.drv_ampl};
+    for (int b = 0; b < 2; b++) c->Hqj[b].assign((size_t)2 * d * d, 0.0);
+    for (int i = 0; i < d; i++) {
+        for (int j = std::max(0, i - 2); j <= std::min(d - 1, i + 2); j++) {
+            // (L^dag L)_ij = sum_k L_ki L_kj, ascending k as the reference zgemm
+            double s = 0.0;
+            for (int k = std::max(0, std::max(i, j) - 1); k <= std::min(d - 1, std::min(i, j) + 1); k++)
+                s += Lel(k, i) * Lel(k, j);
+            const double ldl = g * s;
+            double h = 0.0, hdrv = 0.0;
+            if (i == j) {
+                h = c->Hdiag[i];
+                hdrv = c->Hdrv[i];
+            } else if (i == j + 1) {
+                h = c->Hoff[j];
+            } else if (j == i + 1) {
+                h = c->Hoff[i];
+            }
+            const double hre = h + 0.5 * 0.0;          // Im(diss_part) = 0
+            const double him = 0.0 - 0.5 * ldl;
+            for (int b = 0; b < 2; b++) {
+                const size_t o = 2 * ((size_t)i * d + j);
+                c->Hqj[b][o] = him + E[b] * 0.0;
+                c->Hqj[b][o + 1] = 0.0 - (hre + E[b] * hdrv);
+            }
+        }
+    }
+}
+
+// exp(A dt) for the dense complex A on the device by scaling and squaring of a
+// degree-18 Taylor polynomial evaluated with the Paterson-Stockmeyer scheme
+// (M. S. Paterson, L. J. Stockmeyer, SIAM J. Comput. 2 (1973) 60-66):
+// ||A dt / 2^s||_1 <= 1 bounds the truncation error by 1/19! * e < 3e-17.
+static int device_expm(qjx_ctx *c, const double *dA, double anorm1, double dt, double *out,
+                       double *work)
+{
+    const int n = c->d;
+    const size_t nn2 = (size_t)2 * n * n;
+    double *X = work, *X2 = work + nn2, *X3 = work + 2 * nn2, *X4 = work + 3 * nn2,
+           *P = work + 4 * nn2, *Tm = work + 5 * nn2;
+    const double a1 = anorm1 * dt;
+    int s = 0;
+    if (a1 > 1.0) s = (int)std::ceil(std::log2(a1));
+    hipStream_t st = c->stream;
+    qjx::launch_zaxpby_eye(n, dA, std::ldexp(dt, -s), 0.0, X, st);
+    qjx::launch_zgemm(n, X, X, X2, st);
+    qjx::launch_zgemm(n, X2, X, X3, st);
+    qjx::launch_zgemm(n, X2, X2, X4, st);
+    double cf[19];
+    cf[0] = 1.0;
+    for (int j = 1; j <= 18; j++) cf[j] = cf[j - 1] / double(j);
+    // P = c16 I + c17 X + c18 X2
+    qjx::launch_zaxpby_eye(n, X, cf[17], cf[16], P, st);
+    qjx::launch_zaxpy(n, X2, cf[18], P, st);
+    for (int blk = 3; blk >= 0; blk--) {
+        qjx::launch_zgemm(n, X4, P, Tm, st);
+        std::swap(P, Tm);
+        // P += c_{4blk} I + c_{4blk+1} X + c_{4blk+2} X2 + c_{4blk+3} X3
+        qjx::launch_zaxpby_eye(n, X, cf[4 * blk + 1], cf[4 * blk], Tm, st);
+        qjx::launch_zaxpy(n, Tm, 1.0, P, st);
+        qjx::launch_zaxpy(n, X2, cf[4 * blk + 2], P, st);
+        qjx::launch_zaxpy(n, X3, cf[4 * blk + 3], P, st);
+    }
+    for (int k = 0; k < s; k++) {
+        qjx::launch_zgemm(n, P, P, Tm, st);
+        std::swap(P, Tm);
+    }
+    HIP_TRY(hipMemcpyAsync(out, P, nn2 * sizeof(double), hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipGetLastError());
+    return QJX_OK;
+}
+
+static double norm1_host(const std::vector<double> &A, int n)
+{
+    double best = 0.0;
+    for (int j = 0; j < n; j++) {
+        double s = 0.0;
+        for (int i = 0; i < n; i++) {
+            const size_t o = 2 * ((size_t)i * n + j);
+            s += std::hypot(A[o], A[o + 1]);
+        }
+        best = std::max(best, s);
+    }
+    return best;
+}
+
+static void *prop_ptr(qjx_ctx *c, int b, int lvl)
+{
+    return (char *)c->props + ((size_t)(b * c->D + lvl)) * c->L * c->L * c->esize;
+}
+
+extern "C" const char *qjx_last_error(void) { return g_err.c_str(); }
+
+namespace qjx {
+void set_last_error(const std::string &msg) { g_err = msg; }
+}
+
+extern "C" int qjx_dim(const qjx_ctx *c) { return c ? c->d : -1; }
+
+static int upload(void **dst, const void *src, size_t bytes)
+{
+    HIP_TRY(hipMalloc(dst, bytes));
+    HIP_TRY(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    return QJX_OK;
+}
+
+extern "C" int qjx_ctx_create(const qjx_model_desc *m, qjx_ctx **out)
+{
+    if (!m || !out) return fail(QJX_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (m->N < 1) return fail(QJX_ERR_ARG, "N must be >= 1");
+    if (m->ex_deep < 1 || m->ex_deep > qjx::kMaxGroups / 2 || m->ex_deep > 30)
+        return fail(QJX_ERR_ARG, "ex_deep must be in [1, 30]");
+    const int S = m->steps_per_period ? m->steps_per_period : 4;
+    if (S < 2 || (S % 2)) return fail(QJX_ERR_ARG, "steps_per_period must be even and >= 2");
+    if (m->precision != QJX_FP64 && m->precision != QJX_FP32)
+        return fail(QJX_ERR_ARG, "precision form not available in this build");
+    if (!(m->drv_freq > 0.0)) return fail(QJX_ERR_ARG, "dimer_drv_freq must be > 0");
+    int dev_count = 0;
+    if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count < 1)
+        return fail(QJX_ERR_HIP, "no HIP device available");
+
+    qjx_ctx *c = new qjx_ctx;
+    c->model = *m;
+    c->model.steps_per_period = S;
+    c->d = m->N + 1;
+    c->dp = (c->d + 31) / 32 * 32;
+    c->L = 2 * c->dp;
+    c->D = m->ex_deep;
+    c->S = S;
+    c->G = 2 * c->D;
+    c->R = c->L / 64;
+    c->prec = m->precision;
+    c->esize = c->prec == QJX_FP64 ? 8 : 4;
+    c->T = 2.0 * kPi / m->drv_freq;
+    c->dt.resize(c->D);
+    double dt = c->T / double(S);
+    for (int i = 0; i < c->D; i++) {
+        c->dt[i] = dt;
+        dt *= 0.5;
+    }
+    build_model(c);
+    int rc = QJX_OK;
+    auto bail = [&](int code) {
+        qjx_ctx_destroy(c);
+        return code;
+    };
+    if (hipStreamCreate(&c->stream) != hipSuccess)
+        return bail(fail(QJX_ERR_HIP, "hipStreamCreate failed"));
+    if ((rc = upload((void **)&c->d_dt, c->dt.data(), sizeof(double) * c->D))) return bail(rc);
+    if ((rc = upload((void **)&c->d_Ldiag, c->Ldiag.data(), sizeof(double) * c->d))) return bail(rc);
+    if ((rc = upload((void **)&c->d_Loff, c->Loff.data(), sizeof(double) * c->Loff.size()))) return bail(rc);
+    if ((rc = upload((void **)&c->d_HdiagE, c->HdiagE.data(), sizeof(double) * c->d))) return bail(rc);
+    if ((rc = upload((void **)&c->d_Hoff, c->Hoff.data(), sizeof(double) * c->Hoff.size()))) return bail(rc);
+    const size_t propbytes = (size_t)2 * c->D * c->L * c->L * c->esize;
+    if (hipMalloc(&c->props, propbytes) != hipSuccess)
+        return bail(fail(QJX_ERR_NOMEM, "propagator tree does not fit in device memory"));
+
+    // Propagator tree (qj_proc.cpp:4-120): U[b][i] = exp(Hqj[b] * dt_i).
+    const size_t nn2 = (size_t)2 * c->d * c->d;
+    double *dA = nullptr, *work = nullptr, *U = nullptr;
+    if (hipMalloc((void **)&dA, nn2 * sizeof(double)) != hipSuccess ||
+        hipMalloc((void **)&work, 6 * nn2 * sizeof(double)) != hipSuccess ||
+        hipMalloc((void **)&U, nn2 * sizeof(double)) != hipSuccess) {
+        dev_free(dA);
+        dev_free(work);
+        dev_free(U);
+        return bail(fail(QJX_ERR_NOMEM, "propagator build workspace does not fit"));
+    }
+    for (int b = 0; b < 2 && rc == QJX_OK; b++) {
+        const double a1 = norm1_host(c->Hqj[b], c->d);
+        // ordered on the engine stream: the previous branch's expm still reads dA
+        if (hipMemcpyAsync(dA, c->Hqj[b].data(), nn2 * sizeof(double), hipMemcpyHostToDevice, c->stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess) {
+            rc = fail(QJX_ERR_HIP, "upload of H_eff failed");
+            break;
+        }
+        for (int i = 0; i < c->D && rc == QJX_OK; i++) {
+            rc = device_expm(c, dA, a1, c->dt[i], U, work);
+            if (rc == QJX_OK) qjx::launch_pack_propagator(c->prec, c->d, c->dp, U, prop_ptr(c, b, i), nullptr, c->stream);
+        }
+    }
+    if (rc == QJX_OK && hipStreamSynchronize(c->stream) != hipSuccess)
+        rc = fail(QJX_ERR_HIP, "propagator build failed");
+    dev_free(dA);
+    dev_free(work);
+    dev_free(U);
+    if (rc != QJX_OK) return bail(rc);
+    *out = c;
+    return QJX_OK;
+}
+
+static void free_ensemble(qjx_ctx *c)
+{
+    for (int i = 0; i < 2; i++) {
+        if (c->slot[i]) (void)hipFree(c->slot[i]);
+        c->slot[i] = nullptr;
+        dev_free(c->lists[i]);
+        dev_free(c->cnts[i]);
+    }
+    dev_free(c->st);
+    dev_free(c->normpart);
+    dev_free(c->err);
+    dev_free(c->counters);
+    dev_free(c->log.time);
+    dev_free(c->log.norm);
+    dev_free(c->log.eta);
+    dev_free(c->log.type);
+    dev_free(c->mean_start);
+    dev_free(c->obs);
+    dev_free(c->adr);
+    dev_free(c->adr_avg);
+    dev_free(c->xfer);
+    if (c->h_cnt) (void)hipHostFree(c->h_cnt);
+    c->h_cnt = nullptr;
+    c->have_ens = false;
+}
+
+extern "C" void qjx_ctx_destroy(qjx_ctx *c)
+{
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_ensemble(c);
+    if (c->props) (void)hipFree(c->props);
+    dev_free(c->d_dt);
+    dev_free(c->d_Ldiag);
+    dev_free(c->d_Loff);
+    dev_free(c->d_HdiagE);
+    dev_free(c->d_Hoff);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+static int alloc_log(qjx_ctx *c, int cap)
+{
+    dev_free(c->log.time);
+    dev_free(c->log.norm);
+    dev_free(c->log.eta);
+    dev_free(c->log.type);
+    c->log.cap = 0;
+    if (c->ens.jump <= 0 || cap <= 0) return QJX_OK;
+    const size_t n = (size_t)c->M * cap;
+    HIP_TRY(hipMalloc((void **)&c->log.time, n * sizeof(double)));
+    HIP_TRY(hipMalloc((void **)&c->log.norm, n * sizeof(double)));
+    HIP_TRY(hipMalloc((void **)&c->log.eta, n * sizeof(double)));
+    HIP_TRY(hipMalloc((void **)&c->log.type, n * sizeof(int)));
+    c->log.cap = cap;
+    return QJX_OK;
+}
+
+extern "C" int qjx_ensemble_init(qjx_ctx *c, const qjx_ensemble_desc *e)
+{
+    if (!c || !e) return fail(QJX_ERR_ARG, "null argument");
+    if (e->num_trajectories < 1) return fail(QJX_ERR_ARG, "num_trajectories must be >= 1");
+    if (e->mns < 1) return fail(QJX_ERR_ARG, "mns must be >= 1");
+    if (e->start_type == 0 && (e->start_state < 0 || e->start_state >= c->d))
+        return fail(QJX_ERR_ARG, "start_state out of range");
+    free_ensemble(c);
+    c->ens = *e;
+    c->M = e->num_trajectories;
+    c->stride = mcg_pow_host(kMcgA, (uint64_t)e->mns);
+    const size_t M = c->M;
+    const size_t slotbytes = M * c->L * c->esize;
+    HIP_TRY(hipMalloc(&c->slot[0], slotbytes));
+    HIP_TRY(hipMalloc(&c->slot[1], slotbytes));
+    HIP_TRY(hipMalloc((void **)&c->st, M * sizeof(qjx::TrajState)));
+    for (int i = 0; i < 2; i++) {
+        HIP_TRY(hipMalloc((void **)&c->lists[i], (size_t)c->G * M * sizeof(int)));
+        HIP_TRY(hipMalloc((void **)&c->cnts[i], qjx::kMaxGroups * sizeof(int)));
+        HIP_TRY(hipMemset(c->cnts[i], 0, qjx::kMaxGroups * sizeof(int)));
+    }
+    HIP_TRY(hipMalloc((void **)&c->normpart, (size_t)c->R * M * sizeof(double)));
+    HIP_TRY(hipMalloc((void **)&c->err, 4 * sizeof(int)));
+    HIP_TRY(hipMemset(c->err, 0, 4 * sizeof(int)));
+    HIP_TRY(hipMalloc((void **)&c->counters, 2 * sizeof(unsigned long long)));
+    HIP_TRY(hipMemset(c->counters, 0, 2 * sizeof(unsigned long long)));
+    HIP_TRY(hipMalloc((void **)&c->mean_start, M * sizeof(double)));
+    HIP_TRY(hipMemset(c->mean_start, 0, M * sizeof(double)));
+    HIP_TRY(hipMalloc((void **)&c->obs, 5 * M * sizeof(double)));
+    HIP_TRY(hipHostMalloc((void **)&c->h_cnt, qjx::kMaxGroups * sizeof(int), hipHostMallocDefault));
+    const int cap = e->log_capacity > 0 ? e->log_capacity : 256;
+    int rc = alloc_log(c, cap);
+    if (rc) return rc;
+    c->jt.assign(M, {});
+    c->jn.assign(M, {});
+    c->je.assign(M, {});
+    c->jty.assign(M, {});
+    qjx::launch_init_ensemble(c->prec, c->slot[0], c->slot[1], c->st, c->M, c->d, c->dp, c->L,
+                              e->seed, e->start_type, e->start_state, c->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->parity = 0;
+    c->have_ens = true;
+    return QJX_OK;
+}
+
+static qjx::StepArgs step_args(qjx_ctx *c, int p, int is_obs)
+{
+    qjx::StepArgs a{};
+    a.props = c->props;
+    a.props_lo = nullptr;
+    a.slot[0] = c->slot[0];
+    a.slot[1] = c->slot[1];
+    a.st = c->st;
+    a.list = c->lists[p];
+    a.cnt = c->cnts[p];
+    a.list_next = c->lists[p ^ 1];
+    a.cnt_next = c->cnts[p ^ 1];
+    a.cnt_clear = c->cnts[p ^ 1];
+    a.normpart = c->normpart;
+    a.dt = c->d_dt;
+    a.Ldiag = c->d_Ldiag;
+    a.Loff = c->d_Loff;
+    a.log = c->log;
+    a.err = c->err;
+    a.counters = c->counters;
+    a.d = c->d;
+    a.dp = c->dp;
+    a.L = c->L;
+    a.D = c->D;
+    a.S = c->S;
+    a.M = c->M;
+    a.G = c->G;
+    a.R = c->R;
+    a.is_obs = is_obs;
+    a.jump = c->ens.jump;
+    a.norm_tol = c->prec == QJX_FP64 ? 0.0 : 1e-4;
+    return a;
+}
+
+static int check_err(qjx_ctx *c)
+{
+    int h[4];
+    HIP_TRY(hipMemcpy(h, c->err, sizeof h, hipMemcpyDeviceToHost));
+    if (h[0]) {
+        const unsigned long long bits = (unsigned long long)(unsigned)h[2] | ((unsigned long long)(unsigned)h[3] << 32);
+        double n;
+        std::memcpy(&n, &bits, sizeof n);
+        char buf[200];
+        std::snprintf(buf, sizeof buf, "norm more than 1: trajectory %d, ||psi||^2 = %.17g", h[1], n);
+        return fail(QJX_ERR_NORM, buf);
+    }
+    return QJX_OK;
+}
+
+// Copy the device jump log into the host vectors and reset the device counters.
+static int drain_log(qjx_ctx *c)
+{
+    if (c->log.cap <= 0) return QJX_OK;
+    const size_t M = c->M, cap = c->log.cap;
+    std::vector<qjx::TrajState> hs(M);
+    HIP_TRY(hipMemcpy(hs.data(), c->st, M * sizeof(qjx::TrajState), hipMemcpyDeviceToHost));
+    bool any = false, over = false;
+    for (size_t tr = 0; tr < M; tr++) {
+        any = any || hs[tr].logn > 0;
+        over = over || hs[tr].logn > (int)cap;
+    }
+    if (!any) return QJX_OK;
+    std::vector<double> t(M * cap), n(M * cap), e(M * cap);
+    std::vector<int> ty(M * cap);
+    HIP_TRY(hipMemcpy(t.data(), c->log.time, t.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(n.data(), c->log.norm, n.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(e.data(), c->log.eta, e.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(ty.data(), c->log.type, ty.size() * sizeof(int), hipMemcpyDeviceToHost));
+    for (size_t tr = 0; tr < M; tr++) {
+        const int k = std::min(hs[tr].logn, (int)cap);
+        for (int i = 0; i < k; i++) {
+            c->jt[tr].push_back(t[tr * cap + i]);
+            c->jn[tr].push_back(n[tr * cap + i]);
+            c->je[tr].push_back(e[tr * cap + i]);
+            c->jty[tr].push_back(ty[tr * cap + i]);
+        }
+    }
+    qjx::launch_reset_logn(c->st, c->M, c->stream);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (over) return fail(QJX_ERR_LOG, "jump log overflow: raise log_capacity or drain more often");
+    return QJX_OK;
+}
+
+// Advance every trajectory by n top-level steps: seed the lists, then launch
+// iterations in chunks until no trajectory has steps left.
+static int run_steps(qjx_ctx *c, int n, int is_obs)
+{
+    if (n <= 0) return QJX_OK;
+    c->parity = 0;
+    qjx::launch_seed_lists(c->st, c->M, n, c->lists[0], c->cnts[0], c->G, c->stream);
+    HIP_TRY(hipGetLastError());
+    int chunk = 8;
+    for (;;) {
+        for (int k = 0; k < chunk; k++) {
+            qjx::StepArgs a = step_args(c, c->parity, is_obs);
+            qjx::launch_step(c->prec, 0, a, c->stride, c->stream);
+            c->parity ^= 1;
+            c->iterations++;
+        }
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(c->h_cnt, c->cnts[c->parity], c->G * sizeof(int), hipMemcpyDeviceToHost,
+                               c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        long long active = 0;
+        for (int g = 0; g < c->G; g++) active += c->h_cnt[g];
+        if (active == 0) break;
+        chunk = std::min(chunk * 2, 256);
+    }
+    return check_err(c);
+}
+
+extern "C" int qjx_trans(qjx_ctx *c, int num_tp_periods)
+{
+    if (!c || !c->have_ens) return fail(QJX_ERR_ARG, "ensemble not initialised");
+    qjx::launch_draw_eta(c->st, c->M, c->stride, c->stream);
+    int rc = run_steps(c, num_tp_periods * c->S, 0);
+    if (rc) return rc;
+    qjx::launch_reset_times(c->st, c->M, c->stream);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return QJX_OK;
+}
+
+extern "C" int qjx_run_periods(qjx_ctx *c, int n, int is_obs)
+{
+    if (!c || !c->have_ens) return fail(QJX_ERR_ARG, "ensemble not initialised");
+    if (n < 0) return fail(QJX_ERR_ARG, "negative period count");
+    int rc = run_steps(c, n * c->S, is_obs);
+    if (rc) return rc;
+    if (is_obs && c->ens.jump > 0) return drain_log(c);
+    return QJX_OK;
+}
+
+extern "C" int qjx_run_steps(qjx_ctx *c, int n, int is_obs)
+{
+    if (!c || !c->have_ens) return fail(QJX_ERR_ARG, "ensemble not initialised");
+    if (n < 0) return fail(QJX_ERR_ARG, "negative step count");
+    int rc = run_steps(c, n, is_obs);
+    if (rc) return rc;
+    if (is_obs && c->ens.jump > 0) return drain_log(c);
+    return QJX_OK;
+}
+
+extern "C" int qjx_chars(qjx_ctx *c, int is_start, double *norm, double *mean, double *dispersion,
+                         double *m2, double *energy, double *adr_avg, double *adr)
+{
+    if (!c || !c->have_ens) return fail(QJX_ERR_ARG, "ensemble not initialised");
+    const size_t M = c->M;
+    const bool need_adr = adr_avg || adr;
+    if (need_adr && !c->adr) {
+        HIP_TRY(hipMalloc((void **)&c->adr, M * c->d * sizeof(double)));
+        HIP_TRY(hipMalloc((void **)&c->adr_avg, (size_t)(1 + 64) * c->d * sizeof(double)));
+    }
+    double *o = c->obs;
+    qjx::launch_chars(c->prec, c->slot[0], c->slot[1], c->st, c->M, c->d, c->dp, c->L, c->d_HdiagE,
+                      c->d_Hoff, is_start, c->mean_start, o, o + M, o + 2 * M, o + 3 * M, o + 4 * M,
+                      need_adr ? c->adr : nullptr, c->stream);
+    if (adr_avg) qjx::launch_adr_avg(c->adr, c->M, c->d, c->adr_avg, c->stream);
+    HIP_TRY(hipGetLastError());
+    double *dst[5] = {norm, mean, dispersion, m2, energy};
+    for (int i = 0; i < 5; i++)
+        if (dst[i]) HIP_TRY(hipMemcpyAsync(dst[i], o + i * M, M * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (adr_avg) HIP_TRY(hipMemcpyAsync(adr_avg, c->adr_avg, c->d * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (adr) HIP_TRY(hipMemcpyAsync(adr, c->adr, M * c->d * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return QJX_OK;
+}
+
+extern "C" int qjx_get_state(qjx_ctx *c, double *psi, double *eta, double *t, int *njumps, uint64_t *rng)
+{
+    if (!c || !c->have_ens) return fail(QJX_ERR_ARG, "ensemble not initialised");
+    const size_t M = c->M;
+    if (psi) {
+        if (!c->xfer) HIP_TRY(hipMalloc((void **)&c->xfer, M * c->d * 2 * sizeof(double)));
+        qjx::launch_export_state(c->prec, c->slot[0], c->slot[1], c->st, c->M, c->d, c->dp, c->L, c->xfer, c->stream);
+        HIP_TRY(hipMemcpyAsync(psi, c->xfer, M * c->d * 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    }
+    std::vector<qjx::TrajState> hs(M);
+    HIP_TRY(hipMemcpyAsync(hs.data(), c->st, M * sizeof(qjx::TrajState), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (size_t tr = 0; tr < M; tr++) {
+        if (eta) eta[tr] = hs[tr].eta;
+        if (t) t[tr] = hs[tr].t;
+        if (njumps) njumps[tr] = hs[tr].njumps;
+        if (rng) rng[tr] = hs[tr].rng;
+    }
+    return QJX_OK;
+}
+
+extern "C" int qjx_set_state(qjx_ctx *c, const double *psi, const double *eta, const double *t,
+                             const uint64_t *rng)
+{
+    if (!c || !c->have_ens) return fail(QJX_ERR_ARG, "ensemble not initialised");
+    const size_t M = c->M;
+    std::vector<qjx::TrajState> hs(M);
+    HIP_TRY(hipMemcpy(hs.data(), c->st, M * sizeof(qjx::TrajState), hipMemcpyDeviceToHost));
+    for (size_t tr = 0; tr < M; tr++) {
+        if (eta) hs[tr].eta = eta[tr];
+        if (t) hs[tr].t = t[tr];
+        if (rng) hs[tr].rng = rng[tr];
+    }
+    HIP_TRY(hipMemcpy(c->st, hs.data(), M * sizeof(qjx::TrajState), hipMemcpyHostToDevice));
+    if (psi) {
+        if (!c->xfer) HIP_TRY(hipMalloc((void **)&c->xfer, M * c->d * 2 * sizeof(double)));
+        HIP_TRY(hipMemcpy(c->xfer, psi, M * c->d * 2 * sizeof(double), hipMemcpyHostToDevice));
+        qjx::launch_import_state(c->prec, c->slot[0], c->slot[1], c->st, c->M, c->d, c->dp, c->L, c->xfer, c->stream);
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    return QJX_OK;
+}
+
+extern "C" int qjx_get_jumps(qjx_ctx *c, int tr, double *times, int *types, double *norms, double *etas, int cap)
+{
+    if (!c || !c->have_ens) return fail(QJX_ERR_ARG, "ensemble not initialised");
+    if (tr < 0 || tr >= c->M) return fail(QJX_ERR_ARG, "trajectory out of range");
+    const int n = (int)c->jt[tr].size();
+    for (int i = 0; i < n && i < cap; i++) {
+        if (times) times[i] = c->jt[tr][i];
+        if (types) types[i] = c->jty[tr][i];
+        if (norms) norms[i] = c->jn[tr][i];
+        if (etas) etas[i] = c->je[tr][i];
+    }
+    return n;
+}
+
+extern "C" int qjx_get_propagator(qjx_ctx *c, int b, int lvl, double *U)
+{
+    if (!c || !U || b < 0 || b > 1 || lvl < 0 || lvl >= c->D) return fail(QJX_ERR_ARG, "bad propagator index");
+    const size_t LL = (size_t)c->L * c->L;
+    std::vector<char> raw(LL * c->esize);
+    HIP_TRY(hipMemcpy(raw.data(), prop_ptr(c, b, lvl), raw.size(), hipMemcpyDeviceToHost));
+    const int d = c->d, dp = c->dp, L = c->L;
+    for (int i = 0; i < d; i++)
+        for (int k = 0; k < d; k++) {
+            // Re U_ik at (row i, col k), Im U_ik at (row dp + i, col k); k-major
+            const size_t ore = (size_t)k * L + i, oim = (size_t)k * L + dp + i;
+            double re, im;
+            if (c->esize == 8) {
+                re = ((const double *)raw.data())[ore];
+                im = ((const double *)raw.data())[oim];
+            } else {
+                re = ((const float *)raw.data())[ore];
+                im = ((const float *)raw.data())[oim];
+            }
+            U[2 * ((size_t)i * d + k)] = re;
+            U[2 * ((size_t)i * d + k) + 1] = im;
+        }
+    return QJX_OK;
+}
+
+extern "C" int qjx_set_propagator(qjx_ctx *c, int b, int lvl, const double *U)
+{
+    if (!c || !U || b < 0 || b > 1 || lvl < 0 || lvl >= c->D) return fail(QJX_ERR_ARG, "bad propagator index");
+    const size_t nn2 = (size_t)2 * c->d * c->d;
+    double *dU = nullptr;
+    HIP_TRY(hipMalloc((void **)&dU, nn2 * sizeof(double)));
+    hipError_t e = hipMemcpy(dU, U, nn2 * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        qjx::launch_pack_propagator(c->prec, c->d, c->dp, dU, prop_ptr(c, b, lvl), nullptr, c->stream);
+        e = hipStreamSynchronize(c->stream);
+    }
+    (void)hipFree(dU);
+    if (e != hipSuccess) return fail(QJX_ERR_HIP, hipGetErrorString(e));
+    return QJX_OK;
+}
+
+extern "C" int qjx_get_stats(qjx_ctx *c, qjx_stats *out)
+{
+    if (!c || !out) return fail(QJX_ERR_ARG, "null argument");
+    unsigned long long h[2] = {0, 0};
+    if (c->counters) HIP_TRY(hipMemcpy(h, c->counters, sizeof h, hipMemcpyDeviceToHost));
+    out->matvecs = (int64_t)h[0];
+    out->iterations = c->iterations;
+    out->jumps = (int64_t)h[1];
+    out->gemm_flops = 8.0 * double(c->d) * double(c->d) * double(out->matvecs);
+    return QJX_OK;
+}

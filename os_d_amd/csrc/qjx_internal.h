@@ -1,0 +1,73 @@
+// qjx_internal.h -- device data layout shared by the HIP kernels and the host engine.
+//
+// HBM layout of one ensemble (d = N + 1 basis states, M trajectories):
+//   * states: two slots of M columns, each column 2*dp reals
+//     [Re psi_0 .. Re psi_{dp-1}, Im psi_0 .. Im psi_{dp-1}] (dp = d rounded up to the
+//     tile multiple; the padding stays zero).  TrajState.cur says which slot holds the
+//     current state; a propagation step reads that slot and writes the other one, so
+//     accepting a step is a bit flip and rejecting it (bisection descent) costs nothing.
+//   * propagators: 2 branches x D levels of U = exp(-i H_eff,b dt_i) in real block form
+//     [[Re U, -Im U], [Im U, Re U]] (2dp x 2dp), stored k-major (column-major) so both
+//     GEMM operand tiles are read along contiguous memory.
+//   * per-trajectory scalar state (TrajState), group lists for the grouped GEMM and
+//     per-row-tile partial norms.
+#pragma once
+#include <stdint.h>
+
+namespace qjx {
+
+enum Precision { kFP64 = 0, kFP32 = 1, kBF16x3 = 2 };
+
+// Scalar state of one trajectory: where it is in the bisection tree and its
+// FP64 bookkeeping (time, eta, RNG).  Mirrors one_period_branch's recursion
+// (reference qj_proc.cpp:271-343): `lvl` is the recursion depth, `cnt0` the
+// counter of the top level, bit i of `bits` the counter of level i >= 1.
+struct TrajState {
+    double t;        // time (times_all)
+    double eta;      // jump threshold (etas_all)
+    uint64_t rng;    // MCG59 leap-frog state x (next output is x * 2^-59)
+    int cnt0;        // steps done at level 0 of the current half period
+    uint32_t bits;   // counters of levels 1..D-1
+    int lvl;         // current level
+    int b;           // branch (half period) 0 / 1
+    int pos;         // top-level steps completed (period = pos / S)
+    int target;      // run until pos == target
+    int cur;         // slot holding the current state
+    int njumps;      // jumps logged (jumps_counts)
+    int logn;        // entries in the device jump log since the last drain
+    int pad;
+};
+
+struct JumpLog {
+    double *time;    // [M * cap]
+    double *norm;
+    double *eta;
+    int *type;
+    int cap;         // records per trajectory between drains (0 = logging off)
+};
+
+// Everything a step kernel needs, passed by value.
+struct StepArgs {
+    const void *props;        // 2*D matrices, each L*L elements (L = 2*dp)
+    const void *props_lo;     // BF16x3 only: low halves
+    void *slot[2];            // state slots, M*L elements each
+    TrajState *st;
+    const int *list;          // [G * M] trajectory ids per group (this iteration)
+    const int *cnt;           // [G]
+    int *list_next;           // [G * M]
+    int *cnt_next;            // [G]
+    int *cnt_clear;           // [G] counters zeroed by the GEMM kernel (next-next use)
+    double *normpart;         // [R * M]
+    const double *dt;         // [D]
+    const double *Ldiag;      // [d]   L_kk
+    const double *Loff;       // [d-1] L_{k+1,k} = -Loff[k], L_{k,k+1} = +Loff[k]
+    JumpLog log;
+    int *err;                 // [4]: code, trajectory, bits of the offending norm
+    unsigned long long *counters;  // [0] matvecs, [1] jumps
+    int d, dp, L, D, S, M, G, R;
+    int is_obs;
+    int jump;                 // logging switch (params.txt "jump")
+    double norm_tol;          // allowed excess of ||psi||^2 over 1 (0 for FP64)
+};
+
+}  // namespace qjx

@@ -1,0 +1,53 @@
+// qjx_kernels.h -- host-callable launchers of the HIP kernels (qjx_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "qjx_internal.h"
+
+namespace qjx {
+
+constexpr int kMaxGroups = 64;   // 2 branches x up to 32 levels
+
+// Per-trajectory initial state: psi = e_{start_state} (start_type 0) or the
+// uniform vector (otherwise); RNG state x_{seed+tr} of the MCG59 leap-frog stream.
+void launch_init_ensemble(int prec, void *slot0, void *slot1, TrajState *st, int M, int d, int dp,
+                          int L, int seed, int start_type, int start_state, hipStream_t s);
+// eta = uniform(); redraw while 0 (experiment.cpp:3259-3265)
+void launch_draw_eta(TrajState *st, int M, uint64_t stride, hipStream_t s);
+// target = pos + n_steps for every trajectory; the group lists then hold every
+// trajectory at level 0 of its current branch.
+void launch_seed_lists(TrajState *st, int M, int n_steps, int *list, int *cnt, int G,
+                       hipStream_t s);
+void launch_reset_times(TrajState *st, int M, hipStream_t s);
+void launch_reset_logn(TrajState *st, int M, hipStream_t s);
+
+// One propagation iteration (grouped GEMM + decide) over the lists in a.
+void launch_step(int prec, int gemm_kind, const StepArgs &a, uint64_t stride, hipStream_t s);
+
+// Observables of every trajectory (core.cpp:665-725): norm, mean, m2, energy;
+// dispersion against mean_start; optional |psi_k|^2 / norm per trajectory.
+void launch_chars(int prec, const void *slot0, const void *slot1, const TrajState *st, int M,
+                  int d, int dp, int L, const double *Hdiag, const double *Hoff, int is_start,
+                  double *mean_start, double *norm, double *mean, double *disp, double *m2,
+                  double *energy, double *adr, hipStream_t s);
+// adr_avg[k] = sum_tr adr[tr][k] / M  (dump.cpp:26-61), fixed summation order.
+void launch_adr_avg(const double *adr, int M, int d, double *out, hipStream_t s);
+// Copy the current state of every trajectory into a dense interleaved complex
+// FP64 array (M x d).
+void launch_export_state(int prec, const void *slot0, const void *slot1, const TrajState *st,
+                         int M, int d, int dp, int L, double *out, hipStream_t s);
+void launch_import_state(int prec, void *slot0, void *slot1, const TrajState *st, int M, int d,
+                         int dp, int L, const double *in, hipStream_t s);
+
+// Dense complex FP64 helpers for the propagator build (row-major n x n, interleaved).
+void launch_zgemm(int n, const double *A, const double *B, double *C, hipStream_t s);
+// C = alpha * A + beta * I  (alpha, beta real)
+void launch_zaxpby_eye(int n, const double *A, double alpha, double beta, double *C, hipStream_t s);
+// C += alpha * A
+void launch_zaxpy(int n, const double *A, double alpha, double *C, hipStream_t s);
+// Write U (n x n complex FP64) as the block-real k-major 2dp x 2dp operand.
+void launch_pack_propagator(int prec, int n, int dp, const double *U, void *dst, void *dst_lo,
+                            hipStream_t s);
+
+}  // namespace qjx

@@ -1,0 +1,209 @@
+"""ctypes mirror of include/qjx.h.
+
+The reference drives one ensemble through strategy objects (NewDelBehavior,
+ExpNewDelBehavior, ExperimentBehavior, PropagateBehavior, CoreBehavior; see
+source/cpp/QJX/QJX/processor.cpp:50-75).  `Engine` exposes the same steps at
+ensemble granularity: build the model and propagator tree, initialise the
+ensemble, transient, observation periods, observables, jump logs.
+
+The library must be built (os_d_amd/build.py); there is no CPU fallback: a
+missing or unloadable libqjx.so raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libqjx.so")
+
+FP64, FP32, BF16X3 = 0, 1, 2
+
+QJX_OK = 0
+ERRORS = {-1: "QJX_ERR_ARG", -2: "QJX_ERR_HIP", -3: "QJX_ERR_NORM", -4: "QJX_ERR_LOG",
+          -5: "QJX_ERR_IO", -6: "QJX_ERR_NOMEM"}
+
+
+class QJXError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (ERRORS.get(code, "?"), code, msg))
+        self.code = code
+
+
+class ModelDesc(ctypes.Structure):
+    _fields_ = [("N", ctypes.c_int), ("prm_E", ctypes.c_double), ("prm_U", ctypes.c_double),
+                ("prm_J", ctypes.c_double), ("drv_ampl", ctypes.c_double),
+                ("drv_freq", ctypes.c_double), ("diss_gamma", ctypes.c_double),
+                ("ex_deep", ctypes.c_int), ("steps_per_period", ctypes.c_int),
+                ("precision", ctypes.c_int)]
+
+
+class EnsembleDesc(ctypes.Structure):
+    _fields_ = [("num_trajectories", ctypes.c_int), ("seed", ctypes.c_int), ("mns", ctypes.c_int),
+                ("start_type", ctypes.c_int), ("start_state", ctypes.c_int), ("jump", ctypes.c_int),
+                ("log_capacity", ctypes.c_int)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("matvecs", ctypes.c_int64), ("iterations", ctypes.c_int64),
+                ("jumps", ctypes.c_int64), ("gemm_flops", ctypes.c_double)]
+
+
+_lib = None
+
+# (name, restype, argtypes) of every symbol declared in include/qjx.h
+_P = ctypes.POINTER
+_D = _P(ctypes.c_double)
+SIGNATURES = [
+    ("qjx_last_error", ctypes.c_char_p, []),
+    ("qjx_ctx_create", ctypes.c_int, [_P(ModelDesc), _P(ctypes.c_void_p)]),
+    ("qjx_ctx_destroy", None, [ctypes.c_void_p]),
+    ("qjx_ensemble_init", ctypes.c_int, [ctypes.c_void_p, _P(EnsembleDesc)]),
+    ("qjx_trans", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("qjx_run_periods", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    ("qjx_run_steps", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    ("qjx_chars", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, _D, _D, _D, _D]),
+    ("qjx_get_state", ctypes.c_int, [ctypes.c_void_p, _D, _D, _D, _P(ctypes.c_int),
+                                     _P(ctypes.c_uint64)]),
+    ("qjx_set_state", ctypes.c_int, [ctypes.c_void_p, _D, _D, _D, _P(ctypes.c_uint64)]),
+    ("qjx_get_jumps", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, _P(ctypes.c_int), _D, _D,
+                                     ctypes.c_int]),
+    ("qjx_get_propagator", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D]),
+    ("qjx_set_propagator", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D]),
+    ("qjx_get_stats", ctypes.c_int, [ctypes.c_void_p, _P(Stats)]),
+    ("qjx_dim", ctypes.c_int, [ctypes.c_void_p]),
+    ("qjx_run_config", ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]),
+]
+
+
+def load_library(path=LIB_PATH):
+    """Load libqjx.so and bind every C-ABI symbol (raises if absent)."""
+    global _lib
+    if _lib is not None and path == LIB_PATH:
+        return _lib
+    if not os.path.exists(path):
+        raise QJXError(-2, "libqjx.so not built at %s (run os_d_amd/build.py)" % path)
+    lib = ctypes.CDLL(path)
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path == LIB_PATH:
+        _lib = lib
+    return lib
+
+
+def _check(lib, rc):
+    if rc < 0:
+        raise QJXError(rc, lib.qjx_last_error().decode())
+    return rc
+
+
+def _ptr(a, ctype=ctypes.c_double):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+class Engine:
+    """One dimer model + one trajectory ensemble on the current HIP device."""
+
+    def __init__(self, N, prm_E=1.0, prm_U=0.5, prm_J=1.0, drv_ampl=1.5, drv_freq=1.0,
+                 diss_gamma=0.1, ex_deep=16, steps_per_period=4, precision=FP64):
+        self.lib = load_library()
+        self.model = ModelDesc(N, prm_E, prm_U, prm_J, drv_ampl, drv_freq, diss_gamma, ex_deep,
+                               steps_per_period, precision)
+        h = ctypes.c_void_p()
+        _check(self.lib, self.lib.qjx_ctx_create(ctypes.byref(self.model), ctypes.byref(h)))
+        self.h = h
+        self.d = N + 1
+        self.M = 0
+
+    def close(self):
+        if self.h:
+            self.lib.qjx_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def ensemble_init(self, num_trajectories, seed=0, mns=1000000, start_type=0, start_state=0,
+                      jump=0, log_capacity=0):
+        self.ens = EnsembleDesc(num_trajectories, seed, mns, start_type, start_state, jump,
+                                log_capacity)
+        _check(self.lib, self.lib.qjx_ensemble_init(self.h, ctypes.byref(self.ens)))
+        self.M = num_trajectories
+
+    def trans(self, num_tp_periods):
+        _check(self.lib, self.lib.qjx_trans(self.h, num_tp_periods))
+
+    def run_periods(self, n, is_obs=1):
+        _check(self.lib, self.lib.qjx_run_periods(self.h, n, is_obs))
+
+    def run_steps(self, n, is_obs=1):
+        _check(self.lib, self.lib.qjx_run_steps(self.h, n, is_obs))
+
+    def chars(self, is_start=0, adr_avg=False, adr=False):
+        M, d = self.M, self.d
+        out = {k: np.zeros(M) for k in ("norm", "mean", "dispersion", "m2", "energy")}
+        aa = np.zeros(d) if adr_avg else None
+        ad = np.zeros((M, d)) if adr else None
+        _check(self.lib, self.lib.qjx_chars(self.h, is_start, _ptr(out["norm"]), _ptr(out["mean"]),
+                                            _ptr(out["dispersion"]), _ptr(out["m2"]),
+                                            _ptr(out["energy"]), _ptr(aa), _ptr(ad)))
+        if adr_avg:
+            out["adr_avg"] = aa
+        if adr:
+            out["adr"] = ad
+        return out
+
+    def get_state(self):
+        M, d = self.M, self.d
+        psi = np.zeros((M, d), dtype=np.complex128)
+        eta, t = np.zeros(M), np.zeros(M)
+        nj = np.zeros(M, dtype=np.int32)
+        rng = np.zeros(M, dtype=np.uint64)
+        _check(self.lib, self.lib.qjx_get_state(self.h, _ptr(psi.view(np.float64)), _ptr(eta), _ptr(t),
+                                                _ptr(nj, ctypes.c_int), _ptr(rng, ctypes.c_uint64)))
+        return {"psi": psi, "eta": eta, "t": t, "njumps": nj, "rng": rng}
+
+    def set_state(self, psi=None, eta=None, t=None, rng=None):
+        psi = None if psi is None else np.ascontiguousarray(psi, dtype=np.complex128)
+        _check(self.lib, self.lib.qjx_set_state(
+            self.h, None if psi is None else _ptr(psi.view(np.float64)),
+            _ptr(None if eta is None else np.ascontiguousarray(eta, dtype=np.float64)),
+            _ptr(None if t is None else np.ascontiguousarray(t, dtype=np.float64)),
+            _ptr(None if rng is None else np.ascontiguousarray(rng, dtype=np.uint64), ctypes.c_uint64)))
+
+    def get_jumps(self, tr):
+        n = _check(self.lib, self.lib.qjx_get_jumps(self.h, tr, None, None, None, None, 0))
+        times, norms, etas = np.zeros(n), np.zeros(n), np.zeros(n)
+        types = np.zeros(n, dtype=np.int32)
+        _check(self.lib, self.lib.qjx_get_jumps(self.h, tr, _ptr(times), _ptr(types, ctypes.c_int),
+                                                _ptr(norms), _ptr(etas), n))
+        return {"times": times, "types": types, "norms": norms, "etas": etas}
+
+    def get_propagator(self, b, lvl):
+        U = np.zeros((self.d, self.d), dtype=np.complex128)
+        _check(self.lib, self.lib.qjx_get_propagator(self.h, b, lvl, _ptr(U.view(np.float64))))
+        return U
+
+    def set_propagator(self, b, lvl, U):
+        U = np.ascontiguousarray(U, dtype=np.complex128)
+        _check(self.lib, self.lib.qjx_set_propagator(self.h, b, lvl, _ptr(U.view(np.float64))))
+
+    def stats(self):
+        s = Stats()
+        _check(self.lib, self.lib.qjx_get_stats(self.h, ctypes.byref(s)))
+        return {"matvecs": s.matvecs, "iterations": s.iterations, "jumps": s.jumps,
+                "gemm_flops": s.gemm_flops}
+
+
+def run_config(config_path, params_path, precision=FP64, steps_per_period=0):
+    """Drop-in runner: the reference's config.txt / params.txt in, its output files out."""
+    lib = load_library()
+    return _check(lib, lib.qjx_run_config(config_path.encode(), params_path.encode(), precision,
+                                          steps_per_period))
