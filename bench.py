@@ -1,0 +1,160 @@
+"""Benchmark: quantum-jump trajectory-periods/s of the dimer engine on one MI355X.
+
+Workload (the precision-form configuration): Bose-Hubbard dimer N = 1000
+(Hilbert dimension d = 1001), J = 1, U = 0.5, E0 = 1, A = 1.5, omega = 1,
+gamma = 0.1, 1e5 trajectories, S = 200 top-level steps per period, ex_deep = 16,
+start state e_0, seed 0.  A bench "step" is one top-level step T/S of the whole
+ensemble, including every bisection refinement and jump it triggers; K steps are
+K/S trajectory-periods per trajectory.  States and propagators are resident in
+HBM before the timed region.
+
+value      : trajectory-periods/s of the FP32 form (FP32 GEMM, FP64 norms, eta,
+             bisection, jumps).
+fp64_value : the same for the FP64 form (same workload, fewer steps).
+roofline   : the step GEMM (dominant kernel), algorithmic flops 8 d^2 per
+             propagator application, timed per launch with HIP events on the
+             engine stream.
+cpu_baseline: the FP64 CPU oracle (OpenMP over trajectories, one zgemv per
+             step as the reference) on a bounded sample of the same workload
+             with the same propagator tree.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "trajectory-periods/s at dim 1001 (FP32-MFMA vs FP64) + p99 ensemble-job latency"
+PEAK = {"fp32": 157.3, "fp64": 78.6}   # MI355X dense matrix TFLOP/s
+
+
+def run_form(qjx, prec, N, M, S, D, warmup, steps):
+    e = qjx.Engine(N, prm_E=1.0, prm_U=0.5, prm_J=1.0, drv_ampl=1.5, drv_freq=1.0,
+                   diss_gamma=0.1, ex_deep=D, steps_per_period=S, precision=prec)
+    e.ensemble_init(M, seed=0, mns=1000000, start_type=0, start_state=0, jump=0)
+    e.trans(0)
+    if warmup:
+        e.run_steps(warmup, 0)
+    s0 = e.stats()
+    e.set_profiling(True)
+    t0 = time.perf_counter()
+    e.run_steps(steps, 0)          # returns after the engine stream is synchronized
+    dt = time.perf_counter() - t0
+    s1 = e.stats()
+    e.set_profiling(False)
+    mv = s1["matvecs"] - s0["matvecs"]
+    res = {
+        "seconds": dt,
+        "tp_per_s": M * steps / S / dt,
+        "matvecs": mv,
+        "matvecs_per_traj_step": mv / (M * steps),
+        "jumps_per_traj_period": (s1["jumps"] - s0["jumps"]) / (M * steps / S),
+        "gemm_ms": s1["gemm_ms"],
+        "decide_ms": s1["decide_ms"],
+        "gemm_launches": s1["gemm_launches"],
+        "gemm_flops": 8.0 * (N + 1) ** 2 * mv,
+    }
+    return e, res
+
+
+def cpu_baseline(engine, N, S, D, budget_s):
+    """FP64 oracle on the host cores, bounded sample, same propagator tree."""
+    from oracle import oracle as orc
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+    threads = min(threads, os.cpu_count() or threads)
+    m = orc.model(N, prm_E=1.0, prm_U=0.5, prm_J=1.0, drv_ampl=1.5, drv_freq=1.0, gamma=0.1, S=S, D=D)
+    ntraj = 2 * threads
+    o = orc.Ensemble(m, ntraj, seed=0, jump=0, threads=threads, build_propagators=False)
+    for b in range(2):
+        for lvl in range(D):
+            o.set_propagator(b, lvl, engine.get_propagator(b, lvl))
+    o.trans(0)
+    t0 = time.perf_counter()
+    nsteps = 0
+    while True:
+        o.steps(1, 0)
+        nsteps += 1
+        if time.perf_counter() - t0 >= budget_s or nsteps >= S:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": ntraj * nsteps / S / dt, "unit": "trajectory-periods/s", "cores": threads,
+            "kind": "port",
+            "sample": "%d trajectories x %d top-level steps (of %d/period) from the start state, "
+                      "d=%d, FP64 oracle, %.1f s" % (ntraj, nsteps, S, N + 1, dt),
+            "matvecs_per_traj_step": o.num_matvecs() / (ntraj * nsteps)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--N", type=int, default=1000)
+    ap.add_argument("--traj", type=int, default=100000)
+    ap.add_argument("--S", type=int, default=200)
+    ap.add_argument("--D", type=int, default=16)
+    ap.add_argument("--fp64-steps", type=int, default=0, help="0 = max(2, steps // 3)")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the trajectory engine on a single "
+              "device (multi-GPU sharding is described in DESIGN.md, not implemented).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from os_d_amd import qjx
+    N, M, S, D = args.N, args.traj, args.S, args.D
+    e32, r32 = run_form(qjx, qjx.FP32, N, M, S, D, args.warmup, args.steps)
+    fp64 = None
+    e64 = None
+    if not args.no_fp64:
+        k64 = args.fp64_steps or max(2, args.steps // 3)
+        e32.close()
+        e64, r64 = run_form(qjx, qjx.FP64, N, M, S, D, args.warmup, k64)
+        fp64 = r64
+    cpu = None
+    if not args.no_cpu:
+        src = e64 if e64 is not None else e32
+        cpu = cpu_baseline(src, N, S, D, args.cpu_seconds)
+
+    launches = max(1, r32["gemm_launches"])
+    avg_ms = r32["gemm_ms"] / launches
+    achieved = r32["gemm_flops"] / (r32["gemm_ms"] * 1e-3) / 1e12 if r32["gemm_ms"] > 0 else 0.0
+    line = {
+        "metric": METRIC,
+        "value": r32["tp_per_s"],
+        "unit": "trajectory-periods/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * r32["seconds"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (start state e_0, seeded MCG59 streams; no dataset)",
+        "config": {"workload": "BH dimer N=%d (d=%d), %d trajectories, S=%d steps/period, ex_deep=%d, "
+                               "FP32 propagation + FP64 norms/bisection/jumps" % (N, N + 1, M, S, D),
+                   "trajectories": M, "dim": N + 1, "steps_per_period": S, "ex_deep": D,
+                   "step": "one top-level step T/S of the whole ensemble incl. refinement"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK["fp32"], "unit": "TFLOP/s",
+                     "frac": achieved / PEAK["fp32"], "traffic": None,
+                     "kernel": "step GEMM (grouped, gathered columns)",
+                     "avg_launch_ms": avg_ms,
+                     "flops_per_launch": r32["gemm_flops"] / launches},
+        "fp32": r32,
+        "fp64_value": fp64["tp_per_s"] if fp64 else None,
+        "fp64": fp64,
+        "fp32_over_fp64": (r32["tp_per_s"] / fp64["tp_per_s"]) if fp64 else None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -81,6 +81,9 @@ typedef struct qjx_stats {
     int64_t iterations;     /* grouped step iterations launched                     */
     int64_t jumps;          /* jumps applied                                        */
     double gemm_flops;      /* 8 d^2 per matvec (complex MAC = 8 real flop)         */
+    double gemm_ms;         /* device time of the step GEMM launches (profiling on)  */
+    double decide_ms;       /* device time of the decide launches (profiling on)     */
+    int64_t gemm_launches;  /* step GEMM launches timed                             */
 } qjx_stats;
 
 const char *qjx_last_error(void);
@@ -129,6 +132,14 @@ int qjx_get_propagator(qjx_ctx *ctx, int b, int lvl, double *U);
 int qjx_set_propagator(qjx_ctx *ctx, int b, int lvl, const double *U);
 
 int qjx_get_stats(qjx_ctx *ctx, qjx_stats *out);
+/* Time every step kernel with HIP events on the engine stream (resets the timers). */
+int qjx_set_profiling(qjx_ctx *ctx, int on);
+
+/* Self-test of the matrix-core operand layouts the GEMM kernels assume (no
+ * reference counterpart): D32 = A32 B32 (32x2 by 2x32, FP32) and D64 = A64 B64
+ * (16x4 by 4x16, FP64), all row-major, computed by one wavefront. */
+int qjx_mfma_probe(const float *A32, const float *B32, float *D32, const double *A64,
+                   const double *B64, double *D64);
 /* Hilbert dimension of the context (N + 1). */
 int qjx_dim(const qjx_ctx *ctx);
 

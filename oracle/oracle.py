@@ -56,6 +56,7 @@ def lib():
         L.qjo_get_propagator.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D]
         L.qjo_trans.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.qjo_periods.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.qjo_steps.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         L.qjo_periods_one.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.qjo_chars.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, _D, _D, _D]
         L.qjo_get_state.argtypes = [ctypes.c_void_p, _D, _D, _D, _P(ctypes.c_int), _P(ctypes.c_uint64)]
@@ -143,6 +144,9 @@ class Ensemble:
 
     def periods(self, n, is_obs=1):
         self._ok(self.L.qjo_periods(self.h, n, is_obs))
+
+    def steps(self, n, is_obs=1):
+        self._ok(self.L.qjo_steps(self.h, n, is_obs))
 
     def periods_one(self, tr, n, is_obs=1):
         self._ok(self.L.qjo_periods_one(self.h, tr, n, is_obs))

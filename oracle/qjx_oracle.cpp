@@ -295,6 +295,7 @@ struct qjo_ens {
     std::vector<cplx> psi, psi_aux;         /* phi_all, phi_all_aux (M x d)         */
     std::vector<double> t, eta, mean_start;
     std::vector<int> njumps;
+    std::vector<int> pos;                   /* top-level steps completed            */
     std::vector<Rng> rng;
     std::vector<std::vector<double>> jt, jn, je;
     std::vector<std::vector<int>> jtype;
@@ -383,6 +384,7 @@ extern "C" qjo_ens *qjo_create(const qjo_model *p, int num_traj, int seed, int m
     e->eta.assign(M, 0.0);
     e->mean_start.assign(M, 0.0);
     e->njumps.assign(M, 0);
+    e->pos.assign(M, 0);
     e->jt.assign(M, {});
     e->jn.assign(M, {});
     e->je.assign(M, {});
@@ -514,6 +516,26 @@ static void level_run(qjo_ens *e, int tr, int b, int lvl, int steps, int is_obs)
 static void one_period(qjo_ens *e, int tr, int is_obs)
 {
     for (int b = 0; b < 2; b++) level_run(e, tr, b, 0, e->p.S / 2, is_obs);
+    e->pos[tr] += e->p.S;
+}
+
+/* n top-level steps of one trajectory from its current position (the same
+ * sequence as whole periods, cut at top-level step boundaries). */
+static void top_steps(qjo_ens *e, int tr, int n, int is_obs)
+{
+    const int half = e->p.S / 2;
+    for (int k = 0; k < n && !e->failed; k++) {
+        const int b = (e->pos[tr] / half) % 2;
+        level_run(e, tr, b, 0, 1, is_obs);
+        e->pos[tr] += 1;
+    }
+}
+
+extern "C" int qjo_steps(qjo_ens *e, int n, int is_obs)
+{
+#pragma omp parallel for schedule(dynamic, 1) num_threads(e->threads)
+    for (int tr = 0; tr < e->M; tr++) top_steps(e, tr, n, is_obs);
+    return e->failed ? -1 : 0;
 }
 
 extern "C" int qjo_periods_one(qjo_ens *e, int tr, int n, int is_obs)

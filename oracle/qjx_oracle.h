@@ -70,6 +70,8 @@ int qjo_get_propagator(const qjo_ens *e, int b, int lvl, double *U);
 int qjo_trans(qjo_ens *e, int num_tp_periods);
 /* Run n periods for every trajectory, logging jumps when is_obs && jump > 0. */
 int qjo_periods(qjo_ens *e, int n, int is_obs);
+/* Run n top-level steps (of T/S) for every trajectory; periods are S steps. */
+int qjo_steps(qjo_ens *e, int n, int is_obs);
 /* Run n periods for one trajectory only (same semantics). */
 int qjo_periods_one(qjo_ens *e, int tr, int n, int is_obs);
 

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -62,6 +63,7 @@ struct qjx_ctx {
     qjx_model_desc model{};
     int d = 0, dp = 0, L = 0, D = 0, S = 0, G = 0, R = 0;
     int prec = 0;
+    int gemm_kind = 0;
     size_t esize = 8;
     double T = 0.0;
     std::vector<double> dt;
@@ -85,6 +87,8 @@ struct qjx_ctx {
     int *lists[2] = {nullptr, nullptr};
     int *cnts[2] = {nullptr, nullptr};
     double *normpart = nullptr;
+    int *jlist = nullptr, *jsrc = nullptr, *jcnt = nullptr;
+    double *jnorm = nullptr;
     int *err = nullptr;
     unsigned long long *counters = nullptr;
     qjx::JumpLog log{};
@@ -99,6 +103,12 @@ struct qjx_ctx {
     std::vector<std::vector<int>> jty;
 
     int64_t iterations = 0;
+
+    // optional per-launch timing of the step kernels (hipEvents on the engine stream)
+    bool profiling = false;
+    std::vector<hipEvent_t> ev;    // 3 per iteration of a chunk
+    double gemm_ms = 0.0, decide_ms = 0.0;
+    int64_t gemm_launches = 0;
 };
 
 // ---------------------------------------------------------------------------
@@ -245,7 +255,7 @@ extern "C" int qjx_ctx_create(const qjx_model_desc *m, qjx_ctx **out)
 {
     if (!m || !out) return fail(QJX_ERR_ARG, "null argument");
     *out = nullptr;
-    if (m->N < 1) return fail(QJX_ERR_ARG, "N must be >= 1");
+    if (m->N < 1 || m->N + 1 > 2048) return fail(QJX_ERR_ARG, "N must be in [1, 2047]");
     if (m->ex_deep < 1 || m->ex_deep > qjx::kMaxGroups / 2 || m->ex_deep > 30)
         return fail(QJX_ERR_ARG, "ex_deep must be in [1, 30]");
     const int S = m->steps_per_period ? m->steps_per_period : 4;
@@ -261,12 +271,15 @@ extern "C" int qjx_ctx_create(const qjx_model_desc *m, qjx_ctx **out)
     c->model = *m;
     c->model.steps_per_period = S;
     c->d = m->N + 1;
-    c->dp = (c->d + 31) / 32 * 32;
+    c->dp = (c->d + 63) / 64 * 64;
     c->L = 2 * c->dp;
     c->D = m->ex_deep;
     c->S = S;
     c->G = 2 * c->D;
-    c->R = c->L / 64;
+    c->gemm_kind = qjx::kGemmMfma;
+    if (const char *env = std::getenv("QJX_GEMM"))
+        if (std::string(env) == "fma") c->gemm_kind = qjx::kGemmFma;
+    c->R = c->L / qjx::gemm_bm(c->gemm_kind);
     c->prec = m->precision;
     c->esize = c->prec == QJX_FP64 ? 8 : 4;
     c->T = 2.0 * kPi / m->drv_freq;
@@ -338,6 +351,10 @@ static void free_ensemble(qjx_ctx *c)
     }
     dev_free(c->st);
     dev_free(c->normpart);
+    dev_free(c->jlist);
+    dev_free(c->jsrc);
+    dev_free(c->jcnt);
+    dev_free(c->jnorm);
     dev_free(c->err);
     dev_free(c->counters);
     dev_free(c->log.time);
@@ -365,6 +382,7 @@ extern "C" void qjx_ctx_destroy(qjx_ctx *c)
     dev_free(c->d_Loff);
     dev_free(c->d_HdiagE);
     dev_free(c->d_Hoff);
+    for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -408,6 +426,11 @@ extern "C" int qjx_ensemble_init(qjx_ctx *c, const qjx_ensemble_desc *e)
         HIP_TRY(hipMemset(c->cnts[i], 0, qjx::kMaxGroups * sizeof(int)));
     }
     HIP_TRY(hipMalloc((void **)&c->normpart, (size_t)c->R * M * sizeof(double)));
+    HIP_TRY(hipMalloc((void **)&c->jlist, M * sizeof(int)));
+    HIP_TRY(hipMalloc((void **)&c->jsrc, M * sizeof(int)));
+    HIP_TRY(hipMalloc((void **)&c->jnorm, M * sizeof(double)));
+    HIP_TRY(hipMalloc((void **)&c->jcnt, sizeof(int)));
+    HIP_TRY(hipMemset(c->jcnt, 0, sizeof(int)));
     HIP_TRY(hipMalloc((void **)&c->err, 4 * sizeof(int)));
     HIP_TRY(hipMemset(c->err, 0, 4 * sizeof(int)));
     HIP_TRY(hipMalloc((void **)&c->counters, 2 * sizeof(unsigned long long)));
@@ -446,6 +469,10 @@ static qjx::StepArgs step_args(qjx_ctx *c, int p, int is_obs)
     a.cnt_next = c->cnts[p ^ 1];
     a.cnt_clear = c->cnts[p ^ 1];
     a.normpart = c->normpart;
+    a.jlist = c->jlist;
+    a.jsrc = c->jsrc;
+    a.jnorm = c->jnorm;
+    a.jcnt = c->jcnt;
     a.dt = c->d_dt;
     a.Ldiag = c->d_Ldiag;
     a.Loff = c->d_Loff;
@@ -524,10 +551,19 @@ static int run_steps(qjx_ctx *c, int n, int is_obs)
     qjx::launch_seed_lists(c->st, c->M, n, c->lists[0], c->cnts[0], c->G, c->stream);
     HIP_TRY(hipGetLastError());
     int chunk = 8;
+    const int max_chunk = 256;
+    if (c->profiling && c->ev.size() < (size_t)3 * max_chunk) {
+        c->ev.resize(3 * max_chunk);
+        for (auto &e : c->ev) HIP_TRY(hipEventCreate(&e));
+    }
     for (;;) {
         for (int k = 0; k < chunk; k++) {
             qjx::StepArgs a = step_args(c, c->parity, is_obs);
-            qjx::launch_step(c->prec, 0, a, c->stride, c->stream);
+            if (c->profiling) HIP_TRY(hipEventRecord(c->ev[3 * k], c->stream));
+            qjx::launch_step_gemm(c->prec, c->gemm_kind, a, c->stream);
+            if (c->profiling) HIP_TRY(hipEventRecord(c->ev[3 * k + 1], c->stream));
+            qjx::launch_step_decide(c->prec, a, c->stride, c->stream);
+            if (c->profiling) HIP_TRY(hipEventRecord(c->ev[3 * k + 2], c->stream));
             c->parity ^= 1;
             c->iterations++;
         }
@@ -535,10 +571,20 @@ static int run_steps(qjx_ctx *c, int n, int is_obs)
         HIP_TRY(hipMemcpyAsync(c->h_cnt, c->cnts[c->parity], c->G * sizeof(int), hipMemcpyDeviceToHost,
                                c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
+        if (c->profiling) {
+            for (int k = 0; k < chunk; k++) {
+                float g = 0.f, dcd = 0.f;
+                HIP_TRY(hipEventElapsedTime(&g, c->ev[3 * k], c->ev[3 * k + 1]));
+                HIP_TRY(hipEventElapsedTime(&dcd, c->ev[3 * k + 1], c->ev[3 * k + 2]));
+                c->gemm_ms += g;
+                c->decide_ms += dcd;
+            }
+            c->gemm_launches += chunk;
+        }
         long long active = 0;
         for (int g = 0; g < c->G; g++) active += c->h_cnt[g];
         if (active == 0) break;
-        chunk = std::min(chunk * 2, 256);
+        chunk = std::min(chunk * 2, max_chunk);
     }
     return check_err(c);
 }
@@ -665,8 +711,8 @@ extern "C" int qjx_get_propagator(qjx_ctx *c, int b, int lvl, double *U)
     const int d = c->d, dp = c->dp, L = c->L;
     for (int i = 0; i < d; i++)
         for (int k = 0; k < d; k++) {
-            // Re U_ik at (row i, col k), Im U_ik at (row dp + i, col k); k-major
-            const size_t ore = (size_t)k * L + i, oim = (size_t)k * L + dp + i;
+            // Re U_ik at (row i, col k), Im U_ik at (row dp + i, col k); row-major
+            const size_t ore = (size_t)i * L + k, oim = (size_t)(dp + i) * L + k;
             double re, im;
             if (c->esize == 8) {
                 re = ((const double *)raw.data())[ore];
@@ -706,5 +752,46 @@ extern "C" int qjx_get_stats(qjx_ctx *c, qjx_stats *out)
     out->iterations = c->iterations;
     out->jumps = (int64_t)h[1];
     out->gemm_flops = 8.0 * double(c->d) * double(c->d) * double(out->matvecs);
+    out->gemm_ms = c->gemm_ms;
+    out->decide_ms = c->decide_ms;
+    out->gemm_launches = c->gemm_launches;
+    return QJX_OK;
+}
+
+extern "C" int qjx_mfma_probe(const float *A32, const float *B32, float *D32, const double *A64,
+                              const double *B64, double *D64)
+{
+    float *a = nullptr, *b = nullptr, *dd = nullptr;
+    double *a6 = nullptr, *b6 = nullptr, *d6 = nullptr;
+    HIP_TRY(hipMalloc((void **)&a, 64 * 4));
+    HIP_TRY(hipMalloc((void **)&b, 64 * 4));
+    HIP_TRY(hipMalloc((void **)&dd, 1024 * 4));
+    HIP_TRY(hipMalloc((void **)&a6, 64 * 8));
+    HIP_TRY(hipMalloc((void **)&b6, 64 * 8));
+    HIP_TRY(hipMalloc((void **)&d6, 256 * 8));
+    HIP_TRY(hipMemcpy(a, A32, 64 * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b, B32, 64 * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(a6, A64, 64 * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b6, B64, 64 * 8, hipMemcpyHostToDevice));
+    qjx::launch_mfma_probe(a, b, dd, a6, b6, d6, nullptr);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(D32, dd, 1024 * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(D64, d6, 256 * 8, hipMemcpyDeviceToHost));
+    dev_free(a);
+    dev_free(b);
+    dev_free(dd);
+    dev_free(a6);
+    dev_free(b6);
+    dev_free(d6);
+    return QJX_OK;
+}
+
+extern "C" int qjx_set_profiling(qjx_ctx *c, int on)
+{
+    if (!c) return fail(QJX_ERR_ARG, "null context");
+    c->profiling = on != 0;
+    c->gemm_ms = c->decide_ms = 0.0;
+    c->gemm_launches = 0;
     return QJX_OK;
 }

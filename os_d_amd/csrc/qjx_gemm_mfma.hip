@@ -1,0 +1,368 @@
+// qjx_gemm_mfma.hip -- grouped propagation GEMM on CDNA4 matrix cores.
+//
+//   Y[:, j] = U_g X[:, j]   for every trajectory j listed in group g = b*D + lvl
+//
+// U_g is the block-real propagator [[Re U, -Im U], [Im U, Re U]] (L x L, L = 2 dp),
+// stored row-major for these kernels so that a row segment of the A tile and a
+// column segment of the X tile are both contiguous 16-byte loads straight into
+// LDS ([row][k] and [col][k] layouts).  Inside a K block of 32 the MFMA k-steps
+// visit k in a permuted order (lane half / quarter h takes k = s + (32/H) h),
+// which lets every lane read its A and B fragments for the whole K block with
+// 16-byte LDS reads; the permutation is the same for every tile and column, so
+// each output element's accumulation order is independent of where its column
+// lands (results do not depend on batch composition).
+//
+// FP32 form : v_mfma_f32_32x32x2f32, 4 waves as 2 x 2, wave tile 64 x 64, WG tile 128 x 128.
+// FP64 form : v_mfma_f64_16x16x4f64, 4 waves as 2 x 2, wave tile 64 x 64, WG tile 128 x 128.
+// Epilogue  : output to the trajectory's spare slot, FP64 sum of squares per column over
+//             the WG's 128 rows into normpart[row tile][trajectory].
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "qjx_internal.h"
+#include "qjx_kernels.h"
+
+namespace qjx {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+constexpr int MF_BM = 128, MF_BN = 128, MF_BK = 32;
+constexpr int MF_THREADS = 256;
+
+__device__ inline void mf_group_prefix(const int *cnt, int G, int per, int *pre)
+{
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int g = 0; g < G; g++) {
+            pre[g] = acc;
+            acc += (cnt[g] + per - 1) / per;
+        }
+        pre[G] = acc;
+    }
+    __syncthreads();
+}
+
+__device__ inline int mf_find_group(const int *pre, int G, int idx)
+{
+    int g = 0;
+    while (g + 1 < G && pre[g + 1] <= idx) g++;
+    return g;
+}
+
+// Resolve this workgroup's tile: group, column range, trajectory ids and slots.
+// Returns false if the block has no tile this iteration.
+__device__ inline bool mf_tile(const StepArgs &a, int *s_pre, int *s_tr, int *s_cur, int &g,
+                               int &ncol)
+{
+    const int tid = threadIdx.x;
+    if (blockIdx.x == 0 && blockIdx.y == 0) {
+        if (tid < a.G) a.cnt_clear[tid] = 0;
+        if (tid == 0) a.jcnt[0] = 0;
+    }
+    mf_group_prefix(a.cnt, a.G, MF_BN, s_pre);
+    const int ct = blockIdx.y;
+    if (ct >= s_pre[a.G]) return false;
+    g = mf_find_group(s_pre, a.G, ct);
+    const int col0 = (ct - s_pre[g]) * MF_BN;
+    ncol = min(MF_BN, a.cnt[g] - col0);
+    if (tid < MF_BN) {
+        if (tid < ncol) {
+            const int tr = a.list[(size_t)g * a.M + col0 + tid];
+            s_tr[tid] = tr;
+            s_cur[tid] = a.st[tr].cur;
+        } else {
+            s_tr[tid] = -1;
+            s_cur[tid] = 0;
+        }
+    }
+    __syncthreads();
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// FP32: v_mfma_f32_32x32x2f32
+//   A (32 x 2): lane l holds A[l % 32][l / 32]
+//   B (2 x 32): lane l holds B[l / 32][l % 32]
+//   D (32 x 32): lane l, register r holds D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32]
+// Within a K block of 32, MFMA step s (0..15) uses k = s + 16 h for lane half h.
+// ---------------------------------------------------------------------------
+constexpr int F32_PITCH = MF_BK + 4;   // LDS row pitch in floats (16-byte aligned, skewed banks)
+
+__global__ __launch_bounds__(MF_THREADS) void step_gemm_mfma_f32(StepArgs a)
+{
+    __shared__ int s_pre[kMaxGroups + 1];
+    __shared__ int s_tr[MF_BN];
+    __shared__ int s_cur[MF_BN];
+    __shared__ __attribute__((aligned(16))) float As[MF_BM * F32_PITCH];
+    __shared__ __attribute__((aligned(16))) float Bs[MF_BN * F32_PITCH];
+    __shared__ double s_norm[2][MF_BN];
+
+    int g, ncol;
+    if (!mf_tile(a, s_pre, s_tr, s_cur, g, ncol)) return;
+
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int wm = wid >> 1, wn = wid & 1;             // wave position in the 2 x 2 layout
+    const int L = a.L;
+    const int r0 = blockIdx.x * MF_BM;
+    const float *U = reinterpret_cast<const float *>(a.props) + (size_t)g * L * L;
+
+    // global -> LDS copy mapping: 128 rows (or columns) x 32 k = 1024 float4,
+    // 4 per thread: row = tid / 8 + 32 q, k4 = (tid % 8) * 4
+    const int cp_k = (tid & 7) * 4;
+    const int cp_r = tid >> 3;
+    const float *arow[4];
+    const float *bcol[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int r = cp_r + 32 * q;
+        arow[q] = U + (size_t)(r0 + r) * L + cp_k;
+        const int tr = s_tr[r];
+        bcol[q] = tr >= 0 ? reinterpret_cast<const float *>(a.slot[s_cur[r]]) + (size_t)tr * L + cp_k : nullptr;
+    }
+
+    floatx16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) acc[i][j][e] = 0.f;
+
+    floatx4 ra[4], rb[4];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            ra[q] = *reinterpret_cast<const floatx4 *>(arow[q] + k0);
+            rb[q] = bcol[q] ? *reinterpret_cast<const floatx4 *>(bcol[q] + k0) : floatx4{0.f, 0.f, 0.f, 0.f};
+        }
+    };
+    auto lstore = [&]() {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int r = cp_r + 32 * q;
+            *reinterpret_cast<floatx4 *>(&As[r * F32_PITCH + cp_k]) = ra[q];
+            *reinterpret_cast<floatx4 *>(&Bs[r * F32_PITCH + cp_k]) = rb[q];
+        }
+    };
+
+    const int h = lane >> 5, li = lane & 31;
+    gload(0);
+    lstore();
+    __syncthreads();
+    for (int k0 = 0; k0 < L; k0 += MF_BK) {
+        const bool more = k0 + MF_BK < L;
+        if (more) gload(k0 + MF_BK);
+        // fragments: 16 consecutive k (= s + 16 h) per lane, 4 x 16-byte reads each
+        floatx4 fa[2][4], fb[2][4];
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            const float *pa = &As[(wm * 64 + i * 32 + li) * F32_PITCH + 16 * h];
+            const float *pb = &Bs[(wn * 64 + i * 32 + li) * F32_PITCH + 16 * h];
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                fa[i][v] = *reinterpret_cast<const floatx4 *>(pa + 4 * v);
+                fb[i][v] = *reinterpret_cast<const floatx4 *>(pb + 4 * v);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < 16; s++) {
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[i][s >> 2][s & 3], fb[j][s >> 2][s & 3],
+                                                                      acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+        if (more) {
+            lstore();
+            __syncthreads();
+        }
+    }
+
+    // epilogue: rows 8 (r/4) + 4 h + (r%4) of each 32 x 32 block, column li
+    double ns[2] = {0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int c = wn * 64 + j * 32 + li;
+        const int tr = s_tr[c];
+        float *y = tr >= 0 ? reinterpret_cast<float *>(a.slot[s_cur[c] ^ 1]) + (size_t)tr * L : nullptr;
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int row = r0 + wm * 64 + i * 32 + 8 * q + 4 * h;
+                floatx4 v = {acc[i][j][4 * q], acc[i][j][4 * q + 1], acc[i][j][4 * q + 2], acc[i][j][4 * q + 3]};
+                if (y) *reinterpret_cast<floatx4 *>(y + row) = v;
+#pragma unroll
+                for (int e = 0; e < 4; e++) ns[j] += (double)v[e] * (double)v[e];
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 2; j++) ns[j] += __shfl_xor(ns[j], 32, 64);
+    if (h == 0) {
+#pragma unroll
+        for (int j = 0; j < 2; j++) s_norm[wm][wn * 64 + j * 32 + li] = ns[j];
+    }
+    __syncthreads();
+    if (tid < ncol) a.normpart[(size_t)blockIdx.x * a.M + s_tr[tid]] = s_norm[0][tid] + s_norm[1][tid];
+}
+
+// ---------------------------------------------------------------------------
+// FP64: v_mfma_f64_16x16x4f64
+//   A (16 x 4): lane l holds A[l % 16][l / 16]
+//   B (4 x 16): lane l holds B[l / 16][l % 16]
+//   D (16 x 16): lane l, register r holds D[4 (l / 16) + r][l % 16]
+// Within a K block of 32, MFMA step s (0..7) uses k = s + 8 q for lane quarter q.
+// ---------------------------------------------------------------------------
+constexpr int F64_PITCH = MF_BK + 2;   // doubles (16-byte aligned rows, skewed banks)
+
+__global__ __launch_bounds__(MF_THREADS) void step_gemm_mfma_f64(StepArgs a)
+{
+    __shared__ int s_pre[kMaxGroups + 1];
+    __shared__ int s_tr[MF_BN];
+    __shared__ int s_cur[MF_BN];
+    __shared__ __attribute__((aligned(16))) double As[MF_BM * F64_PITCH];
+    __shared__ __attribute__((aligned(16))) double Bs[MF_BN * F64_PITCH];
+    __shared__ double s_norm[2][MF_BN];
+
+    int g, ncol;
+    if (!mf_tile(a, s_pre, s_tr, s_cur, g, ncol)) return;
+
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int wm = wid >> 1, wn = wid & 1;
+    const int L = a.L;
+    const int r0 = blockIdx.x * MF_BM;
+    const double *U = reinterpret_cast<const double *>(a.props) + (size_t)g * L * L;
+
+    // 128 rows x 32 k doubles = 2048 double2, 8 per thread: row = tid / 16 + 16 q, k2 = (tid % 16) * 2
+    const int cp_k = (tid & 15) * 2;
+    const int cp_r = tid >> 4;
+    doublex4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = doublex4{0.0, 0.0, 0.0, 0.0};
+
+    typedef double double2v __attribute__((ext_vector_type(2)));
+    double2v ra[8], rb[8];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int r = cp_r + 16 * q;
+            ra[q] = *reinterpret_cast<const double2v *>(U + (size_t)(r0 + r) * L + k0 + cp_k);
+            const int tr = s_tr[r];
+            rb[q] = tr >= 0 ? *reinterpret_cast<const double2v *>(reinterpret_cast<const double *>(a.slot[s_cur[r]]) +
+                                                                 (size_t)tr * L + k0 + cp_k)
+                            : double2v{0.0, 0.0};
+        }
+    };
+    auto lstore = [&]() {
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int r = cp_r + 16 * q;
+            *reinterpret_cast<double2v *>(&As[r * F64_PITCH + cp_k]) = ra[q];
+            *reinterpret_cast<double2v *>(&Bs[r * F64_PITCH + cp_k]) = rb[q];
+        }
+    };
+
+    const int qq = lane >> 4, li = lane & 15;
+    gload(0);
+    lstore();
+    __syncthreads();
+    for (int k0 = 0; k0 < L; k0 += MF_BK) {
+        const bool more = k0 + MF_BK < L;
+        if (more) gload(k0 + MF_BK);
+#pragma unroll
+        for (int half = 0; half < 2; half++) {
+            // 4 consecutive k (= s + 8 qq, s = 4 half .. 4 half + 3) per lane and block
+            double2v fa[4][2], fb[4][2];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const double *pa = &As[(wm * 64 + i * 16 + li) * F64_PITCH + 8 * qq + 4 * half];
+                const double *pb = &Bs[(wn * 64 + i * 16 + li) * F64_PITCH + 8 * qq + 4 * half];
+                fa[i][0] = *reinterpret_cast<const double2v *>(pa);
+                fa[i][1] = *reinterpret_cast<const double2v *>(pa + 2);
+                fb[i][0] = *reinterpret_cast<const double2v *>(pb);
+                fb[i][1] = *reinterpret_cast<const double2v *>(pb + 2);
+            }
+#pragma unroll
+            for (int s = 0; s < 4; s++)
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i][s >> 1][s & 1], fb[j][s >> 1][s & 1],
+                                                                          acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+        if (more) {
+            lstore();
+            __syncthreads();
+        }
+    }
+
+    double ns[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int c = wn * 64 + j * 16 + li;
+        const int tr = s_tr[c];
+        double *y = tr >= 0 ? reinterpret_cast<double *>(a.slot[s_cur[c] ^ 1]) + (size_t)tr * L : nullptr;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int row = r0 + wm * 64 + i * 16 + 4 * qq;
+            if (y) {
+                *reinterpret_cast<double2v *>(y + row) = double2v{acc[i][j][0], acc[i][j][1]};
+                *reinterpret_cast<double2v *>(y + row + 2) = double2v{acc[i][j][2], acc[i][j][3]};
+            }
+#pragma unroll
+            for (int e = 0; e < 4; e++) ns[j] += acc[i][j][e] * acc[i][j][e];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        ns[j] += __shfl_xor(ns[j], 16, 64);
+        ns[j] += __shfl_xor(ns[j], 32, 64);
+    }
+    if (qq == 0) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) s_norm[wm][wn * 64 + j * 16 + li] = ns[j];
+    }
+    __syncthreads();
+    if (tid < ncol) a.normpart[(size_t)blockIdx.x * a.M + s_tr[tid]] = s_norm[0][tid] + s_norm[1][tid];
+}
+
+void launch_step_gemm_mfma(int prec, const StepArgs &a, hipStream_t s)
+{
+    dim3 grid(a.L / MF_BM, (a.M + MF_BN - 1) / MF_BN + a.G);
+    if (prec == kFP64) step_gemm_mfma_f64<<<grid, MF_THREADS, 0, s>>>(a);
+    else step_gemm_mfma_f32<<<grid, MF_THREADS, 0, s>>>(a);
+}
+
+// ---------------------------------------------------------------------------
+// Lane-layout probe for the two MFMA builtins: one wave computes D = A B for a
+// single block with the operand/result layouts documented above and writes D
+// row-major.  Used by a test against a host product.
+// ---------------------------------------------------------------------------
+__global__ void mfma_probe_kernel(const float *A32, const float *B32, float *D32, const double *A64,
+                                  const double *B64, double *D64)
+{
+    const int l = threadIdx.x;
+    floatx16 c = {};
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(A32[(l % 32) * 2 + l / 32], B32[(l / 32) * 32 + l % 32], c, 0, 0, 0);
+    for (int r = 0; r < 16; r++) D32[(8 * (r / 4) + 4 * (l / 32) + r % 4) * 32 + l % 32] = c[r];
+    doublex4 e = {0.0, 0.0, 0.0, 0.0};
+    e = __builtin_amdgcn_mfma_f64_16x16x4f64(A64[(l % 16) * 4 + l / 16], B64[(l / 16) * 16 + l % 16], e, 0, 0, 0);
+    for (int r = 0; r < 4; r++) D64[(4 * (l / 16) + r) * 16 + l % 16] = e[r];
+}
+
+void launch_mfma_probe(const float *A32, const float *B32, float *D32, const double *A64,
+                       const double *B64, double *D64, hipStream_t s)
+{
+    mfma_probe_kernel<<<1, 64, 0, s>>>(A32, B32, D32, A64, B64, D64);
+}
+
+}  // namespace qjx

@@ -61,6 +61,10 @@ struct StepArgs {
     const double *dt;         // [D]
     const double *Ldiag;      // [d]   L_kk
     const double *Loff;       // [d-1] L_{k+1,k} = -Loff[k], L_{k,k+1} = +Loff[k]
+    int *jlist;               // [M] trajectories that jump in this iteration
+    int *jsrc;                // [M] slot holding the pre-jump state (step result)
+    double *jnorm;            // [M] its ||psi||^2
+    int *jcnt;                // [1] entries in jlist (cleared by the GEMM kernel)
     JumpLog log;
     int *err;                 // [4]: code, trajectory, bits of the offending norm
     unsigned long long *counters;  // [0] matvecs, [1] jumps

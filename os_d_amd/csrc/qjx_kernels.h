@@ -8,6 +8,10 @@
 namespace qjx {
 
 constexpr int kMaxGroups = 64;   // 2 branches x up to 32 levels
+constexpr int kGemmFma = 0;      // LDS-tiled vector FMA, 64 x 64 tiles
+constexpr int kGemmMfma = 1;     // matrix cores, 128 x 128 tiles
+// rows of the block-real state per GEMM row tile
+inline int gemm_bm(int kind) { return kind == kGemmMfma ? 128 : 64; }
 
 // Per-trajectory initial state: psi = e_{start_state} (start_type 0) or the
 // uniform vector (otherwise); RNG state x_{seed+tr} of the MCG59 leap-frog stream.
@@ -22,8 +26,14 @@ void launch_seed_lists(TrajState *st, int M, int n_steps, int *list, int *cnt, i
 void launch_reset_times(TrajState *st, int M, hipStream_t s);
 void launch_reset_logn(TrajState *st, int M, hipStream_t s);
 
-// One propagation iteration (grouped GEMM + decide) over the lists in a.
-void launch_step(int prec, int gemm_kind, const StepArgs &a, uint64_t stride, hipStream_t s);
+// One propagation iteration over the lists in a: the grouped GEMM, then the
+// decide kernel (crossing test, bisection bookkeeping, jumps, next lists).
+void launch_step_gemm(int prec, int gemm_kind, const StepArgs &a, hipStream_t s);
+void launch_step_decide(int prec, const StepArgs &a, uint64_t stride, hipStream_t s);
+void launch_step_gemm_mfma(int prec, const StepArgs &a, hipStream_t s);
+// one wave: D32 = A32 (32x2) B32 (2x32), D64 = A64 (16x4) B64 (4x16), row-major
+void launch_mfma_probe(const float *A32, const float *B32, float *D32, const double *A64,
+                       const double *B64, double *D64, hipStream_t s);
 
 // Observables of every trajectory (core.cpp:665-725): norm, mean, m2, energy;
 // dispersion against mean_start; optional |psi_k|^2 / norm per trajectory.

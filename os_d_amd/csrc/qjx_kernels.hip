@@ -95,7 +95,10 @@ __global__ __launch_bounds__(256) void step_gemm_fma(StepArgs a)
     __shared__ double s_norm[BM / 4][BN + 1];
 
     const int tid = threadIdx.x;
-    if (blockIdx.x == 0 && blockIdx.y == 0 && tid < a.G) a.cnt_clear[tid] = 0;
+    if (blockIdx.x == 0 && blockIdx.y == 0) {
+        if (tid < a.G) a.cnt_clear[tid] = 0;
+        if (tid == 0) a.jcnt[0] = 0;
+    }
     group_prefix(a.cnt, a.G, BN, s_pre);
     const int ct = blockIdx.y;
     if (ct >= s_pre[a.G]) return;
@@ -124,18 +127,18 @@ __global__ __launch_bounds__(256) void step_gemm_fma(StepArgs a)
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[i][j] = T(0);
 
-    // load mapping
-    const int a_k = tid / (BM / 4), a_i = (tid % (BM / 4)) * 4;      // BK x BM / 4 threads
+    // load mapping: 4 consecutive k of one row of U (row-major) / one column of X
+    const int a_i = tid / (BK / 4), a_k = (tid % (BK / 4)) * 4;      // BM x BK / 4 threads
     const int b_j = tid / (BK / 4), b_k = (tid % (BK / 4)) * 4;      // BN x BK / 4 threads
     const T *xcol = nullptr;
     if (b_j < BN && s_tr[b_j] >= 0)
         xcol = reinterpret_cast<const T *>(a.slot[s_cur[b_j]]) + (size_t)s_tr[b_j] * a.L;
 
     for (int k0 = 0; k0 < a.L; k0 += BK) {
-        if (a_k < BK) {
-            const T *src = U + (size_t)(k0 + a_k) * a.L + r0 + a_i;
+        if (a_i < BM) {
+            const T *src = U + (size_t)(r0 + a_i) * a.L + k0 + a_k;
 #pragma unroll
-            for (int q = 0; q < 4; q++) As[a_k][a_i + q] = src[q];
+            for (int q = 0; q < 4; q++) As[a_k + q][a_i] = src[q];
         }
         if (b_j < BN) {
 #pragma unroll
@@ -183,48 +186,6 @@ __global__ __launch_bounds__(256) void step_gemm_fma(StepArgs a)
 // ---------------------------------------------------------------------------
 // Decide kernel: crossing test, recursion bookkeeping, jumps, next group lists.
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ void apply_jump(const StepArgs &a, int tr, int src_slot, int dst_slot, double n)
-{
-    // recovery (experiment.cpp:1273-1382) for the single dimer dissipator:
-    // phi <- phi / ||phi||; v = L phi; phi <- v / sqrt(||v||^2).
-    // The state is read from src_slot (the step just taken) and the result is
-    // written to dst_slot (the pre-step state, no longer needed).
-    const int lane = threadIdx.x & 63;
-    const T *src = reinterpret_cast<const T *>(a.slot[src_slot]) + (size_t)tr * a.L;
-    T *dst = reinterpret_cast<T *>(a.slot[dst_slot]) + (size_t)tr * a.L;
-    const double sn = sqrt(n);
-    const int d = a.d, dp = a.dp;
-    double p = 0.0;
-    for (int k = lane; k < d; k += 64) {
-        const double re0 = (double)src[k] / sn, im0 = (double)src[dp + k] / sn;
-        double vre = 0.0, vim = 0.0;
-        if (k > 0) {
-            const double l = -a.Loff[k - 1];
-            vre += l * ((double)src[k - 1] / sn);
-            vim += l * ((double)src[dp + k - 1] / sn);
-        }
-        vre += a.Ldiag[k] * re0;
-        vim += a.Ldiag[k] * im0;
-        if (k + 1 < d) {
-            const double l = a.Loff[k];
-            vre += l * ((double)src[k + 1] / sn);
-            vim += l * ((double)src[dp + k + 1] / sn);
-        }
-        p += vre * vre + vim * vim;
-        dst[k] = (T)vre;
-        dst[dp + k] = (T)vim;
-    }
-    p = wave_sum(p);
-    const double sp = sqrt(p / 1.0);
-    for (int k = lane; k < d; k += 64) {
-        // FP64 form: dst holds v exactly.  Reduced forms re-read the rounded v;
-        // the division is done in FP64 either way.
-        dst[k] = (T)((double)dst[k] / sp);
-        dst[dp + k] = (T)((double)dst[dp + k] / sp);
-    }
-}
-
 // After a completed step at level l (accepted, or the finest-level step that
 // carried a jump; t has already advanced by dt_l): counter++ and, while levels
 // finish, return to the parent (qj_proc.cpp:325-342: t -= dt_parent, then
@@ -263,13 +224,9 @@ __global__ __launch_bounds__(256) void step_decide(StepArgs a, uint64_t stride)
     const int total = s_pre[a.G];
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx == 0) atomicAdd(&a.counters[0], (unsigned long long)total);
-    // whole waves leave together so that the jump loop below runs on full waves
-    if ((int)(blockIdx.x * blockDim.x + (threadIdx.x & ~63)) >= total) return;
-    int tr = -1;
-    bool jumped = false;
-    int jsrc = 0, jdst = 0;
-    double n = 0.0;
     if (idx < total) {
+        int tr;
+        double n = 0.0;
         const int g = find_group(s_pre, a.G, idx);
         tr = a.list[(size_t)g * a.M + (idx - s_pre[g])];
         TrajState s = a.st[tr];
@@ -307,9 +264,12 @@ __global__ __launch_bounds__(256) void step_decide(StepArgs a, uint64_t stride)
                     double e = mcg_next(s.rng, stride);
                     while (e == 0.0) e = mcg_next(s.rng, stride);
                     s.eta = e;
-                    jumped = true;
-                    jsrc = s.cur ^ 1;     // the step just taken
-                    jdst = s.cur;         // jumped state replaces the pre-step state
+                    // the jump kernel reads the step just taken (slot cur ^ 1) and
+                    // writes the jumped state over the pre-step state (slot cur)
+                    const int j = atomicAdd(a.jcnt, 1);
+                    a.jlist[j] = tr;
+                    a.jsrc[j] = s.cur ^ 1;
+                    a.jnorm[j] = n;
                 } else {
                     s.cur ^= 1;
                 }
@@ -330,17 +290,86 @@ __global__ __launch_bounds__(256) void step_decide(StepArgs a, uint64_t stride)
             a.list_next[(size_t)gn * a.M + pos] = tr;
         }
     }
-    // jumps, one wavefront per jumping trajectory
-    unsigned long long mask = __ballot(jumped);
-    if (mask && (threadIdx.x & 63) == 0) atomicAdd(&a.counters[1], (unsigned long long)__popcll(mask));
-    while (mask) {
-        const int l = __ffsll((long long)mask) - 1;
-        mask &= mask - 1;
-        const int jtr = __shfl(tr, l, 64);
-        const int js = __shfl(jsrc, l, 64);
-        const int jd = __shfl(jdst, l, 64);
-        const double jn = __shfl(n, l, 64);
-        apply_jump<T>(a, jtr, js, jd, jn);
+}
+
+// ---------------------------------------------------------------------------
+// Jumps: recovery (experiment.cpp:1273-1382) for the single dimer dissipator,
+// one workgroup per jumping trajectory (persistent over the jump list):
+//   phi <- phi / ||phi||;  v = L phi;  phi <- v / sqrt(||v||^2)
+// in FP64.  L is tridiagonal: L_kk = Ldiag[k], L_{k+1,k} = -Loff[k],
+// L_{k,k+1} = +Loff[k]; v_k sums its three terms in ascending column order.
+// ---------------------------------------------------------------------------
+constexpr int kJumpThreads = 256;
+constexpr int kMaxDim = 2048;                       // largest supported d
+constexpr int kJumpPer = kMaxDim / kJumpThreads;    // elements per thread
+
+template <typename T>
+__global__ __launch_bounds__(kJumpThreads) void step_jump(StepArgs a)
+{
+    __shared__ double pre[kMaxDim + 2];
+    __shared__ double pim[kMaxDim + 2];
+    __shared__ double red[kJumpThreads / 64];
+    const int njump = a.jcnt[0];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int d = a.d, dp = a.dp;
+    if (blockIdx.x == 0 && tid == 0 && njump > 0) atomicAdd(&a.counters[1], (unsigned long long)njump);
+    for (int j = blockIdx.x; j < njump; j += gridDim.x) {
+        const int tr = a.jlist[j];
+        const int src = a.jsrc[j];
+        const double sn = sqrt(a.jnorm[j]);
+        const T *x = reinterpret_cast<const T *>(a.slot[src]) + (size_t)tr * a.L;
+        T *y = reinterpret_cast<T *>(a.slot[src ^ 1]) + (size_t)tr * a.L;
+        // pre/pim[k + 1] = phi_k / ||phi||, zero-padded at both ends
+        for (int k = tid; k < d; k += kJumpThreads) {
+            pre[k + 1] = (double)x[k] / sn;
+            pim[k + 1] = (double)x[dp + k] / sn;
+        }
+        if (tid == 0) {
+            pre[0] = pim[0] = 0.0;
+            pre[d + 1] = pim[d + 1] = 0.0;
+        }
+        __syncthreads();
+        double vre[kJumpPer], vim[kJumpPer];
+        double p = 0.0;
+#pragma unroll
+        for (int m = 0; m < kJumpPer; m++) {
+            const int k = tid + m * kJumpThreads;
+            vre[m] = vim[m] = 0.0;
+            if (k < d) {
+                double r = 0.0, i = 0.0;
+                if (k > 0) {
+                    const double l = -a.Loff[k - 1];
+                    r += l * pre[k];
+                    i += l * pim[k];
+                }
+                r += a.Ldiag[k] * pre[k + 1];
+                i += a.Ldiag[k] * pim[k + 1];
+                if (k + 1 < d) {
+                    const double l = a.Loff[k];
+                    r += l * pre[k + 2];
+                    i += l * pim[k + 2];
+                }
+                vre[m] = r;
+                vim[m] = i;
+                p += r * r + i * i;
+            }
+        }
+        p = wave_sum(p);
+        if (lane == 0) red[wid] = p;
+        __syncthreads();
+        double tot = 0.0;
+#pragma unroll
+        for (int w = 0; w < kJumpThreads / 64; w++) tot += red[w];
+        const double sp = sqrt(tot / 1.0);
+#pragma unroll
+        for (int m = 0; m < kJumpPer; m++) {
+            const int k = tid + m * kJumpThreads;
+            if (k < d) {
+                y[k] = (T)(vre[m] / sp);
+                y[dp + k] = (T)(vim[m] / sp);
+            }
+        }
+        __syncthreads();   // LDS reuse by the next jump of this workgroup
     }
 }
 
@@ -475,20 +504,28 @@ void launch_reset_logn(TrajState *st, int M, hipStream_t s)
 // ---------------------------------------------------------------------------
 // Propagation iteration
 // ---------------------------------------------------------------------------
-template <typename T>
-static void step_impl(const StepArgs &a, uint64_t stride, hipStream_t s)
+void launch_step_gemm(int prec, int gemm_kind, const StepArgs &a, hipStream_t s)
 {
+    if (gemm_kind == kGemmMfma) {
+        launch_step_gemm_mfma(prec, a, s);
+        return;
+    }
     constexpr int BM = 64, BN = 64, BK = 16;
     dim3 grid(a.R, (a.M + BN - 1) / BN + a.G);
-    step_gemm_fma<T, BM, BN, BK><<<grid, 256, 0, s>>>(a);
-    step_decide<T><<<(a.M + 255) / 256, 256, 0, s>>>(a, stride);
+    if (prec == kFP64) step_gemm_fma<double, BM, BN, BK><<<grid, 256, 0, s>>>(a);
+    else step_gemm_fma<float, BM, BN, BK><<<grid, 256, 0, s>>>(a);
 }
 
-void launch_step(int prec, int gemm_kind, const StepArgs &a, uint64_t stride, hipStream_t s)
+void launch_step_decide(int prec, const StepArgs &a, uint64_t stride, hipStream_t s)
 {
-    (void)gemm_kind;
-    if (prec == kFP64) step_impl<double>(a, stride, s);
-    else step_impl<float>(a, stride, s);
+    const int jblocks = a.M < 2048 ? a.M : 2048;
+    if (prec == kFP64) {
+        step_decide<double><<<(a.M + 255) / 256, 256, 0, s>>>(a, stride);
+        step_jump<double><<<jblocks, kJumpThreads, 0, s>>>(a);
+    } else {
+        step_decide<float><<<(a.M + 255) / 256, 256, 0, s>>>(a, stride);
+        step_jump<float><<<jblocks, kJumpThreads, 0, s>>>(a);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -740,8 +777,8 @@ void launch_zaxpy(int n, const double *A, double alpha, double *C, hipStream_t s
     zaxpy_kernel<<<1024, 256, 0, s>>>(n, A, alpha, C);
 }
 
-// Block-real k-major operand: element (i, k) of [[Re U, -Im U], [Im U, Re U]]
-// at k * 2dp + i; rows/columns beyond the n x n blocks are zero.
+// Block-real row-major operand: element (i, k) of [[Re U, -Im U], [Im U, Re U]]
+// at i * 2dp + k; rows/columns beyond the n x n blocks are zero.
 template <typename T>
 __global__ void pack_propagator_kernel(int n, int dp, const double *U, T *dst)
 {
@@ -749,7 +786,7 @@ __global__ void pack_propagator_kernel(int n, int dp, const double *U, T *dst)
     const size_t total = (size_t)L * L;
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total;
          e += (size_t)gridDim.x * blockDim.x) {
-        const int k = (int)(e / L), i = (int)(e % L);
+        const int i = (int)(e / L), k = (int)(e % L);
         const int ib = i / dp, ii = i % dp, kb = k / dp, kk = k % dp;
         double v = 0.0;
         if (ii < n && kk < n) {

@@ -46,7 +46,9 @@ class EnsembleDesc(ctypes.Structure):
 
 class Stats(ctypes.Structure):
     _fields_ = [("matvecs", ctypes.c_int64), ("iterations", ctypes.c_int64),
-                ("jumps", ctypes.c_int64), ("gemm_flops", ctypes.c_double)]
+                ("jumps", ctypes.c_int64), ("gemm_flops", ctypes.c_double),
+                ("gemm_ms", ctypes.c_double), ("decide_ms", ctypes.c_double),
+                ("gemm_launches", ctypes.c_int64)]
 
 
 _lib = None
@@ -72,6 +74,9 @@ SIGNATURES = [
     ("qjx_set_propagator", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D]),
     ("qjx_get_stats", ctypes.c_int, [ctypes.c_void_p, _P(Stats)]),
     ("qjx_dim", ctypes.c_int, [ctypes.c_void_p]),
+    ("qjx_set_profiling", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("qjx_mfma_probe", ctypes.c_int, [_P(ctypes.c_float), _P(ctypes.c_float), _P(ctypes.c_float), _D, _D,
+                                      _D]),
     ("qjx_run_config", ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]),
 ]
 
@@ -199,7 +204,23 @@ class Engine:
         s = Stats()
         _check(self.lib, self.lib.qjx_get_stats(self.h, ctypes.byref(s)))
         return {"matvecs": s.matvecs, "iterations": s.iterations, "jumps": s.jumps,
-                "gemm_flops": s.gemm_flops}
+                "gemm_flops": s.gemm_flops, "gemm_ms": s.gemm_ms, "decide_ms": s.decide_ms,
+                "gemm_launches": s.gemm_launches}
+
+    def set_profiling(self, on=True):
+        _check(self.lib, self.lib.qjx_set_profiling(self.h, 1 if on else 0))
+
+
+def mfma_probe(A32, B32, A64, B64):
+    """Run the matrix-core layout probe; returns (D32, D64)."""
+    lib = load_library()
+    A32, B32 = (np.ascontiguousarray(x, dtype=np.float32) for x in (A32, B32))
+    A64, B64 = (np.ascontiguousarray(x, dtype=np.float64) for x in (A64, B64))
+    D32 = np.zeros((32, 32), dtype=np.float32)
+    D64 = np.zeros((16, 16), dtype=np.float64)
+    _check(lib, lib.qjx_mfma_probe(_ptr(A32, ctypes.c_float), _ptr(B32, ctypes.c_float),
+                                   _ptr(D32, ctypes.c_float), _ptr(A64), _ptr(B64), _ptr(D64)))
+    return D32, D64
 
 
 def run_config(config_path, params_path, precision=FP64, steps_per_period=0):
