@@ -215,7 +215,7 @@ __global__ __launch_bounds__(MF_THREADS) void step_gemm_mfma_f32(StepArgs a)
 // FP64: v_mfma_f64_16x16x4f64
 //   A (16 x 4): lane l holds A[l % 16][l / 16]
 //   B (4 x 16): lane l holds B[l / 16][l % 16]
-//   D (16 x 16): lane l, register r holds D[4 (l / 16) + r][l % 16]
+//   D (16 x 16): lane l, register r holds D[(l / 16) + 4 r][l % 16]
 // Within a K block of 32, MFMA step s (0..7) uses k = s + 8 q for lane quarter q.
 // ---------------------------------------------------------------------------
 constexpr int F64_PITCH = MF_BK + 2;   // doubles (16-byte aligned rows, skewed banks)
@@ -313,13 +313,12 @@ __global__ __launch_bounds__(MF_THREADS) void step_gemm_mfma_f64(StepArgs a)
         double *y = tr >= 0 ? reinterpret_cast<double *>(a.slot[s_cur[c] ^ 1]) + (size_t)tr * L : nullptr;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const int row = r0 + wm * 64 + i * 16 + 4 * qq;
-            if (y) {
-                *reinterpret_cast<double2v *>(y + row) = double2v{acc[i][j][0], acc[i][j][1]};
-                *reinterpret_cast<double2v *>(y + row + 2) = double2v{acc[i][j][2], acc[i][j][3]};
-            }
+            const int row = r0 + wm * 64 + i * 16 + qq;
 #pragma unroll
-            for (int e = 0; e < 4; e++) ns[j] += acc[i][j][e] * acc[i][j][e];
+            for (int e = 0; e < 4; e++) {
+                if (y) y[row + 4 * e] = acc[i][j][e];
+                ns[j] += acc[i][j][e] * acc[i][j][e];
+            }
         }
     }
 #pragma unroll
@@ -356,7 +355,7 @@ __global__ void mfma_probe_kernel(const float *A32, const float *B32, float *D32
     for (int r = 0; r < 16; r++) D32[(8 * (r / 4) + 4 * (l / 32) + r % 4) * 32 + l % 32] = c[r];
     doublex4 e = {0.0, 0.0, 0.0, 0.0};
     e = __builtin_amdgcn_mfma_f64_16x16x4f64(A64[(l % 16) * 4 + l / 16], B64[(l / 16) * 16 + l % 16], e, 0, 0, 0);
-    for (int r = 0; r < 4; r++) D64[(4 * (l / 16) + r) * 16 + l % 16] = e[r];
+    for (int r = 0; r < 4; r++) D64[(l / 16 + 4 * r) * 16 + l % 16] = e[r];
 }
 
 void launch_mfma_probe(const float *A32, const float *B32, float *D32, const double *A64,
