@@ -13,7 +13,7 @@
 // lands (results do not depend on batch composition).
 //
 // FP32 form : v_mfma_f32_32x32x2f32, 4 waves as 2 x 2, wave tile 64 x 64, WG tile 128 x 128.
-// FP64 form : v_mfma_f64_16x16x4f64, 4 waves as 2 x 2, wave tile 64 x 64, WG tile 128 x 128.
+// FP64 form : v_mfma_f64_16x16x4f64, 8 waves as 4 x 2, wave tile 32 x 64, WG tile 128 x 128.
 // Epilogue  : output to the trajectory's spare slot, FP64 sum of squares per column over
 //             the WG's 128 rows into normpart[row tile][trajectory].
 #include <hip/hip_runtime.h>
@@ -118,8 +118,8 @@ __global__ __launch_bounds__(MF_THREADS) void step_gemm_mfma_f32(StepArgs a)
     for (int q = 0; q < 4; q++) {
         const int r = cp_r + 32 * q;
         arow[q] = U + (size_t)(r0 + r) * L + cp_k;
-        const int tr = s_tr[r];
-        bcol[q] = tr >= 0 ? reinterpret_cast<const float *>(a.slot[s_cur[r]]) + (size_t)tr * L + cp_k : nullptr;
+        const int cc = r < ncol ? r : 0;     // columns past ncol re-read a valid one (never stored)
+        bcol[q] = reinterpret_cast<const float *>(a.slot[s_cur[cc]]) + (size_t)s_tr[cc] * L + cp_k;
     }
 
     floatx16 acc[2][2];
@@ -135,7 +135,7 @@ __global__ __launch_bounds__(MF_THREADS) void step_gemm_mfma_f32(StepArgs a)
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             ra[q] = *reinterpret_cast<const floatx4 *>(arow[q] + k0);
-            rb[q] = bcol[q] ? *reinterpret_cast<const floatx4 *>(bcol[q] + k0) : floatx4{0.f, 0.f, 0.f, 0.f};
+            rb[q] = *reinterpret_cast<const floatx4 *>(bcol[q] + k0);
         }
     };
     auto lstore = [&]() {
@@ -187,15 +187,15 @@ __global__ __launch_bounds__(MF_THREADS) void step_gemm_mfma_f32(StepArgs a)
 #pragma unroll
     for (int j = 0; j < 2; j++) {
         const int c = wn * 64 + j * 32 + li;
-        const int tr = s_tr[c];
-        float *y = tr >= 0 ? reinterpret_cast<float *>(a.slot[s_cur[c] ^ 1]) + (size_t)tr * L : nullptr;
+        const bool ok = c < ncol;
+        float *y = reinterpret_cast<float *>(a.slot[s_cur[ok ? c : 0] ^ 1]) + (size_t)s_tr[ok ? c : 0] * L;
 #pragma unroll
         for (int i = 0; i < 2; i++) {
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int row = r0 + wm * 64 + i * 32 + 8 * q + 4 * h;
                 floatx4 v = {acc[i][j][4 * q], acc[i][j][4 * q + 1], acc[i][j][4 * q + 2], acc[i][j][4 * q + 3]};
-                if (y) *reinterpret_cast<floatx4 *>(y + row) = v;
+                if (ok) *reinterpret_cast<floatx4 *>(y + row) = v;
 #pragma unroll
                 for (int e = 0; e < 4; e++) ns[j] += (double)v[e] * (double)v[e];
             }
@@ -216,60 +216,72 @@ __global__ __launch_bounds__(MF_THREADS) void step_gemm_mfma_f32(StepArgs a)
 //   A (16 x 4): lane l holds A[l % 16][l / 16]
 //   B (4 x 16): lane l holds B[l / 16][l % 16]
 //   D (16 x 16): lane l, register r holds D[(l / 16) + 4 r][l % 16]
+// 8 waves as 4 (rows) x 2 (columns), wave tile 32 x 64 (2 x 4 blocks), WG tile 128 x 128.
 // Within a K block of 32, MFMA step s (0..7) uses k = s + 8 q for lane quarter q.
 // ---------------------------------------------------------------------------
 constexpr int F64_PITCH = MF_BK + 2;   // doubles (16-byte aligned rows, skewed banks)
+constexpr int F64_THREADS = 512;
 
-__global__ __launch_bounds__(MF_THREADS) void step_gemm_mfma_f64(StepArgs a)
+__global__ __launch_bounds__(F64_THREADS) void step_gemm_mfma_f64(StepArgs a)
 {
+    typedef double double2v __attribute__((ext_vector_type(2)));
     __shared__ int s_pre[kMaxGroups + 1];
     __shared__ int s_tr[MF_BN];
     __shared__ int s_cur[MF_BN];
     __shared__ __attribute__((aligned(16))) double As[MF_BM * F64_PITCH];
     __shared__ __attribute__((aligned(16))) double Bs[MF_BN * F64_PITCH];
-    __shared__ double s_norm[2][MF_BN];
+    __shared__ double s_norm[4][MF_BN];
 
     int g, ncol;
     if (!mf_tile(a, s_pre, s_tr, s_cur, g, ncol)) return;
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int wm = wid >> 1, wn = wid & 1;
+    const int wm = wid >> 1, wn = wid & 1;             // 4 x 2 waves
     const int L = a.L;
     const int r0 = blockIdx.x * MF_BM;
     const double *U = reinterpret_cast<const double *>(a.props) + (size_t)g * L * L;
 
-    // 128 rows x 32 k doubles = 2048 double2, 8 per thread: row = tid / 16 + 16 q, k2 = (tid % 16) * 2
+    // global -> LDS: 128 rows x 32 k = 2048 double2, 4 per thread:
+    // row = tid / 16 + 32 q, k2 = (tid % 16) * 2.  Columns past ncol re-read a
+    // valid column (their results are never stored).
     const int cp_k = (tid & 15) * 2;
     const int cp_r = tid >> 4;
-    doublex4 acc[4][4];
+    const double *arow = U + (size_t)(r0 + cp_r) * L + cp_k;
+    const size_t astep = (size_t)32 * L;
+    const double *bcol[4];
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int q = 0; q < 4; q++) {
+        const int c = cp_r + 32 * q;
+        const int cc = c < ncol ? c : 0;
+        bcol[q] = reinterpret_cast<const double *>(a.slot[s_cur[cc]]) + (size_t)s_tr[cc] * L + cp_k;
+    }
+
+    doublex4 acc[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[i][j] = doublex4{0.0, 0.0, 0.0, 0.0};
 
-    typedef double double2v __attribute__((ext_vector_type(2)));
-    double2v ra[8], rb[8];
+    double2v ra[4], rb[4];
     auto gload = [&](int k0) {
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int r = cp_r + 16 * q;
-            ra[q] = *reinterpret_cast<const double2v *>(U + (size_t)(r0 + r) * L + k0 + cp_k);
-            const int tr = s_tr[r];
-            rb[q] = tr >= 0 ? *reinterpret_cast<const double2v *>(reinterpret_cast<const double *>(a.slot[s_cur[r]]) +
-                                                                 (size_t)tr * L + k0 + cp_k)
-                            : double2v{0.0, 0.0};
+        for (int q = 0; q < 4; q++) {
+            ra[q] = *reinterpret_cast<const double2v *>(arow + q * astep + k0);
+            rb[q] = *reinterpret_cast<const double2v *>(bcol[q] + k0);
         }
     };
     auto lstore = [&]() {
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int r = cp_r + 16 * q;
+        for (int q = 0; q < 4; q++) {
+            const int r = cp_r + 32 * q;
             *reinterpret_cast<double2v *>(&As[r * F64_PITCH + cp_k]) = ra[q];
             *reinterpret_cast<double2v *>(&Bs[r * F64_PITCH + cp_k]) = rb[q];
         }
     };
 
     const int qq = lane >> 4, li = lane & 15;
+    const double *pa0 = &As[(wm * 32 + li) * F64_PITCH + 8 * qq];
+    const double *pb0 = &Bs[(wn * 64 + li) * F64_PITCH + 8 * qq];
     gload(0);
     lstore();
     __syncthreads();
@@ -279,24 +291,28 @@ __global__ __launch_bounds__(MF_THREADS) void step_gemm_mfma_f64(StepArgs a)
 #pragma unroll
         for (int half = 0; half < 2; half++) {
             // 4 consecutive k (= s + 8 qq, s = 4 half .. 4 half + 3) per lane and block
-            double2v fa[4][2], fb[4][2];
+            double2v fa[2][2], fb[4][2];
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const double *pa = &As[(wm * 64 + i * 16 + li) * F64_PITCH + 8 * qq + 4 * half];
-                const double *pb = &Bs[(wn * 64 + i * 16 + li) * F64_PITCH + 8 * qq + 4 * half];
+            for (int i = 0; i < 2; i++) {
+                const double *pa = pa0 + i * 16 * F64_PITCH + 4 * half;
                 fa[i][0] = *reinterpret_cast<const double2v *>(pa);
                 fa[i][1] = *reinterpret_cast<const double2v *>(pa + 2);
-                fb[i][0] = *reinterpret_cast<const double2v *>(pb);
-                fb[i][1] = *reinterpret_cast<const double2v *>(pb + 2);
             }
 #pragma unroll
-            for (int s = 0; s < 4; s++)
+            for (int j = 0; j < 4; j++) {
+                const double *pb = pb0 + j * 16 * F64_PITCH + 4 * half;
+                fb[j][0] = *reinterpret_cast<const double2v *>(pb);
+                fb[j][1] = *reinterpret_cast<const double2v *>(pb + 2);
+            }
 #pragma unroll
-                for (int i = 0; i < 4; i++)
+            for (int s4 = 0; s4 < 4; s4++)
+#pragma unroll
+                for (int i = 0; i < 2; i++)
 #pragma unroll
                     for (int j = 0; j < 4; j++)
-                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i][s >> 1][s & 1], fb[j][s >> 1][s & 1],
-                                                                          acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i][s4 >> 1][s4 & 1],
+                                                                          fb[j][s4 >> 1][s4 & 1], acc[i][j], 0,
+                                                                          0, 0);
         }
         __syncthreads();
         if (more) {
@@ -309,14 +325,14 @@ __global__ __launch_bounds__(MF_THREADS) void step_gemm_mfma_f64(StepArgs a)
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         const int c = wn * 64 + j * 16 + li;
-        const int tr = s_tr[c];
-        double *y = tr >= 0 ? reinterpret_cast<double *>(a.slot[s_cur[c] ^ 1]) + (size_t)tr * L : nullptr;
+        const bool ok = c < ncol;
+        double *y = reinterpret_cast<double *>(a.slot[s_cur[ok ? c : 0] ^ 1]) + (size_t)s_tr[ok ? c : 0] * L;
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int row = r0 + wm * 64 + i * 16 + qq;
+        for (int i = 0; i < 2; i++) {
+            const int row = r0 + wm * 32 + i * 16 + qq;
 #pragma unroll
             for (int e = 0; e < 4; e++) {
-                if (y) y[row + 4 * e] = acc[i][j][e];
+                if (ok) y[row + 4 * e] = acc[i][j][e];
                 ns[j] += acc[i][j][e] * acc[i][j][e];
             }
         }
@@ -331,13 +347,15 @@ __global__ __launch_bounds__(MF_THREADS) void step_gemm_mfma_f64(StepArgs a)
         for (int j = 0; j < 4; j++) s_norm[wm][wn * 64 + j * 16 + li] = ns[j];
     }
     __syncthreads();
-    if (tid < ncol) a.normpart[(size_t)blockIdx.x * a.M + s_tr[tid]] = s_norm[0][tid] + s_norm[1][tid];
+    if (tid < ncol)
+        a.normpart[(size_t)blockIdx.x * a.M + s_tr[tid]] =
+            ((s_norm[0][tid] + s_norm[1][tid]) + s_norm[2][tid]) + s_norm[3][tid];
 }
 
 void launch_step_gemm_mfma(int prec, const StepArgs &a, hipStream_t s)
 {
     dim3 grid(a.L / MF_BM, (a.M + MF_BN - 1) / MF_BN + a.G);
-    if (prec == kFP64) step_gemm_mfma_f64<<<grid, MF_THREADS, 0, s>>>(a);
+    if (prec == kFP64) step_gemm_mfma_f64<<<grid, F64_THREADS, 0, s>>>(a);
     else step_gemm_mfma_f32<<<grid, MF_THREADS, 0, s>>>(a);
 }
 

@@ -220,13 +220,20 @@ template <typename T>
 __global__ __launch_bounds__(256) void step_decide(StepArgs a, uint64_t stride)
 {
     __shared__ int s_pre[kMaxGroups + 1];
-    group_prefix(a.cnt, a.G, 1, s_pre);
+    __shared__ int s_cnt[kMaxGroups + 1];    // per-group appends of this block (+ jumps)
+    __shared__ int s_base[kMaxGroups + 1];
+    if (threadIdx.x <= kMaxGroups) s_cnt[threadIdx.x] = 0;
+    group_prefix(a.cnt, a.G, 1, s_pre);      // (ends with a barrier)
     const int total = s_pre[a.G];
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx == 0) atomicAdd(&a.counters[0], (unsigned long long)total);
+    if ((int)(blockIdx.x * blockDim.x) >= total) return;   // whole block idle
+    // appends are reserved per block: LDS counters first, one global atomic per group
+    int app_g = -1, app_local = 0;           // next-group append
+    int jmp_local = -1;                      // jump-list append
+    int tr = -1, jsrc = 0;
+    double n = 0.0;
     if (idx < total) {
-        int tr;
-        double n = 0.0;
         const int g = find_group(s_pre, a.G, idx);
         tr = a.list[(size_t)g * a.M + (idx - s_pre[g])];
         TrajState s = a.st[tr];
@@ -266,10 +273,8 @@ __global__ __launch_bounds__(256) void step_decide(StepArgs a, uint64_t stride)
                     s.eta = e;
                     // the jump kernel reads the step just taken (slot cur ^ 1) and
                     // writes the jumped state over the pre-step state (slot cur)
-                    const int j = atomicAdd(a.jcnt, 1);
-                    a.jlist[j] = tr;
-                    a.jsrc[j] = s.cur ^ 1;
-                    a.jnorm[j] = n;
+                    jmp_local = atomicAdd(&s_cnt[kMaxGroups], 1);
+                    jsrc = s.cur ^ 1;
                 } else {
                     s.cur ^= 1;
                 }
@@ -285,10 +290,25 @@ __global__ __launch_bounds__(256) void step_decide(StepArgs a, uint64_t stride)
         }
         a.st[tr] = s;
         if (alive && s.pos < s.target) {
-            const int gn = s.b * a.D + s.lvl;
-            const int pos = atomicAdd(&a.cnt_next[gn], 1);
-            a.list_next[(size_t)gn * a.M + pos] = tr;
+            app_g = s.b * a.D + s.lvl;
+            app_local = atomicAdd(&s_cnt[app_g], 1);
         }
+    }
+    __syncthreads();
+    if (threadIdx.x < a.G) {
+        const int c = s_cnt[threadIdx.x];
+        s_base[threadIdx.x] = c ? atomicAdd(&a.cnt_next[threadIdx.x], c) : 0;
+    } else if (threadIdx.x == kMaxGroups) {
+        const int c = s_cnt[kMaxGroups];
+        s_base[kMaxGroups] = c ? atomicAdd(a.jcnt, c) : 0;
+    }
+    __syncthreads();
+    if (app_g >= 0) a.list_next[(size_t)app_g * a.M + s_base[app_g] + app_local] = tr;
+    if (jmp_local >= 0) {
+        const int j = s_base[kMaxGroups] + jmp_local;
+        a.jlist[j] = tr;
+        a.jsrc[j] = jsrc;
+        a.jnorm[j] = n;
     }
 }
 
