@@ -67,6 +67,8 @@ def lib():
         L.qjo_num_matvecs.argtypes = [ctypes.c_void_p]
         L.qjo_num_jumps.restype = ctypes.c_int64
         L.qjo_num_jumps.argtypes = [ctypes.c_void_p]
+        L.qjl_evolve.argtypes = [_P(Model), ctypes.c_int, ctypes.c_int, _D]
+        L.qjl_rhs.argtypes = [_P(Model), ctypes.c_double, _D, _D]
         _lib = L
     return _lib
 
@@ -92,6 +94,33 @@ def expm(A):
     rc = lib().qjo_expm(A.shape[0], _p(A.view(np.float64)), _p(E.view(np.float64)))
     assert rc == 0
     return E
+
+
+def lindblad_evolve(m, rho, n_periods, steps_per_period=2000):
+    """rho-space RK4 of the dimer Lindblad equation over n_periods periods."""
+    r = np.ascontiguousarray(rho, dtype=np.complex128).copy()
+    rc = lib().qjl_evolve(ctypes.byref(m), steps_per_period, n_periods, _p(r.view(np.float64)))
+    assert rc == 0
+    return r
+
+
+def lindblad_rhs(m, E, rho):
+    r = np.ascontiguousarray(rho, dtype=np.complex128)
+    out = np.zeros_like(r)
+    lib().qjl_rhs(ctypes.byref(m), E, _p(r.view(np.float64)), _p(out.view(np.float64)))
+    return out
+
+
+def lindblad_observables(m, rho):
+    """<k>, <(k - <k>)^2>, energy <H + E H_drv>, purity, diag(rho) of a density matrix."""
+    d = m.N + 1
+    k = np.arange(d)
+    p = np.real(np.diag(rho))
+    mean = float(p @ k)
+    mm = model_matrices(m)
+    Hs = mm["H"] + m.prm_E * mm["Hdrv"]
+    return {"mean": mean, "m2": float(p @ (k - mean) ** 2), "energy": float(np.real(np.trace(rho @ Hs))),
+            "purity": float(np.real(np.trace(rho @ rho))), "adr": p}
 
 
 def mcg59_uniform(k, mns, n):

@@ -1,0 +1,97 @@
+"""The drop-in runner (reference config.txt / params.txt in, output files out) on the
+GPU, checked against the oracle driven through the same STD experiment."""
+import glob
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+CONFIG = """sys_id 0
+task_id 1
+prop_id 0
+is_debug 0
+is_pp 0
+init_fn
+path {path}
+seed 0
+mns 1000000
+num_threads 1
+num_trajectories 24
+num_tp_periods 2
+num_obs_periods 6
+ex_deep 12
+rk_ns 10000
+"""
+
+PARAMS = """N 12
+diss_type 0
+diss_gamma 0.2
+diss_phase 0.0
+dimer_drv_type 0
+dimer_drv_ampl 1.5
+dimer_drv_freq 1.0
+dimer_drv_phase 0.0
+dimer_prm_E 1.0
+dimer_prm_U 0.5
+dimer_prm_J 1.0
+start_type 0
+start_state 0
+num_random_obs 0
+dump_obs 1
+dump_phi 0
+dump_phi_evo 0
+dump_adr_sep 0
+dump_adr_avg 1
+dump_evo_sep 1
+dump_evo_avg 0
+dump_type 0
+dump_num 3
+jump 2
+jumps_counts 0
+"""
+
+SUFFIX = ("_setup(0_1_0)_rnd(0_1000000)_N(12)_diss(0_0.2000_0.0000)_drv(0_1.5000_1.0000_0.0000)"
+          "_prm(1.0000_0.5000_1.0000)_start(0_0).txt")
+
+
+def test_run_config_outputs_match_oracle(tmp_path, oracle_mod):
+    from os_d_amd import qjx
+    print("case: run_config N=12 M=24 S=4", flush=True)
+    out = str(tmp_path) + "/"
+    (tmp_path / "config.txt").write_text(CONFIG.format(path=out))
+    (tmp_path / "params.txt").write_text(PARAMS)
+    qjx.run_config(str(tmp_path / "config.txt"), str(tmp_path / "params.txt"), qjx.FP64, 0)
+    files = sorted(os.path.basename(f) for f in glob.glob(out + "*" + SUFFIX))
+    for name in ("norm", "mean", "mean_start", "dispersion", "m2", "energy", "periods", "mean_evo",
+                 "norm_evo", "energy_evo", "m2_evo", "dispersion_evo", "adr_avg", "jump_times_0",
+                 "diss_types_0", "jump_norms_0", "jump_etas_0"):
+        assert name + SUFFIX in files, name
+    # oracle through the same experiment (trans 2 periods, 3 dumps of 2 periods)
+    m = oracle_mod.model(12, gamma=0.2, S=4, D=12)
+    o = oracle_mod.Ensemble(m, 24, jump=2)
+    o.trans(2)
+    c0 = o.chars(is_start=1)
+    evo = [c0]
+    for _ in range(3):
+        o.periods(2, 1)
+        evo.append(o.chars())
+    def load(name):
+        return np.loadtxt(out + name + SUFFIX)
+    np.testing.assert_allclose(load("mean"), evo[-1]["mean"], rtol=1e-9)
+    np.testing.assert_allclose(load("energy"), evo[-1]["energy"], rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(load("mean_start"), c0["mean"], rtol=1e-12)
+    me = load("mean_evo")                       # rows = dumps, columns = trajectories
+    assert me.shape == (4, 24)
+    for k in range(4):
+        np.testing.assert_allclose(me[k], evo[k]["mean"], rtol=1e-9)
+    assert list(np.atleast_1d(load("periods")).astype(int)) == [0, 2, 4, 6]
+    for tr in (0, 5, 23):
+        jt = np.atleast_1d(np.loadtxt(out + "jump_times_%d" % tr + SUFFIX)) \
+            if os.path.getsize(out + "jump_times_%d" % tr + SUFFIX) else np.zeros(0)
+        ref = o.get_jumps(tr)["times"]
+        np.testing.assert_allclose(jt, ref, rtol=1e-15)
+    text = open(out + "mean" + SUFFIX).read().splitlines()[0]
+    assert "e" in text and len(text.split("e")[0]) == 18      # setprecision(16) scientific
