@@ -11,6 +11,7 @@ HBM before the timed region.
 value      : trajectory-periods/s of the FP32 form (FP32 GEMM, FP64 norms, eta,
              bisection, jumps).
 fp64_value : the same for the FP64 form (same workload, fewer steps).
+bf16x3_value: the same for the 3 x BF16 split-product form (FP32 accumulate).
 roofline   : the step GEMM (dominant kernel), algorithmic flops 8 d^2 per
              propagator application, timed per launch with HIP events on the
              engine stream.
@@ -98,9 +99,24 @@ def main():
     ap.add_argument("--D", type=int, default=16)
     ap.add_argument("--fp64-steps", type=int, default=0, help="0 = max(2, steps // 3)")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-bf16x3", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (read + write), the "
+                         "calibration kernel for the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.copy_probe_mib:
+        import torch
+        n = args.copy_probe_mib * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.mul(x, 1.0, out=y)       # one elementwise kernel: n*4 bytes read, n*4 written
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4,
+                          "launches": 3}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the trajectory engine on a single "
               "device (multi-GPU sharding is described in DESIGN.md, not implemented).",
@@ -110,17 +126,23 @@ def main():
     from os_d_amd import qjx
     N, M, S, D = args.N, args.traj, args.S, args.D
     e32, r32 = run_form(qjx, qjx.FP32, N, M, S, D, args.warmup, args.steps)
+    e32.close()
+    bf = None
+    if not args.no_bf16x3:
+        ebf, bf = run_form(qjx, qjx.BF16X3, N, M, S, D, args.warmup, args.steps)
+        ebf.close()
     fp64 = None
     e64 = None
     if not args.no_fp64:
         k64 = args.fp64_steps or max(2, args.steps // 3)
-        e32.close()
         e64, r64 = run_form(qjx, qjx.FP64, N, M, S, D, args.warmup, k64)
         fp64 = r64
     cpu = None
     if not args.no_cpu:
-        src = e64 if e64 is not None else e32
-        cpu = cpu_baseline(src, N, S, D, args.cpu_seconds)
+        if e64 is None:
+            e64 = qjx.Engine(N, prm_E=1.0, prm_U=0.5, prm_J=1.0, drv_ampl=1.5, drv_freq=1.0,
+                             diss_gamma=0.1, ex_deep=D, steps_per_period=S, precision=qjx.FP64)
+        cpu = cpu_baseline(e64, N, S, D, args.cpu_seconds)
 
     launches = max(1, r32["gemm_launches"])
     avg_ms = r32["gemm_ms"] / launches
@@ -151,6 +173,10 @@ def main():
         "fp64_value": fp64["tp_per_s"] if fp64 else None,
         "fp64": fp64,
         "fp32_over_fp64": (r32["tp_per_s"] / fp64["tp_per_s"]) if fp64 else None,
+        "bf16x3_value": bf["tp_per_s"] if bf else None,
+        "bf16x3": bf,
+        "bf16x3_over_fp64": (bf["tp_per_s"] / fp64["tp_per_s"]) if (bf and fp64) else None,
+        "bf16x3_gemm_tflops_equiv": (bf["gemm_flops"] / (bf["gemm_ms"] * 1e-3) / 1e12) if bf else None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

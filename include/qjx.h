@@ -136,10 +136,11 @@ int qjx_get_stats(qjx_ctx *ctx, qjx_stats *out);
 int qjx_set_profiling(qjx_ctx *ctx, int on);
 
 /* Self-test of the matrix-core operand layouts the GEMM kernels assume (no
- * reference counterpart): D32 = A32 B32 (32x2 by 2x32, FP32) and D64 = A64 B64
- * (16x4 by 4x16, FP64), all row-major, computed by one wavefront. */
+ * reference counterpart): D32 = A32 B32 (32x2 by 2x32, FP32), D64 = A64 B64
+ * (16x4 by 4x16, FP64) and Dbf = Abf Bbf (32x16 by 16x32 with bf16 operands,
+ * FP32 result), all row-major, computed by one wavefront. */
 int qjx_mfma_probe(const float *A32, const float *B32, float *D32, const double *A64,
-                   const double *B64, double *D64);
+                   const double *B64, double *D64, const float *Abf, const float *Bbf, float *Dbf);
 /* Hilbert dimension of the context (N + 1). */
 int qjx_dim(const qjx_ctx *ctx);
 

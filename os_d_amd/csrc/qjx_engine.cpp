@@ -64,7 +64,8 @@ struct qjx_ctx {
     int d = 0, dp = 0, L = 0, D = 0, S = 0, G = 0, R = 0;
     int prec = 0;
     int gemm_kind = 0;
-    size_t esize = 8;
+    size_t esize = 8;              // state element bytes
+    size_t pesize = 8;             // propagator element bytes (per half for BF16x3)
     double T = 0.0;
     std::vector<double> dt;
     hipStream_t stream = nullptr;
@@ -75,6 +76,7 @@ struct qjx_ctx {
 
     // device model
     void *props = nullptr;         // 2*D * L*L elements
+    void *props_lo = nullptr;      // BF16x3: low halves
     double *d_dt = nullptr, *d_Ldiag = nullptr, *d_Loff = nullptr;
     double *d_HdiagE = nullptr, *d_Hoff = nullptr;
 
@@ -233,7 +235,13 @@ static double norm1_host(const std::vector<double> &A, int n)
 
 static void *prop_ptr(qjx_ctx *c, int b, int lvl)
 {
-    return (char *)c->props + ((size_t)(b * c->D + lvl)) * c->L * c->L * c->esize;
+    return (char *)c->props + ((size_t)(b * c->D + lvl)) * c->L * c->L * c->pesize;
+}
+
+static void *prop_lo_ptr(qjx_ctx *c, int b, int lvl)
+{
+    if (!c->props_lo) return nullptr;
+    return (char *)c->props_lo + ((size_t)(b * c->D + lvl)) * c->L * c->L * c->pesize;
 }
 
 extern "C" const char *qjx_last_error(void) { return g_err.c_str(); }
@@ -260,8 +268,8 @@ extern "C" int qjx_ctx_create(const qjx_model_desc *m, qjx_ctx **out)
         return fail(QJX_ERR_ARG, "ex_deep must be in [1, 30]");
     const int S = m->steps_per_period ? m->steps_per_period : 4;
     if (S < 2 || (S % 2)) return fail(QJX_ERR_ARG, "steps_per_period must be even and >= 2");
-    if (m->precision != QJX_FP64 && m->precision != QJX_FP32)
-        return fail(QJX_ERR_ARG, "precision form not available in this build");
+    if (m->precision != QJX_FP64 && m->precision != QJX_FP32 && m->precision != QJX_BF16X3)
+        return fail(QJX_ERR_ARG, "unknown precision form");
     if (!(m->drv_freq > 0.0)) return fail(QJX_ERR_ARG, "dimer_drv_freq must be > 0");
     int dev_count = 0;
     if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count < 1)
@@ -282,6 +290,11 @@ extern "C" int qjx_ctx_create(const qjx_model_desc *m, qjx_ctx **out)
     c->R = c->L / qjx::gemm_bm(c->gemm_kind);
     c->prec = m->precision;
     c->esize = c->prec == QJX_FP64 ? 8 : 4;
+    c->pesize = c->prec == QJX_FP64 ? 8 : c->prec == QJX_FP32 ? 4 : 2;
+    if (c->prec == QJX_BF16X3 && c->gemm_kind != qjx::kGemmMfma) {
+        qjx_ctx_destroy(c);
+        return fail(QJX_ERR_ARG, "the 3xBF16 form needs the matrix-core GEMM");
+    }
     c->T = 2.0 * kPi / m->drv_freq;
     c->dt.resize(c->D);
     double dt = c->T / double(S);
@@ -302,8 +315,10 @@ extern "C" int qjx_ctx_create(const qjx_model_desc *m, qjx_ctx **out)
     if ((rc = upload((void **)&c->d_Loff, c->Loff.data(), sizeof(double) * c->Loff.size()))) return bail(rc);
     if ((rc = upload((void **)&c->d_HdiagE, c->HdiagE.data(), sizeof(double) * c->d))) return bail(rc);
     if ((rc = upload((void **)&c->d_Hoff, c->Hoff.data(), sizeof(double) * c->Hoff.size()))) return bail(rc);
-    const size_t propbytes = (size_t)2 * c->D * c->L * c->L * c->esize;
+    const size_t propbytes = (size_t)2 * c->D * c->L * c->L * c->pesize;
     if (hipMalloc(&c->props, propbytes) != hipSuccess)
+        return bail(fail(QJX_ERR_NOMEM, "propagator tree does not fit in device memory"));
+    if (c->prec == QJX_BF16X3 && hipMalloc(&c->props_lo, propbytes) != hipSuccess)
         return bail(fail(QJX_ERR_NOMEM, "propagator tree does not fit in device memory"));
 
     // Propagator tree (qj_proc.cpp:4-120): U[b][i] = exp(Hqj[b] * dt_i).
@@ -328,7 +343,9 @@ extern "C" int qjx_ctx_create(const qjx_model_desc *m, qjx_ctx **out)
         }
         for (int i = 0; i < c->D && rc == QJX_OK; i++) {
             rc = device_expm(c, dA, a1, c->dt[i], U, work);
-            if (rc == QJX_OK) qjx::launch_pack_propagator(c->prec, c->d, c->dp, U, prop_ptr(c, b, i), nullptr, c->stream);
+            if (rc == QJX_OK)
+                qjx::launch_pack_propagator(c->prec, c->d, c->dp, U, prop_ptr(c, b, i), prop_lo_ptr(c, b, i),
+                                            c->stream);
         }
     }
     if (rc == QJX_OK && hipStreamSynchronize(c->stream) != hipSuccess)
@@ -377,6 +394,7 @@ extern "C" void qjx_ctx_destroy(qjx_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_ensemble(c);
     if (c->props) (void)hipFree(c->props);
+    if (c->props_lo) (void)hipFree(c->props_lo);
     dev_free(c->d_dt);
     dev_free(c->d_Ldiag);
     dev_free(c->d_Loff);
@@ -459,7 +477,7 @@ static qjx::StepArgs step_args(qjx_ctx *c, int p, int is_obs)
 {
     qjx::StepArgs a{};
     a.props = c->props;
-    a.props_lo = nullptr;
+    a.props_lo = c->props_lo;
     a.slot[0] = c->slot[0];
     a.slot[1] = c->slot[1];
     a.st = c->st;
@@ -489,7 +507,8 @@ static qjx::StepArgs step_args(qjx_ctx *c, int p, int is_obs)
     a.R = c->R;
     a.is_obs = is_obs;
     a.jump = c->ens.jump;
-    a.norm_tol = c->prec == QJX_FP64 ? 0.0 : 1e-4;
+    // reduced-precision forms: ||psi||^2 may exceed 1 by the GEMM's rounding
+    a.norm_tol = c->prec == QJX_FP64 ? 0.0 : c->prec == QJX_FP32 ? 1e-4 : 1e-3;
     return a;
 }
 
@@ -706,23 +725,31 @@ extern "C" int qjx_get_propagator(qjx_ctx *c, int b, int lvl, double *U)
 {
     if (!c || !U || b < 0 || b > 1 || lvl < 0 || lvl >= c->D) return fail(QJX_ERR_ARG, "bad propagator index");
     const size_t LL = (size_t)c->L * c->L;
-    std::vector<char> raw(LL * c->esize);
+    std::vector<char> raw(LL * c->pesize), rawlo;
     HIP_TRY(hipMemcpy(raw.data(), prop_ptr(c, b, lvl), raw.size(), hipMemcpyDeviceToHost));
+    if (c->props_lo) {
+        rawlo.resize(raw.size());
+        HIP_TRY(hipMemcpy(rawlo.data(), prop_lo_ptr(c, b, lvl), rawlo.size(), hipMemcpyDeviceToHost));
+    }
+    auto bf = [](const char *p, size_t o) -> double {
+        uint16_t h;
+        std::memcpy(&h, p + 2 * o, 2);
+        const uint32_t u = (uint32_t)h << 16;
+        float f;
+        std::memcpy(&f, &u, 4);
+        return f;
+    };
+    auto el = [&](size_t o) -> double {
+        if (c->pesize == 8) return ((const double *)raw.data())[o];
+        if (c->pesize == 4) return ((const float *)raw.data())[o];
+        return bf(raw.data(), o) + bf(rawlo.data(), o);
+    };
     const int d = c->d, dp = c->dp, L = c->L;
     for (int i = 0; i < d; i++)
         for (int k = 0; k < d; k++) {
             // Re U_ik at (row i, col k), Im U_ik at (row dp + i, col k); row-major
-            const size_t ore = (size_t)i * L + k, oim = (size_t)(dp + i) * L + k;
-            double re, im;
-            if (c->esize == 8) {
-                re = ((const double *)raw.data())[ore];
-                im = ((const double *)raw.data())[oim];
-            } else {
-                re = ((const float *)raw.data())[ore];
-                im = ((const float *)raw.data())[oim];
-            }
-            U[2 * ((size_t)i * d + k)] = re;
-            U[2 * ((size_t)i * d + k) + 1] = im;
+            U[2 * ((size_t)i * d + k)] = el((size_t)i * L + k);
+            U[2 * ((size_t)i * d + k) + 1] = el((size_t)(dp + i) * L + k);
         }
     return QJX_OK;
 }
@@ -735,7 +762,8 @@ extern "C" int qjx_set_propagator(qjx_ctx *c, int b, int lvl, const double *U)
     HIP_TRY(hipMalloc((void **)&dU, nn2 * sizeof(double)));
     hipError_t e = hipMemcpy(dU, U, nn2 * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-        qjx::launch_pack_propagator(c->prec, c->d, c->dp, dU, prop_ptr(c, b, lvl), nullptr, c->stream);
+        qjx::launch_pack_propagator(c->prec, c->d, c->dp, dU, prop_ptr(c, b, lvl), prop_lo_ptr(c, b, lvl),
+                                    c->stream);
         e = hipStreamSynchronize(c->stream);
     }
     (void)hipFree(dU);
@@ -759,31 +787,32 @@ extern "C" int qjx_get_stats(qjx_ctx *c, qjx_stats *out)
 }
 
 extern "C" int qjx_mfma_probe(const float *A32, const float *B32, float *D32, const double *A64,
-                              const double *B64, double *D64)
+                              const double *B64, double *D64, const float *Abf, const float *Bbf, float *Dbf)
 {
-    float *a = nullptr, *b = nullptr, *dd = nullptr;
-    double *a6 = nullptr, *b6 = nullptr, *d6 = nullptr;
-    HIP_TRY(hipMalloc((void **)&a, 64 * 4));
-    HIP_TRY(hipMalloc((void **)&b, 64 * 4));
-    HIP_TRY(hipMalloc((void **)&dd, 1024 * 4));
-    HIP_TRY(hipMalloc((void **)&a6, 64 * 8));
-    HIP_TRY(hipMalloc((void **)&b6, 64 * 8));
-    HIP_TRY(hipMalloc((void **)&d6, 256 * 8));
-    HIP_TRY(hipMemcpy(a, A32, 64 * 4, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(b, B32, 64 * 4, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(a6, A64, 64 * 8, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(b6, B64, 64 * 8, hipMemcpyHostToDevice));
-    qjx::launch_mfma_probe(a, b, dd, a6, b6, d6, nullptr);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipDeviceSynchronize());
-    HIP_TRY(hipMemcpy(D32, dd, 1024 * 4, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(D64, d6, 256 * 8, hipMemcpyDeviceToHost));
-    dev_free(a);
-    dev_free(b);
-    dev_free(dd);
-    dev_free(a6);
-    dev_free(b6);
-    dev_free(d6);
+    // one buffer: A32 64 | B32 64 | D32 1024 | Abf 512 | Bbf 512 | Dbf 1024 floats, then
+    // A64 64 | B64 64 | D64 256 doubles
+    const size_t nf = 64 + 64 + 1024 + 512 + 512 + 1024, nd = 64 + 64 + 256;
+    char *buf = nullptr;
+    HIP_TRY(hipMalloc((void **)&buf, nf * 4 + nd * 8));
+    float *f = (float *)buf;
+    double *dd = (double *)(buf + nf * 4);
+    hipError_t e = hipSuccess;
+    e = e == hipSuccess ? hipMemcpy(f, A32, 64 * 4, hipMemcpyHostToDevice) : e;
+    e = e == hipSuccess ? hipMemcpy(f + 64, B32, 64 * 4, hipMemcpyHostToDevice) : e;
+    e = e == hipSuccess ? hipMemcpy(f + 1152, Abf, 512 * 4, hipMemcpyHostToDevice) : e;
+    e = e == hipSuccess ? hipMemcpy(f + 1664, Bbf, 512 * 4, hipMemcpyHostToDevice) : e;
+    e = e == hipSuccess ? hipMemcpy(dd, A64, 64 * 8, hipMemcpyHostToDevice) : e;
+    e = e == hipSuccess ? hipMemcpy(dd + 64, B64, 64 * 8, hipMemcpyHostToDevice) : e;
+    if (e == hipSuccess) {
+        qjx::launch_mfma_probe(f, f + 64, f + 128, dd, dd + 64, dd + 128, f + 1152, f + 1664, f + 2176, nullptr);
+        e = hipGetLastError();
+    }
+    e = e == hipSuccess ? hipDeviceSynchronize() : e;
+    e = e == hipSuccess ? hipMemcpy(D32, f + 128, 1024 * 4, hipMemcpyDeviceToHost) : e;
+    e = e == hipSuccess ? hipMemcpy(Dbf, f + 2176, 1024 * 4, hipMemcpyDeviceToHost) : e;
+    e = e == hipSuccess ? hipMemcpy(D64, dd + 128, 256 * 8, hipMemcpyDeviceToHost) : e;
+    (void)hipFree(buf);
+    if (e != hipSuccess) return fail(QJX_ERR_HIP, hipGetErrorString(e));
     return QJX_OK;
 }
 

@@ -31,9 +31,11 @@ void launch_reset_logn(TrajState *st, int M, hipStream_t s);
 void launch_step_gemm(int prec, int gemm_kind, const StepArgs &a, hipStream_t s);
 void launch_step_decide(int prec, const StepArgs &a, uint64_t stride, hipStream_t s);
 void launch_step_gemm_mfma(int prec, const StepArgs &a, hipStream_t s);
-// one wave: D32 = A32 (32x2) B32 (2x32), D64 = A64 (16x4) B64 (4x16), row-major
+// one wave: D32 = A32 (32x2) B32 (2x32), D64 = A64 (16x4) B64 (4x16),
+// Dbf = Abf (32x16) Bbf (16x32) through bf16 operands, all row-major
 void launch_mfma_probe(const float *A32, const float *B32, float *D32, const double *A64,
-                       const double *B64, double *D64, hipStream_t s);
+                       const double *B64, double *D64, const float *Abf, const float *Bbf, float *Dbf,
+                       hipStream_t s);
 
 // Observables of every trajectory (core.cpp:665-725): norm, mean, m2, energy;
 // dispersion against mean_start; optional |psi_k|^2 / norm per trajectory.

@@ -819,11 +819,35 @@ __global__ void pack_propagator_kernel(int n, int dp, const double *U, T *dst)
     }
 }
 
+// 3 x BF16 form: hi = bf16(u), lo = bf16(u - hi) of the FP32-rounded element.
+__global__ void pack_propagator_bf16x3_kernel(int n, int dp, const double *U, __bf16 *hi, __bf16 *lo)
+{
+    const int L = 2 * dp;
+    const size_t total = (size_t)L * L;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(e / L), k = (int)(e % L);
+        const int ib = i / dp, ii = i % dp, kb = k / dp, kk = k % dp;
+        double v = 0.0;
+        if (ii < n && kk < n) {
+            const double re = U[2 * ((size_t)ii * n + kk)], im = U[2 * ((size_t)ii * n + kk) + 1];
+            if (ib == kb) v = re;
+            else if (ib == 0) v = -im;
+            else v = im;
+        }
+        const float x = (float)v;
+        const __bf16 h = (__bf16)x;
+        hi[e] = h;
+        lo[e] = (__bf16)(x - (float)h);
+    }
+}
+
 void launch_pack_propagator(int prec, int n, int dp, const double *U, void *dst, void *dst_lo,
                             hipStream_t s)
 {
-    (void)dst_lo;
     if (prec == kFP64) pack_propagator_kernel<double><<<2048, 256, 0, s>>>(n, dp, U, (double *)dst);
+    else if (prec == kBF16x3)
+        pack_propagator_bf16x3_kernel<<<2048, 256, 0, s>>>(n, dp, U, (__bf16 *)dst, (__bf16 *)dst_lo);
     else pack_propagator_kernel<float><<<2048, 256, 0, s>>>(n, dp, U, (float *)dst);
 }
 

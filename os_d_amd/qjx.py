@@ -76,7 +76,7 @@ SIGNATURES = [
     ("qjx_dim", ctypes.c_int, [ctypes.c_void_p]),
     ("qjx_set_profiling", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("qjx_mfma_probe", ctypes.c_int, [_P(ctypes.c_float), _P(ctypes.c_float), _P(ctypes.c_float), _D, _D,
-                                      _D]),
+                                      _D, _P(ctypes.c_float), _P(ctypes.c_float), _P(ctypes.c_float)]),
     ("qjx_run_config", ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]),
 ]
 
@@ -211,16 +211,19 @@ class Engine:
         _check(self.lib, self.lib.qjx_set_profiling(self.h, 1 if on else 0))
 
 
-def mfma_probe(A32, B32, A64, B64):
-    """Run the matrix-core layout probe; returns (D32, D64)."""
+def mfma_probe(A32, B32, A64, B64, Abf, Bbf):
+    """Run the matrix-core layout probe; returns (D32, D64, Dbf)."""
     lib = load_library()
-    A32, B32 = (np.ascontiguousarray(x, dtype=np.float32) for x in (A32, B32))
+    A32, B32, Abf, Bbf = (np.ascontiguousarray(x, dtype=np.float32) for x in (A32, B32, Abf, Bbf))
     A64, B64 = (np.ascontiguousarray(x, dtype=np.float64) for x in (A64, B64))
     D32 = np.zeros((32, 32), dtype=np.float32)
     D64 = np.zeros((16, 16), dtype=np.float64)
+    Dbf = np.zeros((32, 32), dtype=np.float32)
     _check(lib, lib.qjx_mfma_probe(_ptr(A32, ctypes.c_float), _ptr(B32, ctypes.c_float),
-                                   _ptr(D32, ctypes.c_float), _ptr(A64), _ptr(B64), _ptr(D64)))
-    return D32, D64
+                                   _ptr(D32, ctypes.c_float), _ptr(A64), _ptr(B64), _ptr(D64),
+                                   _ptr(Abf, ctypes.c_float), _ptr(Bbf, ctypes.c_float),
+                                   _ptr(Dbf, ctypes.c_float)))
+    return D32, D64, Dbf
 
 
 def run_config(config_path, params_path, precision=FP64, steps_per_period=0):

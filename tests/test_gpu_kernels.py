@@ -22,9 +22,13 @@ def test_mfma_layout_probe():
     B32 = rng.standard_normal((2, 32)).astype(np.float32)
     A64 = rng.standard_normal((16, 4))
     B64 = rng.standard_normal((4, 16))
-    D32, D64 = qjx.mfma_probe(A32, B32, A64, B64)
+    # small integers are exact in bf16, so the bf16 product is exact in FP32
+    Abf = rng.integers(-8, 9, (32, 16)).astype(np.float32)
+    Bbf = rng.integers(-8, 9, (16, 32)).astype(np.float32)
+    D32, D64, Dbf = qjx.mfma_probe(A32, B32, A64, B64, Abf, Bbf)
     np.testing.assert_allclose(D32, A32.astype(np.float64) @ B32.astype(np.float64), rtol=1e-5, atol=1e-5)
     np.testing.assert_allclose(D64, A64 @ B64, rtol=1e-13, atol=1e-13)
+    np.testing.assert_array_equal(Dbf, Abf @ Bbf)
 
 
 @pytest.mark.parametrize("prec", [0, 1])

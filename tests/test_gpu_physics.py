@@ -18,7 +18,7 @@ def _say(msg):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("prec", [0, 1, 2])
 def test_gpu_ensemble_matches_lindblad(oracle_mod, prec):
     from os_d_amd import qjx
     N, M, P, S = 20, 20000, 3, 8
