@@ -61,6 +61,21 @@ def run_form(qjx, prec, N, M, S, D, warmup, steps):
     return e, res
 
 
+def measured_traffic_per_column():
+    """HBM bytes per propagator application of the FP32 step GEMM, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy kernel)."""
+    path = os.path.join(REPO, "profiles", "step_gemm_fp32_traffic_dimer_d1001_M1e5.md")
+    try:
+        vals = {}
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_column_"):
+                k, v = line.split(":")
+                vals[k.strip()] = float(v)
+        return vals["traffic_bytes_per_column_fetch"] + vals["traffic_bytes_per_column_write"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(engine, N, S, D, budget_s):
     """FP64 oracle on the host cores, bounded sample, same propagator tree."""
     from oracle import oracle as orc
@@ -146,6 +161,9 @@ def main():
 
     launches = max(1, r32["gemm_launches"])
     avg_ms = r32["gemm_ms"] / launches
+    per_col = measured_traffic_per_column()
+    cols_per_launch = r32["matvecs"] / launches
+    traffic = per_col * cols_per_launch if (per_col and N == 1000) else None
     achieved = r32["gemm_flops"] / (r32["gemm_ms"] * 1e-3) / 1e12 if r32["gemm_ms"] > 0 else 0.0
     line = {
         "metric": METRIC,
@@ -165,7 +183,10 @@ def main():
                    "trajectories": M, "dim": N + 1, "steps_per_period": S, "ex_deep": D,
                    "step": "one top-level step T/S of the whole ensemble incl. refinement"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK["fp32"], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK["fp32"], "traffic": None,
+                     "frac": achieved / PEAK["fp32"], "traffic": traffic,
+                     "traffic_source": "profiles/step_gemm_fp32_traffic_dimer_d1001_M1e5.md (bytes/column x "
+                                       "columns/launch)",
+                     "algorithmic_bytes_per_launch": 2 * 4 * 2 * 1024 * cols_per_launch if N == 1000 else None,
                      "kernel": "step GEMM (grouped, gathered columns)",
                      "avg_launch_ms": avg_ms,
                      "flops_per_launch": r32["gemm_flops"] / launches},
