@@ -144,6 +144,45 @@ int qjx_mfma_probe(const float *A32, const float *B32, float *D32, const double 
 /* Hilbert dimension of the context (N + 1). */
 int qjx_dim(const qjx_ctx *ctx);
 
+/* ------------------------------------------------------------------------ */
+/* Job pool: continuous batching of ensemble jobs (parameter points) on one  */
+/* GPU.  The reference runs one ensemble per process and sweeps parameters   */
+/* as separate jobs (submit/qjx/submit_dimer.py:128-231); here running jobs  */
+/* share every propagation launch whatever their Hilbert dimension, queued   */
+/* jobs are admitted while their HBM footprint fits the budget, and a job's  */
+/* results do not depend on which other jobs share the launches.             */
+/* ------------------------------------------------------------------------ */
+typedef struct qjx_pool qjx_pool;
+
+typedef struct qjx_job_result {
+    int status;             /* 0 queued, 1 running, 2 done, < 0 error code          */
+    int num_trajectories;
+    int n_periods;
+    int dim;
+    double t_submit;        /* seconds on the pool clock                            */
+    double t_admit;
+    double t_done;
+    int64_t matvecs;        /* propagator applications of this job                  */
+    int64_t jumps;
+    double mean_avg, mean_std;      /* ensemble mean / std of <k> (= <n2>) at the end */
+    double energy_avg, energy_std;  /* ... of the energy                             */
+    double norm_avg;
+} qjx_job_result;
+
+/* precision applies to every job; hbm_budget_gib <= 0 means 80% of free HBM;
+ * max_trajectories bounds the trajectories of all running jobs together. */
+int qjx_pool_create(int precision, double hbm_budget_gib, int max_trajectories, qjx_pool **out);
+void qjx_pool_destroy(qjx_pool *pool);
+/* Queue a job: n_periods periods after eta_0 is drawn (no transient). */
+int qjx_pool_submit(qjx_pool *pool, const qjx_model_desc *model, const qjx_ensemble_desc *ens,
+                    int n_periods, int *job_id);
+/* One scheduling cycle: a chunk of iterations over the running jobs, admission
+ * of queued jobs while it runs, retirement of finished jobs. */
+int qjx_pool_step(qjx_pool *pool, int *running, int *queued);
+/* Status and results; mean / energy (num_trajectories doubles) may be NULL. */
+int qjx_pool_result(qjx_pool *pool, int job_id, qjx_job_result *out, double *mean, double *energy);
+double qjx_pool_clock(const qjx_pool *pool);
+
 /* Drop-in runner: read config.txt / params.txt (reference keys, config.cpp:29-145),
  * run the STD experiment and write the reference's output files into `path`
  * (core.cpp:918-1084, dump.cpp:26-61).  precision selects the GEMM form;

@@ -1,0 +1,65 @@
+// qjx_ctx.h -- the engine context (one dimer model + its ensemble), shared by the
+// engine and the job pool.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/qjx.h"
+#include "qjx_internal.h"
+#include "qjx_runner.h"
+
+struct qjx_ctx {
+    qjx_model_desc model{};
+    int d = 0, dp = 0, L = 0, D = 0, S = 0, G = 0, R = 0;
+    int prec = 0;
+    int gemm_kind = 0;
+    size_t esize = 8;              // state element bytes
+    size_t pesize = 8;             // propagator element bytes (per half for BF16x3)
+    double T = 0.0;
+    std::vector<double> dt;
+    hipStream_t stream = nullptr;
+
+    // model (host)
+    std::vector<double> Hdiag, Hoff, Hdrv, Ldiag, Loff, HdiagE;
+    std::vector<double> Hqj[2];    // dense interleaved complex d x d
+
+    // device model
+    void *props = nullptr;         // 2*D * L*L elements
+    void *props_lo = nullptr;      // BF16x3: low halves
+    double *d_dt = nullptr, *d_Ldiag = nullptr, *d_Loff = nullptr;
+    double *d_HdiagE = nullptr, *d_Hoff = nullptr;
+
+    // ensemble
+    int M = 0;
+    qjx_ensemble_desc ens{};
+    uint64_t stride = 0;
+    void *slot[2] = {nullptr, nullptr};
+    qjx::TrajState *st = nullptr;
+    int *lists[2] = {nullptr, nullptr};
+    int *cnts[2] = {nullptr, nullptr};
+    double *normpart = nullptr;
+    int *err = nullptr;
+    unsigned long long *counters = nullptr;
+    qjx::JumpLog log{};
+    double *mean_start = nullptr, *obs = nullptr, *adr = nullptr, *adr_avg = nullptr;
+    double *xfer = nullptr;        // export/import buffer (M x d complex)
+    bool have_ens = false;
+    qjx::Ens desc{};               // device descriptor of this ensemble
+    qjx::Runner *runner = nullptr; // drives iterations over {this ensemble}
+
+    // host jump logs
+    std::vector<std::vector<double>> jt, jn, je;
+    std::vector<std::vector<int>> jty;
+
+    bool profiling = false;        // per-kernel timing of the iterations
+};
+
+namespace qjx {
+// Internal entry points used by the job pool.
+int ctx_create_kind(const qjx_model_desc *m, int gemm_kind, qjx_ctx **out);
+size_t ensemble_footprint(const qjx_model_desc *m, int M, int gemm_kind);
+size_t model_footprint(const qjx_model_desc *m, int gemm_kind);
+int ctx_seed(qjx_ctx *c, int n_steps, int is_obs);   // draw-free seeding of a run on c->stream
+}  // namespace qjx

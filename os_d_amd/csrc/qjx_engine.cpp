@@ -16,6 +16,7 @@
 #include "../../include/qjx.h"
 #include "qjx_internal.h"
 #include "qjx_kernels.h"
+#include "qjx_runner.h"
 
 namespace {
 
@@ -59,59 +60,7 @@ void dev_free(T *&p)
 
 }  // namespace
 
-struct qjx_ctx {
-    qjx_model_desc model{};
-    int d = 0, dp = 0, L = 0, D = 0, S = 0, G = 0, R = 0;
-    int prec = 0;
-    int gemm_kind = 0;
-    size_t esize = 8;              // state element bytes
-    size_t pesize = 8;             // propagator element bytes (per half for BF16x3)
-    double T = 0.0;
-    std::vector<double> dt;
-    hipStream_t stream = nullptr;
-
-    // model (host)
-    std::vector<double> Hdiag, Hoff, Hdrv, Ldiag, Loff, HdiagE;
-    std::vector<double> Hqj[2];    // dense interleaved complex d x d
-
-    // device model
-    void *props = nullptr;         // 2*D * L*L elements
-    void *props_lo = nullptr;      // BF16x3: low halves
-    double *d_dt = nullptr, *d_Ldiag = nullptr, *d_Loff = nullptr;
-    double *d_HdiagE = nullptr, *d_Hoff = nullptr;
-
-    // ensemble
-    int M = 0;
-    qjx_ensemble_desc ens{};
-    uint64_t stride = 0;
-    void *slot[2] = {nullptr, nullptr};
-    qjx::TrajState *st = nullptr;
-    int *lists[2] = {nullptr, nullptr};
-    int *cnts[2] = {nullptr, nullptr};
-    double *normpart = nullptr;
-    int *jlist = nullptr, *jsrc = nullptr, *jcnt = nullptr;
-    double *jnorm = nullptr;
-    int *err = nullptr;
-    unsigned long long *counters = nullptr;
-    qjx::JumpLog log{};
-    double *mean_start = nullptr, *obs = nullptr, *adr = nullptr, *adr_avg = nullptr;
-    double *xfer = nullptr;        // export/import buffer (M x d complex)
-    int *h_cnt = nullptr;          // pinned
-    bool have_ens = false;
-    int parity = 0;
-
-    // host jump logs
-    std::vector<std::vector<double>> jt, jn, je;
-    std::vector<std::vector<int>> jty;
-
-    int64_t iterations = 0;
-
-    // optional per-launch timing of the step kernels (hipEvents on the engine stream)
-    bool profiling = false;
-    std::vector<hipEvent_t> ev;    // 3 per iteration of a chunk
-    double gemm_ms = 0.0, decide_ms = 0.0;
-    int64_t gemm_launches = 0;
-};
+#include "qjx_ctx.h"
 
 // ---------------------------------------------------------------------------
 // Model (newdel.cpp:3-37, 450-498, 1454-1498, 2285-2388)
@@ -259,7 +208,19 @@ static int upload(void **dst, const void *src, size_t bytes)
     return QJX_OK;
 }
 
+static int env_gemm_kind()
+{
+    if (const char *env = std::getenv("QJX_GEMM"))
+        if (std::string(env) == "fma") return qjx::kGemmFma;
+    return qjx::kGemmMfma;
+}
+
 extern "C" int qjx_ctx_create(const qjx_model_desc *m, qjx_ctx **out)
+{
+    return qjx::ctx_create_kind(m, env_gemm_kind(), out);
+}
+
+int qjx::ctx_create_kind(const qjx_model_desc *m, int gemm_kind, qjx_ctx **out)
 {
     if (!m || !out) return fail(QJX_ERR_ARG, "null argument");
     *out = nullptr;
@@ -284,9 +245,7 @@ extern "C" int qjx_ctx_create(const qjx_model_desc *m, qjx_ctx **out)
     c->D = m->ex_deep;
     c->S = S;
     c->G = 2 * c->D;
-    c->gemm_kind = qjx::kGemmMfma;
-    if (const char *env = std::getenv("QJX_GEMM"))
-        if (std::string(env) == "fma") c->gemm_kind = qjx::kGemmFma;
+    c->gemm_kind = gemm_kind;
     c->R = c->L / qjx::gemm_bm(c->gemm_kind);
     c->prec = m->precision;
     c->esize = c->prec == QJX_FP64 ? 8 : 4;
@@ -368,10 +327,6 @@ static void free_ensemble(qjx_ctx *c)
     }
     dev_free(c->st);
     dev_free(c->normpart);
-    dev_free(c->jlist);
-    dev_free(c->jsrc);
-    dev_free(c->jcnt);
-    dev_free(c->jnorm);
     dev_free(c->err);
     dev_free(c->counters);
     dev_free(c->log.time);
@@ -383,8 +338,8 @@ static void free_ensemble(qjx_ctx *c)
     dev_free(c->adr);
     dev_free(c->adr_avg);
     dev_free(c->xfer);
-    if (c->h_cnt) (void)hipHostFree(c->h_cnt);
-    c->h_cnt = nullptr;
+    if (c->runner) qjx::runner_destroy(c->runner);
+    c->runner = nullptr;
     c->have_ens = false;
 }
 
@@ -400,7 +355,6 @@ extern "C" void qjx_ctx_destroy(qjx_ctx *c)
     dev_free(c->d_Loff);
     dev_free(c->d_HdiagE);
     dev_free(c->d_Hoff);
-    for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -419,6 +373,50 @@ static int alloc_log(qjx_ctx *c, int cap)
     HIP_TRY(hipMalloc((void **)&c->log.eta, n * sizeof(double)));
     HIP_TRY(hipMalloc((void **)&c->log.type, n * sizeof(int)));
     c->log.cap = cap;
+    return QJX_OK;
+}
+
+static void build_desc(qjx_ctx *c)
+{
+    qjx::Ens &E = c->desc;
+    E.props = c->props;
+    E.props_lo = c->props_lo;
+    for (int i = 0; i < 2; i++) {
+        E.slot[i] = c->slot[i];
+        E.list[i] = c->lists[i];
+        E.cnt[i] = c->cnts[i];
+    }
+    E.st = c->st;
+    E.normpart = c->normpart;
+    E.dt = c->d_dt;
+    E.Ldiag = c->d_Ldiag;
+    E.Loff = c->d_Loff;
+    E.log = c->log;
+    E.err = c->err;
+    E.counters = c->counters;
+    E.stride = c->stride;
+    // reduced-precision forms: ||psi||^2 may exceed 1 by the GEMM's rounding
+    E.norm_tol = c->prec == QJX_FP64 ? 0.0 : c->prec == QJX_FP32 ? 1e-4 : 1e-3;
+    E.d = c->d;
+    E.dp = c->dp;
+    E.L = c->L;
+    E.D = c->D;
+    E.S = c->S;
+    E.M = c->M;
+    E.G = c->G;
+    E.R = c->R;
+    E.is_obs = 0;
+    E.jump = c->ens.jump;
+}
+
+static int ensure_runner(qjx_ctx *c)
+{
+    if (c->runner) return QJX_OK;
+    int rc = qjx::runner_create(c->prec, c->gemm_kind, c->stream, c->M, &c->runner);
+    if (rc) return rc;
+    c->runner->profiling = c->profiling;
+    if (const char *env = std::getenv("QJX_GRAPH"))
+        if (std::string(env) == "0") c->runner->graphs = false;
     return QJX_OK;
 }
 
@@ -444,11 +442,6 @@ extern "C" int qjx_ensemble_init(qjx_ctx *c, const qjx_ensemble_desc *e)
         HIP_TRY(hipMemset(c->cnts[i], 0, qjx::kMaxGroups * sizeof(int)));
     }
     HIP_TRY(hipMalloc((void **)&c->normpart, (size_t)c->R * M * sizeof(double)));
-    HIP_TRY(hipMalloc((void **)&c->jlist, M * sizeof(int)));
-    HIP_TRY(hipMalloc((void **)&c->jsrc, M * sizeof(int)));
-    HIP_TRY(hipMalloc((void **)&c->jnorm, M * sizeof(double)));
-    HIP_TRY(hipMalloc((void **)&c->jcnt, sizeof(int)));
-    HIP_TRY(hipMemset(c->jcnt, 0, sizeof(int)));
     HIP_TRY(hipMalloc((void **)&c->err, 4 * sizeof(int)));
     HIP_TRY(hipMemset(c->err, 0, 4 * sizeof(int)));
     HIP_TRY(hipMalloc((void **)&c->counters, 2 * sizeof(unsigned long long)));
@@ -456,7 +449,6 @@ extern "C" int qjx_ensemble_init(qjx_ctx *c, const qjx_ensemble_desc *e)
     HIP_TRY(hipMalloc((void **)&c->mean_start, M * sizeof(double)));
     HIP_TRY(hipMemset(c->mean_start, 0, M * sizeof(double)));
     HIP_TRY(hipMalloc((void **)&c->obs, 5 * M * sizeof(double)));
-    HIP_TRY(hipHostMalloc((void **)&c->h_cnt, qjx::kMaxGroups * sizeof(int), hipHostMallocDefault));
     const int cap = e->log_capacity > 0 ? e->log_capacity : 256;
     int rc = alloc_log(c, cap);
     if (rc) return rc;
@@ -468,48 +460,9 @@ extern "C" int qjx_ensemble_init(qjx_ctx *c, const qjx_ensemble_desc *e)
                               e->seed, e->start_type, e->start_state, c->stream);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
-    c->parity = 0;
+    build_desc(c);
     c->have_ens = true;
     return QJX_OK;
-}
-
-static qjx::StepArgs step_args(qjx_ctx *c, int p, int is_obs)
-{
-    qjx::StepArgs a{};
-    a.props = c->props;
-    a.props_lo = c->props_lo;
-    a.slot[0] = c->slot[0];
-    a.slot[1] = c->slot[1];
-    a.st = c->st;
-    a.list = c->lists[p];
-    a.cnt = c->cnts[p];
-    a.list_next = c->lists[p ^ 1];
-    a.cnt_next = c->cnts[p ^ 1];
-    a.cnt_clear = c->cnts[p ^ 1];
-    a.normpart = c->normpart;
-    a.jlist = c->jlist;
-    a.jsrc = c->jsrc;
-    a.jnorm = c->jnorm;
-    a.jcnt = c->jcnt;
-    a.dt = c->d_dt;
-    a.Ldiag = c->d_Ldiag;
-    a.Loff = c->d_Loff;
-    a.log = c->log;
-    a.err = c->err;
-    a.counters = c->counters;
-    a.d = c->d;
-    a.dp = c->dp;
-    a.L = c->L;
-    a.D = c->D;
-    a.S = c->S;
-    a.M = c->M;
-    a.G = c->G;
-    a.R = c->R;
-    a.is_obs = is_obs;
-    a.jump = c->ens.jump;
-    // reduced-precision forms: ||psi||^2 may exceed 1 by the GEMM's rounding
-    a.norm_tol = c->prec == QJX_FP64 ? 0.0 : c->prec == QJX_FP32 ? 1e-4 : 1e-3;
-    return a;
 }
 
 static int check_err(qjx_ctx *c)
@@ -561,50 +514,20 @@ static int drain_log(qjx_ctx *c)
     return QJX_OK;
 }
 
-// Advance every trajectory by n top-level steps: seed the lists, then launch
-// iterations in chunks until no trajectory has steps left.
+// Advance every trajectory by n top-level steps: seed the lists, then run
+// iteration chunks until no trajectory has steps left.
 static int run_steps(qjx_ctx *c, int n, int is_obs)
 {
     if (n <= 0) return QJX_OK;
-    c->parity = 0;
-    qjx::launch_seed_lists(c->st, c->M, n, c->lists[0], c->cnts[0], c->G, c->stream);
+    int rc = ensure_runner(c);
+    if (rc) return rc;
+    qjx::launch_seed(c->st, c->M, n, c->lists[0], c->cnts[0], c->D, c->stream);
     HIP_TRY(hipGetLastError());
-    int chunk = 8;
-    const int max_chunk = 256;
-    if (c->profiling && c->ev.size() < (size_t)3 * max_chunk) {
-        c->ev.resize(3 * max_chunk);
-        for (auto &e : c->ev) HIP_TRY(hipEventCreate(&e));
-    }
-    for (;;) {
-        for (int k = 0; k < chunk; k++) {
-            qjx::StepArgs a = step_args(c, c->parity, is_obs);
-            if (c->profiling) HIP_TRY(hipEventRecord(c->ev[3 * k], c->stream));
-            qjx::launch_step_gemm(c->prec, c->gemm_kind, a, c->stream);
-            if (c->profiling) HIP_TRY(hipEventRecord(c->ev[3 * k + 1], c->stream));
-            qjx::launch_step_decide(c->prec, a, c->stride, c->stream);
-            if (c->profiling) HIP_TRY(hipEventRecord(c->ev[3 * k + 2], c->stream));
-            c->parity ^= 1;
-            c->iterations++;
-        }
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(c->h_cnt, c->cnts[c->parity], c->G * sizeof(int), hipMemcpyDeviceToHost,
-                               c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-        if (c->profiling) {
-            for (int k = 0; k < chunk; k++) {
-                float g = 0.f, dcd = 0.f;
-                HIP_TRY(hipEventElapsedTime(&g, c->ev[3 * k], c->ev[3 * k + 1]));
-                HIP_TRY(hipEventElapsedTime(&dcd, c->ev[3 * k + 1], c->ev[3 * k + 2]));
-                c->gemm_ms += g;
-                c->decide_ms += dcd;
-            }
-            c->gemm_launches += chunk;
-        }
-        long long active = 0;
-        for (int g = 0; g < c->G; g++) active += c->h_cnt[g];
-        if (active == 0) break;
-        chunk = std::min(chunk * 2, max_chunk);
-    }
+    c->desc.is_obs = is_obs;
+    rc = qjx::runner_set_table(c->runner, &c->desc, 1);
+    if (rc) return rc;
+    rc = qjx::runner_run(c->runner, nullptr);
+    if (rc) return rc;
     return check_err(c);
 }
 
@@ -777,12 +700,12 @@ extern "C" int qjx_get_stats(qjx_ctx *c, qjx_stats *out)
     unsigned long long h[2] = {0, 0};
     if (c->counters) HIP_TRY(hipMemcpy(h, c->counters, sizeof h, hipMemcpyDeviceToHost));
     out->matvecs = (int64_t)h[0];
-    out->iterations = c->iterations;
+    out->iterations = c->runner ? c->runner->iterations : 0;
     out->jumps = (int64_t)h[1];
     out->gemm_flops = 8.0 * double(c->d) * double(c->d) * double(out->matvecs);
-    out->gemm_ms = c->gemm_ms;
-    out->decide_ms = c->decide_ms;
-    out->gemm_launches = c->gemm_launches;
+    out->gemm_ms = c->runner ? c->runner->gemm_ms : 0.0;
+    out->decide_ms = c->runner ? c->runner->tail_ms + c->runner->plan_ms : 0.0;
+    out->gemm_launches = c->runner ? c->runner->gemm_launches : 0;
     return QJX_OK;
 }
 
@@ -820,7 +743,209 @@ extern "C" int qjx_set_profiling(qjx_ctx *c, int on)
 {
     if (!c) return fail(QJX_ERR_ARG, "null context");
     c->profiling = on != 0;
-    c->gemm_ms = c->decide_ms = 0.0;
-    c->gemm_launches = 0;
+    if (c->runner) {
+        c->runner->profiling = c->profiling;
+        c->runner->plan_ms = c->runner->gemm_ms = c->runner->tail_ms = 0.0;
+        c->runner->gemm_launches = 0;
+    }
     return QJX_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Runner: iterations over a table of ensembles
+// ---------------------------------------------------------------------------
+namespace qjx {
+
+int runner_create(int prec, int gemm_kind, hipStream_t stream, int capacity, Runner **out)
+{
+    *out = nullptr;
+    Runner *r = new Runner;
+    r->prec = prec;
+    r->gemm_kind = gemm_kind;
+    r->stream = stream;
+    r->capacity = capacity;
+    auto bail = [&](int code) {
+        runner_destroy(r);
+        return code;
+    };
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return bail(fail(QJX_ERR_HIP, "device query failed"));
+    int g = cus * gemm_blocks_per_cu(prec, gemm_kind);
+    g = std::max(8, g / 8 * 8);
+    r->grid.gemm = g;
+    r->grid.decide = std::max(1, std::min((capacity + 255) / 256, cus * 8));
+    r->grid.jump = std::max(1, std::min(capacity, 2048));
+    if (hipMalloc((void **)&r->d_ens, kMaxEns * sizeof(Ens)) != hipSuccess ||
+        hipMalloc((void **)&r->d_nens, sizeof(int)) != hipSuccess ||
+        hipMalloc((void **)&r->d_plan, sizeof(Plan)) != hipSuccess ||
+        hipMalloc((void **)&r->d_jl, std::max(capacity, 1) * sizeof(JumpEnt)) != hipSuccess ||
+        hipMalloc((void **)&r->d_jcnt, sizeof(int)) != hipSuccess ||
+        hipHostMalloc((void **)&r->h_total, sizeof(int), hipHostMallocDefault) != hipSuccess)
+        return bail(fail(QJX_ERR_NOMEM, "runner buffers do not fit"));
+    if (hipMemsetAsync(r->d_plan, 0, sizeof(Plan), stream) != hipSuccess ||
+        hipMemsetAsync(r->d_jcnt, 0, sizeof(int), stream) != hipSuccess ||
+        hipMemsetAsync(r->d_nens, 0, sizeof(int), stream) != hipSuccess)
+        return bail(fail(QJX_ERR_HIP, "runner init failed"));
+    *out = r;
+    return QJX_OK;
+}
+
+void runner_destroy(Runner *r)
+{
+    if (!r) return;
+    if (r->stream) (void)hipStreamSynchronize(r->stream);
+    if (r->exec) (void)hipGraphExecDestroy(r->exec);
+    if (r->graph) (void)hipGraphDestroy(r->graph);
+    for (hipEvent_t e : r->ev) (void)hipEventDestroy(e);
+    dev_free(r->d_ens);
+    dev_free(r->d_nens);
+    dev_free(r->d_plan);
+    dev_free(r->d_jl);
+    dev_free(r->d_jcnt);
+    if (r->h_total) (void)hipHostFree(r->h_total);
+    delete r;
+}
+
+int runner_set_table(Runner *r, const Ens *ens, int n)
+{
+    if (n < 0 || n > kMaxEns) return fail(QJX_ERR_ARG, "too many ensembles in one runner");
+    long long tot = 0;
+    for (int i = 0; i < n; i++) tot += ens[i].M;
+    if (tot > r->capacity) return fail(QJX_ERR_ARG, "runner capacity exceeded");
+    if (n) HIP_TRY(hipMemcpyAsync(r->d_ens, ens, n * sizeof(Ens), hipMemcpyHostToDevice, r->stream));
+    r->nens = n;
+    // the count goes through pinned memory so the copy is stream-ordered
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    *r->h_total = n;
+    HIP_TRY(hipMemcpyAsync(r->d_nens, r->h_total, sizeof(int), hipMemcpyHostToDevice, r->stream));
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    return QJX_OK;
+}
+
+static RunArgs run_args(Runner *r, int parity)
+{
+    RunArgs a;
+    a.ens = r->d_ens;
+    a.nens = r->d_nens;
+    a.plan = r->d_plan;
+    a.jl = r->d_jl;
+    a.jcnt = r->d_jcnt;
+    a.parity = parity;
+    a.bn = gemm_bm(r->gemm_kind) == 128 ? 128 : 64;
+    return a;
+}
+
+int runner_chunk(Runner *r, int iters)
+{
+    if (iters <= 0 || (iters & 1)) return fail(QJX_ERR_ARG, "chunk length must be even");
+    if (r->profiling) {
+        if (r->ev.size() < (size_t)4 * iters) {
+            const size_t old = r->ev.size();
+            r->ev.resize(4 * (size_t)iters);
+            for (size_t i = old; i < r->ev.size(); i++) HIP_TRY(hipEventCreate(&r->ev[i]));
+        }
+        for (int k = 0; k < iters; k++) {
+            const RunArgs a = run_args(r, k & 1);
+            HIP_TRY(hipEventRecord(r->ev[4 * k], r->stream));
+            launch_plan(a, r->stream);
+            HIP_TRY(hipEventRecord(r->ev[4 * k + 1], r->stream));
+            launch_gemm(r->prec, r->gemm_kind, a, r->grid, r->stream);
+            HIP_TRY(hipEventRecord(r->ev[4 * k + 2], r->stream));
+            launch_tail(r->prec, a, r->grid, r->stream);
+            HIP_TRY(hipEventRecord(r->ev[4 * k + 3], r->stream));
+        }
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(r->stream));
+        for (int k = 0; k < iters; k++) {
+            float p = 0.f, g = 0.f, t = 0.f;
+            HIP_TRY(hipEventElapsedTime(&p, r->ev[4 * k], r->ev[4 * k + 1]));
+            HIP_TRY(hipEventElapsedTime(&g, r->ev[4 * k + 1], r->ev[4 * k + 2]));
+            HIP_TRY(hipEventElapsedTime(&t, r->ev[4 * k + 2], r->ev[4 * k + 3]));
+            r->plan_ms += p;
+            r->gemm_ms += g;
+            r->tail_ms += t;
+        }
+        r->gemm_launches += iters;
+    } else if (r->graphs) {
+        if (!r->exec) {
+            // two iterations (parity 0 then 1) captured once; every argument lives in
+            // device memory, so later table changes need no new capture
+            HIP_TRY(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
+            launch_iteration(r->prec, r->gemm_kind, run_args(r, 0), r->grid, r->stream);
+            launch_iteration(r->prec, r->gemm_kind, run_args(r, 1), r->grid, r->stream);
+            HIP_TRY(hipStreamEndCapture(r->stream, &r->graph));
+            HIP_TRY(hipGraphInstantiate(&r->exec, r->graph, nullptr, nullptr, 0));
+        }
+        for (int k = 0; k < iters; k += 2) HIP_TRY(hipGraphLaunch(r->exec, r->stream));
+    } else {
+        for (int k = 0; k < iters; k++) launch_iteration(r->prec, r->gemm_kind, run_args(r, k & 1), r->grid, r->stream);
+        HIP_TRY(hipGetLastError());
+    }
+    r->iterations += iters;
+    return QJX_OK;
+}
+
+int runner_next_total(Runner *r, int *total)
+{
+    HIP_TRY(hipMemcpyAsync(r->h_total, &r->d_plan->next_total[0], sizeof(int), hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    *total = *r->h_total;
+    return QJX_OK;
+}
+
+int runner_run(Runner *r, long long *iterations)
+{
+    int chunk = 8;
+    long long done = 0;
+    for (;;) {
+        int rc = runner_chunk(r, chunk);
+        if (rc) return rc;
+        done += chunk;
+        int total = 0;
+        rc = runner_next_total(r, &total);
+        if (rc) return rc;
+        if (total == 0) break;
+        chunk = std::min(chunk * 2, 256);
+    }
+    if (iterations) *iterations = done;
+    return QJX_OK;
+}
+
+}  // namespace qjx
+
+// ---------------------------------------------------------------------------
+// Helpers for the job pool
+// ---------------------------------------------------------------------------
+namespace qjx {
+
+static int padded_dim(int N) { return (N + 1 + 63) / 64 * 64; }
+
+size_t model_footprint(const qjx_model_desc *m, int gemm_kind)
+{
+    (void)gemm_kind;
+    const size_t L = 2 * (size_t)padded_dim(m->N);
+    const size_t pe = m->precision == QJX_FP64 ? 8 : 4;     // BF16x3: hi + lo = 4 bytes
+    const size_t d = m->N + 1;
+    // propagator tree + the build workspace (8 dense complex d x d, freed after the build)
+    return (size_t)2 * m->ex_deep * L * L * pe + 8 * d * d * 16 + (1 << 20);
+}
+
+size_t ensemble_footprint(const qjx_model_desc *m, int M, int gemm_kind)
+{
+    const size_t L = 2 * (size_t)padded_dim(m->N);
+    const size_t es = m->precision == QJX_FP64 ? 8 : 4;
+    const size_t R = L / gemm_bm(gemm_kind);
+    const size_t G = 2 * (size_t)m->ex_deep;
+    return (size_t)M * (2 * L * es + 2 * G * 4 + R * 8 + sizeof(TrajState) + 7 * 8) + (1 << 20);
+}
+
+int ctx_seed(qjx_ctx *c, int n_steps, int is_obs)
+{
+    launch_seed(c->st, c->M, n_steps, c->lists[0], c->cnts[0], c->D, c->stream);
+    HIP_TRY(hipGetLastError());
+    c->desc.is_obs = is_obs;
+    return QJX_OK;
+}
+
+}  // namespace qjx

@@ -7,8 +7,8 @@
 //     current state; a propagation step reads that slot and writes the other one, so
 //     accepting a step is a bit flip and rejecting it (bisection descent) costs nothing.
 //   * propagators: 2 branches x D levels of U = exp(-i H_eff,b dt_i) in real block form
-//     [[Re U, -Im U], [Im U, Re U]] (2dp x 2dp), stored k-major (column-major) so both
-//     GEMM operand tiles are read along contiguous memory.
+//     [[Re U, -Im U], [Im U, Re U]] (2dp x 2dp), row-major, so a row segment of U and a
+//     column segment of the state are both contiguous.
 //   * per-trajectory scalar state (TrajState), group lists for the grouped GEMM and
 //     per-row-tile partial norms.
 #pragma once
@@ -46,32 +46,58 @@ struct JumpLog {
     int cap;         // records per trajectory between drains (0 = logging off)
 };
 
-// Everything a step kernel needs, passed by value.
-struct StepArgs {
-    const void *props;        // 2*D matrices, each L*L elements (L = 2*dp)
+constexpr int kMaxDim = 2048;                        // largest Hilbert dimension d
+constexpr int kMaxGroups = 64;                       // 2 branches x up to 32 levels
+constexpr int kMaxEns = 64;                          // ensembles sharing one iteration
+constexpr int kPlanItems = kMaxEns * kMaxGroups;     // (ensemble, group) items
+
+// Device-resident descriptor of one ensemble (one dimer model and its
+// trajectories).  A table of these drives every step kernel, so ensembles of
+// different Hilbert dimension share each launch.
+struct Ens {
+    const void *props;        // 2*D matrices, each L*L elements (L = 2*dp), row-major
     const void *props_lo;     // BF16x3 only: low halves
     void *slot[2];            // state slots, M*L elements each
     TrajState *st;
-    const int *list;          // [G * M] trajectory ids per group (this iteration)
-    const int *cnt;           // [G]
-    int *list_next;           // [G * M]
-    int *cnt_next;            // [G]
-    int *cnt_clear;           // [G] counters zeroed by the GEMM kernel (next-next use)
+    int *list[2];             // per parity: [G * M] trajectory ids per group
+    int *cnt[2];              // per parity: [kMaxGroups] list lengths
     double *normpart;         // [R * M]
     const double *dt;         // [D]
     const double *Ldiag;      // [d]   L_kk
     const double *Loff;       // [d-1] L_{k+1,k} = -Loff[k], L_{k,k+1} = +Loff[k]
-    int *jlist;               // [M] trajectories that jump in this iteration
-    int *jsrc;                // [M] slot holding the pre-jump state (step result)
-    double *jnorm;            // [M] its ||psi||^2
-    int *jcnt;                // [1] entries in jlist (cleared by the GEMM kernel)
     JumpLog log;
     int *err;                 // [4]: code, trajectory, bits of the offending norm
     unsigned long long *counters;  // [0] matvecs, [1] jumps
+    uint64_t stride;          // MCG59 leap-frog multiplier a^mns
+    double norm_tol;          // allowed excess of ||psi||^2 over 1 (0 for FP64)
     int d, dp, L, D, S, M, G, R;
     int is_obs;
     int jump;                 // logging switch (params.txt "jump")
-    double norm_tol;          // allowed excess of ||psi||^2 over 1 (0 for FP64)
+};
+
+// Schedule of one iteration, written by the plan kernel from the list lengths:
+// prefixes over items (ensemble e, group g) -> e * kMaxGroups + g.
+struct Plan {
+    int tile_pre[kPlanItems + 1];   // GEMM tiles (row tiles x column tiles)
+    int ent_pre[kPlanItems + 1];    // list entries (trajectories stepping)
+    int next_total[2];              // entries appended for the following iteration
+    int xnext[8];                   // per-XCD tile claim counters of the GEMM
+};
+
+struct JumpEnt {
+    int ens, tr, src, pad;    // ensemble, trajectory, slot holding the step result
+    double n;                 // its ||psi||^2
+};
+
+// Kernel argument of every step kernel (constant for a runner except parity).
+struct RunArgs {
+    const Ens *ens;           // [kMaxEns]
+    const int *nens;          // number of ensembles in the table (device)
+    Plan *plan;
+    JumpEnt *jl;              // jumps of this iteration
+    int *jcnt;
+    int parity;               // lists read this iteration: list[parity]
+    int bn;                   // GEMM column tile
 };
 
 }  // namespace qjx

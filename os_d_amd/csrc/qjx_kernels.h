@@ -7,7 +7,6 @@
 
 namespace qjx {
 
-constexpr int kMaxGroups = 64;   // 2 branches x up to 32 levels
 constexpr int kGemmFma = 0;      // LDS-tiled vector FMA, 64 x 64 tiles
 constexpr int kGemmMfma = 1;     // matrix cores, 128 x 128 tiles
 // rows of the block-real state per GEMM row tile
@@ -19,18 +18,27 @@ void launch_init_ensemble(int prec, void *slot0, void *slot1, TrajState *st, int
                           int L, int seed, int start_type, int start_state, hipStream_t s);
 // eta = uniform(); redraw while 0 (experiment.cpp:3259-3265)
 void launch_draw_eta(TrajState *st, int M, uint64_t stride, hipStream_t s);
-// target = pos + n_steps for every trajectory; the group lists then hold every
-// trajectory at level 0 of its current branch.
-void launch_seed_lists(TrajState *st, int M, int n_steps, int *list, int *cnt, int G,
-                       hipStream_t s);
+// target = pos + n_steps for every trajectory; list/cnt (one parity of the
+// ensemble's group lists) then hold every trajectory at level 0 of its branch.
+void launch_seed(TrajState *st, int M, int n_steps, int *list, int *cnt, int D, hipStream_t s);
 void launch_reset_times(TrajState *st, int M, hipStream_t s);
 void launch_reset_logn(TrajState *st, int M, hipStream_t s);
 
-// One propagation iteration over the lists in a: the grouped GEMM, then the
-// decide kernel (crossing test, bisection bookkeeping, jumps, next lists).
-void launch_step_gemm(int prec, int gemm_kind, const StepArgs &a, hipStream_t s);
-void launch_step_decide(int prec, const StepArgs &a, uint64_t stride, hipStream_t s);
-void launch_step_gemm_mfma(int prec, const StepArgs &a, hipStream_t s);
+// Persistent grid sizes of the iteration kernels.
+struct StepGrid {
+    int gemm;     // workgroups of the GEMM (multiple of 8: XCD-aware tile order)
+    int decide;   // workgroups of 256 threads
+    int jump;     // workgroups of 256 threads
+};
+// One propagation iteration over the ensembles of a.ens: plan, grouped GEMM,
+// decide (crossing test, bisection bookkeeping, next lists), jumps.
+void launch_iteration(int prec, int gemm_kind, const RunArgs &a, const StepGrid &g, hipStream_t s);
+// the three parts of an iteration, for per-kernel timing
+void launch_plan(const RunArgs &a, hipStream_t s);
+void launch_gemm(int prec, int gemm_kind, const RunArgs &a, const StepGrid &g, hipStream_t s);
+void launch_tail(int prec, const RunArgs &a, const StepGrid &g, hipStream_t s);
+// resident GEMM workgroups per compute unit (occupancy of the chosen kernel)
+int gemm_blocks_per_cu(int prec, int gemm_kind);
 // one wave: D32 = A32 (32x2) B32 (2x32), D64 = A64 (16x4) B64 (4x16),
 // Dbf = Abf (32x16) Bbf (16x32) through bf16 operands, all row-major
 void launch_mfma_probe(const float *A32, const float *B32, float *D32, const double *A64,

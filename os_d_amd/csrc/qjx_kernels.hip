@@ -1,397 +1,15 @@
-// qjx_kernels.hip -- HIP kernels of the MI355X trajectory engine (gfx950).
-//
-// One propagation "iteration" advances every active trajectory by exactly one
-// matrix-vector step at its own (branch, level) of the bisection tree:
-//   step_gemm   : grouped GEMM  Y_g = U_g X_g  over the trajectories listed in
-//                 group g = b*D + lvl, columns gathered by trajectory id, output
-//                 written to the trajectory's spare slot, FP64 partial ||y||^2
-//                 per row tile in the epilogue.
-//   step_decide : per trajectory, the FP64 crossing test and the recursion of
-//                 one_period_branch (reference qj_proc.cpp:271-343) as a state
-//                 machine; jumps (experiment.cpp:1273-1382) are applied by the
-//                 whole wavefront; the trajectory is appended to the list of the
-//                 group it steps in next.
+// qjx_kernels.hip -- HIP kernels of the MI355X trajectory engine (gfx950) outside
+// the propagation iteration (qjx_step.hip): ensemble initialisation, observables,
+// state export/import and the propagator build (dense complex FP64 GEMM,
+// Taylor-polynomial helpers, block-real packing).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "qjx_device.h"
 #include "qjx_internal.h"
 #include "qjx_kernels.h"
 
 namespace qjx {
-
-// ---------------------------------------------------------------------------
-// MCG59 (MKL VSL_BRNG_MCG59): x <- a x mod 2^59, a = 13^13; output x * 2^-59.
-// The per-trajectory leap-frog stream uses the multiplier a^mns.
-// ---------------------------------------------------------------------------
-__host__ __device__ inline uint64_t mcg_mul(uint64_t x, uint64_t y)
-{
-    return (x * y) & ((1ULL << 59) - 1ULL);
-}
-
-__host__ __device__ inline uint64_t mcg_pow(uint64_t a, uint64_t e)
-{
-    uint64_t r = 1;
-    while (e) {
-        if (e & 1ULL) r = mcg_mul(r, a);
-        a = mcg_mul(a, a);
-        e >>= 1;
-    }
-    return r;
-}
-
-__device__ inline double mcg_next(uint64_t &x, uint64_t stride)
-{
-    const double u = (double)x * 0x1p-59;
-    x = mcg_mul(x, stride);
-    return u;
-}
-
-// ---------------------------------------------------------------------------
-// Small helpers
-// ---------------------------------------------------------------------------
-template <typename T> struct Real2;
-template <> struct Real2<float> { typedef float2 type; };
-template <> struct Real2<double> { typedef double2 type; };
-
-__device__ inline double wave_sum(double v)
-{
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// Prefix over group counts (G <= kMaxGroups), computed by thread 0 into LDS.
-__device__ inline void group_prefix(const int *cnt, int G, int per, int *pre)
-{
-    if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int g = 0; g < G; g++) {
-            pre[g] = acc;
-            acc += (cnt[g] + per - 1) / per;
-        }
-        pre[G] = acc;
-    }
-    __syncthreads();
-}
-
-__device__ inline int find_group(const int *pre, int G, int idx)
-{
-    int g = 0;
-    while (g + 1 < G && pre[g + 1] <= idx) g++;
-    return g;
-}
-
-// ---------------------------------------------------------------------------
-// Grouped propagation GEMM, vector-FMA form (LDS-tiled, 4x4 register tile per
-// thread).  Tile BM x BN over (rows of the block-real state, trajectories).
-// ---------------------------------------------------------------------------
-template <typename T, int BM, int BN, int BK>
-__global__ __launch_bounds__(256) void step_gemm_fma(StepArgs a)
-{
-    __shared__ int s_pre[kMaxGroups + 1];
-    __shared__ T As[BK][BM + 4];
-    __shared__ T Bs[BK][BN + 1];
-    __shared__ int s_tr[BN];
-    __shared__ int s_cur[BN];
-    __shared__ double s_norm[BM / 4][BN + 1];
-
-    const int tid = threadIdx.x;
-    if (blockIdx.x == 0 && blockIdx.y == 0) {
-        if (tid < a.G) a.cnt_clear[tid] = 0;
-        if (tid == 0) a.jcnt[0] = 0;
-    }
-    group_prefix(a.cnt, a.G, BN, s_pre);
-    const int ct = blockIdx.y;
-    if (ct >= s_pre[a.G]) return;
-    const int g = find_group(s_pre, a.G, ct);
-    const int col0 = (ct - s_pre[g]) * BN;
-    const int ncol = min(BN, a.cnt[g] - col0);
-    const size_t LL = (size_t)a.L * a.L;
-    const T *U = reinterpret_cast<const T *>(a.props) + (size_t)g * LL;
-    if (tid < BN) {
-        if (tid < ncol) {
-            const int tr = a.list[(size_t)g * a.M + col0 + tid];
-            s_tr[tid] = tr;
-            s_cur[tid] = a.st[tr].cur;
-        } else {
-            s_tr[tid] = -1;
-            s_cur[tid] = 0;
-        }
-    }
-    __syncthreads();
-
-    const int r0 = blockIdx.x * BM;
-    const int tx = tid % 16, ty = tid / 16;
-    T acc[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = T(0);
-
-    // load mapping: 4 consecutive k of one row of U (row-major) / one column of X
-    const int a_i = tid / (BK / 4), a_k = (tid % (BK / 4)) * 4;      // BM x BK / 4 threads
-    const int b_j = tid / (BK / 4), b_k = (tid % (BK / 4)) * 4;      // BN x BK / 4 threads
-    const T *xcol = nullptr;
-    if (b_j < BN && s_tr[b_j] >= 0)
-        xcol = reinterpret_cast<const T *>(a.slot[s_cur[b_j]]) + (size_t)s_tr[b_j] * a.L;
-
-    for (int k0 = 0; k0 < a.L; k0 += BK) {
-        if (a_i < BM) {
-            const T *src = U + (size_t)(r0 + a_i) * a.L + k0 + a_k;
-#pragma unroll
-            for (int q = 0; q < 4; q++) As[a_k + q][a_i] = src[q];
-        }
-        if (b_j < BN) {
-#pragma unroll
-            for (int q = 0; q < 4; q++) Bs[b_k + q][b_j] = xcol ? xcol[k0 + b_k + q] : T(0);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < BK; kk++) {
-            T av[4], bv[4];
-#pragma unroll
-            for (int i = 0; i < 4; i++) av[i] = As[kk][ty * 4 + i];
-#pragma unroll
-            for (int j = 0; j < 4; j++) bv[j] = Bs[kk][tx + 16 * j];
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-#pragma unroll
-                for (int j = 0; j < 4; j++) acc[i][j] += av[i] * bv[j];
-        }
-        __syncthreads();
-    }
-
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const int c = tx + 16 * j;
-        double ns = 0.0;
-        if (c < ncol) {
-            T *y = reinterpret_cast<T *>(a.slot[s_cur[c] ^ 1]) + (size_t)s_tr[c] * a.L + r0 + ty * 4;
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                y[i] = acc[i][j];
-                const double v = (double)acc[i][j];
-                ns += v * v;
-            }
-        }
-        s_norm[ty][c] = ns;
-    }
-    __syncthreads();
-    if (tid < ncol) {
-        double s = 0.0;
-        for (int q = 0; q < BM / 4; q++) s += s_norm[q][tid];
-        a.normpart[(size_t)blockIdx.x * a.M + s_tr[tid]] = s;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Decide kernel: crossing test, recursion bookkeeping, jumps, next group lists.
-// ---------------------------------------------------------------------------
-// After a completed step at level l (accepted, or the finest-level step that
-// carried a jump; t has already advanced by dt_l): counter++ and, while levels
-// finish, return to the parent (qj_proc.cpp:325-342: t -= dt_parent, then
-// counter++ and t += dt_parent).  Level 0 finishes after S/2 steps and closes
-// the half period; a level l >= 1 finishes after 2 steps.
-__device__ inline void complete_step(TrajState &s, int l, const double *dt, int S)
-{
-    for (;;) {
-        if (l == 0) {
-            s.cnt0++;
-            s.pos++;
-            if (s.cnt0 == S / 2) {
-                s.cnt0 = 0;
-                s.b ^= 1;
-            }
-            break;
-        }
-        const uint32_t bit = 1u << l;
-        if (!(s.bits & bit)) {
-            s.bits |= bit;
-            break;
-        }
-        s.bits &= ~bit;
-        l -= 1;
-        s.t -= dt[l];
-        s.t += dt[l];
-    }
-    s.lvl = l;
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void step_decide(StepArgs a, uint64_t stride)
-{
-    __shared__ int s_pre[kMaxGroups + 1];
-    __shared__ int s_cnt[kMaxGroups + 1];    // per-group appends of this block (+ jumps)
-    __shared__ int s_base[kMaxGroups + 1];
-    if (threadIdx.x <= kMaxGroups) s_cnt[threadIdx.x] = 0;
-    group_prefix(a.cnt, a.G, 1, s_pre);      // (ends with a barrier)
-    const int total = s_pre[a.G];
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx == 0) atomicAdd(&a.counters[0], (unsigned long long)total);
-    if ((int)(blockIdx.x * blockDim.x) >= total) return;   // whole block idle
-    // appends are reserved per block: LDS counters first, one global atomic per group
-    int app_g = -1, app_local = 0;           // next-group append
-    int jmp_local = -1;                      // jump-list append
-    int tr = -1, jsrc = 0;
-    double n = 0.0;
-    if (idx < total) {
-        const int g = find_group(s_pre, a.G, idx);
-        tr = a.list[(size_t)g * a.M + (idx - s_pre[g])];
-        TrajState s = a.st[tr];
-        for (int r = 0; r < a.R; r++) n += a.normpart[(size_t)r * a.M + tr];
-        const int lvl = s.lvl;
-        const double dt = a.dt[lvl];
-        bool alive = true;
-        if (n > 1.0 + a.norm_tol) {
-            // experiment.cpp:1247-1251 throws: record the first offender, stop it.
-            if (atomicCAS(&a.err[0], 0, 1) == 0) {
-                a.err[1] = tr;
-                const unsigned long long bits = (unsigned long long)__double_as_longlong(n);
-                a.err[2] = (int)(bits & 0xffffffffULL);
-                a.err[3] = (int)(bits >> 32);
-            }
-            alive = false;
-        } else {
-            const bool crossed = !(n > s.eta);             // experiment.cpp:1253-1260
-            if (lvl == a.D - 1) {
-                if (crossed) {
-                    if (a.jump > 0 && a.is_obs == 1) {
-                        if (s.logn < a.log.cap) {
-                            const size_t o = (size_t)tr * a.log.cap + s.logn;
-                            a.log.time[o] = s.t + dt;
-                            a.log.norm[o] = n;
-                            a.log.eta[o] = s.eta;
-                            a.log.type[o] = 0;
-                        }
-                        s.logn++;
-                        s.njumps++;
-                    }
-                    // recovery draws `ran` (one channel, so channel 0 is chosen),
-                    // then eta is redrawn while it is 0 (qj_proc.cpp:310-314).
-                    (void)mcg_next(s.rng, stride);
-                    double e = mcg_next(s.rng, stride);
-                    while (e == 0.0) e = mcg_next(s.rng, stride);
-                    s.eta = e;
-                    // the jump kernel reads the step just taken (slot cur ^ 1) and
-                    // writes the jumped state over the pre-step state (slot cur)
-                    jmp_local = atomicAdd(&s_cnt[kMaxGroups], 1);
-                    jsrc = s.cur ^ 1;
-                } else {
-                    s.cur ^= 1;
-                }
-                s.t += dt;
-                complete_step(s, lvl, a.dt, a.S);
-            } else if (crossed) {
-                s.lvl = lvl + 1;          // bisect from the pre-step state
-            } else {
-                s.cur ^= 1;
-                s.t += dt;
-                complete_step(s, lvl, a.dt, a.S);
-            }
-        }
-        a.st[tr] = s;
-        if (alive && s.pos < s.target) {
-            app_g = s.b * a.D + s.lvl;
-            app_local = atomicAdd(&s_cnt[app_g], 1);
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x < a.G) {
-        const int c = s_cnt[threadIdx.x];
-        s_base[threadIdx.x] = c ? atomicAdd(&a.cnt_next[threadIdx.x], c) : 0;
-    } else if (threadIdx.x == kMaxGroups) {
-        const int c = s_cnt[kMaxGroups];
-        s_base[kMaxGroups] = c ? atomicAdd(a.jcnt, c) : 0;
-    }
-    __syncthreads();
-    if (app_g >= 0) a.list_next[(size_t)app_g * a.M + s_base[app_g] + app_local] = tr;
-    if (jmp_local >= 0) {
-        const int j = s_base[kMaxGroups] + jmp_local;
-        a.jlist[j] = tr;
-        a.jsrc[j] = jsrc;
-        a.jnorm[j] = n;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Jumps: recovery (experiment.cpp:1273-1382) for the single dimer dissipator,
-// one workgroup per jumping trajectory (persistent over the jump list):
-//   phi <- phi / ||phi||;  v = L phi;  phi <- v / sqrt(||v||^2)
-// in FP64.  L is tridiagonal: L_kk = Ldiag[k], L_{k+1,k} = -Loff[k],
-// L_{k,k+1} = +Loff[k]; v_k sums its three terms in ascending column order.
-// ---------------------------------------------------------------------------
-constexpr int kJumpThreads = 256;
-constexpr int kMaxDim = 2048;                       // largest supported d
-constexpr int kJumpPer = kMaxDim / kJumpThreads;    // elements per thread
-
-template <typename T>
-__global__ __launch_bounds__(kJumpThreads) void step_jump(StepArgs a)
-{
-    __shared__ double pre[kMaxDim + 2];
-    __shared__ double pim[kMaxDim + 2];
-    __shared__ double red[kJumpThreads / 64];
-    const int njump = a.jcnt[0];
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int d = a.d, dp = a.dp;
-    if (blockIdx.x == 0 && tid == 0 && njump > 0) atomicAdd(&a.counters[1], (unsigned long long)njump);
-    for (int j = blockIdx.x; j < njump; j += gridDim.x) {
-        const int tr = a.jlist[j];
-        const int src = a.jsrc[j];
-        const double sn = sqrt(a.jnorm[j]);
-        const T *x = reinterpret_cast<const T *>(a.slot[src]) + (size_t)tr * a.L;
-        T *y = reinterpret_cast<T *>(a.slot[src ^ 1]) + (size_t)tr * a.L;
-        // pre/pim[k + 1] = phi_k / ||phi||, zero-padded at both ends
-        for (int k = tid; k < d; k += kJumpThreads) {
-            pre[k + 1] = (double)x[k] / sn;
-            pim[k + 1] = (double)x[dp + k] / sn;
-        }
-        if (tid == 0) {
-            pre[0] = pim[0] = 0.0;
-            pre[d + 1] = pim[d + 1] = 0.0;
-        }
-        __syncthreads();
-        double vre[kJumpPer], vim[kJumpPer];
-        double p = 0.0;
-#pragma unroll
-        for (int m = 0; m < kJumpPer; m++) {
-            const int k = tid + m * kJumpThreads;
-            vre[m] = vim[m] = 0.0;
-            if (k < d) {
-                double r = 0.0, i = 0.0;
-                if (k > 0) {
-                    const double l = -a.Loff[k - 1];
-                    r += l * pre[k];
-                    i += l * pim[k];
-                }
-                r += a.Ldiag[k] * pre[k + 1];
-                i += a.Ldiag[k] * pim[k + 1];
-                if (k + 1 < d) {
-                    const double l = a.Loff[k];
-                    r += l * pre[k + 2];
-                    i += l * pim[k + 2];
-                }
-                vre[m] = r;
-                vim[m] = i;
-                p += r * r + i * i;
-            }
-        }
-        p = wave_sum(p);
-        if (lane == 0) red[wid] = p;
-        __syncthreads();
-        double tot = 0.0;
-#pragma unroll
-        for (int w = 0; w < kJumpThreads / 64; w++) tot += red[w];
-        const double sp = sqrt(tot / 1.0);
-#pragma unroll
-        for (int m = 0; m < kJumpPer; m++) {
-            const int k = tid + m * kJumpThreads;
-            if (k < d) {
-                y[k] = (T)(vre[m] / sp);
-                y[dp + k] = (T)(vim[m] / sp);
-            }
-        }
-        __syncthreads();   // LDS reuse by the next jump of this workgroup
-    }
-}
 
 }  // namespace qjx
 
@@ -470,35 +88,6 @@ void launch_draw_eta(TrajState *st, int M, uint64_t stride, hipStream_t s)
     draw_eta_kernel<<<(M + 255) / 256, 256, 0, s>>>(st, M, stride);
 }
 
-__global__ void zero_ints_kernel(int *p, int n)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
-}
-
-// Every trajectory sits at a top-level step boundary between runs, so it starts
-// at level 0 of its current branch.
-__global__ void seed_lists_kernel(TrajState *st, int M, int n, int *list, int *cnt, int D)
-{
-    const int tr = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tr >= M) return;
-    TrajState s = st[tr];
-    s.target = s.pos + n;
-    st[tr] = s;
-    if (n > 0) {
-        const int g = s.b * D;
-        const int p = atomicAdd(&cnt[g], 1);
-        list[(size_t)g * M + p] = tr;
-    }
-}
-
-void launch_seed_lists(TrajState *st, int M, int n_steps, int *list, int *cnt, int G,
-                       hipStream_t s)
-{
-    zero_ints_kernel<<<1, kMaxGroups, 0, s>>>(cnt, kMaxGroups);
-    seed_lists_kernel<<<(M + 255) / 256, 256, 0, s>>>(st, M, n_steps, list, cnt, G / 2);
-}
-
 __global__ void reset_times_kernel(TrajState *st, int M)
 {
     const int tr = blockIdx.x * blockDim.x + threadIdx.x;
@@ -519,33 +108,6 @@ __global__ void reset_logn_kernel(TrajState *st, int M)
 void launch_reset_logn(TrajState *st, int M, hipStream_t s)
 {
     reset_logn_kernel<<<(M + 255) / 256, 256, 0, s>>>(st, M);
-}
-
-// ---------------------------------------------------------------------------
-// Propagation iteration
-// ---------------------------------------------------------------------------
-void launch_step_gemm(int prec, int gemm_kind, const StepArgs &a, hipStream_t s)
-{
-    if (gemm_kind == kGemmMfma) {
-        launch_step_gemm_mfma(prec, a, s);
-        return;
-    }
-    constexpr int BM = 64, BN = 64, BK = 16;
-    dim3 grid(a.R, (a.M + BN - 1) / BN + a.G);
-    if (prec == kFP64) step_gemm_fma<double, BM, BN, BK><<<grid, 256, 0, s>>>(a);
-    else step_gemm_fma<float, BM, BN, BK><<<grid, 256, 0, s>>>(a);
-}
-
-void launch_step_decide(int prec, const StepArgs &a, uint64_t stride, hipStream_t s)
-{
-    const int jblocks = a.M < 2048 ? a.M : 2048;
-    if (prec == kFP64) {
-        step_decide<double><<<(a.M + 255) / 256, 256, 0, s>>>(a, stride);
-        step_jump<double><<<jblocks, kJumpThreads, 0, s>>>(a);
-    } else {
-        step_decide<float><<<(a.M + 255) / 256, 256, 0, s>>>(a, stride);
-        step_jump<float><<<jblocks, kJumpThreads, 0, s>>>(a);
-    }
 }
 
 // ---------------------------------------------------------------------------

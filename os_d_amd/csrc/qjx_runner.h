@@ -1,0 +1,47 @@
+// qjx_runner.h -- host driver of the propagation iterations over a table of
+// ensembles (one or many models), with optional HIP-graph capture.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "qjx_internal.h"
+#include "qjx_kernels.h"
+
+namespace qjx {
+
+struct Runner {
+    int prec = 0, gemm_kind = 0;
+    hipStream_t stream = nullptr;     // not owned
+    Ens *d_ens = nullptr;             // [kMaxEns]
+    int *d_nens = nullptr;
+    Plan *d_plan = nullptr;
+    JumpEnt *d_jl = nullptr;          // [capacity]
+    int *d_jcnt = nullptr;
+    int *h_total = nullptr;           // pinned
+    int capacity = 0;                 // trajectories the table may hold in total
+    StepGrid grid{};
+    int nens = 0;
+    bool graphs = true;               // replay a captured pair of iterations
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool profiling = false;           // time each kernel with events (no graphs)
+    std::vector<hipEvent_t> ev;
+    double plan_ms = 0.0, gemm_ms = 0.0, tail_ms = 0.0;
+    long long gemm_launches = 0, iterations = 0;
+};
+
+int runner_create(int prec, int gemm_kind, hipStream_t stream, int capacity, Runner **out);
+void runner_destroy(Runner *r);
+// Copy the ensemble table (n descriptors) to the device (on r->stream).
+int runner_set_table(Runner *r, const Ens *ens, int n);
+// Launch `iters` iterations (even: the lists read next are parity 0 again).
+int runner_chunk(Runner *r, int iters);
+// Entries the next iteration will process (synchronises r->stream).
+int runner_next_total(Runner *r, int *total);
+// Chunks of growing length until no trajectory in the table has steps left.
+int runner_run(Runner *r, long long *iterations);
+
+void set_last_error(const std::string &msg);
+
+}  // namespace qjx

@@ -1,0 +1,986 @@
+// qjx_step.hip -- one propagation iteration over a table of ensembles (gfx950).
+//
+// Every active trajectory of every ensemble in the table advances by exactly one
+// matrix-vector step at its own (branch, level) of the bisection tree
+// (reference qj_proc.cpp:271-343):
+//   plan   : prefixes over items (ensemble e, group g = b*D + lvl) of GEMM tiles and
+//            list entries; clears the counters the iteration appends to.
+//   gemm   : grouped GEMM  Y = U_g X  over each item's trajectories (columns gathered by
+//            trajectory id), output to the spare slot, FP64 partial ||y||^2 per row
+//            tile.  Persistent workgroups claim tiles from per-XCD ranges.
+//   decide : FP64 crossing test and the recursion as a state machine; jumps queued.
+//   jump   : recovery (experiment.cpp:1273-1382), one workgroup per jump.
+// All arguments live in device memory (RunArgs points at them), so an iteration's
+// launches are identical from one iteration to the next except for the parity
+// and can be captured in a HIP graph.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "qjx_device.h"
+#include "qjx_internal.h"
+#include "qjx_kernels.h"
+
+namespace qjx {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+typedef double double2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+// largest item with pre[item] <= t (pre non-decreasing, pre[n] > t)
+__device__ inline int plan_search(const int *pre, int n, int t)
+{
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pre[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------------------------
+// plan: one workgroup of 1024 threads, 4 consecutive items per thread (all of
+// one ensemble, since kMaxGroups is a multiple of 4).
+// ---------------------------------------------------------------------------
+constexpr int kPlanThreads = 1024;
+static_assert(kPlanItems == 4 * kPlanThreads, "plan layout");
+
+__global__ __launch_bounds__(kPlanThreads) void plan_kernel(RunArgs a)
+{
+    __shared__ int s_t[kPlanThreads];
+    __shared__ int s_e[kPlanThreads];
+    const int tid = threadIdx.x;
+    const int nens = *a.nens;
+    const int nitems = nens * kMaxGroups;
+    const int p = a.parity;
+    int tl[4], el[4], tsum = 0, esum = 0;
+    const int item0 = 4 * tid;
+    const int e = item0 / kMaxGroups;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        tl[q] = el[q] = 0;
+        if (item0 + q < nitems) {
+            const int g = (item0 + q) % kMaxGroups;
+            const Ens &E = a.ens[e];
+            const int c = g < E.G ? E.cnt[p][g] : 0;
+            el[q] = c;
+            tl[q] = ((c + a.bn - 1) / a.bn) * E.R;
+            E.cnt[p ^ 1][g] = 0;      // the decide kernel of this iteration appends here
+        }
+        tsum += tl[q];
+        esum += el[q];
+    }
+    if (esum > 0) atomicAdd(&a.ens[e].counters[0], (unsigned long long)esum);
+    s_t[tid] = tsum;
+    s_e[tid] = esum;
+    __syncthreads();
+    // inclusive Hillis-Steele scan over the threads that hold items
+    int nthr = 1;
+    while (nthr < (nitems + 3) / 4) nthr <<= 1;
+    for (int off = 1; off < nthr; off <<= 1) {
+        const int vt = tid >= off ? s_t[tid - off] : 0;
+        const int ve = tid >= off ? s_e[tid - off] : 0;
+        __syncthreads();
+        s_t[tid] += vt;
+        s_e[tid] += ve;
+        __syncthreads();
+    }
+    int tb = s_t[tid] - tsum, eb = s_e[tid] - esum;   // exclusive bases
+    Plan *pl = a.plan;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        if (item0 + q < nitems) {
+            pl->tile_pre[item0 + q] = tb;
+            pl->ent_pre[item0 + q] = eb;
+        }
+        tb += tl[q];
+        eb += el[q];
+    }
+    if (tid == 0) {
+        pl->tile_pre[nitems] = s_t[nthr - 1];
+        pl->ent_pre[nitems] = s_e[nthr - 1];
+        pl->next_total[p ^ 1] = 0;
+        *a.jcnt = 0;
+    }
+    if (tid < 8) pl->xnext[tid] = 0;
+}
+
+// Claim this persistent workgroup's next tile.  The tile space is cut into 8
+// contiguous ranges, one per XCD (workgroup w runs on XCD w % 8), so the row tiles
+// of a column tile are claimed on one XCD and share its L2; a workgroup whose range
+// is exhausted takes tiles from the other XCDs' ranges.  Thread 0 claims with an
+// atomic and broadcasts through LDS.
+struct TileRef {
+    int e, g, rt, col0, ncol;
+};
+
+template <int BN>
+__device__ inline bool tile_next(const RunArgs &a, int nitems, int total, int *s_claim, TileRef &r)
+{
+    if (threadIdx.x == 0) {
+        int t = -1;
+        const int x0 = blockIdx.x & 7;
+        for (int i = 0; i < 8 && t < 0; i++) {
+            const int x = (x0 + i) & 7;
+            const int lo = (int)(((long long)total * x) >> 3), hi = (int)(((long long)total * (x + 1)) >> 3);
+            if (hi <= lo) continue;
+            if (a.plan->xnext[x] >= hi - lo) continue;           // cheap pre-check
+            const int c = atomicAdd(&a.plan->xnext[x], 1);
+            if (c < hi - lo) t = lo + c;
+        }
+        *s_claim = t;
+    }
+    __syncthreads();
+    const int t = *s_claim;
+    __syncthreads();
+    if (t < 0) return false;
+    const int item = plan_search(a.plan->tile_pre, nitems, t);
+    const int local = t - a.plan->tile_pre[item];
+    r.e = item / kMaxGroups;
+    r.g = item % kMaxGroups;
+    const Ens &E = a.ens[r.e];
+    r.rt = local % E.R;
+    r.col0 = (local / E.R) * BN;
+    r.ncol = min(BN, E.cnt[a.parity][r.g] - r.col0);
+    return true;
+}
+
+template <int BN>
+__device__ inline void load_cols(const RunArgs &a, const TileRef &r, int *s_tr, int *s_cur)
+{
+    const Ens &E = a.ens[r.e];
+    const int tid = threadIdx.x;
+    if (tid < BN) {
+        // columns past ncol repeat the first one: loaded, never stored
+        const int c = tid < r.ncol ? tid : 0;
+        const int tr = E.list[a.parity][(size_t)r.g * E.M + r.col0 + c];
+        s_tr[tid] = tr;
+        s_cur[tid] = E.st[tr].cur;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// GEMM tile, FP32: v_mfma_f32_32x32x2f32
+//   A (32 x 2): lane l holds A[l % 32][l / 32]; B (2 x 32): B[l / 32][l % 32]
+//   D (32 x 32): lane l, register r holds D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32]
+// 4 waves as 2 x 2, wave tile 64 x 64, WG tile 128 x 128, K block 32; within a K
+// block, MFMA step s (0..15) takes k = s + 16 h for lane half h, so each lane reads
+// its 16 fragment values per block with four 16-byte LDS reads.  The k order is
+// the same for every column, so results do not depend on where a column lands.
+// ---------------------------------------------------------------------------
+constexpr int MF_BM = 128, MF_BN = 128, MF_BK = 32;
+constexpr int F32_PITCH = MF_BK + 4;
+
+__global__ __launch_bounds__(256) void gemm_f32(RunArgs a)
+{
+    __shared__ int s_tr[MF_BN];
+    __shared__ int s_cur[MF_BN];
+    __shared__ __attribute__((aligned(16))) float As[MF_BM * F32_PITCH];
+    __shared__ __attribute__((aligned(16))) float Bs[MF_BN * F32_PITCH];
+    __shared__ double s_norm[2][MF_BN];
+    const int nitems = *a.nens * kMaxGroups;
+    const int total = a.plan->tile_pre[nitems];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int wm = wid >> 1, wn = wid & 1;
+    const int h = lane >> 5, li = lane & 31;
+    const int cp_k = (tid & 7) * 4, cp_r = tid >> 3;
+    __shared__ int s_claim;
+    TileRef r;
+    while (tile_next<MF_BN>(a, nitems, total, &s_claim, r)) {
+        load_cols<MF_BN>(a, r, s_tr, s_cur);
+        __syncthreads();
+        const Ens &E = a.ens[r.e];
+        const int L = E.L, r0 = r.rt * MF_BM;
+        const float *U = reinterpret_cast<const float *>(E.props) + (size_t)r.g * L * L;
+        const float *arow = U + (size_t)(r0 + cp_r) * L + cp_k;
+        const size_t astep = (size_t)32 * L;
+        const float *bcol[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int c = cp_r + 32 * q;
+            bcol[q] = reinterpret_cast<const float *>(E.slot[s_cur[c]]) + (size_t)s_tr[c] * L + cp_k;
+        }
+        floatx16 acc[2][2];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+#pragma unroll
+                for (int e = 0; e < 16; e++) acc[i][j][e] = 0.f;
+        floatx4 ra[4], rb[4];
+        auto gload = [&](int k0) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                ra[q] = *reinterpret_cast<const floatx4 *>(arow + q * astep + k0);
+                rb[q] = *reinterpret_cast<const floatx4 *>(bcol[q] + k0);
+            }
+        };
+        auto lstore = [&]() {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int rr = cp_r + 32 * q;
+                *reinterpret_cast<floatx4 *>(&As[rr * F32_PITCH + cp_k]) = ra[q];
+                *reinterpret_cast<floatx4 *>(&Bs[rr * F32_PITCH + cp_k]) = rb[q];
+            }
+        };
+        gload(0);
+        lstore();
+        __syncthreads();
+        for (int k0 = 0; k0 < L; k0 += MF_BK) {
+            const bool more = k0 + MF_BK < L;
+            if (more) gload(k0 + MF_BK);
+            floatx4 fa[2][4], fb[2][4];
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const float *pa = &As[(wm * 64 + i * 32 + li) * F32_PITCH + 16 * h];
+                const float *pb = &Bs[(wn * 64 + i * 32 + li) * F32_PITCH + 16 * h];
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    fa[i][v] = *reinterpret_cast<const floatx4 *>(pa + 4 * v);
+                    fb[i][v] = *reinterpret_cast<const floatx4 *>(pb + 4 * v);
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < 16; s++)
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+#pragma unroll
+                    for (int j = 0; j < 2; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[i][s >> 2][s & 3],
+                                                                          fb[j][s >> 2][s & 3], acc[i][j], 0, 0, 0);
+            __syncthreads();
+            if (more) {
+                lstore();
+                __syncthreads();
+            }
+        }
+        double ns[2] = {0.0, 0.0};
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int c = wn * 64 + j * 32 + li;
+            const bool ok = c < r.ncol;
+            float *y = reinterpret_cast<float *>(E.slot[s_cur[c] ^ 1]) + (size_t)s_tr[c] * L;
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int row = r0 + wm * 64 + i * 32 + 8 * q + 4 * h;
+                    floatx4 v = {acc[i][j][4 * q], acc[i][j][4 * q + 1], acc[i][j][4 * q + 2], acc[i][j][4 * q + 3]};
+                    if (ok) *reinterpret_cast<floatx4 *>(y + row) = v;
+#pragma unroll
+                    for (int e = 0; e < 4; e++) ns[j] += (double)v[e] * (double)v[e];
+                }
+        }
+#pragma unroll
+        for (int j = 0; j < 2; j++) ns[j] += __shfl_xor(ns[j], 32, 64);
+        if (h == 0) {
+#pragma unroll
+            for (int j = 0; j < 2; j++) s_norm[wm][wn * 64 + j * 32 + li] = ns[j];
+        }
+        __syncthreads();
+        if (tid < r.ncol) E.normpart[(size_t)r.rt * E.M + s_tr[tid]] = s_norm[0][tid] + s_norm[1][tid];
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// GEMM tile, 3 x BF16 split product, FP32 accumulate: x = hi + lo (hi = bf16(x),
+// lo = bf16(x - hi)) for both operands, U X ~= Uhi Xhi + Uhi Xlo + Ulo Xhi.
+// v_mfma_f32_32x32x16_bf16: lane l holds 8 consecutive k of row / column l % 32
+// (lane half h = l / 32 selects which 8); D as in the FP32 form.  The propagator
+// is split when packed, the state tile when staged into LDS.
+// ---------------------------------------------------------------------------
+constexpr int BF_PITCH = MF_BK + 8;
+
+__global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
+{
+    __shared__ int s_tr[MF_BN];
+    __shared__ int s_cur[MF_BN];
+    __shared__ __attribute__((aligned(16))) __bf16 Ah[MF_BM * BF_PITCH];
+    __shared__ __attribute__((aligned(16))) __bf16 Al[MF_BM * BF_PITCH];
+    __shared__ __attribute__((aligned(16))) __bf16 Bh[MF_BN * BF_PITCH];
+    __shared__ __attribute__((aligned(16))) __bf16 Bl[MF_BN * BF_PITCH];
+    __shared__ double s_norm[2][MF_BN];
+    const int nitems = *a.nens * kMaxGroups;
+    const int total = a.plan->tile_pre[nitems];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int wm = wid >> 1, wn = wid & 1;
+    const int h = lane >> 5, li = lane & 31;
+    const int ca_k = (tid & 3) * 8, ca_r = tid >> 2;
+    const int cb_k = (tid & 7) * 4, cb_r = tid >> 3;
+    __shared__ int s_claim;
+    TileRef r;
+    while (tile_next<MF_BN>(a, nitems, total, &s_claim, r)) {
+        load_cols<MF_BN>(a, r, s_tr, s_cur);
+        __syncthreads();
+        const Ens &E = a.ens[r.e];
+        const int L = E.L, r0 = r.rt * MF_BM;
+        const size_t LL = (size_t)L * L;
+        const __bf16 *Uh = reinterpret_cast<const __bf16 *>(E.props) + (size_t)r.g * LL;
+        const __bf16 *Ul = reinterpret_cast<const __bf16 *>(E.props_lo) + (size_t)r.g * LL;
+        const float *bcol[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int c = cb_r + 32 * q;
+            bcol[q] = reinterpret_cast<const float *>(E.slot[s_cur[c]]) + (size_t)s_tr[c] * L + cb_k;
+        }
+        floatx16 acc[2][2];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+#pragma unroll
+                for (int e = 0; e < 16; e++) acc[i][j][e] = 0.f;
+        bf16x8 rah[2], ral[2];
+        floatx4 rb[4];
+        auto gload = [&](int k0) {
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const size_t o = (size_t)(r0 + ca_r + 64 * q) * L + k0 + ca_k;
+                rah[q] = *reinterpret_cast<const bf16x8 *>(Uh + o);
+                ral[q] = *reinterpret_cast<const bf16x8 *>(Ul + o);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) rb[q] = *reinterpret_cast<const floatx4 *>(bcol[q] + k0);
+        };
+        auto lstore = [&]() {
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const int rr = ca_r + 64 * q;
+                *reinterpret_cast<bf16x8 *>(&Ah[rr * BF_PITCH + ca_k]) = rah[q];
+                *reinterpret_cast<bf16x8 *>(&Al[rr * BF_PITCH + ca_k]) = ral[q];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int rr = cb_r + 32 * q;
+                bf16x4 h4, l4;
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const float x = rb[q][e];
+                    const __bf16 hb = (__bf16)x;
+                    h4[e] = hb;
+                    l4[e] = (__bf16)(x - (float)hb);
+                }
+                *reinterpret_cast<bf16x4 *>(&Bh[rr * BF_PITCH + cb_k]) = h4;
+                *reinterpret_cast<bf16x4 *>(&Bl[rr * BF_PITCH + cb_k]) = l4;
+            }
+        };
+        gload(0);
+        lstore();
+        __syncthreads();
+        for (int k0 = 0; k0 < L; k0 += MF_BK) {
+            const bool more = k0 + MF_BK < L;
+            if (more) gload(k0 + MF_BK);
+#pragma unroll
+            for (int c = 0; c < MF_BK / 16; c++) {
+                bf16x8 fah[2], fal[2], fbh[2], fbl[2];
+#pragma unroll
+                for (int i = 0; i < 2; i++) {
+                    const int oa = (wm * 64 + i * 32 + li) * BF_PITCH + 16 * c + 8 * h;
+                    const int ob = (wn * 64 + i * 32 + li) * BF_PITCH + 16 * c + 8 * h;
+                    fah[i] = *reinterpret_cast<const bf16x8 *>(&Ah[oa]);
+                    fal[i] = *reinterpret_cast<const bf16x8 *>(&Al[oa]);
+                    fbh[i] = *reinterpret_cast<const bf16x8 *>(&Bh[ob]);
+                    fbl[i] = *reinterpret_cast<const bf16x8 *>(&Bl[ob]);
+                }
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+#pragma unroll
+                    for (int j = 0; j < 2; j++) {
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fal[i], fbh[j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fah[i], fbl[j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fah[i], fbh[j], acc[i][j], 0, 0, 0);
+                    }
+            }
+            __syncthreads();
+            if (more) {
+                lstore();
+                __syncthreads();
+            }
+        }
+        double ns[2] = {0.0, 0.0};
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int c = wn * 64 + j * 32 + li;
+            const bool ok = c < r.ncol;
+            float *y = reinterpret_cast<float *>(E.slot[s_cur[c] ^ 1]) + (size_t)s_tr[c] * L;
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int row = r0 + wm * 64 + i * 32 + 8 * q + 4 * h;
+                    floatx4 v = {acc[i][j][4 * q], acc[i][j][4 * q + 1], acc[i][j][4 * q + 2], acc[i][j][4 * q + 3]};
+                    if (ok) *reinterpret_cast<floatx4 *>(y + row) = v;
+#pragma unroll
+                    for (int e = 0; e < 4; e++) ns[j] += (double)v[e] * (double)v[e];
+                }
+        }
+#pragma unroll
+        for (int j = 0; j < 2; j++) ns[j] += __shfl_xor(ns[j], 32, 64);
+        if (h == 0) {
+#pragma unroll
+            for (int j = 0; j < 2; j++) s_norm[wm][wn * 64 + j * 32 + li] = ns[j];
+        }
+        __syncthreads();
+        if (tid < r.ncol) E.normpart[(size_t)r.rt * E.M + s_tr[tid]] = s_norm[0][tid] + s_norm[1][tid];
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// GEMM tile, FP64: v_mfma_f64_16x16x4f64
+//   A (16 x 4): lane l holds A[l % 16][l / 16]; B (4 x 16): B[l / 16][l % 16]
+//   D (16 x 16): lane l, register r holds D[(l / 16) + 4 r][l % 16]
+// 8 waves as 4 x 2, wave tile 32 x 64, WG tile 128 x 128; within a K block of 32,
+// MFMA step s (0..7) takes k = s + 8 q for lane quarter q.
+// ---------------------------------------------------------------------------
+constexpr int F64_PITCH = MF_BK + 2;
+
+__global__ __launch_bounds__(512) void gemm_f64(RunArgs a)
+{
+    __shared__ int s_tr[MF_BN];
+    __shared__ int s_cur[MF_BN];
+    __shared__ __attribute__((aligned(16))) double As[MF_BM * F64_PITCH];
+    __shared__ __attribute__((aligned(16))) double Bs[MF_BN * F64_PITCH];
+    __shared__ double s_norm[4][MF_BN];
+    const int nitems = *a.nens * kMaxGroups;
+    const int total = a.plan->tile_pre[nitems];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int wm = wid >> 1, wn = wid & 1;
+    const int qq = lane >> 4, li = lane & 15;
+    const int cp_k = (tid & 15) * 2, cp_r = tid >> 4;
+    __shared__ int s_claim;
+    TileRef r;
+    while (tile_next<MF_BN>(a, nitems, total, &s_claim, r)) {
+        load_cols<MF_BN>(a, r, s_tr, s_cur);
+        __syncthreads();
+        const Ens &E = a.ens[r.e];
+        const int L = E.L, r0 = r.rt * MF_BM;
+        const double *U = reinterpret_cast<const double *>(E.props) + (size_t)r.g * L * L;
+        const double *arow = U + (size_t)(r0 + cp_r) * L + cp_k;
+        const size_t astep = (size_t)32 * L;
+        const double *bcol[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int c = cp_r + 32 * q;
+            bcol[q] = reinterpret_cast<const double *>(E.slot[s_cur[c]]) + (size_t)s_tr[c] * L + cp_k;
+        }
+        doublex4 acc[2][4];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] = doublex4{0.0, 0.0, 0.0, 0.0};
+        double2v ra[4], rb[4];
+        auto gload = [&](int k0) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                ra[q] = *reinterpret_cast<const double2v *>(arow + q * astep + k0);
+                rb[q] = *reinterpret_cast<const double2v *>(bcol[q] + k0);
+            }
+        };
+        auto lstore = [&]() {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int rr = cp_r + 32 * q;
+                *reinterpret_cast<double2v *>(&As[rr * F64_PITCH + cp_k]) = ra[q];
+                *reinterpret_cast<double2v *>(&Bs[rr * F64_PITCH + cp_k]) = rb[q];
+            }
+        };
+        const double *pa0 = &As[(wm * 32 + li) * F64_PITCH + 8 * qq];
+        const double *pb0 = &Bs[(wn * 64 + li) * F64_PITCH + 8 * qq];
+        gload(0);
+        lstore();
+        __syncthreads();
+        for (int k0 = 0; k0 < L; k0 += MF_BK) {
+            const bool more = k0 + MF_BK < L;
+            if (more) gload(k0 + MF_BK);
+#pragma unroll
+            for (int half = 0; half < 2; half++) {
+                double2v fa[2][2], fb[4][2];
+#pragma unroll
+                for (int i = 0; i < 2; i++) {
+                    const double *pa = pa0 + i * 16 * F64_PITCH + 4 * half;
+                    fa[i][0] = *reinterpret_cast<const double2v *>(pa);
+                    fa[i][1] = *reinterpret_cast<const double2v *>(pa + 2);
+                }
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const double *pb = pb0 + j * 16 * F64_PITCH + 4 * half;
+                    fb[j][0] = *reinterpret_cast<const double2v *>(pb);
+                    fb[j][1] = *reinterpret_cast<const double2v *>(pb + 2);
+                }
+#pragma unroll
+                for (int s4 = 0; s4 < 4; s4++)
+#pragma unroll
+                    for (int i = 0; i < 2; i++)
+#pragma unroll
+                        for (int j = 0; j < 4; j++)
+                            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i][s4 >> 1][s4 & 1],
+                                                                              fb[j][s4 >> 1][s4 & 1], acc[i][j],
+                                                                              0, 0, 0);
+            }
+            __syncthreads();
+            if (more) {
+                lstore();
+                __syncthreads();
+            }
+        }
+        double ns[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int c = wn * 64 + j * 16 + li;
+            const bool ok = c < r.ncol;
+            double *y = reinterpret_cast<double *>(E.slot[s_cur[c] ^ 1]) + (size_t)s_tr[c] * L;
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const int row = r0 + wm * 32 + i * 16 + qq;
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    if (ok) y[row + 4 * e] = acc[i][j][e];
+                    ns[j] += acc[i][j][e] * acc[i][j][e];
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            ns[j] += __shfl_xor(ns[j], 16, 64);
+            ns[j] += __shfl_xor(ns[j], 32, 64);
+        }
+        if (qq == 0) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) s_norm[wm][wn * 64 + j * 16 + li] = ns[j];
+        }
+        __syncthreads();
+        if (tid < r.ncol)
+            E.normpart[(size_t)r.rt * E.M + s_tr[tid]] =
+                ((s_norm[0][tid] + s_norm[1][tid]) + s_norm[2][tid]) + s_norm[3][tid];
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// GEMM tile, vector FMA (reference form for checks and tiny sizes): 64 x 64 tile,
+// 4 x 4 register tile per thread, K block 16.
+// ---------------------------------------------------------------------------
+constexpr int FM_BM = 64, FM_BN = 64, FM_BK = 16;
+
+template <typename T>
+__global__ __launch_bounds__(256) void gemm_fma(RunArgs a)
+{
+    __shared__ int s_tr[FM_BN];
+    __shared__ int s_cur[FM_BN];
+    __shared__ T As[FM_BK][FM_BM + 4];
+    __shared__ T Bs[FM_BK][FM_BN + 1];
+    __shared__ double s_norm[FM_BM / 4][FM_BN + 1];
+    const int nitems = *a.nens * kMaxGroups;
+    const int total = a.plan->tile_pre[nitems];
+    const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
+    const int a_i = tid / (FM_BK / 4), a_k = (tid % (FM_BK / 4)) * 4;
+    const int b_j = tid / (FM_BK / 4), b_k = (tid % (FM_BK / 4)) * 4;
+    __shared__ int s_claim;
+    TileRef r;
+    while (tile_next<FM_BN>(a, nitems, total, &s_claim, r)) {
+        load_cols<FM_BN>(a, r, s_tr, s_cur);
+        __syncthreads();
+        const Ens &E = a.ens[r.e];
+        const int L = E.L, r0 = r.rt * FM_BM;
+        const T *U = reinterpret_cast<const T *>(E.props) + (size_t)r.g * L * L;
+        const T *xcol = reinterpret_cast<const T *>(E.slot[s_cur[b_j]]) + (size_t)s_tr[b_j] * L;
+        T acc[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] = T(0);
+        for (int k0 = 0; k0 < L; k0 += FM_BK) {
+            const T *src = U + (size_t)(r0 + a_i) * L + k0 + a_k;
+#pragma unroll
+            for (int q = 0; q < 4; q++) As[a_k + q][a_i] = src[q];
+#pragma unroll
+            for (int q = 0; q < 4; q++) Bs[b_k + q][b_j] = xcol[k0 + b_k + q];
+            __syncthreads();
+#pragma unroll
+            for (int kk = 0; kk < FM_BK; kk++) {
+                T av[4], bv[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) av[i] = As[kk][ty * 4 + i];
+#pragma unroll
+                for (int j = 0; j < 4; j++) bv[j] = Bs[kk][tx + 16 * j];
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) acc[i][j] += av[i] * bv[j];
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int c = tx + 16 * j;
+            double ns = 0.0;
+            if (c < r.ncol) {
+                T *y = reinterpret_cast<T *>(E.slot[s_cur[c] ^ 1]) + (size_t)s_tr[c] * L + r0 + ty * 4;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    y[i] = acc[i][j];
+                    const double v = (double)acc[i][j];
+                    ns += v * v;
+                }
+            }
+            s_norm[ty][c] = ns;
+        }
+        __syncthreads();
+        if (tid < r.ncol) {
+            double s = 0.0;
+            for (int q = 0; q < FM_BM / 4; q++) s += s_norm[q][tid];
+            E.normpart[(size_t)r.rt * E.M + s_tr[tid]] = s;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// decide: after a completed step at level l (accepted, or the finest-level step
+// that carried a jump; t has already advanced by dt_l): counter++ and, while levels
+// finish, return to the parent (qj_proc.cpp:325-342: t -= dt_parent, then counter++
+// and t += dt_parent).  Level 0 finishes after S/2 steps and closes the half period;
+// a level l >= 1 finishes after 2 steps.
+// ---------------------------------------------------------------------------
+__device__ inline void complete_step(TrajState &s, int l, const double *dt, int S)
+{
+    for (;;) {
+        if (l == 0) {
+            s.cnt0++;
+            s.pos++;
+            if (s.cnt0 == S / 2) {
+                s.cnt0 = 0;
+                s.b ^= 1;
+            }
+            break;
+        }
+        const uint32_t bit = 1u << l;
+        if (!(s.bits & bit)) {
+            s.bits |= bit;
+            break;
+        }
+        s.bits &= ~bit;
+        l -= 1;
+        s.t -= dt[l];
+        s.t += dt[l];
+    }
+    s.lvl = l;
+}
+
+__global__ __launch_bounds__(256) void decide_kernel(RunArgs a)
+{
+    const int nitems = *a.nens * kMaxGroups;
+    const Plan *pl = a.plan;
+    const int total = pl->ent_pre[nitems];
+    const int p = a.parity;
+    const int lane = threadIdx.x & 63;
+    const int gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nw = (gridDim.x * blockDim.x) >> 6;
+    for (int base = gw * 64; base < total; base += nw * 64) {
+        const int idx = base + lane;
+        int e = -1, tr = -1, key = -1, jsrc = 0;
+        bool jumped = false;
+        double n = 0.0;
+        if (idx < total) {
+            const int item = plan_search(pl->ent_pre, nitems, idx);
+            e = item / kMaxGroups;
+            const int g = item % kMaxGroups;
+            const Ens &E = a.ens[e];
+            tr = E.list[p][(size_t)g * E.M + (idx - pl->ent_pre[item])];
+            TrajState s = E.st[tr];
+            for (int rt = 0; rt < E.R; rt++) n += E.normpart[(size_t)rt * E.M + tr];
+            const int lvl = s.lvl;
+            const double dt = E.dt[lvl];
+            bool alive = true;
+            if (n > 1.0 + E.norm_tol) {
+                // experiment.cpp:1247-1251 throws: record the first offender, stop it.
+                if (atomicCAS(&E.err[0], 0, 1) == 0) {
+                    E.err[1] = tr;
+                    const unsigned long long bits = (unsigned long long)__double_as_longlong(n);
+                    E.err[2] = (int)(bits & 0xffffffffULL);
+                    E.err[3] = (int)(bits >> 32);
+                }
+                alive = false;
+            } else {
+                const bool crossed = !(n > s.eta);          // experiment.cpp:1253-1260
+                if (lvl == E.D - 1) {
+                    if (crossed) {
+                        if (E.jump > 0 && E.is_obs == 1) {
+                            if (s.logn < E.log.cap) {
+                                const size_t o = (size_t)tr * E.log.cap + s.logn;
+                                E.log.time[o] = s.t + dt;
+                                E.log.norm[o] = n;
+                                E.log.eta[o] = s.eta;
+                                E.log.type[o] = 0;
+                            }
+                            s.logn++;
+                            s.njumps++;
+                        }
+                        // recovery draws `ran` (one channel, so channel 0 is chosen),
+                        // then eta is redrawn while it is 0 (qj_proc.cpp:310-314).
+                        (void)mcg_next(s.rng, E.stride);
+                        double ev = mcg_next(s.rng, E.stride);
+                        while (ev == 0.0) ev = mcg_next(s.rng, E.stride);
+                        s.eta = ev;
+                        // the jump kernel reads the step just taken (slot cur ^ 1) and
+                        // writes the jumped state over the pre-step state (slot cur)
+                        jumped = true;
+                        jsrc = s.cur ^ 1;
+                    } else {
+                        s.cur ^= 1;
+                    }
+                    s.t += dt;
+                    complete_step(s, lvl, E.dt, E.S);
+                } else if (crossed) {
+                    s.lvl = lvl + 1;      // bisect from the pre-step state
+                } else {
+                    s.cur ^= 1;
+                    s.t += dt;
+                    complete_step(s, lvl, E.dt, E.S);
+                }
+            }
+            E.st[tr] = s;
+            if (alive && s.pos < s.target) key = e * kMaxGroups + s.b * E.D + s.lvl;
+        }
+        // appends, one atomic per distinct (ensemble, group) in the wave
+        const unsigned long long app = __ballot(key >= 0);
+        if (lane == 0 && app) atomicAdd(&a.plan->next_total[p ^ 1], __popcll(app));
+        int k2 = key;
+        for (;;) {
+            const unsigned long long m = __ballot(k2 >= 0);
+            if (!m) break;
+            const int leader = __ffsll((long long)m) - 1;
+            const int lk = __shfl(k2, leader, 64);
+            const unsigned long long same = __ballot(k2 == lk);
+            int basei = 0;
+            if (lane == leader) basei = atomicAdd(&a.ens[lk / kMaxGroups].cnt[p ^ 1][lk % kMaxGroups], __popcll(same));
+            basei = __shfl(basei, leader, 64);
+            if (k2 == lk) {
+                const Ens &E = a.ens[e];
+                const int rank = __popcll(same & ((1ULL << lane) - 1ULL));
+                E.list[p ^ 1][(size_t)(lk % kMaxGroups) * E.M + basei + rank] = tr;
+                k2 = -1;
+            }
+        }
+        // jumps: one slot reservation per wave, jump counters per ensemble
+        const unsigned long long jm = __ballot(jumped);
+        if (jm) {
+            int jb = 0;
+            if (lane == 0) jb = atomicAdd(a.jcnt, __popcll(jm));
+            jb = __shfl(jb, 0, 64);
+            if (jumped) {
+                JumpEnt je;
+                je.ens = e;
+                je.tr = tr;
+                je.src = jsrc;
+                je.pad = 0;
+                je.n = n;
+                a.jl[jb + __popcll(jm & ((1ULL << lane) - 1ULL))] = je;
+            }
+            int ek = jumped ? e : -1;
+            for (;;) {
+                const unsigned long long m = __ballot(ek >= 0);
+                if (!m) break;
+                const int leader = __ffsll((long long)m) - 1;
+                const int le = __shfl(ek, leader, 64);
+                const unsigned long long same = __ballot(ek == le);
+                if (lane == leader) atomicAdd(&a.ens[le].counters[1], (unsigned long long)__popcll(same));
+                if (ek == le) ek = -1;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// jump: recovery (experiment.cpp:1273-1382) for the single dimer dissipator, one
+// workgroup per queued jump (persistent):
+//   phi <- phi / ||phi||;  v = L phi;  phi <- v / sqrt(||v||^2)      (FP64)
+// L is tridiagonal: L_kk = Ldiag[k], L_{k+1,k} = -Loff[k], L_{k,k+1} = +Loff[k];
+// v_k sums its three terms in ascending column order.
+// ---------------------------------------------------------------------------
+constexpr int kJumpThreads = 256;
+constexpr int kJumpPer = kMaxDim / kJumpThreads;
+
+template <typename T>
+__global__ __launch_bounds__(kJumpThreads) void jump_kernel(RunArgs a)
+{
+    __shared__ double pre[kMaxDim + 2];
+    __shared__ double pim[kMaxDim + 2];
+    __shared__ double red[kJumpThreads / 64];
+    const int njump = *a.jcnt;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    for (int j = blockIdx.x; j < njump; j += gridDim.x) {
+        const JumpEnt je = a.jl[j];
+        const Ens &E = a.ens[je.ens];
+        const int d = E.d, dp = E.dp;
+        const double sn = sqrt(je.n);
+        const T *x = reinterpret_cast<const T *>(E.slot[je.src]) + (size_t)je.tr * E.L;
+        T *y = reinterpret_cast<T *>(E.slot[je.src ^ 1]) + (size_t)je.tr * E.L;
+        for (int k = tid; k < d; k += kJumpThreads) {
+            pre[k + 1] = (double)x[k] / sn;
+            pim[k + 1] = (double)x[dp + k] / sn;
+        }
+        if (tid == 0) {
+            pre[0] = pim[0] = 0.0;
+            pre[d + 1] = pim[d + 1] = 0.0;
+        }
+        __syncthreads();
+        double vre[kJumpPer], vim[kJumpPer];
+        double p = 0.0;
+#pragma unroll
+        for (int m = 0; m < kJumpPer; m++) {
+            const int k = tid + m * kJumpThreads;
+            vre[m] = vim[m] = 0.0;
+            if (k < d) {
+                double rr = 0.0, ii = 0.0;
+                if (k > 0) {
+                    const double l = -E.Loff[k - 1];
+                    rr += l * pre[k];
+                    ii += l * pim[k];
+                }
+                rr += E.Ldiag[k] * pre[k + 1];
+                ii += E.Ldiag[k] * pim[k + 1];
+                if (k + 1 < d) {
+                    const double l = E.Loff[k];
+                    rr += l * pre[k + 2];
+                    ii += l * pim[k + 2];
+                }
+                vre[m] = rr;
+                vim[m] = ii;
+                p += rr * rr + ii * ii;
+            }
+        }
+        p = wave_sum(p);
+        if (lane == 0) red[wid] = p;
+        __syncthreads();
+        double tot = 0.0;
+#pragma unroll
+        for (int w = 0; w < kJumpThreads / 64; w++) tot += red[w];
+        const double sp = sqrt(tot / 1.0);
+#pragma unroll
+        for (int m = 0; m < kJumpPer; m++) {
+            const int k = tid + m * kJumpThreads;
+            if (k < d) {
+                y[k] = (T)(vre[m] / sp);
+                y[dp + k] = (T)(vim[m] / sp);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// seed: every trajectory sits at a top-level step boundary between runs, so it
+// starts at level 0 of its current branch; target = pos + n.
+// ---------------------------------------------------------------------------
+__global__ void seed_zero_kernel(int *cnt)
+{
+    if (threadIdx.x < kMaxGroups) cnt[threadIdx.x] = 0;
+}
+
+__global__ void seed_kernel(TrajState *st, int M, int n, int *list, int *cnt, int D)
+{
+    const int tr = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tr >= M) return;
+    TrajState s = st[tr];
+    s.target = s.pos + n;
+    st[tr] = s;
+    if (n > 0) {
+        const int g = s.b * D;
+        const int q = atomicAdd(&cnt[g], 1);
+        list[(size_t)g * M + q] = tr;
+    }
+}
+
+void launch_seed(TrajState *st, int M, int n_steps, int *list, int *cnt, int D, hipStream_t s)
+{
+    seed_zero_kernel<<<1, kMaxGroups, 0, s>>>(cnt);
+    seed_kernel<<<(M + 255) / 256, 256, 0, s>>>(st, M, n_steps, list, cnt, D);
+}
+
+void launch_plan(const RunArgs &a, hipStream_t s)
+{
+    plan_kernel<<<1, kPlanThreads, 0, s>>>(a);
+}
+
+void launch_gemm(int prec, int gemm_kind, const RunArgs &a, const StepGrid &g, hipStream_t s)
+{
+    if (gemm_kind == kGemmFma) {
+        if (prec == kFP64) gemm_fma<double><<<g.gemm, 256, 0, s>>>(a);
+        else gemm_fma<float><<<g.gemm, 256, 0, s>>>(a);
+    } else if (prec == kFP64) {
+        gemm_f64<<<g.gemm, 512, 0, s>>>(a);
+    } else if (prec == kBF16x3) {
+        gemm_bf16x3<<<g.gemm, 256, 0, s>>>(a);
+    } else {
+        gemm_f32<<<g.gemm, 256, 0, s>>>(a);
+    }
+}
+
+void launch_tail(int prec, const RunArgs &a, const StepGrid &g, hipStream_t s)
+{
+    decide_kernel<<<g.decide, 256, 0, s>>>(a);
+    if (prec == kFP64) jump_kernel<double><<<g.jump, kJumpThreads, 0, s>>>(a);
+    else jump_kernel<float><<<g.jump, kJumpThreads, 0, s>>>(a);
+}
+
+void launch_iteration(int prec, int gemm_kind, const RunArgs &a, const StepGrid &g, hipStream_t s)
+{
+    launch_plan(a, s);
+    launch_gemm(prec, gemm_kind, a, g, s);
+    launch_tail(prec, a, g, s);
+}
+
+int gemm_blocks_per_cu(int prec, int gemm_kind)
+{
+    int n = 0;
+    hipError_t e;
+    if (gemm_kind == kGemmFma)
+        e = prec == kFP64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_fma<double>, 256, 0)
+                          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_fma<float>, 256, 0);
+    else if (prec == kFP64)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_f64, 512, 0);
+    else if (prec == kBF16x3)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_bf16x3, 256, 0);
+    else
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_f32, 256, 0);
+    return (e == hipSuccess && n > 0) ? n : 1;
+}
+
+// ---------------------------------------------------------------------------
+// Lane-layout probe of the three matrix-core builtins (one wave, row-major I/O).
+// ---------------------------------------------------------------------------
+__global__ void mfma_probe_kernel(const float *A32, const float *B32, float *D32, const double *A64,
+                                  const double *B64, double *D64, const float *Abf, const float *Bbf,
+                                  float *Dbf)
+{
+    const int l = threadIdx.x;
+    floatx16 c = {};
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(A32[(l % 32) * 2 + l / 32], B32[(l / 32) * 32 + l % 32], c, 0, 0, 0);
+    for (int r = 0; r < 16; r++) D32[(8 * (r / 4) + 4 * (l / 32) + r % 4) * 32 + l % 32] = c[r];
+    doublex4 e = {0.0, 0.0, 0.0, 0.0};
+    e = __builtin_amdgcn_mfma_f64_16x16x4f64(A64[(l % 16) * 4 + l / 16], B64[(l / 16) * 16 + l % 16], e, 0, 0, 0);
+    for (int r = 0; r < 4; r++) D64[(l / 16 + 4 * r) * 16 + l % 16] = e[r];
+    bf16x8 av, bv;
+    for (int q = 0; q < 8; q++) {
+        av[q] = (__bf16)Abf[(l % 32) * 16 + 8 * (l / 32) + q];
+        bv[q] = (__bf16)Bbf[(8 * (l / 32) + q) * 32 + l % 32];
+    }
+    floatx16 cb = {};
+    cb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, cb, 0, 0, 0);
+    for (int r = 0; r < 16; r++) Dbf[(8 * (r / 4) + 4 * (l / 32) + r % 4) * 32 + l % 32] = cb[r];
+}
+
+void launch_mfma_probe(const float *A32, const float *B32, float *D32, const double *A64,
+                       const double *B64, double *D64, const float *Abf, const float *Bbf, float *Dbf,
+                       hipStream_t s)
+{
+    mfma_probe_kernel<<<1, 64, 0, s>>>(A32, B32, D32, A64, B64, D64, Abf, Bbf, Dbf);
+}
+
+}  // namespace qjx

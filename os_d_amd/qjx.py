@@ -51,6 +51,15 @@ class Stats(ctypes.Structure):
                 ("gemm_launches", ctypes.c_int64)]
 
 
+class JobResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("num_trajectories", ctypes.c_int), ("n_periods", ctypes.c_int),
+                ("dim", ctypes.c_int), ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double),
+                ("t_done", ctypes.c_double), ("matvecs", ctypes.c_int64), ("jumps", ctypes.c_int64),
+                ("mean_avg", ctypes.c_double), ("mean_std", ctypes.c_double),
+                ("energy_avg", ctypes.c_double), ("energy_std", ctypes.c_double),
+                ("norm_avg", ctypes.c_double)]
+
+
 _lib = None
 
 # (name, restype, argtypes) of every symbol declared in include/qjx.h
@@ -78,6 +87,13 @@ SIGNATURES = [
     ("qjx_mfma_probe", ctypes.c_int, [_P(ctypes.c_float), _P(ctypes.c_float), _P(ctypes.c_float), _D, _D,
                                       _D, _P(ctypes.c_float), _P(ctypes.c_float), _P(ctypes.c_float)]),
     ("qjx_run_config", ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]),
+    ("qjx_pool_create", ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int, _P(ctypes.c_void_p)]),
+    ("qjx_pool_destroy", None, [ctypes.c_void_p]),
+    ("qjx_pool_submit", ctypes.c_int, [ctypes.c_void_p, _P(ModelDesc), _P(EnsembleDesc), ctypes.c_int,
+                                       _P(ctypes.c_int)]),
+    ("qjx_pool_step", ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int), _P(ctypes.c_int)]),
+    ("qjx_pool_result", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _P(JobResult), _D, _D]),
+    ("qjx_pool_clock", ctypes.c_double, [ctypes.c_void_p]),
 ]
 
 
@@ -209,6 +225,66 @@ class Engine:
 
     def set_profiling(self, on=True):
         _check(self.lib, self.lib.qjx_set_profiling(self.h, 1 if on else 0))
+
+
+class Pool:
+    """Continuous batching of ensemble jobs on the current HIP device."""
+
+    def __init__(self, precision=FP32, hbm_budget_gib=0.0, max_trajectories=1 << 20):
+        self.lib = load_library()
+        h = ctypes.c_void_p()
+        _check(self.lib, self.lib.qjx_pool_create(precision, hbm_budget_gib, max_trajectories, ctypes.byref(h)))
+        self.h = h
+        self.sizes = {}
+
+    def close(self):
+        if self.h:
+            self.lib.qjx_pool_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, N, num_trajectories, n_periods, seed=0, mns=1000000, start_type=0, start_state=0,
+               prm_E=1.0, prm_U=0.5, prm_J=1.0, drv_ampl=1.5, drv_freq=1.0, diss_gamma=0.1, ex_deep=16,
+               steps_per_period=4):
+        m = ModelDesc(N, prm_E, prm_U, prm_J, drv_ampl, drv_freq, diss_gamma, ex_deep, steps_per_period, 0)
+        e = EnsembleDesc(num_trajectories, seed, mns, start_type, start_state, 0, 0)
+        jid = ctypes.c_int()
+        _check(self.lib, self.lib.qjx_pool_submit(self.h, ctypes.byref(m), ctypes.byref(e), n_periods,
+                                                  ctypes.byref(jid)))
+        self.sizes[jid.value] = num_trajectories
+        return jid.value
+
+    def step(self):
+        r, q = ctypes.c_int(), ctypes.c_int()
+        _check(self.lib, self.lib.qjx_pool_step(self.h, ctypes.byref(r), ctypes.byref(q)))
+        return r.value, q.value
+
+    def clock(self):
+        return self.lib.qjx_pool_clock(self.h)
+
+    def result(self, jid, arrays=False):
+        res = JobResult()
+        M = self.sizes[jid]
+        mean = np.zeros(M) if arrays else None
+        energy = np.zeros(M) if arrays else None
+        _check(self.lib, self.lib.qjx_pool_result(self.h, jid, ctypes.byref(res), _ptr(mean), _ptr(energy)))
+        out = {k: getattr(res, k) for k, _ in JobResult._fields_}
+        if arrays:
+            out["mean"] = mean
+            out["energy"] = energy
+        return out
+
+    def drain(self, max_cycles=1000000):
+        for _ in range(max_cycles):
+            r, q = self.step()
+            if r == 0 and q == 0:
+                return
+        raise QJXError(-1, "pool did not drain")
 
 
 def mfma_probe(A32, B32, A64, B64, Abf, Bbf):
