@@ -774,7 +774,7 @@ int runner_create(int prec, int gemm_kind, hipStream_t stream, int capacity, Run
     int g = cus * gemm_blocks_per_cu(prec, gemm_kind);
     g = std::max(8, g / 8 * 8);
     r->grid.gemm = g;
-    r->grid.decide = std::max(1, std::min((capacity + 255) / 256, cus * 8));
+    r->grid.decide = std::max(1, (capacity + 255) / 256);   // one entry per thread
     r->grid.jump = std::max(1, std::min(capacity, 2048));
     if (hipMalloc((void **)&r->d_ens, kMaxEns * sizeof(Ens)) != hipSuccess ||
         hipMalloc((void **)&r->d_nens, sizeof(int)) != hipSuccess ||

@@ -117,8 +117,18 @@ struct TileRef {
     int e, g, rt, col0, ncol;
 };
 
+// Copy the tile prefix of this iteration's plan into LDS (once per workgroup).
+__device__ inline int load_tile_pre(const RunArgs &a, int *s_pre)
+{
+    const int nitems = *a.nens * kMaxGroups;
+    for (int i = threadIdx.x; i <= nitems; i += blockDim.x) s_pre[i] = a.plan->tile_pre[i];
+    __syncthreads();
+    return nitems;
+}
+
 template <int BN>
-__device__ inline bool tile_next(const RunArgs &a, int nitems, int total, int *s_claim, TileRef &r)
+__device__ inline bool tile_next(const RunArgs &a, const int *s_pre, int nitems, int total, int *s_claim,
+                                 TileRef &r)
 {
     if (threadIdx.x == 0) {
         int t = -1;
@@ -137,8 +147,8 @@ __device__ inline bool tile_next(const RunArgs &a, int nitems, int total, int *s
     const int t = *s_claim;
     __syncthreads();
     if (t < 0) return false;
-    const int item = plan_search(a.plan->tile_pre, nitems, t);
-    const int local = t - a.plan->tile_pre[item];
+    const int item = plan_search(s_pre, nitems, t);
+    const int local = t - s_pre[item];
     r.e = item / kMaxGroups;
     r.g = item % kMaxGroups;
     const Ens &E = a.ens[r.e];
@@ -181,15 +191,16 @@ __global__ __launch_bounds__(256) void gemm_f32(RunArgs a)
     __shared__ __attribute__((aligned(16))) float As[MF_BM * F32_PITCH];
     __shared__ __attribute__((aligned(16))) float Bs[MF_BN * F32_PITCH];
     __shared__ double s_norm[2][MF_BN];
-    const int nitems = *a.nens * kMaxGroups;
-    const int total = a.plan->tile_pre[nitems];
+    __shared__ int s_tpre[kPlanItems + 1];
+    const int nitems = load_tile_pre(a, s_tpre);
+    const int total = s_tpre[nitems];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wm = wid >> 1, wn = wid & 1;
     const int h = lane >> 5, li = lane & 31;
     const int cp_k = (tid & 7) * 4, cp_r = tid >> 3;
     __shared__ int s_claim;
     TileRef r;
-    while (tile_next<MF_BN>(a, nitems, total, &s_claim, r)) {
+    while (tile_next<MF_BN>(a, s_tpre, nitems, total, &s_claim, r)) {
         load_cols<MF_BN>(a, r, s_tr, s_cur);
         __syncthreads();
         const Ens &E = a.ens[r.e];
@@ -304,8 +315,9 @@ __global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
     __shared__ __attribute__((aligned(16))) __bf16 Bh[MF_BN * BF_PITCH];
     __shared__ __attribute__((aligned(16))) __bf16 Bl[MF_BN * BF_PITCH];
     __shared__ double s_norm[2][MF_BN];
-    const int nitems = *a.nens * kMaxGroups;
-    const int total = a.plan->tile_pre[nitems];
+    __shared__ int s_tpre[kPlanItems + 1];
+    const int nitems = load_tile_pre(a, s_tpre);
+    const int total = s_tpre[nitems];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wm = wid >> 1, wn = wid & 1;
     const int h = lane >> 5, li = lane & 31;
@@ -313,7 +325,7 @@ __global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
     const int cb_k = (tid & 7) * 4, cb_r = tid >> 3;
     __shared__ int s_claim;
     TileRef r;
-    while (tile_next<MF_BN>(a, nitems, total, &s_claim, r)) {
+    while (tile_next<MF_BN>(a, s_tpre, nitems, total, &s_claim, r)) {
         load_cols<MF_BN>(a, r, s_tr, s_cur);
         __syncthreads();
         const Ens &E = a.ens[r.e];
@@ -446,15 +458,16 @@ __global__ __launch_bounds__(512) void gemm_f64(RunArgs a)
     __shared__ __attribute__((aligned(16))) double As[MF_BM * F64_PITCH];
     __shared__ __attribute__((aligned(16))) double Bs[MF_BN * F64_PITCH];
     __shared__ double s_norm[4][MF_BN];
-    const int nitems = *a.nens * kMaxGroups;
-    const int total = a.plan->tile_pre[nitems];
+    __shared__ int s_tpre[kPlanItems + 1];
+    const int nitems = load_tile_pre(a, s_tpre);
+    const int total = s_tpre[nitems];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wm = wid >> 1, wn = wid & 1;
     const int qq = lane >> 4, li = lane & 15;
     const int cp_k = (tid & 15) * 2, cp_r = tid >> 4;
     __shared__ int s_claim;
     TileRef r;
-    while (tile_next<MF_BN>(a, nitems, total, &s_claim, r)) {
+    while (tile_next<MF_BN>(a, s_tpre, nitems, total, &s_claim, r)) {
         load_cols<MF_BN>(a, r, s_tr, s_cur);
         __syncthreads();
         const Ens &E = a.ens[r.e];
@@ -575,14 +588,15 @@ __global__ __launch_bounds__(256) void gemm_fma(RunArgs a)
     __shared__ T As[FM_BK][FM_BM + 4];
     __shared__ T Bs[FM_BK][FM_BN + 1];
     __shared__ double s_norm[FM_BM / 4][FM_BN + 1];
-    const int nitems = *a.nens * kMaxGroups;
-    const int total = a.plan->tile_pre[nitems];
+    __shared__ int s_tpre[kPlanItems + 1];
+    const int nitems = load_tile_pre(a, s_tpre);
+    const int total = s_tpre[nitems];
     const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
     const int a_i = tid / (FM_BK / 4), a_k = (tid % (FM_BK / 4)) * 4;
     const int b_j = tid / (FM_BK / 4), b_k = (tid % (FM_BK / 4)) * 4;
     __shared__ int s_claim;
     TileRef r;
-    while (tile_next<FM_BN>(a, nitems, total, &s_claim, r)) {
+    while (tile_next<FM_BN>(a, s_tpre, nitems, total, &s_claim, r)) {
         load_cols<FM_BN>(a, r, s_tr, s_cur);
         __syncthreads();
         const Ens &E = a.ens[r.e];
@@ -672,127 +686,129 @@ __device__ inline void complete_step(TrajState &s, int l, const double *dt, int 
     s.lvl = l;
 }
 
+// Appends are aggregated per workgroup: LDS counters for the groups of a window of
+// two ensembles (the workgroup's entries are sorted by item, so they almost always
+// fall in it), one global atomic per non-empty counter; entries outside the window
+// reserve their slot directly.
+constexpr int kWin = 2 * kMaxGroups;
+
 __global__ __launch_bounds__(256) void decide_kernel(RunArgs a)
 {
+    __shared__ int s_pre[kPlanItems + 1];
+    __shared__ int s_cnt[kWin + 3];      // [kWin] jumps, [kWin+1..2] jumps per window ensemble
+    __shared__ int s_base[kWin + 1];
+    __shared__ int s_next;
     const int nitems = *a.nens * kMaxGroups;
     const Plan *pl = a.plan;
-    const int total = pl->ent_pre[nitems];
+    for (int i = threadIdx.x; i <= nitems; i += blockDim.x) s_pre[i] = pl->ent_pre[i];
+    for (int i = threadIdx.x; i < kWin + 3; i += blockDim.x) s_cnt[i] = 0;
+    if (threadIdx.x == 0) s_next = 0;
+    __syncthreads();
+    const int total = s_pre[nitems];
+    const int base0 = blockIdx.x * blockDim.x;
+    if (base0 >= total) return;                       // whole workgroup idle
     const int p = a.parity;
-    const int lane = threadIdx.x & 63;
-    const int gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int nw = (gridDim.x * blockDim.x) >> 6;
-    for (int base = gw * 64; base < total; base += nw * 64) {
-        const int idx = base + lane;
-        int e = -1, tr = -1, key = -1, jsrc = 0;
-        bool jumped = false;
-        double n = 0.0;
-        if (idx < total) {
-            const int item = plan_search(pl->ent_pre, nitems, idx);
-            e = item / kMaxGroups;
-            const int g = item % kMaxGroups;
-            const Ens &E = a.ens[e];
-            tr = E.list[p][(size_t)g * E.M + (idx - pl->ent_pre[item])];
-            TrajState s = E.st[tr];
-            for (int rt = 0; rt < E.R; rt++) n += E.normpart[(size_t)rt * E.M + tr];
-            const int lvl = s.lvl;
-            const double dt = E.dt[lvl];
-            bool alive = true;
-            if (n > 1.0 + E.norm_tol) {
-                // experiment.cpp:1247-1251 throws: record the first offender, stop it.
-                if (atomicCAS(&E.err[0], 0, 1) == 0) {
-                    E.err[1] = tr;
-                    const unsigned long long bits = (unsigned long long)__double_as_longlong(n);
-                    E.err[2] = (int)(bits & 0xffffffffULL);
-                    E.err[3] = (int)(bits >> 32);
-                }
-                alive = false;
-            } else {
-                const bool crossed = !(n > s.eta);          // experiment.cpp:1253-1260
-                if (lvl == E.D - 1) {
-                    if (crossed) {
-                        if (E.jump > 0 && E.is_obs == 1) {
-                            if (s.logn < E.log.cap) {
-                                const size_t o = (size_t)tr * E.log.cap + s.logn;
-                                E.log.time[o] = s.t + dt;
-                                E.log.norm[o] = n;
-                                E.log.eta[o] = s.eta;
-                                E.log.type[o] = 0;
-                            }
-                            s.logn++;
-                            s.njumps++;
+    const int idx = base0 + threadIdx.x;
+    const int e_lo = plan_search(s_pre, nitems, base0) / kMaxGroups;
+    int e = -1, tr = -1, key = -1, local = -1, direct = -1, jlocal = -1, jsrc = 0;
+    double n = 0.0;
+    if (idx < total) {
+        const int item = plan_search(s_pre, nitems, idx);
+        e = item / kMaxGroups;
+        const int g = item % kMaxGroups;
+        const Ens &E = a.ens[e];
+        tr = E.list[p][(size_t)g * E.M + (idx - s_pre[item])];
+        TrajState s = E.st[tr];
+        for (int rt = 0; rt < E.R; rt++) n += E.normpart[(size_t)rt * E.M + tr];
+        const int lvl = s.lvl;
+        const double dt = E.dt[lvl];
+        bool alive = true;
+        if (n > 1.0 + E.norm_tol) {
+            // experiment.cpp:1247-1251 throws: record the first offender, stop it.
+            if (atomicCAS(&E.err[0], 0, 1) == 0) {
+                E.err[1] = tr;
+                const unsigned long long bits = (unsigned long long)__double_as_longlong(n);
+                E.err[2] = (int)(bits & 0xffffffffULL);
+                E.err[3] = (int)(bits >> 32);
+            }
+            alive = false;
+        } else {
+            const bool crossed = !(n > s.eta);              // experiment.cpp:1253-1260
+            if (lvl == E.D - 1) {
+                if (crossed) {
+                    if (E.jump > 0 && E.is_obs == 1) {
+                        if (s.logn < E.log.cap) {
+                            const size_t o = (size_t)tr * E.log.cap + s.logn;
+                            E.log.time[o] = s.t + dt;
+                            E.log.norm[o] = n;
+                            E.log.eta[o] = s.eta;
+                            E.log.type[o] = 0;
                         }
-                        // recovery draws `ran` (one channel, so channel 0 is chosen),
-                        // then eta is redrawn while it is 0 (qj_proc.cpp:310-314).
-                        (void)mcg_next(s.rng, E.stride);
-                        double ev = mcg_next(s.rng, E.stride);
-                        while (ev == 0.0) ev = mcg_next(s.rng, E.stride);
-                        s.eta = ev;
-                        // the jump kernel reads the step just taken (slot cur ^ 1) and
-                        // writes the jumped state over the pre-step state (slot cur)
-                        jumped = true;
-                        jsrc = s.cur ^ 1;
-                    } else {
-                        s.cur ^= 1;
+                        s.logn++;
+                        s.njumps++;
                     }
-                    s.t += dt;
-                    complete_step(s, lvl, E.dt, E.S);
-                } else if (crossed) {
-                    s.lvl = lvl + 1;      // bisect from the pre-step state
+                    // recovery draws `ran` (one channel, so channel 0 is chosen),
+                    // then eta is redrawn while it is 0 (qj_proc.cpp:310-314).
+                    (void)mcg_next(s.rng, E.stride);
+                    double ev = mcg_next(s.rng, E.stride);
+                    while (ev == 0.0) ev = mcg_next(s.rng, E.stride);
+                    s.eta = ev;
+                    // the jump kernel reads the step just taken (slot cur ^ 1) and
+                    // writes the jumped state over the pre-step state (slot cur)
+                    jsrc = s.cur ^ 1;
+                    jlocal = atomicAdd(&s_cnt[kWin], 1);
+                    if (e - e_lo < 2) atomicAdd(&s_cnt[kWin + 1 + (e - e_lo)], 1);
+                    else atomicAdd(&E.counters[1], 1ULL);
                 } else {
                     s.cur ^= 1;
-                    s.t += dt;
-                    complete_step(s, lvl, E.dt, E.S);
                 }
-            }
-            E.st[tr] = s;
-            if (alive && s.pos < s.target) key = e * kMaxGroups + s.b * E.D + s.lvl;
-        }
-        // appends, one atomic per distinct (ensemble, group) in the wave
-        const unsigned long long app = __ballot(key >= 0);
-        if (lane == 0 && app) atomicAdd(&a.plan->next_total[p ^ 1], __popcll(app));
-        int k2 = key;
-        for (;;) {
-            const unsigned long long m = __ballot(k2 >= 0);
-            if (!m) break;
-            const int leader = __ffsll((long long)m) - 1;
-            const int lk = __shfl(k2, leader, 64);
-            const unsigned long long same = __ballot(k2 == lk);
-            int basei = 0;
-            if (lane == leader) basei = atomicAdd(&a.ens[lk / kMaxGroups].cnt[p ^ 1][lk % kMaxGroups], __popcll(same));
-            basei = __shfl(basei, leader, 64);
-            if (k2 == lk) {
-                const Ens &E = a.ens[e];
-                const int rank = __popcll(same & ((1ULL << lane) - 1ULL));
-                E.list[p ^ 1][(size_t)(lk % kMaxGroups) * E.M + basei + rank] = tr;
-                k2 = -1;
+                s.t += dt;
+                complete_step(s, lvl, E.dt, E.S);
+            } else if (crossed) {
+                s.lvl = lvl + 1;          // bisect from the pre-step state
+            } else {
+                s.cur ^= 1;
+                s.t += dt;
+                complete_step(s, lvl, E.dt, E.S);
             }
         }
-        // jumps: one slot reservation per wave, jump counters per ensemble
-        const unsigned long long jm = __ballot(jumped);
-        if (jm) {
-            int jb = 0;
-            if (lane == 0) jb = atomicAdd(a.jcnt, __popcll(jm));
-            jb = __shfl(jb, 0, 64);
-            if (jumped) {
-                JumpEnt je;
-                je.ens = e;
-                je.tr = tr;
-                je.src = jsrc;
-                je.pad = 0;
-                je.n = n;
-                a.jl[jb + __popcll(jm & ((1ULL << lane) - 1ULL))] = je;
-            }
-            int ek = jumped ? e : -1;
-            for (;;) {
-                const unsigned long long m = __ballot(ek >= 0);
-                if (!m) break;
-                const int leader = __ffsll((long long)m) - 1;
-                const int le = __shfl(ek, leader, 64);
-                const unsigned long long same = __ballot(ek == le);
-                if (lane == leader) atomicAdd(&a.ens[le].counters[1], (unsigned long long)__popcll(same));
-                if (ek == le) ek = -1;
-            }
+        E.st[tr] = s;
+        if (alive && s.pos < s.target) {
+            const int gn = s.b * E.D + s.lvl;
+            key = (e - e_lo) * kMaxGroups + gn;
+            atomicAdd(&s_next, 1);
+            if (e - e_lo < 2) local = atomicAdd(&s_cnt[key], 1);
+            else direct = atomicAdd(&E.cnt[p ^ 1][gn], 1);
         }
+    }
+    __syncthreads();
+    const int nens = *a.nens;
+    for (int t = threadIdx.x; t < kWin; t += blockDim.x) {
+        const int c = s_cnt[t];
+        const int ee = e_lo + t / kMaxGroups;
+        s_base[t] = (c && ee < nens) ? atomicAdd(&a.ens[ee].cnt[p ^ 1][t % kMaxGroups], c) : 0;
+    }
+    if (threadIdx.x == 0) {
+        s_base[kWin] = s_cnt[kWin] ? atomicAdd(a.jcnt, s_cnt[kWin]) : 0;
+        if (s_next) atomicAdd(&a.plan->next_total[p ^ 1], s_next);
+        for (int w = 0; w < 2; w++)
+            if (s_cnt[kWin + 1 + w]) atomicAdd(&a.ens[e_lo + w].counters[1], (unsigned long long)s_cnt[kWin + 1 + w]);
+    }
+    __syncthreads();
+    if (key >= 0) {
+        const Ens &E = a.ens[e];
+        const int gn = key % kMaxGroups;
+        const int pos = local >= 0 ? s_base[key] + local : direct;
+        E.list[p ^ 1][(size_t)gn * E.M + pos] = tr;
+    }
+    if (jlocal >= 0) {
+        JumpEnt je;
+        je.ens = e;
+        je.tr = tr;
+        je.src = jsrc;
+        je.pad = 0;
+        je.n = n;
+        a.jl[s_base[kWin] + jlocal] = je;
     }
 }
 
