@@ -188,11 +188,11 @@ __global__ __launch_bounds__(256) void gemm_f32(RunArgs a)
 {
     __shared__ int s_tr[MF_BN];
     __shared__ int s_cur[MF_BN];
-    __shared__ __attribute__((aligned(16))) float As[MF_BM * F32_PITCH];
-    __shared__ __attribute__((aligned(16))) float Bs[MF_BN * F32_PITCH];
+    __shared__ __attribute__((aligned(16))) float As[2][MF_BM * F32_PITCH];   // double-buffered
+    __shared__ __attribute__((aligned(16))) float Bs[2][MF_BN * F32_PITCH];
     __shared__ double s_norm[2][MF_BN];
-    __shared__ int s_tpre[kPlanItems + 1];
-    const int nitems = load_tile_pre(a, s_tpre);
+    const int *s_tpre = a.plan->tile_pre;     // read through L2: one search per tile
+    const int nitems = *a.nens * kMaxGroups;
     const int total = s_tpre[nitems];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wm = wid >> 1, wn = wid & 1;
@@ -229,25 +229,26 @@ __global__ __launch_bounds__(256) void gemm_f32(RunArgs a)
                 rb[q] = *reinterpret_cast<const floatx4 *>(bcol[q] + k0);
             }
         };
-        auto lstore = [&]() {
+        auto lstore = [&](int buf) {
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int rr = cp_r + 32 * q;
-                *reinterpret_cast<floatx4 *>(&As[rr * F32_PITCH + cp_k]) = ra[q];
-                *reinterpret_cast<floatx4 *>(&Bs[rr * F32_PITCH + cp_k]) = rb[q];
+                *reinterpret_cast<floatx4 *>(&As[buf][rr * F32_PITCH + cp_k]) = ra[q];
+                *reinterpret_cast<floatx4 *>(&Bs[buf][rr * F32_PITCH + cp_k]) = rb[q];
             }
         };
         gload(0);
-        lstore();
+        lstore(0);
         __syncthreads();
+        int cur = 0;
         for (int k0 = 0; k0 < L; k0 += MF_BK) {
             const bool more = k0 + MF_BK < L;
             if (more) gload(k0 + MF_BK);
             floatx4 fa[2][4], fb[2][4];
 #pragma unroll
             for (int i = 0; i < 2; i++) {
-                const float *pa = &As[(wm * 64 + i * 32 + li) * F32_PITCH + 16 * h];
-                const float *pb = &Bs[(wn * 64 + i * 32 + li) * F32_PITCH + 16 * h];
+                const float *pa = &As[cur][(wm * 64 + i * 32 + li) * F32_PITCH + 16 * h];
+                const float *pb = &Bs[cur][(wn * 64 + i * 32 + li) * F32_PITCH + 16 * h];
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
                     fa[i][v] = *reinterpret_cast<const floatx4 *>(pa + 4 * v);
@@ -262,11 +263,10 @@ __global__ __launch_bounds__(256) void gemm_f32(RunArgs a)
                     for (int j = 0; j < 2; j++)
                         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[i][s >> 2][s & 3],
                                                                           fb[j][s >> 2][s & 3], acc[i][j], 0, 0, 0);
+            // the other buffer was last read before the previous barrier
+            if (more) lstore(cur ^ 1);
             __syncthreads();
-            if (more) {
-                lstore();
-                __syncthreads();
-            }
+            cur ^= 1;
         }
         double ns[2] = {0.0, 0.0};
 #pragma unroll
@@ -302,21 +302,29 @@ __global__ __launch_bounds__(256) void gemm_f32(RunArgs a)
 // lo = bf16(x - hi)) for both operands, U X ~= Uhi Xhi + Uhi Xlo + Ulo Xhi.
 // v_mfma_f32_32x32x16_bf16: lane l holds 8 consecutive k of row / column l % 32
 // (lane half h = l / 32 selects which 8); D as in the FP32 form.  The propagator
-// is split when packed, the state tile when staged into LDS.
+// is split when packed, the state tile when staged into LDS.  LDS rows are 64 bytes
+// (32 bf16) with the 16-byte chunk index XOR-swizzled by (row / 4) % 4, which keeps
+// the fragment reads conflict-free; tiles are double-buffered.
 // ---------------------------------------------------------------------------
-constexpr int BF_PITCH = MF_BK + 8;
+constexpr int BF_ROW = MF_BK;          // bf16 per LDS row, no padding
+
+__device__ inline int bf_off(int row, int k)   // element offset of (row, k), k < 32
+{
+    const int chunk = (k >> 3) ^ ((row >> 2) & 3);
+    return row * BF_ROW + chunk * 8 + (k & 7);
+}
 
 __global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
 {
     __shared__ int s_tr[MF_BN];
     __shared__ int s_cur[MF_BN];
-    __shared__ __attribute__((aligned(16))) __bf16 Ah[MF_BM * BF_PITCH];
-    __shared__ __attribute__((aligned(16))) __bf16 Al[MF_BM * BF_PITCH];
-    __shared__ __attribute__((aligned(16))) __bf16 Bh[MF_BN * BF_PITCH];
-    __shared__ __attribute__((aligned(16))) __bf16 Bl[MF_BN * BF_PITCH];
+    __shared__ __attribute__((aligned(16))) __bf16 Ah[2][MF_BM * BF_ROW];
+    __shared__ __attribute__((aligned(16))) __bf16 Al[2][MF_BM * BF_ROW];
+    __shared__ __attribute__((aligned(16))) __bf16 Bh[2][MF_BN * BF_ROW];
+    __shared__ __attribute__((aligned(16))) __bf16 Bl[2][MF_BN * BF_ROW];
     __shared__ double s_norm[2][MF_BN];
-    __shared__ int s_tpre[kPlanItems + 1];
-    const int nitems = load_tile_pre(a, s_tpre);
+    const int *s_tpre = a.plan->tile_pre;
+    const int nitems = *a.nens * kMaxGroups;
     const int total = s_tpre[nitems];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wm = wid >> 1, wn = wid & 1;
@@ -358,12 +366,12 @@ __global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
 #pragma unroll
             for (int q = 0; q < 4; q++) rb[q] = *reinterpret_cast<const floatx4 *>(bcol[q] + k0);
         };
-        auto lstore = [&]() {
+        auto lstore = [&](int buf) {
 #pragma unroll
             for (int q = 0; q < 2; q++) {
                 const int rr = ca_r + 64 * q;
-                *reinterpret_cast<bf16x8 *>(&Ah[rr * BF_PITCH + ca_k]) = rah[q];
-                *reinterpret_cast<bf16x8 *>(&Al[rr * BF_PITCH + ca_k]) = ral[q];
+                *reinterpret_cast<bf16x8 *>(&Ah[buf][bf_off(rr, ca_k)]) = rah[q];
+                *reinterpret_cast<bf16x8 *>(&Al[buf][bf_off(rr, ca_k)]) = ral[q];
             }
 #pragma unroll
             for (int q = 0; q < 4; q++) {
@@ -376,13 +384,14 @@ __global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
                     h4[e] = hb;
                     l4[e] = (__bf16)(x - (float)hb);
                 }
-                *reinterpret_cast<bf16x4 *>(&Bh[rr * BF_PITCH + cb_k]) = h4;
-                *reinterpret_cast<bf16x4 *>(&Bl[rr * BF_PITCH + cb_k]) = l4;
+                *reinterpret_cast<bf16x4 *>(&Bh[buf][bf_off(rr, cb_k)]) = h4;
+                *reinterpret_cast<bf16x4 *>(&Bl[buf][bf_off(rr, cb_k)]) = l4;
             }
         };
         gload(0);
-        lstore();
+        lstore(0);
         __syncthreads();
+        int cur = 0;
         for (int k0 = 0; k0 < L; k0 += MF_BK) {
             const bool more = k0 + MF_BK < L;
             if (more) gload(k0 + MF_BK);
@@ -391,12 +400,12 @@ __global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
                 bf16x8 fah[2], fal[2], fbh[2], fbl[2];
 #pragma unroll
                 for (int i = 0; i < 2; i++) {
-                    const int oa = (wm * 64 + i * 32 + li) * BF_PITCH + 16 * c + 8 * h;
-                    const int ob = (wn * 64 + i * 32 + li) * BF_PITCH + 16 * c + 8 * h;
-                    fah[i] = *reinterpret_cast<const bf16x8 *>(&Ah[oa]);
-                    fal[i] = *reinterpret_cast<const bf16x8 *>(&Al[oa]);
-                    fbh[i] = *reinterpret_cast<const bf16x8 *>(&Bh[ob]);
-                    fbl[i] = *reinterpret_cast<const bf16x8 *>(&Bl[ob]);
+                    const int oa = bf_off(wm * 64 + i * 32 + li, 16 * c + 8 * h);
+                    const int ob = bf_off(wn * 64 + i * 32 + li, 16 * c + 8 * h);
+                    fah[i] = *reinterpret_cast<const bf16x8 *>(&Ah[cur][oa]);
+                    fal[i] = *reinterpret_cast<const bf16x8 *>(&Al[cur][oa]);
+                    fbh[i] = *reinterpret_cast<const bf16x8 *>(&Bh[cur][ob]);
+                    fbl[i] = *reinterpret_cast<const bf16x8 *>(&Bl[cur][ob]);
                 }
 #pragma unroll
                 for (int i = 0; i < 2; i++)
@@ -407,11 +416,9 @@ __global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
                         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fah[i], fbh[j], acc[i][j], 0, 0, 0);
                     }
             }
+            if (more) lstore(cur ^ 1);
             __syncthreads();
-            if (more) {
-                lstore();
-                __syncthreads();
-            }
+            cur ^= 1;
         }
         double ns[2] = {0.0, 0.0};
 #pragma unroll
@@ -458,8 +465,8 @@ __global__ __launch_bounds__(512) void gemm_f64(RunArgs a)
     __shared__ __attribute__((aligned(16))) double As[MF_BM * F64_PITCH];
     __shared__ __attribute__((aligned(16))) double Bs[MF_BN * F64_PITCH];
     __shared__ double s_norm[4][MF_BN];
-    __shared__ int s_tpre[kPlanItems + 1];
-    const int nitems = load_tile_pre(a, s_tpre);
+    const int *s_tpre = a.plan->tile_pre;     // read through L2: one search per tile
+    const int nitems = *a.nens * kMaxGroups;
     const int total = s_tpre[nitems];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wm = wid >> 1, wn = wid & 1;
@@ -502,6 +509,7 @@ __global__ __launch_bounds__(512) void gemm_f64(RunArgs a)
                 *reinterpret_cast<double2v *>(&Bs[rr * F64_PITCH + cp_k]) = rb[q];
             }
         };
+        // (single-buffered: at one workgroup per CU a second buffer measured slower)
         const double *pa0 = &As[(wm * 32 + li) * F64_PITCH + 8 * qq];
         const double *pb0 = &Bs[(wn * 64 + li) * F64_PITCH + 8 * qq];
         gload(0);
@@ -588,8 +596,8 @@ __global__ __launch_bounds__(256) void gemm_fma(RunArgs a)
     __shared__ T As[FM_BK][FM_BM + 4];
     __shared__ T Bs[FM_BK][FM_BN + 1];
     __shared__ double s_norm[FM_BM / 4][FM_BN + 1];
-    __shared__ int s_tpre[kPlanItems + 1];
-    const int nitems = load_tile_pre(a, s_tpre);
+    const int *s_tpre = a.plan->tile_pre;     // read through L2: one search per tile
+    const int nitems = *a.nens * kMaxGroups;
     const int total = s_tpre[nitems];
     const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
     const int a_i = tid / (FM_BK / 4), a_k = (tid % (FM_BK / 4)) * 4;
