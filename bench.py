@@ -103,6 +103,66 @@ def cpu_baseline(engine, N, S, D, budget_s):
             "matvecs_per_traj_step": o.num_matvecs() / (ntraj * nsteps)}
 
 
+SERVE_MIXES = {
+    # the serving configuration as specified
+    "specified": {"N": [100, 250, 500, 1000, 2000], "logM": (3.0, 5.0), "periods": (50, 500)},
+    # same arrival process, jobs small enough for the offered load to fit the device
+    "scaled": {"N": [100, 250, 500], "logM": (2.0, 3.0), "periods": (1, 5)},
+}
+
+
+def serving(qjx, seconds, rate, seed, budget_gib, capacity, mix="specified"):
+    """Poisson stream of ensemble jobs through the pool (continuous batching).
+
+    Each job is a dimer parameter point: N from the mix, trajectories log-uniform,
+    periods uniform, A in [0.5, 3.5], U in [0.1, 1.0], gamma in [0.01, 0.5]; the
+    reference's QJ tree (4 top-level steps per period, ex_deep 16); FP32 form.
+    Arrivals stop after `seconds`; latency = completion - submission of the jobs
+    completed within the window."""
+    import numpy as np
+    spec = SERVE_MIXES[mix]
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=int(rate * seconds * 2) + 16))
+    arrivals = arrivals[arrivals < seconds]
+    pool = qjx.Pool(qjx.FP32, hbm_budget_gib=budget_gib, max_trajectories=capacity)
+    jobs = []
+    t0 = pool.clock()
+    nxt = 0
+    while True:
+        now = pool.clock() - t0
+        while nxt < len(arrivals) and arrivals[nxt] <= now:
+            N = int(rng.choice(spec["N"]))
+            M = int(round(10 ** rng.uniform(*spec["logM"])))
+            P = int(rng.integers(spec["periods"][0], spec["periods"][1] + 1))
+            jid = pool.submit(N, M, P, seed=int(rng.integers(0, 1 << 20)), drv_ampl=float(rng.uniform(0.5, 3.5)),
+                              prm_U=float(rng.uniform(0.1, 1.0)), diss_gamma=float(rng.uniform(0.01, 0.5)),
+                              steps_per_period=4, ex_deep=16)
+            jobs.append((jid, N, M, P))
+            nxt += 1
+        if now >= seconds:
+            break
+        r, q = pool.step()
+        if r == 0 and q == 0 and nxt >= len(arrivals):
+            break
+    t_end = pool.clock() - t0
+    done, lat, tp, admitted = 0, [], 0.0, 0
+    for jid, N, M, P in jobs:
+        res = pool.result(jid)
+        if res["status"] != 0:
+            admitted += 1
+        if res["status"] == 2:
+            done += 1
+            lat.append(res["t_done"] - res["t_submit"])
+            tp += M * P
+    pool.close()
+    lat = sorted(lat)
+    pct = lambda p: lat[min(len(lat) - 1, int(p * len(lat)))] if lat else None
+    return {"mix": mix, "jobs": {"N": spec["N"], "trajectories_log10": spec["logM"], "periods": spec["periods"]},
+            "offered_jobs_per_s": rate, "window_s": t_end, "submitted": len(jobs), "admitted": admitted,
+            "completed": done, "p50_latency_s": pct(0.5), "p99_latency_s": pct(0.99),
+            "completed_trajectory_periods_per_s": tp / t_end if t_end > 0 else 0.0}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +177,10 @@ def main():
     ap.add_argument("--no-bf16x3", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=30.0, help="serving leg window (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-budget-gib", type=float, default=0.0)
+    ap.add_argument("--serve-capacity", type=int, default=2000000)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (read + write), the "
                          "calibration kernel for the FETCH_SIZE / WRITE_SIZE counters")
@@ -159,6 +223,16 @@ def main():
                              diss_gamma=0.1, ex_deep=D, steps_per_period=S, precision=qjx.FP64)
         cpu = cpu_baseline(e64, N, S, D, args.cpu_seconds)
 
+    serve = None
+    if args.serve_seconds > 0:
+        if e64 is not None:
+            e64.close()
+            e64 = None
+        serve = {"specified": serving(qjx, args.serve_seconds, args.serve_rate, 12345, args.serve_budget_gib,
+                                      args.serve_capacity, "specified"),
+                 "scaled": serving(qjx, args.serve_seconds, args.serve_rate, 12345, args.serve_budget_gib,
+                                   args.serve_capacity, "scaled")}
+
     launches = max(1, r32["gemm_launches"])
     avg_ms = r32["gemm_ms"] / launches
     per_col = measured_traffic_per_column()
@@ -199,6 +273,9 @@ def main():
         "bf16x3_over_fp64": (bf["tp_per_s"] / fp64["tp_per_s"]) if (bf and fp64) else None,
         "bf16x3_gemm_tflops_equiv": (bf["gemm_flops"] / (bf["gemm_ms"] * 1e-3) / 1e12) if bf else None,
         "cpu_baseline": cpu,
+        "p99_job_latency_s": serve["scaled"]["p99_latency_s"] if serve else None,
+        "p99_job_latency_note": "from the scaled job mix (the specified mix completes no job within the window)",
+        "serving": serve,
     }
     print(json.dumps(line), flush=True)
 

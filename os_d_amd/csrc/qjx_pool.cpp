@@ -13,11 +13,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/qjx.h"
@@ -63,6 +67,15 @@ struct qjx_pool {
     std::deque<int> queue;             // waiting for admission
     std::vector<int> running;          // ids in the table order
     bool table_dirty = false;
+    // model builds run on a builder thread so that running jobs keep iterating
+    std::thread builder;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<int> to_build;          // admitted (budget reserved), not yet built
+    std::deque<int> built;             // built and seeded, waiting to join the table
+    int building = 0;                  // jobs handed to the builder and not yet in `built`
+    bool stop = false;
+    int device = 0;
     std::chrono::steady_clock::time_point t0;
     double now() const
     {
@@ -73,6 +86,34 @@ struct qjx_pool {
 extern "C" double qjx_pool_clock(const qjx_pool *p)
 {
     return p ? p->now() : 0.0;
+}
+
+static int admit(qjx_pool *p, Job *j);
+
+static void builder_main(qjx_pool *p)
+{
+    (void)hipSetDevice(p->device);
+    for (;;) {
+        int id;
+        Job *j;
+        {
+            std::unique_lock<std::mutex> lk(p->mu);
+            p->cv.wait(lk, [&] { return p->stop || !p->to_build.empty(); });
+            if (p->stop) return;
+            id = p->to_build.front();
+            p->to_build.pop_front();
+            j = p->jobs[id];          // `jobs` grows under the same lock
+        }
+        const int rc = admit(p, j);
+        if (rc) {
+            j->err = rc;
+            j->msg = qjx_last_error();
+            if (j->ctx) qjx_ctx_destroy(j->ctx);
+            j->ctx = nullptr;
+        }
+        std::lock_guard<std::mutex> lk(p->mu);
+        p->built.push_back(id);
+    }
 }
 
 extern "C" int qjx_pool_create(int precision, double hbm_budget_gib, int max_trajectories, qjx_pool **out)
@@ -104,6 +145,8 @@ extern "C" int qjx_pool_create(int precision, double hbm_budget_gib, int max_tra
     }
     if (const char *env = std::getenv("QJX_GRAPH"))
         if (std::string(env) == "0") p->runner->graphs = false;
+    (void)hipGetDevice(&p->device);
+    p->builder = std::thread(builder_main, p);
     *out = p;
     return QJX_OK;
 }
@@ -111,6 +154,12 @@ extern "C" int qjx_pool_create(int precision, double hbm_budget_gib, int max_tra
 extern "C" void qjx_pool_destroy(qjx_pool *p)
 {
     if (!p) return;
+    {
+        std::lock_guard<std::mutex> lk(p->mu);
+        p->stop = true;
+    }
+    p->cv.notify_all();
+    if (p->builder.joinable()) p->builder.join();
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     for (Job *j : p->jobs) {
         if (j->ctx) qjx_ctx_destroy(j->ctx);
@@ -141,15 +190,19 @@ extern "C" int qjx_pool_submit(qjx_pool *p, const qjx_model_desc *m, const qjx_e
         return pool_fail(QJX_ERR_NOMEM, "job footprint exceeds the pool's HBM budget");
     }
     j->t_submit = p->now();
+    std::lock_guard<std::mutex> lk(p->mu);
+    j->id = (int)p->jobs.size();
     p->jobs.push_back(j);
     p->queue.push_back(j->id);
     *job_id = j->id;
     return QJX_OK;
 }
 
-// Build the job's model and ensemble (own stream), draw eta_0 and seed its run.
+// Build the job's model and ensemble (own stream), draw eta_0 and seed its run
+// (builder thread).
 static int admit(qjx_pool *p, Job *j)
 {
+    (void)p;
     int rc = qjx::ctx_create_kind(&j->m, p->gemm_kind, &j->ctx);
     if (rc) return rc;
     rc = qjx_ensemble_init(j->ctx, &j->e);
@@ -189,18 +242,14 @@ static int retire(qjx_pool *p, Job *j)
     qjx_ctx_destroy(c);
     j->ctx = nullptr;
     p->used -= j->footprint;
+    p->in_table -= j->e.num_trajectories;
     return QJX_OK;
 }
 
 static int sync_table(qjx_pool *p)
 {
     std::vector<qjx::Ens> tab;
-    int tot = 0;
-    for (int id : p->running) {
-        tab.push_back(p->jobs[id]->ctx->desc);
-        tot += p->jobs[id]->ctx->M;
-    }
-    p->in_table = tot;
+    for (int id : p->running) tab.push_back(p->jobs[id]->ctx->desc);
     p->table_dirty = false;
     return qjx::runner_set_table(p->runner, tab.data(), (int)tab.size());
 }
@@ -214,29 +263,37 @@ extern "C" int qjx_pool_step(qjx_pool *p, int *running, int *queued)
     if (p->table_dirty && (rc = sync_table(p))) return rc;
     const bool busy = !p->running.empty();
     if (busy && (rc = qjx::runner_chunk(p->runner, 16))) return rc;
-    // admission (overlaps the chunk just launched)
-    while (!p->queue.empty() && (int)p->running.size() < qjx::kMaxEns) {
-        Job *j = p->jobs[p->queue.front()];
-        if (p->used + j->footprint > p->budget) break;
-        if (p->in_table + j->e.num_trajectories > p->capacity) break;
-        p->queue.pop_front();
-        p->used += j->footprint;
-        j->t_admit = p->now();
-        rc = admit(p, j);
-        if (rc) {
-            j->state = kFailed;
-            j->err = rc;
-            j->msg = qjx_last_error();
-            if (j->ctx) qjx_ctx_destroy(j->ctx);
-            j->ctx = nullptr;
-            p->used -= j->footprint;
-            continue;
+    // built jobs join the table at the next cycle; admission reserves budget and
+    // hands the build to the builder thread (it overlaps the chunk just launched)
+    {
+        std::lock_guard<std::mutex> lk(p->mu);
+        while (!p->built.empty()) {
+            Job *j = p->jobs[p->built.front()];
+            p->built.pop_front();
+            p->building--;
+            if (j->err) {
+                j->state = kFailed;
+                p->used -= j->footprint;
+                p->in_table -= j->e.num_trajectories;
+                continue;
+            }
+            j->state = kRunning;
+            p->running.push_back(j->id);
+            p->table_dirty = true;
         }
-        j->state = kRunning;
-        p->running.push_back(j->id);
-        p->in_table += j->e.num_trajectories;
-        p->table_dirty = true;
+        while (!p->queue.empty() && (int)p->running.size() + p->building < qjx::kMaxEns) {
+            Job *j = p->jobs[p->queue.front()];
+            if (p->used + j->footprint > p->budget) break;
+            if (p->in_table + j->e.num_trajectories > p->capacity) break;
+            p->queue.pop_front();
+            p->used += j->footprint;
+            p->in_table += j->e.num_trajectories;
+            j->t_admit = p->now();
+            p->to_build.push_back(j->id);
+            p->building++;
+        }
     }
+    p->cv.notify_one();
     if (busy) {
         if (hipStreamSynchronize(p->stream) != hipSuccess) return pool_fail(QJX_ERR_HIP, "pool iteration failed");
         // a job is finished when its lists for the next iteration are empty
@@ -260,7 +317,10 @@ extern "C" int qjx_pool_step(qjx_pool *p, int *running, int *queued)
         p->running.swap(still);
     }
     if (running) *running = (int)p->running.size();
-    if (queued) *queued = (int)p->queue.size();
+    if (queued) {
+        std::lock_guard<std::mutex> lk(p->mu);
+        *queued = (int)p->queue.size() + p->building;
+    }
     return QJX_OK;
 }
 

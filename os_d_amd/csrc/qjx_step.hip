@@ -708,13 +708,13 @@ __global__ __launch_bounds__(256) void decide_kernel(RunArgs a)
     __shared__ int s_next;
     const int nitems = *a.nens * kMaxGroups;
     const Plan *pl = a.plan;
+    const int total = pl->ent_pre[nitems];
+    const int base0 = blockIdx.x * blockDim.x;
+    if (base0 >= total) return;                       // whole workgroup idle
     for (int i = threadIdx.x; i <= nitems; i += blockDim.x) s_pre[i] = pl->ent_pre[i];
     for (int i = threadIdx.x; i < kWin + 3; i += blockDim.x) s_cnt[i] = 0;
     if (threadIdx.x == 0) s_next = 0;
     __syncthreads();
-    const int total = s_pre[nitems];
-    const int base0 = blockIdx.x * blockDim.x;
-    if (base0 >= total) return;                       // whole workgroup idle
     const int p = a.parity;
     const int idx = base0 + threadIdx.x;
     const int e_lo = plan_search(s_pre, nitems, base0) / kMaxGroups;
