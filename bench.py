@@ -103,6 +103,75 @@ def cpu_baseline(engine, N, S, D, budget_s):
             "matvecs_per_traj_step": o.num_matvecs() / (ntraj * nsteps)}
 
 
+C4_SEED = 12345
+
+
+def batched_instances(n=32, seed=C4_SEED):
+    """The batched uneven-ensembles workload: n dimer instances, N uniform on
+    {50..2000}, A ~ U[0.5, 3.5], U ~ U[0.1, 1.0], gamma ~ U[0.01, 0.5],
+    trajectories log-uniform on [1e3, 5e4]; numpy PCG64 seeded with `seed`
+    (the manifest is printed with the results)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        N = int(rng.integers(50, 2001))
+        A = float(rng.uniform(0.5, 3.5))
+        U = float(rng.uniform(0.1, 1.0))
+        g = float(rng.uniform(0.01, 0.5))
+        M = int(round(10 ** rng.uniform(3.0, np.log10(5e4))))
+        out.append({"N": N, "M": M, "drv_ampl": A, "prm_U": U, "diss_gamma": g, "seed": i})
+    return out
+
+
+def _instance_engine(qjx, p, S, D, prec):
+    e = qjx.Engine(p["N"], prm_E=1.0, prm_U=p["prm_U"], prm_J=1.0, drv_ampl=p["drv_ampl"], drv_freq=1.0,
+                   diss_gamma=p["diss_gamma"], ex_deep=D, steps_per_period=S, precision=prec)
+    e.ensemble_init(p["M"], seed=p["seed"], mns=1000000, start_type=0, start_state=0, jump=0)
+    e.trans(0)
+    return e
+
+
+def batched(qjx, S, D, warmup, steps, prec, alone):
+    """All instances in one batch: every iteration is one persistent grouped GEMM
+    over the 32 ensembles (HIP-graph replay, launch shape from the tuner).  With
+    `alone`, the same steps are also run instance by instance for comparison."""
+    inst = batched_instances()
+    t0 = time.perf_counter()
+    engines = [_instance_engine(qjx, p, S, D, prec) for p in inst]
+    build_s = time.perf_counter() - t0
+    b = qjx.Batch(engines)
+    if warmup:
+        b.run_steps(warmup, 0)
+    s0 = b.stats()
+    t0 = time.perf_counter()
+    b.run_steps(steps, 0)
+    dt = time.perf_counter() - t0
+    s1 = b.stats()
+    tp = sum(p["M"] for p in inst) * steps / S
+    res = {"instances": len(inst), "manifest_seed": C4_SEED, "trajectories": sum(p["M"] for p in inst),
+           "dims": [p["N"] + 1 for p in inst], "steps": steps, "warmup": warmup, "seconds": dt,
+           "tp_per_s": tp / dt, "matvecs": s1["matvecs"] - s0["matvecs"],
+           "effective_tflops": (s1["gemm_flops"] - s0["gemm_flops"]) / dt / 1e12,
+           "iterations": s1["iterations"] - s0["iterations"], "gemm_bn": s1["gemm_bn"],
+           "gemm_grid": s1["gemm_grid"], "build_s": build_s, "manifest": inst}
+    b.close()
+    if alone:
+        t_alone = 0.0
+        for e in engines:
+            if warmup:
+                e.run_steps(warmup, 0)
+            t0 = time.perf_counter()
+            e.run_steps(steps, 0)
+            t_alone += time.perf_counter() - t0
+        res["alone_seconds"] = t_alone
+        res["alone_tp_per_s"] = tp / t_alone
+        res["batch_speedup"] = t_alone / dt
+    for e in engines:
+        e.close()
+    return res
+
+
 SERVE_MIXES = {
     # the serving configuration as specified
     "specified": {"N": [100, 250, 500, 1000, 2000], "logM": (3.0, 5.0), "periods": (50, 500)},
@@ -172,10 +241,15 @@ def main():
     ap.add_argument("--traj", type=int, default=100000)
     ap.add_argument("--S", type=int, default=200)
     ap.add_argument("--D", type=int, default=16)
-    ap.add_argument("--fp64-steps", type=int, default=0, help="0 = max(2, steps // 3)")
+    ap.add_argument("--fp64-steps", type=int, default=0, help="0 = the same as --steps")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-bf16x3", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--batch-steps", type=int, default=4,
+                    help="top-level steps of the 32-instance batched workload (0 = skip)")
+    ap.add_argument("--batch-alone", type=int, default=1,
+                    help="also time the batched workload instance by instance")
+    ap.add_argument("--only-batch", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=30.0, help="serving leg window (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
@@ -204,6 +278,9 @@ def main():
 
     from os_d_amd import qjx
     N, M, S, D = args.N, args.traj, args.S, args.D
+    if args.only_batch:
+        print(json.dumps(batched(qjx, S, D, 1, args.batch_steps, qjx.FP32, bool(args.batch_alone))), flush=True)
+        return
     e32, r32 = run_form(qjx, qjx.FP32, N, M, S, D, args.warmup, args.steps)
     e32.close()
     bf = None
@@ -213,7 +290,7 @@ def main():
     fp64 = None
     e64 = None
     if not args.no_fp64:
-        k64 = args.fp64_steps or max(2, args.steps // 3)
+        k64 = args.fp64_steps or args.steps
         e64, r64 = run_form(qjx, qjx.FP64, N, M, S, D, args.warmup, k64)
         fp64 = r64
     cpu = None
@@ -222,6 +299,13 @@ def main():
             e64 = qjx.Engine(N, prm_E=1.0, prm_U=0.5, prm_J=1.0, drv_ampl=1.5, drv_freq=1.0,
                              diss_gamma=0.1, ex_deep=D, steps_per_period=S, precision=qjx.FP64)
         cpu = cpu_baseline(e64, N, S, D, args.cpu_seconds)
+
+    batch = None
+    if args.batch_steps > 0:
+        if e64 is not None:
+            e64.close()
+            e64 = None
+        batch = batched(qjx, S, D, 1, args.batch_steps, qjx.FP32, bool(args.batch_alone))
 
     serve = None
     if args.serve_seconds > 0:
@@ -273,6 +357,7 @@ def main():
         "bf16x3_over_fp64": (bf["tp_per_s"] / fp64["tp_per_s"]) if (bf and fp64) else None,
         "bf16x3_gemm_tflops_equiv": (bf["gemm_flops"] / (bf["gemm_ms"] * 1e-3) / 1e12) if bf else None,
         "cpu_baseline": cpu,
+        "batched_uneven": batch,
         "p99_job_latency_s": serve["scaled"]["p99_latency_s"] if serve else None,
         "p99_job_latency_note": "from the scaled job mix (the specified mix completes no job within the window)",
         "serving": serve,

@@ -84,6 +84,8 @@ typedef struct qjx_stats {
     double gemm_ms;         /* device time of the step GEMM launches (profiling on)  */
     double decide_ms;       /* device time of the decide launches (profiling on)     */
     int64_t gemm_launches;  /* step GEMM launches timed                             */
+    int32_t gemm_bn;        /* column tile width of the step GEMM (launch-shape tuner) */
+    int32_t gemm_grid;      /* persistent GEMM workgroups                           */
 } qjx_stats;
 
 const char *qjx_last_error(void);
@@ -152,6 +154,24 @@ int qjx_dim(const qjx_ctx *ctx);
 /* jobs are admitted while their HBM footprint fits the budget, and a job's  */
 /* results do not depend on which other jobs share the launches.             */
 /* ------------------------------------------------------------------------ */
+/* Batches: several ensembles of one precision and any dimensions advanced
+ * together, one grouped GEMM per iteration over all of them.  No counterpart in
+ * the reference, which runs one parameter point per process (the job scripts
+ * under submit/ sweep parameters by launching processes); a sweep becomes one
+ * batch.  The contexts stay owned by the caller and must outlive the batch;
+ * each must hold an initialised ensemble.  Results per context are identical
+ * to running it alone. */
+typedef struct qjx_batch qjx_batch;
+int qjx_batch_create(qjx_ctx *const *ctxs, int n, qjx_batch **out);
+void qjx_batch_destroy(qjx_batch *batch);
+/* n_steps top-level steps of every member (its own T / steps_per_period each). */
+int qjx_batch_run_steps(qjx_batch *batch, int n_steps, int is_obs);
+/* n_periods periods of every member (n_periods * steps_per_period steps each). */
+int qjx_batch_run_periods(qjx_batch *batch, int n_periods, int is_obs);
+/* Totals over the members (matvecs, jumps, flops) and the batch's own timings. */
+int qjx_batch_get_stats(qjx_batch *batch, qjx_stats *out);
+int qjx_batch_set_profiling(qjx_batch *batch, int on);
+
 typedef struct qjx_pool qjx_pool;
 
 typedef struct qjx_job_result {

@@ -10,7 +10,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/qjx.h"
@@ -706,6 +709,8 @@ extern "C" int qjx_get_stats(qjx_ctx *c, qjx_stats *out)
     out->gemm_ms = c->runner ? c->runner->gemm_ms : 0.0;
     out->decide_ms = c->runner ? c->runner->tail_ms + c->runner->plan_ms : 0.0;
     out->gemm_launches = c->runner ? c->runner->gemm_launches : 0;
+    out->gemm_bn = c->runner ? c->runner->bn : 0;
+    out->gemm_grid = c->runner ? c->runner->grid.gemm : 0;
     return QJX_OK;
 }
 
@@ -752,6 +757,127 @@ extern "C" int qjx_set_profiling(qjx_ctx *c, int on)
 }
 
 // ---------------------------------------------------------------------------
+// Batches of contexts sharing one runner
+// ---------------------------------------------------------------------------
+struct qjx_batch {
+    std::vector<qjx_ctx *> ctx;
+    hipStream_t stream = nullptr;
+    qjx::Runner *runner = nullptr;
+};
+
+extern "C" void qjx_batch_destroy(qjx_batch *b)
+{
+    if (!b) return;
+    qjx::runner_destroy(b->runner);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+extern "C" int qjx_batch_create(qjx_ctx *const *ctxs, int n, qjx_batch **out)
+{
+    if (!ctxs || !out) return fail(QJX_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (n < 1 || n > qjx::kMaxEns) return fail(QJX_ERR_ARG, "a batch holds 1 to 64 contexts");
+    long long total = 0;
+    for (int i = 0; i < n; i++) {
+        const qjx_ctx *c = ctxs[i];
+        if (!c || !c->have_ens) return fail(QJX_ERR_ARG, "batch member without an initialised ensemble");
+        if (c->prec != ctxs[0]->prec || c->gemm_kind != ctxs[0]->gemm_kind)
+            return fail(QJX_ERR_ARG, "batch members must share the precision form");
+        for (int j = 0; j < i; j++)
+            if (ctxs[j] == c) return fail(QJX_ERR_ARG, "a context appears twice in the batch");
+        total += c->M;
+    }
+    if (total > (1LL << 30)) return fail(QJX_ERR_ARG, "too many trajectories in one batch");
+    qjx_batch *b = new qjx_batch;
+    b->ctx.assign(ctxs, ctxs + n);
+    if (hipStreamCreate(&b->stream) != hipSuccess) {
+        qjx_batch_destroy(b);
+        return fail(QJX_ERR_HIP, "stream creation failed");
+    }
+    int rc = qjx::runner_create(ctxs[0]->prec, ctxs[0]->gemm_kind, b->stream, (int)total, &b->runner);
+    if (rc) {
+        qjx_batch_destroy(b);
+        return rc;
+    }
+    if (const char *env = std::getenv("QJX_GRAPH"))
+        if (std::string(env) == "0") b->runner->graphs = false;
+    *out = b;
+    return QJX_OK;
+}
+
+static int batch_run(qjx_batch *b, int n, bool periods, int is_obs)
+{
+    if (!b) return fail(QJX_ERR_ARG, "null batch");
+    if (n < 0) return fail(QJX_ERR_ARG, "negative step count");
+    if (n == 0) return QJX_OK;
+    std::vector<qjx::Ens> table;
+    for (qjx_ctx *c : b->ctx) {
+        HIP_TRY(hipStreamSynchronize(c->stream));      // the member's own work is done
+        const long long steps = periods ? (long long)n * c->S : n;
+        if (steps > (1LL << 30)) return fail(QJX_ERR_ARG, "step count too large");
+        qjx::launch_seed(c->st, c->M, (int)steps, c->lists[0], c->cnts[0], c->D, b->stream);
+        c->desc.is_obs = is_obs;
+        table.push_back(c->desc);
+    }
+    HIP_TRY(hipGetLastError());
+    int rc = qjx::runner_set_table(b->runner, table.data(), (int)table.size());
+    if (rc) return rc;
+    rc = qjx::runner_run(b->runner, nullptr);
+    if (rc) return rc;
+    for (qjx_ctx *c : b->ctx) {
+        rc = check_err(c);
+        if (rc) return rc;
+        if (is_obs && c->ens.jump > 0) {
+            rc = drain_log(c);
+            if (rc) return rc;
+        }
+    }
+    return QJX_OK;
+}
+
+extern "C" int qjx_batch_run_steps(qjx_batch *b, int n_steps, int is_obs)
+{
+    return batch_run(b, n_steps, false, is_obs);
+}
+
+extern "C" int qjx_batch_run_periods(qjx_batch *b, int n_periods, int is_obs)
+{
+    return batch_run(b, n_periods, true, is_obs);
+}
+
+extern "C" int qjx_batch_get_stats(qjx_batch *b, qjx_stats *out)
+{
+    if (!b || !out) return fail(QJX_ERR_ARG, "null argument");
+    *out = qjx_stats{};
+    for (qjx_ctx *c : b->ctx) {
+        unsigned long long h[2] = {0, 0};
+        HIP_TRY(hipMemcpy(h, c->counters, sizeof h, hipMemcpyDeviceToHost));
+        out->matvecs += (int64_t)h[0];
+        out->jumps += (int64_t)h[1];
+        out->gemm_flops += 8.0 * double(c->d) * double(c->d) * double(h[0]);
+    }
+    const qjx::Runner *r = b->runner;
+    out->iterations = r->iterations;
+    out->gemm_ms = r->gemm_ms;
+    out->decide_ms = r->tail_ms + r->plan_ms;
+    out->gemm_launches = r->gemm_launches;
+    out->gemm_bn = r->bn;
+    out->gemm_grid = r->grid.gemm;
+    return QJX_OK;
+}
+
+extern "C" int qjx_batch_set_profiling(qjx_batch *b, int on)
+{
+    if (!b) return fail(QJX_ERR_ARG, "null batch");
+    qjx::Runner *r = b->runner;
+    r->profiling = on != 0;
+    r->plan_ms = r->gemm_ms = r->tail_ms = 0.0;
+    r->gemm_launches = 0;
+    return QJX_OK;
+}
+
+// ---------------------------------------------------------------------------
 // Runner: iterations over a table of ensembles
 // ---------------------------------------------------------------------------
 namespace qjx {
@@ -771,9 +897,19 @@ int runner_create(int prec, int gemm_kind, hipStream_t stream, int capacity, Run
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return bail(fail(QJX_ERR_HIP, "device query failed"));
-    int g = cus * gemm_blocks_per_cu(prec, gemm_kind);
-    g = std::max(8, g / 8 * 8);
-    r->grid.gemm = g;
+    r->cus = cus;
+    int bn = gemm_kind == kGemmFma ? 64 : 128;
+    r->autotune = gemm_kind != kGemmFma;
+    if (const char *env = std::getenv("QJX_TUNE"))
+        if (std::string(env) == "0") r->autotune = false;
+    if (const char *env = std::getenv("QJX_BN")) {
+        const std::string v(env);
+        if (gemm_kind != kGemmFma && (v == "64" || v == "128")) {
+            bn = std::stoi(v);
+            r->autotune = false;
+        }
+    }
+    runner_apply_shape(r, bn, gemm_blocks_per_cu(prec, gemm_kind, bn));
     r->grid.decide = std::max(1, (capacity + 255) / 256);   // one entry per thread
     r->grid.jump = std::max(1, std::min(capacity, 2048));
     if (hipMalloc((void **)&r->d_ens, kMaxEns * sizeof(Ens)) != hipSuccess ||
@@ -815,6 +951,7 @@ int runner_set_table(Runner *r, const Ens *ens, int n)
     if (tot > r->capacity) return fail(QJX_ERR_ARG, "runner capacity exceeded");
     if (n) HIP_TRY(hipMemcpyAsync(r->d_ens, ens, n * sizeof(Ens), hipMemcpyHostToDevice, r->stream));
     r->nens = n;
+    r->h_ens.assign(ens, ens + n);
     // the count goes through pinned memory so the copy is stream-ordered
     HIP_TRY(hipStreamSynchronize(r->stream));
     *r->h_total = n;
@@ -832,8 +969,156 @@ static RunArgs run_args(Runner *r, int parity)
     a.jl = r->d_jl;
     a.jcnt = r->d_jcnt;
     a.parity = parity;
-    a.bn = gemm_bm(r->gemm_kind) == 128 ? 128 : 64;
+    a.bn = r->bn;
     return a;
+}
+
+void runner_apply_shape(Runner *r, int bn, int wg_per_cu)
+{
+    int g = r->cus * std::max(1, wg_per_cu);
+    g = std::max(8, g / 8 * 8);          // a multiple of the 8 XCDs
+    if (r->bn == bn && r->grid.gemm == g) return;
+    r->bn = bn;
+    r->wg_per_cu = std::max(1, wg_per_cu);
+    r->grid.gemm = g;
+    if (r->exec) {
+        (void)hipStreamSynchronize(r->stream);
+        (void)hipGraphExecDestroy(r->exec);
+        r->exec = nullptr;
+    }
+    if (r->graph) {
+        (void)hipGraphDestroy(r->graph);
+        r->graph = nullptr;
+    }
+}
+
+// Process-wide cache of tuned shapes, keyed by precision, the largest operand
+// dimension of the table and log2 of its trajectory count.
+struct ShapeKey {
+    int prec, gemm_kind, lmax, log2m;
+    bool operator<(const ShapeKey &o) const
+    {
+        return std::tie(prec, gemm_kind, lmax, log2m) < std::tie(o.prec, o.gemm_kind, o.lmax, o.log2m);
+    }
+};
+static std::mutex g_shape_mu;
+static std::map<ShapeKey, std::pair<int, int>> g_shapes;
+
+static ShapeKey shape_key(const Runner *r)
+{
+    int lmax = 0;
+    long long m = 0;
+    for (const Ens &E : r->h_ens) {
+        lmax = std::max(lmax, E.L);
+        m += E.M;
+    }
+    int lg = 0;
+    while ((1LL << (lg + 1)) <= m) lg++;
+    return ShapeKey{r->prec, r->gemm_kind, lmax, lg};
+}
+
+static int read_cols(Runner *r, unsigned long long *cols)
+{
+    HIP_TRY(hipMemcpyAsync(cols, &r->d_plan->cols, sizeof *cols, hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    return QJX_OK;
+}
+
+// `iters` iterations with the GEMM bracketed by events: device ms of the GEMM
+// launches and list entries processed.
+static int timed_iterations(Runner *r, int iters, double *ms, unsigned long long *cols)
+{
+    if (r->ev.size() < 2 * (size_t)iters) {
+        const size_t old = r->ev.size();
+        r->ev.resize(2 * (size_t)iters);
+        for (size_t i = old; i < r->ev.size(); i++) HIP_TRY(hipEventCreate(&r->ev[i]));
+    }
+    unsigned long long c0 = 0, c1 = 0;
+    int rc = read_cols(r, &c0);
+    if (rc) return rc;
+    for (int k = 0; k < iters; k++) {
+        const RunArgs a = run_args(r, k & 1);
+        launch_plan(a, r->stream);
+        HIP_TRY(hipEventRecord(r->ev[2 * k], r->stream));
+        launch_gemm(r->prec, r->gemm_kind, a, r->grid, r->stream);
+        HIP_TRY(hipEventRecord(r->ev[2 * k + 1], r->stream));
+        launch_tail(r->prec, a, r->grid, r->stream);
+    }
+    HIP_TRY(hipGetLastError());
+    rc = read_cols(r, &c1);
+    if (rc) return rc;
+    double t = 0.0;
+    for (int k = 0; k < iters; k++) {
+        float g = 0.f;
+        HIP_TRY(hipEventElapsedTime(&g, r->ev[2 * k], r->ev[2 * k + 1]));
+        t += g;
+    }
+    *ms = t;
+    *cols = c1 - c0;
+    r->iterations += iters;
+    if (r->profiling) {
+        r->gemm_ms += t;
+        r->gemm_launches += iters;
+    }
+    return QJX_OK;
+}
+
+int runner_autotune(Runner *r)
+{
+    if (r->tuned || !r->autotune || r->nens == 0) return QJX_OK;
+    const ShapeKey key = shape_key(r);
+    {
+        std::lock_guard<std::mutex> lk(g_shape_mu);
+        auto it = g_shapes.find(key);
+        if (it != g_shapes.end()) {
+            runner_apply_shape(r, it->second.first, it->second.second);
+            r->tuned = true;
+            return QJX_OK;
+        }
+    }
+    struct Cand {
+        int bn, wpc;
+        double ms;
+        unsigned long long cols;
+    };
+    std::vector<Cand> cand;
+    for (int bn : {128, 64}) {
+        const int occ = gemm_blocks_per_cu(r->prec, r->gemm_kind, bn);
+        cand.push_back({bn, occ, 0.0, 0});
+        if (occ > 1) cand.push_back({bn, occ / 2, 0.0, 0});
+    }
+    const int saved_bn = r->bn, saved_wpc = r->wg_per_cu;
+    constexpr int kIters = 4, kRounds = 2;
+    for (int round = 0; round < kRounds; round++) {
+        for (Cand &c : cand) {
+            runner_apply_shape(r, c.bn, c.wpc);
+            double ms = 0.0;
+            unsigned long long cols = 0;
+            int rc = timed_iterations(r, kIters, &ms, &cols);
+            if (rc) return rc;
+            c.ms += ms;
+            c.cols += cols;
+            int total = 0;
+            rc = runner_next_total(r, &total);
+            if (rc) return rc;
+            if (total == 0) {      // the table ran out of work: try again next time
+                runner_apply_shape(r, saved_bn, saved_wpc);
+                return QJX_OK;
+            }
+        }
+    }
+    const Cand *best = nullptr;
+    for (const Cand &c : cand)
+        if (c.cols > 0 && (!best || c.ms / c.cols < best->ms / best->cols)) best = &c;
+    if (!best) {
+        runner_apply_shape(r, saved_bn, saved_wpc);
+        return QJX_OK;
+    }
+    runner_apply_shape(r, best->bn, best->wpc);
+    r->tuned = true;
+    std::lock_guard<std::mutex> lk(g_shape_mu);
+    g_shapes[key] = {best->bn, best->wpc};
+    return QJX_OK;
 }
 
 int runner_chunk(Runner *r, int iters)
@@ -906,6 +1191,16 @@ int runner_run(Runner *r, long long *iterations)
         rc = runner_next_total(r, &total);
         if (rc) return rc;
         if (total == 0) break;
+        if (!r->tuned && r->autotune) {
+            // shape the GEMM once the bisection has spread the table over its groups
+            const long long before = r->iterations;
+            rc = runner_autotune(r);
+            if (rc) return rc;
+            done += r->iterations - before;
+            rc = runner_next_total(r, &total);
+            if (rc) return rc;
+            if (total == 0) break;
+        }
         chunk = std::min(chunk * 2, 256);
     }
     if (iterations) *iterations = done;

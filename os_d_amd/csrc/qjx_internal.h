@@ -82,6 +82,7 @@ struct Plan {
     int ent_pre[kPlanItems + 1];    // list entries (trajectories stepping)
     int next_total[2];              // entries appended for the following iteration
     int xnext[8];                   // per-XCD tile claim counters of the GEMM
+    unsigned long long cols;        // list entries planned since the runner was created
 };
 
 struct JumpEnt {

@@ -37,8 +37,9 @@ void launch_iteration(int prec, int gemm_kind, const RunArgs &a, const StepGrid 
 void launch_plan(const RunArgs &a, hipStream_t s);
 void launch_gemm(int prec, int gemm_kind, const RunArgs &a, const StepGrid &g, hipStream_t s);
 void launch_tail(int prec, const RunArgs &a, const StepGrid &g, hipStream_t s);
-// resident GEMM workgroups per compute unit (occupancy of the chosen kernel)
-int gemm_blocks_per_cu(int prec, int gemm_kind);
+// resident GEMM workgroups per compute unit (occupancy of the kernel for the
+// column tile width bn)
+int gemm_blocks_per_cu(int prec, int gemm_kind, int bn);
 // one wave: D32 = A32 (32x2) B32 (2x32), D64 = A64 (16x4) B64 (4x16),
 // Dbf = Abf (32x16) Bbf (16x32) through bf16 operands, all row-major
 void launch_mfma_probe(const float *A32, const float *B32, float *D32, const double *A64,

@@ -21,7 +21,13 @@ struct Runner {
     int *h_total = nullptr;           // pinned
     int capacity = 0;                 // trajectories the table may hold in total
     StepGrid grid{};
+    int cus = 0;                      // compute units of the device
+    int bn = 128;                     // GEMM column tile width (64 or 128)
+    int wg_per_cu = 1;                // persistent GEMM workgroups per compute unit
+    bool autotune = true;             // pick (bn, wg_per_cu) by timing the live table
+    bool tuned = false;
     int nens = 0;
+    std::vector<Ens> h_ens;           // host copy of the table
     bool graphs = true;               // replay a captured pair of iterations
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -41,6 +47,13 @@ int runner_chunk(Runner *r, int iters);
 int runner_next_total(Runner *r, int *total);
 // Chunks of growing length until no trajectory in the table has steps left.
 int runner_run(Runner *r, long long *iterations);
+// Launch-shape tuner: pick the GEMM column tile width and the persistent grid
+// for the current table, from a process-wide cache or by timing the candidates
+// on the table's own iterations (they are real steps; results do not depend on
+// the shape).  No-op once tuned, for the FMA kernel, or with autotune off.
+int runner_autotune(Runner *r);
+// Use tile width bn and wg_per_cu workgroups per CU (drops a captured graph).
+void runner_apply_shape(Runner *r, int bn, int wg_per_cu);
 
 void set_last_error(const std::string &msg);
 

@@ -102,6 +102,7 @@ __global__ __launch_bounds__(kPlanThreads) void plan_kernel(RunArgs a)
     if (tid == 0) {
         pl->tile_pre[nitems] = s_t[nthr - 1];
         pl->ent_pre[nitems] = s_e[nthr - 1];
+        pl->cols += (unsigned long long)s_e[nthr - 1];
         pl->next_total[p ^ 1] = 0;
         *a.jcnt = 0;
     }
@@ -176,21 +177,26 @@ __device__ inline void load_cols(const RunArgs &a, const TileRef &r, int *s_tr, 
 // GEMM tile, FP32: v_mfma_f32_32x32x2f32
 //   A (32 x 2): lane l holds A[l % 32][l / 32]; B (2 x 32): B[l / 32][l % 32]
 //   D (32 x 32): lane l, register r holds D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32]
-// 4 waves as 2 x 2, wave tile 64 x 64, WG tile 128 x 128, K block 32; within a K
-// block, MFMA step s (0..15) takes k = s + 16 h for lane half h, so each lane reads
-// its 16 fragment values per block with four 16-byte LDS reads.  The k order is
-// the same for every column, so results do not depend on where a column lands.
+// 4 waves as 2 x 2, wave tile 64 x BN/2, WG tile 128 x BN (BN = 128 or 64, picked
+// by the launch-shape tuner), K block 32; within a K block, MFMA step s (0..15)
+// takes k = s + 16 h for lane half h, so each lane reads its 16 fragment values per
+// block with four 16-byte LDS reads.  The k order is the same for every column and
+// the row split over waves does not depend on BN, so results do not depend on
+// where a column lands or on the tile width.
 // ---------------------------------------------------------------------------
-constexpr int MF_BM = 128, MF_BN = 128, MF_BK = 32;
+constexpr int MF_BM = 128, MF_BK = 32;
 constexpr int F32_PITCH = MF_BK + 4;
 
+template <int BN>
 __global__ __launch_bounds__(256) void gemm_f32(RunArgs a)
 {
-    __shared__ int s_tr[MF_BN];
-    __shared__ int s_cur[MF_BN];
+    constexpr int NJ = BN / 64;          // 32-column MFMA tiles per wave
+    constexpr int NQ = BN / 32;          // B-tile rows (columns of X) copied per thread
+    __shared__ int s_tr[BN];
+    __shared__ int s_cur[BN];
     __shared__ __attribute__((aligned(16))) float As[2][MF_BM * F32_PITCH];   // double-buffered
-    __shared__ __attribute__((aligned(16))) float Bs[2][MF_BN * F32_PITCH];
-    __shared__ double s_norm[2][MF_BN];
+    __shared__ __attribute__((aligned(16))) float Bs[2][BN * F32_PITCH];
+    __shared__ double s_norm[2][BN];
     const int *s_tpre = a.plan->tile_pre;     // read through L2: one search per tile
     const int nitems = *a.nens * kMaxGroups;
     const int total = s_tpre[nitems];
@@ -200,42 +206,41 @@ __global__ __launch_bounds__(256) void gemm_f32(RunArgs a)
     const int cp_k = (tid & 7) * 4, cp_r = tid >> 3;
     __shared__ int s_claim;
     TileRef r;
-    while (tile_next<MF_BN>(a, s_tpre, nitems, total, &s_claim, r)) {
-        load_cols<MF_BN>(a, r, s_tr, s_cur);
+    while (tile_next<BN>(a, s_tpre, nitems, total, &s_claim, r)) {
+        load_cols<BN>(a, r, s_tr, s_cur);
         __syncthreads();
         const Ens &E = a.ens[r.e];
         const int L = E.L, r0 = r.rt * MF_BM;
         const float *U = reinterpret_cast<const float *>(E.props) + (size_t)r.g * L * L;
         const float *arow = U + (size_t)(r0 + cp_r) * L + cp_k;
         const size_t astep = (size_t)32 * L;
-        const float *bcol[4];
+        const float *bcol[NQ];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
+        for (int q = 0; q < NQ; q++) {
             const int c = cp_r + 32 * q;
             bcol[q] = reinterpret_cast<const float *>(E.slot[s_cur[c]]) + (size_t)s_tr[c] * L + cp_k;
         }
-        floatx16 acc[2][2];
+        floatx16 acc[2][NJ];
 #pragma unroll
         for (int i = 0; i < 2; i++)
 #pragma unroll
-            for (int j = 0; j < 2; j++)
+            for (int j = 0; j < NJ; j++)
 #pragma unroll
                 for (int e = 0; e < 16; e++) acc[i][j][e] = 0.f;
-        floatx4 ra[4], rb[4];
+        floatx4 ra[4], rb[NQ];
         auto gload = [&](int k0) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                ra[q] = *reinterpret_cast<const floatx4 *>(arow + q * astep + k0);
-                rb[q] = *reinterpret_cast<const floatx4 *>(bcol[q] + k0);
-            }
+            for (int q = 0; q < 4; q++) ra[q] = *reinterpret_cast<const floatx4 *>(arow + q * astep + k0);
+#pragma unroll
+            for (int q = 0; q < NQ; q++) rb[q] = *reinterpret_cast<const floatx4 *>(bcol[q] + k0);
         };
         auto lstore = [&](int buf) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int rr = cp_r + 32 * q;
-                *reinterpret_cast<floatx4 *>(&As[buf][rr * F32_PITCH + cp_k]) = ra[q];
-                *reinterpret_cast<floatx4 *>(&Bs[buf][rr * F32_PITCH + cp_k]) = rb[q];
-            }
+            for (int q = 0; q < 4; q++)
+                *reinterpret_cast<floatx4 *>(&As[buf][(cp_r + 32 * q) * F32_PITCH + cp_k]) = ra[q];
+#pragma unroll
+            for (int q = 0; q < NQ; q++)
+                *reinterpret_cast<floatx4 *>(&Bs[buf][(cp_r + 32 * q) * F32_PITCH + cp_k]) = rb[q];
         };
         gload(0);
         lstore(0);
@@ -244,23 +249,25 @@ __global__ __launch_bounds__(256) void gemm_f32(RunArgs a)
         for (int k0 = 0; k0 < L; k0 += MF_BK) {
             const bool more = k0 + MF_BK < L;
             if (more) gload(k0 + MF_BK);
-            floatx4 fa[2][4], fb[2][4];
+            floatx4 fa[2][4], fb[NJ][4];
 #pragma unroll
             for (int i = 0; i < 2; i++) {
                 const float *pa = &As[cur][(wm * 64 + i * 32 + li) * F32_PITCH + 16 * h];
-                const float *pb = &Bs[cur][(wn * 64 + i * 32 + li) * F32_PITCH + 16 * h];
 #pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    fa[i][v] = *reinterpret_cast<const floatx4 *>(pa + 4 * v);
-                    fb[i][v] = *reinterpret_cast<const floatx4 *>(pb + 4 * v);
-                }
+                for (int v = 0; v < 4; v++) fa[i][v] = *reinterpret_cast<const floatx4 *>(pa + 4 * v);
+            }
+#pragma unroll
+            for (int j = 0; j < NJ; j++) {
+                const float *pb = &Bs[cur][(wn * (BN / 2) + j * 32 + li) * F32_PITCH + 16 * h];
+#pragma unroll
+                for (int v = 0; v < 4; v++) fb[j][v] = *reinterpret_cast<const floatx4 *>(pb + 4 * v);
             }
 #pragma unroll
             for (int s = 0; s < 16; s++)
 #pragma unroll
                 for (int i = 0; i < 2; i++)
 #pragma unroll
-                    for (int j = 0; j < 2; j++)
+                    for (int j = 0; j < NJ; j++)
                         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[i][s >> 2][s & 3],
                                                                           fb[j][s >> 2][s & 3], acc[i][j], 0, 0, 0);
             // the other buffer was last read before the previous barrier
@@ -268,10 +275,11 @@ __global__ __launch_bounds__(256) void gemm_f32(RunArgs a)
             __syncthreads();
             cur ^= 1;
         }
-        double ns[2] = {0.0, 0.0};
+        double ns[NJ];
 #pragma unroll
-        for (int j = 0; j < 2; j++) {
-            const int c = wn * 64 + j * 32 + li;
+        for (int j = 0; j < NJ; j++) {
+            ns[j] = 0.0;
+            const int c = wn * (BN / 2) + j * 32 + li;
             const bool ok = c < r.ncol;
             float *y = reinterpret_cast<float *>(E.slot[s_cur[c] ^ 1]) + (size_t)s_tr[c] * L;
 #pragma unroll
@@ -286,10 +294,10 @@ __global__ __launch_bounds__(256) void gemm_f32(RunArgs a)
                 }
         }
 #pragma unroll
-        for (int j = 0; j < 2; j++) ns[j] += __shfl_xor(ns[j], 32, 64);
+        for (int j = 0; j < NJ; j++) ns[j] += __shfl_xor(ns[j], 32, 64);
         if (h == 0) {
 #pragma unroll
-            for (int j = 0; j < 2; j++) s_norm[wm][wn * 64 + j * 32 + li] = ns[j];
+            for (int j = 0; j < NJ; j++) s_norm[wm][wn * (BN / 2) + j * 32 + li] = ns[j];
         }
         __syncthreads();
         if (tid < r.ncol) E.normpart[(size_t)r.rt * E.M + s_tr[tid]] = s_norm[0][tid] + s_norm[1][tid];
@@ -314,15 +322,18 @@ __device__ inline int bf_off(int row, int k)   // element offset of (row, k), k 
     return row * BF_ROW + chunk * 8 + (k & 7);
 }
 
+template <int BN>
 __global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
 {
-    __shared__ int s_tr[MF_BN];
-    __shared__ int s_cur[MF_BN];
+    constexpr int NJ = BN / 64;          // 32-column MFMA tiles per wave
+    constexpr int NQ = BN / 32;          // B-tile rows (columns of X) copied per thread
+    __shared__ int s_tr[BN];
+    __shared__ int s_cur[BN];
     __shared__ __attribute__((aligned(16))) __bf16 Ah[2][MF_BM * BF_ROW];
     __shared__ __attribute__((aligned(16))) __bf16 Al[2][MF_BM * BF_ROW];
-    __shared__ __attribute__((aligned(16))) __bf16 Bh[2][MF_BN * BF_ROW];
-    __shared__ __attribute__((aligned(16))) __bf16 Bl[2][MF_BN * BF_ROW];
-    __shared__ double s_norm[2][MF_BN];
+    __shared__ __attribute__((aligned(16))) __bf16 Bh[2][BN * BF_ROW];
+    __shared__ __attribute__((aligned(16))) __bf16 Bl[2][BN * BF_ROW];
+    __shared__ double s_norm[2][BN];
     const int *s_tpre = a.plan->tile_pre;
     const int nitems = *a.nens * kMaxGroups;
     const int total = s_tpre[nitems];
@@ -333,29 +344,29 @@ __global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
     const int cb_k = (tid & 7) * 4, cb_r = tid >> 3;
     __shared__ int s_claim;
     TileRef r;
-    while (tile_next<MF_BN>(a, s_tpre, nitems, total, &s_claim, r)) {
-        load_cols<MF_BN>(a, r, s_tr, s_cur);
+    while (tile_next<BN>(a, s_tpre, nitems, total, &s_claim, r)) {
+        load_cols<BN>(a, r, s_tr, s_cur);
         __syncthreads();
         const Ens &E = a.ens[r.e];
         const int L = E.L, r0 = r.rt * MF_BM;
         const size_t LL = (size_t)L * L;
         const __bf16 *Uh = reinterpret_cast<const __bf16 *>(E.props) + (size_t)r.g * LL;
         const __bf16 *Ul = reinterpret_cast<const __bf16 *>(E.props_lo) + (size_t)r.g * LL;
-        const float *bcol[4];
+        const float *bcol[NQ];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
+        for (int q = 0; q < NQ; q++) {
             const int c = cb_r + 32 * q;
             bcol[q] = reinterpret_cast<const float *>(E.slot[s_cur[c]]) + (size_t)s_tr[c] * L + cb_k;
         }
-        floatx16 acc[2][2];
+        floatx16 acc[2][NJ];
 #pragma unroll
         for (int i = 0; i < 2; i++)
 #pragma unroll
-            for (int j = 0; j < 2; j++)
+            for (int j = 0; j < NJ; j++)
 #pragma unroll
                 for (int e = 0; e < 16; e++) acc[i][j][e] = 0.f;
         bf16x8 rah[2], ral[2];
-        floatx4 rb[4];
+        floatx4 rb[NQ];
         auto gload = [&](int k0) {
 #pragma unroll
             for (int q = 0; q < 2; q++) {
@@ -364,7 +375,7 @@ __global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
                 ral[q] = *reinterpret_cast<const bf16x8 *>(Ul + o);
             }
 #pragma unroll
-            for (int q = 0; q < 4; q++) rb[q] = *reinterpret_cast<const floatx4 *>(bcol[q] + k0);
+            for (int q = 0; q < NQ; q++) rb[q] = *reinterpret_cast<const floatx4 *>(bcol[q] + k0);
         };
         auto lstore = [&](int buf) {
 #pragma unroll
@@ -374,7 +385,7 @@ __global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
                 *reinterpret_cast<bf16x8 *>(&Al[buf][bf_off(rr, ca_k)]) = ral[q];
             }
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
+            for (int q = 0; q < NQ; q++) {
                 const int rr = cb_r + 32 * q;
                 bf16x4 h4, l4;
 #pragma unroll
@@ -397,20 +408,23 @@ __global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
             if (more) gload(k0 + MF_BK);
 #pragma unroll
             for (int c = 0; c < MF_BK / 16; c++) {
-                bf16x8 fah[2], fal[2], fbh[2], fbl[2];
+                bf16x8 fah[2], fal[2], fbh[NJ], fbl[NJ];
 #pragma unroll
                 for (int i = 0; i < 2; i++) {
                     const int oa = bf_off(wm * 64 + i * 32 + li, 16 * c + 8 * h);
-                    const int ob = bf_off(wn * 64 + i * 32 + li, 16 * c + 8 * h);
                     fah[i] = *reinterpret_cast<const bf16x8 *>(&Ah[cur][oa]);
                     fal[i] = *reinterpret_cast<const bf16x8 *>(&Al[cur][oa]);
-                    fbh[i] = *reinterpret_cast<const bf16x8 *>(&Bh[cur][ob]);
-                    fbl[i] = *reinterpret_cast<const bf16x8 *>(&Bl[cur][ob]);
+                }
+#pragma unroll
+                for (int j = 0; j < NJ; j++) {
+                    const int ob = bf_off(wn * (BN / 2) + j * 32 + li, 16 * c + 8 * h);
+                    fbh[j] = *reinterpret_cast<const bf16x8 *>(&Bh[cur][ob]);
+                    fbl[j] = *reinterpret_cast<const bf16x8 *>(&Bl[cur][ob]);
                 }
 #pragma unroll
                 for (int i = 0; i < 2; i++)
 #pragma unroll
-                    for (int j = 0; j < 2; j++) {
+                    for (int j = 0; j < NJ; j++) {
                         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fal[i], fbh[j], acc[i][j], 0, 0, 0);
                         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fah[i], fbl[j], acc[i][j], 0, 0, 0);
                         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fah[i], fbh[j], acc[i][j], 0, 0, 0);
@@ -420,10 +434,11 @@ __global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
             __syncthreads();
             cur ^= 1;
         }
-        double ns[2] = {0.0, 0.0};
+        double ns[NJ];
 #pragma unroll
-        for (int j = 0; j < 2; j++) {
-            const int c = wn * 64 + j * 32 + li;
+        for (int j = 0; j < NJ; j++) {
+            ns[j] = 0.0;
+            const int c = wn * (BN / 2) + j * 32 + li;
             const bool ok = c < r.ncol;
             float *y = reinterpret_cast<float *>(E.slot[s_cur[c] ^ 1]) + (size_t)s_tr[c] * L;
 #pragma unroll
@@ -438,10 +453,10 @@ __global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
                 }
         }
 #pragma unroll
-        for (int j = 0; j < 2; j++) ns[j] += __shfl_xor(ns[j], 32, 64);
+        for (int j = 0; j < NJ; j++) ns[j] += __shfl_xor(ns[j], 32, 64);
         if (h == 0) {
 #pragma unroll
-            for (int j = 0; j < 2; j++) s_norm[wm][wn * 64 + j * 32 + li] = ns[j];
+            for (int j = 0; j < NJ; j++) s_norm[wm][wn * (BN / 2) + j * 32 + li] = ns[j];
         }
         __syncthreads();
         if (tid < r.ncol) E.normpart[(size_t)r.rt * E.M + s_tr[tid]] = s_norm[0][tid] + s_norm[1][tid];
@@ -453,18 +468,21 @@ __global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
 // GEMM tile, FP64: v_mfma_f64_16x16x4f64
 //   A (16 x 4): lane l holds A[l % 16][l / 16]; B (4 x 16): B[l / 16][l % 16]
 //   D (16 x 16): lane l, register r holds D[(l / 16) + 4 r][l % 16]
-// 8 waves as 4 x 2, wave tile 32 x 64, WG tile 128 x 128; within a K block of 32,
-// MFMA step s (0..7) takes k = s + 8 q for lane quarter q.
+// 8 waves as 4 x 2, wave tile 32 x BN/2, WG tile 128 x BN (BN = 128 or 64); within
+// a K block of 32, MFMA step s (0..7) takes k = s + 8 q for lane quarter q.
 // ---------------------------------------------------------------------------
 constexpr int F64_PITCH = MF_BK + 2;
 
+template <int BN>
 __global__ __launch_bounds__(512) void gemm_f64(RunArgs a)
 {
-    __shared__ int s_tr[MF_BN];
-    __shared__ int s_cur[MF_BN];
+    constexpr int NJ = BN / 32;          // 16-column MFMA tiles per wave
+    constexpr int NQ = BN / 32;          // B-tile rows (columns of X) copied per thread
+    __shared__ int s_tr[BN];
+    __shared__ int s_cur[BN];
     __shared__ __attribute__((aligned(16))) double As[MF_BM * F64_PITCH];
-    __shared__ __attribute__((aligned(16))) double Bs[MF_BN * F64_PITCH];
-    __shared__ double s_norm[4][MF_BN];
+    __shared__ __attribute__((aligned(16))) double Bs[BN * F64_PITCH];
+    __shared__ double s_norm[4][BN];
     const int *s_tpre = a.plan->tile_pre;     // read through L2: one search per tile
     const int nitems = *a.nens * kMaxGroups;
     const int total = s_tpre[nitems];
@@ -474,44 +492,43 @@ __global__ __launch_bounds__(512) void gemm_f64(RunArgs a)
     const int cp_k = (tid & 15) * 2, cp_r = tid >> 4;
     __shared__ int s_claim;
     TileRef r;
-    while (tile_next<MF_BN>(a, s_tpre, nitems, total, &s_claim, r)) {
-        load_cols<MF_BN>(a, r, s_tr, s_cur);
+    while (tile_next<BN>(a, s_tpre, nitems, total, &s_claim, r)) {
+        load_cols<BN>(a, r, s_tr, s_cur);
         __syncthreads();
         const Ens &E = a.ens[r.e];
         const int L = E.L, r0 = r.rt * MF_BM;
         const double *U = reinterpret_cast<const double *>(E.props) + (size_t)r.g * L * L;
         const double *arow = U + (size_t)(r0 + cp_r) * L + cp_k;
         const size_t astep = (size_t)32 * L;
-        const double *bcol[4];
+        const double *bcol[NQ];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
+        for (int q = 0; q < NQ; q++) {
             const int c = cp_r + 32 * q;
             bcol[q] = reinterpret_cast<const double *>(E.slot[s_cur[c]]) + (size_t)s_tr[c] * L + cp_k;
         }
-        doublex4 acc[2][4];
+        doublex4 acc[2][NJ];
 #pragma unroll
         for (int i = 0; i < 2; i++)
 #pragma unroll
-            for (int j = 0; j < 4; j++) acc[i][j] = doublex4{0.0, 0.0, 0.0, 0.0};
-        double2v ra[4], rb[4];
+            for (int j = 0; j < NJ; j++) acc[i][j] = doublex4{0.0, 0.0, 0.0, 0.0};
+        double2v ra[4], rb[NQ];
         auto gload = [&](int k0) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                ra[q] = *reinterpret_cast<const double2v *>(arow + q * astep + k0);
-                rb[q] = *reinterpret_cast<const double2v *>(bcol[q] + k0);
-            }
+            for (int q = 0; q < 4; q++) ra[q] = *reinterpret_cast<const double2v *>(arow + q * astep + k0);
+#pragma unroll
+            for (int q = 0; q < NQ; q++) rb[q] = *reinterpret_cast<const double2v *>(bcol[q] + k0);
         };
         auto lstore = [&]() {
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int rr = cp_r + 32 * q;
-                *reinterpret_cast<double2v *>(&As[rr * F64_PITCH + cp_k]) = ra[q];
-                *reinterpret_cast<double2v *>(&Bs[rr * F64_PITCH + cp_k]) = rb[q];
-            }
+            for (int q = 0; q < 4; q++)
+                *reinterpret_cast<double2v *>(&As[(cp_r + 32 * q) * F64_PITCH + cp_k]) = ra[q];
+#pragma unroll
+            for (int q = 0; q < NQ; q++)
+                *reinterpret_cast<double2v *>(&Bs[(cp_r + 32 * q) * F64_PITCH + cp_k]) = rb[q];
         };
         // (single-buffered: at one workgroup per CU a second buffer measured slower)
         const double *pa0 = &As[(wm * 32 + li) * F64_PITCH + 8 * qq];
-        const double *pb0 = &Bs[(wn * 64 + li) * F64_PITCH + 8 * qq];
+        const double *pb0 = &Bs[(wn * (BN / 2) + li) * F64_PITCH + 8 * qq];
         gload(0);
         lstore();
         __syncthreads();
@@ -520,7 +537,7 @@ __global__ __launch_bounds__(512) void gemm_f64(RunArgs a)
             if (more) gload(k0 + MF_BK);
 #pragma unroll
             for (int half = 0; half < 2; half++) {
-                double2v fa[2][2], fb[4][2];
+                double2v fa[2][2], fb[NJ][2];
 #pragma unroll
                 for (int i = 0; i < 2; i++) {
                     const double *pa = pa0 + i * 16 * F64_PITCH + 4 * half;
@@ -528,7 +545,7 @@ __global__ __launch_bounds__(512) void gemm_f64(RunArgs a)
                     fa[i][1] = *reinterpret_cast<const double2v *>(pa + 2);
                 }
 #pragma unroll
-                for (int j = 0; j < 4; j++) {
+                for (int j = 0; j < NJ; j++) {
                     const double *pb = pb0 + j * 16 * F64_PITCH + 4 * half;
                     fb[j][0] = *reinterpret_cast<const double2v *>(pb);
                     fb[j][1] = *reinterpret_cast<const double2v *>(pb + 2);
@@ -538,7 +555,7 @@ __global__ __launch_bounds__(512) void gemm_f64(RunArgs a)
 #pragma unroll
                     for (int i = 0; i < 2; i++)
 #pragma unroll
-                        for (int j = 0; j < 4; j++)
+                        for (int j = 0; j < NJ; j++)
                             acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i][s4 >> 1][s4 & 1],
                                                                               fb[j][s4 >> 1][s4 & 1], acc[i][j],
                                                                               0, 0, 0);
@@ -549,10 +566,11 @@ __global__ __launch_bounds__(512) void gemm_f64(RunArgs a)
                 __syncthreads();
             }
         }
-        double ns[4] = {0.0, 0.0, 0.0, 0.0};
+        double ns[NJ];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int c = wn * 64 + j * 16 + li;
+        for (int j = 0; j < NJ; j++) {
+            ns[j] = 0.0;
+            const int c = wn * (BN / 2) + j * 16 + li;
             const bool ok = c < r.ncol;
             double *y = reinterpret_cast<double *>(E.slot[s_cur[c] ^ 1]) + (size_t)s_tr[c] * L;
 #pragma unroll
@@ -566,13 +584,13 @@ __global__ __launch_bounds__(512) void gemm_f64(RunArgs a)
             }
         }
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
+        for (int j = 0; j < NJ; j++) {
             ns[j] += __shfl_xor(ns[j], 16, 64);
             ns[j] += __shfl_xor(ns[j], 32, 64);
         }
         if (qq == 0) {
 #pragma unroll
-            for (int j = 0; j < 4; j++) s_norm[wm][wn * 64 + j * 16 + li] = ns[j];
+            for (int j = 0; j < NJ; j++) s_norm[wm][wn * (BN / 2) + j * 16 + li] = ns[j];
         }
         __syncthreads();
         if (tid < r.ncol)
@@ -938,11 +956,14 @@ void launch_gemm(int prec, int gemm_kind, const RunArgs &a, const StepGrid &g, h
         if (prec == kFP64) gemm_fma<double><<<g.gemm, 256, 0, s>>>(a);
         else gemm_fma<float><<<g.gemm, 256, 0, s>>>(a);
     } else if (prec == kFP64) {
-        gemm_f64<<<g.gemm, 512, 0, s>>>(a);
+        if (a.bn == 64) gemm_f64<64><<<g.gemm, 512, 0, s>>>(a);
+        else gemm_f64<128><<<g.gemm, 512, 0, s>>>(a);
     } else if (prec == kBF16x3) {
-        gemm_bf16x3<<<g.gemm, 256, 0, s>>>(a);
+        if (a.bn == 64) gemm_bf16x3<64><<<g.gemm, 256, 0, s>>>(a);
+        else gemm_bf16x3<128><<<g.gemm, 256, 0, s>>>(a);
     } else {
-        gemm_f32<<<g.gemm, 256, 0, s>>>(a);
+        if (a.bn == 64) gemm_f32<64><<<g.gemm, 256, 0, s>>>(a);
+        else gemm_f32<128><<<g.gemm, 256, 0, s>>>(a);
     }
 }
 
@@ -960,19 +981,23 @@ void launch_iteration(int prec, int gemm_kind, const RunArgs &a, const StepGrid 
     launch_tail(prec, a, g, s);
 }
 
-int gemm_blocks_per_cu(int prec, int gemm_kind)
+int gemm_blocks_per_cu(int prec, int gemm_kind, int bn)
 {
     int n = 0;
     hipError_t e;
+    const bool narrow = bn == 64;
     if (gemm_kind == kGemmFma)
         e = prec == kFP64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_fma<double>, 256, 0)
                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_fma<float>, 256, 0);
     else if (prec == kFP64)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_f64, 512, 0);
+        e = narrow ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_f64<64>, 512, 0)
+                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_f64<128>, 512, 0);
     else if (prec == kBF16x3)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_bf16x3, 256, 0);
+        e = narrow ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_bf16x3<64>, 256, 0)
+                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_bf16x3<128>, 256, 0);
     else
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_f32, 256, 0);
+        e = narrow ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_f32<64>, 256, 0)
+                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_f32<128>, 256, 0);
     return (e == hipSuccess && n > 0) ? n : 1;
 }
 

@@ -48,7 +48,12 @@ class Stats(ctypes.Structure):
     _fields_ = [("matvecs", ctypes.c_int64), ("iterations", ctypes.c_int64),
                 ("jumps", ctypes.c_int64), ("gemm_flops", ctypes.c_double),
                 ("gemm_ms", ctypes.c_double), ("decide_ms", ctypes.c_double),
-                ("gemm_launches", ctypes.c_int64)]
+                ("gemm_launches", ctypes.c_int64), ("gemm_bn", ctypes.c_int32),
+                ("gemm_grid", ctypes.c_int32)]
+
+
+def _stats_dict(s):
+    return {k: getattr(s, k) for k, _ in Stats._fields_}
 
 
 class JobResult(ctypes.Structure):
@@ -87,6 +92,12 @@ SIGNATURES = [
     ("qjx_mfma_probe", ctypes.c_int, [_P(ctypes.c_float), _P(ctypes.c_float), _P(ctypes.c_float), _D, _D,
                                       _D, _P(ctypes.c_float), _P(ctypes.c_float), _P(ctypes.c_float)]),
     ("qjx_run_config", ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]),
+    ("qjx_batch_create", ctypes.c_int, [_P(ctypes.c_void_p), ctypes.c_int, _P(ctypes.c_void_p)]),
+    ("qjx_batch_destroy", None, [ctypes.c_void_p]),
+    ("qjx_batch_run_steps", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    ("qjx_batch_run_periods", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    ("qjx_batch_get_stats", ctypes.c_int, [ctypes.c_void_p, _P(Stats)]),
+    ("qjx_batch_set_profiling", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("qjx_pool_create", ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int, _P(ctypes.c_void_p)]),
     ("qjx_pool_destroy", None, [ctypes.c_void_p]),
     ("qjx_pool_submit", ctypes.c_int, [ctypes.c_void_p, _P(ModelDesc), _P(EnsembleDesc), ctypes.c_int,
@@ -219,12 +230,49 @@ class Engine:
     def stats(self):
         s = Stats()
         _check(self.lib, self.lib.qjx_get_stats(self.h, ctypes.byref(s)))
-        return {"matvecs": s.matvecs, "iterations": s.iterations, "jumps": s.jumps,
-                "gemm_flops": s.gemm_flops, "gemm_ms": s.gemm_ms, "decide_ms": s.decide_ms,
-                "gemm_launches": s.gemm_launches}
+        return _stats_dict(s)
 
     def set_profiling(self, on=True):
         _check(self.lib, self.lib.qjx_set_profiling(self.h, 1 if on else 0))
+
+
+class Batch:
+    """Several engines (one precision, any dimensions) advanced together: one
+    grouped GEMM per iteration over all of their trajectories.  The engines stay
+    usable on their own and must outlive the batch."""
+
+    def __init__(self, engines):
+        self.lib = load_library()
+        self.engines = list(engines)
+        arr = (ctypes.c_void_p * len(self.engines))(*[e.h.value for e in self.engines])
+        h = ctypes.c_void_p()
+        _check(self.lib, self.lib.qjx_batch_create(arr, len(self.engines), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.qjx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run_steps(self, n, is_obs=0):
+        _check(self.lib, self.lib.qjx_batch_run_steps(self.h, n, is_obs))
+
+    def run_periods(self, n, is_obs=0):
+        _check(self.lib, self.lib.qjx_batch_run_periods(self.h, n, is_obs))
+
+    def stats(self):
+        s = Stats()
+        _check(self.lib, self.lib.qjx_batch_get_stats(self.h, ctypes.byref(s)))
+        return _stats_dict(s)
+
+    def set_profiling(self, on=True):
+        _check(self.lib, self.lib.qjx_batch_set_profiling(self.h, 1 if on else 0))
 
 
 class Pool:

@@ -1,0 +1,110 @@
+"""Batches and launch shapes: results do not depend on how the work is launched.
+
+* A batch advances several ensembles of different Hilbert dimension in the same
+  grouped GEMMs; each member's trajectories must end bitwise identical to a run
+  of that ensemble alone (states, etas, times, jump counts, RNG states).
+* The step GEMM's column tile width (64 or 128) and persistent grid are picked
+  by the launch-shape tuner; every shape must give bitwise identical results.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MEMBERS = [
+    dict(N=20, M=64, seed=3, drv_ampl=1.0, diss_gamma=0.3),
+    dict(N=70, M=200, seed=4, drv_ampl=2.5, diss_gamma=0.1),
+    dict(N=150, M=96, seed=5, drv_ampl=1.5, diss_gamma=0.05),
+]
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _engine(prec, N, M, seed, drv_ampl, diss_gamma, S=8, D=8):
+    from os_d_amd import qjx
+    e = qjx.Engine(N, prm_E=1.0, prm_U=0.5, prm_J=1.0, drv_ampl=drv_ampl, drv_freq=1.0,
+                   diss_gamma=diss_gamma, ex_deep=D, steps_per_period=S, precision=prec)
+    e.ensemble_init(M, seed=seed, mns=1000000, start_type=0, start_state=0, jump=0)
+    e.trans(0)
+    return e
+
+
+def _same_state(a, b):
+    for k in ("psi", "eta", "t", "njumps", "rng"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("prec", [1, 0, 2])
+def test_batch_member_equals_alone(prec):
+    from os_d_amd import qjx
+    _say("case: batch vs alone prec=%d" % prec)
+    alone = []
+    for m in MEMBERS:
+        e = _engine(prec, **m)
+        e.run_steps(6, 0)
+        alone.append((e.get_state(), e.stats()["matvecs"]))
+        e.close()
+    engines = [_engine(prec, **m) for m in MEMBERS]
+    b = qjx.Batch(engines)
+    b.run_steps(4, 0)
+    b.run_steps(2, 0)
+    st = b.stats()
+    _say("batch: matvecs %d, iterations %d, bn %d" % (st["matvecs"], st["iterations"], st["gemm_bn"]))
+    for e, (s_alone, mv) in zip(engines, alone):
+        _same_state(e.get_state(), s_alone)
+        assert e.stats()["matvecs"] == mv
+    assert st["matvecs"] == sum(mv for _, mv in alone)
+    b.close()
+    for e in engines:
+        e.close()
+
+
+@pytest.mark.parametrize("prec", [1, 0, 2])
+def test_tile_width_does_not_change_results(prec, monkeypatch):
+    _say("case: tile width 64 vs 128 prec=%d" % prec)
+    res = {}
+    for bn in ("64", "128"):
+        monkeypatch.setenv("QJX_BN", bn)
+        e = _engine(prec, N=150, M=300, seed=11, drv_ampl=1.5, diss_gamma=0.1)
+        e.run_steps(5, 0)
+        s = e.stats()
+        assert s["gemm_bn"] == int(bn)
+        res[bn] = e.get_state()
+        e.close()
+    _same_state(res["64"], res["128"])
+
+
+def test_autotuner_picks_a_shape_with_identical_results(monkeypatch):
+    _say("case: autotuner")
+    monkeypatch.setenv("QJX_BN", "128")
+    e = _engine(1, N=200, M=2000, seed=2, drv_ampl=1.5, diss_gamma=0.2)
+    e.run_steps(12, 0)
+    ref = e.get_state()
+    e.close()
+    monkeypatch.delenv("QJX_BN")
+    monkeypatch.delenv("QJX_TUNE", raising=False)
+    e = _engine(1, N=200, M=2000, seed=2, drv_ampl=1.5, diss_gamma=0.2)
+    e.run_steps(12, 0)
+    s = e.stats()
+    _say("tuned: bn %d grid %d" % (s["gemm_bn"], s["gemm_grid"]))
+    assert s["gemm_bn"] in (64, 128)
+    assert s["gemm_grid"] > 0 and s["gemm_grid"] % 8 == 0
+    _same_state(e.get_state(), ref)
+    e.close()
+
+
+def test_batch_rejects_mixed_precision():
+    from os_d_amd import qjx
+    _say("case: batch precision check")
+    a = _engine(1, **MEMBERS[0])
+    b = _engine(0, **MEMBERS[1])
+    with pytest.raises(qjx.QJXError):
+        qjx.Batch([a, b])
+    a.close()
+    b.close()
