@@ -10,14 +10,19 @@ HBM before the timed region.
 
 value      : trajectory-periods/s of the FP32 form (FP32 GEMM, FP64 norms, eta,
              bisection, jumps).
-fp64_value : the same for the FP64 form (same workload, fewer steps).
+fp64_value : the same for the FP64 form (same workload and steps).
 bf16x3_value: the same for the 3 x BF16 split-product form (FP32 accumulate).
 roofline   : the step GEMM (dominant kernel), algorithmic flops 8 d^2 per
-             propagator application, timed per launch with HIP events on the
-             engine stream.
+             propagator application, every launch of the FP32 engine (warmup and
+             timed steps) timed with HIP events on the engine stream.
 cpu_baseline: the FP64 CPU oracle (OpenMP over trajectories, one zgemv per
              step as the reference) on a bounded sample of the same workload
              with the same propagator tree.
+batched_uneven: 32 dimer instances of different dimension (seeded manifest) in
+             one batch: one grouped GEMM per iteration over all of them, HIP-graph
+             replay, launch shape from the tuner; also timed instance by instance.
+serving    : a Poisson stream of ensemble jobs through the pool (continuous
+             batching, admission by HBM footprint), p50 / p99 job latency.
 """
 import argparse
 import json
@@ -37,10 +42,12 @@ def run_form(qjx, prec, N, M, S, D, warmup, steps):
                    diss_gamma=0.1, ex_deep=D, steps_per_period=S, precision=prec)
     e.ensemble_init(M, seed=0, mns=1000000, start_type=0, start_state=0, jump=0)
     e.trans(0)
+    # every step launch of this engine is timed with HIP events (warmup included),
+    # so the per-launch figures cover the same launches a kernel trace of the run sees
+    e.set_profiling(True)
     if warmup:
         e.run_steps(warmup, 0)
     s0 = e.stats()
-    e.set_profiling(True)
     t0 = time.perf_counter()
     e.run_steps(steps, 0)          # returns after the engine stream is synchronized
     dt = time.perf_counter() - t0
@@ -53,10 +60,16 @@ def run_form(qjx, prec, N, M, S, D, warmup, steps):
         "matvecs": mv,
         "matvecs_per_traj_step": mv / (M * steps),
         "jumps_per_traj_period": (s1["jumps"] - s0["jumps"]) / (M * steps / S),
-        "gemm_ms": s1["gemm_ms"],
-        "decide_ms": s1["decide_ms"],
-        "gemm_launches": s1["gemm_launches"],
+        "gemm_ms": s1["gemm_ms"] - s0["gemm_ms"],
+        "decide_ms": s1["decide_ms"] - s0["decide_ms"],
+        "gemm_launches": s1["gemm_launches"] - s0["gemm_launches"],
         "gemm_flops": 8.0 * (N + 1) ** 2 * mv,
+        "gemm_bn": s1["gemm_bn"],
+        "gemm_grid": s1["gemm_grid"],
+        # all launches of the engine (warmup + timed)
+        "all_gemm_ms": s1["gemm_ms"],
+        "all_gemm_launches": s1["gemm_launches"],
+        "all_matvecs": s1["matvecs"],
     }
     return e, res
 
@@ -317,12 +330,13 @@ def main():
                  "scaled": serving(qjx, args.serve_seconds, args.serve_rate, 12345, args.serve_budget_gib,
                                    args.serve_capacity, "scaled")}
 
-    launches = max(1, r32["gemm_launches"])
-    avg_ms = r32["gemm_ms"] / launches
+    launches = max(1, r32["all_gemm_launches"])
+    avg_ms = r32["all_gemm_ms"] / launches
     per_col = measured_traffic_per_column()
-    cols_per_launch = r32["matvecs"] / launches
+    cols_per_launch = r32["all_matvecs"] / launches
+    flops_per_launch = 8.0 * (N + 1) ** 2 * cols_per_launch
     traffic = per_col * cols_per_launch if (per_col and N == 1000) else None
-    achieved = r32["gemm_flops"] / (r32["gemm_ms"] * 1e-3) / 1e12 if r32["gemm_ms"] > 0 else 0.0
+    achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
     line = {
         "metric": METRIC,
         "value": r32["tp_per_s"],
@@ -345,9 +359,12 @@ def main():
                      "traffic_source": "profiles/step_gemm_fp32_traffic_dimer_d1001_M1e5.md (bytes/column x "
                                        "columns/launch)",
                      "algorithmic_bytes_per_launch": 2 * 4 * 2 * 1024 * cols_per_launch if N == 1000 else None,
-                     "kernel": "step GEMM (grouped, gathered columns)",
+                     "kernel": "qjx::gemm_f32<%d> (grouped step GEMM, gathered columns)" % r32["gemm_bn"],
+                     "launches": launches,
                      "avg_launch_ms": avg_ms,
-                     "flops_per_launch": r32["gemm_flops"] / launches},
+                     "flops_per_launch": flops_per_launch,
+                     "timed_steps_gemm_tflops": (r32["gemm_flops"] / (r32["gemm_ms"] * 1e-3) / 1e12
+                                                 if r32["gemm_ms"] > 0 else None)},
         "fp32": r32,
         "fp64_value": fp64["tp_per_s"] if fp64 else None,
         "fp64": fp64,

@@ -67,6 +67,7 @@ struct qjx_pool {
     std::deque<int> queue;             // waiting for admission
     std::vector<int> running;          // ids in the table order
     bool table_dirty = false;
+    long long cycles = 0;                 // scheduling cycles that launched iterations
     // model builds run on a builder thread so that running jobs keep iterating
     std::thread builder;
     std::mutex mu;
@@ -262,7 +263,10 @@ extern "C" int qjx_pool_step(qjx_pool *p, int *running, int *queued)
     int rc = QJX_OK;
     if (p->table_dirty && (rc = sync_table(p))) return rc;
     const bool busy = !p->running.empty();
+    // the launch shape is tuned once, after the first jobs' bisections have spread
+    if (busy && p->cycles >= 4 && (rc = qjx::runner_autotune(p->runner))) return rc;
     if (busy && (rc = qjx::runner_chunk(p->runner, 16))) return rc;
+    if (busy) p->cycles++;
     // built jobs join the table at the next cycle; admission reserves budget and
     // hands the build to the builder thread (it overlaps the chunk just launched)
     {
