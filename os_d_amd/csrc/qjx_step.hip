@@ -29,6 +29,23 @@ typedef double double2v __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
+// Loads and stores of device buffers through the global address space: the
+// pointers come out of the ensemble table, where the compiler cannot see their
+// address space and would emit flat instructions, whose completion an LDS wait
+// (lgkmcnt) also has to cover -- that would serialise every prefetch of the next K
+// block with the fragment reads of the current one.
+#define QJX_GLOBAL __attribute__((address_space(1)))
+template <typename V, typename T>
+__device__ inline V gld(const T *p)
+{
+    return *(const QJX_GLOBAL V *)(p);
+}
+template <typename V, typename T>
+__device__ inline void gst(T *p, V v)
+{
+    *(QJX_GLOBAL V *)(p) = v;
+}
+
 // largest item with pre[item] <= t (pre non-decreasing, pre[n] > t)
 __device__ inline int plan_search(const int *pre, int n, int t)
 {
@@ -155,7 +172,7 @@ __device__ inline bool tile_next(const RunArgs &a, const int *s_pre, int nitems,
     const Ens &E = a.ens[r.e];
     r.rt = local % E.R;
     r.col0 = (local / E.R) * BN;
-    r.ncol = min(BN, E.cnt[a.parity][r.g] - r.col0);
+    r.ncol = min(BN, gld<int>(E.cnt[a.parity] + r.g) - r.col0);
     return true;
 }
 
@@ -167,9 +184,9 @@ __device__ inline void load_cols(const RunArgs &a, const TileRef &r, int *s_tr, 
     if (tid < BN) {
         // columns past ncol repeat the first one: loaded, never stored
         const int c = tid < r.ncol ? tid : 0;
-        const int tr = E.list[a.parity][(size_t)r.g * E.M + r.col0 + c];
+        const int tr = gld<int>(E.list[a.parity] + (size_t)r.g * E.M + r.col0 + c);
         s_tr[tid] = tr;
-        s_cur[tid] = E.st[tr].cur;
+        s_cur[tid] = gld<int>(&E.st[tr].cur);
     }
 }
 
@@ -230,9 +247,9 @@ __global__ __launch_bounds__(256) void gemm_f32(RunArgs a)
         floatx4 ra[4], rb[NQ];
         auto gload = [&](int k0) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) ra[q] = *reinterpret_cast<const floatx4 *>(arow + q * astep + k0);
+            for (int q = 0; q < 4; q++) ra[q] = gld<floatx4>(arow + q * astep + k0);
 #pragma unroll
-            for (int q = 0; q < NQ; q++) rb[q] = *reinterpret_cast<const floatx4 *>(bcol[q] + k0);
+            for (int q = 0; q < NQ; q++) rb[q] = gld<floatx4>(bcol[q] + k0);
         };
         auto lstore = [&](int buf) {
 #pragma unroll
@@ -288,7 +305,7 @@ __global__ __launch_bounds__(256) void gemm_f32(RunArgs a)
                 for (int q = 0; q < 4; q++) {
                     const int row = r0 + wm * 64 + i * 32 + 8 * q + 4 * h;
                     floatx4 v = {acc[i][j][4 * q], acc[i][j][4 * q + 1], acc[i][j][4 * q + 2], acc[i][j][4 * q + 3]};
-                    if (ok) *reinterpret_cast<floatx4 *>(y + row) = v;
+                    if (ok) gst<floatx4>(y + row, v);
 #pragma unroll
                     for (int e = 0; e < 4; e++) ns[j] += (double)v[e] * (double)v[e];
                 }
@@ -300,7 +317,7 @@ __global__ __launch_bounds__(256) void gemm_f32(RunArgs a)
             for (int j = 0; j < NJ; j++) s_norm[wm][wn * (BN / 2) + j * 32 + li] = ns[j];
         }
         __syncthreads();
-        if (tid < r.ncol) E.normpart[(size_t)r.rt * E.M + s_tr[tid]] = s_norm[0][tid] + s_norm[1][tid];
+        if (tid < r.ncol) gst<double>(E.normpart + (size_t)r.rt * E.M + s_tr[tid], s_norm[0][tid] + s_norm[1][tid]);
         __syncthreads();
     }
 }
@@ -371,11 +388,11 @@ __global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
 #pragma unroll
             for (int q = 0; q < 2; q++) {
                 const size_t o = (size_t)(r0 + ca_r + 64 * q) * L + k0 + ca_k;
-                rah[q] = *reinterpret_cast<const bf16x8 *>(Uh + o);
-                ral[q] = *reinterpret_cast<const bf16x8 *>(Ul + o);
+                rah[q] = gld<bf16x8>(Uh + o);
+                ral[q] = gld<bf16x8>(Ul + o);
             }
 #pragma unroll
-            for (int q = 0; q < NQ; q++) rb[q] = *reinterpret_cast<const floatx4 *>(bcol[q] + k0);
+            for (int q = 0; q < NQ; q++) rb[q] = gld<floatx4>(bcol[q] + k0);
         };
         auto lstore = [&](int buf) {
 #pragma unroll
@@ -421,14 +438,23 @@ __global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
                     fbh[j] = *reinterpret_cast<const bf16x8 *>(&Bh[cur][ob]);
                     fbl[j] = *reinterpret_cast<const bf16x8 *>(&Bl[cur][ob]);
                 }
+                // the three products go round the 2 x NJ accumulators in turn, so
+                // consecutive MFMAs never depend on each other
 #pragma unroll
                 for (int i = 0; i < 2; i++)
 #pragma unroll
-                    for (int j = 0; j < NJ; j++) {
+                    for (int j = 0; j < NJ; j++)
                         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fal[i], fbh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+#pragma unroll
+                    for (int j = 0; j < NJ; j++)
                         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fah[i], fbl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+#pragma unroll
+                    for (int j = 0; j < NJ; j++)
                         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fah[i], fbh[j], acc[i][j], 0, 0, 0);
-                    }
             }
             if (more) lstore(cur ^ 1);
             __syncthreads();
@@ -447,7 +473,7 @@ __global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
                 for (int q = 0; q < 4; q++) {
                     const int row = r0 + wm * 64 + i * 32 + 8 * q + 4 * h;
                     floatx4 v = {acc[i][j][4 * q], acc[i][j][4 * q + 1], acc[i][j][4 * q + 2], acc[i][j][4 * q + 3]};
-                    if (ok) *reinterpret_cast<floatx4 *>(y + row) = v;
+                    if (ok) gst<floatx4>(y + row, v);
 #pragma unroll
                     for (int e = 0; e < 4; e++) ns[j] += (double)v[e] * (double)v[e];
                 }
@@ -459,7 +485,7 @@ __global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
             for (int j = 0; j < NJ; j++) s_norm[wm][wn * (BN / 2) + j * 32 + li] = ns[j];
         }
         __syncthreads();
-        if (tid < r.ncol) E.normpart[(size_t)r.rt * E.M + s_tr[tid]] = s_norm[0][tid] + s_norm[1][tid];
+        if (tid < r.ncol) gst<double>(E.normpart + (size_t)r.rt * E.M + s_tr[tid], s_norm[0][tid] + s_norm[1][tid]);
         __syncthreads();
     }
 }
@@ -514,9 +540,9 @@ __global__ __launch_bounds__(512) void gemm_f64(RunArgs a)
         double2v ra[4], rb[NQ];
         auto gload = [&](int k0) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) ra[q] = *reinterpret_cast<const double2v *>(arow + q * astep + k0);
+            for (int q = 0; q < 4; q++) ra[q] = gld<double2v>(arow + q * astep + k0);
 #pragma unroll
-            for (int q = 0; q < NQ; q++) rb[q] = *reinterpret_cast<const double2v *>(bcol[q] + k0);
+            for (int q = 0; q < NQ; q++) rb[q] = gld<double2v>(bcol[q] + k0);
         };
         auto lstore = [&]() {
 #pragma unroll
@@ -578,7 +604,7 @@ __global__ __launch_bounds__(512) void gemm_f64(RunArgs a)
                 const int row = r0 + wm * 32 + i * 16 + qq;
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
-                    if (ok) y[row + 4 * e] = acc[i][j][e];
+                    if (ok) gst<double>(y + row + 4 * e, acc[i][j][e]);
                     ns[j] += acc[i][j][e] * acc[i][j][e];
                 }
             }
@@ -594,8 +620,8 @@ __global__ __launch_bounds__(512) void gemm_f64(RunArgs a)
         }
         __syncthreads();
         if (tid < r.ncol)
-            E.normpart[(size_t)r.rt * E.M + s_tr[tid]] =
-                ((s_norm[0][tid] + s_norm[1][tid]) + s_norm[2][tid]) + s_norm[3][tid];
+            gst<double>(E.normpart + (size_t)r.rt * E.M + s_tr[tid],
+                        ((s_norm[0][tid] + s_norm[1][tid]) + s_norm[2][tid]) + s_norm[3][tid]);
         __syncthreads();
     }
 }
