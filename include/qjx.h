@@ -117,6 +117,13 @@ int qjx_run_steps(qjx_ctx *ctx, int n, int is_obs);
 int qjx_chars(qjx_ctx *ctx, int is_start, double *norm, double *mean, double *dispersion,
               double *m2, double *energy, double *adr_avg, double *adr);
 
+/* Ensemble density matrix rho = (1/M) sum_tr |psi_tr><psi_tr| / ||psi_tr||^2
+ * (d x d complex, row-major interleaved; may be NULL) and its purity Tr rho^2
+ * (may be NULL).  The reference's QJ path outputs only per-trajectory
+ * observables; its Lindblad solver writes the final rho of the same model
+ * (CQdiss_os_dimer characteristics.cpp:3-6), which this estimates at any dump. */
+int qjx_density(qjx_ctx *ctx, double *rho, double *purity);
+
 /* Current states (num_trajectories x d interleaved complex), eta, time, logged
  * jump counts and RNG states; any pointer may be NULL. */
 int qjx_get_state(qjx_ctx *ctx, double *psi, double *eta, double *t, int *njumps, uint64_t *rng);

@@ -590,6 +590,35 @@ extern "C" int qjx_chars(qjx_ctx *c, int is_start, double *norm, double *mean, d
     return QJX_OK;
 }
 
+extern "C" int qjx_density(qjx_ctx *c, double *rho, double *purity)
+{
+    if (!c || !c->have_ens) return fail(QJX_ERR_ARG, "ensemble not initialised");
+    const int splits = qjx::density_splits(c->M, c->L);
+    const size_t nrho = 2 * (size_t)c->d * c->d;
+    double *w = nullptr, *part = nullptr, *drho = nullptr;
+    auto release = [&]() {
+        dev_free(w);
+        dev_free(part);
+        dev_free(drho);
+    };
+    if (hipMalloc((void **)&w, ((size_t)c->M + 1) * sizeof(double)) != hipSuccess ||
+        hipMalloc((void **)&part, (size_t)splits * c->L * c->L * sizeof(double)) != hipSuccess ||
+        hipMalloc((void **)&drho, nrho * sizeof(double)) != hipSuccess) {
+        release();
+        return fail(QJX_ERR_NOMEM, "density scratch does not fit");
+    }
+    qjx::launch_density(c->prec, c->slot[0], c->slot[1], c->st, c->M, c->d, c->dp, c->L, w, part, splits,
+                        drho, w + c->M, c->stream);
+    int rc = QJX_OK;
+    if (hipGetLastError() != hipSuccess ||
+        (rho && hipMemcpyAsync(rho, drho, nrho * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess) ||
+        (purity && hipMemcpyAsync(purity, w + c->M, sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess) ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        rc = fail(QJX_ERR_HIP, "density launch failed");
+    release();
+    return rc;
+}
+
 extern "C" int qjx_get_state(qjx_ctx *c, double *psi, double *eta, double *t, int *njumps, uint64_t *rng)
 {
     if (!c || !c->have_ens) return fail(QJX_ERR_ARG, "ensemble not initialised");

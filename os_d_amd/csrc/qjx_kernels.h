@@ -61,6 +61,14 @@ void launch_export_state(int prec, const void *slot0, const void *slot1, const T
 void launch_import_state(int prec, void *slot0, void *slot1, const TrajState *st, int M, int d,
                          int dp, int L, const double *in, hipStream_t s);
 
+// Ensemble density matrix rho = (1/M) sum psi psi^+ / ||psi||^2 (d x d complex
+// FP64, row-major interleaved) and purity Tr rho^2.  Scratch: w[M] and
+// part[splits][L][L] doubles, splits = density_splits(M, L).
+int density_splits(int M, int L);
+void launch_density(int prec, const void *slot0, const void *slot1, const TrajState *st, int M, int d,
+                    int dp, int L, double *w, double *part, int splits, double *rho, double *purity,
+                    hipStream_t s);
+
 // Dense complex FP64 helpers for the propagator build (row-major n x n, interleaved).
 void launch_zgemm(int n, const double *A, const double *B, double *C, hipStream_t s);
 // C = alpha * A + beta * I  (alpha, beta real)

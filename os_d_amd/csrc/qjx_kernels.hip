@@ -5,13 +5,11 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "qjx_device.h"
 #include "qjx_internal.h"
 #include "qjx_kernels.h"
-
-namespace qjx {
-
-}  // namespace qjx
 
 namespace qjx {
 
@@ -411,6 +409,145 @@ void launch_pack_propagator(int prec, int n, int dp, const double *U, void *dst,
     else if (prec == kBF16x3)
         pack_propagator_bf16x3_kernel<<<2048, 256, 0, s>>>(n, dp, U, (__bf16 *)dst, (__bf16 *)dst_lo);
     else pack_propagator_kernel<float><<<2048, 256, 0, s>>>(n, dp, U, (float *)dst);
+}
+
+// ---------------------------------------------------------------------------
+// Ensemble density matrix rho = (1/M) sum_tr psi psi^+ / ||psi||^2 and its purity
+// Tr rho^2.  G = X W X^T over the block-real state columns X (L x M) with
+// W = diag(1 / (M ||psi_tr||^2)) is accumulated as split-K partial sums of 64 x 64
+// tiles (FP64, fixed order); rho is read off the four dp x dp blocks of G:
+// Re rho = G_rr + G_ii, Im rho = G_ir - G_ri.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void density_weight_kernel(const T *slot0, const T *slot1,
+                                                             const TrajState *st, int M, int L, double *w)
+{
+    const int tr = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    if (tr >= M) return;
+    const T *x = (st[tr].cur ? slot1 : slot0) + (size_t)tr * L;
+    double s = 0.0;
+    for (int k = lane; k < L; k += 64) {
+        const double v = (double)x[k];
+        s += v * v;
+    }
+    s = wave_sum(s);
+    if (lane == 0) w[tr] = s > 0.0 ? 1.0 / ((double)M * s) : 0.0;
+}
+
+constexpr int kDenTile = 64, kDenK = 16;
+
+template <typename T>
+__global__ __launch_bounds__(256) void density_gram_kernel(const T *slot0, const T *slot1,
+                                                           const TrajState *st, const double *w, int M,
+                                                           int L, int kper, double *part)
+{
+    __shared__ double As[kDenK][kDenTile + 1];
+    __shared__ double Bs[kDenK][kDenTile + 1];
+    const int a0 = blockIdx.x * kDenTile, b0 = blockIdx.y * kDenTile;
+    const int k_begin = blockIdx.z * kper, k_end = min(M, k_begin + kper);
+    const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
+    double acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
+    for (int k0 = k_begin; k0 < k_end; k0 += kDenK) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int e = tid + 256 * q;          // kDenK x kDenTile values per operand
+            const int kk = e / kDenTile, r = e % kDenTile;
+            const int tr = k0 + kk;
+            double va = 0.0, vb = 0.0;
+            if (tr < k_end) {
+                const T *x = (st[tr].cur ? slot1 : slot0) + (size_t)tr * L;
+                va = (double)x[a0 + r] * w[tr];
+                vb = (double)x[b0 + r];
+            }
+            As[kk][r] = va;
+            Bs[kk][r] = vb;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kDenK; kk++) {
+            double av[4], bv[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) av[i] = As[kk][ty + 16 * i];
+#pragma unroll
+            for (int j = 0; j < 4; j++) bv[j] = Bs[kk][tx + 16 * j];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[i][j] += av[i] * bv[j];
+        }
+        __syncthreads();
+    }
+    double *out = part + (size_t)blockIdx.z * L * L;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) out[(size_t)(a0 + ty + 16 * i) * L + b0 + tx + 16 * j] = acc[i][j];
+}
+
+__global__ void density_reduce_kernel(const double *part, int splits, int d, int dp, int L, double *rho)
+{
+    const size_t n = (size_t)d * d;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        const int a = (int)(e / d), b = (int)(e % d);
+        double re = 0.0, im = 0.0;
+        for (int s = 0; s < splits; s++) {
+            const double *G = part + (size_t)s * L * L;
+            re += G[(size_t)a * L + b] + G[(size_t)(dp + a) * L + dp + b];
+            im += G[(size_t)(dp + a) * L + b] - G[(size_t)a * L + dp + b];
+        }
+        rho[2 * e] = re;
+        rho[2 * e + 1] = im;
+    }
+}
+
+// Tr rho^2 = sum |rho_ab|^2 (rho Hermitian): one workgroup, fixed order.
+__global__ __launch_bounds__(1024) void purity_kernel(const double *rho, size_t n, double *out)
+{
+    __shared__ double s[1024];
+    double v = 0.0;
+    for (size_t e = threadIdx.x; e < n; e += 1024) v += rho[2 * e] * rho[2 * e] + rho[2 * e + 1] * rho[2 * e + 1];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = s[0];
+}
+
+int density_splits(int M, int L)
+{
+    const int tiles = (L / kDenTile) * (L / kDenTile);
+    int sp = std::max(1, std::min(16, 2048 / std::max(1, tiles)));
+    sp = std::min(sp, std::max(1, (M + kDenK - 1) / kDenK));
+    return sp;
+}
+
+void launch_density(int prec, const void *slot0, const void *slot1, const TrajState *st, int M, int d,
+                    int dp, int L, double *w, double *part, int splits, double *rho, double *purity,
+                    hipStream_t s)
+{
+    const int wblocks = (M + 3) / 4;          // four waves (trajectories) per workgroup
+    int kper = (M + splits - 1) / splits;
+    kper = (kper + kDenK - 1) / kDenK * kDenK;
+    const dim3 grid(L / kDenTile, L / kDenTile, splits);
+    if (prec == kFP64) {
+        density_weight_kernel<double><<<wblocks, 256, 0, s>>>((const double *)slot0, (const double *)slot1,
+                                                              st, M, L, w);
+        density_gram_kernel<double><<<grid, 256, 0, s>>>((const double *)slot0, (const double *)slot1, st, w,
+                                                         M, L, kper, part);
+    } else {
+        density_weight_kernel<float><<<wblocks, 256, 0, s>>>((const float *)slot0, (const float *)slot1, st,
+                                                             M, L, w);
+        density_gram_kernel<float><<<grid, 256, 0, s>>>((const float *)slot0, (const float *)slot1, st, w, M,
+                                                        L, kper, part);
+    }
+    density_reduce_kernel<<<1024, 256, 0, s>>>(part, splits, d, dp, L, rho);
+    purity_kernel<<<1, 1024, 0, s>>>(rho, (size_t)d * d, purity);
 }
 
 }  // namespace qjx

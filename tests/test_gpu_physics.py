@@ -38,11 +38,10 @@ def test_gpu_ensemble_matches_lindblad(oracle_mod, prec):
             v, sig = c[k].mean(), c[k].std() / np.sqrt(M)
             _say("period %d %s: qj %.6f lindblad %.6f (%.2f sigma)" % (p + 1, k, v, lind[k], (v - lind[k]) / sig))
             assert abs(v - lind[k]) < 3.0 * sig
-    # ensemble density matrix from the states: rho_qj = mean_tr psi psi^dag / <psi|psi>
-    psi = e.get_state()["psi"]
-    psi = psi / np.linalg.norm(psi, axis=1, keepdims=True)
-    rho_qj = psi.T @ psi.conj() / M
-    pur_qj = np.real(np.trace(rho_qj @ rho_qj))
-    _say("purity: qj %.5f lindblad %.5f" % (pur_qj, lind["purity"]))
+    # ensemble density matrix on the GPU: rho_qj = mean_tr psi psi^dag / <psi|psi>
+    rho_qj, pur_qj = e.density()
+    err = np.abs(rho_qj - rho).max()
+    _say("purity: qj %.5f lindblad %.5f; max |rho_qj - rho_lindblad| %.2e" % (pur_qj, lind["purity"], err))
     assert abs(pur_qj - lind["purity"]) < 0.01
+    assert err < 0.02
     e.close()
