@@ -53,6 +53,7 @@ def run_form(qjx, prec, N, M, S, D, warmup, steps):
     dt = time.perf_counter() - t0
     s1 = e.stats()
     e.set_profiling(False)
+    obs = observables(e, N, M)
     mv = s1["matvecs"] - s0["matvecs"]
     res = {
         "seconds": dt,
@@ -70,8 +71,32 @@ def run_form(qjx, prec, N, M, S, D, warmup, steps):
         "all_gemm_ms": s1["gemm_ms"],
         "all_gemm_launches": s1["gemm_launches"],
         "all_matvecs": s1["matvecs"],
+        "observables": obs,
     }
     return e, res
+
+
+def observables(e, N, M):
+    """<n1 - n2>/N over the ensemble (k = n2 in the dimer basis, so n1 - n2 = N - 2k)
+    with its standard error, and the purity of the ensemble density matrix."""
+    import numpy as np
+    c = e.chars()
+    x = (N - 2.0 * c["mean"]) / N
+    _, pur = e.density(rho=False)
+    return {"n1_minus_n2_over_N": float(x.mean()), "sigma": float(x.std() / np.sqrt(M)), "purity": pur}
+
+
+def observable_errors(form, ref):
+    """Error of a reduced-precision run's observables against the FP64 run (same
+    workload, seeds and steps; independent jump sequences, so the mean's error is
+    judged against the combined standard error)."""
+    if not form or not ref:
+        return None
+    a, b = form["observables"], ref["observables"]
+    d = a["n1_minus_n2_over_N"] - b["n1_minus_n2_over_N"]
+    s = (a["sigma"] ** 2 + b["sigma"] ** 2) ** 0.5
+    return {"n1_minus_n2_over_N_error": d, "combined_sigma": s, "error_in_sigma": d / s if s > 0 else None,
+            "purity_error": a["purity"] - b["purity"]}
 
 
 def measured_traffic_per_column():
@@ -374,6 +399,7 @@ def main():
         "bf16x3_over_fp64": (bf["tp_per_s"] / fp64["tp_per_s"]) if (bf and fp64) else None,
         "bf16x3_gemm_tflops_equiv": (bf["gemm_flops"] / (bf["gemm_ms"] * 1e-3) / 1e12) if bf else None,
         "cpu_baseline": cpu,
+        "observables_vs_fp64": {"fp32": observable_errors(r32, fp64), "bf16x3": observable_errors(bf, fp64)},
         "batched_uneven": batch,
         "p99_job_latency_s": serve["scaled"]["p99_latency_s"] if serve else None,
         "p99_job_latency_note": "from the scaled job mix (the specified mix completes no job within the window)",
