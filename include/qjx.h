@@ -86,6 +86,9 @@ typedef struct qjx_stats {
     int64_t gemm_launches;  /* step GEMM launches timed                             */
     int32_t gemm_bn;        /* column tile width of the step GEMM (launch-shape tuner) */
     int32_t gemm_grid;      /* persistent GEMM workgroups                           */
+    double gemm_band_flops; /* flops of the K blocks the matrix-core GEMM multiplied:
+                               2 x 128 x 32 per (column, row tile, K block) inside the
+                               propagators' band tables (exact zeros skipped)      */
 } qjx_stats;
 
 const char *qjx_last_error(void);

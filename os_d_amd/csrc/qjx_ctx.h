@@ -28,6 +28,8 @@ struct qjx_ctx {
     // device model
     void *props = nullptr;         // 2*D * L*L elements
     void *props_lo = nullptr;      // BF16x3: low halves
+    int *krange = nullptr;         // [G][R][2] K-block ranges of the propagators' row tiles
+    bool dense = false;            // QJX_DENSE=1: every K block (no band table)
     double *d_dt = nullptr, *d_Ldiag = nullptr, *d_Loff = nullptr;
     double *d_HdiagE = nullptr, *d_Hoff = nullptr;
 

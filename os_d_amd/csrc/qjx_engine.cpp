@@ -190,10 +190,21 @@ static void *prop_ptr(qjx_ctx *c, int b, int lvl)
     return (char *)c->props + ((size_t)(b * c->D + lvl)) * c->L * c->L * c->pesize;
 }
 
+static void band_of(qjx_ctx *c, int b, int lvl);
+
 static void *prop_lo_ptr(qjx_ctx *c, int b, int lvl)
 {
     if (!c->props_lo) return nullptr;
     return (char *)c->props_lo + ((size_t)(b * c->D + lvl)) * c->L * c->L * c->pesize;
+}
+
+// Band table of propagator (b, lvl) on the engine stream (MFMA kernels only read it).
+static void band_of(qjx_ctx *c, int b, int lvl)
+{
+    const int g = b * c->D + lvl;
+    qjx::launch_band(c->prec, prop_ptr(c, b, lvl), prop_lo_ptr(c, b, lvl), c->L, c->dp,
+                     c->dense || c->gemm_kind != qjx::kGemmMfma, c->krange + (size_t)2 * g * (c->L / 128),
+                     c->stream);
 }
 
 extern "C" const char *qjx_last_error(void) { return g_err.c_str(); }
@@ -282,6 +293,9 @@ int qjx::ctx_create_kind(const qjx_model_desc *m, int gemm_kind, qjx_ctx **out)
         return bail(fail(QJX_ERR_NOMEM, "propagator tree does not fit in device memory"));
     if (c->prec == QJX_BF16X3 && hipMalloc(&c->props_lo, propbytes) != hipSuccess)
         return bail(fail(QJX_ERR_NOMEM, "propagator tree does not fit in device memory"));
+    if (hipMalloc((void **)&c->krange, (size_t)2 * c->G * (c->L / 128) * sizeof(int)) != hipSuccess)
+        return bail(fail(QJX_ERR_NOMEM, "band table does not fit in device memory"));
+    if (const char *env = std::getenv("QJX_DENSE")) c->dense = std::string(env) == "1";
 
     // Propagator tree (qj_proc.cpp:4-120): U[b][i] = exp(Hqj[b] * dt_i).
     const size_t nn2 = (size_t)2 * c->d * c->d;
@@ -305,9 +319,11 @@ int qjx::ctx_create_kind(const qjx_model_desc *m, int gemm_kind, qjx_ctx **out)
         }
         for (int i = 0; i < c->D && rc == QJX_OK; i++) {
             rc = device_expm(c, dA, a1, c->dt[i], U, work);
-            if (rc == QJX_OK)
+            if (rc == QJX_OK) {
                 qjx::launch_pack_propagator(c->prec, c->d, c->dp, U, prop_ptr(c, b, i), prop_lo_ptr(c, b, i),
                                             c->stream);
+                band_of(c, b, i);
+            }
         }
     }
     if (rc == QJX_OK && hipStreamSynchronize(c->stream) != hipSuccess)
@@ -353,6 +369,7 @@ extern "C" void qjx_ctx_destroy(qjx_ctx *c)
     free_ensemble(c);
     if (c->props) (void)hipFree(c->props);
     if (c->props_lo) (void)hipFree(c->props_lo);
+    dev_free(c->krange);
     dev_free(c->d_dt);
     dev_free(c->d_Ldiag);
     dev_free(c->d_Loff);
@@ -384,6 +401,7 @@ static void build_desc(qjx_ctx *c)
     qjx::Ens &E = c->desc;
     E.props = c->props;
     E.props_lo = c->props_lo;
+    E.krange = c->krange;
     for (int i = 0; i < 2; i++) {
         E.slot[i] = c->slot[i];
         E.list[i] = c->lists[i];
@@ -447,8 +465,8 @@ extern "C" int qjx_ensemble_init(qjx_ctx *c, const qjx_ensemble_desc *e)
     HIP_TRY(hipMalloc((void **)&c->normpart, (size_t)c->R * M * sizeof(double)));
     HIP_TRY(hipMalloc((void **)&c->err, 4 * sizeof(int)));
     HIP_TRY(hipMemset(c->err, 0, 4 * sizeof(int)));
-    HIP_TRY(hipMalloc((void **)&c->counters, 2 * sizeof(unsigned long long)));
-    HIP_TRY(hipMemset(c->counters, 0, 2 * sizeof(unsigned long long)));
+    HIP_TRY(hipMalloc((void **)&c->counters, 4 * sizeof(unsigned long long)));
+    HIP_TRY(hipMemset(c->counters, 0, 4 * sizeof(unsigned long long)));
     HIP_TRY(hipMalloc((void **)&c->mean_start, M * sizeof(double)));
     HIP_TRY(hipMemset(c->mean_start, 0, M * sizeof(double)));
     HIP_TRY(hipMalloc((void **)&c->obs, 5 * M * sizeof(double)));
@@ -719,6 +737,7 @@ extern "C" int qjx_set_propagator(qjx_ctx *c, int b, int lvl, const double *U)
     if (e == hipSuccess) {
         qjx::launch_pack_propagator(c->prec, c->d, c->dp, dU, prop_ptr(c, b, lvl), prop_lo_ptr(c, b, lvl),
                                     c->stream);
+        band_of(c, b, lvl);
         e = hipStreamSynchronize(c->stream);
     }
     (void)hipFree(dU);
@@ -729,12 +748,13 @@ extern "C" int qjx_set_propagator(qjx_ctx *c, int b, int lvl, const double *U)
 extern "C" int qjx_get_stats(qjx_ctx *c, qjx_stats *out)
 {
     if (!c || !out) return fail(QJX_ERR_ARG, "null argument");
-    unsigned long long h[2] = {0, 0};
+    unsigned long long h[3] = {0, 0, 0};
     if (c->counters) HIP_TRY(hipMemcpy(h, c->counters, sizeof h, hipMemcpyDeviceToHost));
     out->matvecs = (int64_t)h[0];
     out->iterations = c->runner ? c->runner->iterations : 0;
     out->jumps = (int64_t)h[1];
     out->gemm_flops = 8.0 * double(c->d) * double(c->d) * double(out->matvecs);
+    out->gemm_band_flops = double(h[2]) * (2.0 * 128 * 32);
     out->gemm_ms = c->runner ? c->runner->gemm_ms : 0.0;
     out->decide_ms = c->runner ? c->runner->tail_ms + c->runner->plan_ms : 0.0;
     out->gemm_launches = c->runner ? c->runner->gemm_launches : 0;
@@ -880,11 +900,12 @@ extern "C" int qjx_batch_get_stats(qjx_batch *b, qjx_stats *out)
     if (!b || !out) return fail(QJX_ERR_ARG, "null argument");
     *out = qjx_stats{};
     for (qjx_ctx *c : b->ctx) {
-        unsigned long long h[2] = {0, 0};
+        unsigned long long h[3] = {0, 0, 0};
         HIP_TRY(hipMemcpy(h, c->counters, sizeof h, hipMemcpyDeviceToHost));
         out->matvecs += (int64_t)h[0];
         out->jumps += (int64_t)h[1];
         out->gemm_flops += 8.0 * double(c->d) * double(c->d) * double(h[0]);
+        out->gemm_band_flops += double(h[2]) * (2.0 * 128 * 32);
     }
     const qjx::Runner *r = b->runner;
     out->iterations = r->iterations;

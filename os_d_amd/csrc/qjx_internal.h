@@ -57,6 +57,8 @@ constexpr int kPlanItems = kMaxEns * kMaxGroups;     // (ensemble, group) items
 struct Ens {
     const void *props;        // 2*D matrices, each L*L elements (L = 2*dp), row-major
     const void *props_lo;     // BF16x3 only: low halves
+    const int *krange;        // [G][R][2]: K blocks (of 32) [lo, hi) holding every nonzero of
+                              // row tile rt within each half of the operand (exact zeros skipped)
     void *slot[2];            // state slots, M*L elements each
     TrajState *st;
     int *list[2];             // per parity: [G * M] trajectory ids per group
@@ -67,7 +69,8 @@ struct Ens {
     const double *Loff;       // [d-1] L_{k+1,k} = -Loff[k], L_{k,k+1} = +Loff[k]
     JumpLog log;
     int *err;                 // [4]: code, trajectory, bits of the offending norm
-    unsigned long long *counters;  // [0] matvecs, [1] jumps
+    unsigned long long *counters;  // [0] matvecs, [1] jumps, [2] GEMM work in (column x K block
+                                   // of one 128-row tile) units
     uint64_t stride;          // MCG59 leap-frog multiplier a^mns
     double norm_tol;          // allowed excess of ||psi||^2 over 1 (0 for FP64)
     int d, dp, L, D, S, M, G, R;

@@ -550,4 +550,66 @@ void launch_density(int prec, const void *slot0, const void *slot1, const TrajSt
     purity_kernel<<<1, 1024, 0, s>>>(rho, (size_t)d * d, purity);
 }
 
+// ---------------------------------------------------------------------------
+// Band table of a packed operand: for each 128-row tile, the K blocks (of 32
+// columns, counted within a half of the block-real operand) between the first
+// and the last nonzero entry of its rows in either half.  The step GEMM skips the
+// blocks outside, which hold exact zeros only, so its results do not change.
+// The propagators of a tridiagonal H_eff are banded: the Taylor polynomial of
+// A / 2^s has bandwidth 18 and each squaring doubles it, and entries far from the
+// diagonal underflow, in FP32 storage soonest.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ inline bool nonzero_at(const T *P, const T *Plo, size_t o)
+{
+    return P[o] != T(0) || (Plo && Plo[o] != T(0));
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void band_kernel(const T *P, const T *Plo, int L, int dp, int *out)
+{
+    __shared__ int s_lo, s_hi;
+    if (threadIdx.x == 0) {
+        s_lo = dp;
+        s_hi = 0;
+    }
+    __syncthreads();
+    const int r0 = blockIdx.x * 128;
+    int lo = dp, hi = 0;
+    for (int e = threadIdx.x; e < 128 * L; e += blockDim.x) {
+        const int r = r0 + e / L, k = e % L;
+        if (nonzero_at(P, Plo, (size_t)r * L + k)) {
+            const int kk = k < dp ? k : k - dp;
+            lo = min(lo, kk);
+            hi = max(hi, kk + 1);
+        }
+    }
+    atomicMin(&s_lo, lo);
+    atomicMax(&s_hi, hi);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const bool any = s_hi > s_lo;
+        out[2 * blockIdx.x] = any ? s_lo / 32 : 0;
+        out[2 * blockIdx.x + 1] = any ? (s_hi + 31) / 32 : 0;
+    }
+}
+
+__global__ void band_dense_kernel(int R, int dp, int *out)
+{
+    const int rt = blockIdx.x * blockDim.x + threadIdx.x;
+    if (rt >= R) return;
+    out[2 * rt] = 0;
+    out[2 * rt + 1] = dp / 32;
+}
+
+void launch_band(int prec, const void *P, const void *Plo, int L, int dp, bool dense, int *out, hipStream_t s)
+{
+    const int R = L / 128;
+    if (dense) band_dense_kernel<<<(R + 63) / 64, 64, 0, s>>>(R, dp, out);
+    else if (prec == kFP64) band_kernel<double><<<R, 256, 0, s>>>((const double *)P, nullptr, L, dp, out);
+    else if (prec == kBF16x3)
+        band_kernel<__bf16><<<R, 256, 0, s>>>((const __bf16 *)P, (const __bf16 *)Plo, L, dp, out);
+    else band_kernel<float><<<R, 256, 0, s>>>((const float *)P, nullptr, L, dp, out);
+}
+
 }  // namespace qjx

@@ -204,6 +204,26 @@ __device__ inline void load_cols(const RunArgs &a, const TileRef &r, int *s_tr, 
 constexpr int MF_BM = 128, MF_BK = 32;
 constexpr int F32_PITCH = MF_BK + 4;
 
+// The K blocks a tile multiplies: [kb0, kb0 + nb) of the operand's first half,
+// then the same blocks of its second half (offset dp).  The blocks outside hold
+// exact zeros of the propagator's row tile (band table), so skipping them leaves
+// every accumulator bit-identical; nb = 0 leaves the tile zero.
+struct KRange {
+    int kb0, nb, hb;
+    __device__ int count() const { return 2 * nb; }
+    __device__ int at(int i) const { return (i < nb ? kb0 + i : hb + kb0 + i - nb) * MF_BK; }
+};
+
+__device__ inline KRange tile_krange(const Ens &E, const TileRef &r)
+{
+    const int *p = E.krange + 2 * ((size_t)r.g * E.R + r.rt);
+    KRange k;
+    k.kb0 = gld<int>(p);
+    k.nb = gld<int>(p + 1) - k.kb0;
+    k.hb = E.dp / MF_BK;
+    return k;
+}
+
 template <int BN>
 __global__ __launch_bounds__(256) void gemm_f32(RunArgs a)
 {
@@ -259,13 +279,17 @@ __global__ __launch_bounds__(256) void gemm_f32(RunArgs a)
             for (int q = 0; q < NQ; q++)
                 *reinterpret_cast<floatx4 *>(&Bs[buf][(cp_r + 32 * q) * F32_PITCH + cp_k]) = rb[q];
         };
-        gload(0);
-        lstore(0);
+        const KRange kr = tile_krange(E, r);
+        const int nk = kr.count();
+        if (nk > 0) {
+            gload(kr.at(0));
+            lstore(0);
+        }
         __syncthreads();
         int cur = 0;
-        for (int k0 = 0; k0 < L; k0 += MF_BK) {
-            const bool more = k0 + MF_BK < L;
-            if (more) gload(k0 + MF_BK);
+        for (int i = 0; i < nk; i++) {
+            const bool more = i + 1 < nk;
+            if (more) gload(kr.at(i + 1));
             floatx4 fa[2][4], fb[NJ][4];
 #pragma unroll
             for (int i = 0; i < 2; i++) {
@@ -318,6 +342,7 @@ __global__ __launch_bounds__(256) void gemm_f32(RunArgs a)
         }
         __syncthreads();
         if (tid < r.ncol) gst<double>(E.normpart + (size_t)r.rt * E.M + s_tr[tid], s_norm[0][tid] + s_norm[1][tid]);
+        if (tid == 0) atomicAdd(&E.counters[2], (unsigned long long)r.ncol * nk);
         __syncthreads();
     }
 }
@@ -416,13 +441,17 @@ __global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
                 *reinterpret_cast<bf16x4 *>(&Bl[buf][bf_off(rr, cb_k)]) = l4;
             }
         };
-        gload(0);
-        lstore(0);
+        const KRange kr = tile_krange(E, r);
+        const int nk = kr.count();
+        if (nk > 0) {
+            gload(kr.at(0));
+            lstore(0);
+        }
         __syncthreads();
         int cur = 0;
-        for (int k0 = 0; k0 < L; k0 += MF_BK) {
-            const bool more = k0 + MF_BK < L;
-            if (more) gload(k0 + MF_BK);
+        for (int i = 0; i < nk; i++) {
+            const bool more = i + 1 < nk;
+            if (more) gload(kr.at(i + 1));
 #pragma unroll
             for (int c = 0; c < MF_BK / 16; c++) {
                 bf16x8 fah[2], fal[2], fbh[NJ], fbl[NJ];
@@ -486,6 +515,7 @@ __global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
         }
         __syncthreads();
         if (tid < r.ncol) gst<double>(E.normpart + (size_t)r.rt * E.M + s_tr[tid], s_norm[0][tid] + s_norm[1][tid]);
+        if (tid == 0) atomicAdd(&E.counters[2], (unsigned long long)r.ncol * nk);
         __syncthreads();
     }
 }
@@ -555,12 +585,16 @@ __global__ __launch_bounds__(512) void gemm_f64(RunArgs a)
         // (single-buffered: at one workgroup per CU a second buffer measured slower)
         const double *pa0 = &As[(wm * 32 + li) * F64_PITCH + 8 * qq];
         const double *pb0 = &Bs[(wn * (BN / 2) + li) * F64_PITCH + 8 * qq];
-        gload(0);
-        lstore();
+        const KRange kr = tile_krange(E, r);
+        const int nk = kr.count();
+        if (nk > 0) {
+            gload(kr.at(0));
+            lstore();
+        }
         __syncthreads();
-        for (int k0 = 0; k0 < L; k0 += MF_BK) {
-            const bool more = k0 + MF_BK < L;
-            if (more) gload(k0 + MF_BK);
+        for (int i = 0; i < nk; i++) {
+            const bool more = i + 1 < nk;
+            if (more) gload(kr.at(i + 1));
 #pragma unroll
             for (int half = 0; half < 2; half++) {
                 double2v fa[2][2], fb[NJ][2];
@@ -622,6 +656,7 @@ __global__ __launch_bounds__(512) void gemm_f64(RunArgs a)
         if (tid < r.ncol)
             gst<double>(E.normpart + (size_t)r.rt * E.M + s_tr[tid],
                         ((s_norm[0][tid] + s_norm[1][tid]) + s_norm[2][tid]) + s_norm[3][tid]);
+        if (tid == 0) atomicAdd(&E.counters[2], (unsigned long long)r.ncol * nk);
         __syncthreads();
     }
 }

@@ -49,7 +49,7 @@ class Stats(ctypes.Structure):
                 ("jumps", ctypes.c_int64), ("gemm_flops", ctypes.c_double),
                 ("gemm_ms", ctypes.c_double), ("decide_ms", ctypes.c_double),
                 ("gemm_launches", ctypes.c_int64), ("gemm_bn", ctypes.c_int32),
-                ("gemm_grid", ctypes.c_int32)]
+                ("gemm_grid", ctypes.c_int32), ("gemm_band_flops", ctypes.c_double)]
 
 
 def _stats_dict(s):

@@ -108,3 +108,24 @@ def test_batch_rejects_mixed_precision():
         qjx.Batch([a, b])
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("prec", [1, 0, 2])
+def test_band_skipping_does_not_change_results(prec, monkeypatch):
+    """The step GEMM multiplies only the K blocks inside each propagator row tile's
+    band (the rest are exact zeros): states must equal a dense run bitwise."""
+    _say("case: band vs dense prec=%d" % prec)
+    res, work = {}, {}
+    for dense in ("1", "0"):
+        monkeypatch.setenv("QJX_DENSE", dense)
+        e = _engine(prec, N=150, M=300, seed=8, drv_ampl=1.5, diss_gamma=0.1, S=200, D=16)
+        e.run_steps(6, 0)
+        res[dense] = e.get_state()
+        work[dense] = e.stats()["gemm_band_flops"]
+        e.close()
+    _say("GEMM flops dense %.3e band %.3e (ratio %.3f)" % (work["1"], work["0"], work["0"] / work["1"]))
+    _same_state(res["0"], res["1"])
+    assert work["0"] <= work["1"]
+    if prec != 0:
+        # FP32 storage underflows far from the diagonal: the band is narrow
+        assert work["0"] < 0.8 * work["1"]
