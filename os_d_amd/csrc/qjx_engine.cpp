@@ -444,7 +444,8 @@ static int ensure_runner(qjx_ctx *c)
 extern "C" int qjx_ensemble_init(qjx_ctx *c, const qjx_ensemble_desc *e)
 {
     if (!c || !e) return fail(QJX_ERR_ARG, "null argument");
-    if (e->num_trajectories < 1) return fail(QJX_ERR_ARG, "num_trajectories must be >= 1");
+    if (e->num_trajectories < 1 || e->num_trajectories > (1 << 30))
+        return fail(QJX_ERR_ARG, "num_trajectories must be in [1, 2^30]");
     if (e->mns < 1) return fail(QJX_ERR_ARG, "mns must be >= 1");
     if (e->start_type == 0 && (e->start_state < 0 || e->start_state >= c->d))
         return fail(QJX_ERR_ARG, "start_state out of range");

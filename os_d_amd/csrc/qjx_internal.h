@@ -61,7 +61,7 @@ struct Ens {
                               // row tile rt within each half of the operand (exact zeros skipped)
     void *slot[2];            // state slots, M*L elements each
     TrajState *st;
-    int *list[2];             // per parity: [G * M] trajectory ids per group
+    int *list[2];             // per parity: [G * M] entries (trajectory << 1) | current slot
     int *cnt[2];              // per parity: [kMaxGroups] list lengths
     double *normpart;         // [R * M]
     const double *dt;         // [D]

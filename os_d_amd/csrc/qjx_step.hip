@@ -129,44 +129,60 @@ __global__ __launch_bounds__(kPlanThreads) void plan_kernel(RunArgs a)
 // Claim this persistent workgroup's next tile.  The tile space is cut into 8
 // contiguous ranges, one per XCD (workgroup w runs on XCD w % 8), so the row tiles
 // of a column tile are claimed on one XCD and share its L2; a workgroup whose range
-// is exhausted takes tiles from the other XCDs' ranges.  Thread 0 claims with an
-// atomic and broadcasts through LDS.
+// is exhausted takes tiles from the other XCDs' ranges.  Lane 0 of wave 0 claims
+// with an atomic; wave 0 then finds the tile's item in the plan prefix with a
+// two-level ballot search (two dependent loads instead of log2(items)) and
+// broadcasts through LDS.
 struct TileRef {
     int e, g, rt, col0, ncol;
 };
 
-// Copy the tile prefix of this iteration's plan into LDS (once per workgroup).
-__device__ inline int load_tile_pre(const RunArgs &a, int *s_pre)
-{
-    const int nitems = *a.nens * kMaxGroups;
-    for (int i = threadIdx.x; i <= nitems; i += blockDim.x) s_pre[i] = a.plan->tile_pre[i];
-    __syncthreads();
-    return nitems;
-}
+static_assert(kPlanItems <= 64 * 64, "two-level tile search");
 
 template <int BN>
-__device__ inline bool tile_next(const RunArgs &a, const int *s_pre, int nitems, int total, int *s_claim,
+__device__ inline bool tile_next(const RunArgs &a, const int *pre, int nitems, int total, int *s_claim,
                                  TileRef &r)
 {
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
         int t = -1;
-        const int x0 = blockIdx.x & 7;
-        for (int i = 0; i < 8 && t < 0; i++) {
-            const int x = (x0 + i) & 7;
-            const int lo = (int)(((long long)total * x) >> 3), hi = (int)(((long long)total * (x + 1)) >> 3);
-            if (hi <= lo) continue;
-            if (a.plan->xnext[x] >= hi - lo) continue;           // cheap pre-check
-            const int c = atomicAdd(&a.plan->xnext[x], 1);
-            if (c < hi - lo) t = lo + c;
+        if (lane == 0) {
+            const int x0 = blockIdx.x & 7;
+            for (int i = 0; i < 8 && t < 0; i++) {
+                const int x = (x0 + i) & 7;
+                const int lo = (int)(((long long)total * x) >> 3), hi = (int)(((long long)total * (x + 1)) >> 3);
+                if (hi <= lo) continue;
+                if (a.plan->xnext[x] >= hi - lo) continue;           // cheap pre-check
+                const int c = atomicAdd(&a.plan->xnext[x], 1);
+                if (c < hi - lo) t = lo + c;
+            }
         }
-        *s_claim = t;
+        t = __shfl(t, 0, 64);
+        int item = 0, base = 0;
+        if (t >= 0) {
+            // the largest item with pre[item] <= t (pre non-decreasing, pre[0] = 0,
+            // pre[nitems] = total > t)
+            const int i1 = lane * 64;
+            const unsigned long long m1 = __ballot(i1 < nitems && gld<int>(pre + i1) <= t);
+            const int b = (63 - __clzll(m1)) * 64;
+            const int i2 = b + lane;
+            const int v2 = i2 < nitems ? gld<int>(pre + i2) : 0x7fffffff;
+            const unsigned long long m2 = __ballot(v2 <= t);
+            const int w = 63 - __clzll(m2);
+            item = b + w;
+            base = __shfl(v2, w, 64);
+        }
+        if (lane == 0) {
+            s_claim[0] = t;
+            s_claim[1] = item;
+            s_claim[2] = base;
+        }
     }
     __syncthreads();
-    const int t = *s_claim;
+    const int t = s_claim[0], item = s_claim[1];
+    const int local = t - s_claim[2];
     __syncthreads();
     if (t < 0) return false;
-    const int item = plan_search(s_pre, nitems, t);
-    const int local = t - s_pre[item];
     r.e = item / kMaxGroups;
     r.g = item % kMaxGroups;
     const Ens &E = a.ens[r.e];
@@ -184,9 +200,9 @@ __device__ inline void load_cols(const RunArgs &a, const TileRef &r, int *s_tr, 
     if (tid < BN) {
         // columns past ncol repeat the first one: loaded, never stored
         const int c = tid < r.ncol ? tid : 0;
-        const int tr = gld<int>(E.list[a.parity] + (size_t)r.g * E.M + r.col0 + c);
-        s_tr[tid] = tr;
-        s_cur[tid] = gld<int>(&E.st[tr].cur);
+        const int ent = gld<int>(E.list[a.parity] + (size_t)r.g * E.M + r.col0 + c);
+        s_tr[tid] = ent >> 1;          // list entries are (trajectory << 1) | current slot
+        s_cur[tid] = ent & 1;
     }
 }
 
@@ -241,9 +257,9 @@ __global__ __launch_bounds__(256) void gemm_f32(RunArgs a)
     const int wm = wid >> 1, wn = wid & 1;
     const int h = lane >> 5, li = lane & 31;
     const int cp_k = (tid & 7) * 4, cp_r = tid >> 3;
-    __shared__ int s_claim;
+    __shared__ int s_claim[3];
     TileRef r;
-    while (tile_next<BN>(a, s_tpre, nitems, total, &s_claim, r)) {
+    while (tile_next<BN>(a, s_tpre, nitems, total, s_claim, r)) {
         load_cols<BN>(a, r, s_tr, s_cur);
         __syncthreads();
         const Ens &E = a.ens[r.e];
@@ -384,9 +400,9 @@ __global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
     const int h = lane >> 5, li = lane & 31;
     const int ca_k = (tid & 3) * 8, ca_r = tid >> 2;
     const int cb_k = (tid & 7) * 4, cb_r = tid >> 3;
-    __shared__ int s_claim;
+    __shared__ int s_claim[3];
     TileRef r;
-    while (tile_next<BN>(a, s_tpre, nitems, total, &s_claim, r)) {
+    while (tile_next<BN>(a, s_tpre, nitems, total, s_claim, r)) {
         load_cols<BN>(a, r, s_tr, s_cur);
         __syncthreads();
         const Ens &E = a.ens[r.e];
@@ -546,9 +562,9 @@ __global__ __launch_bounds__(512) void gemm_f64(RunArgs a)
     const int wm = wid >> 1, wn = wid & 1;
     const int qq = lane >> 4, li = lane & 15;
     const int cp_k = (tid & 15) * 2, cp_r = tid >> 4;
-    __shared__ int s_claim;
+    __shared__ int s_claim[3];
     TileRef r;
-    while (tile_next<BN>(a, s_tpre, nitems, total, &s_claim, r)) {
+    while (tile_next<BN>(a, s_tpre, nitems, total, s_claim, r)) {
         load_cols<BN>(a, r, s_tr, s_cur);
         __syncthreads();
         const Ens &E = a.ens[r.e];
@@ -681,9 +697,9 @@ __global__ __launch_bounds__(256) void gemm_fma(RunArgs a)
     const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
     const int a_i = tid / (FM_BK / 4), a_k = (tid % (FM_BK / 4)) * 4;
     const int b_j = tid / (FM_BK / 4), b_k = (tid % (FM_BK / 4)) * 4;
-    __shared__ int s_claim;
+    __shared__ int s_claim[3];
     TileRef r;
-    while (tile_next<FM_BN>(a, s_tpre, nitems, total, &s_claim, r)) {
+    while (tile_next<FM_BN>(a, s_tpre, nitems, total, s_claim, r)) {
         load_cols<FM_BN>(a, r, s_tr, s_cur);
         __syncthreads();
         const Ens &E = a.ens[r.e];
@@ -797,14 +813,14 @@ __global__ __launch_bounds__(256) void decide_kernel(RunArgs a)
     const int p = a.parity;
     const int idx = base0 + threadIdx.x;
     const int e_lo = plan_search(s_pre, nitems, base0) / kMaxGroups;
-    int e = -1, tr = -1, key = -1, local = -1, direct = -1, jlocal = -1, jsrc = 0;
+    int e = -1, tr = -1, key = -1, local = -1, direct = -1, jlocal = -1, jsrc = 0, ncur = 0;
     double n = 0.0;
     if (idx < total) {
         const int item = plan_search(s_pre, nitems, idx);
         e = item / kMaxGroups;
         const int g = item % kMaxGroups;
         const Ens &E = a.ens[e];
-        tr = E.list[p][(size_t)g * E.M + (idx - s_pre[item])];
+        tr = E.list[p][(size_t)g * E.M + (idx - s_pre[item])] >> 1;
         TrajState s = E.st[tr];
         for (int rt = 0; rt < E.R; rt++) n += E.normpart[(size_t)rt * E.M + tr];
         const int lvl = s.lvl;
@@ -860,6 +876,7 @@ __global__ __launch_bounds__(256) void decide_kernel(RunArgs a)
             }
         }
         E.st[tr] = s;
+        ncur = s.cur;            // the jump kernel writes into this slot, too
         if (alive && s.pos < s.target) {
             const int gn = s.b * E.D + s.lvl;
             key = (e - e_lo) * kMaxGroups + gn;
@@ -886,7 +903,7 @@ __global__ __launch_bounds__(256) void decide_kernel(RunArgs a)
         const Ens &E = a.ens[e];
         const int gn = key % kMaxGroups;
         const int pos = local >= 0 ? s_base[key] + local : direct;
-        E.list[p ^ 1][(size_t)gn * E.M + pos] = tr;
+        E.list[p ^ 1][(size_t)gn * E.M + pos] = (tr << 1) | ncur;
     }
     if (jlocal >= 0) {
         JumpEnt je;
@@ -996,7 +1013,7 @@ __global__ void seed_kernel(TrajState *st, int M, int n, int *list, int *cnt, in
     if (n > 0) {
         const int g = s.b * D;
         const int q = atomicAdd(&cnt[g], 1);
-        list[(size_t)g * M + q] = tr;
+        list[(size_t)g * M + q] = (tr << 1) | s.cur;
     }
 }
 
