@@ -961,6 +961,8 @@ int runner_create(int prec, int gemm_kind, hipStream_t stream, int capacity, Run
         }
     }
     runner_apply_shape(r, bn, gemm_blocks_per_cu(prec, gemm_kind, bn));
+    r->sc = 1;
+    if (const char *env = std::getenv("QJX_SC")) r->sc = std::max(1, std::min(16, std::atoi(env)));
     r->grid.decide = std::max(1, (capacity + 255) / 256);   // one entry per thread
     r->grid.jump = std::max(1, std::min(capacity, 2048));
     if (hipMalloc((void **)&r->d_ens, kMaxEns * sizeof(Ens)) != hipSuccess ||
@@ -1021,6 +1023,7 @@ static RunArgs run_args(Runner *r, int parity)
     a.jcnt = r->d_jcnt;
     a.parity = parity;
     a.bn = r->bn;
+    a.sc = r->sc;
     return a;
 }
 

@@ -102,6 +102,7 @@ struct RunArgs {
     int *jcnt;
     int parity;               // lists read this iteration: list[parity]
     int bn;                   // GEMM column tile
+    int sc;                   // column tiles per super-column (tile order, L2 reuse of U)
 };
 
 }  // namespace qjx

@@ -186,9 +186,17 @@ __device__ inline bool tile_next(const RunArgs &a, const int *pre, int nitems, i
     r.e = item / kMaxGroups;
     r.g = item % kMaxGroups;
     const Ens &E = a.ens[r.e];
-    r.rt = local % E.R;
-    r.col0 = (local / E.R) * BN;
-    r.ncol = min(BN, gld<int>(E.cnt[a.parity] + r.g) - r.col0);
+    // tiles of an item: super-columns of a.sc column tiles, each visited row tile
+    // by row tile, so the row tile's propagator band is reused from L2 by the
+    // super-column's column tiles and the column tiles' states by the row tiles
+    const int cnt = gld<int>(E.cnt[a.parity] + r.g);
+    const int nct = (cnt + BN - 1) / BN;
+    const int per = E.R * a.sc;
+    const int sc = local / per, rem = local - sc * per;
+    const int cw = min(a.sc, nct - sc * a.sc);
+    r.rt = rem / cw;
+    r.col0 = (sc * a.sc + rem % cw) * BN;
+    r.ncol = min(BN, cnt - r.col0);
     return true;
 }
 
