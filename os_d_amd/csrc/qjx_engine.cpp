@@ -955,12 +955,20 @@ int runner_create(int prec, int gemm_kind, hipStream_t stream, int capacity, Run
         if (std::string(env) == "0") r->autotune = false;
     if (const char *env = std::getenv("QJX_BN")) {
         const std::string v(env);
-        if (gemm_kind != kGemmFma && (v == "64" || v == "128")) {
+        if (gemm_kind != kGemmFma && (v == "64" || v == "128" || (v == "256" && prec == kBF16x3))) {
             bn = std::stoi(v);
             r->autotune = false;
         }
     }
-    runner_apply_shape(r, bn, gemm_blocks_per_cu(prec, gemm_kind, bn));
+    int waves = bn == 256 ? 8 : 4;       // the 256-column tile has only an 8-wave kernel
+    if (const char *env = std::getenv("QJX_WAVES")) {
+        if (prec == kBF16x3 && gemm_kind == kGemmMfma && std::string(env) == "8") {
+            waves = 8;
+            bn = bn == 256 ? 256 : 128;
+            r->autotune = false;
+        }
+    }
+    runner_apply_shape(r, bn, gemm_blocks_per_cu(prec, gemm_kind, bn, waves), waves);
     r->sc = 1;
     if (const char *env = std::getenv("QJX_SC")) r->sc = std::max(1, std::min(16, std::atoi(env)));
     r->grid.decide = std::max(1, (capacity + 255) / 256);   // one entry per thread
@@ -1027,14 +1035,15 @@ static RunArgs run_args(Runner *r, int parity)
     return a;
 }
 
-void runner_apply_shape(Runner *r, int bn, int wg_per_cu)
+void runner_apply_shape(Runner *r, int bn, int wg_per_cu, int waves)
 {
     int g = r->cus * std::max(1, wg_per_cu);
     g = std::max(8, g / 8 * 8);          // a multiple of the 8 XCDs
-    if (r->bn == bn && r->grid.gemm == g) return;
+    if (r->bn == bn && r->grid.gemm == g && r->grid.waves == waves) return;
     r->bn = bn;
     r->wg_per_cu = std::max(1, wg_per_cu);
     r->grid.gemm = g;
+    r->grid.waves = waves;
     if (r->exec) {
         (void)hipStreamSynchronize(r->stream);
         (void)hipGraphExecDestroy(r->exec);
@@ -1056,7 +1065,10 @@ struct ShapeKey {
     }
 };
 static std::mutex g_shape_mu;
-static std::map<ShapeKey, std::pair<int, int>> g_shapes;
+struct Shape {
+    int bn, wpc, waves;
+};
+static std::map<ShapeKey, Shape> g_shapes;
 
 static ShapeKey shape_key(const Runner *r)
 {
@@ -1125,27 +1137,33 @@ int runner_autotune(Runner *r)
         std::lock_guard<std::mutex> lk(g_shape_mu);
         auto it = g_shapes.find(key);
         if (it != g_shapes.end()) {
-            runner_apply_shape(r, it->second.first, it->second.second);
+            runner_apply_shape(r, it->second.bn, it->second.wpc, it->second.waves);
             r->tuned = true;
             return QJX_OK;
         }
     }
     struct Cand {
-        int bn, wpc;
+        int bn, wpc, waves;
         double ms;
         unsigned long long cols;
     };
     std::vector<Cand> cand;
     for (int bn : {128, 64}) {
-        const int occ = gemm_blocks_per_cu(r->prec, r->gemm_kind, bn);
-        cand.push_back({bn, occ, 0.0, 0});
-        if (occ > 1) cand.push_back({bn, occ / 2, 0.0, 0});
+        const int occ = gemm_blocks_per_cu(r->prec, r->gemm_kind, bn, 4);
+        cand.push_back({bn, occ, 4, 0.0, 0});
+        if (occ > 1) cand.push_back({bn, occ / 2, 4, 0.0, 0});
     }
-    const int saved_bn = r->bn, saved_wpc = r->wg_per_cu;
+    if (r->prec == kBF16x3) {
+        for (int bn : {128, 256}) {
+            const int occ = gemm_blocks_per_cu(r->prec, r->gemm_kind, bn, 8);
+            cand.push_back({bn, occ, 8, 0.0, 0});
+        }
+    }
+    const int saved_bn = r->bn, saved_wpc = r->wg_per_cu, saved_waves = r->grid.waves;
     constexpr int kIters = 4, kRounds = 2;
     for (int round = 0; round < kRounds; round++) {
         for (Cand &c : cand) {
-            runner_apply_shape(r, c.bn, c.wpc);
+            runner_apply_shape(r, c.bn, c.wpc, c.waves);
             double ms = 0.0;
             unsigned long long cols = 0;
             int rc = timed_iterations(r, kIters, &ms, &cols);
@@ -1156,7 +1174,7 @@ int runner_autotune(Runner *r)
             rc = runner_next_total(r, &total);
             if (rc) return rc;
             if (total == 0) {      // the table ran out of work: try again next time
-                runner_apply_shape(r, saved_bn, saved_wpc);
+                runner_apply_shape(r, saved_bn, saved_wpc, saved_waves);
                 return QJX_OK;
             }
         }
@@ -1165,13 +1183,13 @@ int runner_autotune(Runner *r)
     for (const Cand &c : cand)
         if (c.cols > 0 && (!best || c.ms / c.cols < best->ms / best->cols)) best = &c;
     if (!best) {
-        runner_apply_shape(r, saved_bn, saved_wpc);
+        runner_apply_shape(r, saved_bn, saved_wpc, saved_waves);
         return QJX_OK;
     }
-    runner_apply_shape(r, best->bn, best->wpc);
+    runner_apply_shape(r, best->bn, best->wpc, best->waves);
     r->tuned = true;
     std::lock_guard<std::mutex> lk(g_shape_mu);
-    g_shapes[key] = {best->bn, best->wpc};
+    g_shapes[key] = {best->bn, best->wpc, best->waves};
     return QJX_OK;
 }
 

@@ -29,6 +29,7 @@ struct StepGrid {
     int gemm;     // workgroups of the GEMM (multiple of 8: XCD-aware tile order)
     int decide;   // workgroups of 256 threads
     int jump;     // workgroups of 256 threads
+    int waves = 4; // waves per GEMM workgroup (8: the 3xBF16 kernel's 2 x 4 wave layout)
 };
 // One propagation iteration over the ensembles of a.ens: plan, grouped GEMM,
 // decide (crossing test, bisection bookkeeping, next lists), jumps.
@@ -38,8 +39,8 @@ void launch_plan(const RunArgs &a, hipStream_t s);
 void launch_gemm(int prec, int gemm_kind, const RunArgs &a, const StepGrid &g, hipStream_t s);
 void launch_tail(int prec, const RunArgs &a, const StepGrid &g, hipStream_t s);
 // resident GEMM workgroups per compute unit (occupancy of the kernel for the
-// column tile width bn)
-int gemm_blocks_per_cu(int prec, int gemm_kind, int bn);
+// column tile width bn and `waves` waves per workgroup)
+int gemm_blocks_per_cu(int prec, int gemm_kind, int bn, int waves);
 // one wave: D32 = A32 (32x2) B32 (2x32), D64 = A64 (16x4) B64 (4x16),
 // Dbf = Abf (32x16) Bbf (16x32) through bf16 operands, all row-major
 void launch_mfma_probe(const float *A32, const float *B32, float *D32, const double *A64,

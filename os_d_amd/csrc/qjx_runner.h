@@ -53,8 +53,9 @@ int runner_run(Runner *r, long long *iterations);
 // on the table's own iterations (they are real steps; results do not depend on
 // the shape).  No-op once tuned, for the FMA kernel, or with autotune off.
 int runner_autotune(Runner *r);
-// Use tile width bn and wg_per_cu workgroups per CU (drops a captured graph).
-void runner_apply_shape(Runner *r, int bn, int wg_per_cu);
+// Use tile width bn, wg_per_cu workgroups per CU and `waves` waves per GEMM
+// workgroup (drops a captured graph).
+void runner_apply_shape(Runner *r, int bn, int wg_per_cu, int waves);
 
 void set_last_error(const std::string &msg);
 

@@ -378,7 +378,11 @@ __global__ __launch_bounds__(256) void gemm_f32(RunArgs a)
 // (lane half h = l / 32 selects which 8); D as in the FP32 form.  The propagator
 // is split when packed, the state tile when staged into LDS.  LDS rows are 64 bytes
 // (32 bf16) with the 16-byte chunk index XOR-swizzled by (row / 4) % 4, which keeps
-// the fragment reads conflict-free; tiles are double-buffered.
+// the fragment reads conflict-free; tiles are double-buffered.  2 x NWN waves per
+// workgroup (NWN = 2 or 4: 4 waves of 64 x BN/2 or 8 of 64 x BN/4; BN = 256 takes 8
+// waves of 64 x 64): with short band K ranges the fixed cost of a tile (claim,
+// lookup, column gather, first loads, epilogue) weighs, and a 256-column tile
+// halves it per column.
 // ---------------------------------------------------------------------------
 constexpr int BF_ROW = MF_BK;          // bf16 per LDS row, no padding
 
@@ -388,11 +392,14 @@ __device__ inline int bf_off(int row, int k)   // element offset of (row, k), k 
     return row * BF_ROW + chunk * 8 + (k & 7);
 }
 
-template <int BN>
-__global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
+template <int BN, int NWN>
+__global__ __launch_bounds__(128 * NWN) void gemm_bf16x3(RunArgs a)
 {
-    constexpr int NJ = BN / 64;          // 32-column MFMA tiles per wave
-    constexpr int NQ = BN / 32;          // B-tile rows (columns of X) copied per thread
+    constexpr int T = 128 * NWN;         // 2 x NWN waves
+    constexpr int NJ = BN / (32 * NWN);  // 32-column MFMA tiles per wave
+    constexpr int QA = MF_BM / (T / 4);  // A-tile rows copied per thread
+    constexpr int NQ = BN / (T / 8);     // B-tile rows (columns of X) copied per thread
+    static_assert(NJ >= 1 && QA >= 1 && NQ >= 1, "tile shape");
     __shared__ int s_tr[BN];
     __shared__ int s_cur[BN];
     __shared__ __attribute__((aligned(16))) __bf16 Ah[2][MF_BM * BF_ROW];
@@ -404,7 +411,7 @@ __global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
     const int nitems = *a.nens * kMaxGroups;
     const int total = s_tpre[nitems];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int wm = wid >> 1, wn = wid & 1;
+    const int wm = wid / NWN, wn = wid % NWN;
     const int h = lane >> 5, li = lane & 31;
     const int ca_k = (tid & 3) * 8, ca_r = tid >> 2;
     const int cb_k = (tid & 7) * 4, cb_r = tid >> 3;
@@ -421,7 +428,7 @@ __global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
         const float *bcol[NQ];
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
-            const int c = cb_r + 32 * q;
+            const int c = cb_r + (T / 8) * q;
             bcol[q] = reinterpret_cast<const float *>(E.slot[s_cur[c]]) + (size_t)s_tr[c] * L + cb_k;
         }
         floatx16 acc[2][NJ];
@@ -431,12 +438,12 @@ __global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
             for (int j = 0; j < NJ; j++)
 #pragma unroll
                 for (int e = 0; e < 16; e++) acc[i][j][e] = 0.f;
-        bf16x8 rah[2], ral[2];
+        bf16x8 rah[QA], ral[QA];
         floatx4 rb[NQ];
         auto gload = [&](int k0) {
 #pragma unroll
-            for (int q = 0; q < 2; q++) {
-                const size_t o = (size_t)(r0 + ca_r + 64 * q) * L + k0 + ca_k;
+            for (int q = 0; q < QA; q++) {
+                const size_t o = (size_t)(r0 + ca_r + (T / 4) * q) * L + k0 + ca_k;
                 rah[q] = gld<bf16x8>(Uh + o);
                 ral[q] = gld<bf16x8>(Ul + o);
             }
@@ -445,14 +452,14 @@ __global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
         };
         auto lstore = [&](int buf) {
 #pragma unroll
-            for (int q = 0; q < 2; q++) {
-                const int rr = ca_r + 64 * q;
+            for (int q = 0; q < QA; q++) {
+                const int rr = ca_r + (T / 4) * q;
                 *reinterpret_cast<bf16x8 *>(&Ah[buf][bf_off(rr, ca_k)]) = rah[q];
                 *reinterpret_cast<bf16x8 *>(&Al[buf][bf_off(rr, ca_k)]) = ral[q];
             }
 #pragma unroll
             for (int q = 0; q < NQ; q++) {
-                const int rr = cb_r + 32 * q;
+                const int rr = cb_r + (T / 8) * q;
                 bf16x4 h4, l4;
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
@@ -487,7 +494,7 @@ __global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
                 }
 #pragma unroll
                 for (int j = 0; j < NJ; j++) {
-                    const int ob = bf_off(wn * (BN / 2) + j * 32 + li, 16 * c + 8 * h);
+                    const int ob = bf_off(wn * (BN / NWN) + j * 32 + li, 16 * c + 8 * h);
                     fbh[j] = *reinterpret_cast<const bf16x8 *>(&Bh[cur][ob]);
                     fbl[j] = *reinterpret_cast<const bf16x8 *>(&Bl[cur][ob]);
                 }
@@ -517,7 +524,7 @@ __global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
 #pragma unroll
         for (int j = 0; j < NJ; j++) {
             ns[j] = 0.0;
-            const int c = wn * (BN / 2) + j * 32 + li;
+            const int c = wn * (BN / NWN) + j * 32 + li;
             const bool ok = c < r.ncol;
             float *y = reinterpret_cast<float *>(E.slot[s_cur[c] ^ 1]) + (size_t)s_tr[c] * L;
 #pragma unroll
@@ -535,7 +542,7 @@ __global__ __launch_bounds__(256) void gemm_bf16x3(RunArgs a)
         for (int j = 0; j < NJ; j++) ns[j] += __shfl_xor(ns[j], 32, 64);
         if (h == 0) {
 #pragma unroll
-            for (int j = 0; j < NJ; j++) s_norm[wm][wn * (BN / 2) + j * 32 + li] = ns[j];
+            for (int j = 0; j < NJ; j++) s_norm[wm][wn * (BN / NWN) + j * 32 + li] = ns[j];
         }
         __syncthreads();
         if (tid < r.ncol) gst<double>(E.normpart + (size_t)r.rt * E.M + s_tr[tid], s_norm[0][tid] + s_norm[1][tid]);
@@ -1045,8 +1052,10 @@ void launch_gemm(int prec, int gemm_kind, const RunArgs &a, const StepGrid &g, h
         if (a.bn == 64) gemm_f64<64><<<g.gemm, 512, 0, s>>>(a);
         else gemm_f64<128><<<g.gemm, 512, 0, s>>>(a);
     } else if (prec == kBF16x3) {
-        if (a.bn == 64) gemm_bf16x3<64><<<g.gemm, 256, 0, s>>>(a);
-        else gemm_bf16x3<128><<<g.gemm, 256, 0, s>>>(a);
+        if (a.bn == 256) gemm_bf16x3<256, 4><<<g.gemm, 512, 0, s>>>(a);
+        else if (g.waves == 8) gemm_bf16x3<128, 4><<<g.gemm, 512, 0, s>>>(a);
+        else if (a.bn == 64) gemm_bf16x3<64, 2><<<g.gemm, 256, 0, s>>>(a);
+        else gemm_bf16x3<128, 2><<<g.gemm, 256, 0, s>>>(a);
     } else {
         if (a.bn == 64) gemm_f32<64><<<g.gemm, 256, 0, s>>>(a);
         else gemm_f32<128><<<g.gemm, 256, 0, s>>>(a);
@@ -1067,7 +1076,7 @@ void launch_iteration(int prec, int gemm_kind, const RunArgs &a, const StepGrid 
     launch_tail(prec, a, g, s);
 }
 
-int gemm_blocks_per_cu(int prec, int gemm_kind, int bn)
+int gemm_blocks_per_cu(int prec, int gemm_kind, int bn, int waves)
 {
     int n = 0;
     hipError_t e;
@@ -1079,8 +1088,10 @@ int gemm_blocks_per_cu(int prec, int gemm_kind, int bn)
         e = narrow ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_f64<64>, 512, 0)
                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_f64<128>, 512, 0);
     else if (prec == kBF16x3)
-        e = narrow ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_bf16x3<64>, 256, 0)
-                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_bf16x3<128>, 256, 0);
+        e = bn == 256 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_bf16x3<256, 4>, 512, 0)
+            : waves == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_bf16x3<128, 4>, 512, 0)
+            : narrow   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_bf16x3<64, 2>, 256, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_bf16x3<128, 2>, 256, 0);
     else
         e = narrow ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_f32<64>, 256, 0)
                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_f32<128>, 256, 0);

@@ -129,3 +129,19 @@ def test_band_skipping_does_not_change_results(prec, monkeypatch):
     if prec != 0:
         # FP32 storage underflows far from the diagonal: the band is narrow
         assert work["0"] < 0.8 * work["1"]
+
+
+def test_bf16x3_eight_wave_kernel_matches(monkeypatch):
+    """The 3xBF16 GEMM's 8-wave workgroup layout gives the same results as the
+    4-wave one (each output's MFMA sequence and norm sums are unchanged)."""
+    _say("case: bf16x3 4 vs 8 waves")
+    res = {}
+    for waves in ("4", "8"):
+        monkeypatch.setenv("QJX_BN", "128")
+        if waves == "8":
+            monkeypatch.setenv("QJX_WAVES", "8")
+        e = _engine(2, N=150, M=300, seed=13, drv_ampl=1.5, diss_gamma=0.1, S=40)
+        e.run_steps(5, 0)
+        res[waves] = e.get_state()
+        e.close()
+    _same_state(res["4"], res["8"])
