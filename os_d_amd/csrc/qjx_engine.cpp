@@ -955,7 +955,7 @@ int runner_create(int prec, int gemm_kind, hipStream_t stream, int capacity, Run
         if (std::string(env) == "0") r->autotune = false;
     if (const char *env = std::getenv("QJX_BN")) {
         const std::string v(env);
-        if (gemm_kind != kGemmFma && (v == "64" || v == "128" || (v == "256" && prec == kBF16x3))) {
+        if (gemm_kind != kGemmFma && (v == "64" || v == "128" || (v == "256" && prec != kFP64))) {
             bn = std::stoi(v);
             r->autotune = false;
         }
@@ -1154,10 +1154,12 @@ int runner_autotune(Runner *r)
         if (occ > 1) cand.push_back({bn, occ / 2, 4, 0.0, 0});
     }
     if (r->prec == kBF16x3) {
-        for (int bn : {128, 256}) {
-            const int occ = gemm_blocks_per_cu(r->prec, r->gemm_kind, bn, 8);
-            cand.push_back({bn, occ, 8, 0.0, 0});
-        }
+        const int occ = gemm_blocks_per_cu(r->prec, r->gemm_kind, 128, 8);
+        cand.push_back({128, occ, 8, 0.0, 0});
+    }
+    if (r->prec != kFP64) {                // 256-column tiles (8 waves) for the FP32 kernels
+        const int occ = gemm_blocks_per_cu(r->prec, r->gemm_kind, 256, 8);
+        cand.push_back({256, occ, 8, 0.0, 0});
     }
     const int saved_bn = r->bn, saved_wpc = r->wg_per_cu, saved_waves = r->grid.waves;
     constexpr int kIters = 4, kRounds = 2;

@@ -218,8 +218,8 @@ __device__ inline void load_cols(const RunArgs &a, const TileRef &r, int *s_tr, 
 // GEMM tile, FP32: v_mfma_f32_32x32x2f32
 //   A (32 x 2): lane l holds A[l % 32][l / 32]; B (2 x 32): B[l / 32][l % 32]
 //   D (32 x 32): lane l, register r holds D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32]
-// 4 waves as 2 x 2, wave tile 64 x BN/2, WG tile 128 x BN (BN = 128 or 64, picked
-// by the launch-shape tuner), K block 32; within a K block, MFMA step s (0..15)
+// 2 x NWN waves, wave tile 64 x BN/NWN, WG tile 128 x BN (BN = 64 or 128 with 4
+// waves, 256 with 8; picked by the launch-shape tuner), K block 32; within a K block, MFMA step s (0..15)
 // takes k = s + 16 h for lane half h, so each lane reads its 16 fragment values per
 // block with four 16-byte LDS reads.  The k order is the same for every column and
 // the row split over waves does not depend on BN, so results do not depend on
@@ -248,11 +248,14 @@ __device__ inline KRange tile_krange(const Ens &E, const TileRef &r)
     return k;
 }
 
-template <int BN>
-__global__ __launch_bounds__(256) void gemm_f32(RunArgs a)
+template <int BN, int NWN>
+__global__ __launch_bounds__(128 * NWN) void gemm_f32(RunArgs a)
 {
-    constexpr int NJ = BN / 64;          // 32-column MFMA tiles per wave
-    constexpr int NQ = BN / 32;          // B-tile rows (columns of X) copied per thread
+    constexpr int T = 128 * NWN;         // 2 x NWN waves
+    constexpr int NJ = BN / (32 * NWN);  // 32-column MFMA tiles per wave
+    constexpr int QA = MF_BM / (T / 8);  // A-tile rows copied per thread
+    constexpr int NQ = BN / (T / 8);     // B-tile rows (columns of X) copied per thread
+    static_assert(NJ >= 1 && QA >= 1 && NQ >= 1, "tile shape");
     __shared__ int s_tr[BN];
     __shared__ int s_cur[BN];
     __shared__ __attribute__((aligned(16))) float As[2][MF_BM * F32_PITCH];   // double-buffered
@@ -262,7 +265,7 @@ __global__ __launch_bounds__(256) void gemm_f32(RunArgs a)
     const int nitems = *a.nens * kMaxGroups;
     const int total = s_tpre[nitems];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int wm = wid >> 1, wn = wid & 1;
+    const int wm = wid / NWN, wn = wid % NWN;
     const int h = lane >> 5, li = lane & 31;
     const int cp_k = (tid & 7) * 4, cp_r = tid >> 3;
     __shared__ int s_claim[3];
@@ -274,11 +277,11 @@ __global__ __launch_bounds__(256) void gemm_f32(RunArgs a)
         const int L = E.L, r0 = r.rt * MF_BM;
         const float *U = reinterpret_cast<const float *>(E.props) + (size_t)r.g * L * L;
         const float *arow = U + (size_t)(r0 + cp_r) * L + cp_k;
-        const size_t astep = (size_t)32 * L;
+        const size_t astep = (size_t)(T / 8) * L;
         const float *bcol[NQ];
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
-            const int c = cp_r + 32 * q;
+            const int c = cp_r + (T / 8) * q;
             bcol[q] = reinterpret_cast<const float *>(E.slot[s_cur[c]]) + (size_t)s_tr[c] * L + cp_k;
         }
         floatx16 acc[2][NJ];
@@ -288,20 +291,20 @@ __global__ __launch_bounds__(256) void gemm_f32(RunArgs a)
             for (int j = 0; j < NJ; j++)
 #pragma unroll
                 for (int e = 0; e < 16; e++) acc[i][j][e] = 0.f;
-        floatx4 ra[4], rb[NQ];
+        floatx4 ra[QA], rb[NQ];
         auto gload = [&](int k0) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) ra[q] = gld<floatx4>(arow + q * astep + k0);
+            for (int q = 0; q < QA; q++) ra[q] = gld<floatx4>(arow + q * astep + k0);
 #pragma unroll
             for (int q = 0; q < NQ; q++) rb[q] = gld<floatx4>(bcol[q] + k0);
         };
         auto lstore = [&](int buf) {
 #pragma unroll
-            for (int q = 0; q < 4; q++)
-                *reinterpret_cast<floatx4 *>(&As[buf][(cp_r + 32 * q) * F32_PITCH + cp_k]) = ra[q];
+            for (int q = 0; q < QA; q++)
+                *reinterpret_cast<floatx4 *>(&As[buf][(cp_r + (T / 8) * q) * F32_PITCH + cp_k]) = ra[q];
 #pragma unroll
             for (int q = 0; q < NQ; q++)
-                *reinterpret_cast<floatx4 *>(&Bs[buf][(cp_r + 32 * q) * F32_PITCH + cp_k]) = rb[q];
+                *reinterpret_cast<floatx4 *>(&Bs[buf][(cp_r + (T / 8) * q) * F32_PITCH + cp_k]) = rb[q];
         };
         const KRange kr = tile_krange(E, r);
         const int nk = kr.count();
@@ -311,9 +314,9 @@ __global__ __launch_bounds__(256) void gemm_f32(RunArgs a)
         }
         __syncthreads();
         int cur = 0;
-        for (int i = 0; i < nk; i++) {
-            const bool more = i + 1 < nk;
-            if (more) gload(kr.at(i + 1));
+        for (int kb = 0; kb < nk; kb++) {
+            const bool more = kb + 1 < nk;
+            if (more) gload(kr.at(kb + 1));
             floatx4 fa[2][4], fb[NJ][4];
 #pragma unroll
             for (int i = 0; i < 2; i++) {
@@ -323,7 +326,7 @@ __global__ __launch_bounds__(256) void gemm_f32(RunArgs a)
             }
 #pragma unroll
             for (int j = 0; j < NJ; j++) {
-                const float *pb = &Bs[cur][(wn * (BN / 2) + j * 32 + li) * F32_PITCH + 16 * h];
+                const float *pb = &Bs[cur][(wn * (BN / NWN) + j * 32 + li) * F32_PITCH + 16 * h];
 #pragma unroll
                 for (int v = 0; v < 4; v++) fb[j][v] = *reinterpret_cast<const floatx4 *>(pb + 4 * v);
             }
@@ -344,7 +347,7 @@ __global__ __launch_bounds__(256) void gemm_f32(RunArgs a)
 #pragma unroll
         for (int j = 0; j < NJ; j++) {
             ns[j] = 0.0;
-            const int c = wn * (BN / 2) + j * 32 + li;
+            const int c = wn * (BN / NWN) + j * 32 + li;
             const bool ok = c < r.ncol;
             float *y = reinterpret_cast<float *>(E.slot[s_cur[c] ^ 1]) + (size_t)s_tr[c] * L;
 #pragma unroll
@@ -362,7 +365,7 @@ __global__ __launch_bounds__(256) void gemm_f32(RunArgs a)
         for (int j = 0; j < NJ; j++) ns[j] += __shfl_xor(ns[j], 32, 64);
         if (h == 0) {
 #pragma unroll
-            for (int j = 0; j < NJ; j++) s_norm[wm][wn * (BN / 2) + j * 32 + li] = ns[j];
+            for (int j = 0; j < NJ; j++) s_norm[wm][wn * (BN / NWN) + j * 32 + li] = ns[j];
         }
         __syncthreads();
         if (tid < r.ncol) gst<double>(E.normpart + (size_t)r.rt * E.M + s_tr[tid], s_norm[0][tid] + s_norm[1][tid]);
@@ -480,9 +483,9 @@ __global__ __launch_bounds__(128 * NWN) void gemm_bf16x3(RunArgs a)
         }
         __syncthreads();
         int cur = 0;
-        for (int i = 0; i < nk; i++) {
-            const bool more = i + 1 < nk;
-            if (more) gload(kr.at(i + 1));
+        for (int kb = 0; kb < nk; kb++) {
+            const bool more = kb + 1 < nk;
+            if (more) gload(kr.at(kb + 1));
 #pragma unroll
             for (int c = 0; c < MF_BK / 16; c++) {
                 bf16x8 fah[2], fal[2], fbh[NJ], fbl[NJ];
@@ -623,9 +626,9 @@ __global__ __launch_bounds__(512) void gemm_f64(RunArgs a)
             lstore();
         }
         __syncthreads();
-        for (int i = 0; i < nk; i++) {
-            const bool more = i + 1 < nk;
-            if (more) gload(kr.at(i + 1));
+        for (int kb = 0; kb < nk; kb++) {
+            const bool more = kb + 1 < nk;
+            if (more) gload(kr.at(kb + 1));
 #pragma unroll
             for (int half = 0; half < 2; half++) {
                 double2v fa[2][2], fb[NJ][2];
@@ -1057,8 +1060,9 @@ void launch_gemm(int prec, int gemm_kind, const RunArgs &a, const StepGrid &g, h
         else if (a.bn == 64) gemm_bf16x3<64, 2><<<g.gemm, 256, 0, s>>>(a);
         else gemm_bf16x3<128, 2><<<g.gemm, 256, 0, s>>>(a);
     } else {
-        if (a.bn == 64) gemm_f32<64><<<g.gemm, 256, 0, s>>>(a);
-        else gemm_f32<128><<<g.gemm, 256, 0, s>>>(a);
+        if (a.bn == 256) gemm_f32<256, 4><<<g.gemm, 512, 0, s>>>(a);
+        else if (a.bn == 64) gemm_f32<64, 2><<<g.gemm, 256, 0, s>>>(a);
+        else gemm_f32<128, 2><<<g.gemm, 256, 0, s>>>(a);
     }
 }
 
@@ -1093,8 +1097,9 @@ int gemm_blocks_per_cu(int prec, int gemm_kind, int bn, int waves)
             : narrow   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_bf16x3<64, 2>, 256, 0)
                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_bf16x3<128, 2>, 256, 0);
     else
-        e = narrow ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_f32<64>, 256, 0)
-                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_f32<128>, 256, 0);
+        e = bn == 256 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_f32<256, 4>, 512, 0)
+            : narrow  ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_f32<64, 2>, 256, 0)
+                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_f32<128, 2>, 256, 0);
     return (e == hipSuccess && n > 0) ? n : 1;
 }
 

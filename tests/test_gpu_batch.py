@@ -69,7 +69,8 @@ def test_batch_member_equals_alone(prec):
 def test_tile_width_does_not_change_results(prec, monkeypatch):
     _say("case: tile width 64 vs 128 prec=%d" % prec)
     res = {}
-    for bn in ("64", "128"):
+    widths = ("64", "128", "256") if prec != 0 else ("64", "128")
+    for bn in widths:
         monkeypatch.setenv("QJX_BN", bn)
         e = _engine(prec, N=150, M=300, seed=11, drv_ampl=1.5, diss_gamma=0.1)
         e.run_steps(5, 0)
@@ -77,7 +78,8 @@ def test_tile_width_does_not_change_results(prec, monkeypatch):
         assert s["gemm_bn"] == int(bn)
         res[bn] = e.get_state()
         e.close()
-    _same_state(res["64"], res["128"])
+    for bn in widths[1:]:
+        _same_state(res["64"], res[bn])
 
 
 def test_autotuner_picks_a_shape_with_identical_results(monkeypatch):
