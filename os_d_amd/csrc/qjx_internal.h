@@ -63,7 +63,7 @@ struct Ens {
     TrajState *st;
     int *list[2];             // per parity: [G * M] entries (trajectory << 1) | current slot
     int *cnt[2];              // per parity: [kMaxGroups] list lengths
-    double *normpart;         // [R * M]
+    double *normpart;         // [M][R]: FP64 partial sums of y^2 per row tile
     const double *dt;         // [D]
     const double *Ldiag;      // [d]   L_kk
     const double *Loff;       // [d-1] L_{k+1,k} = -Loff[k], L_{k,k+1} = +Loff[k]

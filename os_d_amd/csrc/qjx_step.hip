@@ -368,7 +368,7 @@ __global__ __launch_bounds__(128 * NWN) void gemm_f32(RunArgs a)
             for (int j = 0; j < NJ; j++) s_norm[wm][wn * (BN / NWN) + j * 32 + li] = ns[j];
         }
         __syncthreads();
-        if (tid < r.ncol) gst<double>(E.normpart + (size_t)r.rt * E.M + s_tr[tid], s_norm[0][tid] + s_norm[1][tid]);
+        if (tid < r.ncol) gst<double>(E.normpart + (size_t)s_tr[tid] * E.R + r.rt, s_norm[0][tid] + s_norm[1][tid]);
         if (tid == 0) atomicAdd(&E.counters[2], (unsigned long long)r.ncol * nk);
         __syncthreads();
     }
@@ -548,7 +548,7 @@ __global__ __launch_bounds__(128 * NWN) void gemm_bf16x3(RunArgs a)
             for (int j = 0; j < NJ; j++) s_norm[wm][wn * (BN / NWN) + j * 32 + li] = ns[j];
         }
         __syncthreads();
-        if (tid < r.ncol) gst<double>(E.normpart + (size_t)r.rt * E.M + s_tr[tid], s_norm[0][tid] + s_norm[1][tid]);
+        if (tid < r.ncol) gst<double>(E.normpart + (size_t)s_tr[tid] * E.R + r.rt, s_norm[0][tid] + s_norm[1][tid]);
         if (tid == 0) atomicAdd(&E.counters[2], (unsigned long long)r.ncol * nk);
         __syncthreads();
     }
@@ -688,7 +688,7 @@ __global__ __launch_bounds__(512) void gemm_f64(RunArgs a)
         }
         __syncthreads();
         if (tid < r.ncol)
-            gst<double>(E.normpart + (size_t)r.rt * E.M + s_tr[tid],
+            gst<double>(E.normpart + (size_t)s_tr[tid] * E.R + r.rt,
                         ((s_norm[0][tid] + s_norm[1][tid]) + s_norm[2][tid]) + s_norm[3][tid]);
         if (tid == 0) atomicAdd(&E.counters[2], (unsigned long long)r.ncol * nk);
         __syncthreads();
@@ -769,7 +769,7 @@ __global__ __launch_bounds__(256) void gemm_fma(RunArgs a)
         if (tid < r.ncol) {
             double s = 0.0;
             for (int q = 0; q < FM_BM / 4; q++) s += s_norm[q][tid];
-            E.normpart[(size_t)r.rt * E.M + s_tr[tid]] = s;
+            E.normpart[(size_t)s_tr[tid] * E.R + r.rt] = s;
         }
         __syncthreads();
     }
@@ -840,7 +840,8 @@ __global__ __launch_bounds__(256) void decide_kernel(RunArgs a)
         const Ens &E = a.ens[e];
         tr = E.list[p][(size_t)g * E.M + (idx - s_pre[item])] >> 1;
         TrajState s = E.st[tr];
-        for (int rt = 0; rt < E.R; rt++) n += E.normpart[(size_t)rt * E.M + tr];
+        const double *np = E.normpart + (size_t)tr * E.R;     // R contiguous partials
+        for (int rt = 0; rt < E.R; rt++) n += np[rt];
         const int lvl = s.lvl;
         const double dt = E.dt[lvl];
         bool alive = true;
