@@ -70,6 +70,7 @@ def run_form(qjx, prec, N, M, S, D, warmup, steps):
         "gemm_band_flops": s1["gemm_band_flops"] - s0["gemm_band_flops"],
         "gemm_bn": s1["gemm_bn"],
         "gemm_grid": s1["gemm_grid"],
+        "gemm_kernel": gemm_kernel_name(prec, s1["gemm_bn"], s1["gemm_waves"]),
         # all launches of the engine (warmup + timed)
         "all_gemm_ms": s1["gemm_ms"],
         "all_gemm_launches": s1["gemm_launches"],
@@ -78,6 +79,14 @@ def run_form(qjx, prec, N, M, S, D, warmup, steps):
         "observables": obs,
     }
     return e, res
+
+
+def gemm_kernel_name(prec, bn, waves):
+    """Name of the step GEMM kernel the launch shape selects (as a kernel trace shows it)."""
+    if prec == 0:
+        return "qjx::gemm_f64<%d>" % bn
+    name = "qjx::gemm_f32" if prec == 1 else "qjx::gemm_bf16x3"
+    return "%s<%d, %d>" % (name, bn, 4 if (bn == 256 or waves == 8) else 2)
 
 
 def observables(e, N, M):
@@ -391,7 +400,7 @@ def main():
                      "traffic_source": "profiles/step_gemm_fp32_traffic_dimer_d1001_M1e5.md (bytes/column x "
                                        "columns/launch)",
                      "algorithmic_bytes_per_launch": 2 * 4 * 2 * 1024 * cols_per_launch if N == 1000 else None,
-                     "kernel": "qjx::gemm_f32<%d> (grouped step GEMM, gathered columns)" % r32["gemm_bn"],
+                     "kernel": "%s (grouped step GEMM, gathered columns)" % r32["gemm_kernel"],
                      "launches": launches,
                      "avg_launch_ms": avg_ms,
                      "flops_per_launch": flops_per_launch,

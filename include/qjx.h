@@ -89,6 +89,8 @@ typedef struct qjx_stats {
     double gemm_band_flops; /* flops of the K blocks the matrix-core GEMM multiplied:
                                2 x 128 x 32 per (column, row tile, K block) inside the
                                propagators' band tables (exact zeros skipped)      */
+    int32_t gemm_waves;     /* waves per GEMM workgroup (4 or 8)                     */
+    int32_t pad;
 } qjx_stats;
 
 const char *qjx_last_error(void);

@@ -761,6 +761,7 @@ extern "C" int qjx_get_stats(qjx_ctx *c, qjx_stats *out)
     out->gemm_launches = c->runner ? c->runner->gemm_launches : 0;
     out->gemm_bn = c->runner ? c->runner->bn : 0;
     out->gemm_grid = c->runner ? c->runner->grid.gemm : 0;
+    out->gemm_waves = c->runner ? c->runner->grid.waves : 0;
     return QJX_OK;
 }
 
@@ -915,6 +916,7 @@ extern "C" int qjx_batch_get_stats(qjx_batch *b, qjx_stats *out)
     out->gemm_launches = r->gemm_launches;
     out->gemm_bn = r->bn;
     out->gemm_grid = r->grid.gemm;
+    out->gemm_waves = r->grid.waves;
     return QJX_OK;
 }
 

@@ -49,7 +49,8 @@ class Stats(ctypes.Structure):
                 ("jumps", ctypes.c_int64), ("gemm_flops", ctypes.c_double),
                 ("gemm_ms", ctypes.c_double), ("decide_ms", ctypes.c_double),
                 ("gemm_launches", ctypes.c_int64), ("gemm_bn", ctypes.c_int32),
-                ("gemm_grid", ctypes.c_int32), ("gemm_band_flops", ctypes.c_double)]
+                ("gemm_grid", ctypes.c_int32), ("gemm_band_flops", ctypes.c_double),
+                ("gemm_waves", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
 
 def _stats_dict(s):
