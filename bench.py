@@ -203,9 +203,11 @@ def batched(qjx, S, D, warmup, steps, prec, alone):
     res = {"instances": len(inst), "manifest_seed": C4_SEED, "trajectories": sum(p["M"] for p in inst),
            "dims": [p["N"] + 1 for p in inst], "steps": steps, "warmup": warmup, "seconds": dt,
            "tp_per_s": tp / dt, "matvecs": s1["matvecs"] - s0["matvecs"],
-           "effective_tflops": (s1["gemm_flops"] - s0["gemm_flops"]) / dt / 1e12,
+           "effective_band_tflops": (s1["gemm_band_flops"] - s0["gemm_band_flops"]) / dt / 1e12,
+           "effective_dense_equivalent_tflops": (s1["gemm_flops"] - s0["gemm_flops"]) / dt / 1e12,
            "iterations": s1["iterations"] - s0["iterations"], "gemm_bn": s1["gemm_bn"],
-           "gemm_grid": s1["gemm_grid"], "build_s": build_s, "manifest": inst}
+           "gemm_grid": s1["gemm_grid"], "gemm_kernel": gemm_kernel_name(prec, s1["gemm_bn"], s1["gemm_waves"]),
+           "build_s": build_s, "manifest": inst}
     b.close()
     if alone:
         t_alone = 0.0
