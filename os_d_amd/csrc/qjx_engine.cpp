@@ -1015,6 +1015,22 @@ int runner_set_table(Runner *r, const Ens *ens, int n)
     if (n) HIP_TRY(hipMemcpyAsync(r->d_ens, ens, n * sizeof(Ens), hipMemcpyHostToDevice, r->stream));
     r->nens = n;
     r->h_ens.assign(ens, ens + n);
+    // the jump kernel's LDS follows the table's largest dimension; a larger one
+    // needs a new capture of the iteration graph
+    int jdim = 64;
+    for (int i = 0; i < n; i++) jdim = std::max(jdim, ens[i].d);
+    if (jdim > r->grid.jdim || (r->grid.jdim == kMaxDim && jdim < kMaxDim)) {
+        r->grid.jdim = jdim;
+        if (r->exec) {
+            HIP_TRY(hipStreamSynchronize(r->stream));
+            (void)hipGraphExecDestroy(r->exec);
+            r->exec = nullptr;
+        }
+        if (r->graph) {
+            (void)hipGraphDestroy(r->graph);
+            r->graph = nullptr;
+        }
+    }
     // the count goes through pinned memory so the copy is stream-ordered
     HIP_TRY(hipStreamSynchronize(r->stream));
     *r->h_total = n;

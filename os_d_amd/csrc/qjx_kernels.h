@@ -30,6 +30,7 @@ struct StepGrid {
     int decide;   // workgroups of 256 threads
     int jump;     // workgroups of 256 threads
     int waves = 4; // waves per GEMM workgroup (8: the 3xBF16 kernel's 2 x 4 wave layout)
+    int jdim = kMaxDim; // largest Hilbert dimension of the table (jump kernel LDS)
 };
 // One propagation iteration over the ensembles of a.ens: plan, grouped GEMM,
 // decide (crossing test, bisection bookkeeping, next lists), jumps.

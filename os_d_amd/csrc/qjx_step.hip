@@ -945,11 +945,13 @@ __global__ __launch_bounds__(256) void decide_kernel(RunArgs a)
 constexpr int kJumpThreads = 256;
 constexpr int kJumpPer = kMaxDim / kJumpThreads;
 
+// LDS holds the normalised state: 2 x (jdim + 2) doubles, jdim = the largest d
+// of the table (dynamic, so small models keep more workgroups resident).
 template <typename T>
-__global__ __launch_bounds__(kJumpThreads) void jump_kernel(RunArgs a)
+__global__ __launch_bounds__(kJumpThreads) void jump_kernel(RunArgs a, int jdim)
 {
-    __shared__ double pre[kMaxDim + 2];
-    __shared__ double pim[kMaxDim + 2];
+    extern __shared__ double jump_lds[];
+    double *pre = jump_lds, *pim = jump_lds + (jdim + 2);
     __shared__ double red[kJumpThreads / 64];
     const int njump = *a.jcnt;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -1070,8 +1072,9 @@ void launch_gemm(int prec, int gemm_kind, const RunArgs &a, const StepGrid &g, h
 void launch_tail(int prec, const RunArgs &a, const StepGrid &g, hipStream_t s)
 {
     decide_kernel<<<g.decide, 256, 0, s>>>(a);
-    if (prec == kFP64) jump_kernel<double><<<g.jump, kJumpThreads, 0, s>>>(a);
-    else jump_kernel<float><<<g.jump, kJumpThreads, 0, s>>>(a);
+    const size_t lds = 2 * (size_t)(g.jdim + 2) * sizeof(double);
+    if (prec == kFP64) jump_kernel<double><<<g.jump, kJumpThreads, lds, s>>>(a, g.jdim);
+    else jump_kernel<float><<<g.jump, kJumpThreads, lds, s>>>(a, g.jdim);
 }
 
 void launch_iteration(int prec, int gemm_kind, const RunArgs &a, const StepGrid &g, hipStream_t s)
