@@ -133,17 +133,22 @@ def test_band_skipping_does_not_change_results(prec, monkeypatch):
         assert work["0"] < 0.8 * work["1"]
 
 
-def test_bf16x3_eight_wave_kernel_matches(monkeypatch):
-    """The 3xBF16 GEMM's 8-wave workgroup layout gives the same results as the
-    4-wave one (each output's MFMA sequence and norm sums are unchanged)."""
-    _say("case: bf16x3 4 vs 8 waves")
+@pytest.mark.parametrize("N", [150, 700])
+def test_bf16x3_kernel_variants_match(N, monkeypatch):
+    """The 3xBF16 GEMM variants (4 or 8 waves, 128 or 256 columns) give identical
+    results: each output's MFMA sequence and norm sums are the same."""
+    _say("case: bf16x3 variants N=%d" % N)
     res = {}
-    for waves in ("4", "8"):
-        monkeypatch.setenv("QJX_BN", "128")
-        if waves == "8":
-            monkeypatch.setenv("QJX_WAVES", "8")
-        e = _engine(2, N=150, M=300, seed=13, drv_ampl=1.5, diss_gamma=0.1, S=40)
+    shapes = {"128x4": ("128", None), "128x8": ("128", "8"), "256x8": ("256", None)}
+    for name, (bn, waves) in shapes.items():
+        monkeypatch.setenv("QJX_BN", bn)
+        if waves is None:
+            monkeypatch.delenv("QJX_WAVES", raising=False)
+        else:
+            monkeypatch.setenv("QJX_WAVES", waves)
+        e = _engine(2, N=N, M=300, seed=13, drv_ampl=1.5, diss_gamma=0.1, S=40)
         e.run_steps(5, 0)
-        res[waves] = e.get_state()
+        res[name] = e.get_state()
         e.close()
-    _same_state(res["4"], res["8"])
+    for name in list(shapes)[1:]:
+        _same_state(res["128x4"], res[name])
