@@ -26,6 +26,8 @@ cpu_baseline: the FP64 CPU oracle (OpenMP over trajectories, one zgemv per
 batched_uneven: 32 dimer instances of different dimension (seeded manifest) in
              one batch: one grouped GEMM per iteration over all of them, HIP-graph
              replay, launch shape from the tuner; also timed instance by instance.
+lindblad   : the GPU density-matrix (Lindblad, rho-space RK4) integrator at N = 100
+             with 1e4 RK4 steps per period, against the CPU restatement.
 serving    : a Poisson stream of ensemble jobs through the pool (continuous
              batching, admission by HBM footprint), p50 / p99 job latency.
 """
@@ -241,6 +243,40 @@ def batched(qjx, S, D, warmup, steps, prec, alone):
     return res
 
 
+def lindblad(qjx, N=100, steps=10000, periods=2, cpu_steps=200):
+    """The Lindblad density-matrix integrator (rho-space RK4, FP64) on the
+    trajectory workload's model at N = 100 with 1e4 RK4 steps per period (the
+    configuration the reference's f-basis solver is run on for this model), on the
+    GPU and, for a bounded number of steps, on the host with the CPU restatement
+    (single thread).  Both produce the same bits."""
+    import numpy as np
+    from oracle import oracle as orc
+    params = dict(prm_E=1.0, prm_U=0.5, prm_J=1.0, drv_ampl=1.5, drv_freq=1.0)
+    lb = qjx.Lindblad(N, steps_per_period=steps, diss_gamma=0.1, **params)
+    lb.run_periods(1)                       # warm-up: graph capture, caches
+    t0 = time.perf_counter()
+    lb.run_periods(periods)
+    dt = time.perf_counter() - t0
+    ob = lb.observables()
+    lb.close()
+    # the CPU sample: cpu_steps RK4 steps of the same size and cost, as one period of
+    # a drive whose frequency is scaled by steps / cpu_steps
+    cpu_params = dict(params, drv_freq=params["drv_freq"] * steps / cpu_steps)
+    m = orc.model(N, gamma=0.1, **cpu_params)
+    rho = np.zeros((N + 1, N + 1), complex)
+    rho[0, 0] = 1.0
+    t0 = time.perf_counter()
+    orc.lindblad_evolve(m, rho, 1, steps_per_period=cpu_steps)
+    dtc = time.perf_counter() - t0
+    cpu_steps_per_s = cpu_steps / dtc
+    gpu_steps_per_s = periods * steps / dt
+    return {"N": N, "dim": N + 1, "rk4_steps_per_period": steps, "gpu_periods_per_s": periods / dt,
+            "gpu_rk4_steps_per_s": gpu_steps_per_s, "cpu_rk4_steps_per_s": cpu_steps_per_s,
+            "cpu_sample": "%d RK4 steps at d=%d, one host thread, the CPU restatement" % (cpu_steps, N + 1),
+            "gpu_over_cpu": gpu_steps_per_s / cpu_steps_per_s,
+            "observables_after": {"mean": ob["mean"], "energy": ob["energy"], "purity": ob["purity"]}}
+
+
 SERVE_MIXES = {
     # the serving configuration as specified
     "specified": {"N": [100, 250, 500, 1000, 2000], "logM": (3.0, 5.0), "periods": (50, 500)},
@@ -319,6 +355,7 @@ def main():
     ap.add_argument("--batch-alone", type=int, default=1,
                     help="also time the batched workload instance by instance")
     ap.add_argument("--only-batch", action="store_true")
+    ap.add_argument("--no-lindblad", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=30.0, help="serving leg window (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
@@ -375,6 +412,10 @@ def main():
             e64.close()
             e64 = None
         batch = batched(qjx, S, D, 1, args.batch_steps, qjx.FP32, bool(args.batch_alone))
+
+    lind = None
+    if not args.no_lindblad:
+        lind = lindblad(qjx)
 
     serve = None
     if args.serve_seconds > 0:
@@ -439,6 +480,7 @@ def main():
         "cpu_baseline": cpu,
         "observables_vs_fp64": {"fp32": observable_errors(r32, fp64), "bf16x3": observable_errors(bf, fp64)},
         "batched_uneven": batch,
+        "lindblad": lind,
         "p99_job_latency_s": serve["scaled"]["p99_latency_s"] if serve else None,
         "p99_job_latency_note": "from the scaled job mix (the specified mix completes no job within the window)",
         "serving": serve,

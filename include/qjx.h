@@ -184,6 +184,24 @@ int qjx_batch_run_periods(qjx_batch *batch, int n_periods, int is_obs);
 int qjx_batch_get_stats(qjx_batch *batch, qjx_stats *out);
 int qjx_batch_set_profiling(qjx_batch *batch, int on);
 
+/* The dimer's Lindblad equation integrated in rho space on the GPU (FP64, RK4,
+ * steps_per_period even, the drive constant over each step, steps aligned with the
+ * half periods): the density matrix the trajectory ensemble estimates.  Replaces
+ * the reference's f-basis integrator CQdiss_os_dimer calcODE_std + calcRho
+ * (CQdiss/calcODE.cpp:310-389, CalcRho.cpp:4-67), with rho itself as the state.
+ * rho starts as |e_0><e_0|; rho is d x d complex, row-major interleaved. */
+typedef struct qjx_lindblad qjx_lindblad;
+int qjx_lindblad_create(const qjx_model_desc *model, int steps_per_period, qjx_lindblad **out);
+void qjx_lindblad_destroy(qjx_lindblad *lb);
+int qjx_lindblad_set_rho(qjx_lindblad *lb, const double *rho);
+int qjx_lindblad_get_rho(qjx_lindblad *lb, double *rho);
+int qjx_lindblad_run_periods(qjx_lindblad *lb, int n_periods);
+/* <k>, <(k - <k>)^2>, Tr(rho (H + E H_drv)), Tr(rho^2) and diag(rho) (d doubles);
+ * any pointer may be NULL. */
+int qjx_lindblad_observables(qjx_lindblad *lb, double *mean, double *m2, double *energy,
+                             double *purity, double *adr);
+int qjx_lindblad_dim(const qjx_lindblad *lb);
+
 typedef struct qjx_pool qjx_pool;
 
 typedef struct qjx_job_result {

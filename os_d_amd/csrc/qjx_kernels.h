@@ -67,6 +67,8 @@ void launch_import_state(int prec, void *slot0, void *slot1, const TrajState *st
 // FP64, row-major interleaved) and purity Tr rho^2.  Scratch: w[M] and
 // part[splits][L][L] doubles, splits = density_splits(M, L).
 int density_splits(int M, int L);
+// out[0] = sum |rho_e|^2 over n complex entries (fixed order, one workgroup)
+void launch_purity(const double *rho, size_t n, double *out, hipStream_t s);
 void launch_density(int prec, const void *slot0, const void *slot1, const TrajState *st, int M, int d,
                     int dp, int L, double *w, double *part, int splits, double *rho, double *purity,
                     hipStream_t s);

@@ -519,6 +519,11 @@ __global__ __launch_bounds__(1024) void purity_kernel(const double *rho, size_t 
     if (threadIdx.x == 0) out[0] = s[0];
 }
 
+void launch_purity(const double *rho, size_t n, double *out, hipStream_t s)
+{
+    purity_kernel<<<1, 1024, 0, s>>>(rho, n, out);
+}
+
 int density_splits(int M, int L)
 {
     const int tiles = (L / kDenTile) * (L / kDenTile);

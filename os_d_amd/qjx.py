@@ -100,6 +100,13 @@ SIGNATURES = [
     ("qjx_batch_run_periods", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     ("qjx_batch_get_stats", ctypes.c_int, [ctypes.c_void_p, _P(Stats)]),
     ("qjx_batch_set_profiling", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("qjx_lindblad_create", ctypes.c_int, [_P(ModelDesc), ctypes.c_int, _P(ctypes.c_void_p)]),
+    ("qjx_lindblad_destroy", None, [ctypes.c_void_p]),
+    ("qjx_lindblad_set_rho", ctypes.c_int, [ctypes.c_void_p, _D]),
+    ("qjx_lindblad_get_rho", ctypes.c_int, [ctypes.c_void_p, _D]),
+    ("qjx_lindblad_run_periods", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("qjx_lindblad_observables", ctypes.c_int, [ctypes.c_void_p, _D, _D, _D, _D, _D]),
+    ("qjx_lindblad_dim", ctypes.c_int, [ctypes.c_void_p]),
     ("qjx_pool_create", ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int, _P(ctypes.c_void_p)]),
     ("qjx_pool_destroy", None, [ctypes.c_void_p]),
     ("qjx_pool_submit", ctypes.c_int, [ctypes.c_void_p, _P(ModelDesc), _P(EnsembleDesc), ctypes.c_int,
@@ -283,6 +290,49 @@ class Batch:
 
     def set_profiling(self, on=True):
         _check(self.lib, self.lib.qjx_batch_set_profiling(self.h, 1 if on else 0))
+
+
+class Lindblad:
+    """The dimer's Lindblad equation integrated in rho space on the GPU (FP64 RK4)."""
+
+    def __init__(self, N, steps_per_period=2000, prm_E=1.0, prm_U=0.5, prm_J=1.0, drv_ampl=1.5, drv_freq=1.0,
+                 diss_gamma=0.1):
+        self.lib = load_library()
+        m = ModelDesc(N, prm_E, prm_U, prm_J, drv_ampl, drv_freq, diss_gamma, 1, 4, 0)
+        h = ctypes.c_void_p()
+        _check(self.lib, self.lib.qjx_lindblad_create(ctypes.byref(m), steps_per_period, ctypes.byref(h)))
+        self.h = h
+        self.d = N + 1
+
+    def close(self):
+        if self.h:
+            self.lib.qjx_lindblad_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_rho(self, rho):
+        r = np.ascontiguousarray(rho, dtype=np.complex128)
+        _check(self.lib, self.lib.qjx_lindblad_set_rho(self.h, _ptr(r.view(np.float64))))
+
+    def get_rho(self):
+        r = np.zeros((self.d, self.d), dtype=np.complex128)
+        _check(self.lib, self.lib.qjx_lindblad_get_rho(self.h, _ptr(r.view(np.float64))))
+        return r
+
+    def run_periods(self, n):
+        _check(self.lib, self.lib.qjx_lindblad_run_periods(self.h, n))
+
+    def observables(self):
+        v = np.zeros(4)
+        adr = np.zeros(self.d)
+        _check(self.lib, self.lib.qjx_lindblad_observables(self.h, _ptr(v[0:]), _ptr(v[1:]), _ptr(v[2:]),
+                                                           _ptr(v[3:]), _ptr(adr)))
+        return {"mean": v[0], "m2": v[1], "energy": v[2], "purity": v[3], "adr": adr}
 
 
 class Pool:
