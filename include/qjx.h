@@ -95,10 +95,30 @@ typedef struct qjx_stats {
 
 const char *qjx_last_error(void);
 
+/* Runge-Kutta propagation (prop_id 1) of the same model. */
+typedef struct qjx_rk_desc {
+    int rk_ns;              /* rk_ns: RK4 steps per period                          */
+    int drv_type;           /* dimer_drv_type: 1 = E + A sin(omega t + phase), any
+                               other value the square drive E +- A per half period  */
+    double drv_phase;       /* dimer_drv_phase                                      */
+} qjx_rk_desc;
+
 /* Build H, H_drv, L, H_eff,b and the propagator tree exp(-i H_eff,b dt_i),
  * dt_0 = T / steps_per_period, dt_i = dt_{i-1} / 2, on the current HIP device. */
 int qjx_ctx_create(const qjx_model_desc *model, qjx_ctx **out);
 void qjx_ctx_destroy(qjx_ctx *ctx);
+
+/* A context whose trajectories take rk_ns RK4 steps per period instead of the
+ * propagator tree: RKPropagateBehavior (propagator.cpp:66-129) with init_rk
+ * (rk_proc.cpp:56-63), rk_period (core.cpp:439-450) and rk_step_dimer / rk_int_dimer /
+ * rk_right_part_dimer (core.cpp:11033-11115, 11592-11619, 11886-11948).  Every other
+ * call below works on it unchanged (ensemble_init, trans, run_periods, run_steps
+ * (RK steps), chars, density, state, jumps, stats: matvecs counts right-hand-side
+ * evaluations); it has no propagators and is not batched.  model->precision must be
+ * QJX_FP64; ex_deep and steps_per_period are ignored.  Arithmetic: each multiply-add
+ * one fma, the band of H_part in ascending column order; the stage arguments use the
+ * full rk_step for every partial step, as arg_upd does (rk_proc.cpp:141-174). */
+int qjx_ctx_create_rk(const qjx_model_desc *model, const qjx_rk_desc *rk, qjx_ctx **out);
 
 /* Allocate the ensemble; states start at e_{start_state} (or uniform), RNG
  * streams at leap-frog(seed + tr, mns) of MCG59 seeded 777. */

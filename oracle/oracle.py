@@ -20,7 +20,9 @@ def build():
     os.makedirs(BUILD, exist_ok=True)
     srcs = [os.path.join(HERE, s) for s in SOURCES if os.path.exists(os.path.join(HERE, s))]
     tmp = LIB + ".tmp"
-    cmd = ["g++", "-O2", "-std=c++17", "-fopenmp", "-fPIC", "-shared"] + srcs + ["-o", tmp]
+    # fma() is the hardware instruction (-mfma); nothing else is contracted.
+    cmd = ["g++", "-O2", "-std=c++17", "-fopenmp", "-fPIC", "-shared", "-mfma", "-ffp-contract=off"] + \
+        srcs + ["-o", tmp]
     subprocess.run(cmd, check=True, capture_output=True, text=True)
     os.replace(tmp, LIB)
     return LIB
@@ -67,6 +69,15 @@ def lib():
         L.qjo_num_matvecs.argtypes = [ctypes.c_void_p]
         L.qjo_num_jumps.restype = ctypes.c_int64
         L.qjo_num_jumps.argtypes = [ctypes.c_void_p]
+        L.qjo_rk_setup.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        L.qjo_rk_sin.restype = ctypes.c_double
+        L.qjo_rk_sin.argtypes = [ctypes.c_double]
+        L.qjo_rk_drive.restype = ctypes.c_double
+        L.qjo_rk_drive.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.qjo_rk_rhs.argtypes = [ctypes.c_void_p, ctypes.c_double, _D, _D]
+        L.qjo_rk_norm.restype = ctypes.c_double
+        L.qjo_rk_norm.argtypes = [ctypes.c_int, _D]
+        L.qjo_rk_int.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
         L.qjl_evolve.argtypes = [_P(Model), ctypes.c_int, ctypes.c_int, _D]
         L.qjl_rhs.argtypes = [_P(Model), ctypes.c_double, _D, _D]
         _lib = L
@@ -121,6 +132,15 @@ def lindblad_observables(m, rho):
     Hs = mm["H"] + m.prm_E * mm["Hdrv"]
     return {"mean": mean, "m2": float(p @ (k - mean) ** 2), "energy": float(np.real(np.trace(rho @ Hs))),
             "purity": float(np.real(np.trace(rho @ rho))), "adr": p}
+
+
+def rk_sin(x):
+    return lib().qjo_rk_sin(float(x))
+
+
+def rk_norm(x):
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    return lib().qjo_rk_norm(x.shape[0], _p(x.view(np.float64)))
 
 
 def mcg59_uniform(k, mns, n):
@@ -207,6 +227,22 @@ class Ensemble:
             _p(None if eta is None else np.ascontiguousarray(eta, dtype=np.float64)),
             _p(None if t is None else np.ascontiguousarray(t, dtype=np.float64)),
             _p(None if rng is None else np.ascontiguousarray(rng, dtype=np.uint64), ctypes.c_uint64)))
+
+    def rk_setup(self, rk_ns, drv_type=0, drv_phase=0.0):
+        """Switch to Runge-Kutta propagation (prop_id 1)."""
+        self._ok(self.L.qjo_rk_setup(self.h, rk_ns, drv_type, drv_phase))
+
+    def rk_drive(self, t):
+        return self.L.qjo_rk_drive(self.h, t)
+
+    def rk_rhs(self, t, x):
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        k = np.zeros_like(x)
+        self._ok(self.L.qjo_rk_rhs(self.h, t, _p(x.view(np.float64)), _p(k.view(np.float64))))
+        return k
+
+    def rk_int(self, tr, step):
+        self._ok(self.L.qjo_rk_int(self.h, tr, step))
 
     def get_jumps(self, tr):
         n = self.L.qjo_get_jumps(self.h, tr, None, None, None, None, 0)

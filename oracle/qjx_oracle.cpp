@@ -190,6 +190,7 @@ struct Model {
     int N = 0, d = 0;
     double T = 0.0;
     std::vector<cplx> H, Hdrv, L, Hqj[2];
+    std::vector<cplx> Hp;                 /* H_part = H - i/2 gamma' L^dag L      */
 };
 
 static void build_model(const qjo_model *p, Model *m)
@@ -227,6 +228,9 @@ static void build_model(const qjo_model *p, Model *m)
             LdL[(size_t)i * d + j] = g * s;
         }
     const double E[2] = {p->prm_E + p->drv_ampl, p->prm_E - p->drv_ampl};
+    m->Hp.assign(dd, 0.0);
+    for (size_t idx = 0; idx < dd; idx++)
+        m->Hp[idx] = cplx(m->H[idx].real() + 0.5 * LdL[idx].imag(), 0.0 - 0.5 * LdL[idx].real());
     for (int b = 0; b < 2; b++) {
         m->Hqj[b].assign(dd, 0.0);
         for (size_t idx = 0; idx < dd; idx++) {
@@ -300,6 +304,10 @@ struct qjo_ens {
     std::vector<std::vector<double>> jt, jn, je;
     std::vector<std::vector<int>> jtype;
     std::vector<int64_t> matvecs, jumps_done;
+    /* Runge-Kutta propagation (prop_id 1), set by qjo_rk_setup */
+    int prop = 0, rk_ns = 0, drv_type = 0, lanes_E = 2;
+    double drv_phase = 0.0, rk_step = 0.0;
+    std::vector<double> b_dre, b_dim, b_o1re, b_o1im, b_o2im, b_drv, b_Ld, b_Lm, b_Lp;
     std::string err;
     volatile int failed = 0;
 };
@@ -531,22 +539,373 @@ static void top_steps(qjo_ens *e, int tr, int n, int is_obs)
     }
 }
 
+/* ------------------------------------------------------------------------- */
+/* Runge-Kutta trajectories (prop_id 1): rk_period (core.cpp:439-450),       */
+/* rk_step_dimer (core.cpp:11886-11948), rk_int_dimer (core.cpp:11592-11619), */
+/* rk_right_part_dimer (core.cpp:11033-11115), set_init_args / arg_upd /     */
+/* rk_final / rk_recovery / save_phi_prev / restore_from_prev                */
+/* (rk_proc.cpp:124-345), init_rk (rk_proc.cpp:56-63).                        */
+/*                                                                           */
+/* The reference leaves the floating-point order to MKL's zgemv and to the   */
+/* compiler; this restatement fixes one (rounding-level differences only):    */
+/*  - each multiply-add is one fma; the products H_part x and H_drv x run over */
+/*    the band |i - j| <= 2 in ascending j (every other entry of H_part and    */
+/*    H_drv is an exact zero, so the dense zgemv adds the same nonzero terms;  */
+/*    qjo_rk_setup checks that);                                               */
+/*  - the RK4 update is phi + acc * (step / 6) with one fma;                   */
+/*  - ||x||^2 sums 64 chunks of E consecutive entries (E = the smallest power  */
+/*    of two >= 2 with 64 E >= d), each in ascending order with fma, and adds  */
+/*    the 64 partial sums by a butterfly over lane distances 32, 16, .., 1;     */
+/*  - sin (dimer_drv_type 1) is rk_sin below, not the C library's.            */
+/* Reproduced as the reference has them: the stage arguments use the global   */
+/* rk_step whatever step rk_int is called with (arg_upd reads ed->rk_step,    */
+/* rk_proc.cpp:150-173), time advances by step * 0.5 twice per integration,  */
+/* a crossing restores the step-start state, moves time back by the step,     */
+/* integrates the linear-interpolation fraction, jumps, draws ran then eta    */
+/* (redrawn while 0) and integrates the rest (recursively: a loop here).      */
+/* ------------------------------------------------------------------------- */
+
+static int rk_lanes(int d)
+{
+    int E = 2;
+    while (64 * E < d) E *= 2;
+    return E;
+}
+
+static double rk_norm(const cplx *x, int d, int E)
+{
+    double part[64];
+    for (int l = 0; l < 64; l++) {
+        double p = 0.0;
+        for (int j = 0; j < E && l * E + j < d; j++) {
+            const cplx &v = x[l * E + j];
+            p = std::fma(v.real(), v.real(), p);
+            p = std::fma(v.imag(), v.imag(), p);
+        }
+        part[l] = p;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        double nxt[64];
+        for (int l = 0; l < 64; l++) nxt[l] = part[l] + part[l ^ off];
+        std::memcpy(part, nxt, sizeof part);
+    }
+    return part[0];
+}
+
+/* sin x: quadrant k = nearest integer to x 2/pi, r = x - k pi/2 with pi/2 split
+ * into 33 + 33 + 53 significant bits (Cody & Waite, "Software Manual for the
+ * Elementary Functions", 1980; k pi/2 parts are exact for |k| < 2^20), then the
+ * Taylor series of sin r or cos r on |r| <= pi/4 (truncation < 1e-26) by Horner. */
+static double rk_sin(double x)
+{
+    const double k = std::nearbyint(x * 0x1.45f306dc9c883p-1);
+    double r = x - k * 0x1.921fb54400000p+0;
+    r = r - k * 0x1.0b4611a600000p-34;
+    r = r - k * 0x1.3198a2e037073p-69;
+    const double z = r * r;
+    double ps = -0x1.761b41316381ap-75;
+    ps = std::fma(ps, z, 0x1.71b8ef6dcf572p-66);
+    ps = std::fma(ps, z, -0x1.2f49b46814157p-57);
+    ps = std::fma(ps, z, 0x1.952c77030ad4ap-49);
+    ps = std::fma(ps, z, -0x1.ae7f3e733b81fp-41);
+    ps = std::fma(ps, z, 0x1.6124613a86d09p-33);
+    ps = std::fma(ps, z, -0x1.ae64567f544e4p-26);
+    ps = std::fma(ps, z, 0x1.71de3a556c734p-19);
+    ps = std::fma(ps, z, -0x1.a01a01a01a01ap-13);
+    ps = std::fma(ps, z, 0x1.1111111111111p-7);
+    ps = std::fma(ps, z, -0x1.5555555555555p-3);
+    double pc = 0x1.f2cf01972f578p-80;
+    pc = std::fma(pc, z, -0x1.0ce396db7f853p-70);
+    pc = std::fma(pc, z, 0x1.e542ba4020225p-62);
+    pc = std::fma(pc, z, -0x1.6827863b97d97p-53);
+    pc = std::fma(pc, z, 0x1.ae7f3e733b81fp-45);
+    pc = std::fma(pc, z, -0x1.93974a8c07c9dp-37);
+    pc = std::fma(pc, z, 0x1.1eed8eff8d898p-29);
+    pc = std::fma(pc, z, -0x1.27e4fb7789f5cp-22);
+    pc = std::fma(pc, z, 0x1.a01a01a01a01ap-16);
+    pc = std::fma(pc, z, -0x1.6c16c16c16c17p-10);
+    pc = std::fma(pc, z, 0x1.5555555555555p-5);
+    pc = std::fma(pc, z, -0x1.0000000000000p-1);
+    const double s = std::fma(r * z, ps, r);
+    const double c = std::fma(z, pc, 1.0);
+    const int q = (int)((long long)k & 3);
+    return q == 0 ? s : q == 1 ? c : q == 2 ? -s : -c;
+}
+
+/* E(t) of rk_right_part_dimer (core.cpp:11061-11088): any type but 1 is the
+ * square drive. */
+static double rk_drive(const qjo_ens *e, double t)
+{
+    if (e->drv_type == 1) return e->p.prm_E + e->p.drv_ampl * rk_sin(e->p.drv_freq * t + e->drv_phase);
+    const double mt = std::fmod(t, e->m.T);
+    return mt < e->m.T * 0.5 ? e->p.prm_E + e->p.drv_ampl : e->p.prm_E - e->p.drv_ampl;
+}
+
+/* k = -i (H_part + E H_drv) x: k.re = +(nd.im + E drv.im), k.im = -(nd.re + E drv.re)
+ * with nd = H_part x, drv = H_drv x (core.cpp:11090-11114). */
+static void rk_rhs(const qjo_ens *e, double E, const cplx *x, cplx *k)
+{
+    const int d = e->m.d;
+    for (int i = 0; i < d; i++) {
+        double re = 0.0, im = 0.0;
+        auto mac = [&](double a, double b, const cplx &v) {
+            re = std::fma(a, v.real(), re);
+            re = std::fma(-b, v.imag(), re);
+            im = std::fma(a, v.imag(), im);
+            im = std::fma(b, v.real(), im);
+        };
+        auto mac_im = [&](double b, const cplx &v) {      /* entry 0 + i b */
+            re = std::fma(-b, v.imag(), re);
+            im = std::fma(b, v.real(), im);
+        };
+        if (i >= 2) mac_im(e->b_o2im[i - 2], x[i - 2]);
+        if (i >= 1) mac(e->b_o1re[i - 1], e->b_o1im[i - 1], x[i - 1]);
+        mac(e->b_dre[i], e->b_dim[i], x[i]);
+        if (i + 1 < d) mac(e->b_o1re[i], e->b_o1im[i], x[i + 1]);
+        if (i + 2 < d) mac_im(e->b_o2im[i], x[i + 2]);
+        const double dre = e->b_drv[i] * x[i].real();
+        const double dim = e->b_drv[i] * x[i].imag();
+        k[i] = cplx(std::fma(E, dim, im), -std::fma(E, dre, re));
+    }
+}
+
+struct RkWork {
+    std::vector<cplx> nw, arg, k, acc;
+    explicit RkWork(int d) : nw(d), arg(d), k(d), acc(d) {}
+};
+
+/* rk_int_dimer: out = RK4(phi, step) from time t; t advances by step * 0.5 twice. */
+static void rk_int(const qjo_ens *e, double &t, double step, const cplx *phi, cplx *out, RkWork &w)
+{
+    const int d = e->m.d;
+    const double h = 0.5 * e->rk_step, H = 1.0 * e->rk_step;
+    cplx *arg = w.arg.data(), *k = w.k.data(), *acc = w.acc.data();
+    rk_rhs(e, rk_drive(e, t), phi, k);
+    for (int i = 0; i < d; i++) {
+        acc[i] = k[i];
+        arg[i] = cplx(std::fma(h, k[i].real(), phi[i].real()), std::fma(h, k[i].imag(), phi[i].imag()));
+    }
+    t = t + step * 0.5;
+    rk_rhs(e, rk_drive(e, t), arg, k);
+    for (int i = 0; i < d; i++) {
+        acc[i] = cplx(std::fma(2.0, k[i].real(), acc[i].real()), std::fma(2.0, k[i].imag(), acc[i].imag()));
+        arg[i] = cplx(std::fma(h, k[i].real(), phi[i].real()), std::fma(h, k[i].imag(), phi[i].imag()));
+    }
+    rk_rhs(e, rk_drive(e, t), arg, k);
+    for (int i = 0; i < d; i++) {
+        acc[i] = cplx(std::fma(2.0, k[i].real(), acc[i].real()), std::fma(2.0, k[i].imag(), acc[i].imag()));
+        arg[i] = cplx(std::fma(H, k[i].real(), phi[i].real()), std::fma(H, k[i].imag(), phi[i].imag()));
+    }
+    t = t + step * 0.5;
+    rk_rhs(e, rk_drive(e, t), arg, k);
+    const double c6 = step / 6.0;
+    for (int i = 0; i < d; i++) {
+        const double ar = acc[i].real() + k[i].real(), ai = acc[i].imag() + k[i].imag();
+        out[i] = cplx(std::fma(ar, c6, phi[i].real()), std::fma(ai, c6, phi[i].imag()));
+    }
+}
+
+/* rk_recovery (rk_proc.cpp:198-306) with one dissipator: normalise, draw ran,
+ * phi = L phi / ||L phi||. */
+static void rk_recovery(qjo_ens *e, int tr, cplx *phi, double nsq, int is_obs, RkWork &w)
+{
+    const int d = e->m.d;
+    const double nrm = std::sqrt(nsq);
+    for (int i = 0; i < d; i++) phi[i] = cplx(phi[i].real() / nrm, phi[i].imag() / nrm);
+    (void)e->rng[tr].next();                                /* ran; index 0 */
+    cplx *res = w.k.data();
+    for (int i = 0; i < d; i++) {
+        double re = 0.0, im = 0.0;
+        if (i >= 1) {
+            re = std::fma(e->b_Lm[i], phi[i - 1].real(), re);
+            im = std::fma(e->b_Lm[i], phi[i - 1].imag(), im);
+        }
+        re = std::fma(e->b_Ld[i], phi[i].real(), re);
+        im = std::fma(e->b_Ld[i], phi[i].imag(), im);
+        if (i + 1 < d) {
+            re = std::fma(e->b_Lp[i], phi[i + 1].real(), re);
+            im = std::fma(e->b_Lp[i], phi[i + 1].imag(), im);
+        }
+        res[i] = cplx(re, im);
+    }
+    const double g = rk_norm(res, d, e->lanes_E) * 1.0;
+    const double sg = std::sqrt(g / 1.0);
+    for (int i = 0; i < d; i++) phi[i] = cplx(res[i].real() / sg, res[i].imag() / sg);
+    if (e->jump > 0 && is_obs == 1) e->jtype[tr].push_back(0);
+    e->jumps_done[tr] += 1;
+}
+
+/* rk_step_dimer as a loop over its tail recursion.  `cached` carries ||phi||^2
+ * of the current state between steps (recomputing it gives the same bits). */
+static void rk_step(qjo_ens *e, int tr, double step, int is_obs, double &cached, bool &have, RkWork &w)
+{
+    const int d = e->m.d, E = e->lanes_E;
+    cplx *phi = &e->psi[(size_t)tr * d];
+    double &t = e->t[tr];
+    double &eta = e->eta[tr];
+    for (int crossings = 0;; crossings++) {
+        const double prev = have ? cached : rk_norm(phi, d, E);
+        rk_int(e, t, step, phi, w.nw.data(), w);
+        e->matvecs[tr] += 4;
+        const double n = rk_norm(w.nw.data(), d, E);
+        if (n > 1.0) {                                      /* experiment.cpp:1247-1251 */
+            char buf[160];
+            std::snprintf(buf, sizeof buf, "norm more than 1 (trajectory %d, norm %.17g)", tr, n);
+            set_err(e, buf);
+            return;
+        }
+        /* the reference recurses without end on a NaN norm; stop instead */
+        if (!(n == n) || crossings >= (1 << 16)) {
+            set_err(e, "non-finite norm or endless crossings in one RK step");
+            return;
+        }
+        if (n > eta) {                                      /* is_norm_crossed == 0 */
+            std::memcpy(phi, w.nw.data(), sizeof(cplx) * d);
+            cached = n;
+            have = true;
+            return;
+        }
+        const double diff = prev - n;
+        const double bp = (prev - eta) / diff;
+        const double ep = 1.0 - bp;
+        const double bstep = step * bp, estep = step * ep;
+        t = t - step;                                       /* restore_from_prev */
+        rk_int(e, t, bstep, phi, w.nw.data(), w);
+        e->matvecs[tr] += 4;
+        std::memcpy(phi, w.nw.data(), sizeof(cplx) * d);
+        const double jn = rk_norm(phi, d, E);
+        if (e->jump > 0 && is_obs == 1) {
+            e->jt[tr].push_back(t);
+            e->jn[tr].push_back(jn);
+            e->je[tr].push_back(eta);
+            e->njumps[tr]++;
+        }
+        rk_recovery(e, tr, phi, jn, is_obs, w);
+        eta = e->rng[tr].next();
+        while (eta == 0.0) eta = e->rng[tr].next();
+        have = false;
+        step = estep;
+    }
+}
+
+/* n RK steps of one trajectory; step s of the run has period_id = pos / rk_ns,
+ * step_id = pos % rk_ns and starts at period_id T + step_id rk_step. */
+static void rk_steps(qjo_ens *e, int tr, int n, int is_obs)
+{
+    RkWork w(e->m.d);
+    double cached = 0.0;
+    bool have = false;
+    for (int s = 0; s < n && !e->failed; s++) {
+        const int pos = e->pos[tr];
+        const int period_id = pos / e->rk_ns, step_id = pos % e->rk_ns;
+        e->t[tr] = double(period_id) * e->m.T + double(step_id) * e->rk_step;
+        rk_step(e, tr, e->rk_step, is_obs, cached, have, w);
+        e->pos[tr] += 1;
+    }
+}
+
+extern "C" int qjo_rk_setup(qjo_ens *e, int rk_ns, int drv_type, double drv_phase)
+{
+    if (!e || rk_ns < 1) return -1;
+    const int d = e->m.d;
+    const Model &m = e->m;
+    auto hp = [&](int i, int j) -> const cplx & { return m.Hp[(size_t)i * d + j]; };
+    /* the band form must add exactly the dense zgemv's nonzero terms */
+    for (int i = 0; i < d; i++)
+        for (int j = 0; j < d; j++) {
+            const int o = j - i;
+            const cplx a = hp(i, j), b = hp(j, i);
+            if (a.real() != b.real() || a.imag() != b.imag()) { set_err(e, "H_part is not symmetric"); return -1; }
+            if ((o > 2 || o < -2) && (a != 0.0 || m.L[(size_t)i * d + j] != 0.0)) {
+                set_err(e, "H_part or L has entries outside the band"); return -1;
+            }
+            if ((o == 2 || o == -2) && a.real() != 0.0) { set_err(e, "H_part(i, i+2) is not imaginary"); return -1; }
+            if ((o == 2 || o == -2) && m.L[(size_t)i * d + j] != 0.0) { set_err(e, "L is not tridiagonal"); return -1; }
+            if (o != 0 && m.Hdrv[(size_t)i * d + j] != 0.0) { set_err(e, "H_drv is not diagonal"); return -1; }
+            if (m.Hdrv[(size_t)i * d + j].imag() != 0.0 || m.L[(size_t)i * d + j].imag() != 0.0) {
+                set_err(e, "H_drv or L is not real"); return -1;
+            }
+        }
+    e->prop = 1;
+    e->rk_ns = rk_ns;
+    e->drv_type = drv_type;
+    e->drv_phase = drv_phase;
+    e->rk_step = m.T / double(rk_ns);                     /* rk_proc.cpp:62 */
+    e->lanes_E = rk_lanes(d);
+    e->b_dre.assign(d, 0.0); e->b_dim.assign(d, 0.0); e->b_drv.assign(d, 0.0);
+    e->b_o1re.assign(d, 0.0); e->b_o1im.assign(d, 0.0); e->b_o2im.assign(d, 0.0);
+    e->b_Ld.assign(d, 0.0); e->b_Lm.assign(d, 0.0); e->b_Lp.assign(d, 0.0);
+    for (int i = 0; i < d; i++) {
+        e->b_dre[i] = hp(i, i).real();
+        e->b_dim[i] = hp(i, i).imag();
+        e->b_drv[i] = m.Hdrv[(size_t)i * d + i].real();
+        e->b_Ld[i] = m.L[(size_t)i * d + i].real();
+        if (i + 1 < d) {
+            e->b_o1re[i] = hp(i, i + 1).real();
+            e->b_o1im[i] = hp(i, i + 1).imag();
+            e->b_Lp[i] = m.L[(size_t)i * d + i + 1].real();
+        }
+        if (i >= 1) e->b_Lm[i] = m.L[(size_t)i * d + i - 1].real();
+        if (i + 2 < d) e->b_o2im[i] = hp(i, i + 2).imag();
+    }
+    for (int tr = 0; tr < e->M; tr++) e->pos[tr] = 0;
+    return 0;
+}
+
+extern "C" double qjo_rk_sin(double x) { return rk_sin(x); }
+
+extern "C" double qjo_rk_drive(const qjo_ens *e, double t) { return rk_drive(e, t); }
+
+extern "C" int qjo_rk_rhs(const qjo_ens *e, double t, const double *x, double *k)
+{
+    if (!e || e->prop != 1) return -1;
+    rk_rhs(e, rk_drive(e, t), reinterpret_cast<const cplx *>(x), reinterpret_cast<cplx *>(k));
+    return 0;
+}
+
+extern "C" double qjo_rk_norm(int d, const double *x)
+{
+    return rk_norm(reinterpret_cast<const cplx *>(x), d, rk_lanes(d));
+}
+
+extern "C" int qjo_rk_int(qjo_ens *e, int tr, double step)
+{
+    if (!e || e->prop != 1 || tr < 0 || tr >= e->M) return -1;
+    RkWork w(e->m.d);
+    cplx *phi = &e->psi[(size_t)tr * e->m.d];
+    rk_int(e, e->t[tr], step, phi, w.nw.data(), w);
+    std::memcpy(phi, w.nw.data(), sizeof(cplx) * e->m.d);
+    return 0;
+}
+
 extern "C" int qjo_steps(qjo_ens *e, int n, int is_obs)
 {
 #pragma omp parallel for schedule(dynamic, 1) num_threads(e->threads)
-    for (int tr = 0; tr < e->M; tr++) top_steps(e, tr, n, is_obs);
+    for (int tr = 0; tr < e->M; tr++) {
+        if (e->prop == 1) rk_steps(e, tr, n, is_obs);
+        else top_steps(e, tr, n, is_obs);
+    }
     return e->failed ? -1 : 0;
 }
 
 extern "C" int qjo_periods_one(qjo_ens *e, int tr, int n, int is_obs)
 {
     if (tr < 0 || tr >= e->M) return -1;
+    if (e->prop == 1) {
+        rk_steps(e, tr, n * e->rk_ns, is_obs);
+        return e->failed ? -1 : 0;
+    }
     for (int p = 0; p < n && !e->failed; p++) one_period(e, tr, is_obs);
     return e->failed ? -1 : 0;
 }
 
 extern "C" int qjo_periods(qjo_ens *e, int n, int is_obs)
 {
+    if (e->prop == 1) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(e->threads)
+        for (int tr = 0; tr < e->M; tr++) rk_steps(e, tr, n * e->rk_ns, is_obs);
+        return e->failed ? -1 : 0;
+    }
     for (int p = 0; p < n && !e->failed; p++) {
 #pragma omp parallel for schedule(dynamic, 1) num_threads(e->threads)
         for (int tr = 0; tr < e->M; tr++) one_period(e, tr, is_obs);
@@ -562,6 +921,14 @@ extern "C" int qjo_trans(qjo_ens *e, int num_tp_periods)
         double &eta = e->eta[tr];
         eta = 0.0;
         while (eta == 0.0) eta = e->rng[tr].next();
+        if (e->prop == 1) {
+            /* transient periods are numbered from 0, and so are the observed ones
+             * (obser_process starts at dump_periods[0] = 0, experiment.cpp:476-488) */
+            e->pos[tr] = 0;
+            rk_steps(e, tr, num_tp_periods * e->rk_ns, 0);
+            e->pos[tr] = 0;
+            continue;
+        }
         for (int p = 0; p < num_tp_periods && !e->failed; p++) one_period(e, tr, 0);
     }
     for (int tr = 0; tr < e->M; tr++) e->t[tr] = 0.0;

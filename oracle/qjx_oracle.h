@@ -16,6 +16,12 @@
  *   - stepping / jumps:        brute-force restatements on tiny inputs, norm-monotonicity
  *                              and crossing invariants, gamma = 0 => no jumps, dark state.
  *   - ensemble observables:    the Lindblad rho-space RK4 oracle (lindblad_oracle.h).
+ *   - RK path (qjo_rk_*):      the band RHS against the dense -i (H_part + E H_drv) x, RK4
+ *                              order-4 convergence to expm, the sub-step closed form with
+ *                              the reference's global-rk_step stage arguments, the sine
+ *                              against the C library within 2 ulp, the sine drive against
+ *                              a DOP853 solution, RNG draw counts, and ensemble averages
+ *                              against the Lindblad oracle.
  * No reference file was compiled or imported; the reference needs Eigen and MKL
  * headers that are not present, so it is unbuildable here.
  */
@@ -91,6 +97,22 @@ int qjo_get_jumps(const qjo_ens *e, int tr, double *times, int *types, double *n
 /* Counters: matrix-vector products applied (bulk + refinement), jumps applied. */
 int64_t qjo_num_matvecs(const qjo_ens *e);
 int64_t qjo_num_jumps(const qjo_ens *e);
+
+/* Runge-Kutta propagation (prop_id 1): switch the ensemble to rk_ns RK4 steps
+ * per period with drive type drv_type (1: E + A sin(omega t + phase), else the
+ * square drive) and phase drv_phase; qjo_trans / qjo_periods / qjo_steps then run
+ * RK steps.  Propagators are not needed (create with build_propagators = 0). */
+int qjo_rk_setup(qjo_ens *e, int rk_ns, int drv_type, double drv_phase);
+/* The sine used by the drive (Cody-Waite reduction + Taylor polynomials). */
+double qjo_rk_sin(double x);
+/* E(t) of the drive. */
+double qjo_rk_drive(const qjo_ens *e, double t);
+/* k = -i (H_part + E(t) H_drv) x (x, k: d interleaved complex). */
+int qjo_rk_rhs(const qjo_ens *e, double t, const double *x, double *k);
+/* ||x||^2 in the chunked summation order of the RK path. */
+double qjo_rk_norm(int d, const double *x);
+/* One RK4 integration of trajectory tr over `step` from its current time. */
+int qjo_rk_int(qjo_ens *e, int tr, double step);
 
 #ifdef __cplusplus
 }

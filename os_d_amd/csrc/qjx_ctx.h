@@ -56,6 +56,16 @@ struct qjx_ctx {
     std::vector<std::vector<int>> jty;
 
     bool profiling = false;        // per-kernel timing of the iterations
+
+    // Runge-Kutta propagation (prop_id 1, qjx_ctx_create_rk): no propagator tree;
+    // every trajectory takes rk_ns RK4 steps per period in the RK kernel.
+    int prop = 0;                  // 0 quantum-jump tree, 1 Runge-Kutta
+    int rk_ns = 0, rk_drv_type = 0;
+    double rk_phase = 0.0, rk_step = 0.0;
+    long long rk_pos = 0;          // RK steps done since period 0 of the current phase
+    double *d_rkbands = nullptr;   // band coefficients, lane-major (qjx_rk.h)
+    long long rk_launches = 0;
+    double rk_ms = 0.0;            // device time of the RK launches (profiling on)
 };
 
 namespace qjx {

@@ -19,6 +19,7 @@
 #include "../../include/qjx.h"
 #include "qjx_internal.h"
 #include "qjx_kernels.h"
+#include "qjx_rk.h"
 #include "qjx_runner.h"
 
 namespace {
@@ -336,6 +337,177 @@ int qjx::ctx_create_kind(const qjx_model_desc *m, int gemm_kind, qjx_ctx **out)
     return QJX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Runge-Kutta propagation (prop_id 1): RKPropagateBehavior::init_prop_data ->
+// init_rk (propagator.cpp:66-69, rk_proc.cpp:56-63); the model matrices of
+// init_hamiltonians_qj (newdel.cpp:2285-2365) as band coefficients.
+// ---------------------------------------------------------------------------
+
+// H_part = H - i/2 gamma' L^dag L (newdel.cpp:2327-2352) on its band |i - j| <= 2,
+// H_drv and L, in the RK kernel's lane-major layout: entry i = l E + j of array a
+// at (a E + j) 64 + l, zero beyond d.  (L^dag L)_ij sums L_ki L_kj in ascending k
+// as build_model does, so H_part(i, j) and H_part(j, i) are the same number.
+static void build_rk_bands(const qjx_ctx *c, int E, std::vector<double> &out)
+{
+    const int d = c->d;
+    const double g = c->model.diss_gamma / double(d - 1);
+    auto Lel = [&](int i, int j) -> double {
+        if (i == j) return c->Ldiag[i];
+        if (i == j + 1) return -c->Loff[j];
+        if (j == i + 1) return c->Loff[i];
+        return 0.0;
+    };
+    auto hpart = [&](int i, int j, double *re, double *im) {
+        double s = 0.0;
+        for (int k = std::max(0, std::max(i, j) - 1); k <= std::min(d - 1, std::min(i, j) + 1); k++)
+            s += Lel(k, i) * Lel(k, j);
+        const double ldl = g * s;
+        double h = 0.0;
+        if (i == j) h = c->Hdiag[i];
+        else if (i == j + 1) h = c->Hoff[j];
+        else if (j == i + 1) h = c->Hoff[i];
+        *re = h + 0.5 * 0.0;        // Im(diss_part) = 0
+        *im = 0.0 - 0.5 * ldl;
+    };
+    out.assign((size_t)qjx::kRkBands * E * 64, 0.0);
+    auto at = [&](int a, int i) -> double & { return out[((size_t)a * E + i % E) * 64 + i / E]; };
+    for (int i = 0; i < d; i++) {
+        double re, im;
+        hpart(i, i, &re, &im);
+        at(qjx::kRkDre, i) = re;
+        at(qjx::kRkDim, i) = im;
+        at(qjx::kRkDrv, i) = c->Hdrv[i];
+        at(qjx::kRkLd, i) = Lel(i, i);
+        if (i + 1 < d) {
+            hpart(i, i + 1, &re, &im);
+            at(qjx::kRkO1re, i) = re;
+            at(qjx::kRkO1im, i) = im;
+            at(qjx::kRkLp, i) = Lel(i, i + 1);
+        }
+        if (i + 2 < d) {
+            hpart(i, i + 2, &re, &im);
+            at(qjx::kRkO2im, i) = im;
+        }
+        if (i >= 1) at(qjx::kRkLm, i) = Lel(i, i - 1);
+    }
+}
+
+extern "C" int qjx_ctx_create_rk(const qjx_model_desc *m, const qjx_rk_desc *rk, qjx_ctx **out)
+{
+    if (!m || !rk || !out) return fail(QJX_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (m->N < 1 || m->N + 1 > qjx::kMaxDim) return fail(QJX_ERR_ARG, "N must be in [1, 2047]");
+    if (m->precision != QJX_FP64)
+        return fail(QJX_ERR_ARG, "the Runge-Kutta path is FP64 only (complex128, as the reference)");
+    if (rk->rk_ns < 1) return fail(QJX_ERR_ARG, "rk_ns must be >= 1");
+    if (!(m->drv_freq > 0.0)) return fail(QJX_ERR_ARG, "dimer_drv_freq must be > 0");
+    int dev_count = 0;
+    if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count < 1)
+        return fail(QJX_ERR_HIP, "no HIP device available");
+    qjx_ctx *c = new qjx_ctx;
+    c->model = *m;
+    c->prop = 1;
+    c->d = m->N + 1;
+    c->dp = (c->d + 63) / 64 * 64;
+    c->L = 2 * c->dp;
+    c->D = 1;
+    c->S = 2;
+    c->G = 2;
+    c->gemm_kind = qjx::kGemmMfma;
+    c->R = c->L / qjx::gemm_bm(c->gemm_kind);
+    c->prec = QJX_FP64;
+    c->esize = c->pesize = 8;
+    c->T = 2.0 * kPi / m->drv_freq;                     // newdel.cpp:31
+    c->dt.assign(1, c->T);
+    c->rk_ns = rk->rk_ns;
+    c->rk_drv_type = rk->drv_type;
+    c->rk_phase = rk->drv_phase;
+    c->rk_step = c->T / double(rk->rk_ns);              // rk_proc.cpp:62
+    build_model(c);
+    auto bail = [&](int code) {
+        qjx_ctx_destroy(c);
+        return code;
+    };
+    int rc = QJX_OK;
+    if (hipStreamCreate(&c->stream) != hipSuccess) return bail(fail(QJX_ERR_HIP, "hipStreamCreate failed"));
+    if ((rc = upload((void **)&c->d_dt, c->dt.data(), sizeof(double)))) return bail(rc);
+    if ((rc = upload((void **)&c->d_Ldiag, c->Ldiag.data(), sizeof(double) * c->d))) return bail(rc);
+    if ((rc = upload((void **)&c->d_Loff, c->Loff.data(), sizeof(double) * c->Loff.size()))) return bail(rc);
+    if ((rc = upload((void **)&c->d_HdiagE, c->HdiagE.data(), sizeof(double) * c->d))) return bail(rc);
+    if ((rc = upload((void **)&c->d_Hoff, c->Hoff.data(), sizeof(double) * c->Hoff.size()))) return bail(rc);
+    std::vector<double> bands;
+    build_rk_bands(c, qjx::rk_lanes(c->d), bands);
+    if ((rc = upload((void **)&c->d_rkbands, bands.data(), sizeof(double) * bands.size()))) return bail(rc);
+    *out = c;
+    return QJX_OK;
+}
+
+// Advance every trajectory by n RK steps (rk_period, core.cpp:439-450, over all
+// trajectories).  With logging on, one period per launch and a drain after each;
+// otherwise launches of bounded work.
+static int check_err(qjx_ctx *c);
+static int drain_log(qjx_ctx *c);
+
+static int rk_run(qjx_ctx *c, long long n, int is_obs)
+{
+    const bool logging = is_obs && c->ens.jump > 0;
+    const double work = std::max(1.0, double(c->M) * double(c->d));
+    const long long chunk = logging ? c->rk_ns : std::max(1LL, (long long)(2.0e10 / work));
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    if (c->profiling) {
+        HIP_TRY(hipEventCreate(&ev[0]));
+        HIP_TRY(hipEventCreate(&ev[1]));
+    }
+    int rc = QJX_OK;
+    while (n > 0 && rc == QJX_OK) {
+        const long long k = std::min(n, chunk);
+        qjx::RkArgs a{};
+        a.bands = c->d_rkbands;
+        a.slot0 = c->slot[0];
+        a.slot1 = c->slot[1];
+        a.st = c->st;
+        a.log = c->log;
+        a.err = c->err;
+        a.counters = c->counters;
+        a.stride = c->stride;
+        a.pos0 = c->rk_pos;
+        a.T = c->T;
+        a.rk_step = c->rk_step;
+        a.prm_E = c->model.prm_E;
+        a.ampl = c->model.drv_ampl;
+        a.freq = c->model.drv_freq;
+        a.phase = c->rk_phase;
+        a.M = c->M;
+        a.d = c->d;
+        a.dp = c->dp;
+        a.L = c->L;
+        a.nsteps = (int)k;
+        a.rk_ns = c->rk_ns;
+        a.drv_type = c->rk_drv_type;
+        a.jump = c->ens.jump;
+        a.is_obs = is_obs;
+        if (c->profiling && hipEventRecord(ev[0], c->stream) != hipSuccess) rc = fail(QJX_ERR_HIP, "event record");
+        const hipError_t e = qjx::launch_rk(a, c->stream);
+        if (e != hipSuccess) {
+            rc = fail(QJX_ERR_HIP, std::string("RK launch: ") + hipGetErrorString(e));
+            break;
+        }
+        if (c->profiling && hipEventRecord(ev[1], c->stream) != hipSuccess) rc = fail(QJX_ERR_HIP, "event record");
+        c->rk_launches++;
+        c->rk_pos += k;
+        n -= k;
+        if (rc == QJX_OK) rc = check_err(c);      // synchronises
+        if (rc == QJX_OK && c->profiling) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) c->rk_ms += ms;
+        }
+        if (rc == QJX_OK && logging) rc = drain_log(c);
+    }
+    for (auto &x : ev)
+        if (x) (void)hipEventDestroy(x);
+    return rc;
+}
+
 static void free_ensemble(qjx_ctx *c)
 {
     for (int i = 0; i < 2; i++) {
@@ -375,6 +547,7 @@ extern "C" void qjx_ctx_destroy(qjx_ctx *c)
     dev_free(c->d_Loff);
     dev_free(c->d_HdiagE);
     dev_free(c->d_Hoff);
+    dev_free(c->d_rkbands);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -491,6 +664,11 @@ static int check_err(qjx_ctx *c)
 {
     int h[4];
     HIP_TRY(hipMemcpy(h, c->err, sizeof h, hipMemcpyDeviceToHost));
+    if (h[0] == 2) {
+        char buf[200];
+        std::snprintf(buf, sizeof buf, "trajectory %d: non-finite norm or endless crossings in one RK step", h[1]);
+        return fail(QJX_ERR_NORM, buf);
+    }
     if (h[0]) {
         const unsigned long long bits = (unsigned long long)(unsigned)h[2] | ((unsigned long long)(unsigned)h[3] << 32);
         double n;
@@ -538,12 +716,16 @@ static int drain_log(qjx_ctx *c)
 
 // Advance every trajectory by n top-level steps: seed the lists, then run
 // iteration chunks until no trajectory has steps left.
-static int run_steps(qjx_ctx *c, int n, int is_obs)
+static long long steps_per_period(const qjx_ctx *c) { return c->prop == 1 ? c->rk_ns : c->S; }
+
+static int run_steps(qjx_ctx *c, long long n, int is_obs)
 {
     if (n <= 0) return QJX_OK;
+    if (c->prop == 1) return rk_run(c, n, is_obs);
+    if (n > (1LL << 30)) return fail(QJX_ERR_ARG, "too many steps in one call");
     int rc = ensure_runner(c);
     if (rc) return rc;
-    qjx::launch_seed(c->st, c->M, n, c->lists[0], c->cnts[0], c->D, c->stream);
+    qjx::launch_seed(c->st, c->M, (int)n, c->lists[0], c->cnts[0], c->D, c->stream);
     HIP_TRY(hipGetLastError());
     c->desc.is_obs = is_obs;
     rc = qjx::runner_set_table(c->runner, &c->desc, 1);
@@ -556,9 +738,14 @@ static int run_steps(qjx_ctx *c, int n, int is_obs)
 extern "C" int qjx_trans(qjx_ctx *c, int num_tp_periods)
 {
     if (!c || !c->have_ens) return fail(QJX_ERR_ARG, "ensemble not initialised");
+    if (num_tp_periods < 0) return fail(QJX_ERR_ARG, "negative period count");
     qjx::launch_draw_eta(c->st, c->M, c->stride, c->stream);
-    int rc = run_steps(c, num_tp_periods * c->S, 0);
+    // RK: transient periods are numbered from 0, and so are the observed ones
+    // (obser_process starts at dump_periods[0] = 0, experiment.cpp:476-488)
+    c->rk_pos = 0;
+    int rc = run_steps(c, (long long)num_tp_periods * steps_per_period(c), 0);
     if (rc) return rc;
+    c->rk_pos = 0;
     qjx::launch_reset_times(c->st, c->M, c->stream);
     HIP_TRY(hipStreamSynchronize(c->stream));
     return QJX_OK;
@@ -568,7 +755,7 @@ extern "C" int qjx_run_periods(qjx_ctx *c, int n, int is_obs)
 {
     if (!c || !c->have_ens) return fail(QJX_ERR_ARG, "ensemble not initialised");
     if (n < 0) return fail(QJX_ERR_ARG, "negative period count");
-    int rc = run_steps(c, n * c->S, is_obs);
+    int rc = run_steps(c, (long long)n * steps_per_period(c), is_obs);
     if (rc) return rc;
     if (is_obs && c->ens.jump > 0) return drain_log(c);
     return QJX_OK;
@@ -697,6 +884,7 @@ extern "C" int qjx_get_jumps(qjx_ctx *c, int tr, double *times, int *types, doub
 
 extern "C" int qjx_get_propagator(qjx_ctx *c, int b, int lvl, double *U)
 {
+    if (c && c->prop == 1) return fail(QJX_ERR_ARG, "a Runge-Kutta context has no propagators");
     if (!c || !U || b < 0 || b > 1 || lvl < 0 || lvl >= c->D) return fail(QJX_ERR_ARG, "bad propagator index");
     const size_t LL = (size_t)c->L * c->L;
     std::vector<char> raw(LL * c->pesize), rawlo;
@@ -730,6 +918,7 @@ extern "C" int qjx_get_propagator(qjx_ctx *c, int b, int lvl, double *U)
 
 extern "C" int qjx_set_propagator(qjx_ctx *c, int b, int lvl, const double *U)
 {
+    if (c && c->prop == 1) return fail(QJX_ERR_ARG, "a Runge-Kutta context has no propagators");
     if (!c || !U || b < 0 || b > 1 || lvl < 0 || lvl >= c->D) return fail(QJX_ERR_ARG, "bad propagator index");
     const size_t nn2 = (size_t)2 * c->d * c->d;
     double *dU = nullptr;
@@ -762,6 +951,13 @@ extern "C" int qjx_get_stats(qjx_ctx *c, qjx_stats *out)
     out->gemm_bn = c->runner ? c->runner->bn : 0;
     out->gemm_grid = c->runner ? c->runner->grid.gemm : 0;
     out->gemm_waves = c->runner ? c->runner->grid.waves : 0;
+    if (c->prop == 1) {                 // RK: matvecs = right-hand-side evaluations
+        out->iterations = c->rk_launches;
+        out->gemm_flops = 0.0;
+        out->gemm_band_flops = 0.0;
+        out->gemm_ms = c->rk_ms;
+        out->gemm_launches = c->rk_launches;
+    }
     return QJX_OK;
 }
 
@@ -799,6 +995,7 @@ extern "C" int qjx_set_profiling(qjx_ctx *c, int on)
 {
     if (!c) return fail(QJX_ERR_ARG, "null context");
     c->profiling = on != 0;
+    c->rk_ms = 0.0;
     if (c->runner) {
         c->runner->profiling = c->profiling;
         c->runner->plan_ms = c->runner->gemm_ms = c->runner->tail_ms = 0.0;
@@ -829,6 +1026,9 @@ extern "C" int qjx_batch_create(qjx_ctx *const *ctxs, int n, qjx_batch **out)
     if (!ctxs || !out) return fail(QJX_ERR_ARG, "null argument");
     *out = nullptr;
     if (n < 1 || n > qjx::kMaxEns) return fail(QJX_ERR_ARG, "a batch holds 1 to 64 contexts");
+    for (int i = 0; i < n; i++)
+        if (ctxs[i] && ctxs[i]->prop == 1)
+            return fail(QJX_ERR_ARG, "Runge-Kutta contexts are not batched (no step GEMM to share)");
     long long total = 0;
     for (int i = 0; i < n; i++) {
         const qjx_ctx *c = ctxs[i];

@@ -38,6 +38,10 @@ class ModelDesc(ctypes.Structure):
                 ("precision", ctypes.c_int)]
 
 
+class RkDesc(ctypes.Structure):
+    _fields_ = [("rk_ns", ctypes.c_int), ("drv_type", ctypes.c_int), ("drv_phase", ctypes.c_double)]
+
+
 class EnsembleDesc(ctypes.Structure):
     _fields_ = [("num_trajectories", ctypes.c_int), ("seed", ctypes.c_int), ("mns", ctypes.c_int),
                 ("start_type", ctypes.c_int), ("start_state", ctypes.c_int), ("jump", ctypes.c_int),
@@ -74,6 +78,7 @@ _D = _P(ctypes.c_double)
 SIGNATURES = [
     ("qjx_last_error", ctypes.c_char_p, []),
     ("qjx_ctx_create", ctypes.c_int, [_P(ModelDesc), _P(ctypes.c_void_p)]),
+    ("qjx_ctx_create_rk", ctypes.c_int, [_P(ModelDesc), _P(RkDesc), _P(ctypes.c_void_p)]),
     ("qjx_ctx_destroy", None, [ctypes.c_void_p]),
     ("qjx_ensemble_init", ctypes.c_int, [ctypes.c_void_p, _P(EnsembleDesc)]),
     ("qjx_trans", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
@@ -147,15 +152,28 @@ def _ptr(a, ctype=ctypes.c_double):
 
 
 class Engine:
-    """One dimer model + one trajectory ensemble on the current HIP device."""
+    """One dimer model + one trajectory ensemble on the current HIP device.
+
+    prop="qj" (prop_id 0) propagates with the bisection tree of exp(-i H_eff dt);
+    prop="rk" (prop_id 1) takes rk_ns RK4 steps per period with drive type drv_type
+    (1: E + A sin(omega t + drv_phase), else the square drive), FP64 only."""
 
     def __init__(self, N, prm_E=1.0, prm_U=0.5, prm_J=1.0, drv_ampl=1.5, drv_freq=1.0,
-                 diss_gamma=0.1, ex_deep=16, steps_per_period=4, precision=FP64):
+                 diss_gamma=0.1, ex_deep=16, steps_per_period=4, precision=FP64, prop="qj",
+                 rk_ns=1000, drv_type=0, drv_phase=0.0):
         self.lib = load_library()
         self.model = ModelDesc(N, prm_E, prm_U, prm_J, drv_ampl, drv_freq, diss_gamma, ex_deep,
                                steps_per_period, precision)
         h = ctypes.c_void_p()
-        _check(self.lib, self.lib.qjx_ctx_create(ctypes.byref(self.model), ctypes.byref(h)))
+        if prop == "rk":
+            self.rk = RkDesc(rk_ns, drv_type, drv_phase)
+            _check(self.lib, self.lib.qjx_ctx_create_rk(ctypes.byref(self.model), ctypes.byref(self.rk),
+                                                        ctypes.byref(h)))
+        elif prop == "qj":
+            _check(self.lib, self.lib.qjx_ctx_create(ctypes.byref(self.model), ctypes.byref(h)))
+        else:
+            raise ValueError("prop must be 'qj' or 'rk'")
+        self.prop = prop
         self.h = h
         self.d = N + 1
         self.M = 0
