@@ -1,7 +1,7 @@
 // qjx_config.cpp -- the drop-in runner: the reference's config.txt / params.txt in,
 // its STD-experiment output files out, with the propagation on the GPU engine.
 //
-// Restates, for sys_id 0 / task_id 1 / prop_id 0 (reference files under
+// Restates, for sys_id 0 / task_id 1 / prop_id 0 and 1 (reference files under
 // source/cpp/QJX/QJX/):
 //   init_params / set_param (config.cpp:29-145), the single-run main
 //   (main.cpp:3-192) and Processor::process (processor.cpp:50-75),
@@ -246,8 +246,9 @@ extern "C" int qjx_run_config(const char *config_path, const char *params_path, 
     }
     {
         Params p{c.params, ""};
-        if (c.sys_id != 0 || c.task_id != 1 || c.prop_id != 0)
-            CFG_FAIL(QJX_ERR_ARG, "only sys_id 0 (dimer), task_id 1 (STD), prop_id 0 (QJ) are implemented");
+        if (c.sys_id != 0 || c.task_id != 1 || (c.prop_id != 0 && c.prop_id != 1))
+            CFG_FAIL(QJX_ERR_ARG,
+                     "only sys_id 0 (dimer), task_id 1 (STD), prop_id 0 (QJ) or 1 (RK) are implemented");
         sfx = suffix_param(c, p);
         qjx_model_desc md{};
         md.N = int(p.get("N"));
@@ -274,6 +275,8 @@ extern "C" int qjx_run_config(const char *config_path, const char *params_path, 
         const int num_random_obs = int(p.get("num_random_obs"));
         const int start_type = int(p.get("start_type"));
         const int start_state = int(p.get("start_state"));
+        const int drv_type = int(p.get("dimer_drv_type"));
+        const double drv_phase = p.get("dimer_drv_phase");
         if (!p.missing.empty()) CFG_FAIL(QJX_ERR_ARG, "params.txt lacks key " + p.missing);
         if (num_random_obs != 0)
             CFG_FAIL(QJX_ERR_ARG, "num_random_obs > 0 (random observables) is not implemented; set it to 0");
@@ -283,7 +286,18 @@ extern "C" int qjx_run_config(const char *config_path, const char *params_path, 
         dps = dump_periods(dump_type, dump_num, c.num_obs_periods);
         dnt = (int)dps.size();
 
-        API_CHECK(qjx_ctx_create(&md, &ctx));
+        if (c.prop_id == 1) {
+            // RKPropagateBehavior (propagator.cpp:66-129): complex128 as the reference,
+            // whatever GEMM form was asked for the quantum-jump path
+            qjx_rk_desc rk{};
+            rk.rk_ns = c.rk_ns;
+            rk.drv_type = drv_type;
+            rk.drv_phase = drv_phase;
+            md.precision = QJX_FP64;
+            API_CHECK(qjx_ctx_create_rk(&md, &rk, &ctx));
+        } else {
+            API_CHECK(qjx_ctx_create(&md, &ctx));
+        }
         M = c.num_trajectories;
         d = md.N + 1;
         qjx_ensemble_desc ed{};

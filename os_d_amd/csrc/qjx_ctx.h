@@ -64,6 +64,8 @@ struct qjx_ctx {
     double rk_phase = 0.0, rk_step = 0.0;
     long long rk_pos = 0;          // RK steps done since period 0 of the current phase
     double *d_rkbands = nullptr;   // band coefficients, lane-major (qjx_rk.h)
+    double *d_rktab = nullptr;     // drive table of one launch (3 doubles per step)
+    long long rk_tab_steps = 0;    // steps d_rktab holds
     long long rk_launches = 0;
     double rk_ms = 0.0;            // device time of the RK launches (profiling on)
 };

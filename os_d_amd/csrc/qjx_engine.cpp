@@ -452,7 +452,15 @@ static int rk_run(qjx_ctx *c, long long n, int is_obs)
 {
     const bool logging = is_obs && c->ens.jump > 0;
     const double work = std::max(1.0, double(c->M) * double(c->d));
-    const long long chunk = logging ? c->rk_ns : std::max(1LL, (long long)(2.0e10 / work));
+    const long long chunk = std::min(logging ? (long long)c->rk_ns : std::max(1LL, (long long)(2.0e10 / work)),
+                                     1LL << 22);
+    const long long tab_steps = std::min(chunk, n);
+    if (tab_steps > c->rk_tab_steps) {
+        dev_free(c->d_rktab);
+        c->rk_tab_steps = 0;
+        HIP_TRY(hipMalloc((void **)&c->d_rktab, (size_t)tab_steps * 3 * sizeof(double)));
+        c->rk_tab_steps = tab_steps;
+    }
     hipEvent_t ev[2] = {nullptr, nullptr};
     if (c->profiling) {
         HIP_TRY(hipEventCreate(&ev[0]));
@@ -463,6 +471,7 @@ static int rk_run(qjx_ctx *c, long long n, int is_obs)
         const long long k = std::min(n, chunk);
         qjx::RkArgs a{};
         a.bands = c->d_rkbands;
+        a.tab = c->d_rktab;
         a.slot0 = c->slot[0];
         a.slot1 = c->slot[1];
         a.st = c->st;
@@ -548,6 +557,7 @@ extern "C" void qjx_ctx_destroy(qjx_ctx *c)
     dev_free(c->d_HdiagE);
     dev_free(c->d_Hoff);
     dev_free(c->d_rkbands);
+    dev_free(c->d_rktab);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

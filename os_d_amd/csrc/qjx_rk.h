@@ -33,6 +33,7 @@ inline int rk_lanes(int d)
 
 struct RkArgs {
     const double *bands;      // [kRkBands][E][64]
+    double *tab;              // [3 nsteps]: drives of each step's full integration (scratch)
     void *slot0, *slot1;      // FP64 state columns (L doubles: Re at [0, dp), Im at [dp, 2 dp))
     TrajState *st;
     JumpLog log;
@@ -46,8 +47,9 @@ struct RkArgs {
     int jump, is_obs;
 };
 
-// Advance every trajectory by a.nsteps RK steps (one wavefront per trajectory).
-// Returns hipErrorInvalidValue if d exceeds the kernel's largest lane width.
+// Advance every trajectory by a.nsteps RK steps (one wavefront per trajectory);
+// a.tab must hold 3 a.nsteps doubles.  Returns hipErrorInvalidValue if d exceeds
+// the kernel's largest lane width.
 hipError_t launch_rk(const RkArgs &a, hipStream_t s);
 
 }  // namespace qjx

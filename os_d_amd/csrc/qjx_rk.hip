@@ -79,16 +79,23 @@ __device__ inline double rk_drive(const RkArgs &a, double t)
     return mt < a.T * 0.5 ? a.prm_E + a.ampl : a.prm_E - a.ampl;
 }
 
-__device__ inline double shfl_up1(double v, int lane)
+// Neighbour exchange by wavefront shifts (DPP wave_shr:1 / wave_shl:1, one
+// v_mov_dpp per dword): shfl_up1 gives lane l the value of lane l-1 (lane 0: 0),
+// shfl_down1 the value of lane l+1 (lane 63: 0).
+__device__ inline double shfl_up1(double v, int)
 {
-    const double u = __shfl_up(v, 1, 64);
-    return lane == 0 ? 0.0 : u;
+    const unsigned long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffULL), 0x138, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xf, 0xf, true);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
-__device__ inline double shfl_down1(double v, int lane)
+__device__ inline double shfl_down1(double v, int)
 {
-    const double u = __shfl_down(v, 1, 64);
-    return lane == 63 ? 0.0 : u;
+    const unsigned long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffULL), 0x130, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xf, 0xf, true);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
 template <int E>
@@ -190,25 +197,48 @@ __device__ inline void rk_stage(double (&xr)[E], double (&xi)[E], const double (
     }
 }
 
-// rk_int_dimer: x = RK4(phi, step) from time t; t advances by step * 0.5 twice.
-// The stage arguments use the global rk_step whatever step is (arg_upd reads
-// ed->rk_step, rk_proc.cpp:150-173).
-template <int E>
-__device__ inline void rk_integrate(const RkArgs &a, double &t, double step, const double (&pr)[E],
-                                    const double (&pi)[E], double (&xr)[E], double (&xi)[E], double (&ar)[E],
-                                    double (&ai)[E], const double *lds, const Halo &h, int lane)
+// E(t) at the three stage times of an integration from t: t, t + step/2 (stages 2
+// and 3) and t + step, with t advanced by step * 0.5 twice as rk_int_dimer does.
+__device__ inline void rk_drives(const RkArgs &a, double &t, double step, double &e0, double &e1, double &e2)
 {
-    const double hh = 0.5 * a.rk_step, H = 1.0 * a.rk_step;
-    // E at t, t + step/2 (stages 2 and 3) and t + step, one drive evaluation site
-    double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+    e0 = e1 = e2 = 0.0;
 #pragma unroll 1
-    for (int q = 0; q < 3; q++) {
+    for (int q = 0; q < 3; q++) {      // one drive evaluation site
         const double v = rk_drive(a, t);
         e0 = q == 0 ? v : e0;
         e1 = q == 1 ? v : e1;
         e2 = q == 2 ? v : e2;
         if (q < 2) t = t + step * 0.5;
     }
+}
+
+// The drives of the first (full rk_step) integration of each step of a launch:
+// they depend on the step index only, so one thread per step writes them and the
+// trajectories read them (crossing sub-steps evaluate their own).
+__global__ void rk_drive_table_kernel(RkArgs a)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nsteps) return;
+    const long long gs = a.pos0 + k;
+    const long long period_id = gs / a.rk_ns, step_id = gs % a.rk_ns;
+    double t = double(period_id) * a.T + double(step_id) * a.rk_step;
+    double e0, e1, e2;
+    rk_drives(a, t, a.rk_step, e0, e1, e2);
+    a.tab[3 * k] = e0;
+    a.tab[3 * k + 1] = e1;
+    a.tab[3 * k + 2] = e2;
+}
+
+// rk_int_dimer: x = RK4(phi, step) with stage drives e0, e1, e2.  The stage
+// arguments use the global rk_step whatever step is (arg_upd reads ed->rk_step,
+// rk_proc.cpp:150-173).
+template <int E>
+__device__ inline void rk_integrate(const RkArgs &a, double step, double e0, double e1, double e2,
+                                    const double (&pr)[E], const double (&pi)[E], double (&xr)[E],
+                                    double (&xi)[E], double (&ar)[E], double (&ai)[E], const double *lds,
+                                    const Halo &h, int lane)
+{
+    const double hh = 0.5 * a.rk_step, H = 1.0 * a.rk_step;
 #pragma unroll
     for (int j = 0; j < E; j++) {
         xr[j] = pr[j];
@@ -261,7 +291,18 @@ __global__ __launch_bounds__(64 * kWaves) void rk_kernel(RkArgs a)
         bool partial = false;   // integrating the part of a crossed step before its jump
         int crossings = 0;
         for (;;) {   // rk_step_dimer, its tail recursion as a loop (one integration site)
-            rk_integrate<E>(a, t, step, pr, pi, xr, xi, ar, ai, rk_lds, h, lane);
+            double e0, e1, e2;
+            if (!partial && crossings == 0) {   // the step's full integration: tabulated drives
+                const int kk = __builtin_amdgcn_readfirstlane(k);
+                e0 = a.tab[3 * kk];
+                e1 = a.tab[3 * kk + 1];
+                e2 = a.tab[3 * kk + 2];
+                t = t + step * 0.5;
+                t = t + step * 0.5;
+            } else {
+                rk_drives(a, t, step, e0, e1, e2);
+            }
+            rk_integrate<E>(a, step, e0, e1, e2, pr, pi, xr, xi, ar, ai, rk_lds, h, lane);
             integrations++;
             if (!partial) {
                 const double n = wave_norm<E>(xr, xi);
@@ -386,6 +427,9 @@ hipError_t launch_rk_e(const RkArgs &a, hipStream_t s)
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
+    rk_drive_table_kernel<<<(a.nsteps + 255) / 256, 256, 0, s>>>(a);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
     const int blocks = (a.M + kWaves - 1) / kWaves;
     rk_kernel<E><<<blocks, 64 * kWaves, lds, s>>>(a);
     return hipGetLastError();

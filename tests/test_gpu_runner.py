@@ -95,3 +95,44 @@ def test_run_config_outputs_match_oracle(tmp_path, oracle_mod):
         np.testing.assert_allclose(jt, ref, rtol=1e-15)
     text = open(out + "mean" + SUFFIX).read().splitlines()[0]
     assert "e" in text and len(text.split("e")[0]) == 18      # setprecision(16) scientific
+
+
+SUFFIX_RK = ("_setup(0_1_1)_rnd(0_1000000)_N(12)_diss(0_0.2000_0.0000)_drv(1_1.5000_1.0000_0.3000)"
+             "_prm(1.0000_0.5000_1.0000)_start(0_0).txt")
+
+
+def test_run_config_rk_outputs_match_oracle(tmp_path, oracle_mod):
+    """prop_id 1 in config.txt selects the Runge-Kutta path (RKPropagateBehavior,
+    propagator.cpp:66-129) with rk_ns from config.txt and the sine drive of
+    params.txt; the files equal the oracle's RK run through the same experiment."""
+    from os_d_amd import qjx
+    print("case: run_config RK N=12 M=24 rk_ns=2000", flush=True)
+    out = str(tmp_path) + "/"
+    cfg = CONFIG.format(path=out).replace("prop_id 0", "prop_id 1").replace("rk_ns 10000", "rk_ns 2000")
+    prm = PARAMS.replace("dimer_drv_type 0", "dimer_drv_type 1").replace("dimer_drv_phase 0.0",
+                                                                          "dimer_drv_phase 0.3")
+    (tmp_path / "config.txt").write_text(cfg)
+    (tmp_path / "params.txt").write_text(prm)
+    # the GEMM form argument does not apply: the RK path is FP64
+    qjx.run_config(str(tmp_path / "config.txt"), str(tmp_path / "params.txt"), qjx.FP32, 0)
+    m = oracle_mod.model(12, gamma=0.2, D=1)
+    o = oracle_mod.Ensemble(m, 24, jump=2, build_propagators=False)
+    o.rk_setup(2000, 1, 0.3)
+    o.trans(2)
+    c0 = o.chars(is_start=1)
+    evo = [c0]
+    for _ in range(3):
+        o.periods(2, 1)
+        evo.append(o.chars())
+
+    def load(name):
+        return np.loadtxt(out + name + SUFFIX_RK)
+    np.testing.assert_allclose(load("mean"), evo[-1]["mean"], rtol=1e-12)
+    np.testing.assert_allclose(load("mean_start"), c0["mean"], rtol=1e-12)
+    me = load("mean_evo")
+    for k in range(4):
+        np.testing.assert_allclose(me[k], evo[k]["mean"], rtol=1e-12)
+    for tr in (0, 7, 23):
+        f = out + "jump_times_%d" % tr + SUFFIX_RK
+        jt = np.atleast_1d(np.loadtxt(f)) if os.path.getsize(f) else np.zeros(0)
+        assert np.array_equal(jt, o.get_jumps(tr)["times"])
