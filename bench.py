@@ -277,6 +277,65 @@ def lindblad(qjx, N=100, steps=10000, periods=2, cpu_steps=200):
             "observables_after": {"mean": ob["mean"], "energy": ob["energy"], "purity": ob["purity"]}}
 
 
+RK_FLOPS_PER_ENTRY_STEP = 182.0   # FP64 flops per state entry per RK step (fma = 2)
+
+
+def rk_leg(qjx, N, rk_ns, M, steps, cpu_budget_s, drv_type=0, with_cpu=True):
+    """The Runge-Kutta trajectory path (prop_id 1): M trajectories of the dimer at
+    N with rk_ns RK4 steps per period, `steps` steps timed on the GPU (kernel time
+    from HIP events on the engine stream and wall time), and the CPU restatement on
+    the host cores for a bounded sample of the same steps (same bits)."""
+    from oracle import oracle as orc
+    params = dict(prm_E=1.0, prm_U=0.5, prm_J=1.0, drv_ampl=1.5, drv_freq=1.0)
+    e = qjx.Engine(N, diss_gamma=0.1, prop="rk", rk_ns=rk_ns, drv_type=drv_type, **params)
+    e.ensemble_init(M, seed=0, jump=0)
+    e.trans(0)
+    e.run_steps(min(steps, 200), 0)              # warm-up (code objects, caches)
+    s0 = e.stats()
+    e.set_profiling(1)
+    t0 = time.perf_counter()
+    e.run_steps(steps, 0)
+    wall = time.perf_counter() - t0
+    s1 = e.stats()
+    e.close()
+    kernel_s = s1["gemm_ms"] * 1e-3
+    rhs = s1["matvecs"] - s0["matvecs"]
+    flops = RK_FLOPS_PER_ENTRY_STEP * (N + 1) * rhs / 4.0
+    res = {"N": N, "dim": N + 1, "rk_ns": rk_ns, "drv_type": drv_type, "trajectories": M, "steps": steps,
+           "trajectory_periods_per_s": M * steps / rk_ns / wall,
+           "trajectory_steps_per_s": M * steps / wall,
+           "wall_s": wall, "kernel_s": kernel_s, "launches": s1["gemm_launches"],
+           "integrations_per_step": rhs / 4.0 / (M * steps), "jumps": s1["jumps"] - s0["jumps"],
+           "roofline": {"bound": "fp64-valu", "achieved": flops / kernel_s / 1e12 if kernel_s > 0 else None,
+                        "peak": PEAK["fp64"], "unit": "TFLOP/s",
+                        "frac": flops / kernel_s / 1e12 / PEAK["fp64"] if kernel_s > 0 else None,
+                        "flops_note": "%.0f FP64 flops per state entry per RK4 integration (4 stencil stages "
+                                      "of the pentadiagonal H_part + diagonal drive, RK updates, norm)"
+                                      % RK_FLOPS_PER_ENTRY_STEP,
+                        "kernel": "rk_kernel (one wavefront per trajectory, state in registers)"}}
+    if with_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+        threads = min(threads, os.cpu_count() or threads)
+        m = orc.model(N, gamma=0.1, D=1, **params)
+        ntraj = 2 * threads
+        o = orc.Ensemble(m, ntraj, seed=0, jump=0, threads=threads, build_propagators=False)
+        o.rk_setup(rk_ns, drv_type, 0.0)
+        o.trans(0)
+        t0 = time.perf_counter()
+        n = 0
+        chunk = max(1, rk_ns // 100)
+        while time.perf_counter() - t0 < cpu_budget_s and n < 100 * rk_ns:
+            o.steps(chunk, 0)
+            n += chunk
+        dtc = time.perf_counter() - t0
+        cpu = ntraj * n / rk_ns / dtc
+        res["cpu_baseline"] = {"value": cpu, "unit": "trajectory-periods/s", "cores": threads, "kind": "port",
+                               "sample": "%d trajectories x %d RK steps at d=%d, CPU restatement, %.1f s"
+                                         % (ntraj, n, N + 1, dtc)}
+        res["gpu_over_cpu"] = res["trajectory_periods_per_s"] / cpu
+    return res
+
+
 SERVE_MIXES = {
     # the serving configuration as specified
     "specified": {"N": [100, 250, 500, 1000, 2000], "logM": (3.0, 5.0), "periods": (50, 500)},
@@ -356,6 +415,9 @@ def main():
                     help="also time the batched workload instance by instance")
     ap.add_argument("--only-batch", action="store_true")
     ap.add_argument("--no-lindblad", action="store_true")
+    ap.add_argument("--no-rk", action="store_true")
+    ap.add_argument("--only-rk", action="store_true", help="only the Runge-Kutta trajectory legs")
+    ap.add_argument("--rk-cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=30.0, help="serving leg window (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
@@ -384,6 +446,16 @@ def main():
 
     from os_d_amd import qjx
     N, M, S, D = args.N, args.traj, args.S, args.D
+
+    def rk_legs(with_cpu):
+        # the reference's CPU configuration (N=100) with its default rk_ns, and the
+        # headline dimension with a step that keeps RK4 stable (h |H| ~ 0.5)
+        return {"d101": rk_leg(qjx, 100, 10000, 10000, 20000, args.rk_cpu_seconds, with_cpu=with_cpu),
+                "d1001": rk_leg(qjx, 1000, 40000, 4096, 4000, args.rk_cpu_seconds, with_cpu=with_cpu)}
+
+    if args.only_rk:
+        print(json.dumps({"rk": rk_legs(not args.no_cpu)}), flush=True)
+        return
     if args.only_batch:
         print(json.dumps(batched(qjx, S, D, 1, args.batch_steps, qjx.FP32, bool(args.batch_alone))), flush=True)
         return
@@ -416,6 +488,13 @@ def main():
     lind = None
     if not args.no_lindblad:
         lind = lindblad(qjx)
+
+    rk = None
+    if not args.no_rk:
+        if e64 is not None:
+            e64.close()
+            e64 = None
+        rk = rk_legs(not args.no_cpu)
 
     serve = None
     if args.serve_seconds > 0:
@@ -481,6 +560,7 @@ def main():
         "observables_vs_fp64": {"fp32": observable_errors(r32, fp64), "bf16x3": observable_errors(bf, fp64)},
         "batched_uneven": batch,
         "lindblad": lind,
+        "rk_trajectories": rk,
         "p99_job_latency_s": serve["scaled"]["p99_latency_s"] if serve else None,
         "p99_job_latency_note": "from the scaled job mix (the specified mix completes no job within the window)",
         "serving": serve,

@@ -250,8 +250,15 @@ __device__ inline void rk_integrate(const RkArgs &a, double step, double e0, dou
     rk_stage<E, kStageLast>(xr, xi, pr, pi, ar, ai, e2, step / 6.0, lds, h, lane);
 }
 
+// Waves per SIMD the register allocation aims for at each lane width.
 template <int E>
-__global__ __launch_bounds__(64 * kWaves) void rk_kernel(RkArgs a)
+struct RkOcc {
+    static constexpr int waves = E <= 2 ? 4 : E == 4 ? 3 : E == 8 ? 2 : 1;
+};
+
+template <int E>
+__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(RkOcc<E>::waves)))
+void rk_kernel(RkArgs a)
 {
     extern __shared__ double rk_lds[];
     constexpr int NB = kRkStageBands * E * 64;
