@@ -254,9 +254,12 @@ int qjx_pool_result(qjx_pool *pool, int job_id, qjx_job_result *out, double *mea
 double qjx_pool_clock(const qjx_pool *pool);
 
 /* Drop-in runner: read config.txt / params.txt (reference keys, config.cpp:29-145),
- * run the STD experiment and write the reference's output files into `path`
- * (core.cpp:918-1084, dump.cpp:26-61).  precision selects the GEMM form;
- * steps_per_period = 0 means the reference's 4. */
+ * run the STD experiment (task_id 1) or the STD deep experiment (task_id 4: a dump
+ * after each of the 2 deep_num_steps sub-periods of a period, experiment.cpp:769-862)
+ * with QJ (prop_id 0) or RK (prop_id 1) propagation, and write the reference's output
+ * files into `path` (core.cpp:918-1084, dump.cpp:26-61).  precision selects the GEMM
+ * form of the QJ path; steps_per_period = 0 means the reference's 4 (task 4 fixes it
+ * at 4 deep_num_steps). */
 int qjx_run_config(const char *config_path, const char *params_path, int precision,
                    int steps_per_period);
 

@@ -1,13 +1,15 @@
 // qjx_config.cpp -- the drop-in runner: the reference's config.txt / params.txt in,
 // its STD-experiment output files out, with the propagation on the GPU engine.
 //
-// Restates, for sys_id 0 / task_id 1 / prop_id 0 and 1 (reference files under
+// Restates, for sys_id 0 / task_id 1 and 4 (STD, STD deep) / prop_id 0 and 1 (reference files under
 // source/cpp/QJX/QJX/):
 //   init_params / set_param (config.cpp:29-145), the single-run main
 //   (main.cpp:3-192) and Processor::process (processor.cpp:50-75),
 //   DimerOutputBehavior::suffix_param (outputter.cpp:3-44, 353-388),
 //   init_dump_periods (newdel_exp.cpp:408-455),
 //   StdExperimentBehaviour::trans_process / obser_process (experiment.cpp:411-542),
+//   StdDeepExperimentBehaviour (experiment.cpp:769-862), init_dump_periods_deep
+//   (newdel_exp.cpp:377-406),
 //   DimerCoreBehaviour::dump_std / dump_std_evo (core.cpp:918-1084),
 //   dump_adr_single / dump_adr_avg (dump.cpp:3-61), dump_phi(_evo) (core.cpp:12567-12613),
 //   the text writers (utils.cpp:64-467).
@@ -230,6 +232,8 @@ extern "C" int qjx_run_config(const char *config_path, const char *params_path, 
     std::string sfx;
     int M = 0, d = 0, jump = 0, dump_obs = 0, dump_evo_sep = 0, dump_evo_avg = 0, dump_adr_sep = 0,
         dump_adr_avg = 0, dump_phi = 0, dump_phi_evo = 0, jumps_lim = 0, dnt = 0;
+    bool deep = false;
+    int deep_steps = 0, sub_steps = 0;   // StdDeep: sub-periods per half period, steps per sub-period
     std::vector<int> dps;
     std::vector<double> norm_evo, mean_evo, disp_evo, m2_evo, energy_evo, adr, adr_avg, psi;
     std::vector<double> norm, mean_start, mean, disp, m2, energy;
@@ -246,9 +250,10 @@ extern "C" int qjx_run_config(const char *config_path, const char *params_path, 
     }
     {
         Params p{c.params, ""};
-        if (c.sys_id != 0 || c.task_id != 1 || (c.prop_id != 0 && c.prop_id != 1))
-            CFG_FAIL(QJX_ERR_ARG,
-                     "only sys_id 0 (dimer), task_id 1 (STD), prop_id 0 (QJ) or 1 (RK) are implemented");
+        if (c.sys_id != 0 || (c.task_id != 1 && c.task_id != 4) || (c.prop_id != 0 && c.prop_id != 1))
+            CFG_FAIL(QJX_ERR_ARG, "only sys_id 0 (dimer), task_id 1 (STD) or 4 (STD deep), prop_id 0 (QJ) "
+                                  "or 1 (RK) are implemented");
+        deep = c.task_id == 4;
         sfx = suffix_param(c, p);
         qjx_model_desc md{};
         md.N = int(p.get("N"));
@@ -277,20 +282,37 @@ extern "C" int qjx_run_config(const char *config_path, const char *params_path, 
         const int start_state = int(p.get("start_state"));
         const int drv_type = int(p.get("dimer_drv_type"));
         const double drv_phase = p.get("dimer_drv_phase");
+        if (deep) deep_steps = int(p.get("deep_num_steps"));
         if (!p.missing.empty()) CFG_FAIL(QJX_ERR_ARG, "params.txt lacks key " + p.missing);
         if (num_random_obs != 0)
             CFG_FAIL(QJX_ERR_ARG, "num_random_obs > 0 (random observables) is not implemented; set it to 0");
-        if (dump_type != 0 && dump_type != 1) CFG_FAIL(QJX_ERR_ARG, "dump_type must be 0 or 1");
-        if (dump_num < 1 || (dump_type == 0 && c.num_obs_periods < dump_num))
-            CFG_FAIL(QJX_ERR_ARG, "dump_num must be in [1, num_obs_periods]");
-        dps = dump_periods(dump_type, dump_num, c.num_obs_periods);
+        if (deep) {
+            // StdDeep (task 4): a dump after each of the 2 deep_num_steps sub-periods of
+            // every period; dump_type / dump_num do not apply (init_dump_periods_deep,
+            // newdel_exp.cpp:377-406).  The QJ tree's branch head is T / (2 deep_num_steps)
+            // with two children (init_split_structure_dimer_deep, core.cpp:9928-9981): 4
+            // deep_num_steps top-level steps per period, 2 per sub-period.  The RK step is
+            // T / (rk_ns 2 deep_num_steps) (init_rk_deep, rk_proc.cpp:65-76).
+            if (deep_steps < 1) CFG_FAIL(QJX_ERR_ARG, "deep_num_steps must be >= 1");
+            const int sub = 2 * deep_steps;
+            dps.assign((size_t)sub * c.num_obs_periods + 1, 0);
+            for (int pid = 0; pid < c.num_obs_periods; pid++)
+                for (int k = 0; k < sub; k++) dps[(size_t)pid * sub + k + 1] = pid;
+            md.steps_per_period = 4 * deep_steps;
+            sub_steps = c.prop_id == 1 ? c.rk_ns : 2;
+        } else {
+            if (dump_type != 0 && dump_type != 1) CFG_FAIL(QJX_ERR_ARG, "dump_type must be 0 or 1");
+            if (dump_num < 1 || (dump_type == 0 && c.num_obs_periods < dump_num))
+                CFG_FAIL(QJX_ERR_ARG, "dump_num must be in [1, num_obs_periods]");
+            dps = dump_periods(dump_type, dump_num, c.num_obs_periods);
+        }
         dnt = (int)dps.size();
 
         if (c.prop_id == 1) {
             // RKPropagateBehavior (propagator.cpp:66-129): complex128 as the reference,
             // whatever GEMM form was asked for the quantum-jump path
             qjx_rk_desc rk{};
-            rk.rk_ns = c.rk_ns;
+            rk.rk_ns = deep ? c.rk_ns * 2 * deep_steps : c.rk_ns;
             rk.drv_type = drv_type;
             rk.drv_phase = drv_phase;
             md.precision = QJX_FP64;
@@ -369,38 +391,57 @@ extern "C" int qjx_run_config(const char *config_path, const char *params_path, 
         IO_CHECK(adr_single(false));
         if (dump_evo_avg == 1) IO_CHECK(adr_avg_dump(false));
 
-        // obser_process (experiment.cpp:448-542)
-        {
-            int pr = phi_evo(false);
-            if (pr < 0) CFG_FAIL(QJX_ERR_HIP, qjx_last_error());
-            IO_CHECK(pr);
-        }
-        bool all_finished = false;
-        for (int dump_id = 1; dump_id < dnt; dump_id++) {
-            if (c.is_pp == 1) std::cout << "dump_id: " << dump_id << std::endl;
-            const int n = dps[dump_id] - dps[dump_id - 1];
-            const bool per_period = jump > 0 || jumps_lim > 0;
-            if (per_period) {
-                for (int k = 0; k < n; k++) {
-                    API_CHECK(qjx_run_periods(ctx, 1, 1));
-                    if (jumps_lim > 0) {
-                        API_CHECK(qjx_get_state(ctx, nullptr, nullptr, nullptr, nj.data(), nullptr));
-                        all_finished = std::all_of(nj.begin(), nj.end(), [&](int v) { return v >= jumps_lim; });
-                        if (all_finished) break;
-                    }
+        if (deep) {
+            // StdDeepExperimentBehaviour::obser_process (experiment.cpp:806-862) with
+            // ex_period_obs_deep / rk_period_obs_deep (core.cpp:114-153, 490-530): chars
+            // after every sub-period, the adr average appended once per period, no phi
+            // dumps, no jumps_counts stop
+            const int sub = 2 * deep_steps;
+            for (int pid = 0; pid < c.num_obs_periods; pid++) {
+                if (c.is_pp == 1) std::cout << "dump_id: " << pid + 1 << std::endl;
+                for (int k = 0; k < sub; k++) {
+                    API_CHECK(qjx_run_steps(ctx, sub_steps, 1));
+                    API_CHECK(chars(0, pid * sub + k + 1));
+                    IO_CHECK(adr_single(true));
                 }
-            } else if (n > 0) {
-                API_CHECK(qjx_run_periods(ctx, n, 1));
+                if (dump_evo_avg == 1) IO_CHECK(adr_avg_dump(true));
             }
-            API_CHECK(chars(0, dump_id));
-            IO_CHECK(adr_single(true));
+        }
+
+        // obser_process (experiment.cpp:448-542)
+        if (!deep) {
             {
-                int pr = phi_evo(true);
+                int pr = phi_evo(false);
                 if (pr < 0) CFG_FAIL(QJX_ERR_HIP, qjx_last_error());
                 IO_CHECK(pr);
             }
-            if (dump_evo_avg == 1) IO_CHECK(adr_avg_dump(true));
-            if (all_finished && jumps_lim > 0) break;
+            bool all_finished = false;
+            for (int dump_id = 1; dump_id < dnt; dump_id++) {
+                if (c.is_pp == 1) std::cout << "dump_id: " << dump_id << std::endl;
+                const int n = dps[dump_id] - dps[dump_id - 1];
+                const bool per_period = jump > 0 || jumps_lim > 0;
+                if (per_period) {
+                    for (int k = 0; k < n; k++) {
+                        API_CHECK(qjx_run_periods(ctx, 1, 1));
+                        if (jumps_lim > 0) {
+                            API_CHECK(qjx_get_state(ctx, nullptr, nullptr, nullptr, nj.data(), nullptr));
+                            all_finished = std::all_of(nj.begin(), nj.end(), [&](int v) { return v >= jumps_lim; });
+                            if (all_finished) break;
+                        }
+                    }
+                } else if (n > 0) {
+                    API_CHECK(qjx_run_periods(ctx, n, 1));
+                }
+                API_CHECK(chars(0, dump_id));
+                IO_CHECK(adr_single(true));
+                {
+                    int pr = phi_evo(true);
+                    if (pr < 0) CFG_FAIL(QJX_ERR_HIP, qjx_last_error());
+                    IO_CHECK(pr);
+                }
+                if (dump_evo_avg == 1) IO_CHECK(adr_avg_dump(true));
+                if (all_finished && jumps_lim > 0) break;
+            }
         }
 
         // dump_std (core.cpp:918-957)
@@ -412,14 +453,14 @@ extern "C" int qjx_run_config(const char *config_path, const char *params_path, 
             IO_CHECK(save_doubles(path + "m2" + sfx, m2.data(), M, false));
             IO_CHECK(save_doubles(path + "energy" + sfx, energy.data(), M, false));
         }
-        // dump_phi (core.cpp:12567-12589)
-        if (dump_phi == 1) {
+        // dump_phi (core.cpp:12567-12589); StdDeep writes no phi files
+        if (dump_phi == 1 && !deep) {
             psi.assign((size_t)M * d * 2, 0.0);
             API_CHECK(qjx_get_state(ctx, psi.data(), nullptr, nullptr, nullptr, nullptr));
             for (int tr = 0; tr < M; tr++)
                 IO_CHECK(save_complex(path + "phi_" + std::to_string(tr) + sfx, &psi[(size_t)tr * d * 2], d, false));
         }
-        {
+        if (!deep) {
             int pr = phi_evo(true);
             if (pr < 0) CFG_FAIL(QJX_ERR_HIP, qjx_last_error());
             IO_CHECK(pr);

@@ -136,3 +136,56 @@ def test_run_config_rk_outputs_match_oracle(tmp_path, oracle_mod):
         f = out + "jump_times_%d" % tr + SUFFIX_RK
         jt = np.atleast_1d(np.loadtxt(f)) if os.path.getsize(f) else np.zeros(0)
         assert np.array_equal(jt, o.get_jumps(tr)["times"])
+
+
+def _deep_files(tmp_path, prop_id, k, rk_ns=2000, drv_type=0):
+    out = str(tmp_path) + "/"
+    cfg = CONFIG.format(path=out).replace("task_id 1", "task_id 4").replace(
+        "prop_id 0", "prop_id %d" % prop_id).replace("rk_ns 10000", "rk_ns %d" % rk_ns).replace(
+        "num_obs_periods 6", "num_obs_periods 2")
+    prm = PARAMS.replace("dimer_drv_type 0", "dimer_drv_type %d" % drv_type) + "deep_num_steps %d\n" % k
+    (tmp_path / "config.txt").write_text(cfg)
+    (tmp_path / "params.txt").write_text(prm)
+    return out
+
+
+@pytest.mark.parametrize("prop_id", [0, 1])
+def test_run_config_deep_outputs_match_oracle(tmp_path, oracle_mod, prop_id):
+    """task_id 4 (StdDeepExperimentBehaviour, experiment.cpp:769-862): a dump after
+    every sub-period, 2 deep_num_steps per period.  QJ: the deep tree's branch head is
+    T/(2k) with two children (core.cpp:9928-9981), i.e. 4k top-level steps per period;
+    RK: rk_ns steps per sub-period of T/(2k) (rk_proc.cpp:65-76, core.cpp:490-530)."""
+    from os_d_amd import qjx
+    k, rk_ns = 3, 400
+    print("case: run_config deep prop_id=%d k=%d" % (prop_id, k), flush=True)
+    out = _deep_files(tmp_path, prop_id, k, rk_ns=rk_ns, drv_type=1)
+    qjx.run_config(str(tmp_path / "config.txt"), str(tmp_path / "params.txt"), qjx.FP64, 0)
+    sfx = "_setup(0_4_%d)_rnd(0_1000000)_N(12)_diss(0_0.2000_0.0000)_drv(1_1.5000_1.0000_0.0000)" \
+          "_prm(1.0000_0.5000_1.0000)_start(0_0).txt" % prop_id
+    if prop_id == 0:
+        m = oracle_mod.model(12, gamma=0.2, S=4 * k, D=12)
+        o = oracle_mod.Ensemble(m, 24, jump=2)
+        per_sub = 2
+    else:
+        m = oracle_mod.model(12, gamma=0.2, D=1)
+        o = oracle_mod.Ensemble(m, 24, jump=2, build_propagators=False)
+        o.rk_setup(rk_ns * 2 * k, 1, 0.0)
+        per_sub = rk_ns
+    o.trans(2)
+    evo = [o.chars(is_start=1)]
+    for _ in range(2):
+        for _ in range(2 * k):
+            o.steps(per_sub, 1)
+            evo.append(o.chars())
+    periods = np.atleast_1d(np.loadtxt(out + "periods" + sfx)).astype(int)
+    assert list(periods) == [0] + [p for p in range(2) for _ in range(2 * k)]
+    me = np.loadtxt(out + "mean_evo" + sfx)
+    assert me.shape == (4 * k + 1, 24)
+    rtol = 1e-9 if prop_id == 0 else 1e-12
+    for i in range(4 * k + 1):
+        np.testing.assert_allclose(me[i], evo[i]["mean"], rtol=rtol)
+    for tr in (0, 11):
+        f = out + "jump_times_%d" % tr + sfx
+        jt = np.atleast_1d(np.loadtxt(f)) if os.path.getsize(f) else np.zeros(0)
+        np.testing.assert_allclose(jt, o.get_jumps(tr)["times"], rtol=1e-15 if prop_id else 1e-12)
+    assert not glob.glob(out + "phi_*")
