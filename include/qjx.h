@@ -160,6 +160,15 @@ int qjx_set_state(qjx_ctx *ctx, const double *psi, const double *eta, const doub
 int qjx_get_jumps(qjx_ctx *ctx, int tr, double *times, int *types, double *norms, double *etas,
                   int cap);
 
+/* Distribution of the times between consecutive logged jumps of every trajectory,
+ * as the reference's post-processing bins the jump_times files
+ * (plot/mpipks/qjx/dimer/times_between_jumps.m:41-131): num_decades * bins_per_decade
+ * log-spaced bins from bin_begin; pdf (may be NULL) is normalised to unit integral
+ * over the in-range intervals, counts (may be NULL) are the raw bin counts, outside
+ * (may be NULL) the intervals outside [bin_begin, bin_begin 10^num_decades]. */
+int qjx_jump_interval_pdf(qjx_ctx *ctx, double bin_begin, int num_decades, int bins_per_decade,
+                          double *pdf, int64_t *counts, int64_t *outside);
+
 /* Propagator of branch b, level lvl as dense d x d complex (FP64 copy of the
  * operand the GEMM uses); set replaces it (e.g. to share one tree between runs). */
 int qjx_get_propagator(qjx_ctx *ctx, int b, int lvl, double *U);

@@ -892,6 +892,44 @@ extern "C" int qjx_get_jumps(qjx_ctx *c, int tr, double *times, int *types, doub
     return n;
 }
 
+// Waiting-time distribution of the logged jumps, binned as the reference's
+// post-processing does (plot/mpipks/qjx/dimer/times_between_jumps.m:41-131): log-spaced
+// bins [b0 10^((i-1)/n), b0 10^(i/n)), bin of x = floor((log10 x - log10 b0) nb /
+// (log10 b_end - log10 b0 + eps)) + 1, x outside [b0, b_end] counted apart, the pdf
+// normalised by the in-range count and the bin widths.
+extern "C" int qjx_jump_interval_pdf(qjx_ctx *c, double bin_begin, int num_decades, int bins_per_decade,
+                                     double *pdf, int64_t *counts, int64_t *outside)
+{
+    if (!c || !c->have_ens) return fail(QJX_ERR_ARG, "ensemble not initialised");
+    if (!(bin_begin > 0.0) || num_decades < 1 || bins_per_decade < 1)
+        return fail(QJX_ERR_ARG, "bin_begin must be > 0, num_decades and bins_per_decade >= 1");
+    const int nb = num_decades * bins_per_decade;
+    const double bin_end = bin_begin * std::pow(10.0, num_decades);
+    const double eps = 2.220446049250313e-16;   // MATLAB eps
+    const double lb = std::log10(bin_begin), span = std::log10(bin_end) - lb + eps;
+    std::vector<int64_t> cnt(nb, 0);
+    int64_t out = 0, total = 0;
+    for (const auto &jt : c->jt)
+        for (size_t i = 1; i < jt.size(); i++) {
+            const double x = jt[i] - jt[i - 1];
+            if (x >= bin_begin && x <= bin_end) {
+                const int b = (int)std::floor((std::log10(x) - lb) * nb / span);
+                cnt[std::min(std::max(b, 0), nb - 1)]++;
+                total++;
+            } else {
+                out++;
+            }
+        }
+    for (int i = 0; i < nb; i++) {
+        const double lo = bin_begin * std::pow(10.0, double(i) / bins_per_decade);
+        const double hi = bin_begin * std::pow(10.0, double(i + 1) / bins_per_decade);
+        if (pdf) pdf[i] = total ? double(cnt[i]) / (double(total) * (hi - lo)) : 0.0;
+        if (counts) counts[i] = cnt[i];
+    }
+    if (outside) *outside = out;
+    return QJX_OK;
+}
+
 extern "C" int qjx_get_propagator(qjx_ctx *c, int b, int lvl, double *U)
 {
     if (c && c->prop == 1) return fail(QJX_ERR_ARG, "a Runge-Kutta context has no propagators");

@@ -91,6 +91,8 @@ SIGNATURES = [
     ("qjx_set_state", ctypes.c_int, [ctypes.c_void_p, _D, _D, _D, _P(ctypes.c_uint64)]),
     ("qjx_get_jumps", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, _P(ctypes.c_int), _D, _D,
                                      ctypes.c_int]),
+    ("qjx_jump_interval_pdf", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int, _D,
+                                             _P(ctypes.c_int64), _P(ctypes.c_int64)]),
     ("qjx_get_propagator", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D]),
     ("qjx_set_propagator", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D]),
     ("qjx_get_stats", ctypes.c_int, [ctypes.c_void_p, _P(Stats)]),
@@ -261,6 +263,18 @@ class Engine:
     def set_propagator(self, b, lvl, U):
         U = np.ascontiguousarray(U, dtype=np.complex128)
         _check(self.lib, self.lib.qjx_set_propagator(self.h, b, lvl, _ptr(U.view(np.float64))))
+
+    def jump_interval_pdf(self, bin_begin=1e-10, num_decades=15, bins_per_decade=10):
+        """Waiting-time pdf of the logged jumps (times_between_jumps.m binning):
+        returns (bin borders, pdf, counts, outside)."""
+        nb = num_decades * bins_per_decade
+        pdf = np.zeros(nb)
+        cnt = np.zeros(nb, dtype=np.int64)
+        out = np.zeros(1, dtype=np.int64)
+        _check(self.lib, self.lib.qjx_jump_interval_pdf(self.h, bin_begin, num_decades, bins_per_decade, _ptr(pdf),
+                                                        _ptr(cnt, ctypes.c_int64), _ptr(out, ctypes.c_int64)))
+        borders = bin_begin * 10.0 ** (np.arange(nb + 1) / bins_per_decade)
+        return borders, pdf, cnt, int(out[0])
 
     def stats(self):
         s = Stats()

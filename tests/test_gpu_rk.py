@@ -152,3 +152,24 @@ def test_rk_bench_size_invariants():
     st = e.stats()
     assert st["jumps"] == nj.sum()
     e.close()
+
+
+def test_jump_interval_pdf_matches_histogram():
+    """qjx_jump_interval_pdf bins the waiting times as times_between_jumps.m:41-131;
+    the counts equal numpy's histogram over the same log-spaced borders and the pdf
+    integrates to one."""
+    from os_d_amd import qjx
+    _say("case: jump interval pdf")
+    e = qjx.Engine(30, drv_ampl=1.5, diss_gamma=0.3, prop="rk", rk_ns=2000, drv_type=1, **P)
+    e.ensemble_init(64, seed=7, jump=1)
+    e.trans(0)
+    e.run_periods(2, 1)
+    borders, pdf, cnt, outside = e.jump_interval_pdf(1e-10, 15, 10)
+    diffs = np.concatenate([np.diff(e.get_jumps(tr)["times"]) for tr in range(64)])
+    inside = diffs[(diffs >= borders[0]) & (diffs <= borders[-1])]
+    ref, _ = np.histogram(inside, bins=borders)
+    assert cnt.sum() > 100
+    assert np.array_equal(cnt, ref)
+    assert outside == len(diffs) - len(inside)
+    assert abs(np.sum(pdf * np.diff(borders)) - 1.0) < 1e-12
+    e.close()

@@ -189,3 +189,33 @@ def test_run_config_deep_outputs_match_oracle(tmp_path, oracle_mod, prop_id):
         jt = np.atleast_1d(np.loadtxt(f)) if os.path.getsize(f) else np.zeros(0)
         np.testing.assert_allclose(jt, o.get_jumps(tr)["times"], rtol=1e-15 if prop_id else 1e-12)
     assert not glob.glob(out + "phi_*")
+
+
+def test_run_config_jumps_counts_stop(tmp_path, oracle_mod):
+    """jumps_counts L > 0 (obser_process, experiment.cpp:481-499): periods run one at a
+    time and the observation stops at the first period after which every trajectory
+    has logged at least L jumps; the dumps after it stay zero."""
+    from os_d_amd import qjx
+    L = 3
+    print("case: run_config jumps_counts=%d" % L, flush=True)
+    out = str(tmp_path) + "/"
+    cfg = CONFIG.format(path=out).replace("num_obs_periods 6", "num_obs_periods 40")
+    prm = PARAMS.replace("jumps_counts 0", "jumps_counts %d" % L).replace("dump_num 3", "dump_num 40")
+    (tmp_path / "config.txt").write_text(cfg)
+    (tmp_path / "params.txt").write_text(prm)
+    qjx.run_config(str(tmp_path / "config.txt"), str(tmp_path / "params.txt"), qjx.FP64, 0)
+    m = oracle_mod.model(12, gamma=0.2, S=4, D=12)
+    o = oracle_mod.Ensemble(m, 24, jump=2)
+    o.trans(2)
+    o.chars(is_start=1)
+    p_stop = 0
+    while p_stop < 40:
+        o.periods(1, 1)
+        p_stop += 1
+        if (o.get_state()["njumps"] >= L).all():
+            break
+    assert p_stop < 40
+    me = np.loadtxt(out + "mean_evo" + SUFFIX)
+    np.testing.assert_allclose(me[p_stop], o.chars()["mean"], rtol=1e-9)
+    assert (me[p_stop + 1:] == 0).all()
+    np.testing.assert_allclose(np.loadtxt(out + "mean" + SUFFIX), o.chars()["mean"], rtol=1e-9)
