@@ -231,6 +231,55 @@ int qjx_lindblad_observables(qjx_lindblad *lb, double *mean, double *m2, double 
                              double *purity, double *adr);
 int qjx_lindblad_dim(const qjx_lindblad *lb);
 
+/* ------------------------------------------------------------------------ */
+/* LPN task (task_id 7): growth rates of observable distances between paired  */
+/* trajectories.  Replaces LpnMultNewDelBehaviour / LpnMultExperimentBehaviour */
+/* (newdel_exp.cpp:29-48, experiment.cpp:137-286) with lambda_lpn(_now),       */
+/* var_trajectory_lpn, calc_chars_lpn, calc_delta_s/f (experiment.cpp:2221-2393,*/
+/* 3065-3134; core.cpp:727-915) and init_random_obs / get_random_obs          */
+/* (newdel.cpp:3652-3722, experiment.cpp:2023-2078).  Works on a context of    */
+/* either propagation (QJ or RK) with an initialised ensemble of M (even)      */
+/* trajectories: tr < M/2 are the originals, tr + M/2 their partners.          */
+/* ------------------------------------------------------------------------ */
+typedef struct qjx_lpn_desc {
+    int lpn_type;           /* lpn_type: -1 |<k> - <k>'| / d, -2 |E - E'| / max|H_ii|,
+                               >= 0 |o - o'| of random observable lpn_type          */
+    int lpn_eps_deep;       /* lpn_eps_deep: bisection steps of the perturbation size */
+    double lpn_eps_error;   /* lpn_eps_error                                        */
+    double lpn_eps_high;    /* lpn_eps_high (log10 of the largest perturbation)     */
+    double lpn_eps_low;     /* lpn_eps_low                                          */
+    double lpn_delta_s;     /* lpn_delta_s: target distance after a renormalisation */
+    double lpn_delta_f_high;/* lpn_delta_f_high                                     */
+    double lpn_delta_f_low; /* lpn_delta_f_low                                      */
+    int lambda_per_periods; /* lambda_per_periods: 0 renormalise on the thresholds, else every period */
+    int save_lambdas;       /* save_lambdas: keep every renormalisation's ratio and distances */
+    int num_random_obs;     /* num_random_obs                                       */
+    int random_obs_seed;    /* random_obs_seed                                      */
+    int random_obs_mns;     /* random_obs_mns                                       */
+    int random_obs_type;    /* random_obs_type: 0 raw X X^dag, 1 / Tr, 2 / (Tr / d)  */
+} qjx_lpn_desc;
+typedef struct qjx_lpn qjx_lpn;
+int qjx_lpn_create(qjx_ctx *ctx, const qjx_lpn_desc *desc, qjx_lpn **out);
+void qjx_lpn_destroy(qjx_lpn *lpn);
+/* trans_process: the transient, then every partner copies its original (state,
+ * RNG stream, eta) and is perturbed to distance lpn_delta_s; times restart at 0. */
+int qjx_lpn_trans(qjx_lpn *lpn, int num_tp_periods);
+/* n observation periods (logging jumps as qjx_run_periods with is_obs = 1). */
+int qjx_lpn_run_periods(qjx_lpn *lpn, int n);
+/* Per trajectory (arrays of M, any may be NULL): lambda_now (the reference's
+ * "lambda" output), the accumulated log growth, renormalisations, the lpn mean and
+ * energy, the distance after the last renormalisation. */
+int qjx_lpn_get(qjx_lpn *lpn, double *lambda_now, double *lambda, int *num_renorms, double *mean_lpn,
+                double *energy_lpn, double *delta_s);
+/* Random observables (M x num_random_obs complex, interleaved) of the last period
+ * (calc_chars_std) and the lpn copies (calc_chars_lpn); either may be NULL. */
+int qjx_lpn_random_obs(qjx_lpn *lpn, double *obs, double *obs_lpn);
+/* Saved renormalisations of trajectory tr (save_lambdas > 0): delta_f / delta_s,
+ * delta_s, delta_f; copies at most cap, returns the count. */
+int qjx_lpn_lambdas(qjx_lpn *lpn, int tr, double *ratio, double *ds, double *df, int cap);
+/* Random observable obs as d x d complex (interleaved, row-major). */
+int qjx_lpn_random_obs_matrix(qjx_lpn *lpn, int obs, double *R);
+
 typedef struct qjx_pool qjx_pool;
 
 typedef struct qjx_job_result {

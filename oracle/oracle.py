@@ -28,6 +28,22 @@ def build():
     return LIB
 
 
+class LpnDesc(ctypes.Structure):
+    _fields_ = [("lpn_type", ctypes.c_int), ("lpn_eps_deep", ctypes.c_int), ("lpn_eps_error", ctypes.c_double),
+                ("lpn_eps_high", ctypes.c_double), ("lpn_eps_low", ctypes.c_double),
+                ("lpn_delta_s", ctypes.c_double), ("lpn_delta_f_high", ctypes.c_double),
+                ("lpn_delta_f_low", ctypes.c_double), ("lambda_per_periods", ctypes.c_int),
+                ("save_lambdas", ctypes.c_int), ("num_random_obs", ctypes.c_int),
+                ("random_obs_seed", ctypes.c_int), ("random_obs_mns", ctypes.c_int),
+                ("random_obs_type", ctypes.c_int)]
+
+
+LPN_DEFAULTS = dict(lpn_type=-1, lpn_eps_deep=100, lpn_eps_error=1e-10, lpn_eps_high=100.0, lpn_eps_low=-100.0,
+                    lpn_delta_s=1e-2, lpn_delta_f_high=1e-2, lpn_delta_f_low=1e-2, lambda_per_periods=1,
+                    save_lambdas=0, num_random_obs=0, random_obs_seed=1, random_obs_mns=1000000,
+                    random_obs_type=2)
+
+
 class Model(ctypes.Structure):
     _fields_ = [("N", ctypes.c_int), ("prm_E", ctypes.c_double), ("prm_U", ctypes.c_double),
                 ("prm_J", ctypes.c_double), ("drv_ampl", ctypes.c_double),
@@ -78,6 +94,16 @@ def lib():
         L.qjo_rk_norm.restype = ctypes.c_double
         L.qjo_rk_norm.argtypes = [ctypes.c_int, _D]
         L.qjo_rk_int.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+        L.qjo_mcg31_uniform.argtypes = [ctypes.c_uint, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                        ctypes.c_double, _D]
+        L.qjo_mcg31_gaussian.argtypes = [ctypes.c_uint, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D]
+        L.qjo_random_obs_matrix.argtypes = [ctypes.c_int] * 5 + [_D]
+        L.qjo_lpn_setup.argtypes = [ctypes.c_void_p, _P(LpnDesc)]
+        L.qjo_lpn_trans.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.qjo_lpn_period.argtypes = [ctypes.c_void_p]
+        L.qjo_lpn_get.argtypes = [ctypes.c_void_p, _D, _D, _P(ctypes.c_int), _D, _D, _D, _D, _D]
+        L.qjo_lpn_random_obs.argtypes = [ctypes.c_void_p, _D, _D]
+        L.qjo_lpn_lambdas.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, ctypes.c_int]
         L.qjl_evolve.argtypes = [_P(Model), ctypes.c_int, ctypes.c_int, _D]
         L.qjl_rhs.argtypes = [_P(Model), ctypes.c_double, _D, _D]
         _lib = L
@@ -132,6 +158,24 @@ def lindblad_observables(m, rho):
     Hs = mm["H"] + m.prm_E * mm["Hdrv"]
     return {"mean": mean, "m2": float(p @ (k - mean) ** 2), "energy": float(np.real(np.trace(rho @ Hs))),
             "purity": float(np.real(np.trace(rho @ rho))), "adr": p}
+
+
+def mcg31_uniform(seed, k, n, count, lo=0.0, hi=1.0):
+    out = np.zeros(count)
+    lib().qjo_mcg31_uniform(seed, k, n, count, lo, hi, _p(out))
+    return out
+
+
+def mcg31_gaussian(seed, k, n, count):
+    out = np.zeros(count)
+    lib().qjo_mcg31_gaussian(seed, k, n, count, _p(out))
+    return out
+
+
+def random_obs_matrix(d, seed, mns, typ, obs):
+    R = np.zeros((d, d), dtype=np.complex128)
+    lib().qjo_random_obs_matrix(d, seed, mns, typ, obs, _p(R.view(np.float64)))
+    return R
 
 
 def rk_sin(x):
@@ -243,6 +287,41 @@ class Ensemble:
 
     def rk_int(self, tr, step):
         self._ok(self.L.qjo_rk_int(self.h, tr, step))
+
+    def lpn_setup(self, **kw):
+        """LPN bookkeeping (task_id 7); keywords are the params.txt keys."""
+        p = dict(LPN_DEFAULTS, **kw)
+        self.lpn_nobs = p["num_random_obs"]
+        self._ok(self.L.qjo_lpn_setup(self.h, ctypes.byref(LpnDesc(**p))))
+
+    def lpn_trans(self, n):
+        self._ok(self.L.qjo_lpn_trans(self.h, n))
+
+    def lpn_period(self):
+        self._ok(self.L.qjo_lpn_period(self.h))
+
+    def lpn_get(self):
+        M = self.M
+        out = {k: np.zeros(M) for k in ("lambda_now", "lambda", "mean_lpn", "energy_lpn", "delta_s", "mean",
+                                        "energy")}
+        nr = np.zeros(M, dtype=np.int32)
+        self._ok(self.L.qjo_lpn_get(self.h, _p(out["lambda_now"]), _p(out["lambda"]), _p(nr, ctypes.c_int),
+                                    _p(out["mean_lpn"]), _p(out["energy_lpn"]), _p(out["delta_s"]),
+                                    _p(out["mean"]), _p(out["energy"])))
+        out["num_renorms"] = nr
+        return out
+
+    def lpn_random_obs(self):
+        o = np.zeros((self.M, max(self.lpn_nobs, 1)), dtype=np.complex128)
+        ol = np.zeros_like(o)
+        self._ok(self.L.qjo_lpn_random_obs(self.h, _p(o.view(np.float64)), _p(ol.view(np.float64))))
+        return o[:, :self.lpn_nobs], ol[:, :self.lpn_nobs]
+
+    def lpn_lambdas(self, tr):
+        n = self.L.qjo_lpn_lambdas(self.h, tr, None, None, None, 0)
+        r, ds, df = np.zeros(n), np.zeros(n), np.zeros(n)
+        self.L.qjo_lpn_lambdas(self.h, tr, _p(r), _p(ds), _p(df), n)
+        return r, ds, df
 
     def get_jumps(self, tr):
         n = self.L.qjo_get_jumps(self.h, tr, None, None, None, None, 0)

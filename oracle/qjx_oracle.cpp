@@ -308,6 +308,8 @@ struct qjo_ens {
     int prop = 0, rk_ns = 0, drv_type = 0, lanes_E = 2;
     double drv_phase = 0.0, rk_step = 0.0;
     std::vector<double> b_dre, b_dim, b_o1re, b_o1im, b_o2im, b_drv, b_Ld, b_Lm, b_Lp;
+    struct Lpn *lpn = nullptr;              /* LPN bookkeeping, set by qjo_lpn_setup */
+    ~qjo_ens();
     std::string err;
     volatile int failed = 0;
 };
@@ -1032,3 +1034,453 @@ extern "C" int64_t qjo_num_jumps(const qjo_ens *e)
     for (auto v : e->jumps_done) s += v;
     return s;
 }
+
+/* ------------------------------------------------------------------------- */
+/* LPN (task_id 7, LpnMultExperimentBehaviour, experiment.cpp:137-286): the   */
+/* first M/2 trajectories are the originals, trajectory tr + M/2 is a copy of */
+/* tr (state, RNG stream, eta) perturbed to a chosen observable distance, and */
+/* the logarithm of the distance growth accumulates into lambda.  Random      */
+/* observables: init_random_obs / get_random_obs (newdel.cpp:3652-3722,       */
+/* experiment.cpp:2023-2078).                                                  */
+/* ------------------------------------------------------------------------- */
+
+/* MKL VSL MCG31m1: x <- a x mod (2^31 - 1), a = 1132489760; the [lo, hi) uniform
+ * is fma(double(x), (hi - lo) (1 / m), lo) (pinned by tests/golden/mcg31_mkl.json). */
+static const uint64_t kMcg31A = 1132489760ULL, kMcg31M = 2147483647ULL;
+
+struct Mcg31 {
+    uint64_t x = 1, stride = 1;
+    double next(double lo, double hi)
+    {
+        const double r = std::fma(double(x), (hi - lo) * (1.0 / double(kMcg31M)), lo);
+        x = x * stride % kMcg31M;
+        return r;
+    }
+};
+
+static uint64_t mcg31_pow(uint64_t a, uint64_t e)
+{
+    uint64_t r = 1;
+    a %= kMcg31M;
+    while (e) {
+        if (e & 1ULL) r = r * a % kMcg31M;
+        a = a * a % kMcg31M;
+        e >>= 1;
+    }
+    return r;
+}
+
+/* vslNewStream(MCG31, seed) + vslLeapfrogStream(k, n) */
+static Mcg31 mcg31_stream(uint64_t seed, int k, int n)
+{
+    Mcg31 s;
+    uint64_t x0 = seed % kMcg31M;
+    if (x0 == 0) x0 = 1;
+    s.x = x0 * mcg31_pow(kMcg31A, (uint64_t)k) % kMcg31M;
+    s.stride = mcg31_pow(kMcg31A, (uint64_t)n);
+    return s;
+}
+
+/* VSL_RNG_METHOD_GAUSSIAN_BOXMULLER: x = sqrt(-2 ln u1) sin(2 pi u2) from two
+ * consecutive uniforms (MKL's own evaluation agrees to 2e-8 relative). */
+static double box_muller(Mcg31 &s)
+{
+    const double u1 = s.next(0.0, 1.0);
+    const double u2 = s.next(0.0, 1.0);
+    return std::sqrt(-2.0 * std::log(u1)) * std::sin(2.0 * kPi * u2);
+}
+
+extern "C" void qjo_mcg31_uniform(unsigned seed, int k, int n, int count, double lo, double hi, double *out)
+{
+    Mcg31 s = mcg31_stream(seed, k, n);
+    for (int i = 0; i < count; i++) out[i] = s.next(lo, hi);
+}
+
+extern "C" void qjo_mcg31_gaussian(unsigned seed, int k, int n, int count, double *out)
+{
+    Mcg31 s = mcg31_stream(seed, k, n);
+    for (int i = 0; i < count; i++) out[i] = box_muller(s);
+}
+
+/* init_random_obs: X = (G_re + i G_im) / 2 with d^2 + d^2 Gaussians of stream
+ * leapfrog(seed + obs, mns) of MCG31(3873247); R = X X^dag (ascending k); type > 0
+ * divides by Tr R, type 2 by Tr R / d. */
+static void random_obs_matrix(int d, int seed, int mns, int type, int obs, std::vector<cplx> &R)
+{
+    const size_t dd = (size_t)d * d;
+    Mcg31 s = mcg31_stream(3873247ULL, seed + obs, mns);
+    std::vector<double> gr(dd), gi(dd);
+    for (size_t i = 0; i < dd; i++) gr[i] = box_muller(s);
+    for (size_t i = 0; i < dd; i++) gi[i] = box_muller(s);
+    std::vector<double> xr(dd), xi(dd);
+    for (size_t i = 0; i < dd; i++) {
+        xr[i] = 0.5 * gr[i];
+        xi[i] = 0.5 * gi[i];
+    }
+    R.assign(dd, 0.0);
+    for (int i = 0; i < d; i++)
+        for (int j = 0; j < d; j++) {
+            double re = 0.0, im = 0.0;
+            for (int k = 0; k < d; k++) {
+                const double ar = xr[(size_t)i * d + k], ai = xi[(size_t)i * d + k];
+                const double br = xr[(size_t)j * d + k], bi = -xi[(size_t)j * d + k];
+                re = re + (ar * br - ai * bi);
+                im = im + (ar * bi + ai * br);
+            }
+            R[(size_t)i * d + j] = cplx(re, im);
+        }
+    if (type > 0) {
+        double tr = 0.0;
+        for (int i = 0; i < d; i++) tr += R[(size_t)i * d + i].real();
+        if (type == 2) tr = tr / double(d);
+        for (size_t i = 0; i < dd; i++) R[i] = cplx(R[i].real() / tr, R[i].imag() / tr);
+    }
+}
+
+extern "C" int qjo_random_obs_matrix(int d, int seed, int mns, int type, int obs, double *out)
+{
+    std::vector<cplx> R;
+    random_obs_matrix(d, seed, mns, type, obs, R);
+    std::memcpy(out, R.data(), sizeof(cplx) * R.size());
+    return 0;
+}
+
+/* get_random_obs: <phi|R|phi> / ||phi||^2 as the reference evaluates it. */
+static cplx random_obs_value(const cplx *phi, const cplx *R, int d, std::vector<cplx> &tmp)
+{
+    const double n = std::sqrt(norm_square(phi, d));
+    tmp.resize(2 * (size_t)d);
+    cplx *pn = tmp.data(), *mult = tmp.data() + d;
+    for (int i = 0; i < d; i++) pn[i] = cplx(phi[i].real() / n, phi[i].imag() / n);
+    for (int i = 0; i < d; i++) {
+        double re = 0.0, im = 0.0;
+        for (int j = 0; j < d; j++) {
+            const cplx &a = R[(size_t)i * d + j];
+            re = re + (a.real() * pn[j].real() - a.imag() * pn[j].imag());
+            im = im + (a.real() * pn[j].imag() + a.imag() * pn[j].real());
+        }
+        mult[i] = cplx(re, im);
+    }
+    double rr = 0.0, ri = 0.0;
+    for (int i = 0; i < d; i++) {
+        const double cr = pn[i].real(), ci = -pn[i].imag();
+        rr = rr + (cr * mult[i].real() - ci * mult[i].imag());
+        ri = ri + (ci * mult[i].real() + cr * mult[i].imag());
+    }
+    return cplx(rr, ri);
+}
+
+/* mean and energy of one state as calc_chars_std computes them (the loops of qjo_chars) */
+static void mean_energy(const qjo_ens *e, const cplx *phi, double *mean, double *energy)
+{
+    const int d = e->m.d;
+    const double E0 = e->p.prm_E;
+    const double nrm = norm_square(phi, d);
+    double mn = 0.0;
+    for (int k = 0; k < d; k++)
+        mn += double(k) * ((phi[k].real() * phi[k].real() + phi[k].imag() * phi[k].imag()) * (1.0 / nrm));
+    const double sn = std::sqrt(nrm);
+    double en = 0.0;
+    for (int i = 0; i < d; i++) {
+        double sre = 0.0, sim = 0.0;
+        for (int j = 0; j < d; j++) {
+            const size_t idx = (size_t)i * d + j;
+            const double hv = e->m.H[idx].real() + E0 * e->m.Hdrv[idx].real();
+            sre += hv * phi[j].real() / sn - 0.0 * phi[j].imag() / sn;
+            sim += hv * phi[j].imag() / sn + 0.0 * phi[j].real() / sn;
+        }
+        en += phi[i].real() / sn * sre + phi[i].imag() / sn * sim;
+    }
+    *mean = mn;
+    *energy = en;
+}
+
+struct Lpn {
+    qjo_lpn_desc p{};
+    int M = 0, half = 0, period = 0;
+    double max_energy = 0.0, T = 0.0;
+    std::vector<std::vector<cplx>> R;               /* random observables */
+    std::vector<Mcg31> var;                          /* streams_var */
+    std::vector<double> mean, energy, mean_lpn, energy_lpn, lambda, lambda_now, delta_s;
+    std::vector<int> num_renorms;
+    std::vector<std::vector<cplx>> obs, obs_lpn;     /* [tr][obs] */
+    std::vector<std::vector<double>> lambdas, deltas_s, deltas_f;
+};
+
+static Lpn *lpn_of(qjo_ens *e);
+
+/* calc_chars_std's mean, energy and random observables of trajectory tr */
+static void lpn_chars_std(qjo_ens *e, Lpn &L, int tr)
+{
+    const int d = e->m.d;
+    const cplx *phi = &e->psi[(size_t)tr * d];
+    mean_energy(e, phi, &L.mean[tr], &L.energy[tr]);
+    std::vector<cplx> tmp;
+    for (size_t o = 0; o < L.R.size(); o++) L.obs[tr][o] = random_obs_value(phi, L.R[o].data(), d, tmp);
+}
+
+/* calc_chars_lpn (core.cpp:761-792) */
+static void lpn_chars_lpn(qjo_ens *e, Lpn &L, int tr)
+{
+    const int d = e->m.d;
+    const cplx *phi = &e->psi[(size_t)tr * d];
+    mean_energy(e, phi, &L.mean_lpn[tr], &L.energy_lpn[tr]);
+    std::vector<cplx> tmp;
+    for (size_t o = 0; o < L.R.size(); o++) L.obs_lpn[tr][o] = random_obs_value(phi, L.R[o].data(), d, tmp);
+}
+
+/* calc_delta_s (from the lpn values) and calc_delta_f (from the std values),
+ * core.cpp:839-915 */
+static double lpn_delta(const qjo_ens *e, const Lpn &L, int tr, int base, bool from_lpn)
+{
+    const int d = e->m.d;
+    const std::vector<double> &mn = from_lpn ? L.mean_lpn : L.mean;
+    const std::vector<double> &en = from_lpn ? L.energy_lpn : L.energy;
+    const std::vector<std::vector<cplx>> &ob = from_lpn ? L.obs_lpn : L.obs;
+    if (L.p.lpn_type == -1) return std::fabs(mn[tr] - mn[base]) / double(d);
+    if (L.p.lpn_type == -2) return std::fabs(en[tr] - en[base]) / L.max_energy;
+    const cplx b = ob[base][L.p.lpn_type], v = ob[tr][L.p.lpn_type];
+    const double tmp = std::pow(b.real() - v.real(), 2) + std::pow(b.imag() - v.imag(), 2);
+    return std::sqrt(tmp);
+}
+
+/* var_trajectory_lpn (experiment.cpp:2278-2393): phi = base + eps var, renormalised
+ * to the base norm, eps bisected in log10 between lpn_eps_low and lpn_eps_high
+ * until |delta_s - lpn_delta_s| < lpn_eps_error or lpn_eps_deep halvings. */
+static void lpn_var(qjo_ens *e, Lpn &L, int tr, int base)
+{
+    const int d = e->m.d;
+    double lo = L.p.lpn_eps_low, hi = L.p.lpn_eps_high, diff = hi - lo;
+    int deep = 0;
+    double eps = std::pow(10.0, (lo + hi) * 0.5);
+    cplx *phi = &e->psi[(size_t)tr * d];
+    const cplx *orig = &e->psi[(size_t)base * d];
+    std::vector<double> v(2 * (size_t)d);
+    for (int i = 0; i < 2 * d; i++) v[i] = L.var[tr].next(-1.0, 1.0);
+    std::vector<cplx> var(d);
+    for (int i = 0; i < d; i++) var[i] = cplx(v[i], v[d + i]);
+    const double nv2 = norm_square(var.data(), d);
+    for (int i = 0; i < d; i++) var[i] = cplx(var[i].real() / std::sqrt(nv2), var[i].imag() / std::sqrt(nv2));
+    const double n2 = norm_square(orig, d);
+    while (deep < L.p.lpn_eps_deep) {
+        for (int i = 0; i < d; i++)
+            phi[i] = cplx(orig[i].real() + eps * var[i].real(), orig[i].imag() + eps * var[i].imag());
+        const double nm2 = norm_square(phi, d);
+        for (int i = 0; i < d; i++) phi[i] = cplx(phi[i].real() / std::sqrt(nm2), phi[i].imag() / std::sqrt(nm2));
+        for (int i = 0; i < d; i++) phi[i] = cplx(phi[i].real() * std::sqrt(n2), phi[i].imag() * std::sqrt(n2));
+        lpn_chars_lpn(e, L, tr);
+        const double ds = lpn_delta(e, L, tr, base, true);
+        if (std::fabs(ds - L.p.lpn_delta_s) < L.p.lpn_eps_error) break;
+        if (ds > L.p.lpn_delta_s) {
+            hi -= 0.5 * diff;
+            diff = hi - lo;
+            deep++;
+            eps = std::pow(10.0, (lo + hi) * 0.5);
+        }
+        if (ds < L.p.lpn_delta_s) {
+            lo += 0.5 * diff;
+            diff = hi - lo;
+            deep++;
+            eps = std::pow(10.0, (lo + hi) * 0.5);
+        }
+        if (!(ds > L.p.lpn_delta_s) && !(ds < L.p.lpn_delta_s)) break;   /* equal or NaN */
+    }
+}
+
+/* copy_trajectory_lpn / copy_stream_lpn (experiment.cpp:2221-2256) */
+static void lpn_copy(qjo_ens *e, int tr, int base, bool state)
+{
+    const int d = e->m.d;
+    e->rng[tr] = e->rng[base];
+    e->eta[tr] = e->eta[base];
+    if (state) std::memcpy(&e->psi[(size_t)tr * d], &e->psi[(size_t)base * d], sizeof(cplx) * d);
+}
+
+/* calc_chars_lpn_start (core.cpp:727-759) */
+static void lpn_start(qjo_ens *e, Lpn &L, int tr, int base)
+{
+    L.lambda[tr] = 0.0;
+    L.lambda_now[tr] = 0.0;
+    L.mean_lpn[tr] = L.mean[tr];
+    L.energy_lpn[tr] = L.energy[tr];
+    L.delta_s[tr] = lpn_delta(e, L, tr, base, false);
+    L.obs_lpn[tr] = L.obs[tr];
+}
+
+extern "C" int qjo_lpn_setup(qjo_ens *e, const qjo_lpn_desc *desc)
+{
+    if (!e || !desc || e->M < 2 || (e->M % 2)) return -1;
+    if (desc->lpn_type < -2 || desc->lpn_type >= desc->num_random_obs) {
+        set_err(e, "wrong lpn_type");
+        return -1;
+    }
+    if (!(desc->lpn_eps_error > 0.0)) {
+        set_err(e, "lpn_eps_error must be > 0");
+        return -1;
+    }
+    delete e->lpn;
+    Lpn *L = new Lpn;
+    e->lpn = L;
+    L->p = *desc;
+    L->M = e->M;
+    L->half = e->M / 2;
+    L->T = e->m.T;
+    const int d = e->m.d, M = e->M;
+    for (int i = 0; i < d; i++)                              /* init_obs_lpn, newdel_exp.cpp:540-550 */
+        L->max_energy = std::max(L->max_energy, std::fabs(e->m.H[(size_t)i * d + i].real()));
+    L->R.resize(desc->num_random_obs);
+    for (int o = 0; o < desc->num_random_obs; o++)
+        random_obs_matrix(d, desc->random_obs_seed, desc->random_obs_mns, desc->random_obs_type, o, L->R[o]);
+    L->var.resize(M);
+    for (int tr = 0; tr < M; tr++) L->var[tr] = mcg31_stream(777, tr, M);   /* init_streams_var */
+    L->mean.assign(M, 0.0);
+    L->energy.assign(M, 0.0);
+    L->mean_lpn.assign(M, 0.0);
+    L->energy_lpn.assign(M, 0.0);
+    L->lambda.assign(M, 0.0);
+    L->lambda_now.assign(M, 0.0);
+    L->delta_s.assign(M, 0.0);
+    L->num_renorms.assign(M, 0);
+    L->obs.assign(M, std::vector<cplx>(desc->num_random_obs));
+    L->obs_lpn.assign(M, std::vector<cplx>(desc->num_random_obs));
+    L->lambdas.assign(M, {});
+    L->deltas_s.assign(M, {});
+    L->deltas_f.assign(M, {});
+    return 0;
+}
+
+static Lpn *lpn_of(qjo_ens *e) { return e ? e->lpn : nullptr; }
+
+/* LpnMultExperimentBehaviour::trans_process (experiment.cpp:137-190) */
+extern "C" int qjo_lpn_trans(qjo_ens *e, int num_tp_periods)
+{
+    Lpn *L = lpn_of(e);
+    if (!L) return -1;
+    const int half = L->half, M = L->M;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(e->threads)
+    for (int tr = 0; tr < half; tr++) {
+        double &eta = e->eta[tr];
+        eta = 0.0;
+        while (eta == 0.0) eta = e->rng[tr].next();
+        if (e->prop == 1) {
+            e->pos[tr] = 0;
+            rk_steps(e, tr, num_tp_periods * e->rk_ns, 0);
+            e->pos[tr] = 0;
+        } else {
+            for (int p = 0; p < num_tp_periods && !e->failed; p++) one_period(e, tr, 0);
+        }
+    }
+    if (e->failed) return -1;
+    for (int tr = 0; tr < half; tr++) {
+        lpn_chars_std(e, *L, tr);
+        lpn_start(e, *L, tr, tr);
+    }
+    for (int tr = 0; tr < M; tr++) {
+        const int base = tr >= half ? tr - half : tr;
+        if (tr >= half) {
+            lpn_copy(e, tr, base, true);
+            e->pos[tr] = e->pos[base];
+            lpn_var(e, *L, tr, base);
+        }
+        e->t[tr] = 0.0;                                     /* resresh_times */
+        lpn_chars_std(e, *L, tr);
+        lpn_start(e, *L, tr, base);
+    }
+    L->period = 0;
+    return 0;
+}
+
+/* one observation period of LpnMultExperimentBehaviour::obser_process
+ * (experiment.cpp:218-252) with lambda_lpn / lambda_lpn_now (experiment.cpp:3065-3134) */
+extern "C" int qjo_lpn_period(qjo_ens *e)
+{
+    Lpn *L = lpn_of(e);
+    if (!L) return -1;
+    const int half = L->half, M = L->M;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(e->threads)
+    for (int tr = 0; tr < M; tr++) {
+        if (e->prop == 1) rk_steps(e, tr, e->rk_ns, 1);
+        else one_period(e, tr, 1);
+    }
+    if (e->failed) return -1;
+    for (int tr = 0; tr < M; tr++) lpn_chars_std(e, *L, tr);
+    for (int tr = 0; tr < half; tr++) lpn_chars_lpn(e, *L, tr);
+    const double curr_time = double(L->period + 1) * L->T;
+    for (int tr = half; tr < M; tr++) {
+        const int base = tr - half;
+        const double df = lpn_delta(e, *L, tr, base, false);
+        const bool renorm = L->p.lambda_per_periods != 0 ||
+                            df > std::max(L->p.lpn_delta_f_high, L->delta_s[tr] * 10.0) || df < L->p.lpn_delta_f_low;
+        if (renorm) {
+            L->lambda[tr] += std::log(df / L->delta_s[tr] + 1.0e-16);
+            L->num_renorms[tr] += 1;
+            if (L->p.save_lambdas > 0) {
+                L->lambdas[tr].push_back(df / L->delta_s[tr]);
+                L->deltas_s[tr].push_back(L->delta_s[tr]);
+                L->deltas_f[tr].push_back(df);
+            }
+            lpn_copy(e, tr, base, false);
+            lpn_var(e, *L, tr, base);
+            lpn_chars_lpn(e, *L, tr);
+            L->delta_s[tr] = lpn_delta(e, *L, tr, base, true);
+            L->lambda_now[tr] = L->lambda[tr] / curr_time;
+        } else {
+            lpn_chars_lpn(e, *L, tr);
+            L->lambda_now[tr] = (L->lambda[tr] + std::log(df / L->delta_s[tr] + 1.0e-16)) / curr_time;
+        }
+    }
+    L->period++;
+    return 0;
+}
+
+extern "C" int qjo_lpn_get(qjo_ens *e, double *lambda_now, double *lambda, int *num_renorms, double *mean_lpn,
+                           double *energy_lpn, double *delta_s, double *mean, double *energy)
+{
+    Lpn *L = lpn_of(e);
+    if (!L) return -1;
+    for (int tr = 0; tr < L->M; tr++) {
+        if (lambda_now) lambda_now[tr] = L->lambda_now[tr];
+        if (lambda) lambda[tr] = L->lambda[tr];
+        if (num_renorms) num_renorms[tr] = L->num_renorms[tr];
+        if (mean_lpn) mean_lpn[tr] = L->mean_lpn[tr];
+        if (energy_lpn) energy_lpn[tr] = L->energy_lpn[tr];
+        if (delta_s) delta_s[tr] = L->delta_s[tr];
+        if (mean) mean[tr] = L->mean[tr];
+        if (energy) energy[tr] = L->energy[tr];
+    }
+    return 0;
+}
+
+extern "C" int qjo_lpn_random_obs(qjo_ens *e, double *obs, double *obs_lpn)
+{
+    Lpn *L = lpn_of(e);
+    if (!L) return -1;
+    const size_t no = L->R.size();
+    for (int tr = 0; tr < L->M; tr++)
+        for (size_t o = 0; o < no; o++) {
+            if (obs) {
+                obs[2 * (tr * no + o)] = L->obs[tr][o].real();
+                obs[2 * (tr * no + o) + 1] = L->obs[tr][o].imag();
+            }
+            if (obs_lpn) {
+                obs_lpn[2 * (tr * no + o)] = L->obs_lpn[tr][o].real();
+                obs_lpn[2 * (tr * no + o) + 1] = L->obs_lpn[tr][o].imag();
+            }
+        }
+    return 0;
+}
+
+extern "C" int qjo_lpn_lambdas(qjo_ens *e, int tr, double *ratio, double *ds, double *df, int cap)
+{
+    Lpn *L = lpn_of(e);
+    if (!L || tr < 0 || tr >= L->M) return -1;
+    const int n = (int)L->lambdas[tr].size();
+    for (int i = 0; i < n && i < cap; i++) {
+        if (ratio) ratio[i] = L->lambdas[tr][i];
+        if (ds) ds[i] = L->deltas_s[tr][i];
+        if (df) df[i] = L->deltas_f[tr][i];
+    }
+    return n;
+}
+
+qjo_ens::~qjo_ens() { delete lpn; }

@@ -114,6 +114,30 @@ double qjo_rk_norm(int d, const double *x);
 /* One RK4 integration of trajectory tr over `step` from its current time. */
 int qjo_rk_int(qjo_ens *e, int tr, double step);
 
+/* MKL VSL MCG31m1 stream leapfrog(k, n) of seed: count uniforms on [lo, hi), or
+ * count Box-Muller Gaussians (two uniforms each). */
+void qjo_mcg31_uniform(unsigned seed, int k, int n, int count, double lo, double hi, double *out);
+void qjo_mcg31_gaussian(unsigned seed, int k, int n, int count, double *out);
+/* Random observable obs (d x d complex) of init_random_obs. */
+int qjo_random_obs_matrix(int d, int seed, int mns, int type, int obs, double *out);
+
+/* LPN (task_id 7): params.txt keys of the same names. */
+typedef struct qjo_lpn_desc {
+    int lpn_type;             /* -1 <n2>/d, -2 energy / max|H_ii|, >= 0 random observable */
+    int lpn_eps_deep;
+    double lpn_eps_error, lpn_eps_high, lpn_eps_low, lpn_delta_s, lpn_delta_f_high, lpn_delta_f_low;
+    int lambda_per_periods;   /* 0: renormalise on thresholds, else every period */
+    int save_lambdas;
+    int num_random_obs, random_obs_seed, random_obs_mns, random_obs_type;
+} qjo_lpn_desc;
+int qjo_lpn_setup(qjo_ens *e, const qjo_lpn_desc *desc);
+int qjo_lpn_trans(qjo_ens *e, int num_tp_periods);
+int qjo_lpn_period(qjo_ens *e);
+int qjo_lpn_get(qjo_ens *e, double *lambda_now, double *lambda, int *num_renorms, double *mean_lpn,
+                double *energy_lpn, double *delta_s, double *mean, double *energy);
+int qjo_lpn_random_obs(qjo_ens *e, double *obs, double *obs_lpn);
+int qjo_lpn_lambdas(qjo_ens *e, int tr, double *ratio, double *ds, double *df, int cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,7 +19,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 LIB_SOURCES = ["qjx_kernels.hip", "qjx_step.hip", "qjx_lindblad.hip", "qjx_rk.hip", "qjx_engine.cpp",
-               "qjx_config.cpp", "qjx_pool.cpp"]
+               "qjx_config.cpp", "qjx_pool.cpp", "qjx_lpn.cpp"]
 
 
 def _run(cmd):

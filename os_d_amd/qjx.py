@@ -42,6 +42,16 @@ class RkDesc(ctypes.Structure):
     _fields_ = [("rk_ns", ctypes.c_int), ("drv_type", ctypes.c_int), ("drv_phase", ctypes.c_double)]
 
 
+class LpnDesc(ctypes.Structure):
+    _fields_ = [("lpn_type", ctypes.c_int), ("lpn_eps_deep", ctypes.c_int), ("lpn_eps_error", ctypes.c_double),
+                ("lpn_eps_high", ctypes.c_double), ("lpn_eps_low", ctypes.c_double),
+                ("lpn_delta_s", ctypes.c_double), ("lpn_delta_f_high", ctypes.c_double),
+                ("lpn_delta_f_low", ctypes.c_double), ("lambda_per_periods", ctypes.c_int),
+                ("save_lambdas", ctypes.c_int), ("num_random_obs", ctypes.c_int),
+                ("random_obs_seed", ctypes.c_int), ("random_obs_mns", ctypes.c_int),
+                ("random_obs_type", ctypes.c_int)]
+
+
 class EnsembleDesc(ctypes.Structure):
     _fields_ = [("num_trajectories", ctypes.c_int), ("seed", ctypes.c_int), ("mns", ctypes.c_int),
                 ("start_type", ctypes.c_int), ("start_state", ctypes.c_int), ("jump", ctypes.c_int),
@@ -114,6 +124,14 @@ SIGNATURES = [
     ("qjx_lindblad_run_periods", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("qjx_lindblad_observables", ctypes.c_int, [ctypes.c_void_p, _D, _D, _D, _D, _D]),
     ("qjx_lindblad_dim", ctypes.c_int, [ctypes.c_void_p]),
+    ("qjx_lpn_create", ctypes.c_int, [ctypes.c_void_p, _P(LpnDesc), _P(ctypes.c_void_p)]),
+    ("qjx_lpn_destroy", None, [ctypes.c_void_p]),
+    ("qjx_lpn_trans", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("qjx_lpn_run_periods", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("qjx_lpn_get", ctypes.c_int, [ctypes.c_void_p, _D, _D, _P(ctypes.c_int), _D, _D, _D]),
+    ("qjx_lpn_random_obs", ctypes.c_int, [ctypes.c_void_p, _D, _D]),
+    ("qjx_lpn_lambdas", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, ctypes.c_int]),
+    ("qjx_lpn_random_obs_matrix", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D]),
     ("qjx_pool_create", ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int, _P(ctypes.c_void_p)]),
     ("qjx_pool_destroy", None, [ctypes.c_void_p]),
     ("qjx_pool_submit", ctypes.c_int, [ctypes.c_void_p, _P(ModelDesc), _P(EnsembleDesc), ctypes.c_int,
@@ -447,3 +465,70 @@ def run_config(config_path, params_path, precision=FP64, steps_per_period=0):
     lib = load_library()
     return _check(lib, lib.qjx_run_config(config_path.encode(), params_path.encode(), precision,
                                           steps_per_period))
+
+
+LPN_DEFAULTS = dict(lpn_type=-1, lpn_eps_deep=100, lpn_eps_error=1e-10, lpn_eps_high=100.0, lpn_eps_low=-100.0,
+                    lpn_delta_s=1e-2, lpn_delta_f_high=1e-2, lpn_delta_f_low=1e-2, lambda_per_periods=1,
+                    save_lambdas=0, num_random_obs=0, random_obs_seed=1, random_obs_mns=1000000,
+                    random_obs_type=2)
+
+
+class Lpn:
+    """The LPN task (task_id 7) over an Engine whose ensemble is initialised with an
+    even number of trajectories; keyword arguments are the params.txt keys."""
+
+    def __init__(self, engine, **kw):
+        self.lib = engine.lib
+        self.engine = engine
+        p = dict(LPN_DEFAULTS, **kw)
+        self.desc = LpnDesc(**p)
+        self.nobs = p["num_random_obs"]
+        h = ctypes.c_void_p()
+        _check(self.lib, self.lib.qjx_lpn_create(engine.h, ctypes.byref(self.desc), ctypes.byref(h)))
+        self.h = h
+        self.M = engine.M
+
+    def close(self):
+        if self.h:
+            self.lib.qjx_lpn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def trans(self, n):
+        _check(self.lib, self.lib.qjx_lpn_trans(self.h, n))
+
+    def run_periods(self, n):
+        _check(self.lib, self.lib.qjx_lpn_run_periods(self.h, n))
+
+    def get(self):
+        M = self.M
+        out = {k: np.zeros(M) for k in ("lambda_now", "lambda", "mean_lpn", "energy_lpn", "delta_s")}
+        nr = np.zeros(M, dtype=np.int32)
+        _check(self.lib, self.lib.qjx_lpn_get(self.h, _ptr(out["lambda_now"]), _ptr(out["lambda"]),
+                                              _ptr(nr, ctypes.c_int), _ptr(out["mean_lpn"]),
+                                              _ptr(out["energy_lpn"]), _ptr(out["delta_s"])))
+        out["num_renorms"] = nr
+        return out
+
+    def random_obs(self):
+        o = np.zeros((self.M, max(self.nobs, 1)), dtype=np.complex128)
+        ol = np.zeros_like(o)
+        _check(self.lib, self.lib.qjx_lpn_random_obs(self.h, _ptr(o.view(np.float64)), _ptr(ol.view(np.float64))))
+        return o[:, :self.nobs], ol[:, :self.nobs]
+
+    def lambdas(self, tr):
+        n = _check(self.lib, self.lib.qjx_lpn_lambdas(self.h, tr, None, None, None, 0))
+        r, ds, df = np.zeros(n), np.zeros(n), np.zeros(n)
+        self.lib.qjx_lpn_lambdas(self.h, tr, _ptr(r), _ptr(ds), _ptr(df), n)
+        return r, ds, df
+
+    def random_obs_matrix(self, obs):
+        d = self.engine.d
+        R = np.zeros((d, d), dtype=np.complex128)
+        _check(self.lib, self.lib.qjx_lpn_random_obs_matrix(self.h, obs, _ptr(R.view(np.float64))))
+        return R

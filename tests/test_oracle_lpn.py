@@ -1,0 +1,142 @@
+"""Pins the LPN restatement (task_id 7) and its random numbers.
+
+Reference (source/cpp/QJX/QJX/): init_streams_var (newdel_exp.cpp:313-330),
+init_random_obs / get_random_obs (newdel.cpp:3652-3722, experiment.cpp:2023-2078),
+LpnMultExperimentBehaviour (experiment.cpp:137-286), var_trajectory_lpn
+(experiment.cpp:2278-2393), lambda_lpn(_now) (experiment.cpp:3065-3134),
+calc_chars_lpn / calc_delta_s / calc_delta_f (core.cpp:727-915).
+
+* MCG31 uniforms, plain and leap-frogged, on [0, 1) and [-1, 1): bit-exact against
+  Intel MKL golden vectors (tests/golden/mcg31_mkl.json, gen_mcg31_fixture.c).
+* Box-Muller Gaussians: within 2e-8 relative of MKL's (MKL evaluates the transform
+  with internal functions whose accuracy is not published; the restatement uses the
+  C library's log, sin and sqrt, the same on both sides of every comparison below).
+* Random observables: Hermitian, positive semi-definite, the trace normalisation of
+  each random_obs_type, and equal to numpy's X X^dag of the same Gaussians.
+* LPN bookkeeping: after the transient each partner carries its original's RNG
+  state and eta, the original's norm, and an observable distance within
+  lpn_eps_error of lpn_delta_s; with renormalisation every period, lambda is the sum
+  of the logged log ratios and lambda_now = lambda / t.
+"""
+import json
+import math
+import os
+import struct
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+FIX = os.path.join(os.path.dirname(__file__), "golden", "mcg31_mkl.json")
+
+
+def _bits(h):
+    return struct.unpack(">d", bytes.fromhex(h))[0]
+
+
+def test_mcg31_uniform_matches_mkl_bitwise():
+    g = json.load(open(FIX))
+    plain = np.array([_bits(h) for h in g["plain_1024"]])
+    assert np.array_equal(O.mcg31_uniform(777, 0, 1, 1024), plain)
+    v = g["var"]
+    for s in v["streams"]:
+        ref = np.array([_bits(h) for h in s["bits"]])
+        got = O.mcg31_uniform(v["seed"], s["k"], v["nstreams"], len(ref), v["a"], v["b"])
+        assert np.array_equal(got, ref), s["k"]
+
+
+def test_mcg31_box_muller_matches_mkl():
+    g = json.load(open(FIX))["gauss"]
+    for s in g["streams"]:
+        ref = np.array([_bits(h) for h in s["bits"]])
+        got = O.mcg31_gaussian(g["seed"], s["k"], g["mns"], len(ref))
+        rel = np.abs(got - ref) / np.abs(ref)
+        assert rel.max() < 2e-8, (s["k"], rel.max())
+        # the pairing (u1 radius, u2 angle) is fixed: the other pairings miss by O(1)
+        u = O.mcg31_uniform(g["seed"], s["k"], g["mns"], 2 * len(ref))
+        swapped = np.sqrt(-2 * np.log(u[1::2])) * np.sin(2 * np.pi * u[0::2])
+        assert np.abs(swapped - ref).max() > 0.1
+
+
+@pytest.mark.parametrize("typ", [0, 1, 2])
+def test_random_observable_matrix(typ):
+    d, seed, mns = 11, 1, 1000000
+    for obs in (0, 3):
+        R = O.random_obs_matrix(d, seed, mns, typ, obs)
+        gs = O.mcg31_gaussian(3873247, seed + obs, mns, 2 * d * d)
+        X = 0.5 * (gs[:d * d] + 1j * gs[d * d:]).reshape(d, d)
+        ref = X @ X.conj().T
+        if typ > 0:
+            ref = ref / (np.trace(ref).real / (d if typ == 2 else 1))
+        assert np.abs(R - ref).max() < 1e-13 * np.abs(ref).max()
+        assert np.abs(R - R.conj().T).max() < 1e-14 * np.abs(R).max()
+        assert np.linalg.eigvalsh(0.5 * (R + R.conj().T)).min() > -1e-12
+        if typ == 1:
+            assert abs(np.trace(R).real - 1.0) < 1e-13
+        if typ == 2:
+            assert abs(np.trace(R).real - d) < 1e-12
+
+
+def _lpn_ens(lpn_type, M=8, **kw):
+    m = O.model(10, prm_E=1.0, prm_U=0.5, prm_J=1.0, drv_ampl=1.5, drv_freq=1.0, gamma=0.1, S=4, D=12)
+    e = O.Ensemble(m, M, seed=2, jump=1)
+    e.lpn_setup(lpn_type=lpn_type, num_random_obs=2, **kw)
+    return e
+
+
+@pytest.mark.parametrize("lpn_type", [-1, -2, 0, 1])
+def test_lpn_partners_after_transient(lpn_type):
+    M = 8
+    e = _lpn_ens(lpn_type, M)
+    e.lpn_trans(2)
+    s = e.get_state()
+    half = M // 2
+    assert np.array_equal(s["rng"][half:], s["rng"][:half])
+    assert np.array_equal(s["eta"][half:], s["eta"][:half])
+    n = np.sum(np.abs(s["psi"]) ** 2, axis=1)
+    np.testing.assert_allclose(n[half:], n[:half], rtol=1e-14)
+    assert not np.array_equal(s["psi"][half:], s["psi"][:half])
+    g = e.lpn_get()
+    assert (np.abs(g["delta_s"][half:] - 1e-2) < 1e-10).all()
+    assert (g["delta_s"][:half] == 0).all() and (g["lambda"] == 0).all()
+    # the partner's distance as the observable itself gives it
+    if lpn_type == -1:
+        np.testing.assert_allclose(np.abs(g["mean"][half:] - g["mean"][:half]) / 11, 1e-2, atol=1e-10)
+    if lpn_type >= 0:
+        o, _ = e.lpn_random_obs()
+        np.testing.assert_allclose(np.abs(o[half:, lpn_type] - o[:half, lpn_type]), 1e-2, atol=1e-10)
+
+
+def test_lpn_lambda_bookkeeping_every_period():
+    M, P = 8, 5
+    e = _lpn_ens(-1, M, lambda_per_periods=1, save_lambdas=1)
+    e.lpn_trans(1)
+    for _ in range(P):
+        e.lpn_period()
+    g = e.lpn_get()
+    half = M // 2
+    assert (g["num_renorms"][half:] == P).all() and (g["num_renorms"][:half] == 0).all()
+    T = 2 * math.pi
+    for tr in range(half, M):
+        r, ds, df = e.lpn_lambdas(tr)
+        assert len(r) == P
+        np.testing.assert_allclose(r, df / ds, rtol=1e-15)
+        assert abs(g["lambda"][tr] - math.fsum(np.log(r + 1e-16))) < 1e-12
+        assert abs(g["lambda_now"][tr] - g["lambda"][tr] / (P * T)) < 1e-14
+        assert (np.abs(ds - 1e-2) < 1e-10).all()
+
+
+def test_lpn_threshold_renormalisation():
+    """lambda_per_periods 0: a partner is renormalised only when its distance leaves
+    [lpn_delta_f_low, max(lpn_delta_f_high, 10 delta_s)]."""
+    M, P = 8, 6
+    e = _lpn_ens(-1, M, lambda_per_periods=0, save_lambdas=1, lpn_delta_f_high=5e-2, lpn_delta_f_low=2e-3)
+    e.lpn_trans(1)
+    for _ in range(P):
+        e.lpn_period()
+    g = e.lpn_get()
+    for tr in range(M // 2, M):
+        r, ds, df = e.lpn_lambdas(tr)
+        assert len(r) == g["num_renorms"][tr] <= P
+        assert ((df > np.maximum(5e-2, 10 * ds)) | (df < 2e-3)).all()
