@@ -279,6 +279,12 @@ int qjx_lpn_random_obs(qjx_lpn *lpn, double *obs, double *obs_lpn);
 int qjx_lpn_lambdas(qjx_lpn *lpn, int tr, double *ratio, double *ds, double *df, int cap);
 /* Random observable obs as d x d complex (interleaved, row-major). */
 int qjx_lpn_random_obs_matrix(qjx_lpn *lpn, int obs, double *R);
+/* calc_chars_std of the last period, taken before that period's renormalisations
+ * (arrays of M), the current abs_diag_rho (M x d, rewritten by calc_chars_lpn as in
+ * the reference) and its average over trajectories [begin, end) as dump_adr_avg /
+ * dump_adr_avg_mult form it (dump.cpp:26-98); any pointer may be NULL. */
+int qjx_lpn_chars(qjx_lpn *lpn, double *norm, double *mean, double *dispersion, double *m2, double *energy,
+                  double *adr, int begin, int end, double *adr_avg);
 
 typedef struct qjx_pool qjx_pool;
 

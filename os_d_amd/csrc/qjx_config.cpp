@@ -1,7 +1,7 @@
 // qjx_config.cpp -- the drop-in runner: the reference's config.txt / params.txt in,
 // its STD-experiment output files out, with the propagation on the GPU engine.
 //
-// Restates, for sys_id 0 / task_id 1 and 4 (STD, STD deep) / prop_id 0 and 1 (reference files under
+// Restates, for sys_id 0 / task_id 1, 4 and 7 (STD, STD deep, LPN) / prop_id 0 and 1 (reference files under
 // source/cpp/QJX/QJX/):
 //   init_params / set_param (config.cpp:29-145), the single-run main
 //   (main.cpp:3-192) and Processor::process (processor.cpp:50-75),
@@ -199,6 +199,186 @@ bool save_complex(const std::string &fn, const double *v, size_t n, bool append)
     return true;
 }
 
+// save_2d_vector<std::complex<double>> (utils.h:53-71): rows = dumps, columns =
+// trajectories, each entry as std::complex prints it, "(re,im)".
+bool save_2d_complex(const std::string &fn, const std::vector<std::vector<double>> &v)
+{
+    std::ofstream o(fn);
+    if (!o.is_open()) return false;
+    o << std::setprecision(16) << std::scientific;
+    const size_t cols = v.size(), rows = cols ? v[0].size() / 2 : 0;
+    for (size_t r = 0; r < rows; r++)
+        for (size_t c = 0; c < cols; c++)
+            o << "(" << v[c][2 * r] << "," << v[c][2 * r + 1] << ")" << (c + 1 < cols ? "\t" : "\n");
+    return true;
+}
+
+// LpnMultNewDelBehaviour / LpnMultExperimentBehaviour (newdel_exp.cpp:29-48,
+// experiment.cpp:137-286) with dump_std / dump_lpn / dump_std_evo / dump_lpn_evo
+// (core.cpp:918-1130) and dump_adr_single / dump_adr_avg(_mult) (dump.cpp:3-98) on an
+// initialised context.  Returns QJX_OK or an error (message in g_cfg_err).
+int run_lpn(const RunCfg &c, Params &p, qjx_ctx *ctx, const std::string &sfx, const std::vector<int> &dps, int M,
+            int d)
+{
+    const std::string &path = c.path;
+    const int dnt = (int)dps.size();
+    qjx_lpn_desc ld{};
+    ld.lpn_type = int(p.get("lpn_type"));
+    ld.lpn_eps_deep = int(p.get("lpn_eps_deep"));
+    ld.lpn_eps_error = p.get("lpn_eps_error");
+    ld.lpn_eps_high = p.get("lpn_eps_high");
+    ld.lpn_eps_low = p.get("lpn_eps_low");
+    ld.lpn_delta_s = p.get("lpn_delta_s");
+    ld.lpn_delta_f_high = p.get("lpn_delta_f_high");
+    ld.lpn_delta_f_low = p.get("lpn_delta_f_low");
+    ld.lambda_per_periods = int(p.get("lambda_per_periods"));
+    ld.save_lambdas = int(p.get("save_lambdas"));
+    ld.num_random_obs = int(p.get("num_random_obs"));
+    ld.random_obs_seed = int(p.get("random_obs_seed"));
+    ld.random_obs_mns = int(p.get("random_obs_mns"));
+    ld.random_obs_type = int(p.get("random_obs_type"));
+    const int jump = int(p.get("jump"));
+    const int dump_obs = int(p.get("dump_obs"));
+    const int dump_evo_sep = int(p.get("dump_evo_sep"));
+    const int dump_evo_avg = int(p.get("dump_evo_avg"));
+    const int dump_adr_sep = int(p.get("dump_adr_sep"));
+    const int dump_adr_avg = int(p.get("dump_adr_avg"));
+    if (!p.missing.empty()) {
+        g_cfg_err = "params.txt lacks key " + p.missing;
+        return QJX_ERR_ARG;
+    }
+    qjx_lpn *L = nullptr;
+    int rc = qjx_lpn_create(ctx, &ld, &L);
+    if (rc < 0) {
+        g_cfg_err = qjx_last_error();
+        return rc;
+    }
+    const int no = ld.num_random_obs, half = M / 2;
+    std::vector<double> norm(M), mean(M), disp(M), m2(M), energy(M), mean_start(M), adr((size_t)M * d), avg(d);
+    std::vector<double> lambda_now(M), mean_lpn(M), energy_lpn(M);
+    std::vector<int> renorms(M);
+    std::vector<double> norm_evo((size_t)M * dnt, 0.0), mean_evo = norm_evo, disp_evo = norm_evo, m2_evo = norm_evo,
+                        energy_evo = norm_evo, lambda_evo = norm_evo, mean_lpn_evo = norm_evo,
+                        energy_lpn_evo = norm_evo;
+    // random observables per obs and trajectory: (re, im) per dump
+    std::vector<std::vector<std::vector<double>>> ro_evo(no, std::vector<std::vector<double>>(M)),
+        ro_lpn_evo = ro_evo;
+    std::vector<double> ro((size_t)2 * M * std::max(no, 1)), ro_lpn = ro;
+    auto fail = [&](int code, const std::string &msg) {
+        g_cfg_err = msg;
+        qjx_lpn_destroy(L);
+        return code;
+    };
+    // evo_chars_std / evo_chars_lpn at dump_id, then dump_adr_single
+    auto record = [&](int dump_id, bool append_adr) -> int {
+        int r = qjx_lpn_chars(L, norm.data(), mean.data(), disp.data(), m2.data(), energy.data(), adr.data(), 0, 0,
+                              nullptr);
+        if (r == QJX_OK) r = qjx_lpn_get(L, lambda_now.data(), nullptr, renorms.data(), mean_lpn.data(),
+                                        energy_lpn.data(), nullptr);
+        if (r == QJX_OK) r = qjx_lpn_random_obs(L, ro.data(), ro_lpn.data());
+        if (r < 0) return r;
+        for (int tr = 0; tr < M; tr++) {
+            const size_t i = (size_t)tr * dnt + dump_id;
+            norm_evo[i] = norm[tr];
+            mean_evo[i] = mean[tr];
+            disp_evo[i] = disp[tr];
+            m2_evo[i] = m2[tr];
+            energy_evo[i] = energy[tr];
+            lambda_evo[i] = lambda_now[tr];
+            mean_lpn_evo[i] = mean_lpn[tr];
+            energy_lpn_evo[i] = energy_lpn[tr];
+            for (int o = 0; o < no; o++) {
+                const size_t k = 2 * ((size_t)tr * no + o);
+                ro_evo[o][tr].push_back(ro[k]);
+                ro_evo[o][tr].push_back(ro[k + 1]);
+                ro_lpn_evo[o][tr].push_back(ro_lpn[k]);
+                ro_lpn_evo[o][tr].push_back(ro_lpn[k + 1]);
+            }
+            if (dump_evo_sep == 1 && dump_adr_sep == 1 &&
+                !save_doubles(path + "adr_" + std::to_string(tr) + sfx, &adr[(size_t)tr * d], d, append_adr))
+                return QJX_ERR_IO;
+        }
+        return QJX_OK;
+    };
+    auto adr_avg_dump = [&](int begin, int end, bool append) -> int {
+        if (dump_adr_avg != 1) return QJX_OK;
+        int r = qjx_lpn_chars(L, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, begin, end, avg.data());
+        if (r < 0) return r;
+        return save_doubles(path + "adr_avg" + sfx, avg.data(), d, append) ? QJX_OK : QJX_ERR_IO;
+    };
+
+    // trans_process (experiment.cpp:137-190)
+    if ((rc = qjx_lpn_trans(L, c.num_tp_periods)) < 0) return fail(rc, qjx_last_error());
+    if ((rc = record(0, false)) != QJX_OK) return fail(rc, rc == QJX_ERR_IO ? "unable to write output file" : qjx_last_error());
+    mean_start = mean;
+    if (dump_evo_avg == 1 && (rc = adr_avg_dump(0, M, false)) != QJX_OK) return fail(rc, "unable to write output file");
+    // obser_process (experiment.cpp:193-286): every period advances all trajectories
+    // and updates the pairs; observables are recorded at the dump periods
+    for (int dump_id = 1; dump_id < dnt; dump_id++) {
+        if (c.is_pp == 1) std::cout << "dump_id: " << dump_id << std::endl;
+        if ((rc = qjx_lpn_run_periods(L, dps[dump_id] - dps[dump_id - 1])) < 0) return fail(rc, qjx_last_error());
+        if ((rc = record(dump_id, true)) != QJX_OK)
+            return fail(rc, rc == QJX_ERR_IO ? "unable to write output file" : qjx_last_error());
+        if (dump_evo_avg == 1 && (rc = adr_avg_dump(0, half, true)) != QJX_OK)
+            return fail(rc, "unable to write output file");
+    }
+    bool ok = true;
+    if (dump_obs == 1) {                     // dump_std (core.cpp:918-957) and dump_lpn (core.cpp:959-1004)
+        ok = ok && save_doubles(path + "norm" + sfx, norm.data(), M, false);
+        ok = ok && save_doubles(path + "mean_start" + sfx, mean_start.data(), M, false);
+        ok = ok && save_doubles(path + "mean" + sfx, mean.data(), M, false);
+        ok = ok && save_doubles(path + "dispersion" + sfx, disp.data(), M, false);
+        ok = ok && save_doubles(path + "m2" + sfx, m2.data(), M, false);
+        ok = ok && save_doubles(path + "energy" + sfx, energy.data(), M, false);
+        ok = ok && save_doubles(path + "lambda" + sfx, lambda_now.data(), M, false);
+        ok = ok && save_ints(path + "num_renorms" + sfx, renorms.data(), M, false);
+        ok = ok && save_doubles(path + "mean_lpn" + sfx, mean_lpn.data(), M, false);
+        ok = ok && save_doubles(path + "energy_lpn" + sfx, energy_lpn.data(), M, false);
+        if (ld.save_lambdas > 0)
+            for (int tr = 0; tr < M && ok; tr++) {
+                const int n = qjx_lpn_lambdas(L, tr, nullptr, nullptr, nullptr, 0);
+                std::vector<double> r(n), ds(n), df(n);
+                qjx_lpn_lambdas(L, tr, r.data(), ds.data(), df.data(), n);
+                ok = ok && save_doubles(path + "delta_f_by_delta_s_" + std::to_string(tr) + sfx, r.data(), n, false);
+                ok = ok && save_doubles(path + "deltas_s_" + std::to_string(tr) + sfx, ds.data(), n, false);
+                ok = ok && save_doubles(path + "deltas_f_" + std::to_string(tr) + sfx, df.data(), n, false);
+            }
+    }
+    if (ok && dump_evo_sep == 1 && dump_obs == 1) {   // dump_std_evo and dump_lpn_evo (core.cpp:1008-1130)
+        ok = ok && save_ints(path + "periods" + sfx, dps.data(), dnt, false);
+        ok = ok && save_2d_inv(path + "norm_evo" + sfx, norm_evo, dnt, M);
+        ok = ok && save_2d_inv(path + "mean_evo" + sfx, mean_evo, dnt, M);
+        ok = ok && save_2d_inv(path + "dispersion_evo" + sfx, disp_evo, dnt, M);
+        ok = ok && save_2d_inv(path + "m2_evo" + sfx, m2_evo, dnt, M);
+        ok = ok && save_2d_inv(path + "energy_evo" + sfx, energy_evo, dnt, M);
+        for (int o = 0; o < no && ok; o++)
+            ok = save_2d_complex(path + "random_obs_evo_" + std::to_string(ld.random_obs_seed + o) + sfx, ro_evo[o]);
+        if (jump > 0)
+            for (int tr = 0; tr < M && ok; tr++) {
+                const int n = qjx_get_jumps(ctx, tr, nullptr, nullptr, nullptr, nullptr, 0);
+                std::vector<double> t(n), nr(n), et(n);
+                std::vector<int> ty(n);
+                qjx_get_jumps(ctx, tr, t.data(), ty.data(), nr.data(), et.data(), n);
+                ok = ok && save_doubles(path + "jump_times_" + std::to_string(tr) + sfx, t.data(), n, false);
+                ok = ok && save_ints(path + "diss_types_" + std::to_string(tr) + sfx, ty.data(), n, false);
+                if (jump > 1) {
+                    ok = ok && save_doubles(path + "jump_norms_" + std::to_string(tr) + sfx, nr.data(), n, false);
+                    ok = ok && save_doubles(path + "jump_etas_" + std::to_string(tr) + sfx, et.data(), n, false);
+                }
+            }
+        ok = ok && save_2d_inv(path + "lambda_evo" + sfx, lambda_evo, dnt, M);
+        ok = ok && save_2d_inv(path + "mean_lpn_evo" + sfx, mean_lpn_evo, dnt, M);
+        ok = ok && save_2d_inv(path + "energy_lpn_evo" + sfx, energy_lpn_evo, dnt, M);
+        for (int o = 0; o < no && ok; o++)
+            ok = save_2d_complex(path + "random_obs_lpn_evo_" + std::to_string(ld.random_obs_seed + o) + sfx,
+                                 ro_lpn_evo[o]);
+    }
+    if (ok && dump_evo_avg == 0) ok = adr_avg_dump(0, half, true) == QJX_OK;
+    if (!ok) return fail(QJX_ERR_IO, "unable to write output file");
+    qjx_lpn_destroy(L);
+    return QJX_OK;
+}
+
 }  // namespace
 
 #define CFG_FAIL(code, msg)          \
@@ -250,9 +430,10 @@ extern "C" int qjx_run_config(const char *config_path, const char *params_path, 
     }
     {
         Params p{c.params, ""};
-        if (c.sys_id != 0 || (c.task_id != 1 && c.task_id != 4) || (c.prop_id != 0 && c.prop_id != 1))
-            CFG_FAIL(QJX_ERR_ARG, "only sys_id 0 (dimer), task_id 1 (STD) or 4 (STD deep), prop_id 0 (QJ) "
-                                  "or 1 (RK) are implemented");
+        if (c.sys_id != 0 || (c.task_id != 1 && c.task_id != 4 && c.task_id != 7) ||
+            (c.prop_id != 0 && c.prop_id != 1))
+            CFG_FAIL(QJX_ERR_ARG, "only sys_id 0 (dimer), task_id 1 (STD), 4 (STD deep) or 7 (LPN), prop_id 0 "
+                                  "(QJ) or 1 (RK) are implemented");
         deep = c.task_id == 4;
         sfx = suffix_param(c, p);
         qjx_model_desc md{};
@@ -284,8 +465,8 @@ extern "C" int qjx_run_config(const char *config_path, const char *params_path, 
         const double drv_phase = p.get("dimer_drv_phase");
         if (deep) deep_steps = int(p.get("deep_num_steps"));
         if (!p.missing.empty()) CFG_FAIL(QJX_ERR_ARG, "params.txt lacks key " + p.missing);
-        if (num_random_obs != 0)
-            CFG_FAIL(QJX_ERR_ARG, "num_random_obs > 0 (random observables) is not implemented; set it to 0");
+        if (num_random_obs != 0 && c.task_id != 7)
+            CFG_FAIL(QJX_ERR_ARG, "random observables (num_random_obs > 0) are implemented for task_id 7 only");
         if (deep) {
             // StdDeep (task 4): a dump after each of the 2 deep_num_steps sub-periods of
             // every period; dump_type / dump_num do not apply (init_dump_periods_deep,
@@ -334,6 +515,10 @@ extern "C" int qjx_run_config(const char *config_path, const char *params_path, 
         const double T = 2.0 * 3.1415926535897932384626433832795 / md.drv_freq;
         ed.log_capacity = jump > 0 ? int(std::ceil(1.5 * md.diss_gamma * (md.N + 2) * T)) + 64 : 0;
         API_CHECK(qjx_ensemble_init(ctx, &ed));
+        if (c.task_id == 7) {
+            rc = run_lpn(c, p, ctx, sfx, dps, M, d);
+            goto done;
+        }
 
         const std::string &path = c.path;
         norm.assign(M, 0.0);

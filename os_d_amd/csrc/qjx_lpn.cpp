@@ -111,6 +111,7 @@ struct qjx_lpn {
     std::vector<std::vector<C>> R;                  // random observables, d x d
     std::vector<Mcg31> var;                         // perturbation streams (streams_var)
     std::vector<double> mean, energy, mean_lpn, energy_lpn, lambda, lambda_now, delta_s;
+    std::vector<double> norm, disp, m2, mean_start, adr;   // calc_chars_std; adr is M x d
     std::vector<int> num_renorms;
     std::vector<std::vector<C>> obs, obs_lpn;       // [tr][obs]
     std::vector<std::vector<double>> lambdas, deltas_s, deltas_f;
@@ -170,14 +171,35 @@ struct qjx_lpn {
         return {rr, ri};
     }
 
-    void chars_std(int tr)
+    // abs_diag_rho of trajectory tr (both calc_chars_std and calc_chars_lpn write it)
+    void write_adr(int tr)
     {
+        const C *f = phi(tr);
+        const double nrm = norm_square(f, d);
+        double *a = adr.data() + (size_t)tr * d;
+        for (int k = 0; k < d; k++) a[k] = (f[k].re * f[k].re + f[k].im * f[k].im) * (1.0 / nrm);
+    }
+
+    // calc_chars_std(_start) (core.cpp:665-725): norm, mean, dispersion against
+    // mean_start, m2, energy, abs_diag_rho, random observables
+    void chars_std(int tr, bool is_start = false)
+    {
+        norm[tr] = norm_square(phi(tr), d);
+        write_adr(tr);
         mean_energy(phi(tr), &mean[tr], &energy[tr]);
+        if (is_start) mean_start[tr] = mean[tr];
+        disp[tr] = (mean[tr] - mean_start[tr]) * (mean[tr] - mean_start[tr]);
+        const double *a = adr.data() + (size_t)tr * d;
+        double mm2 = 0.0;
+        for (int k = 0; k < d; k++) mm2 += (double(k) - mean[tr]) * (double(k) - mean[tr]) * a[k];
+        m2[tr] = mm2;
         for (size_t o = 0; o < R.size(); o++) obs[tr][o] = random_obs(phi(tr), R[o]);
     }
 
+    // calc_chars_lpn (core.cpp:761-792)
     void chars_lpn(int tr)
     {
+        write_adr(tr);
         mean_energy(phi(tr), &mean_lpn[tr], &energy_lpn[tr]);
         for (size_t o = 0; o < R.size(); o++) obs_lpn[tr][o] = random_obs(phi(tr), R[o]);
     }
@@ -325,6 +347,11 @@ extern "C" int qjx_lpn_create(qjx_ctx *c, const qjx_lpn_desc *desc, qjx_lpn **ou
     L->lambda_now.assign(M, 0.0);
     L->delta_s.assign(M, 0.0);
     L->num_renorms.assign(M, 0);
+    L->norm.assign(M, 0.0);
+    L->disp.assign(M, 0.0);
+    L->m2.assign(M, 0.0);
+    L->mean_start.assign(M, 0.0);
+    L->adr.assign((size_t)M * d, 0.0);
     L->obs.assign(M, std::vector<C>(desc->num_random_obs, C{0.0, 0.0}));
     L->obs_lpn.assign(M, std::vector<C>(desc->num_random_obs, C{0.0, 0.0}));
     L->lambdas.assign(M, {});
@@ -349,7 +376,7 @@ extern "C" int qjx_lpn_trans(qjx_lpn *L, int num_tp_periods)
     if (rc) return rc;
     if ((rc = L->pull())) return rc;
     for (int tr = 0; tr < L->half; tr++) {
-        L->chars_std(tr);
+        L->chars_std(tr, true);
         L->start(tr, tr);
     }
     for (int tr = 0; tr < L->M; tr++) {
@@ -358,7 +385,7 @@ extern "C" int qjx_lpn_trans(qjx_lpn *L, int num_tp_periods)
             L->copy(tr, base, true);
             L->perturb(tr, base);
         }
-        L->chars_std(tr);
+        L->chars_std(tr, true);
         L->start(tr, base);
     }
     if ((rc = L->push())) return rc;
@@ -464,5 +491,32 @@ extern "C" int qjx_lpn_random_obs_matrix(qjx_lpn *L, int obs, double *R)
     if (!L || !R) return lpn_fail(QJX_ERR_ARG, "null argument");
     if (obs < 0 || obs >= (int)L->R.size()) return lpn_fail(QJX_ERR_ARG, "random observable out of range");
     std::memcpy(R, L->R[obs].data(), sizeof(C) * L->R[obs].size());
+    return QJX_OK;
+}
+
+// The last period's calc_chars_std values (before any renormalisation of that
+// period), the current abs_diag_rho (M x d) and its average over trajectories
+// [begin, end) as dump_adr_avg(_mult) forms it (dump.cpp:26-98).  Any pointer may
+// be NULL.
+extern "C" int qjx_lpn_chars(qjx_lpn *L, double *norm, double *mean, double *dispersion, double *m2,
+                             double *energy, double *adr, int begin, int end, double *adr_avg)
+{
+    if (!L) return lpn_fail(QJX_ERR_ARG, "null LPN handle");
+    if (adr_avg && (begin < 0 || end > L->M || end <= begin)) return lpn_fail(QJX_ERR_ARG, "bad trajectory range");
+    const int M = L->M, d = L->d;
+    for (int tr = 0; tr < M; tr++) {
+        if (norm) norm[tr] = L->norm[tr];
+        if (mean) mean[tr] = L->mean[tr];
+        if (dispersion) dispersion[tr] = L->disp[tr];
+        if (m2) m2[tr] = L->m2[tr];
+        if (energy) energy[tr] = L->energy[tr];
+    }
+    if (adr) std::memcpy(adr, L->adr.data(), sizeof(double) * L->adr.size());
+    if (adr_avg) {
+        const double n = double(end - begin);
+        for (int k = 0; k < d; k++) adr_avg[k] = 0.0;
+        for (int tr = begin; tr < end; tr++)
+            for (int k = 0; k < d; k++) adr_avg[k] += L->adr[(size_t)tr * d + k] / n;
+    }
     return QJX_OK;
 }

@@ -132,6 +132,7 @@ SIGNATURES = [
     ("qjx_lpn_random_obs", ctypes.c_int, [ctypes.c_void_p, _D, _D]),
     ("qjx_lpn_lambdas", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, ctypes.c_int]),
     ("qjx_lpn_random_obs_matrix", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D]),
+    ("qjx_lpn_chars", ctypes.c_int, [ctypes.c_void_p, _D, _D, _D, _D, _D, _D, ctypes.c_int, ctypes.c_int, _D]),
     ("qjx_pool_create", ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int, _P(ctypes.c_void_p)]),
     ("qjx_pool_destroy", None, [ctypes.c_void_p]),
     ("qjx_pool_submit", ctypes.c_int, [ctypes.c_void_p, _P(ModelDesc), _P(EnsembleDesc), ctypes.c_int,
@@ -526,6 +527,21 @@ class Lpn:
         r, ds, df = np.zeros(n), np.zeros(n), np.zeros(n)
         self.lib.qjx_lpn_lambdas(self.h, tr, _ptr(r), _ptr(ds), _ptr(df), n)
         return r, ds, df
+
+    def chars(self, begin=0, end=None):
+        """calc_chars_std of the last period (before its renormalisations), abs_diag_rho
+        and its average over trajectories [begin, end)."""
+        M, d = self.M, self.engine.d
+        end = M // 2 if end is None else end
+        out = {k: np.zeros(M) for k in ("norm", "mean", "dispersion", "m2", "energy")}
+        adr = np.zeros((M, d))
+        avg = np.zeros(d)
+        _check(self.lib, self.lib.qjx_lpn_chars(self.h, _ptr(out["norm"]), _ptr(out["mean"]), _ptr(out["dispersion"]),
+                                                _ptr(out["m2"]), _ptr(out["energy"]), _ptr(adr), begin, end,
+                                                _ptr(avg)))
+        out["adr"] = adr
+        out["adr_avg"] = avg
+        return out
 
     def random_obs_matrix(self, obs):
         d = self.engine.d

@@ -219,3 +219,75 @@ def test_run_config_jumps_counts_stop(tmp_path, oracle_mod):
     np.testing.assert_allclose(me[p_stop], o.chars()["mean"], rtol=1e-9)
     assert (me[p_stop + 1:] == 0).all()
     np.testing.assert_allclose(np.loadtxt(out + "mean" + SUFFIX), o.chars()["mean"], rtol=1e-9)
+
+
+LPN_KEYS = """lpn_type {lpn_type}
+lpn_eps_deep 100
+lpn_eps_error 1e-10
+lpn_eps_high 100
+lpn_eps_low -100
+lpn_delta_s 1.0e-2
+lpn_delta_f_high 5.0e-2
+lpn_delta_f_low 2.0e-3
+lambda_per_periods {lpp}
+save_lambdas 1
+random_obs_seed 1
+random_obs_mns 1000000
+random_obs_type 2
+"""
+
+
+@pytest.mark.parametrize("lpn_type,lpp", [(0, 0), (-1, 1)])
+def test_run_config_lpn_outputs_match_oracle(tmp_path, oracle_mod, lpn_type, lpp):
+    """task_id 7 (LpnMultExperimentBehaviour, experiment.cpp:137-286) with RK
+    propagation: lambda, num_renorms, the lpn and std evolutions and the random
+    observables written by the runner equal the oracle's LPN run."""
+    from os_d_amd import qjx
+    print("case: run_config lpn type=%d lpp=%d" % (lpn_type, lpp), flush=True)
+    out = str(tmp_path) + "/"
+    M = 16
+    cfg = CONFIG.format(path=out).replace("task_id 1", "task_id 7").replace("prop_id 0", "prop_id 1").replace(
+        "rk_ns 10000", "rk_ns 2000").replace("num_trajectories 24", "num_trajectories %d" % M).replace(
+        "num_obs_periods 6", "num_obs_periods 4")
+    prm = PARAMS.replace("dimer_drv_type 0", "dimer_drv_type 1").replace("num_random_obs 0", "num_random_obs 1") \
+        .replace("dump_num 3", "dump_num 2").replace("jump 2", "jump 1").replace("N 12", "N 10") \
+        + LPN_KEYS.format(lpn_type=lpn_type, lpp=lpp)
+    (tmp_path / "config.txt").write_text(cfg)
+    (tmp_path / "params.txt").write_text(prm)
+    qjx.run_config(str(tmp_path / "config.txt"), str(tmp_path / "params.txt"), qjx.FP64, 0)
+    sfx = "_setup(0_7_1)_rnd(0_1000000)_N(10)_diss(0_0.2000_0.0000)_drv(1_1.5000_1.0000_0.0000)" \
+          "_prm(1.0000_0.5000_1.0000)_start(0_0).txt"
+    m = oracle_mod.model(10, gamma=0.2, D=1)
+    o = oracle_mod.Ensemble(m, M, jump=1, build_propagators=False)
+    o.rk_setup(2000, 1, 0.0)
+    o.lpn_setup(lpn_type=lpn_type, lambda_per_periods=lpp, save_lambdas=1, lpn_delta_f_high=5e-2,
+                lpn_delta_f_low=2e-3, num_random_obs=1)
+    o.lpn_trans(2)
+    evo = [o.lpn_get()]
+    obs = [o.lpn_random_obs()[0][:, 0]]
+    for _ in range(2):
+        o.lpn_period()
+        o.lpn_period()
+        evo.append(o.lpn_get())
+        obs.append(o.lpn_random_obs()[0][:, 0])
+
+    def load(name):
+        return np.loadtxt(out + name + sfx)
+    g = evo[-1]
+    assert np.array_equal(load("lambda"), g["lambda_now"])
+    assert np.array_equal(load("num_renorms").astype(int), g["num_renorms"])
+    assert np.array_equal(load("mean_lpn"), g["mean_lpn"])
+    le, me = load("lambda_evo"), load("mean_evo")
+    assert le.shape == (3, M)
+    for k in range(3):
+        assert np.array_equal(le[k], evo[k]["lambda_now"])
+        assert np.array_equal(me[k], evo[k]["mean"])
+    rows = open(out + "random_obs_evo_1" + sfx).read().split("\n")[:3]
+    for k, row in enumerate(rows):
+        vals = [complex(*map(float, c.strip("()").split(","))) for c in row.split("\t")]
+        assert np.array_equal(np.array(vals), obs[k])
+    for tr in range(M // 2, M):
+        r, ds, df = o.lpn_lambdas(tr)
+        f = out + "delta_f_by_delta_s_%d" % tr + sfx
+        got = np.atleast_1d(np.loadtxt(f)) if os.path.getsize(f) else np.zeros(0)
+        assert np.array_equal(got, r)
