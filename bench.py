@@ -336,6 +336,48 @@ def rk_leg(qjx, N, rk_ns, M, steps, cpu_budget_s, drv_type=0, with_cpu=True):
     return res
 
 
+def lpn_leg(qjx, M, periods, cpu_periods, with_cpu=True):
+    """The LPN task (task_id 7) on the reference's own LPN parameter set
+    (source/cpp/QJX/QJX/params.txt and config.txt: N = 10, E = 0, U = 0.3, J = 1, sine
+    drive A = 3.4, omega = 1, gamma = 0.1, lpn_type -1, lambda_per_periods 1, RK with
+    rk_ns = 10000) with M trajectories (M / 2 pairs): GPU propagation plus host
+    bookkeeping per period, against the CPU restatement on the host cores."""
+    from oracle import oracle as orc
+    params = dict(prm_E=0.0, prm_U=0.3, prm_J=1.0, drv_ampl=3.4, drv_freq=1.0)
+    lpn = dict(lpn_type=-1, lambda_per_periods=1)
+    e = qjx.Engine(10, diss_gamma=0.1, prop="rk", rk_ns=10000, drv_type=1, **params)
+    e.ensemble_init(M, seed=1, jump=0)
+    L = qjx.Lpn(e, **lpn)
+    L.trans(0)
+    L.run_periods(1)                                    # warm-up
+    t0 = time.perf_counter()
+    L.run_periods(periods)
+    wall = time.perf_counter() - t0
+    lam = L.get()["lambda_now"][M // 2:]
+    L.close()
+    e.close()
+    res = {"N": 10, "dim": 11, "trajectories": M, "pairs": M // 2, "rk_ns": 10000, "periods": periods,
+           "lpn_periods_per_s": periods / wall, "trajectory_periods_per_s": M * periods / wall,
+           "lambda_mean": float(lam.mean()), "lambda_std": float(lam.std())}
+    if with_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+        threads = min(threads, os.cpu_count() or threads)
+        m = orc.model(10, gamma=0.1, D=1, **params)
+        o = orc.Ensemble(m, M, seed=1, jump=0, threads=threads, build_propagators=False)
+        o.rk_setup(10000, 1, 0.0)
+        o.lpn_setup(**lpn)
+        o.lpn_trans(0)
+        t0 = time.perf_counter()
+        for _ in range(cpu_periods):
+            o.lpn_period()
+        dtc = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": M * cpu_periods / dtc, "unit": "trajectory-periods/s", "cores": threads,
+                               "kind": "port", "sample": "%d trajectories x %d LPN periods, CPU restatement, %.1f s"
+                                                         % (M, cpu_periods, dtc)}
+        res["gpu_over_cpu"] = res["trajectory_periods_per_s"] / res["cpu_baseline"]["value"]
+    return res
+
+
 SERVE_MIXES = {
     # the serving configuration as specified
     "specified": {"N": [100, 250, 500, 1000, 2000], "logM": (3.0, 5.0), "periods": (50, 500)},
@@ -418,6 +460,8 @@ def main():
     ap.add_argument("--no-rk", action="store_true")
     ap.add_argument("--only-rk", action="store_true", help="only the Runge-Kutta trajectory legs")
     ap.add_argument("--rk-cpu-seconds", type=float, default=8.0)
+    ap.add_argument("--no-lpn", action="store_true")
+    ap.add_argument("--only-lpn", action="store_true", help="only the LPN task legs")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=30.0, help="serving leg window (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
@@ -453,8 +497,15 @@ def main():
         return {"d101": rk_leg(qjx, 100, 10000, 10000, 20000, args.rk_cpu_seconds, with_cpu=with_cpu),
                 "d1001": rk_leg(qjx, 1000, 40000, 4096, 4000, args.rk_cpu_seconds, with_cpu=with_cpu)}
 
+    def lpn_legs(with_cpu):
+        # the reference's 100 trajectories, and enough pairs to fill the device
+        return {"M100": lpn_leg(qjx, 100, 5, 1, with_cpu), "M4096": lpn_leg(qjx, 4096, 5, 1, with_cpu)}
+
     if args.only_rk:
         print(json.dumps({"rk": rk_legs(not args.no_cpu)}), flush=True)
+        return
+    if args.only_lpn:
+        print(json.dumps({"lpn": lpn_legs(not args.no_cpu)}), flush=True)
         return
     if args.only_batch:
         print(json.dumps(batched(qjx, S, D, 1, args.batch_steps, qjx.FP32, bool(args.batch_alone))), flush=True)
@@ -495,6 +546,10 @@ def main():
             e64.close()
             e64 = None
         rk = rk_legs(not args.no_cpu)
+
+    lpn = None
+    if not args.no_lpn:
+        lpn = lpn_legs(not args.no_cpu)
 
     serve = None
     if args.serve_seconds > 0:
@@ -561,6 +616,7 @@ def main():
         "batched_uneven": batch,
         "lindblad": lind,
         "rk_trajectories": rk,
+        "lpn": lpn,
         "p99_job_latency_s": serve["scaled"]["p99_latency_s"] if serve else None,
         "p99_job_latency_note": "from the scaled job mix (the specified mix completes no job within the window)",
         "serving": serve,
