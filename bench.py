@@ -461,6 +461,7 @@ def main():
     ap.add_argument("--only-rk", action="store_true", help="only the Runge-Kutta trajectory legs")
     ap.add_argument("--rk-cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-lpn", action="store_true")
+    ap.add_argument("--peak-probe", action="store_true", help="only the measured device ceilings")
     ap.add_argument("--only-lpn", action="store_true", help="only the LPN task legs")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=30.0, help="serving leg window (0 = skip)")
@@ -490,6 +491,9 @@ def main():
 
     from os_d_amd import qjx
     N, M, S, D = args.N, args.traj, args.S, args.D
+    if args.peak_probe:
+        print(json.dumps({"measured_peaks": qjx.peak_probe()}), flush=True)
+        return
 
     def rk_legs(with_cpu):
         # the reference's CPU configuration (N=100) with its default rk_ns, and the
@@ -561,6 +565,7 @@ def main():
                  "scaled": serving(qjx, args.serve_seconds, args.serve_rate, 12345, args.serve_budget_gib,
                                    args.serve_capacity, "scaled")}
 
+    peaks = qjx.peak_probe()
     launches = max(1, r32["all_gemm_launches"])
     avg_ms = r32["all_gemm_ms"] / launches
     per_col = measured_traffic_per_column()
@@ -590,6 +595,8 @@ def main():
                    "step": "one top-level step T/S of the whole ensemble incl. refinement"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK["fp32"], "unit": "TFLOP/s",
                      "frac": achieved / PEAK["fp32"], "traffic": traffic,
+                     "peak_measured": peaks["f32_mfma_tflops"],
+                     "frac_of_measured": achieved / peaks["f32_mfma_tflops"],
                      "traffic_source": "profiles/step_gemm_fp32_traffic_dimer_d1001_M1e5.md (bytes/column x "
                                        "columns/launch)",
                      "algorithmic_bytes_per_launch": 2 * 4 * 2 * 1024 * cols_per_launch if N == 1000 else None,
@@ -602,6 +609,7 @@ def main():
                      "dense_equivalent_tflops": dense_flops_per_launch / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else None,
                      "timed_steps_band_tflops": (r32["gemm_band_flops"] / (r32["gemm_ms"] * 1e-3) / 1e12
                                                  if r32["gemm_ms"] > 0 else None)},
+        "measured_peaks": peaks,
         "fp32": r32,
         "fp64_value": fp64["tp_per_s"] if fp64 else None,
         "fp64": fp64,

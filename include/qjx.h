@@ -184,6 +184,13 @@ int qjx_set_profiling(qjx_ctx *ctx, int on);
  * FP32 result), all row-major, computed by one wavefront. */
 int qjx_mfma_probe(const float *A32, const float *B32, float *D32, const double *A64,
                    const double *B64, double *D64, const float *Abf, const float *Bbf, float *Dbf);
+/* Measured ceilings of the current device (no reference counterpart): FP32, FP64
+ * and BF16 matrix-core rates from back-to-back independent MFMA chains on every
+ * compute unit, the FP64 vector FMA rate, and the HBM bandwidth of a 2 GiB device
+ * copy (bytes read + written per second); any pointer may be NULL. */
+int qjx_peak_probe(double *f32_mfma_tflops, double *f64_mfma_tflops, double *bf16_mfma_tflops,
+                   double *f64_valu_tflops, double *hbm_gbps);
+
 /* Hilbert dimension of the context (N + 1). */
 int qjx_dim(const qjx_ctx *ctx);
 

@@ -6,6 +6,7 @@ qjx        : the drop-in command-line runner (config.txt / params.txt in, refere
 """
 import os
 import subprocess
+from concurrent.futures import ThreadPoolExecutor
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(ROOT)
@@ -20,6 +21,11 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 LIB_SOURCES = ["qjx_kernels.hip", "qjx_step.hip", "qjx_lindblad.hip", "qjx_rk.hip", "qjx_engine.cpp",
                "qjx_config.cpp", "qjx_pool.cpp", "qjx_lpn.cpp"]
+# The throughput probe keeps MFMA accumulators in ordinary VGPRs: with the default
+# AGPR form the compiler copies each FP64 accumulator chain through v_accvgpr moves
+# every iteration and the probe would measure the moves, not the matrix core.
+PROBE_SOURCE = "qjx_peak.hip"
+PROBE_FLAGS = ["-mllvm", "-amdgpu-mfma-vgpr-form"]
 
 
 def _run(cmd):
@@ -33,9 +39,19 @@ def _run(cmd):
 
 def build_lib():
     os.makedirs(LIBDIR, exist_ok=True)
-    srcs = [s for s in LIB_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    srcs = [s for s in LIB_SOURCES + [PROBE_SOURCE] if os.path.exists(os.path.join(CSRC, s))]
+    inc = ["-I", os.path.join(REPO, "include")]
+
+    def compile_one(src):
+        obj = os.path.join(LIBDIR, os.path.splitext(src)[0] + ".o")
+        extra = PROBE_FLAGS if src == PROBE_SOURCE else []
+        _run([HIPCC] + HIP_FLAGS + extra + inc + ["-c", src, "-o", obj])
+        return obj
+
+    with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as pool:
+        objs = list(pool.map(compile_one, srcs))
     tmp = LIB + ".tmp"
-    _run([HIPCC] + HIP_FLAGS + ["-shared", "-I", os.path.join(REPO, "include")] + srcs + ["-o", tmp])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", tmp])
     os.replace(tmp, LIB)
     if os.path.exists(os.path.join(CSRC, "qjx_main.cpp")):
         tmp = CLI + ".tmp"

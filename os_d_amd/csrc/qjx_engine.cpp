@@ -408,7 +408,7 @@ extern "C" int qjx_ctx_create_rk(const qjx_model_desc *m, const qjx_rk_desc *rk,
     c->model = *m;
     c->prop = 1;
     c->d = m->N + 1;
-    c->dp = (c->d + 63) / 64 * 64;
+    c->dp = 64 * qjx::rk_lanes(c->d);   // whole lane chunks: the RK kernel reads and writes 64 E entries
     c->L = 2 * c->dp;
     c->D = 1;
     c->S = 2;

@@ -34,7 +34,8 @@ inline int rk_lanes(int d)
 struct RkArgs {
     const double *bands;      // [kRkBands][E][64]
     double *tab;              // [3 nsteps]: drives of each step's full integration (scratch)
-    void *slot0, *slot1;      // FP64 state columns (L doubles: Re at [0, dp), Im at [dp, 2 dp))
+    void *slot0, *slot1;      // FP64 state columns (L doubles: Re at [0, dp), Im at [dp, 2 dp));
+                              // dp = 64 E, the padding beyond d zero
     TrajState *st;
     JumpLog log;
     int *err;                 // [4]: code, trajectory, bits of the offending norm

@@ -271,11 +271,12 @@ void rk_kernel(RkArgs a)
     TrajState s = a.st[tr];
     double *col = (double *)(s.cur ? a.slot1 : a.slot0) + (size_t)tr * a.L;
     double pr[E], pi[E], xr[E], xi[E], ar[E], ai[E];
+    // dp = 64 E for an RK context: every entry is inside the column (the padding
+    // beyond d is zero and stays zero), so the loads and stores need no guard
 #pragma unroll
     for (int j = 0; j < E; j++) {
-        const int i = lane * E + j;
-        pr[j] = i < a.d ? col[i] : 0.0;
-        pi[j] = i < a.d ? col[a.dp + i] : 0.0;
+        pr[j] = col[lane * E + j];
+        pi[j] = col[a.dp + lane * E + j];
     }
     Halo h;
     h.o1re = lane ? rk_lds[(kRkO1re * E + E - 1) * 64 + lane - 1] : 0.0;
@@ -412,11 +413,8 @@ void rk_kernel(RkArgs a)
     s.t = t;
 #pragma unroll
     for (int j = 0; j < E; j++) {
-        const int i = lane * E + j;
-        if (i < a.d) {
-            col[i] = pr[j];
-            col[a.dp + i] = pi[j];
-        }
+        col[lane * E + j] = pr[j];
+        col[a.dp + lane * E + j] = pi[j];
     }
     if (lane == 0) {
         a.st[tr] = s;
@@ -447,6 +445,7 @@ hipError_t launch_rk_e(const RkArgs &a, hipStream_t s)
 hipError_t launch_rk(const RkArgs &a, hipStream_t s)
 {
     if (a.M <= 0 || a.nsteps <= 0) return hipSuccess;
+    if (a.dp != 64 * rk_lanes(a.d) || a.L != 2 * a.dp) return hipErrorInvalidValue;
     switch (rk_lanes(a.d)) {
     case 2: return launch_rk_e<2>(a, s);
     case 4: return launch_rk_e<4>(a, s);

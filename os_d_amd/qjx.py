@@ -107,6 +107,7 @@ SIGNATURES = [
     ("qjx_set_propagator", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D]),
     ("qjx_get_stats", ctypes.c_int, [ctypes.c_void_p, _P(Stats)]),
     ("qjx_dim", ctypes.c_int, [ctypes.c_void_p]),
+    ("qjx_peak_probe", ctypes.c_int, [_D, _D, _D, _D, _D]),
     ("qjx_set_profiling", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("qjx_mfma_probe", ctypes.c_int, [_P(ctypes.c_float), _P(ctypes.c_float), _P(ctypes.c_float), _D, _D,
                                       _D, _P(ctypes.c_float), _P(ctypes.c_float), _P(ctypes.c_float)]),
@@ -548,3 +549,13 @@ class Lpn:
         R = np.zeros((d, d), dtype=np.complex128)
         _check(self.lib, self.lib.qjx_lpn_random_obs_matrix(self.h, obs, _ptr(R.view(np.float64))))
         return R
+
+
+def peak_probe():
+    """Measured device ceilings: FP32 / FP64 / BF16 MFMA and FP64 vector TFLOP/s,
+    HBM copy GB/s."""
+    lib = load_library()
+    v = [ctypes.c_double() for _ in range(5)]
+    _check(lib, lib.qjx_peak_probe(*[ctypes.byref(x) for x in v]))
+    return dict(zip(("f32_mfma_tflops", "f64_mfma_tflops", "bf16_mfma_tflops", "f64_valu_tflops", "hbm_gbps"),
+                    (x.value for x in v)))
