@@ -248,8 +248,19 @@ __device__ inline KRange tile_krange(const Ens &E, const TileRef &r)
     return k;
 }
 
+// the same, held in scalar registers: every lane of the workgroup reads the same
+// table entry, so the K loop's bounds and block offsets are wave-uniform
+__device__ inline KRange tile_krange_uniform(const Ens &E, const TileRef &r)
+{
+    KRange k = tile_krange(E, r);
+    k.kb0 = __builtin_amdgcn_readfirstlane(k.kb0);
+    k.nb = __builtin_amdgcn_readfirstlane(k.nb);
+    k.hb = __builtin_amdgcn_readfirstlane(k.hb);
+    return k;
+}
+
 template <int BN, int NWN>
-__global__ __launch_bounds__(128 * NWN) void gemm_f32(RunArgs a)
+__global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) void gemm_f32(RunArgs a)
 {
     constexpr int T = 128 * NWN;         // 2 x NWN waves
     constexpr int NJ = BN / (32 * NWN);  // 32-column MFMA tiles per wave
@@ -291,57 +302,77 @@ __global__ __launch_bounds__(128 * NWN) void gemm_f32(RunArgs a)
             for (int j = 0; j < NJ; j++)
 #pragma unroll
                 for (int e = 0; e < 16; e++) acc[i][j][e] = 0.f;
-        floatx4 ra[QA], rb[NQ];
-        auto gload = [&](int k0) {
+        // two register stages, as in the 3xBF16 kernel: block kb is multiplied out of
+        // LDS while block kb + 1 waits in registers and block kb + 2 is in flight
+        struct Stage {
+            floatx4 a[QA], b[NQ];
+        } st0, st1;
+        auto gload = [&](Stage &sg, int k0) {
 #pragma unroll
-            for (int q = 0; q < QA; q++) ra[q] = gld<floatx4>(arow + q * astep + k0);
+            for (int q = 0; q < QA; q++) sg.a[q] = gld<floatx4>(arow + q * astep + k0);
 #pragma unroll
-            for (int q = 0; q < NQ; q++) rb[q] = gld<floatx4>(bcol[q] + k0);
+            for (int q = 0; q < NQ; q++) sg.b[q] = gld<floatx4>(bcol[q] + k0);
         };
-        auto lstore = [&](int buf) {
+        auto lstore = [&](const Stage &sg, int buf) {
 #pragma unroll
             for (int q = 0; q < QA; q++)
-                *reinterpret_cast<floatx4 *>(&As[buf][(cp_r + (T / 8) * q) * F32_PITCH + cp_k]) = ra[q];
+                *reinterpret_cast<floatx4 *>(&As[buf][(cp_r + (T / 8) * q) * F32_PITCH + cp_k]) = sg.a[q];
 #pragma unroll
             for (int q = 0; q < NQ; q++)
-                *reinterpret_cast<floatx4 *>(&Bs[buf][(cp_r + (T / 8) * q) * F32_PITCH + cp_k]) = rb[q];
+                *reinterpret_cast<floatx4 *>(&Bs[buf][(cp_r + (T / 8) * q) * F32_PITCH + cp_k]) = sg.b[q];
         };
-        const KRange kr = tile_krange(E, r);
-        const int nk = kr.count();
-        if (nk > 0) {
-            gload(kr.at(0));
-            lstore(0);
-        }
-        __syncthreads();
-        int cur = 0;
-        for (int kb = 0; kb < nk; kb++) {
-            const bool more = kb + 1 < nk;
-            if (more) gload(kr.at(kb + 1));
-            floatx4 fa[2][4], fb[NJ][4];
+        // fragments in four k slices of 4 MFMA steps each (steps 4v..4v+3 read one
+        // 16-byte LDS word per operand and lane): fewer live registers, same order
+        auto mma = [&](int buf) {
 #pragma unroll
-            for (int i = 0; i < 2; i++) {
-                const float *pa = &As[cur][(wm * 64 + i * 32 + li) * F32_PITCH + 16 * h];
-#pragma unroll
-                for (int v = 0; v < 4; v++) fa[i][v] = *reinterpret_cast<const floatx4 *>(pa + 4 * v);
-            }
-#pragma unroll
-            for (int j = 0; j < NJ; j++) {
-                const float *pb = &Bs[cur][(wn * (BN / NWN) + j * 32 + li) * F32_PITCH + 16 * h];
-#pragma unroll
-                for (int v = 0; v < 4; v++) fb[j][v] = *reinterpret_cast<const floatx4 *>(pb + 4 * v);
-            }
-#pragma unroll
-            for (int s = 0; s < 16; s++)
+            for (int v = 0; v < 4; v++) {
+                floatx4 fa[2], fb[NJ];
 #pragma unroll
                 for (int i = 0; i < 2; i++)
+                    fa[i] = *reinterpret_cast<const floatx4 *>(
+                        &As[buf][(wm * 64 + i * 32 + li) * F32_PITCH + 16 * h + 4 * v]);
 #pragma unroll
-                    for (int j = 0; j < NJ; j++)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[i][s >> 2][s & 3],
-                                                                          fb[j][s >> 2][s & 3], acc[i][j], 0, 0, 0);
-            // the other buffer was last read before the previous barrier
-            if (more) lstore(cur ^ 1);
+                for (int j = 0; j < NJ; j++)
+                    fb[j] = *reinterpret_cast<const floatx4 *>(
+                        &Bs[buf][(wn * (BN / NWN) + j * 32 + li) * F32_PITCH + 16 * h + 4 * v]);
+#pragma unroll
+                for (int e = 0; e < 4; e++)
+#pragma unroll
+                    for (int i = 0; i < 2; i++)
+#pragma unroll
+                        for (int j = 0; j < NJ; j++)
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[i][e], fb[j][e], acc[i][j], 0, 0, 0);
+            }
+        };
+        const KRange kr = tile_krange_uniform(E, r);
+        const int nk = kr.count();
+        if (nk > 0) {
+            // refills past the last block reload the last block (in bounds, never
+            // stored), so every iteration issues the same loads and the wait before
+            // each stage's LDS store covers only that stage
+            auto blk = [&](int kb) { return kr.at(min(kb, nk - 1)); };
+            // issued stage by stage (not interleaved), as the loop body expects them
+            gload(st0, blk(0));
+            __builtin_amdgcn_sched_barrier(0);
+            gload(st1, blk(1));
+            __builtin_amdgcn_sched_barrier(0);
+            lstore(st0, 0);
+            gload(st0, blk(2));
             __syncthreads();
-            cur ^= 1;
+            auto iter = [&](Stage &nxt, int kb) {
+                const int buf = kb & 1;
+                __builtin_amdgcn_sched_barrier(0);
+                lstore(nxt, buf ^ 1);
+                gload(nxt, blk(kb + 3));
+                mma(buf);
+                __syncthreads();
+            };
+            int kb = 0;
+            for (; kb + 1 < nk; kb += 2) {
+                iter(st1, kb);
+                iter(st0, kb + 1);
+            }
+            if (kb < nk) iter(st1, kb);
         }
         double ns[NJ];
 #pragma unroll
@@ -441,24 +472,28 @@ __global__ __launch_bounds__(128 * NWN) void gemm_bf16x3(RunArgs a)
             for (int j = 0; j < NJ; j++)
 #pragma unroll
                 for (int e = 0; e < 16; e++) acc[i][j][e] = 0.f;
-        bf16x8 rah[QA], ral[QA];
-        floatx4 rb[NQ];
-        auto gload = [&](int k0) {
+        // two register stages: while block kb is multiplied out of LDS, block kb + 1
+        // waits in registers for the other LDS buffer and block kb + 2 is in flight
+        struct Stage {
+            bf16x8 ah[QA], al[QA];
+            floatx4 b[NQ];
+        } st0, st1;
+        auto gload = [&](Stage &sg, int k0) {
 #pragma unroll
             for (int q = 0; q < QA; q++) {
                 const size_t o = (size_t)(r0 + ca_r + (T / 4) * q) * L + k0 + ca_k;
-                rah[q] = gld<bf16x8>(Uh + o);
-                ral[q] = gld<bf16x8>(Ul + o);
+                sg.ah[q] = gld<bf16x8>(Uh + o);
+                sg.al[q] = gld<bf16x8>(Ul + o);
             }
 #pragma unroll
-            for (int q = 0; q < NQ; q++) rb[q] = gld<floatx4>(bcol[q] + k0);
+            for (int q = 0; q < NQ; q++) sg.b[q] = gld<floatx4>(bcol[q] + k0);
         };
-        auto lstore = [&](int buf) {
+        auto lstore = [&](const Stage &sg, int buf) {
 #pragma unroll
             for (int q = 0; q < QA; q++) {
                 const int rr = ca_r + (T / 4) * q;
-                *reinterpret_cast<bf16x8 *>(&Ah[buf][bf_off(rr, ca_k)]) = rah[q];
-                *reinterpret_cast<bf16x8 *>(&Al[buf][bf_off(rr, ca_k)]) = ral[q];
+                *reinterpret_cast<bf16x8 *>(&Ah[buf][bf_off(rr, ca_k)]) = sg.ah[q];
+                *reinterpret_cast<bf16x8 *>(&Al[buf][bf_off(rr, ca_k)]) = sg.al[q];
             }
 #pragma unroll
             for (int q = 0; q < NQ; q++) {
@@ -466,7 +501,7 @@ __global__ __launch_bounds__(128 * NWN) void gemm_bf16x3(RunArgs a)
                 bf16x4 h4, l4;
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
-                    const float x = rb[q][e];
+                    const float x = sg.b[q][e];
                     const __bf16 hb = (__bf16)x;
                     h4[e] = hb;
                     l4[e] = (__bf16)(x - (float)hb);
@@ -475,31 +510,21 @@ __global__ __launch_bounds__(128 * NWN) void gemm_bf16x3(RunArgs a)
                 *reinterpret_cast<bf16x4 *>(&Bl[buf][bf_off(rr, cb_k)]) = l4;
             }
         };
-        const KRange kr = tile_krange(E, r);
-        const int nk = kr.count();
-        if (nk > 0) {
-            gload(kr.at(0));
-            lstore(0);
-        }
-        __syncthreads();
-        int cur = 0;
-        for (int kb = 0; kb < nk; kb++) {
-            const bool more = kb + 1 < nk;
-            if (more) gload(kr.at(kb + 1));
+        auto mma = [&](int buf) {
 #pragma unroll
             for (int c = 0; c < MF_BK / 16; c++) {
                 bf16x8 fah[2], fal[2], fbh[NJ], fbl[NJ];
 #pragma unroll
                 for (int i = 0; i < 2; i++) {
                     const int oa = bf_off(wm * 64 + i * 32 + li, 16 * c + 8 * h);
-                    fah[i] = *reinterpret_cast<const bf16x8 *>(&Ah[cur][oa]);
-                    fal[i] = *reinterpret_cast<const bf16x8 *>(&Al[cur][oa]);
+                    fah[i] = *reinterpret_cast<const bf16x8 *>(&Ah[buf][oa]);
+                    fal[i] = *reinterpret_cast<const bf16x8 *>(&Al[buf][oa]);
                 }
 #pragma unroll
                 for (int j = 0; j < NJ; j++) {
                     const int ob = bf_off(wn * (BN / NWN) + j * 32 + li, 16 * c + 8 * h);
-                    fbh[j] = *reinterpret_cast<const bf16x8 *>(&Bh[cur][ob]);
-                    fbl[j] = *reinterpret_cast<const bf16x8 *>(&Bl[cur][ob]);
+                    fbh[j] = *reinterpret_cast<const bf16x8 *>(&Bh[buf][ob]);
+                    fbl[j] = *reinterpret_cast<const bf16x8 *>(&Bl[buf][ob]);
                 }
                 // the three products go round the 2 x NJ accumulators in turn, so
                 // consecutive MFMAs never depend on each other
@@ -519,9 +544,40 @@ __global__ __launch_bounds__(128 * NWN) void gemm_bf16x3(RunArgs a)
                     for (int j = 0; j < NJ; j++)
                         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fah[i], fbh[j], acc[i][j], 0, 0, 0);
             }
-            if (more) lstore(cur ^ 1);
+        };
+        const KRange kr = tile_krange_uniform(E, r);
+        const int nk = kr.count();
+        if (nk > 0) {
+            // refills past the last block reload the last block (in bounds, never
+            // stored), so every iteration issues the same loads and the waits before
+            // each stage's LDS store cover only that stage
+            auto blk = [&](int kb) { return kr.at(min(kb, nk - 1)); };
+            // issued stage by stage (not interleaved), as the loop body expects them
+            gload(st0, blk(0));
+            __builtin_amdgcn_sched_barrier(0);
+            gload(st1, blk(1));
+            __builtin_amdgcn_sched_barrier(0);
+            lstore(st0, 0);
+            gload(st0, blk(2));
             __syncthreads();
-            cur ^= 1;
+            // iteration kb: store block kb + 1 (its LDS buffer was last read before the
+            // previous barrier), refill its registers with block kb + 3, multiply block kb
+            auto iter = [&](Stage &nxt, int kb) {
+                const int buf = kb & 1;
+                // keep the next stage's conversions out of this iteration: hoisted,
+                // they would wait for loads that are meant to stay in flight
+                __builtin_amdgcn_sched_barrier(0);
+                lstore(nxt, buf ^ 1);
+                gload(nxt, blk(kb + 3));
+                mma(buf);
+                __syncthreads();
+            };
+            int kb = 0;
+            for (; kb + 1 < nk; kb += 2) {
+                iter(st1, kb);
+                iter(st0, kb + 1);
+            }
+            if (kb < nk) iter(st1, kb);
         }
         double ns[NJ];
 #pragma unroll
