@@ -569,7 +569,7 @@ static void top_steps(qjo_ens *e, int tr, int n, int is_obs)
 
 static int rk_lanes(int d)
 {
-    int E = 2;
+    int E = 1;
     while (64 * E < d) E *= 2;
     return E;
 }

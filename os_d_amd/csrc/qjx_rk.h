@@ -23,10 +23,11 @@ enum RkBand {
 };
 constexpr int kRkStageBands = kRkLd;   // arrays staged in LDS (the stencil's)
 
-// Entries per lane: the smallest power of two >= 2 with 64 E >= d.
+// Entries per lane: the smallest power of two with 64 E >= d (one entry per lane up
+// to d = 64).
 inline int rk_lanes(int d)
 {
-    int E = 2;
+    int E = 1;
     while (64 * E < d) E *= 2;
     return E;
 }

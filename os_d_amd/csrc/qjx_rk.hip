@@ -133,10 +133,22 @@ __device__ inline void rk_stage(double (&xr)[E], double (&xi)[E], const double (
     // stage instead of being hoisted out of the step loop into registers.
     int lo = lane;
     asm volatile("" : "+v"(lo));
-    const double l2r = shfl_up1(xr[E - 2], lane), l2i = shfl_up1(xi[E - 2], lane);
+    // entries lE-2, lE-1 (left) and lE+E, lE+E+1 (right); with one entry per lane
+    // the second neighbours are two lanes away
     const double l1r = shfl_up1(xr[E - 1], lane), l1i = shfl_up1(xi[E - 1], lane);
     const double r1r = shfl_down1(xr[0], lane), r1i = shfl_down1(xi[0], lane);
-    const double r2r = shfl_down1(xr[1], lane), r2i = shfl_down1(xi[1], lane);
+    double l2r, l2i, r2r, r2i;
+    if constexpr (E == 1) {
+        l2r = shfl_up1(l1r, lane);
+        l2i = shfl_up1(l1i, lane);
+        r2r = shfl_down1(r1r, lane);
+        r2i = shfl_down1(r1i, lane);
+    } else {
+        l2r = shfl_up1(xr[E - 2], lane);
+        l2i = shfl_up1(xi[E - 2], lane);
+        r2r = shfl_down1(xr[1], lane);
+        r2i = shfl_down1(xi[1], lane);
+    }
     double wr2 = l2r, wi2 = l2i, wr1 = l1r, wi1 = l1i;
     double c1re = h.o1re, c1im = h.o1im, c2a = h.o2a, c2b = h.o2b;
 #pragma unroll
@@ -281,7 +293,11 @@ void rk_kernel(RkArgs a)
     Halo h;
     h.o1re = lane ? rk_lds[(kRkO1re * E + E - 1) * 64 + lane - 1] : 0.0;
     h.o1im = lane ? rk_lds[(kRkO1im * E + E - 1) * 64 + lane - 1] : 0.0;
-    h.o2a = lane ? rk_lds[(kRkO2im * E + E - 2) * 64 + lane - 1] : 0.0;
+    // H_part(lE-2, lE): entry E-2 of lane l-1, or with one entry per lane lane l-2's
+    if constexpr (E == 1)
+        h.o2a = lane >= 2 ? rk_lds[kRkO2im * 64 + lane - 2] : 0.0;
+    else
+        h.o2a = lane ? rk_lds[(kRkO2im * E + E - 2) * 64 + lane - 1] : 0.0;
     h.o2b = lane ? rk_lds[(kRkO2im * E + E - 1) * 64 + lane - 1] : 0.0;
     const double *Ld = a.bands + (size_t)kRkLd * E * 64;
     const double *Lm = a.bands + (size_t)kRkLm * E * 64;
@@ -447,6 +463,7 @@ hipError_t launch_rk(const RkArgs &a, hipStream_t s)
     if (a.M <= 0 || a.nsteps <= 0) return hipSuccess;
     if (a.dp != 64 * rk_lanes(a.d) || a.L != 2 * a.dp) return hipErrorInvalidValue;
     switch (rk_lanes(a.d)) {
+    case 1: return launch_rk_e<1>(a, s);
     case 2: return launch_rk_e<2>(a, s);
     case 4: return launch_rk_e<4>(a, s);
     case 8: return launch_rk_e<8>(a, s);

@@ -51,8 +51,11 @@ def _same(o, e, M, logs=True):
 
 
 @pytest.mark.parametrize("N,A,gamma,drv_type,rk_ns,M,tp,mode,amount", [
-    (20, 1.5, 0.3, 0, 1000, 96, 1, "periods", 2),      # E = 2, square drive
-    (20, 2.0, 0.2, 1, 1000, 96, 1, "periods", 2),      # E = 2, sine drive
+    (1, 1.5, 0.3, 1, 1000, 64, 1, "periods", 2),       # d = 2: E = 1, halo from zero lanes
+    (20, 1.5, 0.3, 0, 1000, 96, 1, "periods", 2),      # E = 1, square drive
+    (20, 2.0, 0.2, 1, 1000, 96, 1, "periods", 2),      # E = 1, sine drive
+    (63, 1.5, 0.2, 1, 2000, 64, 0, "periods", 1),      # d = 64: E = 1, every lane used
+    (64, 1.5, 0.2, 1, 2000, 64, 0, "periods", 1),      # d = 65: E = 2
     (100, 1.5, 0.1, 1, 4000, 64, 0, "periods", 1),     # E = 2
     (200, 1.5, 0.1, 0, 8000, 32, 0, "steps", 3000),    # E = 4, across the half period
     (300, 1.5, 0.1, 1, 12000, 32, 0, "steps", 3000),   # E = 8
