@@ -305,7 +305,7 @@ struct qjo_ens {
     std::vector<std::vector<int>> jtype;
     std::vector<int64_t> matvecs, jumps_done;
     /* Runge-Kutta propagation (prop_id 1), set by qjo_rk_setup */
-    int prop = 0, rk_ns = 0, drv_type = 0, lanes_E = 2;
+    int prop = 0, rk_ns = 0, drv_type = 0;
     double drv_phase = 0.0, rk_step = 0.0;
     std::vector<double> b_dre, b_dim, b_o1re, b_o1im, b_o2im, b_drv, b_Ld, b_Lm, b_Lp;
     struct Lpn *lpn = nullptr;              /* LPN bookkeeping, set by qjo_lpn_setup */
@@ -567,17 +567,25 @@ static void top_steps(qjo_ens *e, int tr, int n, int is_obs)
 /* (redrawn while 0) and integrates the rest (recursively: a loop here).      */
 /* ------------------------------------------------------------------------- */
 
+/* The GPU kernel's layout fixes the summation tree of ||x||^2: P lanes per
+ * trajectory (32 for 64 < d <= 128, else 64), lane l holding entries
+ * [l E, l E + E) with E the smallest power of two such that P E >= d; each lane sums
+ * its entries with fma, then the butterfly adds partners at lane distances P/2..1. */
+static int rk_group(int d) { return d > 64 && d <= 128 ? 32 : 64; }
+
 static int rk_lanes(int d)
 {
+    const int P = rk_group(d);
     int E = 1;
-    while (64 * E < d) E *= 2;
+    while (P * E < d) E *= 2;
     return E;
 }
 
-static double rk_norm(const cplx *x, int d, int E)
+static double rk_norm(const cplx *x, int d)
 {
+    const int P = rk_group(d), E = rk_lanes(d);
     double part[64];
-    for (int l = 0; l < 64; l++) {
+    for (int l = 0; l < P; l++) {
         double p = 0.0;
         for (int j = 0; j < E && l * E + j < d; j++) {
             const cplx &v = x[l * E + j];
@@ -586,10 +594,10 @@ static double rk_norm(const cplx *x, int d, int E)
         }
         part[l] = p;
     }
-    for (int off = 32; off > 0; off >>= 1) {
+    for (int off = P / 2; off > 0; off >>= 1) {
         double nxt[64];
-        for (int l = 0; l < 64; l++) nxt[l] = part[l] + part[l ^ off];
-        std::memcpy(part, nxt, sizeof part);
+        for (int l = 0; l < P; l++) nxt[l] = part[l] + part[l ^ off];
+        std::memcpy(part, nxt, sizeof(double) * P);
     }
     return part[0];
 }
@@ -730,7 +738,7 @@ static void rk_recovery(qjo_ens *e, int tr, cplx *phi, double nsq, int is_obs, R
         }
         res[i] = cplx(re, im);
     }
-    const double g = rk_norm(res, d, e->lanes_E) * 1.0;
+    const double g = rk_norm(res, d) * 1.0;
     const double sg = std::sqrt(g / 1.0);
     for (int i = 0; i < d; i++) phi[i] = cplx(res[i].real() / sg, res[i].imag() / sg);
     if (e->jump > 0 && is_obs == 1) e->jtype[tr].push_back(0);
@@ -741,15 +749,15 @@ static void rk_recovery(qjo_ens *e, int tr, cplx *phi, double nsq, int is_obs, R
  * of the current state between steps (recomputing it gives the same bits). */
 static void rk_step(qjo_ens *e, int tr, double step, int is_obs, double &cached, bool &have, RkWork &w)
 {
-    const int d = e->m.d, E = e->lanes_E;
+    const int d = e->m.d;
     cplx *phi = &e->psi[(size_t)tr * d];
     double &t = e->t[tr];
     double &eta = e->eta[tr];
     for (int crossings = 0;; crossings++) {
-        const double prev = have ? cached : rk_norm(phi, d, E);
+        const double prev = have ? cached : rk_norm(phi, d);
         rk_int(e, t, step, phi, w.nw.data(), w);
         e->matvecs[tr] += 4;
-        const double n = rk_norm(w.nw.data(), d, E);
+        const double n = rk_norm(w.nw.data(), d);
         if (n > 1.0) {                                      /* experiment.cpp:1247-1251 */
             char buf[160];
             std::snprintf(buf, sizeof buf, "norm more than 1 (trajectory %d, norm %.17g)", tr, n);
@@ -775,7 +783,7 @@ static void rk_step(qjo_ens *e, int tr, double step, int is_obs, double &cached,
         rk_int(e, t, bstep, phi, w.nw.data(), w);
         e->matvecs[tr] += 4;
         std::memcpy(phi, w.nw.data(), sizeof(cplx) * d);
-        const double jn = rk_norm(phi, d, E);
+        const double jn = rk_norm(phi, d);
         if (e->jump > 0 && is_obs == 1) {
             e->jt[tr].push_back(t);
             e->jn[tr].push_back(jn);
@@ -833,7 +841,6 @@ extern "C" int qjo_rk_setup(qjo_ens *e, int rk_ns, int drv_type, double drv_phas
     e->drv_type = drv_type;
     e->drv_phase = drv_phase;
     e->rk_step = m.T / double(rk_ns);                     /* rk_proc.cpp:62 */
-    e->lanes_E = rk_lanes(d);
     e->b_dre.assign(d, 0.0); e->b_dim.assign(d, 0.0); e->b_drv.assign(d, 0.0);
     e->b_o1re.assign(d, 0.0); e->b_o1im.assign(d, 0.0); e->b_o2im.assign(d, 0.0);
     e->b_Ld.assign(d, 0.0); e->b_Lm.assign(d, 0.0); e->b_Lp.assign(d, 0.0);
@@ -867,7 +874,7 @@ extern "C" int qjo_rk_rhs(const qjo_ens *e, double t, const double *x, double *k
 
 extern "C" double qjo_rk_norm(int d, const double *x)
 {
-    return rk_norm(reinterpret_cast<const cplx *>(x), d, rk_lanes(d));
+    return rk_norm(reinterpret_cast<const cplx *>(x), d);
 }
 
 extern "C" int qjo_rk_int(qjo_ens *e, int tr, double step)

@@ -345,9 +345,9 @@ int qjx::ctx_create_kind(const qjx_model_desc *m, int gemm_kind, qjx_ctx **out)
 
 // H_part = H - i/2 gamma' L^dag L (newdel.cpp:2327-2352) on its band |i - j| <= 2,
 // H_drv and L, in the RK kernel's lane-major layout: entry i = l E + j of array a
-// at (a E + j) 64 + l, zero beyond d.  (L^dag L)_ij sums L_ki L_kj in ascending k
+// at (a E + j) P + l (P lanes per trajectory), zero beyond d.  (L^dag L)_ij sums L_ki L_kj in ascending k
 // as build_model does, so H_part(i, j) and H_part(j, i) are the same number.
-static void build_rk_bands(const qjx_ctx *c, int E, std::vector<double> &out)
+static void build_rk_bands(const qjx_ctx *c, int E, int P, std::vector<double> &out)
 {
     const int d = c->d;
     const double g = c->model.diss_gamma / double(d - 1);
@@ -369,8 +369,8 @@ static void build_rk_bands(const qjx_ctx *c, int E, std::vector<double> &out)
         *re = h + 0.5 * 0.0;        // Im(diss_part) = 0
         *im = 0.0 - 0.5 * ldl;
     };
-    out.assign((size_t)qjx::kRkBands * E * 64, 0.0);
-    auto at = [&](int a, int i) -> double & { return out[((size_t)a * E + i % E) * 64 + i / E]; };
+    out.assign((size_t)qjx::kRkBands * E * P, 0.0);
+    auto at = [&](int a, int i) -> double & { return out[((size_t)a * E + i % E) * P + i / E]; };
     for (int i = 0; i < d; i++) {
         double re, im;
         hpart(i, i, &re, &im);
@@ -408,7 +408,7 @@ extern "C" int qjx_ctx_create_rk(const qjx_model_desc *m, const qjx_rk_desc *rk,
     c->model = *m;
     c->prop = 1;
     c->d = m->N + 1;
-    c->dp = 64 * qjx::rk_lanes(c->d);   // whole lane chunks: the RK kernel reads and writes 64 E entries
+    c->dp = qjx::rk_group(c->d) * qjx::rk_lanes(c->d);   // whole lane chunks: the RK kernel reads and writes P E entries
     c->L = 2 * c->dp;
     c->D = 1;
     c->S = 2;
@@ -436,7 +436,7 @@ extern "C" int qjx_ctx_create_rk(const qjx_model_desc *m, const qjx_rk_desc *rk,
     if ((rc = upload((void **)&c->d_HdiagE, c->HdiagE.data(), sizeof(double) * c->d))) return bail(rc);
     if ((rc = upload((void **)&c->d_Hoff, c->Hoff.data(), sizeof(double) * c->Hoff.size()))) return bail(rc);
     std::vector<double> bands;
-    build_rk_bands(c, qjx::rk_lanes(c->d), bands);
+    build_rk_bands(c, qjx::rk_lanes(c->d), qjx::rk_group(c->d), bands);
     if ((rc = upload((void **)&c->d_rkbands, bands.data(), sizeof(double) * bands.size()))) return bail(rc);
     *out = c;
     return QJX_OK;

@@ -7,8 +7,9 @@
 
 namespace qjx {
 
-// Band coefficient arrays of the RK right-hand side, each 64 E doubles in
-// lane-major order: element i = l E + j (lane l, slot j) sits at (a E + j) 64 + l.
+// Band coefficient arrays of the RK right-hand side, each P E doubles in
+// lane-major order: element i = l E + j (lane l of a trajectory's P, slot j) sits at
+// (a E + j) P + l.
 enum RkBand {
     kRkDre = 0,   // Re H_part(i, i)
     kRkDim,       // Im H_part(i, i)
@@ -23,20 +24,26 @@ enum RkBand {
 };
 constexpr int kRkStageBands = kRkLd;   // arrays staged in LDS (the stencil's)
 
-// Entries per lane: the smallest power of two with 64 E >= d (one entry per lane up
-// to d = 64).
+// Lanes per trajectory: 32 (two trajectories per wavefront) for 64 < d <= 128, where
+// one wavefront of two entries per lane would spend as much on its cross-lane halo
+// and norm as on the stencil; 64 otherwise.
+inline int rk_group(int d) { return d > 64 && d <= 128 ? 32 : 64; }
+
+// Entries per lane: the smallest power of two with P E >= d (one entry per lane up
+// to d = 64, four from d = 65 with 32 lanes).
 inline int rk_lanes(int d)
 {
+    const int P = rk_group(d);
     int E = 1;
-    while (64 * E < d) E *= 2;
+    while (P * E < d) E *= 2;
     return E;
 }
 
 struct RkArgs {
-    const double *bands;      // [kRkBands][E][64]
+    const double *bands;      // [kRkBands][E][P]
     double *tab;              // [3 nsteps]: drives of each step's full integration (scratch)
     void *slot0, *slot1;      // FP64 state columns (L doubles: Re at [0, dp), Im at [dp, 2 dp));
-                              // dp = 64 E, the padding beyond d zero
+                              // dp = P E, the padding beyond d zero
     TrajState *st;
     JumpLog log;
     int *err;                 // [4]: code, trajectory, bits of the offending norm
@@ -49,7 +56,7 @@ struct RkArgs {
     int jump, is_obs;
 };
 
-// Advance every trajectory by a.nsteps RK steps (one wavefront per trajectory);
+// Advance every trajectory by a.nsteps RK steps (P lanes per trajectory);
 // a.tab must hold 3 a.nsteps doubles.  Returns hipErrorInvalidValue if d exceeds
 // the kernel's largest lane width.
 hipError_t launch_rk(const RkArgs &a, hipStream_t s);

@@ -8,18 +8,21 @@
 // The reference evaluates k = -i (H_part + E(t) H_drv) phi with two dense zgemv
 // per RK stage.  H_part = H - i/2 gamma' L^dag L is pentadiagonal and H_drv is
 // diagonal, so every other term is an exact zero: the stage is a five-point
-// stencil, bandwidth- and latency-bound, not a GEMM.  One wavefront owns one
-// trajectory for a whole launch: lane l holds entries [l E, l E + E) of phi, of
-// the stage argument and of the RK accumulator in registers; the stencil's
-// neighbours across lanes come from four cross-lane shuffles per stage; the
-// band coefficients (shared by all trajectories) sit in LDS, lane-major so a
-// wavefront's reads are conflict-free; ||phi||^2 is a lane sum plus a butterfly.
+// stencil, bandwidth- and latency-bound, not a GEMM.  P lanes own one trajectory
+// for a whole launch (P = 64, or 32 for 64 < d <= 128: two trajectories per
+// wavefront): lane l of the group holds entries [l E, l E + E) of phi, of the stage
+// argument and of the RK accumulator in registers; the stencil's neighbours across
+// lanes come from four cross-lane shuffles per stage (wavefront-wide: at the edge
+// between two trajectories the shifted-in values meet band coefficients that are
+// exact zeros, those of columns outside 0..d-1); the band coefficients (shared by
+// all trajectories) sit in LDS, lane-major so a wavefront's reads are
+// conflict-free; ||phi||^2 is a lane sum plus a butterfly over the group.
 // Nothing touches HBM between the first load and the final store of the state
 // except jump records.
 //
 // Arithmetic order (bit-identical to the CPU restatement): one fma per
 // multiply-add, terms in ascending column order, RK4 update phi + acc (step/6),
-// ||x||^2 summed per lane then by the butterfly over lane distances 32..1.
+// ||x||^2 summed per lane then by the butterfly over lane distances P/2..1.
 // Nothing else is contracted.
 #include <hip/hip_runtime.h>
 
@@ -98,8 +101,10 @@ __device__ inline double shfl_down1(double v, int)
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
-template <int E>
-__device__ inline double wave_norm(const double (&xr)[E], const double (&xi)[E])
+// ||x||^2 of the group's trajectory: lane sums, then the butterfly over lane
+// distances P/2..1 (every lane of the group gets the same bits)
+template <int E, int P>
+__device__ inline double group_norm(const double (&xr)[E], const double (&xi)[E])
 {
     double p = 0.0;
 #pragma unroll
@@ -107,7 +112,12 @@ __device__ inline double wave_norm(const double (&xr)[E], const double (&xi)[E])
         p = __builtin_fma(xr[j], xr[j], p);
         p = __builtin_fma(xi[j], xi[j], p);
     }
-    return wave_sum(p);
+    if constexpr (P == 64) {
+        return wave_sum(p);
+    } else {
+        for (int off = P / 2; off > 0; off >>= 1) p += __shfl_xor(p, off, 64);
+        return p;
+    }
 }
 
 // Coefficients of the stencil this lane needs from its left neighbour lane:
@@ -124,14 +134,14 @@ enum { kStageFirst = 1, kStageMid = 2, kStageLast = 4 };
 //   kStageLast:  x = phi + (acc + k) c (c = step / 6)
 // x is updated in place: the stencil of entry j reads the original x[j-2], x[j-1]
 // from a two-entry window and x[j+1], x[j+2] before they are overwritten.
-template <int E, int MODE>
+template <int E, int P, int MODE>
 __device__ inline void rk_stage(double (&xr)[E], double (&xi)[E], const double (&pr)[E], const double (&pi)[E],
                                 double (&ar)[E], double (&ai)[E], double Ed, double c, const double *lds,
-                                const Halo &h, int lane)
+                                const Halo &h, int lane, int ll)
 {
     // An offset the compiler cannot see through: the band reads stay inside the
     // stage instead of being hoisted out of the step loop into registers.
-    int lo = lane;
+    int lo = ll;
     asm volatile("" : "+v"(lo));
     // entries lE-2, lE-1 (left) and lE+E, lE+E+1 (right); with one entry per lane
     // the second neighbours are two lanes away
@@ -158,12 +168,12 @@ __device__ inline void rk_stage(double (&xr)[E], double (&xi)[E], const double (
         const double p1i = j + 1 < E ? xi[j + 1] : r1i;
         const double p2r = j + 2 < E ? xr[j + 2] : (j + 2 == E ? r1r : r2r);
         const double p2i = j + 2 < E ? xi[j + 2] : (j + 2 == E ? r1i : r2i);
-        const double dre = lds[(kRkDre * E + j) * 64 + lo];
-        const double dim = lds[(kRkDim * E + j) * 64 + lo];
-        const double o1re = lds[(kRkO1re * E + j) * 64 + lo];
-        const double o1im = lds[(kRkO1im * E + j) * 64 + lo];
-        const double o2 = lds[(kRkO2im * E + j) * 64 + lo];
-        const double drv = lds[(kRkDrv * E + j) * 64 + lo];
+        const double dre = lds[(kRkDre * E + j) * P + lo];
+        const double dim = lds[(kRkDim * E + j) * P + lo];
+        const double o1re = lds[(kRkO1re * E + j) * P + lo];
+        const double o1im = lds[(kRkO1im * E + j) * P + lo];
+        const double o2 = lds[(kRkO2im * E + j) * P + lo];
+        const double drv = lds[(kRkDrv * E + j) * P + lo];
         double re = 0.0, im = 0.0;
         re = __builtin_fma(-c2a, wi2, re);          // column i-2 (purely imaginary entry)
         im = __builtin_fma(c2a, wr2, im);
@@ -244,11 +254,11 @@ __global__ void rk_drive_table_kernel(RkArgs a)
 // rk_int_dimer: x = RK4(phi, step) with stage drives e0, e1, e2.  The stage
 // arguments use the global rk_step whatever step is (arg_upd reads ed->rk_step,
 // rk_proc.cpp:150-173).
-template <int E>
+template <int E, int P>
 __device__ inline void rk_integrate(const RkArgs &a, double step, double e0, double e1, double e2,
                                     const double (&pr)[E], const double (&pi)[E], double (&xr)[E],
                                     double (&xi)[E], double (&ar)[E], double (&ai)[E], const double *lds,
-                                    const Halo &h, int lane)
+                                    const Halo &h, int lane, int ll)
 {
     const double hh = 0.5 * a.rk_step, H = 1.0 * a.rk_step;
 #pragma unroll
@@ -256,10 +266,10 @@ __device__ inline void rk_integrate(const RkArgs &a, double step, double e0, dou
         xr[j] = pr[j];
         xi[j] = pi[j];
     }
-    rk_stage<E, kStageFirst>(xr, xi, pr, pi, ar, ai, e0, hh, lds, h, lane);
-    rk_stage<E, kStageMid>(xr, xi, pr, pi, ar, ai, e1, hh, lds, h, lane);
-    rk_stage<E, kStageMid>(xr, xi, pr, pi, ar, ai, e1, H, lds, h, lane);
-    rk_stage<E, kStageLast>(xr, xi, pr, pi, ar, ai, e2, step / 6.0, lds, h, lane);
+    rk_stage<E, P, kStageFirst>(xr, xi, pr, pi, ar, ai, e0, hh, lds, h, lane, ll);
+    rk_stage<E, P, kStageMid>(xr, xi, pr, pi, ar, ai, e1, hh, lds, h, lane, ll);
+    rk_stage<E, P, kStageMid>(xr, xi, pr, pi, ar, ai, e1, H, lds, h, lane, ll);
+    rk_stage<E, P, kStageLast>(xr, xi, pr, pi, ar, ai, e2, step / 6.0, lds, h, lane, ll);
 }
 
 // Waves per SIMD the register allocation aims for at each lane width.
@@ -268,16 +278,18 @@ struct RkOcc {
     static constexpr int waves = E <= 2 ? 4 : E == 4 ? 3 : E == 8 ? 2 : 1;
 };
 
-template <int E>
+template <int E, int P>
 __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(RkOcc<E>::waves)))
 void rk_kernel(RkArgs a)
 {
     extern __shared__ double rk_lds[];
-    constexpr int NB = kRkStageBands * E * 64;
+    constexpr int NB = kRkStageBands * E * P;
     for (int i = threadIdx.x; i < NB; i += 64 * kWaves) rk_lds[i] = a.bands[i];
     __syncthreads();
     const int lane = threadIdx.x & 63;
-    const int tr = blockIdx.x * kWaves + (int)(threadIdx.x >> 6);
+    const int ll = P == 64 ? lane : lane & (P - 1);      // lane within the trajectory's group
+    const int tr = P == 64 ? blockIdx.x * kWaves + (int)(threadIdx.x >> 6)
+                           : (blockIdx.x * kWaves + (int)(threadIdx.x >> 6)) * (64 / P) + lane / P;
     if (tr >= a.M) return;
 
     TrajState s = a.st[tr];
@@ -287,25 +299,26 @@ void rk_kernel(RkArgs a)
     // beyond d is zero and stays zero), so the loads and stores need no guard
 #pragma unroll
     for (int j = 0; j < E; j++) {
-        pr[j] = col[lane * E + j];
-        pi[j] = col[a.dp + lane * E + j];
+        pr[j] = col[ll * E + j];
+        pi[j] = col[a.dp + ll * E + j];
     }
     Halo h;
-    h.o1re = lane ? rk_lds[(kRkO1re * E + E - 1) * 64 + lane - 1] : 0.0;
-    h.o1im = lane ? rk_lds[(kRkO1im * E + E - 1) * 64 + lane - 1] : 0.0;
+    // zero at the group's first lane: those columns lie before entry 0
+    h.o1re = ll ? rk_lds[(kRkO1re * E + E - 1) * P + ll - 1] : 0.0;
+    h.o1im = ll ? rk_lds[(kRkO1im * E + E - 1) * P + ll - 1] : 0.0;
     // H_part(lE-2, lE): entry E-2 of lane l-1, or with one entry per lane lane l-2's
     if constexpr (E == 1)
-        h.o2a = lane >= 2 ? rk_lds[kRkO2im * 64 + lane - 2] : 0.0;
+        h.o2a = ll >= 2 ? rk_lds[kRkO2im * P + ll - 2] : 0.0;
     else
-        h.o2a = lane ? rk_lds[(kRkO2im * E + E - 2) * 64 + lane - 1] : 0.0;
-    h.o2b = lane ? rk_lds[(kRkO2im * E + E - 1) * 64 + lane - 1] : 0.0;
-    const double *Ld = a.bands + (size_t)kRkLd * E * 64;
-    const double *Lm = a.bands + (size_t)kRkLm * E * 64;
-    const double *Lp = a.bands + (size_t)kRkLp * E * 64;
+        h.o2a = ll ? rk_lds[(kRkO2im * E + E - 2) * P + ll - 1] : 0.0;
+    h.o2b = ll ? rk_lds[(kRkO2im * E + E - 1) * P + ll - 1] : 0.0;
+    const double *Ld = a.bands + (size_t)kRkLd * E * P;
+    const double *Lm = a.bands + (size_t)kRkLm * E * P;
+    const double *Lp = a.bands + (size_t)kRkLp * E * P;
 
     unsigned long long integrations = 0, jumps = 0;
     bool failed = false;
-    double pn = wave_norm<E>(pr, pi);       // ||phi||^2 of the current state
+    double pn = group_norm<E, P>(pr, pi);       // ||phi||^2 of the current state
     double t = s.t;
     for (int k = 0; k < a.nsteps && !failed; k++) {
         const long long gs = a.pos0 + k;
@@ -326,16 +339,16 @@ void rk_kernel(RkArgs a)
             } else {
                 rk_drives(a, t, step, e0, e1, e2);
             }
-            rk_integrate<E>(a, step, e0, e1, e2, pr, pi, xr, xi, ar, ai, rk_lds, h, lane);
+            rk_integrate<E, P>(a, step, e0, e1, e2, pr, pi, xr, xi, ar, ai, rk_lds, h, lane, ll);
             integrations++;
             if (!partial) {
-                const double n = wave_norm<E>(xr, xi);
+                const double n = group_norm<E, P>(xr, xi);
                 // is_norm_crossed throws for ||phi||^2 > 1 (experiment.cpp:1247-1251); a
                 // NaN norm or an endless run of crossings (the reference would recurse
                 // without end) stops the trajectory with error code 2
                 const int code = n > 1.0 ? 1 : (!(n == n) || crossings >= kRkMaxCrossings) ? 2 : 0;
                 if (code) {
-                    if (lane == 0 && atomicCAS(&a.err[0], 0, code) == 0) {
+                    if (ll == 0 && atomicCAS(&a.err[0], 0, code) == 0) {
                         const unsigned long long bits = __double_as_longlong(n);
                         a.err[1] = tr;
                         a.err[2] = (int)(bits & 0xffffffffULL);
@@ -371,9 +384,9 @@ void rk_kernel(RkArgs a)
                 pr[j] = xr[j];
                 pi[j] = xi[j];
             }
-            const double jn = wave_norm<E>(pr, pi);
+            const double jn = group_norm<E, P>(pr, pi);
             if (a.jump > 0 && a.is_obs == 1) {
-                if (lane == 0 && s.logn < a.log.cap) {
+                if (ll == 0 && s.logn < a.log.cap) {
                     const size_t o = (size_t)tr * a.log.cap + s.logn;
                     a.log.time[o] = t;
                     a.log.norm[o] = jn;
@@ -391,7 +404,7 @@ void rk_kernel(RkArgs a)
                 pi[j] = pi[j] / nrm;
             }
             (void)mcg_next(s.rng, a.stride);
-            int lo = lane;
+            int lo = ll;
             asm volatile("" : "+v"(lo));
             const double lr = shfl_up1(pr[E - 1], lane), li = shfl_up1(pi[E - 1], lane);
             const double rr = shfl_down1(pr[0], lane), ri = shfl_down1(pi[0], lane);
@@ -399,7 +412,7 @@ void rk_kernel(RkArgs a)
             for (int j = 0; j < E; j++) {
                 const double mr = j ? pr[j - 1] : lr, mi = j ? pi[j - 1] : li;
                 const double qr = j + 1 < E ? pr[j + 1] : rr, qi = j + 1 < E ? pi[j + 1] : ri;
-                const double cm = Lm[j * 64 + lo], cd = Ld[j * 64 + lo], cp = Lp[j * 64 + lo];
+                const double cm = Lm[j * P + lo], cd = Ld[j * P + lo], cp = Lp[j * P + lo];
                 double re = 0.0, im = 0.0;
                 re = __builtin_fma(cm, mr, re);
                 im = __builtin_fma(cm, mi, im);
@@ -410,7 +423,7 @@ void rk_kernel(RkArgs a)
                 ar[j] = re;
                 ai[j] = im;
             }
-            const double g = wave_norm<E>(ar, ai) * 1.0;
+            const double g = group_norm<E, P>(ar, ai) * 1.0;
             const double sg = sqrt(g / 1.0);
 #pragma unroll
             for (int j = 0; j < E; j++) {
@@ -421,7 +434,7 @@ void rk_kernel(RkArgs a)
             double ev = mcg_next(s.rng, a.stride);
             while (ev == 0.0) ev = mcg_next(s.rng, a.stride);
             s.eta = ev;
-            pn = wave_norm<E>(pr, pi);
+            pn = group_norm<E, P>(pr, pi);
             step = estep;
             partial = false;
         }
@@ -429,30 +442,31 @@ void rk_kernel(RkArgs a)
     s.t = t;
 #pragma unroll
     for (int j = 0; j < E; j++) {
-        col[lane * E + j] = pr[j];
-        col[a.dp + lane * E + j] = pi[j];
+        col[ll * E + j] = pr[j];
+        col[a.dp + ll * E + j] = pi[j];
     }
-    if (lane == 0) {
+    if (ll == 0) {
         a.st[tr] = s;
         atomicAdd(&a.counters[0], 4ULL * integrations);
         atomicAdd(&a.counters[1], jumps);
     }
 }
 
-template <int E>
+template <int E, int P>
 hipError_t launch_rk_e(const RkArgs &a, hipStream_t s)
 {
-    const size_t lds = (size_t)kRkStageBands * E * 64 * sizeof(double);
+    const size_t lds = (size_t)kRkStageBands * E * P * sizeof(double);
     if (lds > 64 * 1024) {
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&rk_kernel<E>),
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&rk_kernel<E, P>),
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
     rk_drive_table_kernel<<<(a.nsteps + 255) / 256, 256, 0, s>>>(a);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const int blocks = (a.M + kWaves - 1) / kWaves;
-    rk_kernel<E><<<blocks, 64 * kWaves, lds, s>>>(a);
+    constexpr int per_block = kWaves * (64 / P);   // trajectories per workgroup
+    const int blocks = (a.M + per_block - 1) / per_block;
+    rk_kernel<E, P><<<blocks, 64 * kWaves, lds, s>>>(a);
     return hipGetLastError();
 }
 
@@ -461,14 +475,15 @@ hipError_t launch_rk_e(const RkArgs &a, hipStream_t s)
 hipError_t launch_rk(const RkArgs &a, hipStream_t s)
 {
     if (a.M <= 0 || a.nsteps <= 0) return hipSuccess;
-    if (a.dp != 64 * rk_lanes(a.d) || a.L != 2 * a.dp) return hipErrorInvalidValue;
+    if (a.dp != rk_group(a.d) * rk_lanes(a.d) || a.L != 2 * a.dp) return hipErrorInvalidValue;
+    if (rk_group(a.d) == 32) return rk_lanes(a.d) == 4 ? launch_rk_e<4, 32>(a, s) : hipErrorInvalidValue;
     switch (rk_lanes(a.d)) {
-    case 1: return launch_rk_e<1>(a, s);
-    case 2: return launch_rk_e<2>(a, s);
-    case 4: return launch_rk_e<4>(a, s);
-    case 8: return launch_rk_e<8>(a, s);
-    case 16: return launch_rk_e<16>(a, s);
-    case 32: return launch_rk_e<32>(a, s);
+    case 1: return launch_rk_e<1, 64>(a, s);
+    case 2: return launch_rk_e<2, 64>(a, s);
+    case 4: return launch_rk_e<4, 64>(a, s);
+    case 8: return launch_rk_e<8, 64>(a, s);
+    case 16: return launch_rk_e<16, 64>(a, s);
+    case 32: return launch_rk_e<32, 64>(a, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -55,8 +55,10 @@ def _same(o, e, M, logs=True):
     (20, 1.5, 0.3, 0, 1000, 96, 1, "periods", 2),      # E = 1, square drive
     (20, 2.0, 0.2, 1, 1000, 96, 1, "periods", 2),      # E = 1, sine drive
     (63, 1.5, 0.2, 1, 2000, 64, 0, "periods", 1),      # d = 64: E = 1, every lane used
-    (64, 1.5, 0.2, 1, 2000, 64, 0, "periods", 1),      # d = 65: E = 2
-    (100, 1.5, 0.1, 1, 4000, 64, 0, "periods", 1),     # E = 2
+    (64, 1.5, 0.2, 1, 2000, 63, 0, "periods", 1),      # d = 65: 32 lanes x 4, two per wavefront, odd M
+    (100, 1.5, 0.1, 1, 4000, 64, 0, "periods", 1),     # 32 lanes x 4
+    (127, 1.5, 0.1, 0, 4000, 33, 1, "periods", 1),     # d = 128: 32 lanes x 4, every lane used
+    (128, 1.5, 0.1, 1, 4000, 32, 0, "steps", 2000),    # d = 129: 64 lanes x 4
     (200, 1.5, 0.1, 0, 8000, 32, 0, "steps", 3000),    # E = 4, across the half period
     (300, 1.5, 0.1, 1, 12000, 32, 0, "steps", 3000),   # E = 8
     (1000, 1.5, 0.1, 1, 40000, 16, 0, "steps", 1500),  # E = 16
