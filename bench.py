@@ -312,7 +312,7 @@ def rk_leg(qjx, N, rk_ns, M, steps, cpu_budget_s, drv_type=0, with_cpu=True):
                         "flops_note": "%.0f FP64 flops per state entry per RK4 integration (4 stencil stages "
                                       "of the pentadiagonal H_part + diagonal drive, RK updates, norm)"
                                       % RK_FLOPS_PER_ENTRY_STEP,
-                        "kernel": "rk_kernel (one wavefront per trajectory, state in registers)"}}
+                        "kernel": "rk_kernel (32 or 64 lanes per trajectory, state in registers)"}}
     if with_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
         threads = min(threads, os.cpu_count() or threads)
