@@ -1,0 +1,110 @@
+"""Edges of the quantum-jump path on the GPU: the smallest dimension and a single
+trajectory against the oracle, the largest dimension through invariants that need
+no CPU run, and an empty ensemble.
+
+Reference: the dimer's Hilbert dimension is N + 1 (DimerNewDelBehaviour::init_sizes,
+newdel.cpp:3-37); the engine accepts N in [1, 2047].  Per trajectory the MCG59
+stream is consumed as eta_0, then (ran, eta) per jump (experiment.cpp:3261-3265,
+1303; qj_proc.cpp:310-314).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MCG59_A, MCG59_M = 302875106592253, 1 << 59
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+# seeds whose lone trajectory jumps within the horizon (stream 0 draws a small
+# eta_0 and does not)
+@pytest.mark.parametrize("N,M,seed", [(1, 1, 1), (1, 37, 0), (2, 1, 2)])
+def test_fp64_parity_smallest_dimension_and_one_trajectory(oracle_mod, N, M, seed):
+    """d = 2 and 3, and an ensemble of one: the same jumps, eta values, RNG states
+    and times as the oracle with the same propagator tree."""
+    from os_d_amd import qjx
+    _say("case: edges fp64 parity N=%d M=%d seed=%d" % (N, M, seed))
+    S, D = 4, 8
+    m = oracle_mod.model(N, S=S, D=D, gamma=0.5)
+    o = oracle_mod.Ensemble(m, M, seed=seed, jump=2)
+    e = qjx.Engine(N, ex_deep=D, steps_per_period=S, precision=qjx.FP64, diss_gamma=0.5)
+    e.ensemble_init(M, seed=seed, jump=2)
+    for b in range(2):
+        for lvl in range(D):
+            e.set_propagator(b, lvl, o.get_propagator(b, lvl))
+    o.trans(1)
+    e.trans(1)
+    o.periods(4, 1)
+    e.run_periods(4, 1)
+    so, se = o.get_state(), e.get_state()
+    for k in ("rng", "eta", "t", "njumps"):
+        assert np.array_equal(so[k], se[k]), k
+    assert int(se["njumps"].sum()) > 0
+    for tr in range(M):
+        jo, je = o.get_jumps(tr), e.get_jumps(tr)
+        assert np.array_equal(jo["times"], je["times"]), tr
+        assert np.array_equal(jo["etas"], je["etas"]), tr
+    np.testing.assert_allclose(se["psi"], so["psi"], rtol=0, atol=1e-10)
+    assert e.stats()["matvecs"] == o.num_matvecs()
+    e.close()
+
+
+def _max_dim_run(qjx, prec, M, steps):
+    e = qjx.Engine(2047, ex_deep=8, steps_per_period=200, precision=prec, diss_gamma=0.1)
+    e.ensemble_init(M, seed=5, jump=2)
+    e.trans(0)
+    rng0 = e.get_state()["rng"].copy()           # after the eta_0 draw
+    e.run_steps(steps, 1)
+    return e, rng0
+
+
+@pytest.mark.parametrize("prec", [1, 2])
+def test_largest_dimension_invariants(prec, monkeypatch):
+    """d = 2048 (32 row tiles, K ranges up to the full 2 x 2048 columns): every
+    trajectory consumed exactly 1 + 2 jumps draws, logged as many jumps as it
+    counts, stays normalised within the form's tolerance and advanced by exactly the
+    steps run; the result does not depend on the GEMM's tile width or on skipping
+    the band's exact zeros."""
+    from os_d_amd import qjx
+    M, steps = 96, 3
+    _say("case: edges d=2048 prec=%d" % prec)
+    e, rng0 = _max_dim_run(qjx, prec, M, steps)
+    s = e.get_state()
+    nj = s["njumps"]
+    assert nj.sum() > 0
+    stride = pow(MCG59_A, 1000000, MCG59_M)
+    for tr in range(M):
+        assert int(s["rng"][tr]) == int(rng0[tr]) * pow(stride, 2 * int(nj[tr]), MCG59_M) % MCG59_M, tr
+        j = e.get_jumps(tr)
+        assert len(j["times"]) == nj[tr]
+        assert (np.diff(j["times"]) > 0).all()
+    n = e.chars()["norm"]
+    assert ((n > 0.0) & (n <= 1.0 + 1e-3)).all()
+    assert np.allclose(s["t"], steps * 2 * np.pi / 200, rtol=1e-12)
+    e.close()
+    # the same run with another tile width, and multiplying every K block
+    for env in (("QJX_BN", "64"), ("QJX_DENSE", "1")):
+        monkeypatch.setenv(*env)
+        e2, _ = _max_dim_run(qjx, prec, M, steps)
+        s2 = e2.get_state()
+        for k in ("psi", "eta", "t", "njumps", "rng"):
+            assert np.array_equal(s[k], s2[k]), (env, k)
+        e2.close()
+        monkeypatch.delenv(env[0])
+
+
+def test_empty_ensemble_is_rejected():
+    from os_d_amd import qjx
+    e = qjx.Engine(10, ex_deep=4, steps_per_period=4)
+    with pytest.raises(qjx.QJXError):
+        e.ensemble_init(0)
+    e.ensemble_init(3)
+    e.run_periods(1, 0)
+    assert e.get_state()["psi"].shape[0] == 3
+    e.close()
