@@ -16,10 +16,11 @@ fp64_value : the same for the FP64 form (same workload and steps), timed on the
 bf16x3_value: the same for the 3 x BF16 split-product form (FP32 accumulate),
              timed like the FP64 form.
 roofline   : the step GEMM (dominant kernel), algorithmic flops = the K blocks
-             inside the propagators' band tables (2 x 128 x 32 per column, row tile
-             and block; the rest of the dense 8 d^2 per application are exact zeros
-             that are skipped), every launch of the FP32 engine (warmup and timed
-             steps) timed with HIP events on the engine stream.
+             inside the band range of each 32-row fragment of the propagators
+             (2 x 32 x 32 per column, fragment and block; the rest of the dense
+             8 d^2 per application are exact zeros that are skipped), every launch of
+             the FP32 engine (warmup and timed steps) timed with HIP events on the
+             engine stream.
 cpu_baseline: the FP64 CPU oracle (OpenMP over trajectories, one zgemv per
              step as the reference) on a bounded sample of the same workload
              with the same propagator tree.
@@ -604,7 +605,7 @@ def main():
                      "launches": launches,
                      "avg_launch_ms": avg_ms,
                      "flops_per_launch": flops_per_launch,
-                     "flops_note": "2 x 128 x 32 per (column, row tile, K block) inside the band tables",
+                     "flops_note": "2 x 32 x 32 per (column, 32-row fragment, K block) inside the fragments' band ranges",
                      "dense_equivalent_flops_per_launch": dense_flops_per_launch,
                      "dense_equivalent_tflops": dense_flops_per_launch / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else None,
                      "timed_steps_band_tflops": (r32["gemm_band_flops"] / (r32["gemm_ms"] * 1e-3) / 1e12

@@ -29,6 +29,7 @@ struct qjx_ctx {
     void *props = nullptr;         // 2*D * L*L elements
     void *props_lo = nullptr;      // BF16x3: low halves
     int *krange = nullptr;         // [G][R][2] K-block ranges of the propagators' row tiles
+    unsigned *kfrag = nullptr;     // [G][R][2] the same per 32-row fragment (one byte each)
     bool dense = false;            // QJX_DENSE=1: every K block (no band table)
     double *d_dt = nullptr, *d_Ldiag = nullptr, *d_Loff = nullptr;
     double *d_HdiagE = nullptr, *d_Hoff = nullptr;

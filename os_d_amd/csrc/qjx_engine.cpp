@@ -205,7 +205,7 @@ static void band_of(qjx_ctx *c, int b, int lvl)
     const int g = b * c->D + lvl;
     qjx::launch_band(c->prec, prop_ptr(c, b, lvl), prop_lo_ptr(c, b, lvl), c->L, c->dp,
                      c->dense || c->gemm_kind != qjx::kGemmMfma, c->krange + (size_t)2 * g * (c->L / 128),
-                     c->stream);
+                     c->kfrag + (size_t)2 * g * (c->L / 128), c->stream);
 }
 
 extern "C" const char *qjx_last_error(void) { return g_err.c_str(); }
@@ -294,7 +294,8 @@ int qjx::ctx_create_kind(const qjx_model_desc *m, int gemm_kind, qjx_ctx **out)
         return bail(fail(QJX_ERR_NOMEM, "propagator tree does not fit in device memory"));
     if (c->prec == QJX_BF16X3 && hipMalloc(&c->props_lo, propbytes) != hipSuccess)
         return bail(fail(QJX_ERR_NOMEM, "propagator tree does not fit in device memory"));
-    if (hipMalloc((void **)&c->krange, (size_t)2 * c->G * (c->L / 128) * sizeof(int)) != hipSuccess)
+    if (hipMalloc((void **)&c->krange, (size_t)2 * c->G * (c->L / 128) * sizeof(int)) != hipSuccess ||
+        hipMalloc((void **)&c->kfrag, (size_t)2 * c->G * (c->L / 128) * sizeof(unsigned)) != hipSuccess)
         return bail(fail(QJX_ERR_NOMEM, "band table does not fit in device memory"));
     if (const char *env = std::getenv("QJX_DENSE")) c->dense = std::string(env) == "1";
 
@@ -551,6 +552,7 @@ extern "C" void qjx_ctx_destroy(qjx_ctx *c)
     if (c->props) (void)hipFree(c->props);
     if (c->props_lo) (void)hipFree(c->props_lo);
     dev_free(c->krange);
+    dev_free(c->kfrag);
     dev_free(c->d_dt);
     dev_free(c->d_Ldiag);
     dev_free(c->d_Loff);
@@ -585,6 +587,7 @@ static void build_desc(qjx_ctx *c)
     E.props = c->props;
     E.props_lo = c->props_lo;
     E.krange = c->krange;
+    E.kfrag = c->kfrag;
     for (int i = 0; i < 2; i++) {
         E.slot[i] = c->slot[i];
         E.list[i] = c->lists[i];
@@ -992,7 +995,7 @@ extern "C" int qjx_get_stats(qjx_ctx *c, qjx_stats *out)
     out->iterations = c->runner ? c->runner->iterations : 0;
     out->jumps = (int64_t)h[1];
     out->gemm_flops = 8.0 * double(c->d) * double(c->d) * double(out->matvecs);
-    out->gemm_band_flops = double(h[2]) * (2.0 * 128 * 32);
+    out->gemm_band_flops = double(h[2]) * (2.0 * 32 * 32);
     out->gemm_ms = c->runner ? c->runner->gemm_ms : 0.0;
     out->decide_ms = c->runner ? c->runner->tail_ms + c->runner->plan_ms : 0.0;
     out->gemm_launches = c->runner ? c->runner->gemm_launches : 0;
@@ -1155,7 +1158,7 @@ extern "C" int qjx_batch_get_stats(qjx_batch *b, qjx_stats *out)
         out->matvecs += (int64_t)h[0];
         out->jumps += (int64_t)h[1];
         out->gemm_flops += 8.0 * double(c->d) * double(c->d) * double(h[0]);
-        out->gemm_band_flops += double(h[2]) * (2.0 * 128 * 32);
+        out->gemm_band_flops += double(h[2]) * (2.0 * 32 * 32);
     }
     const qjx::Runner *r = b->runner;
     out->iterations = r->iterations;

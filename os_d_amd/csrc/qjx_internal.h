@@ -59,6 +59,8 @@ struct Ens {
     const void *props_lo;     // BF16x3 only: low halves
     const int *krange;        // [G][R][2]: K blocks (of 32) [lo, hi) holding every nonzero of
                               // row tile rt within each half of the operand (exact zeros skipped)
+    const unsigned *kfrag;    // [G][R][2]: the same for each 32-row fragment f of the tile,
+                              // byte f of word 0 (lo) and of word 1 (hi)
     void *slot[2];            // state slots, M*L elements each
     TrajState *st;
     int *list[2];             // per parity: [G * M] entries (trajectory << 1) | current slot
@@ -70,7 +72,7 @@ struct Ens {
     JumpLog log;
     int *err;                 // [4]: code, trajectory, bits of the offending norm
     unsigned long long *counters;  // [0] matvecs, [1] jumps, [2] GEMM work in (column x K block
-                                   // of one 128-row tile) units
+                                   // of one 32-row fragment) units
     uint64_t stride;          // MCG59 leap-frog multiplier a^mns
     double norm_tol;          // allowed excess of ||psi||^2 over 1 (0 for FP64)
     int d, dp, L, D, S, M, G, R;

@@ -73,10 +73,11 @@ void launch_density(int prec, const void *slot0, const void *slot1, const TrajSt
                     int dp, int L, double *w, double *part, int splits, double *rho, double *purity,
                     hipStream_t s);
 
-// K-block ranges [lo, hi) per 128-row tile of a packed operand (out: [L/128][2]);
-// dense = every block.
+// K-block ranges [lo, hi) per 128-row tile of a packed operand (out: [L/128][2]) and
+// per 32-row fragment (frag: [L/128][2], byte f of word 0 / 1 = fragment f's lo /
+// hi); dense = every block.
 void launch_band(int prec, const void *P, const void *Plo, int L, int dp, bool dense, int *out,
-                 hipStream_t s);
+                 unsigned *frag, hipStream_t s);
 
 // Dense complex FP64 helpers for the propagator build (row-major n x n, interleaved).
 void launch_zgemm(int n, const double *A, const double *B, double *C, hipStream_t s);

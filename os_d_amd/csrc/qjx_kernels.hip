@@ -558,8 +558,11 @@ void launch_density(int prec, const void *slot0, const void *slot1, const TrajSt
 // ---------------------------------------------------------------------------
 // Band table of a packed operand: for each 128-row tile, the K blocks (of 32
 // columns, counted within a half of the block-real operand) between the first
-// and the last nonzero entry of its rows in either half.  The step GEMM skips the
-// blocks outside, which hold exact zeros only, so its results do not change.
+// and the last nonzero entry of its rows in either half, and the same for each of
+// its four 32-row fragments (one byte each: starts in frag[0], ends in frag[1]).
+// The step GEMM skips the blocks outside a tile's range and, inside it, the matrix
+// products of a fragment outside the fragment's range: both hold exact zeros only,
+// so its results do not change.  The tile range is the union of its fragments'.
 // The propagators of a tridiagonal H_eff are banded: the Taylor polynomial of
 // A / 2^s has bandwidth 18 and each squaring doubles it, and entries far from the
 // diagonal underflow, in FP32 storage soonest.
@@ -571,50 +574,74 @@ __device__ inline bool nonzero_at(const T *P, const T *Plo, size_t o)
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void band_kernel(const T *P, const T *Plo, int L, int dp, int *out)
+__global__ __launch_bounds__(256) void band_kernel(const T *P, const T *Plo, int L, int dp, int *out,
+                                                   unsigned *frag)
 {
-    __shared__ int s_lo, s_hi;
-    if (threadIdx.x == 0) {
-        s_lo = dp;
-        s_hi = 0;
+    __shared__ int s_lo[4], s_hi[4];
+    if (threadIdx.x < 4) {
+        s_lo[threadIdx.x] = dp;
+        s_hi[threadIdx.x] = 0;
     }
     __syncthreads();
     const int r0 = blockIdx.x * 128;
-    int lo = dp, hi = 0;
-    for (int e = threadIdx.x; e < 128 * L; e += blockDim.x) {
-        const int r = r0 + e / L, k = e % L;
-        if (nonzero_at(P, Plo, (size_t)r * L + k)) {
-            const int kk = k < dp ? k : k - dp;
-            lo = min(lo, kk);
-            hi = max(hi, kk + 1);
+    for (int f = 0; f < 4; f++) {
+        int lo = dp, hi = 0;
+        for (int e = threadIdx.x; e < 32 * L; e += blockDim.x) {
+            const int r = r0 + 32 * f + e / L, k = e % L;
+            if (nonzero_at(P, Plo, (size_t)r * L + k)) {
+                const int kk = k < dp ? k : k - dp;
+                lo = min(lo, kk);
+                hi = max(hi, kk + 1);
+            }
         }
+        atomicMin(&s_lo[f], lo);
+        atomicMax(&s_hi[f], hi);
     }
-    atomicMin(&s_lo, lo);
-    atomicMax(&s_hi, hi);
     __syncthreads();
     if (threadIdx.x == 0) {
-        const bool any = s_hi > s_lo;
-        out[2 * blockIdx.x] = any ? s_lo / 32 : 0;
-        out[2 * blockIdx.x + 1] = any ? (s_hi + 31) / 32 : 0;
+        int lo = dp, hi = 0;
+        for (int f = 0; f < 4; f++) {
+            if (s_hi[f] > s_lo[f]) {
+                lo = min(lo, s_lo[f]);
+                hi = max(hi, s_hi[f]);
+            }
+        }
+        const bool any = hi > lo;
+        const int kb0 = any ? lo / 32 : 0, kb1 = any ? (hi + 31) / 32 : 0;
+        out[2 * blockIdx.x] = kb0;
+        out[2 * blockIdx.x + 1] = kb1;
+        unsigned w0 = 0, w1 = 0;
+        for (int f = 0; f < 4; f++) {
+            const bool fa = s_hi[f] > s_lo[f];        // an all-zero fragment gets an empty range
+            const unsigned b0 = fa ? (unsigned)(s_lo[f] / 32) : (unsigned)kb0;
+            const unsigned b1 = fa ? (unsigned)((s_hi[f] + 31) / 32) : (unsigned)kb0;
+            w0 |= b0 << (8 * f);
+            w1 |= b1 << (8 * f);
+        }
+        frag[2 * blockIdx.x] = w0;
+        frag[2 * blockIdx.x + 1] = w1;
     }
 }
 
-__global__ void band_dense_kernel(int R, int dp, int *out)
+__global__ void band_dense_kernel(int R, int dp, int *out, unsigned *frag)
 {
     const int rt = blockIdx.x * blockDim.x + threadIdx.x;
     if (rt >= R) return;
     out[2 * rt] = 0;
     out[2 * rt + 1] = dp / 32;
+    frag[2 * rt] = 0;
+    frag[2 * rt + 1] = (unsigned)(dp / 32) * 0x01010101u;
 }
 
-void launch_band(int prec, const void *P, const void *Plo, int L, int dp, bool dense, int *out, hipStream_t s)
+void launch_band(int prec, const void *P, const void *Plo, int L, int dp, bool dense, int *out, unsigned *frag,
+                 hipStream_t s)
 {
     const int R = L / 128;
-    if (dense) band_dense_kernel<<<(R + 63) / 64, 64, 0, s>>>(R, dp, out);
-    else if (prec == kFP64) band_kernel<double><<<R, 256, 0, s>>>((const double *)P, nullptr, L, dp, out);
+    if (dense) band_dense_kernel<<<(R + 63) / 64, 64, 0, s>>>(R, dp, out, frag);
+    else if (prec == kFP64) band_kernel<double><<<R, 256, 0, s>>>((const double *)P, nullptr, L, dp, out, frag);
     else if (prec == kBF16x3)
-        band_kernel<__bf16><<<R, 256, 0, s>>>((const __bf16 *)P, (const __bf16 *)Plo, L, dp, out);
-    else band_kernel<float><<<R, 256, 0, s>>>((const float *)P, nullptr, L, dp, out);
+        band_kernel<__bf16><<<R, 256, 0, s>>>((const __bf16 *)P, (const __bf16 *)Plo, L, dp, out, frag);
+    else band_kernel<float><<<R, 256, 0, s>>>((const float *)P, nullptr, L, dp, out, frag);
 }
 
 }  // namespace qjx

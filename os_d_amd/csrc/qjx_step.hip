@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "qjx_device.h"
 #include "qjx_internal.h"
 #include "qjx_kernels.h"
@@ -236,6 +238,7 @@ struct KRange {
     int kb0, nb, hb;
     __device__ int count() const { return 2 * nb; }
     __device__ int at(int i) const { return (i < nb ? kb0 + i : hb + kb0 + i - nb) * MF_BK; }
+    __device__ int half_block(int i) const { return kb0 + (i < nb ? i : i - nb); }   // block within its half
 };
 
 __device__ inline KRange tile_krange(const Ens &E, const TileRef &r)
@@ -258,6 +261,37 @@ __device__ inline KRange tile_krange_uniform(const Ens &E, const TileRef &r)
     k.hb = __builtin_amdgcn_readfirstlane(k.hb);
     return k;
 }
+
+// The K blocks each 32-row fragment of the tile needs (band table kfrag): a
+// fragment's matrix products outside its range multiply exact zeros of the
+// propagator and are skipped, which leaves its accumulators bit-identical.
+struct FragRange {
+    unsigned lo, hi;   // byte f: fragment f's first and past-the-end block within a half
+    __device__ bool active(int f, int b) const
+    {
+        return b >= (int)((lo >> (8 * f)) & 0xffu) && b < (int)((hi >> (8 * f)) & 0xffu);
+    }
+    // fragment-blocks the tile multiplies over both halves of the operand
+    __device__ unsigned work() const
+    {
+        unsigned w = 0;
+#pragma unroll
+        for (int f = 0; f < 4; f++) w += ((hi >> (8 * f)) & 0xffu) - ((lo >> (8 * f)) & 0xffu);
+        return 2 * w;
+    }
+};
+
+__device__ inline FragRange tile_frags(const Ens &E, const TileRef &r)
+{
+    const unsigned *p = E.kfrag + 2 * ((size_t)r.g * E.R + r.rt);
+    FragRange f;
+    f.lo = __builtin_amdgcn_readfirstlane(gld<unsigned>(p));
+    f.hi = __builtin_amdgcn_readfirstlane(gld<unsigned>(p + 1));
+    return f;
+}
+
+template <bool V>
+using Use = std::integral_constant<bool, V>;
 
 template <int BN, int NWN>
 __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) void gemm_f32(RunArgs a)
@@ -322,15 +356,18 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
                 *reinterpret_cast<floatx4 *>(&Bs[buf][(cp_r + (T / 8) * q) * F32_PITCH + cp_k]) = sg.b[q];
         };
         // fragments in four k slices of 4 MFMA steps each (steps 4v..4v+3 read one
-        // 16-byte LDS word per operand and lane): fewer live registers, same order
-        auto mma = [&](int buf) {
+        // 16-byte LDS word per operand and lane): fewer live registers, same order.
+        // U0 / U1: whether the wave's fragment i = 0 / 1 multiplies this block.
+        auto mma_sel = [&](int buf, auto U0, auto U1) {
+            constexpr bool u[2] = {decltype(U0)::value, decltype(U1)::value};
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 floatx4 fa[2], fb[NJ];
 #pragma unroll
                 for (int i = 0; i < 2; i++)
-                    fa[i] = *reinterpret_cast<const floatx4 *>(
-                        &As[buf][(wm * 64 + i * 32 + li) * F32_PITCH + 16 * h + 4 * v]);
+                    if (u[i])
+                        fa[i] = *reinterpret_cast<const floatx4 *>(
+                            &As[buf][(wm * 64 + i * 32 + li) * F32_PITCH + 16 * h + 4 * v]);
 #pragma unroll
                 for (int j = 0; j < NJ; j++)
                     fb[j] = *reinterpret_cast<const floatx4 *>(
@@ -339,13 +376,23 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
                 for (int e = 0; e < 4; e++)
 #pragma unroll
                     for (int i = 0; i < 2; i++)
+                        if (u[i])
 #pragma unroll
-                        for (int j = 0; j < NJ; j++)
-                            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[i][e], fb[j][e], acc[i][j], 0, 0, 0);
+                            for (int j = 0; j < NJ; j++)
+                                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[i][e], fb[j][e], acc[i][j], 0,
+                                                                                  0, 0);
             }
         };
         const KRange kr = tile_krange_uniform(E, r);
         const int nk = kr.count();
+        const FragRange fr = tile_frags(E, r);
+        auto mma = [&](int buf, int kb) {
+            const int b = kr.half_block(kb);
+            const bool a0 = fr.active(2 * wm, b), a1 = fr.active(2 * wm + 1, b);
+            if (a0 && a1) mma_sel(buf, Use<true>(), Use<true>());
+            else if (a0) mma_sel(buf, Use<true>(), Use<false>());
+            else if (a1) mma_sel(buf, Use<false>(), Use<true>());
+        };
         if (nk > 0) {
             // refills past the last block reload the last block (in bounds, never
             // stored), so every iteration issues the same loads and the wait before
@@ -364,7 +411,7 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
                 __builtin_amdgcn_sched_barrier(0);
                 lstore(nxt, buf ^ 1);
                 gload(nxt, blk(kb + 3));
-                mma(buf);
+                mma(buf, kb);
                 __syncthreads();
             };
             int kb = 0;
@@ -400,7 +447,7 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
         }
         __syncthreads();
         if (tid < r.ncol) gst<double>(E.normpart + (size_t)s_tr[tid] * E.R + r.rt, s_norm[0][tid] + s_norm[1][tid]);
-        if (tid == 0) atomicAdd(&E.counters[2], (unsigned long long)r.ncol * nk);
+        if (tid == 0) atomicAdd(&E.counters[2], (unsigned long long)r.ncol * fr.work());
         __syncthreads();
     }
 }
@@ -510,12 +557,15 @@ __global__ __launch_bounds__(128 * NWN) void gemm_bf16x3(RunArgs a)
                 *reinterpret_cast<bf16x4 *>(&Bl[buf][bf_off(rr, cb_k)]) = l4;
             }
         };
-        auto mma = [&](int buf) {
+        // U0 / U1: whether the wave's fragment i = 0 / 1 multiplies this block
+        auto mma_sel = [&](int buf, auto U0, auto U1) {
+            constexpr bool u[2] = {decltype(U0)::value, decltype(U1)::value};
 #pragma unroll
             for (int c = 0; c < MF_BK / 16; c++) {
                 bf16x8 fah[2], fal[2], fbh[NJ], fbl[NJ];
 #pragma unroll
                 for (int i = 0; i < 2; i++) {
+                    if (!u[i]) continue;
                     const int oa = bf_off(wm * 64 + i * 32 + li, 16 * c + 8 * h);
                     fah[i] = *reinterpret_cast<const bf16x8 *>(&Ah[buf][oa]);
                     fal[i] = *reinterpret_cast<const bf16x8 *>(&Al[buf][oa]);
@@ -526,27 +576,38 @@ __global__ __launch_bounds__(128 * NWN) void gemm_bf16x3(RunArgs a)
                     fbh[j] = *reinterpret_cast<const bf16x8 *>(&Bh[buf][ob]);
                     fbl[j] = *reinterpret_cast<const bf16x8 *>(&Bl[buf][ob]);
                 }
-                // the three products go round the 2 x NJ accumulators in turn, so
+                // the three products go round the accumulators in turn, so
                 // consecutive MFMAs never depend on each other
 #pragma unroll
                 for (int i = 0; i < 2; i++)
+                    if (u[i])
 #pragma unroll
-                    for (int j = 0; j < NJ; j++)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fal[i], fbh[j], acc[i][j], 0, 0, 0);
-#pragma unroll
-                for (int i = 0; i < 2; i++)
-#pragma unroll
-                    for (int j = 0; j < NJ; j++)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fah[i], fbl[j], acc[i][j], 0, 0, 0);
+                        for (int j = 0; j < NJ; j++)
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fal[i], fbh[j], acc[i][j], 0, 0, 0);
 #pragma unroll
                 for (int i = 0; i < 2; i++)
+                    if (u[i])
 #pragma unroll
-                    for (int j = 0; j < NJ; j++)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fah[i], fbh[j], acc[i][j], 0, 0, 0);
+                        for (int j = 0; j < NJ; j++)
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fah[i], fbl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+                    if (u[i])
+#pragma unroll
+                        for (int j = 0; j < NJ; j++)
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fah[i], fbh[j], acc[i][j], 0, 0, 0);
             }
         };
         const KRange kr = tile_krange_uniform(E, r);
         const int nk = kr.count();
+        const FragRange fr = tile_frags(E, r);
+        auto mma = [&](int buf, int kb) {
+            const int b = kr.half_block(kb);
+            const bool a0 = fr.active(2 * wm, b), a1 = fr.active(2 * wm + 1, b);
+            if (a0 && a1) mma_sel(buf, Use<true>(), Use<true>());
+            else if (a0) mma_sel(buf, Use<true>(), Use<false>());
+            else if (a1) mma_sel(buf, Use<false>(), Use<true>());
+        };
         if (nk > 0) {
             // refills past the last block reload the last block (in bounds, never
             // stored), so every iteration issues the same loads and the waits before
@@ -569,7 +630,7 @@ __global__ __launch_bounds__(128 * NWN) void gemm_bf16x3(RunArgs a)
                 __builtin_amdgcn_sched_barrier(0);
                 lstore(nxt, buf ^ 1);
                 gload(nxt, blk(kb + 3));
-                mma(buf);
+                mma(buf, kb);
                 __syncthreads();
             };
             int kb = 0;
@@ -605,7 +666,7 @@ __global__ __launch_bounds__(128 * NWN) void gemm_bf16x3(RunArgs a)
         }
         __syncthreads();
         if (tid < r.ncol) gst<double>(E.normpart + (size_t)s_tr[tid] * E.R + r.rt, s_norm[0][tid] + s_norm[1][tid]);
-        if (tid == 0) atomicAdd(&E.counters[2], (unsigned long long)r.ncol * nk);
+        if (tid == 0) atomicAdd(&E.counters[2], (unsigned long long)r.ncol * fr.work());
         __syncthreads();
     }
 }
@@ -677,6 +738,7 @@ __global__ __launch_bounds__(512) void gemm_f64(RunArgs a)
         const double *pb0 = &Bs[(wn * (BN / 2) + li) * F64_PITCH + 8 * qq];
         const KRange kr = tile_krange(E, r);
         const int nk = kr.count();
+        const FragRange fr = tile_frags(E, r);      // the wave's 32 rows are fragment wm
         if (nk > 0) {
             gload(kr.at(0));
             lstore();
@@ -685,8 +747,10 @@ __global__ __launch_bounds__(512) void gemm_f64(RunArgs a)
         for (int kb = 0; kb < nk; kb++) {
             const bool more = kb + 1 < nk;
             if (more) gload(kr.at(kb + 1));
+            const bool act = fr.active(wm, kr.half_block(kb));
 #pragma unroll
             for (int half = 0; half < 2; half++) {
+                if (!act) break;
                 double2v fa[2][2], fb[NJ][2];
 #pragma unroll
                 for (int i = 0; i < 2; i++) {
@@ -746,7 +810,7 @@ __global__ __launch_bounds__(512) void gemm_f64(RunArgs a)
         if (tid < r.ncol)
             gst<double>(E.normpart + (size_t)s_tr[tid] * E.R + r.rt,
                         ((s_norm[0][tid] + s_norm[1][tid]) + s_norm[2][tid]) + s_norm[3][tid]);
-        if (tid == 0) atomicAdd(&E.counters[2], (unsigned long long)r.ncol * nk);
+        if (tid == 0) atomicAdd(&E.counters[2], (unsigned long long)r.ncol * fr.work());
         __syncthreads();
     }
 }
