@@ -1359,6 +1359,72 @@ static int read_cols(Runner *r, unsigned long long *cols)
     return QJX_OK;
 }
 
+// Optional events recorded around the launches of one iteration.
+struct IterEvents {
+    hipEvent_t before_plan = nullptr, before_gemm = nullptr, after_gemm = nullptr, after_tail = nullptr;
+};
+
+// One iteration (plan, GEMM, decide + jump) with each launch checked on its own: a
+// launch the runtime rejects is reported only through hipGetLastError, and inside a
+// stream capture it would otherwise surface as an invalidated capture with no cause.
+// On failure `what` names the step.
+static hipError_t iteration_checked(Runner *r, const RunArgs &a, std::string *what, const IterEvents &ev = {})
+{
+    hipError_t e = hipSuccess;
+    auto record = [&](hipEvent_t x) {
+        if (x && (e = hipEventRecord(x, r->stream)) != hipSuccess) *what = "event record";
+        return e == hipSuccess;
+    };
+    auto launched = [&](const char *name) {
+        if ((e = hipGetLastError()) != hipSuccess) *what = name;
+        return e == hipSuccess;
+    };
+    if (!record(ev.before_plan)) return e;
+    launch_plan(a, r->stream);
+    if (!launched("plan launch") || !record(ev.before_gemm)) return e;
+    launch_gemm(r->prec, r->gemm_kind, a, r->grid, r->stream);
+    if (!launched("GEMM launch") || !record(ev.after_gemm)) return e;
+    launch_tail(r->prec, a, r->grid, r->stream);
+    if (!launched("decide/jump launch") || !record(ev.after_tail)) return e;
+    return hipSuccess;
+}
+
+static int iteration(Runner *r, const RunArgs &a, const IterEvents &ev = {})
+{
+    std::string what;
+    const hipError_t e = iteration_checked(r, a, &what, ev);
+    return e == hipSuccess ? QJX_OK : fail(QJX_ERR_HIP, what + ": " + hipGetErrorString(e));
+}
+
+// Captures two iterations (parity 0 then 1) into the runner's graph; every argument
+// lives in device memory, so later table changes need no new capture.  A capture
+// that only came back invalidated -- work on another thread that synchronises
+// implicitly with this stream can do that, and nothing was executed -- leaves no
+// graph: the caller launches eagerly and the next chunk captures again.
+static int capture_iterations(Runner *r)
+{
+    HIP_TRY(hipGetLastError());            // an earlier launch error is reported as itself
+    HIP_TRY(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
+    std::string what;
+    hipError_t le = iteration_checked(r, run_args(r, 0), &what);
+    if (le == hipSuccess) le = iteration_checked(r, run_args(r, 1), &what);
+    hipGraph_t g = nullptr;
+    const hipError_t ee = hipStreamEndCapture(r->stream, &g);   // always ends the capture
+    if (le == hipSuccess && ee == hipSuccess) {
+        r->graph = g;
+        HIP_TRY(hipGraphInstantiate(&r->exec, r->graph, nullptr, nullptr, 0));
+        return QJX_OK;
+    }
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    const bool invalidated = (le == hipSuccess || le == hipErrorStreamCaptureInvalidated) &&
+                             (ee == hipSuccess || ee == hipErrorStreamCaptureInvalidated);
+    if (invalidated) return QJX_OK;
+    return fail(QJX_ERR_HIP, "capture of the iteration graph failed: " +
+                                 (le != hipSuccess ? what + ": " + hipGetErrorString(le)
+                                                   : std::string(hipGetErrorString(ee))));
+}
+
 // `iters` iterations with the GEMM bracketed by events: device ms of the GEMM
 // launches and list entries processed.
 static int timed_iterations(Runner *r, int iters, double *ms, unsigned long long *cols)
@@ -1372,14 +1438,11 @@ static int timed_iterations(Runner *r, int iters, double *ms, unsigned long long
     int rc = read_cols(r, &c0);
     if (rc) return rc;
     for (int k = 0; k < iters; k++) {
-        const RunArgs a = run_args(r, k & 1);
-        launch_plan(a, r->stream);
-        HIP_TRY(hipEventRecord(r->ev[2 * k], r->stream));
-        launch_gemm(r->prec, r->gemm_kind, a, r->grid, r->stream);
-        HIP_TRY(hipEventRecord(r->ev[2 * k + 1], r->stream));
-        launch_tail(r->prec, a, r->grid, r->stream);
+        IterEvents ev;
+        ev.before_gemm = r->ev[2 * k];
+        ev.after_gemm = r->ev[2 * k + 1];
+        if ((rc = iteration(r, run_args(r, k & 1), ev))) return rc;
     }
-    HIP_TRY(hipGetLastError());
     rc = read_cols(r, &c1);
     if (rc) return rc;
     double t = 0.0;
@@ -1474,16 +1537,14 @@ int runner_chunk(Runner *r, int iters)
             for (size_t i = old; i < r->ev.size(); i++) HIP_TRY(hipEventCreate(&r->ev[i]));
         }
         for (int k = 0; k < iters; k++) {
-            const RunArgs a = run_args(r, k & 1);
-            HIP_TRY(hipEventRecord(r->ev[4 * k], r->stream));
-            launch_plan(a, r->stream);
-            HIP_TRY(hipEventRecord(r->ev[4 * k + 1], r->stream));
-            launch_gemm(r->prec, r->gemm_kind, a, r->grid, r->stream);
-            HIP_TRY(hipEventRecord(r->ev[4 * k + 2], r->stream));
-            launch_tail(r->prec, a, r->grid, r->stream);
-            HIP_TRY(hipEventRecord(r->ev[4 * k + 3], r->stream));
+            IterEvents ev;
+            ev.before_plan = r->ev[4 * k];
+            ev.before_gemm = r->ev[4 * k + 1];
+            ev.after_gemm = r->ev[4 * k + 2];
+            ev.after_tail = r->ev[4 * k + 3];
+            const int rc = iteration(r, run_args(r, k & 1), ev);
+            if (rc) return rc;
         }
-        HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(r->stream));
         for (int k = 0; k < iters; k++) {
             float p = 0.f, g = 0.f, t = 0.f;
@@ -1495,20 +1556,19 @@ int runner_chunk(Runner *r, int iters)
             r->tail_ms += t;
         }
         r->gemm_launches += iters;
-    } else if (r->graphs) {
-        if (!r->exec) {
-            // two iterations (parity 0 then 1) captured once; every argument lives in
-            // device memory, so later table changes need no new capture
-            HIP_TRY(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
-            launch_iteration(r->prec, r->gemm_kind, run_args(r, 0), r->grid, r->stream);
-            launch_iteration(r->prec, r->gemm_kind, run_args(r, 1), r->grid, r->stream);
-            HIP_TRY(hipStreamEndCapture(r->stream, &r->graph));
-            HIP_TRY(hipGraphInstantiate(&r->exec, r->graph, nullptr, nullptr, 0));
-        }
-        for (int k = 0; k < iters; k += 2) HIP_TRY(hipGraphLaunch(r->exec, r->stream));
     } else {
-        for (int k = 0; k < iters; k++) launch_iteration(r->prec, r->gemm_kind, run_args(r, k & 1), r->grid, r->stream);
-        HIP_TRY(hipGetLastError());
+        if (r->graphs && !r->exec) {
+            const int rc = capture_iterations(r);
+            if (rc) return rc;
+        }
+        if (r->exec) {
+            for (int k = 0; k < iters; k += 2) HIP_TRY(hipGraphLaunch(r->exec, r->stream));
+        } else {
+            for (int k = 0; k < iters; k++) {
+                const int rc = iteration(r, run_args(r, k & 1));
+                if (rc) return rc;
+            }
+        }
     }
     r->iterations += iters;
     return QJX_OK;

@@ -134,7 +134,12 @@ extern "C" int qjx_pool_create(int precision, double hbm_budget_gib, int max_tra
         return pool_fail(QJX_ERR_HIP, "no HIP device available");
     }
     p->budget = hbm_budget_gib > 0 ? (size_t)(hbm_budget_gib * (1ULL << 30)) : (size_t)(0.8 * (double)freeb);
-    if (hipStreamCreate(&p->stream) != hipSuccess) {
+    // Non-blocking: the builder thread's null-stream calls (hipMemcpy, hipMemset,
+    // hipMalloc) then neither wait for the running chunk nor invalidate a graph
+    // capture in progress on this stream.  Every hand-over between the two is
+    // ordered by the host instead: a build ends with hipStreamSynchronize on the
+    // job's own stream, and results are read after synchronising this one.
+    if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
         delete p;
         return pool_fail(QJX_ERR_HIP, "hipStreamCreate failed");
     }
