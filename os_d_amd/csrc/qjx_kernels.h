@@ -32,10 +32,9 @@ struct StepGrid {
     int waves = 4; // waves per GEMM workgroup (8: the 3xBF16 kernel's 2 x 4 wave layout)
     int jdim = kMaxDim; // largest Hilbert dimension of the table (jump kernel LDS)
 };
-// One propagation iteration over the ensembles of a.ens: plan, grouped GEMM,
-// decide (crossing test, bisection bookkeeping, next lists), jumps.
-void launch_iteration(int prec, int gemm_kind, const RunArgs &a, const StepGrid &g, hipStream_t s);
-// the three parts of an iteration, for per-kernel timing
+// The three launches of one propagation iteration over the ensembles of a.ens:
+// plan, grouped GEMM, then decide (crossing test, bisection bookkeeping, next
+// lists) and jumps.  The caller checks each launch.
 void launch_plan(const RunArgs &a, hipStream_t s);
 void launch_gemm(int prec, int gemm_kind, const RunArgs &a, const StepGrid &g, hipStream_t s);
 void launch_tail(int prec, const RunArgs &a, const StepGrid &g, hipStream_t s);

@@ -1197,13 +1197,6 @@ void launch_tail(int prec, const RunArgs &a, const StepGrid &g, hipStream_t s)
     else jump_kernel<float><<<g.jump, kJumpThreads, lds, s>>>(a, g.jdim);
 }
 
-void launch_iteration(int prec, int gemm_kind, const RunArgs &a, const StepGrid &g, hipStream_t s)
-{
-    launch_plan(a, s);
-    launch_gemm(prec, gemm_kind, a, g, s);
-    launch_tail(prec, a, g, s);
-}
-
 int gemm_blocks_per_cu(int prec, int gemm_kind, int bn, int waves)
 {
     int n = 0;
