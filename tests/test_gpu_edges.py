@@ -1,6 +1,8 @@
 """Edges of the quantum-jump path on the GPU: the smallest dimension and a single
-trajectory against the oracle, the largest dimension through invariants that need
-no CPU run, and an empty ensemble.
+trajectory against the oracle, dimensions at the GEMM's 128-row tile edges against
+the oracle (on horizons whose conditioning the test checks first) and band against
+dense, the largest dimension through invariants that need no CPU run, and an empty
+ensemble.
 
 Reference: the dimer's Hilbert dimension is N + 1 (DimerNewDelBehaviour::init_sizes,
 newdel.cpp:3-37); the engine accepts N in [1, 2047].  Per trajectory the MCG59
@@ -53,6 +55,95 @@ def test_fp64_parity_smallest_dimension_and_one_trajectory(oracle_mod, N, M, see
     np.testing.assert_allclose(se["psi"], so["psi"], rtol=0, atol=1e-10)
     assert e.stats()["matvecs"] == o.num_matvecs()
     e.close()
+
+
+# d = 128, 129, 256, 257: one full 128-row tile, a tile plus one padded row, and the
+# same at two tiles (the GEMM's row tiles, band tables and fragment ranges)
+TILE_EDGE_N = [127, 128, 255, 256]
+
+
+def _ulp_sensitivity(oracle_mod, m, M, seed, D, periods):
+    """Run the oracle twice, the second time with every propagator entry moved by one
+    unit in the last place (random sign): whether the discrete path (jumps, eta, RNG,
+    times) stays the same, and the largest change of the final states.  Any two FP64
+    implementations that sum in different orders differ by about this much."""
+    a = oracle_mod.Ensemble(m, M, seed=seed, jump=2)
+    b = oracle_mod.Ensemble(m, M, seed=seed, jump=2)
+    rng = np.random.default_rng(0)
+    eps = 2.0 ** -53
+    for br in range(2):
+        for lvl in range(D):
+            U = a.get_propagator(br, lvl)
+            re = U.real * (1 + eps * rng.choice([-1.0, 1.0], U.shape))
+            im = U.imag * (1 + eps * rng.choice([-1.0, 1.0], U.shape))
+            b.set_propagator(br, lvl, re + 1j * im)
+    for o in (a, b):
+        o.trans(1)
+        o.periods(periods, 1)
+    sa, sb = a.get_state(), b.get_state()
+    same = all(np.array_equal(sa[k], sb[k]) for k in ("rng", "eta", "t", "njumps"))
+    return same, float(np.abs(sa["psi"] - sb["psi"]).max())
+
+
+@pytest.mark.parametrize("N", TILE_EDGE_N)
+def test_fp64_parity_at_row_tile_edges(oracle_mod, N):
+    """The same jumps, eta values, RNG states and times as the oracle (shared tree)
+    where the Hilbert dimension fills a row tile exactly or spills one row into the
+    next.  gamma = 0.1 keeps the horizon well conditioned: there, a one-ulp change of
+    the propagators moves the oracle's own final states by under 1e-13 and changes no
+    jump.  At gamma = 0.5 it moves them by up to 7e-10 at d = 128 and flips a near-tie
+    crossing at d = 256, so no FP64 implementation with another summation order
+    could match bit for bit."""
+    from os_d_amd import qjx
+    M, S, D, gamma, seed, periods = 24, 4, 6, 0.1, 11, 2
+    _say("case: edges fp64 row-tile parity N=%d (d=%d) M=%d" % (N, N + 1, M))
+    m = oracle_mod.model(N, S=S, D=D, gamma=gamma)
+    same, dpsi = _ulp_sensitivity(oracle_mod, m, M, seed, D, periods)
+    assert same and dpsi < 1e-12, (same, dpsi)
+    o = oracle_mod.Ensemble(m, M, seed=seed, jump=2)
+    e = qjx.Engine(N, ex_deep=D, steps_per_period=S, precision=qjx.FP64, diss_gamma=gamma)
+    e.ensemble_init(M, seed=seed, jump=2)
+    for b in range(2):
+        for lvl in range(D):
+            e.set_propagator(b, lvl, o.get_propagator(b, lvl))
+    o.trans(1)
+    e.trans(1)
+    o.periods(periods, 1)
+    e.run_periods(periods, 1)
+    so, se = o.get_state(), e.get_state()
+    for k in ("rng", "eta", "t", "njumps"):
+        assert np.array_equal(so[k], se[k]), k
+    assert int(se["njumps"].sum()) > 0
+    for tr in range(M):
+        assert np.array_equal(o.get_jumps(tr)["times"], e.get_jumps(tr)["times"]), tr
+    np.testing.assert_allclose(se["psi"], so["psi"], rtol=0, atol=1e-10)
+    assert e.stats()["matvecs"] == o.num_matvecs()
+    e.close()
+
+
+def _edge_run(qjx, N, prec):
+    e = qjx.Engine(N, ex_deep=8, steps_per_period=16, precision=prec, diss_gamma=0.3)
+    e.ensemble_init(200, seed=4, jump=2, log_capacity=1024)
+    e.trans(0)
+    e.run_steps(24, 1)
+    s = e.get_state()
+    e.close()
+    return s
+
+
+@pytest.mark.parametrize("prec", [1, 2])
+@pytest.mark.parametrize("N", [128, 256])
+def test_band_skipping_exact_at_row_tile_edges(N, prec, monkeypatch):
+    """d = 129 and 257 (one padded row in the last tile): skipping the band's exact
+    zero blocks and fragments leaves every state bit unchanged."""
+    from os_d_amd import qjx
+    _say("case: edges band vs dense N=%d prec=%d" % (N, prec))
+    s = _edge_run(qjx, N, prec)
+    assert s["njumps"].sum() > 0
+    monkeypatch.setenv("QJX_DENSE", "1")
+    s2 = _edge_run(qjx, N, prec)
+    for k in ("psi", "eta", "t", "njumps", "rng"):
+        assert np.array_equal(s[k], s2[k]), k
 
 
 def _max_dim_run(qjx, prec, M, steps):
