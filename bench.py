@@ -485,8 +485,8 @@ def main():
                           "launches": 3}), flush=True)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs the trajectory engine on a single "
-              "device (multi-GPU sharding is described in DESIGN.md, not implemented).",
+        print("bench.py measures one GPU: the trajectory engine runs on a single device "
+              "(splitting the trajectories over several GPUs is designed, not implemented).",
               file=sys.stderr)
         sys.exit(2)
 
