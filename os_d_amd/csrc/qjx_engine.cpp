@@ -1397,10 +1397,10 @@ static int iteration(Runner *r, const RunArgs &a, const IterEvents &ev = {})
 }
 
 // Captures two iterations (parity 0 then 1) into the runner's graph; every argument
-// lives in device memory, so later table changes need no new capture.  A capture
-// that only came back invalidated -- work on another thread that synchronises
-// implicitly with this stream can do that, and nothing was executed -- leaves no
-// graph: the caller launches eagerly and the next chunk captures again.
+// lives in device memory, so later table changes need no new capture.  A HIP call
+// on another thread that synchronises with this stream invalidates the capture,
+// and the runtime can leave the stream in that state, so an invalidated capture is
+// an error: owners with other threads defer captures while those threads work.
 static int capture_iterations(Runner *r)
 {
     HIP_TRY(hipGetLastError());            // an earlier launch error is reported as itself
@@ -1411,15 +1411,24 @@ static int capture_iterations(Runner *r)
     hipGraph_t g = nullptr;
     const hipError_t ee = hipStreamEndCapture(r->stream, &g);   // always ends the capture
     if (le == hipSuccess && ee == hipSuccess) {
+        const hipError_t ie = hipGraphInstantiate(&r->exec, g, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            r->exec = nullptr;
+            return fail(ie == hipErrorOutOfMemory ? QJX_ERR_NOMEM : QJX_ERR_HIP,
+                        std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+        }
         r->graph = g;
-        HIP_TRY(hipGraphInstantiate(&r->exec, r->graph, nullptr, nullptr, 0));
         return QJX_OK;
     }
     if (g) (void)hipGraphDestroy(g);
     (void)hipGetLastError();
     const bool invalidated = (le == hipSuccess || le == hipErrorStreamCaptureInvalidated) &&
                              (ee == hipSuccess || ee == hipErrorStreamCaptureInvalidated);
-    if (invalidated) return QJX_OK;
+    if (invalidated)
+        return fail(QJX_ERR_HIP, "capture of the iteration graph was invalidated: a HIP call on another "
+                                 "thread synchronised with the stream during the capture (keep such calls "
+                                 "out of the capture, or set QJX_GRAPH=0)");
     return fail(QJX_ERR_HIP, "capture of the iteration graph failed: " +
                                  (le != hipSuccess ? what + ": " + hipGetErrorString(le)
                                                    : std::string(hipGetErrorString(ee))));
@@ -1557,7 +1566,7 @@ int runner_chunk(Runner *r, int iters)
         }
         r->gemm_launches += iters;
     } else {
-        if (r->graphs && !r->exec) {
+        if (r->graphs && !r->exec && !r->defer_capture) {
             const int rc = capture_iterations(r);
             if (rc) return rc;
         }

@@ -314,10 +314,8 @@ extern "C" int qjx_lindblad_run_periods(qjx_lindblad *l, int n_periods)
     if (!l) return lb_fail(QJX_ERR_ARG, "null argument");
     if (n_periods < 0) return lb_fail(QJX_ERR_ARG, "negative period count");
     if (n_periods == 0) return QJX_OK;
-    // one captured RK4 step per half period's drive value, replayed.  A capture
-    // that only came back invalidated (legacy null-stream work on another thread
-    // synchronises implicitly with this stream; nothing was executed) leaves no
-    // graph: this call launches that step eagerly and the next call captures again.
+    // one captured RK4 step per half period's drive value, replayed; each launch of
+    // the capture is checked, so a rejected launch is reported as itself
     for (int b = 0; b < 2; b++) {
         if (l->exec[b]) continue;
         LB_TRY(hipGetLastError());                 // an earlier launch error, as itself
@@ -327,28 +325,25 @@ extern "C" int qjx_lindblad_run_periods(qjx_lindblad *l, int n_periods)
         hipGraph_t g = nullptr;
         const hipError_t ee = hipStreamEndCapture(l->stream, &g);   // always ends the capture
         if (le == hipSuccess && ee == hipSuccess) {
+            const hipError_t ie = hipGraphInstantiate(&l->exec[b], g, nullptr, nullptr, 0);
+            if (ie != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                l->exec[b] = nullptr;
+                return lb_fail(ie == hipErrorOutOfMemory ? QJX_ERR_NOMEM : QJX_ERR_HIP,
+                               std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+            }
             l->graph[b] = g;
-            LB_TRY(hipGraphInstantiate(&l->exec[b], l->graph[b], nullptr, nullptr, 0));
             continue;
         }
         if (g) (void)hipGraphDestroy(g);
         (void)hipGetLastError();
-        if ((le != hipSuccess && le != hipErrorStreamCaptureInvalidated) ||
-            (ee != hipSuccess && ee != hipErrorStreamCaptureInvalidated))
-            return lb_fail(QJX_ERR_HIP, std::string("capture of the RK4 step failed: ") +
-                                            hipGetErrorString(le != hipSuccess ? le : ee));
+        return lb_fail(QJX_ERR_HIP, std::string("capture of the RK4 step failed: ") +
+                                        hipGetErrorString(le != hipSuccess ? le : ee));
     }
     const int half = l->steps / 2;
     for (int p = 0; p < n_periods; p++)
         for (int b = 0; b < 2; b++)
-            for (int s = 0; s < half; s++) {
-                if (l->exec[b]) {
-                    LB_TRY(hipGraphLaunch(l->exec[b], l->stream));
-                } else {
-                    launch_step(l, b == 0 ? l->E0 : l->E1, l->stream);
-                    LB_TRY(hipGetLastError());
-                }
-            }
+            for (int s = 0; s < half; s++) LB_TRY(hipGraphLaunch(l->exec[b], l->stream));
     LB_TRY(hipStreamSynchronize(l->stream));
     return QJX_OK;
 }

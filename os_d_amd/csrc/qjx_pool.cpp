@@ -270,6 +270,13 @@ extern "C" int qjx_pool_step(qjx_pool *p, int *running, int *queued)
     const bool busy = !p->running.empty();
     // the launch shape is tuned once, after the first jobs' bisections have spread
     if (busy && p->cycles >= 4 && (rc = qjx::runner_autotune(p->runner))) return rc;
+    // no graph capture while the builder thread may call the runtime: its calls can
+    // invalidate a capture in progress.  An idle builder only waits on `cv`, and no
+    // build starts before this chunk is launched.
+    {
+        std::lock_guard<std::mutex> lk(p->mu);
+        p->runner->defer_capture = p->building > 0;
+    }
     if (busy && (rc = qjx::runner_chunk(p->runner, 16))) return rc;
     if (busy) p->cycles++;
     // built jobs join the table at the next cycle; admission reserves budget and

@@ -30,6 +30,8 @@ struct Runner {
     int nens = 0;
     std::vector<Ens> h_ens;           // host copy of the table
     bool graphs = true;               // replay a captured pair of iterations
+    bool defer_capture = false;       // capture nothing now (another thread of the owner
+                                      // may call the runtime): launch eagerly meanwhile
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool profiling = false;           // time each kernel with events (no graphs)
