@@ -19,13 +19,19 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed FP32 instructions are disabled on purpose (see the design notes).
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-LIB_SOURCES = ["qjx_kernels.hip", "qjx_step.hip", "qjx_lindblad.hip", "qjx_rk.hip", "qjx_engine.cpp",
+LIB_SOURCES = ["qjx_kernels.hip", "qjx_step.hip", "qjx_step_f64.hip", "qjx_lindblad.hip", "qjx_rk.hip", "qjx_engine.cpp",
                "qjx_config.cpp", "qjx_pool.cpp", "qjx_lpn.cpp"]
 # The throughput probe keeps MFMA accumulators in ordinary VGPRs: with the default
 # AGPR form the compiler copies each FP64 accumulator chain through v_accvgpr moves
 # every iteration and the probe would measure the moves, not the matrix core.
 PROBE_SOURCE = "qjx_peak.hip"
 PROBE_FLAGS = ["-mllvm", "-amdgpu-mfma-vgpr-form"]
+# The step kernels are scheduled to keep memory clauses together: on the d=1001
+# workload (1e5 trajectories) the FP32 form runs 1.5% faster and 3xBF16 0.2%
+# faster than with the default scheduler, with bit-identical results.  The FP64
+# GEMM is slower that way, so it lives in qjx_step_f64.hip, built without the flag.
+STEP_SOURCE = "qjx_step.hip"
+STEP_FLAGS = ["-mllvm", "-amdgpu-sched-strategy=max-memory-clause"]
 
 
 def _run(cmd):
@@ -44,7 +50,7 @@ def build_lib():
 
     def compile_one(src):
         obj = os.path.join(LIBDIR, os.path.splitext(src)[0] + ".o")
-        extra = PROBE_FLAGS if src == PROBE_SOURCE else []
+        extra = PROBE_FLAGS if src == PROBE_SOURCE else STEP_FLAGS if src == STEP_SOURCE else []
         _run([HIPCC] + HIP_FLAGS + extra + inc + ["-c", src, "-o", obj])
         return obj
 

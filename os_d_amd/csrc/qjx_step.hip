@@ -49,6 +49,13 @@ __device__ inline void gst(T *p, V v)
 }
 
 // largest item with pre[item] <= t (pre non-decreasing, pre[n] > t)
+// The FP64 GEMM is compiled in a translation unit of its own (qjx_step_f64.hip
+// includes this file with QJX_STEP_F64 defined): the rest of this file is built with
+// the scheduler that keeps memory clauses together, which speeds the FP32 and
+// 3xBF16 GEMMs and slows the FP64 one.
+void launch_gemm_f64(const RunArgs &a, const StepGrid &g, hipStream_t s);
+hipError_t gemm_f64_occupancy(int *n, int bn);
+
 __device__ inline int plan_search(const int *pre, int n, int t)
 {
     int lo = 0, hi = n;
@@ -67,6 +74,7 @@ __device__ inline int plan_search(const int *pre, int n, int t)
 constexpr int kPlanThreads = 1024;
 static_assert(kPlanItems == 4 * kPlanThreads, "plan layout");
 
+#if !defined(QJX_STEP_F64)
 __global__ __launch_bounds__(kPlanThreads) void plan_kernel(RunArgs a)
 {
     __shared__ int s_t[kPlanThreads];
@@ -127,6 +135,7 @@ __global__ __launch_bounds__(kPlanThreads) void plan_kernel(RunArgs a)
     }
     if (tid < 8) pl->xnext[tid] = 0;
 }
+#endif  // !QJX_STEP_F64
 
 // Claim this persistent workgroup's next tile.  The tile space is cut into 8
 // contiguous ranges, one per XCD (workgroup w runs on XCD w % 8), so the row tiles
@@ -293,6 +302,7 @@ __device__ inline FragRange tile_frags(const Ens &E, const TileRef &r)
 template <bool V>
 using Use = std::integral_constant<bool, V>;
 
+#if !defined(QJX_STEP_F64)
 template <int BN, int NWN>
 __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) void gemm_f32(RunArgs a)
 {
@@ -671,6 +681,8 @@ __global__ __launch_bounds__(128 * NWN) void gemm_bf16x3(RunArgs a)
     }
 }
 
+#endif  // !QJX_STEP_F64
+#if defined(QJX_STEP_F64)
 // ---------------------------------------------------------------------------
 // GEMM tile, FP64: v_mfma_f64_16x16x4f64
 //   A (16 x 4): lane l holds A[l % 16][l / 16]; B (4 x 16): B[l / 16][l % 16]
@@ -815,6 +827,19 @@ __global__ __launch_bounds__(512) void gemm_f64(RunArgs a)
     }
 }
 
+void launch_gemm_f64(const RunArgs &a, const StepGrid &g, hipStream_t s)
+{
+    if (a.bn == 64) gemm_f64<64><<<g.gemm, 512, 0, s>>>(a);
+    else gemm_f64<128><<<g.gemm, 512, 0, s>>>(a);
+}
+
+hipError_t gemm_f64_occupancy(int *n, int bn)
+{
+    return bn == 64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(n, gemm_f64<64>, 512, 0)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(n, gemm_f64<128>, 512, 0);
+}
+#endif  // QJX_STEP_F64
+#if !defined(QJX_STEP_F64)
 // ---------------------------------------------------------------------------
 // GEMM tile, vector FMA (reference form for checks and tiny sizes): 64 x 64 tile,
 // 4 x 4 register tile per thread, K block 16.
@@ -1175,8 +1200,7 @@ void launch_gemm(int prec, int gemm_kind, const RunArgs &a, const StepGrid &g, h
         if (prec == kFP64) gemm_fma<double><<<g.gemm, 256, 0, s>>>(a);
         else gemm_fma<float><<<g.gemm, 256, 0, s>>>(a);
     } else if (prec == kFP64) {
-        if (a.bn == 64) gemm_f64<64><<<g.gemm, 512, 0, s>>>(a);
-        else gemm_f64<128><<<g.gemm, 512, 0, s>>>(a);
+        launch_gemm_f64(a, g, s);
     } else if (prec == kBF16x3) {
         if (a.bn == 256) gemm_bf16x3<256, 4><<<g.gemm, 512, 0, s>>>(a);
         else if (g.waves == 8) gemm_bf16x3<128, 4><<<g.gemm, 512, 0, s>>>(a);
@@ -1206,8 +1230,7 @@ int gemm_blocks_per_cu(int prec, int gemm_kind, int bn, int waves)
         e = prec == kFP64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_fma<double>, 256, 0)
                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_fma<float>, 256, 0);
     else if (prec == kFP64)
-        e = narrow ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_f64<64>, 512, 0)
-                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_f64<128>, 512, 0);
+        e = gemm_f64_occupancy(&n, bn);
     else if (prec == kBF16x3)
         e = bn == 256 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_bf16x3<256, 4>, 512, 0)
             : waves == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_bf16x3<128, 4>, 512, 0)
@@ -1251,4 +1274,5 @@ void launch_mfma_probe(const float *A32, const float *B32, float *D32, const dou
     mfma_probe_kernel<<<1, 64, 0, s>>>(A32, B32, D32, A64, B64, D64, Abf, Bbf, Dbf);
 }
 
+#endif  // !QJX_STEP_F64
 }  // namespace qjx
