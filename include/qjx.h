@@ -330,7 +330,8 @@ double qjx_pool_clock(const qjx_pool *pool);
  * with QJ (prop_id 0) or RK (prop_id 1) propagation, and write the reference's output
  * files into `path` (core.cpp:918-1084, dump.cpp:26-61).  precision selects the GEMM
  * form of the QJ path; steps_per_period = 0 means the reference's 4 (task 4 fixes it
- * at 4 deep_num_steps). */
+ * at 8 deep_num_steps: 4 level-0 steps of T / (8 deep_num_steps) per sub-period,
+ * init_split_structure_dimer_deep, core.cpp:9928-9981, qj_proc.cpp:15-24, 345-360). */
 int qjx_run_config(const char *config_path, const char *params_path, int precision,
                    int steps_per_period);
 

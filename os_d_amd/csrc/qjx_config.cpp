@@ -471,16 +471,19 @@ extern "C" int qjx_run_config(const char *config_path, const char *params_path, 
             // StdDeep (task 4): a dump after each of the 2 deep_num_steps sub-periods of
             // every period; dump_type / dump_num do not apply (init_dump_periods_deep,
             // newdel_exp.cpp:377-406).  The QJ tree's branch head is T / (2 deep_num_steps)
-            // with two children (init_split_structure_dimer_deep, core.cpp:9928-9981): 4
-            // deep_num_steps top-level steps per period, 2 per sub-period.  The RK step is
+            // with two children (init_split_structure_dimer_deep, core.cpp:9928-9981); each
+            // child halves that dt twice before its first level (init_split_branches,
+            // qj_proc.cpp:15-24), so level 0 is 2 steps of T / (8 deep_num_steps) per child
+            // and a sub-period runs both children (one_sub_period_deep, qj_proc.cpp:345-360):
+            // 4 level-0 steps per sub-period, 8 deep_num_steps per period.  The RK step is
             // T / (rk_ns 2 deep_num_steps) (init_rk_deep, rk_proc.cpp:65-76).
             if (deep_steps < 1) CFG_FAIL(QJX_ERR_ARG, "deep_num_steps must be >= 1");
             const int sub = 2 * deep_steps;
             dps.assign((size_t)sub * c.num_obs_periods + 1, 0);
             for (int pid = 0; pid < c.num_obs_periods; pid++)
                 for (int k = 0; k < sub; k++) dps[(size_t)pid * sub + k + 1] = pid;
-            md.steps_per_period = 4 * deep_steps;
-            sub_steps = c.prop_id == 1 ? c.rk_ns : 2;
+            md.steps_per_period = 8 * deep_steps;
+            sub_steps = c.prop_id == 1 ? c.rk_ns : 4;
         } else {
             if (dump_type != 0 && dump_type != 1) CFG_FAIL(QJX_ERR_ARG, "dump_type must be 0 or 1");
             if (dump_num < 1 || (dump_type == 0 && c.num_obs_periods < dump_num))

@@ -7,6 +7,8 @@ import sys
 import numpy as np
 import pytest
 
+from deep_tree import reference_deep_tree
+
 pytestmark = pytest.mark.gpu
 
 CONFIG = """sys_id 0
@@ -152,9 +154,10 @@ def _deep_files(tmp_path, prop_id, k, rk_ns=2000, drv_type=0):
 @pytest.mark.parametrize("prop_id", [0, 1])
 def test_run_config_deep_outputs_match_oracle(tmp_path, oracle_mod, prop_id):
     """task_id 4 (StdDeepExperimentBehaviour, experiment.cpp:769-862): a dump after
-    every sub-period, 2 deep_num_steps per period.  QJ: the deep tree's branch head is
-    T/(2k) with two children (core.cpp:9928-9981), i.e. 4k top-level steps per period;
-    RK: rk_ns steps per sub-period of T/(2k) (rk_proc.cpp:65-76, core.cpp:490-530)."""
+    every sub-period, 2 deep_num_steps per period.  QJ: the tree and the steps per
+    sub-period come from the reference's node recursion (tests/deep_tree.py: head
+    T/(2k), two children of 2 steps of T/(8k) at level 0); RK: rk_ns steps per
+    sub-period of T/(2k) (rk_proc.cpp:65-76, core.cpp:490-530)."""
     from os_d_amd import qjx
     k, rk_ns = 3, 400
     print("case: run_config deep prop_id=%d k=%d" % (prop_id, k), flush=True)
@@ -163,9 +166,10 @@ def test_run_config_deep_outputs_match_oracle(tmp_path, oracle_mod, prop_id):
     sfx = "_setup(0_4_%d)_rnd(0_1000000)_N(12)_diss(0_0.2000_0.0000)_drv(1_1.5000_1.0000_0.0000)" \
           "_prm(1.0000_0.5000_1.0000)_start(0_0).txt" % prop_id
     if prop_id == 0:
-        m = oracle_mod.model(12, gamma=0.2, S=4 * k, D=12)
+        tree = reference_deep_tree(2 * np.pi, k, 12)
+        m = oracle_mod.model(12, gamma=0.2, S=tree["S"], D=12)
         o = oracle_mod.Ensemble(m, 24, jump=2)
-        per_sub = 2
+        per_sub = tree["level0_steps_per_sub"]
     else:
         m = oracle_mod.model(12, gamma=0.2, D=1)
         o = oracle_mod.Ensemble(m, 24, jump=2, build_propagators=False)
@@ -188,6 +192,10 @@ def test_run_config_deep_outputs_match_oracle(tmp_path, oracle_mod, prop_id):
         f = out + "jump_times_%d" % tr + sfx
         jt = np.atleast_1d(np.loadtxt(f)) if os.path.getsize(f) else np.zeros(0)
         np.testing.assert_allclose(jt, o.get_jumps(tr)["times"], rtol=1e-15 if prop_id else 1e-12)
+        if prop_id == 0 and jt.size:
+            # jump times land on the finest grid of the deep tree, T / (8k 2^(D-1))
+            q = jt / tree["level_dt"][-1]
+            np.testing.assert_allclose(q, np.rint(q), rtol=0, atol=1e-6)
     assert not glob.glob(out + "phi_*")
 
 
