@@ -114,6 +114,13 @@ def _p(a, t=ctypes.c_double):
     return None if a is None else a.ctypes.data_as(_P(t))
 
 
+def default_threads():
+    """OpenMP threads of the oracle: OMP_NUM_THREADS when set (the host share a
+    job may use), else every CPU."""
+    n = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+    return min(n, os.cpu_count() or n)
+
+
 def model(N, prm_E=1.0, prm_U=0.5, prm_J=1.0, drv_ampl=1.5, drv_freq=1.0, gamma=0.1, S=4, D=16):
     return Model(N, prm_E, prm_U, prm_J, drv_ampl, drv_freq, gamma, S, D)
 
@@ -202,7 +209,7 @@ class Ensemble:
         self.m = m
         self.d = m.N + 1
         self.M = num_traj
-        threads = threads or os.cpu_count() or 1
+        threads = threads or default_threads()
         self.h = self.L.qjo_create(ctypes.byref(m), num_traj, seed, mns, start_type, start_state,
                                    jump, threads, 1 if build_propagators else 0)
         if not self.h:
