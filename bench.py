@@ -387,21 +387,20 @@ SERVE_MIXES = {
 }
 
 
-def serving(qjx, seconds, rate, seed, budget_gib, capacity, mix="specified"):
-    """Poisson stream of ensemble jobs through the pool (continuous batching).
-
-    Each job is a dimer parameter point: N from the mix, trajectories log-uniform,
-    periods uniform, A in [0.5, 3.5], U in [0.1, 1.0], gamma in [0.01, 0.5]; the
-    reference's QJ tree (4 top-level steps per period, ex_deep 16); FP32 form.
-    Arrivals stop after `seconds`; latency = completion - submission of the jobs
-    completed within the window."""
+def serve_mix(qjx, pool, seconds, rate, seed, mix, probe=None, probe_at=None, check=False):
+    """Drive `pool` with a Poisson stream of ensemble jobs for `seconds` on the pool
+    clock (arrivals stop then).  Each job is a dimer parameter point: N from the
+    mix, trajectories log-uniform, periods uniform, A in [0.5, 3.5], U in [0.1, 1.0],
+    gamma in [0.01, 0.5]; the reference's QJ tree (4 top-level steps per period,
+    ex_deep 16).  `probe` (submit keywords) adds one extra job at `probe_at` s.
+    Returns the jobs (dicts with the pool's id) and the largest HBM reservation seen;
+    with `check`, every cycle asserts that the reservation stays within the budget."""
     import numpy as np
     spec = SERVE_MIXES[mix]
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=int(rate * seconds * 2) + 16))
     arrivals = arrivals[arrivals < seconds]
-    pool = qjx.Pool(qjx.FP32, hbm_budget_gib=budget_gib, max_trajectories=capacity)
-    jobs = []
+    jobs, peak = [], 0
     t0 = pool.clock()
     nxt = 0
     while True:
@@ -413,29 +412,60 @@ def serving(qjx, seconds, rate, seed, budget_gib, capacity, mix="specified"):
             jid = pool.submit(N, M, P, seed=int(rng.integers(0, 1 << 20)), drv_ampl=float(rng.uniform(0.5, 3.5)),
                               prm_U=float(rng.uniform(0.1, 1.0)), diss_gamma=float(rng.uniform(0.01, 0.5)),
                               steps_per_period=4, ex_deep=16)
-            jobs.append((jid, N, M, P))
+            jobs.append({"id": jid, "N": N, "M": M, "P": P, "probe": False})
             nxt += 1
+        if probe is not None and probe_at is not None and now >= probe_at:
+            jobs.append({"id": pool.submit(**probe), "N": probe["N"], "M": probe["num_trajectories"],
+                         "P": probe["n_periods"], "probe": True})
+            probe_at = None
         if now >= seconds:
             break
         r, q = pool.step()
-        if r == 0 and q == 0 and nxt >= len(arrivals):
+        u = pool.usage()
+        peak = max(peak, u["used_bytes"])
+        if check:
+            assert u["used_bytes"] <= u["budget_bytes"], u
+            assert u["failed"] == 0, u
+        if r == 0 and q == 0 and nxt >= len(arrivals) and probe_at is None:
             break
+    return jobs, peak
+
+
+def serving(qjx, seconds, rate, seed, budget_gib, capacity, mix="specified"):
+    """Poisson stream of ensemble jobs through the pool (continuous batching, FP32
+    form); latency = completion - submission.  p50 / p99 are over the jobs completed
+    within the window; `p99_all_lower_bound_s` also counts every job still queued or
+    running at the end with its age then (a lower bound of its latency)."""
+    pool = qjx.Pool(qjx.FP32, hbm_budget_gib=budget_gib, max_trajectories=capacity)
+    t0 = pool.clock()
+    jobs, peak = serve_mix(qjx, pool, seconds, rate, seed, mix)
     t_end = pool.clock() - t0
-    done, lat, tp, admitted = 0, [], 0.0, 0
-    for jid, N, M, P in jobs:
-        res = pool.result(jid)
-        if res["status"] != 0:
+    done, lat, lat_all, tp, admitted = 0, [], [], 0.0, 0
+    for j in jobs:
+        res = pool.result(j["id"])
+        if res["status"] != 0 or res["t_admit"] > 0:
             admitted += 1
         if res["status"] == 2:
             done += 1
             lat.append(res["t_done"] - res["t_submit"])
-            tp += M * P
+            lat_all.append(lat[-1])
+            tp += j["M"] * j["P"]
+        else:
+            lat_all.append(t_end + t0 - res["t_submit"])
+    u = pool.usage()
     pool.close()
-    lat = sorted(lat)
-    pct = lambda p: lat[min(len(lat) - 1, int(p * len(lat)))] if lat else None
+    spec = SERVE_MIXES[mix]
+
+    def pct(v, p):
+        v = sorted(v)
+        return v[min(len(v) - 1, int(p * len(v)))] if v else None
     return {"mix": mix, "jobs": {"N": spec["N"], "trajectories_log10": spec["logM"], "periods": spec["periods"]},
             "offered_jobs_per_s": rate, "window_s": t_end, "submitted": len(jobs), "admitted": admitted,
-            "completed": done, "p50_latency_s": pct(0.5), "p99_latency_s": pct(0.99),
+            "completed": done, "failed": u["failed"], "running_at_end": u["running"],
+            "queued_at_end": u["queued"] + u["building"],
+            "p50_latency_s": pct(lat, 0.5), "p99_latency_s": pct(lat, 0.99),
+            "p50_all_lower_bound_s": pct(lat_all, 0.5), "p99_all_lower_bound_s": pct(lat_all, 0.99),
+            "peak_reserved_gib": peak / 2 ** 30, "budget_gib": u["budget_bytes"] / 2 ** 30,
             "completed_trajectory_periods_per_s": tp / t_end if t_end > 0 else 0.0}
 
 

@@ -323,6 +323,23 @@ int qjx_pool_step(qjx_pool *pool, int *running, int *queued);
 /* Status and results; mean / energy (num_trajectories doubles) may be NULL. */
 int qjx_pool_result(qjx_pool *pool, int job_id, qjx_job_result *out, double *mean, double *energy);
 double qjx_pool_clock(const qjx_pool *pool);
+/* Admission state: HBM bytes reserved by admitted jobs (built, being built or
+ * running) against the budget, trajectories in the table against the capacity,
+ * and job counts by state.  No reference counterpart (the reference has no
+ * shared device to admit jobs onto). */
+typedef struct qjx_pool_usage {
+    int64_t used_bytes;
+    int64_t budget_bytes;
+    int64_t trajectories;
+    int64_t capacity;
+    int32_t queued;
+    int32_t building;
+    int32_t running;
+    int32_t done;
+    int32_t failed;
+    int32_t pad;
+} qjx_pool_usage;
+int qjx_pool_get_usage(qjx_pool *pool, qjx_pool_usage *out);
 
 /* Drop-in runner: read config.txt / params.txt (reference keys, config.cpp:29-145),
  * run the STD experiment (task_id 1) or the STD deep experiment (task_id 4: a dump

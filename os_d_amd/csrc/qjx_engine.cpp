@@ -129,6 +129,14 @@ static void build_model(qjx_ctx *c)
     }
 }
 
+// Squarings s of the scaling step for exp(A dt): the smallest s >= 0 with
+// ||A dt / 2^s||_1 <= 1.
+static int expm_squarings(double anorm1, double dt)
+{
+    const double a1 = anorm1 * dt;
+    return a1 > 1.0 ? (int)std::ceil(std::log2(a1)) : 0;
+}
+
 // exp(A dt) for the dense complex A on the device by scaling and squaring of a
 // degree-18 Taylor polynomial evaluated with the Paterson-Stockmeyer scheme
 // (M. S. Paterson, L. J. Stockmeyer, SIAM J. Comput. 2 (1973) 60-66):
@@ -140,9 +148,7 @@ static int device_expm(qjx_ctx *c, const double *dA, double anorm1, double dt, d
     const size_t nn2 = (size_t)2 * n * n;
     double *X = work, *X2 = work + nn2, *X3 = work + 2 * nn2, *X4 = work + 3 * nn2,
            *P = work + 4 * nn2, *Tm = work + 5 * nn2;
-    const double a1 = anorm1 * dt;
-    int s = 0;
-    if (a1 > 1.0) s = (int)std::ceil(std::log2(a1));
+    const int s = expm_squarings(anorm1, dt);
     hipStream_t st = c->stream;
     qjx::launch_zaxpby_eye(n, dA, std::ldexp(dt, -s), 0.0, X, st);
     qjx::launch_zgemm(n, X, X, X2, st);
@@ -304,12 +310,14 @@ int qjx::ctx_create_kind(const qjx_model_desc *m, int gemm_kind, qjx_ctx **out)
     double *dA = nullptr, *work = nullptr, *U = nullptr;
     if (hipMalloc((void **)&dA, nn2 * sizeof(double)) != hipSuccess ||
         hipMalloc((void **)&work, 6 * nn2 * sizeof(double)) != hipSuccess ||
-        hipMalloc((void **)&U, nn2 * sizeof(double)) != hipSuccess) {
+        hipMalloc((void **)&U, 2 * nn2 * sizeof(double)) != hipSuccess) {
         dev_free(dA);
         dev_free(work);
         dev_free(U);
         return bail(fail(QJX_ERR_NOMEM, "propagator build workspace does not fit"));
     }
+    bool direct = false;
+    if (const char *env = std::getenv("QJX_EXPM_DIRECT")) direct = std::string(env) == "1";
     for (int b = 0; b < 2 && rc == QJX_OK; b++) {
         const double a1 = norm1_host(c->Hqj[b], c->d);
         // ordered on the engine stream: the previous branch's expm still reads dA
@@ -319,13 +327,28 @@ int qjx::ctx_create_kind(const qjx_model_desc *m, int gemm_kind, qjx_ctx **out)
             rc = fail(QJX_ERR_HIP, "upload of H_eff failed");
             break;
         }
-        for (int i = 0; i < c->D && rc == QJX_OK; i++) {
-            rc = device_expm(c, dA, a1, c->dt[i], U, work);
+        // Deepest level first.  Where level i needs s_i >= 1 squarings, level i + 1
+        // (dt_i / 2) needs s_i - 1 and starts from the same scaled argument
+        // A dt_i / 2^s_i, so its result is exactly the scaling step's value after
+        // s_i - 1 squarings: one more squaring gives level i bit for bit as the
+        // full scaling and squaring would, at the cost of one product instead of
+        // 7 + s_i.  Levels with s_i = 0 take the Taylor polynomial directly.
+        // QJX_EXPM_DIRECT=1 runs the full scaling and squaring at every level.
+        double *Ucur = U, *Uprev = U + nn2;
+        for (int i = c->D - 1; i >= 0 && rc == QJX_OK; i--) {
+            const int si = expm_squarings(a1, c->dt[i]);
+            if (!direct && i + 1 < c->D && si >= 1 && expm_squarings(a1, c->dt[i + 1]) == si - 1) {
+                qjx::launch_zgemm(c->d, Uprev, Uprev, Ucur, c->stream);
+                if (hipGetLastError() != hipSuccess) rc = fail(QJX_ERR_HIP, "propagator squaring failed");
+            } else {
+                rc = device_expm(c, dA, a1, c->dt[i], Ucur, work);
+            }
             if (rc == QJX_OK) {
-                qjx::launch_pack_propagator(c->prec, c->d, c->dp, U, prop_ptr(c, b, i), prop_lo_ptr(c, b, i),
+                qjx::launch_pack_propagator(c->prec, c->d, c->dp, Ucur, prop_ptr(c, b, i), prop_lo_ptr(c, b, i),
                                             c->stream);
                 band_of(c, b, i);
             }
+            std::swap(Ucur, Uprev);
         }
     }
     if (rc == QJX_OK && hipStreamSynchronize(c->stream) != hipSuccess)
@@ -1634,8 +1657,8 @@ size_t model_footprint(const qjx_model_desc *m, int gemm_kind)
     const size_t L = 2 * (size_t)padded_dim(m->N);
     const size_t pe = m->precision == QJX_FP64 ? 8 : 4;     // BF16x3: hi + lo = 4 bytes
     const size_t d = m->N + 1;
-    // propagator tree + the build workspace (8 dense complex d x d, freed after the build)
-    return (size_t)2 * m->ex_deep * L * L * pe + 8 * d * d * 16 + (1 << 20);
+    // propagator tree + the build workspace (9 dense complex d x d, freed after the build)
+    return (size_t)2 * m->ex_deep * L * L * pe + 9 * d * d * 16 + (1 << 20);
 }
 
 size_t ensemble_footprint(const qjx_model_desc *m, int M, int gemm_kind)

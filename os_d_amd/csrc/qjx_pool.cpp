@@ -5,11 +5,25 @@
 // period, experiment.cpp:448-542; parameter sweeps are separate jobs,
 // submit/qjx/submit_dimer.py).  Here every running job is one entry of a shared
 // ensemble table: each propagation iteration is one set of launches over the
-// trajectories of all running jobs, whatever their Hilbert dimension.  Jobs are
-// admitted while their HBM footprint fits the budget, join the table at the next
-// chunk boundary, and retire when their last trajectory has finished; the others
-// keep running.  A trajectory's arithmetic does not depend on which other jobs
-// share the launches, so a job's results are the same alone and in the mix.
+// trajectories of all running jobs of a lane, whatever their Hilbert dimension.
+//
+// Scheduling:
+//   * Admission takes queued jobs in order of expected device work (shortest first)
+//     and skips a job that does not fit rather than blocking the ones behind it.
+//     A job is admitted while its HBM footprint fits the budget and its
+//     trajectories fit the lane's capacity.
+//   * Two lanes, each its own table and runner: short jobs (expected device work
+//     below a threshold) and long ones.  An iteration's time is set by the largest
+//     jobs in its table, and a trajectory needs one iteration per matrix-vector
+//     step of its bisection chain, so a short job sharing iterations with hour-long
+//     jobs would wait on their GEMMs at every step.  In its own lane it iterates at
+//     its own pace; when both lanes have work each cycle gives them about equal
+//     device time.  Long jobs may not take the budget share reserved for short ones.
+//   * Models are built by several builder threads, each on the job's own stream,
+//     while the lanes iterate.
+// A trajectory's arithmetic does not depend on which other jobs share the
+// launches or in which lane it runs, so a job's results are the same alone and in
+// the mix.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,6 +46,7 @@
 namespace {
 
 enum JobState { kQueued = 0, kRunning = 1, kDone = 2, kFailed = 3 };
+enum Lane { kShort = 0, kLong = 1, kLanes = 2 };
 
 struct Job {
     int id = 0;
@@ -39,10 +54,12 @@ struct Job {
     qjx_ensemble_desc e{};
     int n_periods = 0;
     int state = kQueued;
+    int lane = kShort;
     int err = 0;
     std::string msg;
     qjx_ctx *ctx = nullptr;
     size_t footprint = 0;
+    double work = 0.0;                // expected device seconds (admission order, lane)
     double t_submit = 0, t_admit = 0, t_done = 0;
     std::vector<double> norm, mean, m2, energy;
     long long matvecs = 0, jumps = 0;
@@ -54,27 +71,53 @@ int pool_fail(int code, const std::string &msg)
     return code;
 }
 
+// Expected device seconds of a job: M P (S + 2 D J) propagator applications of
+// 8 d^2 flops, J the expected jumps per period (half the largest rate
+// gamma (N + 2) times T), at a dense-equivalent rate of 3e14 flop/s.  Only its
+// order matters: it ranks jobs for admission and picks their lane.
+double expected_work(const qjx_model_desc &m, int M, int periods)
+{
+    const double T = 2.0 * 3.141592653589793 / m.drv_freq;
+    const double J = 0.5 * m.diss_gamma * (m.N + 2) * T;
+    const int S = m.steps_per_period ? m.steps_per_period : 4;
+    const double d = m.N + 1;
+    return (double)M * periods * (S + 2.0 * m.ex_deep * J) * 8.0 * d * d / 3e14;
+}
+
+double env_double(const char *name, double dflt)
+{
+    const char *v = std::getenv(name);
+    return v ? std::atof(v) : dflt;
+}
+
 }  // namespace
 
 struct qjx_pool {
     int prec = 0;
     int gemm_kind = qjx::kGemmMfma;
-    size_t budget = 0, used = 0;
-    int capacity = 0, in_table = 0;
+    size_t budget = 0, used = 0, used_long = 0;
+    int capacity = 0;
     hipStream_t stream = nullptr;
-    qjx::Runner *runner = nullptr;
-    std::vector<Job *> jobs;           // by id
-    std::deque<int> queue;             // waiting for admission
-    std::vector<int> running;          // ids in the table order
-    bool table_dirty = false;
-    long long cycles = 0;                 // scheduling cycles that launched iterations
-    // model builds run on a builder thread so that running jobs keep iterating
-    std::thread builder;
+    struct LaneState {
+        qjx::Runner *runner = nullptr;
+        std::vector<int> running;     // ids in table order
+        bool dirty = false;
+        long long cycles = 0;         // cycles that launched iterations in this lane
+        long long in_table = 0;       // trajectories admitted to this lane
+        int pending = 0;              // admitted jobs not yet running (building or built)
+        int iters = 16;               // iterations of the lane's next chunk
+        hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    } lane[kLanes];
+    double short_s = 2.0;             // expected seconds below which a job is short
+    double reserve = 0.25;            // budget share long jobs may not take
+    std::vector<Job *> jobs;          // by id
+    std::vector<int> queue;           // waiting for admission
+    std::vector<std::thread> builders;
     std::mutex mu;
     std::condition_variable cv;
-    std::deque<int> to_build;          // admitted (budget reserved), not yet built
-    std::deque<int> built;             // built and seeded, waiting to join the table
-    int building = 0;                  // jobs handed to the builder and not yet in `built`
+    std::deque<int> to_build;         // admitted (budget reserved), not yet built
+    std::deque<int> built;            // built and seeded, waiting to join a lane
+    int building = 0;                 // jobs handed to the builders and not yet in `built`
     bool stop = false;
     int device = 0;
     std::chrono::steady_clock::time_point t0;
@@ -117,6 +160,17 @@ static void builder_main(qjx_pool *p)
     }
 }
 
+static void destroy_lanes(qjx_pool *p)
+{
+    for (auto &L : p->lane) {
+        qjx::runner_destroy(L.runner);
+        L.runner = nullptr;
+        if (L.ev0) (void)hipEventDestroy(L.ev0);
+        if (L.ev1) (void)hipEventDestroy(L.ev1);
+        L.ev0 = L.ev1 = nullptr;
+    }
+}
+
 extern "C" int qjx_pool_create(int precision, double hbm_budget_gib, int max_trajectories, qjx_pool **out)
 {
     if (!out) return pool_fail(QJX_ERR_ARG, "null argument");
@@ -128,31 +182,42 @@ extern "C" int qjx_pool_create(int precision, double hbm_budget_gib, int max_tra
     p->prec = precision;
     p->capacity = max_trajectories;
     p->t0 = std::chrono::steady_clock::now();
+    p->short_s = env_double("QJX_POOL_SHORT_S", 2.0);
+    p->reserve = std::min(0.9, std::max(0.0, env_double("QJX_POOL_RESERVE", 0.25)));
     size_t freeb = 0, totalb = 0;
     if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) {
         delete p;
         return pool_fail(QJX_ERR_HIP, "no HIP device available");
     }
     p->budget = hbm_budget_gib > 0 ? (size_t)(hbm_budget_gib * (1ULL << 30)) : (size_t)(0.8 * (double)freeb);
-    // Non-blocking: the builder thread's null-stream calls (hipMemcpy, hipMemset,
+    // Non-blocking: the builder threads' null-stream calls (hipMemcpy, hipMemset,
     // hipMalloc) then neither wait for the running chunk nor invalidate a graph
-    // capture in progress on this stream.  Every hand-over between the two is
-    // ordered by the host instead: a build ends with hipStreamSynchronize on the
-    // job's own stream, and results are read after synchronising this one.
+    // capture in progress on this stream.  Every hand-over between them is ordered
+    // by the host instead: a build ends with hipStreamSynchronize on the job's own
+    // stream, and results are read after synchronising this one.
     if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
         delete p;
         return pool_fail(QJX_ERR_HIP, "hipStreamCreate failed");
     }
-    int rc = qjx::runner_create(precision, p->gemm_kind, p->stream, max_trajectories, &p->runner);
-    if (rc) {
-        (void)hipStreamDestroy(p->stream);
-        delete p;
-        return rc;
-    }
+    bool graphs = true;
     if (const char *env = std::getenv("QJX_GRAPH"))
-        if (std::string(env) == "0") p->runner->graphs = false;
+        if (std::string(env) == "0") graphs = false;
+    for (auto &L : p->lane) {
+        int rc = qjx::runner_create(precision, p->gemm_kind, p->stream, max_trajectories, &L.runner);
+        if (!rc && (hipEventCreate(&L.ev0) != hipSuccess || hipEventCreate(&L.ev1) != hipSuccess))
+            rc = pool_fail(QJX_ERR_HIP, "hipEventCreate failed");
+        if (rc) {
+            destroy_lanes(p);
+            (void)hipStreamDestroy(p->stream);
+            delete p;
+            return rc;
+        }
+        L.runner->graphs = graphs;
+    }
     (void)hipGetDevice(&p->device);
-    p->builder = std::thread(builder_main, p);
+    int nb = (int)env_double("QJX_POOL_BUILDERS", 4.0);
+    nb = std::max(1, std::min(16, nb));
+    for (int i = 0; i < nb; i++) p->builders.emplace_back(builder_main, p);
     *out = p;
     return QJX_OK;
 }
@@ -165,13 +230,14 @@ extern "C" void qjx_pool_destroy(qjx_pool *p)
         p->stop = true;
     }
     p->cv.notify_all();
-    if (p->builder.joinable()) p->builder.join();
+    for (auto &t : p->builders)
+        if (t.joinable()) t.join();
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     for (Job *j : p->jobs) {
         if (j->ctx) qjx_ctx_destroy(j->ctx);
         delete j;
     }
-    qjx::runner_destroy(p->runner);
+    destroy_lanes(p);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
 }
@@ -183,7 +249,6 @@ extern "C" int qjx_pool_submit(qjx_pool *p, const qjx_model_desc *m, const qjx_e
     if (n_periods < 0 || e->num_trajectories < 1) return pool_fail(QJX_ERR_ARG, "bad job size");
     if (e->num_trajectories > p->capacity) return pool_fail(QJX_ERR_ARG, "job larger than the pool capacity");
     Job *j = new Job;
-    j->id = (int)p->jobs.size();
     j->m = *m;
     j->m.precision = p->prec;
     j->e = *e;
@@ -191,7 +256,10 @@ extern "C" int qjx_pool_submit(qjx_pool *p, const qjx_model_desc *m, const qjx_e
     j->n_periods = n_periods;
     j->footprint = qjx::model_footprint(&j->m, p->gemm_kind) +
                    qjx::ensemble_footprint(&j->m, e->num_trajectories, p->gemm_kind);
-    if (j->footprint > p->budget) {
+    j->work = expected_work(j->m, e->num_trajectories, n_periods);
+    j->lane = j->work <= p->short_s ? kShort : kLong;
+    const double room = j->lane == kShort ? 1.0 : 1.0 - p->reserve;
+    if ((double)j->footprint > room * (double)p->budget) {
         delete j;
         return pool_fail(QJX_ERR_NOMEM, "job footprint exceeds the pool's HBM budget");
     }
@@ -208,7 +276,6 @@ extern "C" int qjx_pool_submit(qjx_pool *p, const qjx_model_desc *m, const qjx_e
 // (builder thread).
 static int admit(qjx_pool *p, Job *j)
 {
-    (void)p;
     int rc = qjx::ctx_create_kind(&j->m, p->gemm_kind, &j->ctx);
     if (rc) return rc;
     rc = qjx_ensemble_init(j->ctx, &j->e);
@@ -218,6 +285,14 @@ static int admit(qjx_pool *p, Job *j)
     if (rc) return rc;
     if (hipStreamSynchronize(j->ctx->stream) != hipSuccess) return pool_fail(QJX_ERR_HIP, "job setup failed");
     return QJX_OK;
+}
+
+// Release a job's reservation (budget, lane capacity); the caller holds p->mu.
+static void release(qjx_pool *p, Job *j)
+{
+    p->used -= j->footprint;
+    if (j->lane == kLong) p->used_long -= j->footprint;
+    p->lane[j->lane].in_table -= j->e.num_trajectories;
 }
 
 static int retire(qjx_pool *p, Job *j)
@@ -237,7 +312,10 @@ static int retire(qjx_pool *p, Job *j)
     j->matvecs = st.matvecs;
     j->jumps = st.jumps;
     int h[4] = {0, 0, 0, 0};
-    if (hipMemcpy(h, c->err, sizeof h, hipMemcpyDeviceToHost) == hipSuccess && h[0]) {
+    const bool norm_err = hipMemcpy(h, c->err, sizeof h, hipMemcpyDeviceToHost) == hipSuccess && h[0];
+    qjx_ctx_destroy(c);
+    std::lock_guard<std::mutex> lk(p->mu);
+    if (norm_err) {
         j->state = kFailed;
         j->err = QJX_ERR_NORM;
         j->msg = "norm more than 1";
@@ -245,104 +323,159 @@ static int retire(qjx_pool *p, Job *j)
         j->state = kDone;
     }
     j->t_done = p->now();
-    qjx_ctx_destroy(c);
     j->ctx = nullptr;
-    p->used -= j->footprint;
-    p->in_table -= j->e.num_trajectories;
+    release(p, j);
     return QJX_OK;
 }
 
-static int sync_table(qjx_pool *p)
+static int sync_table(qjx_pool *p, int l)
 {
+    auto &L = p->lane[l];
     std::vector<qjx::Ens> tab;
-    for (int id : p->running) tab.push_back(p->jobs[id]->ctx->desc);
-    p->table_dirty = false;
-    return qjx::runner_set_table(p->runner, tab.data(), (int)tab.size());
+    for (int id : L.running) tab.push_back(p->jobs[id]->ctx->desc);
+    L.dirty = false;
+    return qjx::runner_set_table(L.runner, tab.data(), (int)tab.size());
 }
 
-// One scheduling cycle: launch a chunk of iterations over the running jobs,
-// admit queued jobs while it runs, then retire the jobs that finished.
+// Admission (caller holds p->mu): queued jobs in order of expected work; a job
+// that does not fit is skipped, not waited for.
+static void admit_queued(qjx_pool *p)
+{
+    if (p->queue.empty()) return;
+    std::stable_sort(p->queue.begin(), p->queue.end(),
+                     [&](int a, int b) { return p->jobs[a]->work < p->jobs[b]->work; });
+    int tables[kLanes];
+    for (int l = 0; l < kLanes; l++) tables[l] = (int)p->lane[l].running.size() + p->lane[l].pending;
+    const size_t long_cap = (size_t)((1.0 - p->reserve) * (double)p->budget);
+    std::vector<int> still;
+    for (size_t k = 0; k < p->queue.size(); k++) {
+        Job *j = p->jobs[p->queue[k]];
+        const int l = j->lane;
+        const bool fits = p->used + j->footprint <= p->budget &&
+                          (l == kShort || p->used_long + j->footprint <= long_cap) &&
+                          p->lane[l].in_table + j->e.num_trajectories <= p->capacity &&
+                          tables[l] < qjx::kMaxEns;
+        if (!fits) {
+            still.push_back(j->id);
+            continue;
+        }
+        p->used += j->footprint;
+        if (l == kLong) p->used_long += j->footprint;
+        p->lane[l].in_table += j->e.num_trajectories;
+        p->lane[l].pending++;
+        tables[l]++;
+        j->t_admit = p->now();
+        p->to_build.push_back(j->id);
+        p->building++;
+    }
+    p->queue.swap(still);
+}
+
+// One scheduling cycle: launch a chunk of iterations in each busy lane, admit
+// queued jobs while it runs, then retire the jobs that finished.
 extern "C" int qjx_pool_step(qjx_pool *p, int *running, int *queued)
 {
     if (!p) return pool_fail(QJX_ERR_ARG, "null pool");
     int rc = QJX_OK;
-    if (p->table_dirty && (rc = sync_table(p))) return rc;
-    const bool busy = !p->running.empty();
-    // the launch shape is tuned once, after the first jobs' bisections have spread
-    if (busy && p->cycles >= 4 && (rc = qjx::runner_autotune(p->runner))) return rc;
-    // no graph capture while the builder thread may call the runtime: its calls can
-    // invalidate a capture in progress.  An idle builder only waits on `cv`, and no
-    // build starts before this chunk is launched.
+    bool busy[kLanes];
+    bool deferred;
     {
         std::lock_guard<std::mutex> lk(p->mu);
-        p->runner->defer_capture = p->building > 0;
+        // no graph capture while builder threads may call the runtime: their calls
+        // can invalidate a capture in progress.  Idle builders only wait on `cv`.
+        deferred = p->building > 0;
     }
-    if (busy && (rc = qjx::runner_chunk(p->runner, 16))) return rc;
-    if (busy) p->cycles++;
-    // built jobs join the table at the next cycle; admission reserves budget and
-    // hands the build to the builder thread (it overlaps the chunk just launched)
+    for (int l = 0; l < kLanes; l++) {
+        auto &L = p->lane[l];
+        if (L.dirty && (rc = sync_table(p, l))) return rc;
+        busy[l] = !L.running.empty();
+        // the launch shape is tuned once, after the first jobs' bisections have spread
+        if (busy[l] && L.cycles >= 4 && (rc = qjx::runner_autotune(L.runner))) return rc;
+        L.runner->defer_capture = deferred;
+    }
+    for (int l = 0; l < kLanes; l++) {
+        if (!busy[l]) continue;
+        auto &L = p->lane[l];
+        if (hipEventRecord(L.ev0, p->stream) != hipSuccess) return pool_fail(QJX_ERR_HIP, "event record failed");
+        if ((rc = qjx::runner_chunk(L.runner, L.iters))) return rc;
+        if (hipEventRecord(L.ev1, p->stream) != hipSuccess) return pool_fail(QJX_ERR_HIP, "event record failed");
+        L.cycles++;
+    }
+    // built jobs join their lane at the next cycle; admission reserves budget and
+    // hands the builds to the builder threads (they overlap the chunks just launched)
     {
         std::lock_guard<std::mutex> lk(p->mu);
         while (!p->built.empty()) {
             Job *j = p->jobs[p->built.front()];
             p->built.pop_front();
             p->building--;
+            p->lane[j->lane].pending--;
             if (j->err) {
                 j->state = kFailed;
-                p->used -= j->footprint;
-                p->in_table -= j->e.num_trajectories;
+                release(p, j);
                 continue;
             }
             j->state = kRunning;
-            p->running.push_back(j->id);
-            p->table_dirty = true;
+            p->lane[j->lane].running.push_back(j->id);
+            p->lane[j->lane].dirty = true;
         }
-        while (!p->queue.empty() && (int)p->running.size() + p->building < qjx::kMaxEns) {
-            Job *j = p->jobs[p->queue.front()];
-            if (p->used + j->footprint > p->budget) break;
-            if (p->in_table + j->e.num_trajectories > p->capacity) break;
-            p->queue.pop_front();
-            p->used += j->footprint;
-            p->in_table += j->e.num_trajectories;
-            j->t_admit = p->now();
-            p->to_build.push_back(j->id);
-            p->building++;
-        }
+        admit_queued(p);
     }
-    p->cv.notify_one();
-    if (busy) {
+    p->cv.notify_all();
+    if (busy[kShort] || busy[kLong]) {
         if (hipStreamSynchronize(p->stream) != hipSuccess) return pool_fail(QJX_ERR_HIP, "pool iteration failed");
-        // a job is finished when its lists for the next iteration are empty
-        std::vector<int> still;
-        for (int id : p->running) {
-            Job *j = p->jobs[id];
-            int cnt[qjx::kMaxGroups];
-            if (hipMemcpy(cnt, j->ctx->cnts[0], sizeof cnt, hipMemcpyDeviceToHost) != hipSuccess)
-                return pool_fail(QJX_ERR_HIP, "pool status read failed");
-            long long left = 0;
-            for (int g = 0; g < j->ctx->G; g++) left += cnt[g];
-            // jobs admitted during this cycle were seeded, so their lists are not empty
-            if (left == 0) {
-                rc = retire(p, j);
-                if (rc) return rc;
-                p->table_dirty = true;
-            } else {
-                still.push_back(id);
-            }
+        // Equal device time for the two lanes when both have work: the long lane's
+        // chunk stays short enough to keep cycles near 0.2 s, and the short lane's
+        // chunk is sized to last about as long.
+        float ms[kLanes] = {0.f, 0.f};
+        for (int l = 0; l < kLanes; l++)
+            if (busy[l] && hipEventElapsedTime(&ms[l], p->lane[l].ev0, p->lane[l].ev1) != hipSuccess)
+                return pool_fail(QJX_ERR_HIP, "event timing failed");
+        auto even = [](double x) { return std::max(2, (int)(x / 2.0) * 2); };
+        auto &S = p->lane[kShort], &Lg = p->lane[kLong];
+        if (busy[kLong] && ms[kLong] > 0.f) {
+            const double per = ms[kLong] / Lg.iters;
+            Lg.iters = std::min(16, even(200.0 / std::max(per, 1e-3)));
         }
-        p->running.swap(still);
+        if (busy[kShort] && ms[kShort] > 0.f) {
+            const double per = ms[kShort] / S.iters;
+            const double target = busy[kLong] ? std::max((double)ms[kLong], 2.0) : 16.0 * per;
+            S.iters = std::min(1024, std::max(16, even(target / std::max(per, 1e-3))));
+        }
+        // a job is finished when its lists for the next iteration are empty
+        for (int l = 0; l < kLanes; l++) {
+            if (!busy[l]) continue;
+            auto &L = p->lane[l];
+            std::vector<int> still;
+            for (int id : L.running) {
+                Job *j = p->jobs[id];
+                int cnt[qjx::kMaxGroups];
+                if (hipMemcpy(cnt, j->ctx->cnts[0], sizeof cnt, hipMemcpyDeviceToHost) != hipSuccess)
+                    return pool_fail(QJX_ERR_HIP, "pool status read failed");
+                long long left = 0;
+                for (int g = 0; g < j->ctx->G; g++) left += cnt[g];
+                // jobs that joined at this cycle were seeded, so their lists are not empty
+                if (left == 0) {
+                    if ((rc = retire(p, j))) return rc;
+                    L.dirty = true;
+                } else {
+                    still.push_back(id);
+                }
+            }
+            L.running.swap(still);
+        }
     }
-    if (running) *running = (int)p->running.size();
-    if (queued) {
-        std::lock_guard<std::mutex> lk(p->mu);
-        *queued = (int)p->queue.size() + p->building;
-    }
+    std::lock_guard<std::mutex> lk(p->mu);
+    if (running) *running = (int)(p->lane[kShort].running.size() + p->lane[kLong].running.size());
+    if (queued) *queued = (int)p->queue.size() + p->building;
     return QJX_OK;
 }
 
 extern "C" int qjx_pool_result(qjx_pool *p, int job_id, qjx_job_result *out, double *mean, double *energy)
 {
-    if (!p || !out || job_id < 0 || job_id >= (int)p->jobs.size()) return pool_fail(QJX_ERR_ARG, "bad job id");
+    if (!p || !out) return pool_fail(QJX_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(p->mu);
+    if (job_id < 0 || job_id >= (int)p->jobs.size()) return pool_fail(QJX_ERR_ARG, "bad job id");
     const Job *j = p->jobs[job_id];
     std::memset(out, 0, sizeof *out);
     out->status = j->state == kFailed ? j->err : j->state;
@@ -371,6 +504,25 @@ extern "C" int qjx_pool_result(qjx_pool *p, int job_id, qjx_job_result *out, dou
         out->norm_avg = sn / M;
         if (mean) std::memcpy(mean, j->mean.data(), M * sizeof(double));
         if (energy) std::memcpy(energy, j->energy.data(), M * sizeof(double));
+    }
+    return QJX_OK;
+}
+
+extern "C" int qjx_pool_get_usage(qjx_pool *p, qjx_pool_usage *out)
+{
+    if (!p || !out) return pool_fail(QJX_ERR_ARG, "null argument");
+    std::memset(out, 0, sizeof *out);
+    std::lock_guard<std::mutex> lk(p->mu);
+    out->used_bytes = (int64_t)p->used;
+    out->budget_bytes = (int64_t)p->budget;
+    out->trajectories = p->lane[kShort].in_table + p->lane[kLong].in_table;
+    out->capacity = p->capacity;
+    out->queued = (int)p->queue.size();
+    out->building = p->building;
+    out->running = (int)(p->lane[kShort].running.size() + p->lane[kLong].running.size());
+    for (const Job *j : p->jobs) {
+        if (j->state == kDone) out->done++;
+        if (j->state == kFailed) out->failed++;
     }
     return QJX_OK;
 }

@@ -80,6 +80,13 @@ class JobResult(ctypes.Structure):
                 ("norm_avg", ctypes.c_double)]
 
 
+class PoolUsage(ctypes.Structure):
+    _fields_ = [("used_bytes", ctypes.c_int64), ("budget_bytes", ctypes.c_int64), ("trajectories", ctypes.c_int64),
+                ("capacity", ctypes.c_int64), ("queued", ctypes.c_int32), ("building", ctypes.c_int32),
+                ("running", ctypes.c_int32), ("done", ctypes.c_int32), ("failed", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
+
+
 _lib = None
 
 # (name, restype, argtypes) of every symbol declared in include/qjx.h
@@ -141,6 +148,7 @@ SIGNATURES = [
     ("qjx_pool_step", ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int), _P(ctypes.c_int)]),
     ("qjx_pool_result", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _P(JobResult), _D, _D]),
     ("qjx_pool_clock", ctypes.c_double, [ctypes.c_void_p]),
+    ("qjx_pool_get_usage", ctypes.c_int, [ctypes.c_void_p, _P(PoolUsage)]),
 ]
 
 
@@ -249,15 +257,19 @@ class Engine:
         _check(self.lib, self.lib.qjx_density(self.h, None if r is None else _ptr(r.view(np.float64)), _ptr(p)))
         return r, float(p[0])
 
-    def get_state(self):
+    def get_state(self, psi=True):
         M, d = self.M, self.d
-        psi = np.zeros((M, d), dtype=np.complex128)
+        ps = np.zeros((M, d), dtype=np.complex128) if psi else None
         eta, t = np.zeros(M), np.zeros(M)
         nj = np.zeros(M, dtype=np.int32)
         rng = np.zeros(M, dtype=np.uint64)
-        _check(self.lib, self.lib.qjx_get_state(self.h, _ptr(psi.view(np.float64)), _ptr(eta), _ptr(t),
-                                                _ptr(nj, ctypes.c_int), _ptr(rng, ctypes.c_uint64)))
-        return {"psi": psi, "eta": eta, "t": t, "njumps": nj, "rng": rng}
+        _check(self.lib, self.lib.qjx_get_state(self.h, None if ps is None else _ptr(ps.view(np.float64)),
+                                                _ptr(eta), _ptr(t), _ptr(nj, ctypes.c_int),
+                                                _ptr(rng, ctypes.c_uint64)))
+        out = {"eta": eta, "t": t, "njumps": nj, "rng": rng}
+        if psi:
+            out["psi"] = ps
+        return out
 
     def set_state(self, psi=None, eta=None, t=None, rng=None):
         psi = None if psi is None else np.ascontiguousarray(psi, dtype=np.complex128)
@@ -438,6 +450,11 @@ class Pool:
             out["mean"] = mean
             out["energy"] = energy
         return out
+
+    def usage(self):
+        u = PoolUsage()
+        _check(self.lib, self.lib.qjx_pool_get_usage(self.h, ctypes.byref(u)))
+        return {k: getattr(u, k) for k, _ in PoolUsage._fields_ if k != "pad"}
 
     def drain(self, max_cycles=1000000):
         for _ in range(max_cycles):

@@ -50,6 +50,24 @@ def test_propagators_match_oracle(oracle_mod, N, S):
     assert worst < 1e-12
 
 
+@pytest.mark.parametrize("N,S", [(100, 4), (700, 4), (300, 200)])
+def test_squaring_chain_tree_equals_direct_expm(N, S, monkeypatch):
+    """The tree is built deepest level first, each level whose scaling step needs
+    s >= 1 squarings as the square of the level below it; that is the full scaling and
+    squaring of every level, bit for bit."""
+    from os_d_amd import qjx
+    _say("case: squaring chain vs direct expm N=%d S=%d" % (N, S))
+    D = 16
+    trees = {}
+    for direct in ("0", "1"):
+        monkeypatch.setenv("QJX_EXPM_DIRECT", direct)
+        e = qjx.Engine(N, ex_deep=D, steps_per_period=S, precision=qjx.FP64)
+        trees[direct] = [e.get_propagator(b, lvl) for b in range(2) for lvl in range(D)]
+        e.close()
+    for a, b in zip(trees["0"], trees["1"]):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("N,S,M,P", [(10, 4, 64, 6), (100, 200, 256, 2)])
 def test_fp64_jump_parity_shared_tree(oracle_mod, N, S, M, P):
     _say("case: fp64 parity shared tree N=%d S=%d M=%d P=%d" % (N, S, M, P))
