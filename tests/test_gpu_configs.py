@@ -200,7 +200,8 @@ def test_c5_specified_serving_mix_admission():
     assert len(jobs) > 50
     assert peak <= u["budget_bytes"]
     assert u["failed"] == 0
-    assert u["queued"] + u["building"] + u["running"] + u["done"] == len(jobs) + 1
+    # every submitted job (the probe included) is queued, building, running or done
+    assert u["queued"] + u["building"] + u["running"] + u["done"] == len(jobs)
     jp = [j["id"] for j in jobs if j["probe"]][0]
     for _ in range(200000):
         if pool.result(jp)["status"] == 2:
