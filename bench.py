@@ -1,6 +1,6 @@
 """Benchmark: quantum-jump trajectory-periods/s of the dimer engine on one MI355X.
 
-Workload (the precision-form configuration): Bose-Hubbard dimer N = 1000
+Headline workload (the precision-form configuration): Bose-Hubbard dimer N = 1000
 (Hilbert dimension d = 1001), J = 1, U = 0.5, E0 = 1, A = 1.5, omega = 1,
 gamma = 0.1, 1e5 trajectories, S = 200 top-level steps per period, ex_deep = 16,
 start state e_0, seed 0.  A bench "step" is one top-level step T/S of the whole
@@ -8,29 +8,34 @@ ensemble, including every bisection refinement and jump it triggers; K steps are
 K/S trajectory-periods per trajectory.  States and propagators are resident in
 HBM before the timed region.
 
-value      : trajectory-periods/s of the FP32 form (FP32 GEMM, FP64 norms, eta,
-             bisection, jumps).
-fp64_value : the same for the FP64 form (same workload and steps), timed on the
-             captured iteration graph as the product runs (no per-launch events);
-             its GEMM rate comes from 2 extra profiled steps.
-bf16x3_value: the same for the 3 x BF16 split-product form (FP32 accumulate),
-             timed like the FP64 form.
-roofline   : the step GEMM (dominant kernel), algorithmic flops = the K blocks
+headline   : one transient period (FP32, from e_0) gives the common starting state;
+             each form (FP32 = `value`, 3 x BF16, FP64) then takes 2 warmup steps
+             and one full period (200 steps, both half periods) timed on the
+             captured iteration graph, as the product runs.  Per-kernel figures
+             come from a separate pass with HIP events on every launch (20 FP32
+             steps: the roofline; 4 steps for the other forms).
+roofline   : the FP32 step GEMM (dominant kernel); algorithmic flops = the K blocks
              inside the band range of each 32-row fragment of the propagators
              (2 x 32 x 32 per column, fragment and block; the rest of the dense
-             8 d^2 per application are exact zeros that are skipped), every launch of
-             the FP32 engine (warmup and timed steps) timed with HIP events on the
-             engine stream.
-cpu_baseline: the FP64 CPU oracle (OpenMP over trajectories, one zgemv per
-             step as the reference) on a bounded sample of the same workload
-             with the same propagator tree.
+             8 d^2 per application are exact zeros that are skipped) per launch of
+             the event pass, over its average launch duration.
+cpu_baseline: the FP64 CPU oracle (OpenMP over trajectories, one dense zgemv per
+             step as the reference) on a bounded sample started from the same
+             post-transient states, with the same propagator tree.
+observables_vs_fp64: each reduced form against FP64 on identical streams over 5
+             periods from e_0 (1e4 trajectories).
+c1, c2     : the plain FP64 drop-in at d = 101 (1000 trajectories x 100 periods,
+             S = 200) and d = 501 (1e4 trajectories on the reference's S = 4 tree),
+             each with its CPU oracle sample.
 batched_uneven: 32 dimer instances of different dimension (seeded manifest) in
              one batch: one grouped GEMM per iteration over all of them, HIP-graph
              replay, launch shape from the tuner; also timed instance by instance.
 lindblad   : the GPU density-matrix (Lindblad, rho-space RK4) integrator at N = 100
              with 1e4 RK4 steps per period, against the CPU restatement.
-serving    : a Poisson stream of ensemble jobs through the pool (continuous
-             batching, admission by HBM footprint), p50 / p99 job latency.
+rk, lpn    : the Runge-Kutta trajectory path and the LPN task, each with a CPU sample.
+serving    : the specified Poisson stream of ensemble jobs through the pool
+             (continuous batching, admission by expected work and HBM footprint),
+             p50 / p99 job latency.
 """
 import argparse
 import json
@@ -45,59 +50,162 @@ METRIC = "trajectory-periods/s at dim 1001 (FP32-MFMA vs FP64) + p99 ensemble-jo
 PEAK = {"fp32": 157.3, "fp64": 78.6}   # MI355X dense matrix TFLOP/s
 
 
-def run_form(qjx, prec, N, M, S, D, warmup, steps, profile=True, profiled_steps=2):
-    """Time `steps` top-level steps of the workload in one precision form.
-
-    profile=True: every launch of the engine (warmup included) is bracketed by HIP
-    events, so the per-launch GEMM figures cover the same launches a kernel trace
-    of the run sees (the roofline form).  profile=False: the timed steps replay the
-    captured iteration graph, as the product runs, and the GEMM rates come from
-    `profiled_steps` extra steps with events."""
-    e = qjx.Engine(N, prm_E=1.0, prm_U=0.5, prm_J=1.0, drv_ampl=1.5, drv_freq=1.0,
-                   diss_gamma=0.1, ex_deep=D, steps_per_period=S, precision=prec)
+def make_engine(qjx, prec, N, S, D, M, **model):
+    p = dict(prm_E=1.0, prm_U=0.5, prm_J=1.0, drv_ampl=1.5, drv_freq=1.0, diss_gamma=0.1)
+    p.update(model)
+    e = qjx.Engine(N, ex_deep=D, steps_per_period=S, precision=prec, **p)
     e.ensemble_init(M, seed=0, mns=1000000, start_type=0, start_state=0, jump=0)
-    e.trans(0)
-    e.set_profiling(profile)
-    if warmup:
-        e.run_steps(warmup, 0)
+    return e
+
+
+def _stats_delta(s0, s1):
+    return {k: s1[k] - s0[k] for k in ("matvecs", "jumps", "gemm_ms", "decide_ms", "gemm_launches",
+                                       "gemm_band_flops", "gemm_flops", "iterations")}
+
+
+def headline(qjx, N, M, S, D, trans_periods, warmup, steps, prof_steps, roofline_steps, forms, cpu_budget_s):
+    """The precision-form workload, every form timed the same way.
+
+    The FP32 engine runs `trans_periods` transient periods from e_0 (the reference's
+    trans_process: no logging, then t := 0).  Its states, etas and RNG states are the
+    common starting point of every form, so all of them time the same regime: each
+    form takes `warmup` steps, then `steps` top-level steps are timed on the captured
+    iteration graph (as the product runs), then `prof_steps` more steps run with HIP
+    events on every launch for the per-kernel figures (`roofline_steps` for FP32,
+    whose launches give the roofline).  Observables are taken after the timed steps,
+    on identical streams, so the reduced forms' errors against FP64 are correlated
+    differences, not independent samples."""
+    out = {}
+    e32 = make_engine(qjx, qjx.FP32, N, S, D, M)
+    t0 = time.perf_counter()
+    e32.trans(trans_periods)
+    out["transient_s"] = time.perf_counter() - t0
+    snap = e32.get_state()
+    snap.pop("njumps")
+    names = {qjx.FP32: "fp32", qjx.BF16X3: "bf16x3", qjx.FP64: "fp64"}
+    for prec in (qjx.FP32, qjx.BF16X3, qjx.FP64):
+        if names[prec] not in forms:
+            continue
+        if prec == qjx.FP32:
+            e = e32
+        else:
+            e = make_engine(qjx, prec, N, S, D, M)
+            e.trans(0)
+            e.set_state(**snap)
+        if warmup:
+            e.run_steps(warmup, 0)
+        s0 = e.stats()
+        t0 = time.perf_counter()
+        e.run_steps(steps, 0)          # returns after the engine stream is synchronized
+        dt = time.perf_counter() - t0
+        s1 = e.stats()
+        obs = observables(e, N, M)
+        st = e.get_state(psi=False)
+        e.set_profiling(True)          # resets the kernel timers
+        p0 = e.stats()
+        e.run_steps(roofline_steps if prec == qjx.FP32 else prof_steps, 0)
+        p1 = e.stats()
+        e.set_profiling(False)
+        tim, prof = _stats_delta(s0, s1), _stats_delta(p0, p1)
+        res = {
+            "seconds": dt, "steps": steps, "warmup": warmup,
+            "tp_per_s": M * steps / S / dt,
+            "timing": "graph replay over %d top-level steps (%.2f periods) after %d transient period(s) and %d "
+                      "warmup steps" % (steps, steps / S, trans_periods, warmup),
+            "matvecs_per_traj_step": tim["matvecs"] / (M * steps),
+            "jumps_per_traj_period": tim["jumps"] / (M * steps / S),
+            "iterations": tim["iterations"],
+            "dense_equivalent_tflops": tim["gemm_flops"] / dt / 1e12,
+            "profiled_steps": prof,
+            "gemm_band_tflops": prof["gemm_band_flops"] / (prof["gemm_ms"] * 1e-3) / 1e12 if prof["gemm_ms"] else None,
+            "gemm_share_of_profiled": None,
+            "gemm_bn": p1["gemm_bn"], "gemm_grid": p1["gemm_grid"],
+            "gemm_kernel": gemm_kernel_name(prec, p1["gemm_bn"], p1["gemm_waves"]),
+            "observables": obs,
+            "rng": st["rng"],
+        }
+        if prec == qjx.FP64 and cpu_budget_s > 0:
+            out["cpu_baseline"] = cpu_baseline(e, N, S, D, cpu_budget_s, start=snap)
+        e.close()
+        out[names[prec]] = res
+    del snap
+    for k in ("fp32", "bf16x3"):
+        if k in out and "fp64" in out:
+            out[k]["same_jump_count_as_fp64"] = float((out[k]["rng"] == out["fp64"]["rng"]).mean())
+    for k in ("fp32", "bf16x3", "fp64"):
+        if k in out:
+            out[k].pop("rng")
+    return out
+
+
+def observables_leg(qjx, N, M, S, D, periods, forms):
+    """<n1 - n2>/N and purity of each reduced form against the FP64 run after every
+    period up to `periods`, from e_0 on identical streams (graph replay), and the
+    share of trajectories whose jump count still equals the FP64 one (equal final
+    RNG states).  Once a trajectory's crossing test falls the other way in one form
+    its later jumps differ, and the forms' ensembles become independent samples:
+    then the error is judged against the combined standard error."""
+    names = {qjx.FP32: "fp32", qjx.BF16X3: "bf16x3", qjx.FP64: "fp64"}
+    per, rng, secs = {}, {}, {}
+    for prec in (qjx.FP64, qjx.FP32, qjx.BF16X3):
+        if names[prec] not in forms and prec != qjx.FP64:
+            continue
+        e = make_engine(qjx, prec, N, S, D, M)
+        e.trans(0)
+        t0 = time.perf_counter()
+        rows = []
+        for _ in range(periods):
+            e.run_periods(1, 0)
+            rows.append(observables(e, N, M))
+        secs[names[prec]] = time.perf_counter() - t0
+        rng[names[prec]] = e.get_state(psi=False)["rng"]
+        per[names[prec]] = rows
+        e.close()
+    out = {"trajectories": M, "periods": periods, "seconds": secs, "fp64": per["fp64"]}
+    for k in ("fp32", "bf16x3"):
+        if k not in per:
+            continue
+        errs = []
+        for a, b in zip(per[k], per["fp64"]):
+            d = a["n1_minus_n2_over_N"] - b["n1_minus_n2_over_N"]
+            sg = (a["sigma"] ** 2 + b["sigma"] ** 2) ** 0.5
+            errs.append({"n1_minus_n2_over_N_error": d, "combined_sigma": sg, "error_in_sigma": d / sg if sg else None,
+                         "purity_error": a["purity"] - b["purity"]})
+        out[k] = {"by_period": errs, "same_jump_count_as_fp64": float((rng[k] == rng["fp64"]).mean()),
+                  "max_abs_n1_minus_n2_over_N_error": max(abs(x["n1_minus_n2_over_N_error"]) for x in errs),
+                  "max_abs_purity_error": max(abs(x["purity_error"]) for x in errs)}
+    return out
+
+
+def drop_in_leg(qjx, N, M, S, D, trans_periods, periods, cpu_budget_s, cpu_traj, label, cpu_periods=1):
+    """The plain FP64 drop-in on one configuration: M trajectories of the dimer at N
+    on the tree (S, D), `trans_periods` transient periods, then `periods` timed
+    periods on graph replay; the CPU oracle (OpenMP over trajectories, one zgemv per
+    step as the reference, the same tree) on a bounded sample from the start state."""
+    e = make_engine(qjx, qjx.FP64, N, S, D, M)
+    t0 = time.perf_counter()
+    e.trans(trans_periods)
+    trans_s = time.perf_counter() - t0
+    start = None
+    if cpu_budget_s > 0 and trans_periods > 0:
+        start = e.get_state()
     s0 = e.stats()
     t0 = time.perf_counter()
-    e.run_steps(steps, 0)          # returns after the engine stream is synchronized
+    e.run_periods(periods, 0)
     dt = time.perf_counter() - t0
     s1 = e.stats()
-    obs = observables(e, N, M)
-    mv = s1["matvecs"] - s0["matvecs"]
-    jumps = s1["jumps"] - s0["jumps"]
-    if not profile:
-        e.set_profiling(True)      # resets the timers
-        s0 = e.stats()
-        e.run_steps(profiled_steps, 0)
-        s1 = e.stats()
-    e.set_profiling(False)
-    res = {
-        "seconds": dt,
-        "tp_per_s": M * steps / S / dt,
-        "matvecs": mv,
-        "matvecs_per_traj_step": mv / (M * steps),
-        "jumps_per_traj_period": jumps / (M * steps / S),
-        "timing": ("HIP events on every launch" if profile else
-                   "graph replay; GEMM figures from %d extra profiled steps" % profiled_steps),
-        "gemm_ms": s1["gemm_ms"] - s0["gemm_ms"],
-        "decide_ms": s1["decide_ms"] - s0["decide_ms"],
-        "gemm_launches": s1["gemm_launches"] - s0["gemm_launches"],
-        "gemm_flops": 8.0 * (N + 1) ** 2 * (s1["matvecs"] - s0["matvecs"]),
-        "gemm_band_flops": s1["gemm_band_flops"] - s0["gemm_band_flops"],
-        "gemm_bn": s1["gemm_bn"],
-        "gemm_grid": s1["gemm_grid"],
-        "gemm_kernel": gemm_kernel_name(prec, s1["gemm_bn"], s1["gemm_waves"]),
-        # all launches of the engine (warmup + timed)
-        "all_gemm_ms": s1["gemm_ms"],
-        "all_gemm_launches": s1["gemm_launches"],
-        "all_matvecs": s1["matvecs"],
-        "all_gemm_band_flops": s1["gemm_band_flops"],
-        "observables": obs,
-    }
-    return e, res
+    d = _stats_delta(s0, s1)
+    res = {"config": label, "N": N, "dim": N + 1, "trajectories": M, "steps_per_period": S, "ex_deep": D,
+           "transient_periods": trans_periods, "timed_periods": periods, "seconds": dt, "transient_s": trans_s,
+           "tp_per_s": M * periods / dt, "matvecs_per_traj_period": d["matvecs"] / (M * periods),
+           "jumps_per_traj_period": d["jumps"] / (M * periods), "iterations": d["iterations"],
+           "dense_equivalent_tflops": d["gemm_flops"] / dt / 1e12}
+    if cpu_budget_s > 0:
+        res["cpu_baseline"] = cpu_baseline(e, N, S, D, cpu_budget_s, ntraj=cpu_traj, start=start,
+                                           max_steps=cpu_periods * S)
+        res["gpu_over_cpu"] = res["tp_per_s"] / res["cpu_baseline"]["value"]
+    e.close()
+    return res
 
 
 def gemm_kernel_name(prec, bn, waves):
@@ -146,30 +254,38 @@ def measured_traffic_per_column():
         return None
 
 
-def cpu_baseline(engine, N, S, D, budget_s):
-    """FP64 oracle on the host cores, bounded sample, same propagator tree."""
+def cpu_baseline(engine, N, S, D, budget_s, ntraj=0, start=None, max_steps=0):
+    """FP64 oracle on the host cores, bounded sample, same propagator tree: ntraj
+    trajectories (default 2 per thread) from the start state, top-level steps until
+    the budget is spent or a period is done."""
+    import numpy as np
     from oracle import oracle as orc
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
-    threads = min(threads, os.cpu_count() or threads)
+    threads = orc.default_threads()
     m = orc.model(N, prm_E=1.0, prm_U=0.5, prm_J=1.0, drv_ampl=1.5, drv_freq=1.0, gamma=0.1, S=S, D=D)
-    ntraj = 2 * threads
+    ntraj = ntraj or 2 * threads
     o = orc.Ensemble(m, ntraj, seed=0, jump=0, threads=threads, build_propagators=False)
     for b in range(2):
         for lvl in range(D):
             o.set_propagator(b, lvl, engine.get_propagator(b, lvl))
     o.trans(0)
+    where = "the start state"
+    if start is not None:
+        o.set_state(psi=start["psi"][:ntraj], eta=start["eta"][:ntraj], t=np.zeros(ntraj),
+                    rng=start["rng"][:ntraj])
+        where = "the GPU run's post-transient states"
     t0 = time.perf_counter()
     nsteps = 0
     while True:
         o.steps(1, 0)
         nsteps += 1
-        if time.perf_counter() - t0 >= budget_s or nsteps >= S:
+        if time.perf_counter() - t0 >= budget_s or nsteps >= (max_steps or S):
             break
     dt = time.perf_counter() - t0
     return {"value": ntraj * nsteps / S / dt, "unit": "trajectory-periods/s", "cores": threads,
             "kind": "port",
-            "sample": "%d trajectories x %d top-level steps (of %d/period) from the start state, "
-                      "d=%d, FP64 oracle, %.1f s" % (ntraj, nsteps, S, N + 1, dt),
+            "sample": "%d trajectories x %d top-level steps (%d per period) from %s, d=%d, FP64 oracle "
+                      "(OpenMP over trajectories, one dense zgemv per step), %.1f s"
+                      % (ntraj, nsteps, S, where, N + 1, dt),
             "matvecs_per_traj_step": o.num_matvecs() / (ntraj * nsteps)}
 
 
@@ -469,36 +585,38 @@ def serving(qjx, seconds, rate, seed, budget_gib, capacity, mix="specified"):
             "completed_trajectory_periods_per_s": tp / t_end if t_end > 0 else 0.0}
 
 
+LEGS = ("headline", "observables", "c1", "c2", "batch", "lindblad", "rk", "lpn", "serving")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=200, help="timed top-level steps per form (200 = one period)")
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--trans", type=int, default=1, help="transient periods before the timed steps")
+    ap.add_argument("--roofline-steps", type=int, default=20,
+                    help="FP32 steps with HIP events on every launch (the roofline pass)")
+    ap.add_argument("--prof-steps", type=int, default=4, help="profiled steps of the other forms")
     ap.add_argument("--N", type=int, default=1000)
     ap.add_argument("--traj", type=int, default=100000)
     ap.add_argument("--S", type=int, default=200)
     ap.add_argument("--D", type=int, default=16)
-    ap.add_argument("--fp64-steps", type=int, default=0, help="0 = the same as --steps")
-    ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--no-bf16x3", action="store_true")
+    ap.add_argument("--forms", default="fp32,bf16x3,fp64")
+    ap.add_argument("--legs", default=",".join(LEGS), help="comma list of " + ",".join(LEGS))
+    ap.add_argument("--obs-traj", type=int, default=10000)
+    ap.add_argument("--obs-periods", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--batch-steps", type=int, default=4,
-                    help="top-level steps of the 32-instance batched workload (0 = skip)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=4)
     ap.add_argument("--batch-alone", type=int, default=1,
                     help="also time the batched workload instance by instance")
-    ap.add_argument("--only-batch", action="store_true")
-    ap.add_argument("--no-lindblad", action="store_true")
-    ap.add_argument("--no-rk", action="store_true")
-    ap.add_argument("--only-rk", action="store_true", help="only the Runge-Kutta trajectory legs")
     ap.add_argument("--rk-cpu-seconds", type=float, default=8.0)
-    ap.add_argument("--no-lpn", action="store_true")
-    ap.add_argument("--peak-probe", action="store_true", help="only the measured device ceilings")
-    ap.add_argument("--only-lpn", action="store_true", help="only the LPN task legs")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serve-seconds", type=float, default=30.0, help="serving leg window (0 = skip)")
+    ap.add_argument("--serve-seconds", type=float, default=90.0, help="serving window on the pool clock")
+    ap.add_argument("--serve-mix", default="specified", choices=sorted(SERVE_MIXES))
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-budget-gib", type=float, default=0.0)
     ap.add_argument("--serve-capacity", type=int, default=2000000)
+    ap.add_argument("--peak-probe", action="store_true", help="only the measured device ceilings")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (read + write), the "
                          "calibration kernel for the FETCH_SIZE / WRITE_SIZE counters")
@@ -525,141 +643,137 @@ def main():
     if args.peak_probe:
         print(json.dumps({"measured_peaks": qjx.peak_probe()}), flush=True)
         return
+    legs = set(x for x in args.legs.split(",") if x)
+    forms = set(x for x in args.forms.split(",") if x)
+    cpu_s = 0.0 if args.no_cpu else args.cpu_seconds
+    with_cpu = not args.no_cpu
 
-    def rk_legs(with_cpu):
+    def say(msg):
+        print("bench: " + msg, file=sys.stderr, flush=True)
+
+    head = None
+    if "headline" in legs:
+        say("headline d=%d M=%d S=%d forms %s" % (N + 1, M, S, ",".join(sorted(forms))))
+        head = headline(qjx, N, M, S, D, args.trans, args.warmup, args.steps, args.prof_steps,
+                        args.roofline_steps, forms, cpu_s if "fp64" in forms else 0.0)
+    obs = None
+    if "observables" in legs:
+        say("observables vs FP64: M=%d x %d periods" % (args.obs_traj, args.obs_periods))
+        obs = observables_leg(qjx, N, args.obs_traj, S, D, args.obs_periods, forms)
+    c1 = c2 = None
+    if "c1" in legs:
+        # the plain CPU configuration: N=100, 1000 trajectories x 100 periods, S=200
+        say("c1 d=101")
+        c1 = drop_in_leg(qjx, 100, 1000, 200, 16, 0, 100, min(cpu_s, 10.0), 0,
+                         "N=100 (d=101), 1000 trajectories x 100 periods, S=200, ex_deep 16, FP64", cpu_periods=100)
+    if "c2" in legs:
+        # the FP64 drop-in at N=500 on the reference's tree (4 top-level steps per period)
+        say("c2 d=501")
+        c2 = drop_in_leg(qjx, 500, 10000, 4, 16, 1, 2, min(cpu_s, 15.0), 0,
+                         "N=500 (d=501), 1e4 trajectories, reference QJ tree (S=4, ex_deep 16), FP64: "
+                         "1 transient + 2 timed periods of the specified 500")
+    batch = None
+    if "batch" in legs and args.batch_steps > 0:
+        say("batched uneven ensembles")
+        batch = batched(qjx, S, D, 1, args.batch_steps, qjx.FP32, bool(args.batch_alone))
+    lind = None
+    if "lindblad" in legs:
+        say("lindblad")
+        lind = lindblad(qjx)
+    rk = None
+    if "rk" in legs:
+        say("rk")
         # the reference's CPU configuration (N=100) with its default rk_ns, and the
         # headline dimension with a step that keeps RK4 stable (h |H| ~ 0.5)
-        return {"d101": rk_leg(qjx, 100, 10000, 10000, 20000, args.rk_cpu_seconds, with_cpu=with_cpu),
-                "d1001": rk_leg(qjx, 1000, 40000, 4096, 4000, args.rk_cpu_seconds, with_cpu=with_cpu)}
-
-    def lpn_legs(with_cpu):
-        # the reference's 100 trajectories, and enough pairs to fill the device
-        return {"M100": lpn_leg(qjx, 100, 5, 1, with_cpu), "M4096": lpn_leg(qjx, 4096, 5, 1, with_cpu)}
-
-    if args.only_rk:
-        print(json.dumps({"rk": rk_legs(not args.no_cpu)}), flush=True)
-        return
-    if args.only_lpn:
-        print(json.dumps({"lpn": lpn_legs(not args.no_cpu)}), flush=True)
-        return
-    if args.only_batch:
-        print(json.dumps(batched(qjx, S, D, 1, args.batch_steps, qjx.FP32, bool(args.batch_alone))), flush=True)
-        return
-    e32, r32 = run_form(qjx, qjx.FP32, N, M, S, D, args.warmup, args.steps)
-    e32.close()
-    bf = None
-    if not args.no_bf16x3:
-        ebf, bf = run_form(qjx, qjx.BF16X3, N, M, S, D, args.warmup, args.steps, profile=False)
-        ebf.close()
-    fp64 = None
-    e64 = None
-    if not args.no_fp64:
-        k64 = args.fp64_steps or args.steps
-        e64, r64 = run_form(qjx, qjx.FP64, N, M, S, D, args.warmup, k64, profile=False)
-        fp64 = r64
-    cpu = None
-    if not args.no_cpu:
-        if e64 is None:
-            e64 = qjx.Engine(N, prm_E=1.0, prm_U=0.5, prm_J=1.0, drv_ampl=1.5, drv_freq=1.0,
-                             diss_gamma=0.1, ex_deep=D, steps_per_period=S, precision=qjx.FP64)
-        cpu = cpu_baseline(e64, N, S, D, args.cpu_seconds)
-
-    batch = None
-    if args.batch_steps > 0:
-        if e64 is not None:
-            e64.close()
-            e64 = None
-        batch = batched(qjx, S, D, 1, args.batch_steps, qjx.FP32, bool(args.batch_alone))
-
-    lind = None
-    if not args.no_lindblad:
-        lind = lindblad(qjx)
-
-    rk = None
-    if not args.no_rk:
-        if e64 is not None:
-            e64.close()
-            e64 = None
-        rk = rk_legs(not args.no_cpu)
-
+        rk = {"d101": rk_leg(qjx, 100, 10000, 10000, 20000, args.rk_cpu_seconds, with_cpu=with_cpu),
+              "d1001": rk_leg(qjx, 1000, 40000, 4096, 4000, args.rk_cpu_seconds, with_cpu=with_cpu)}
     lpn = None
-    if not args.no_lpn:
-        lpn = lpn_legs(not args.no_cpu)
-
+    if "lpn" in legs:
+        say("lpn")
+        # the reference's 100 trajectories, and enough pairs to fill the device
+        lpn = {"M100": lpn_leg(qjx, 100, 5, 1, with_cpu), "M4096": lpn_leg(qjx, 4096, 5, 1, with_cpu)}
     serve = None
-    if args.serve_seconds > 0:
-        if e64 is not None:
-            e64.close()
-            e64 = None
-        serve = {"specified": serving(qjx, args.serve_seconds, args.serve_rate, 12345, args.serve_budget_gib,
-                                      args.serve_capacity, "specified"),
-                 "scaled": serving(qjx, args.serve_seconds, args.serve_rate, 12345, args.serve_budget_gib,
-                                   args.serve_capacity, "scaled")}
-
+    if "serving" in legs and args.serve_seconds > 0:
+        say("serving %s mix, %.0f s" % (args.serve_mix, args.serve_seconds))
+        serve = serving(qjx, args.serve_seconds, args.serve_rate, 12345, args.serve_budget_gib,
+                        args.serve_capacity, args.serve_mix)
     peaks = qjx.peak_probe()
-    launches = max(1, r32["all_gemm_launches"])
-    avg_ms = r32["all_gemm_ms"] / launches
-    per_col = measured_traffic_per_column()
-    cols_per_launch = r32["all_matvecs"] / launches
-    # algorithmic flops: the K blocks inside the propagators' band tables (the
-    # dense 8 d^2 per column counts exact zeros of the banded propagators)
-    flops_per_launch = r32["all_gemm_band_flops"] / launches
-    dense_flops_per_launch = 8.0 * (N + 1) ** 2 * cols_per_launch
-    traffic = per_col * cols_per_launch if (per_col and N == 1000) else None
-    achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
+
     line = {
         "metric": METRIC,
-        "value": r32["tp_per_s"],
+        "value": None,
         "unit": "trajectory-periods/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * r32["seconds"] / args.steps,
+        "ms_per_step": None,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic (start state e_0, seeded MCG59 streams; no dataset)",
         "config": {"workload": "BH dimer N=%d (d=%d), %d trajectories, S=%d steps/period, ex_deep=%d, "
-                               "FP32 propagation + FP64 norms/bisection/jumps" % (N, N + 1, M, S, D),
+                               "FP32 propagation + FP64 norms/bisection/jumps; one period timed after %d "
+                               "transient period(s)" % (N, N + 1, M, S, D, args.trans),
                    "trajectories": M, "dim": N + 1, "steps_per_period": S, "ex_deep": D,
                    "step": "one top-level step T/S of the whole ensemble incl. refinement"},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK["fp32"], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK["fp32"], "traffic": traffic,
-                     "peak_measured": peaks["f32_mfma_tflops"],
-                     "frac_of_measured": achieved / peaks["f32_mfma_tflops"],
-                     "traffic_source": "profiles/step_gemm_fp32_traffic_dimer_d1001_M1e5.md (bytes/column x "
-                                       "columns/launch)",
-                     "algorithmic_bytes_per_launch": 2 * 4 * 2 * 1024 * cols_per_launch if N == 1000 else None,
-                     "kernel": "%s (grouped step GEMM, gathered columns)" % r32["gemm_kernel"],
-                     "launches": launches,
-                     "avg_launch_ms": avg_ms,
-                     "flops_per_launch": flops_per_launch,
-                     "flops_note": "2 x 32 x 32 per (column, 32-row fragment, K block) inside the fragments' band ranges",
-                     "dense_equivalent_flops_per_launch": dense_flops_per_launch,
-                     "dense_equivalent_tflops": dense_flops_per_launch / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else None,
-                     "timed_steps_band_tflops": (r32["gemm_band_flops"] / (r32["gemm_ms"] * 1e-3) / 1e12
-                                                 if r32["gemm_ms"] > 0 else None)},
         "measured_peaks": peaks,
-        "fp32": r32,
-        "fp64_value": fp64["tp_per_s"] if fp64 else None,
-        "fp64": fp64,
-        "fp32_over_fp64": (r32["tp_per_s"] / fp64["tp_per_s"]) if fp64 else None,
-        "bf16x3_value": bf["tp_per_s"] if bf else None,
-        "bf16x3": bf,
-        "bf16x3_over_fp64": (bf["tp_per_s"] / fp64["tp_per_s"]) if (bf and fp64) else None,
-        "bf16x3_gemm_tflops_equiv": (bf["gemm_band_flops"] / (bf["gemm_ms"] * 1e-3) / 1e12) if bf else None,
-        "fp64_gemm_tflops": (fp64["gemm_band_flops"] / (fp64["gemm_ms"] * 1e-3) / 1e12) if fp64 else None,
-        "cpu_baseline": cpu,
-        "observables_vs_fp64": {"fp32": observable_errors(r32, fp64), "bf16x3": observable_errors(bf, fp64)},
-        "batched_uneven": batch,
-        "lindblad": lind,
-        "rk_trajectories": rk,
-        "lpn": lpn,
-        "p99_job_latency_s": serve["scaled"]["p99_latency_s"] if serve else None,
-        "p99_job_latency_note": "from the scaled job mix (the specified mix completes no job within the window)",
-        "serving": serve,
     }
+    if head and "fp32" in head:
+        r32 = head["fp32"]
+        prof = r32["profiled_steps"]
+        launches = max(1, prof["gemm_launches"])
+        avg_ms = prof["gemm_ms"] / launches
+        cols_per_launch = prof["matvecs"] / launches
+        flops_per_launch = prof["gemm_band_flops"] / launches
+        per_col = measured_traffic_per_column()
+        achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
+        line["value"] = r32["tp_per_s"]
+        line["ms_per_step"] = 1e3 * r32["seconds"] / args.steps
+        line["roofline"] = {
+            "bound": "mfma", "achieved": achieved, "peak": PEAK["fp32"], "unit": "TFLOP/s",
+            "frac": achieved / PEAK["fp32"],
+            "traffic": per_col * cols_per_launch if (per_col and N == 1000) else None,
+            "peak_measured": peaks["f32_mfma_tflops"], "frac_of_measured": achieved / peaks["f32_mfma_tflops"],
+            "traffic_source": "profiles/step_gemm_fp32_traffic_dimer_d1001_M1e5.md (bytes/column x columns/launch)",
+            "algorithmic_bytes_per_launch": 2 * 4 * 2 * 1024 * cols_per_launch if N == 1000 else None,
+            "kernel": "%s (grouped step GEMM, gathered columns)" % r32["gemm_kernel"],
+            "launches": launches, "avg_launch_ms": avg_ms, "flops_per_launch": flops_per_launch,
+            "columns_per_launch": cols_per_launch,
+            "flops_note": "2 x 32 x 32 per (column, 32-row fragment, K block) inside the fragments' band ranges; "
+                          "launches of the %d-step HIP-event pass after the timed period" % args.roofline_steps,
+            "dense_equivalent_flops_per_launch": 8.0 * (N + 1) ** 2 * cols_per_launch,
+        }
+        line["fp32"] = r32
+    if head:
+        line["transient_s"] = head.get("transient_s")
+        for k in ("bf16x3", "fp64"):
+            if k in head:
+                line[k + "_value"] = head[k]["tp_per_s"]
+                line[k] = head[k]
+        if "fp32" in head and "fp64" in head:
+            line["fp32_over_fp64"] = head["fp32"]["tp_per_s"] / head["fp64"]["tp_per_s"]
+        if "bf16x3" in head and "fp64" in head:
+            line["bf16x3_over_fp64"] = head["bf16x3"]["tp_per_s"] / head["fp64"]["tp_per_s"]
+        line["cpu_baseline"] = head.get("cpu_baseline")
+        if line["cpu_baseline"] and line["value"]:
+            line["fp32_over_cpu"] = line["value"] / line["cpu_baseline"]["value"]
+        line["observables_after_timed_period_vs_fp64"] = {
+            k: observable_errors(head.get(k), head.get("fp64")) for k in ("fp32", "bf16x3")}
+    line["observables_vs_fp64"] = obs
+    line["c1_fp64_d101"] = c1
+    line["c2_fp64_d501"] = c2
+    line["batched_uneven"] = batch
+    line["lindblad"] = lind
+    line["rk_trajectories"] = rk
+    line["lpn"] = lpn
+    line["serving"] = serve
+    if serve:
+        line["p50_job_latency_s"] = serve["p50_latency_s"]
+        line["p99_job_latency_s"] = serve["p99_latency_s"]
+        line["p99_job_latency_note"] = ("%s mix, %.0f s window: over the %d jobs completed in the window; "
+                                        "p99_all_lower_bound_s counts the unfinished jobs with their age"
+                                        % (serve["mix"], serve["window_s"], serve["completed"]))
     print(json.dumps(line), flush=True)
 
 

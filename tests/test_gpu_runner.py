@@ -299,3 +299,55 @@ def test_run_config_lpn_outputs_match_oracle(tmp_path, oracle_mod, lpn_type, lpp
         f = out + "delta_f_by_delta_s_%d" % tr + sfx
         got = np.atleast_1d(np.loadtxt(f)) if os.path.getsize(f) else np.zeros(0)
         assert np.array_equal(got, r)
+
+
+def test_run_config_uniform_start_and_state_dumps_match_oracle(tmp_path, oracle_mod):
+    """start_type 1 (the uniform state, init_start_state, newdel_exp.cpp:656-679) with
+    every state dump switched on: phi_<tr> at the end (dump_phi, core.cpp:12567-12589),
+    phi_evo_<tr> after the transient, at every dump and once more at the end
+    (dump_phi_evo, core.cpp:12591-12613, called at experiment.cpp:466, 515, 531),
+    adr_<tr> after the transient and at every dump (dump_adr_single, dump.cpp:3-25) and
+    adr_avg at the same points (dump_evo_avg 1, dump.cpp:26-61): every file equals the
+    oracle's run through the same experiment."""
+    from os_d_amd import qjx
+    print("case: run_config uniform start + phi/adr dumps", flush=True)
+    out = str(tmp_path) + "/"
+    M, d = 24, 13
+    prm = PARAMS.replace("start_type 0", "start_type 1").replace("dump_phi 0", "dump_phi 1").replace(
+        "dump_phi_evo 0", "dump_phi_evo 1").replace("dump_adr_sep 0", "dump_adr_sep 1").replace(
+        "dump_evo_avg 0", "dump_evo_avg 1")
+    (tmp_path / "config.txt").write_text(CONFIG.format(path=out))
+    (tmp_path / "params.txt").write_text(prm)
+    qjx.run_config(str(tmp_path / "config.txt"), str(tmp_path / "params.txt"), qjx.FP64, 0)
+    sfx = SUFFIX.replace("_start(0_0)", "_start(1_0)")
+    m = oracle_mod.model(12, gamma=0.2, S=4, D=12)
+    o = oracle_mod.Ensemble(m, M, jump=2, start_type=1)
+    psi0 = o.get_state()["psi"]
+    np.testing.assert_array_equal(psi0, np.full((M, d), np.sqrt(1.0 / d)))
+    o.trans(2)
+    evo = [o.chars(is_start=1, adr=True)]
+    phis = [o.get_state()["psi"]]
+    for _ in range(3):
+        o.periods(2, 1)
+        evo.append(o.chars(adr=True))
+        phis.append(o.get_state()["psi"])
+    phis.append(phis[-1])                          # the final dump_phi_evo(ad, true)
+
+    def cplx(fn):
+        a = np.loadtxt(fn).reshape(-1, 2)
+        return a[:, 0] + 1j * a[:, 1]
+    for tr in (0, 9, 23):
+        np.testing.assert_allclose(cplx(out + "phi_%d" % tr + sfx), phis[-1][tr], rtol=0, atol=1e-12)
+        pe = cplx(out + "phi_evo_%d" % tr + sfx).reshape(len(phis), d)
+        for k in range(len(phis)):
+            np.testing.assert_allclose(pe[k], phis[k][tr], rtol=0, atol=1e-12)
+        ad = np.loadtxt(out + "adr_%d" % tr + sfx).reshape(len(evo), d)
+        for k in range(len(evo)):
+            np.testing.assert_allclose(ad[k], evo[k]["adr"][tr], rtol=1e-10, atol=1e-15)
+    aa = np.loadtxt(out + "adr_avg" + sfx).reshape(len(evo), d)
+    for k in range(len(evo)):
+        np.testing.assert_allclose(aa[k], evo[k]["adr"].mean(axis=0), rtol=1e-10, atol=1e-15)
+    np.testing.assert_allclose(np.loadtxt(out + "mean_start" + sfx), evo[0]["mean"], rtol=1e-12)
+    me = np.loadtxt(out + "mean_evo" + sfx)
+    for k in range(len(evo)):
+        np.testing.assert_allclose(me[k], evo[k]["mean"], rtol=1e-9)
