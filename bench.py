@@ -123,6 +123,10 @@ def headline(qjx, N, M, S, D, trans_periods, warmup, steps, prof_steps, roofline
             "gemm_kernel": gemm_kernel_name(prec, p1["gemm_bn"], p1["gemm_waves"]),
             "observables": obs,
             "rng": st["rng"],
+            # the engine's whole life (transient, tuner trials, warmup, timed and
+            # profiled steps): what a kernel trace or counter pass of the run sees
+            "engine_totals": {"matvecs": p1["matvecs"], "gemm_band_flops": p1["gemm_band_flops"],
+                              "iterations": p1["iterations"]},
         }
         if prec == qjx.FP64 and cpu_budget_s > 0:
             out["cpu_baseline"] = cpu_baseline(e, N, S, D, cpu_budget_s, start=snap)
