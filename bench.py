@@ -243,16 +243,18 @@ def observable_errors(form, ref):
             "purity_error": a["purity"] - b["purity"]}
 
 
-def measured_traffic_per_column():
-    """HBM bytes per propagator application of the FP32 step GEMM, from the committed
+TRAFFIC_PROFILE = os.path.join("profiles", "step_gemm_traffic_dimer_d1001_M1e5.md")
+
+
+def measured_traffic_per_column(form="fp32"):
+    """HBM bytes per propagator application of a form's step GEMM, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy kernel)."""
-    path = os.path.join(REPO, "profiles", "step_gemm_fp32_traffic_dimer_d1001_M1e5.md")
     try:
         vals = {}
-        for line in open(path):
-            if line.startswith("traffic_bytes_per_column_"):
+        for line in open(os.path.join(REPO, TRAFFIC_PROFILE)):
+            if line.startswith(form + "_traffic_bytes_per_column_"):
                 k, v = line.split(":")
-                vals[k.strip()] = float(v)
+                vals[k.strip()[len(form) + 1:]] = float(v)
         return vals["traffic_bytes_per_column_fetch"] + vals["traffic_bytes_per_column_write"]
     except (OSError, KeyError, ValueError):
         return None
@@ -739,7 +741,7 @@ def main():
             "frac": achieved / PEAK["fp32"],
             "traffic": per_col * cols_per_launch if (per_col and N == 1000) else None,
             "peak_measured": peaks["f32_mfma_tflops"], "frac_of_measured": achieved / peaks["f32_mfma_tflops"],
-            "traffic_source": "profiles/step_gemm_fp32_traffic_dimer_d1001_M1e5.md (bytes/column x columns/launch)",
+            "traffic_source": TRAFFIC_PROFILE + " (fp32 bytes/column x columns/launch)",
             "algorithmic_bytes_per_launch": 2 * 4 * 2 * 1024 * cols_per_launch if N == 1000 else None,
             "kernel": "%s (grouped step GEMM, gathered columns)" % r32["gemm_kernel"],
             "launches": launches, "avg_launch_ms": avg_ms, "flops_per_launch": flops_per_launch,
@@ -755,6 +757,12 @@ def main():
             if k in head:
                 line[k + "_value"] = head[k]["tp_per_s"]
                 line[k] = head[k]
+        for k in ("fp32", "bf16x3", "fp64"):
+            per_col = measured_traffic_per_column(k)
+            if k in head and per_col and N == 1000:
+                # L2-miss bytes per second of the form's GEMM over the timed period
+                cols_per_s = head[k]["tp_per_s"] * head[k]["matvecs_per_traj_step"] * S
+                head[k]["gemm_traffic_tb_per_s"] = per_col * cols_per_s / 1e12
         if "fp32" in head and "fp64" in head:
             line["fp32_over_fp64"] = head["fp32"]["tp_per_s"] / head["fp64"]["tp_per_s"]
         if "bf16x3" in head and "fp64" in head:
