@@ -149,6 +149,15 @@ int qjx_chars(qjx_ctx *ctx, int is_start, double *norm, double *mean, double *di
  * (CQdiss_os_dimer characteristics.cpp:3-6), which this estimates at any dump. */
 int qjx_density(qjx_ctx *ctx, double *rho, double *purity);
 
+/* qjx_density without device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Its scratch is one fixed 64 MiB
+ * device array per GPU, held until the process ends; the call allocates nothing.
+ * Limits: d <= 1024 and at most 2^21 trajectories (QJX_ERR_ARG beyond them).  Calls
+ * that overlap on one GPU share that array and race: serialise them.  Results equal
+ * qjx_density's (bit for bit when its split count fits the scratch, else to rounding). */
+int qjx_density_noalloc(qjx_ctx *ctx, double *rho, double *purity);
+
 /* Current states (num_trajectories x d interleaved complex), eta, time, logged
  * jump counts and RNG states; any pointer may be NULL. */
 int qjx_get_state(qjx_ctx *ctx, double *psi, double *eta, double *t, int *njumps, uint64_t *rng);

@@ -103,6 +103,7 @@ SIGNATURES = [
     ("qjx_run_steps", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     ("qjx_chars", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, _D, _D, _D, _D]),
     ("qjx_density", ctypes.c_int, [ctypes.c_void_p, _D, _D]),
+    ("qjx_density_noalloc", ctypes.c_int, [ctypes.c_void_p, _D, _D]),
     ("qjx_get_state", ctypes.c_int, [ctypes.c_void_p, _D, _D, _D, _P(ctypes.c_int),
                                      _P(ctypes.c_uint64)]),
     ("qjx_set_state", ctypes.c_int, [ctypes.c_void_p, _D, _D, _D, _P(ctypes.c_uint64)]),
@@ -255,6 +256,15 @@ class Engine:
         r = np.zeros((d, d), dtype=np.complex128) if rho else None
         p = np.zeros(1)
         _check(self.lib, self.lib.qjx_density(self.h, None if r is None else _ptr(r.view(np.float64)), _ptr(p)))
+        return r, float(p[0])
+
+    def density_noalloc(self, rho=True):
+        """density() through the allocation-free entry point (fixed device scratch)."""
+        d = self.d
+        r = np.zeros((d, d), dtype=np.complex128) if rho else None
+        p = np.zeros(1)
+        _check(self.lib, self.lib.qjx_density_noalloc(self.h, None if r is None else _ptr(r.view(np.float64)),
+                                                      _ptr(p)))
         return r, float(p[0])
 
     def get_state(self, psi=True):
