@@ -149,6 +149,14 @@ int qjx_chars(qjx_ctx *ctx, int is_start, double *norm, double *mean, double *di
  * (CQdiss_os_dimer characteristics.cpp:3-6), which this estimates at any dump. */
 int qjx_density(qjx_ctx *ctx, double *rho, double *purity);
 
+/* Random observables of the STD path (calc_random_obs, core.cpp:694, 724,
+ * 12614-12622): <psi|R_o|psi> / ||psi||^2 for o < num of every trajectory's current
+ * state, out = num_trajectories x num complex.  R_o are init_random_obs's matrices
+ * (newdel.cpp:3652-3722; random_obs_seed + o, random_obs_mns, random_obs_type), built
+ * on the device on first use and kept by the context; the products run in
+ * get_random_obs's order (experiment.cpp:2023-2078). */
+int qjx_random_obs(qjx_ctx *ctx, int num, int seed, int mns, int type, double *out);
+
 /* qjx_density without device allocation.  No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc).  Its scratch is one fixed 64 MiB

@@ -104,6 +104,11 @@ def lib():
         L.qjo_lpn_get.argtypes = [ctypes.c_void_p, _D, _D, _P(ctypes.c_int), _D, _D, _D, _D, _D]
         L.qjo_lpn_random_obs.argtypes = [ctypes.c_void_p, _D, _D]
         L.qjo_lpn_lambdas.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, ctypes.c_int]
+        L.qjo_random_obs.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 4 + [_D]
+        L.qjo_lpn_exp10.restype = ctypes.c_double
+        L.qjo_lpn_exp10.argtypes = [ctypes.c_double]
+        L.qjo_lpn_log.restype = ctypes.c_double
+        L.qjo_lpn_log.argtypes = [ctypes.c_double]
         L.qjl_evolve.argtypes = [_P(Model), ctypes.c_int, ctypes.c_int, _D]
         L.qjl_rhs.argtypes = [_P(Model), ctypes.c_double, _D, _D]
         _lib = L
@@ -294,6 +299,12 @@ class Ensemble:
 
     def rk_int(self, tr, step):
         self._ok(self.L.qjo_rk_int(self.h, tr, step))
+
+    def random_obs(self, num, seed=1, mns=1000000, typ=2):
+        """calc_random_obs of the STD path for every trajectory (M x num complex)."""
+        out = np.zeros((self.M, num), dtype=np.complex128)
+        self._ok(self.L.qjo_random_obs(self.h, num, seed, mns, typ, _p(out.view(np.float64))))
+        return out
 
     def lpn_setup(self, **kw):
         """LPN bookkeeping (task_id 7); keywords are the params.txt keys."""

@@ -1051,6 +1051,74 @@ extern "C" int64_t qjo_num_jumps(const qjo_ens *e)
 /* experiment.cpp:2023-2078).                                                  */
 /* ------------------------------------------------------------------------- */
 
+/* 10^y (the LPN perturbation size) and ln x (the LPN growth), evaluated the same
+ * way here and in the library's device code, where libm's pow and log may differ
+ * from the host's in the last place; both stay within 2 ulp of the C library.
+ * 10^y = 2^n e^r: t = y ln 10 as a double-double, n the nearest integer to t / ln 2,
+ * r = t - n ln 2 with ln 2 split into 32 + 53 significant bits (Cody & Waite), e^r
+ * by its Taylor series to degree 14 on |r| <= ln(2)/2 (truncation < 1e-18, Horner).
+ * ln x = e ln 2 + 2 atanh s for x = 2^e m, m in [sqrt(1/2), sqrt(2)),
+ * s = (m - 1)/(m + 1): the atanh series to s^23 (|s| < 0.172, truncation < 1e-18),
+ * arranged as in fdlibm's log so that the leading term m - 1 is exact. */
+static double lpn_exp10(double y)
+{
+    const double th = y * 0x1.26bb1bbb55516p+1;
+    const double tl = std::fma(y, 0x1.26bb1bbb55516p+1, -th) + y * -0x1.f48ad494ea3e9p-53;
+    const double n = std::nearbyint(th * 0x1.71547652b82fep+0);
+    double r = th - n * 0x1.62e42fee00000p-1;
+    r = r - n * 0x1.a39ef35793c76p-33;
+    r = r + tl;
+    double p = 0x1.93974a8c07c9dp-37;
+    p = std::fma(p, r, 0x1.6124613a86d09p-33);
+    p = std::fma(p, r, 0x1.1eed8eff8d898p-29);
+    p = std::fma(p, r, 0x1.ae64567f544e4p-26);
+    p = std::fma(p, r, 0x1.27e4fb7789f5cp-22);
+    p = std::fma(p, r, 0x1.71de3a556c734p-19);
+    p = std::fma(p, r, 0x1.a01a01a01a01ap-16);
+    p = std::fma(p, r, 0x1.a01a01a01a01ap-13);
+    p = std::fma(p, r, 0x1.6c16c16c16c17p-10);
+    p = std::fma(p, r, 0x1.1111111111111p-7);
+    p = std::fma(p, r, 0x1.5555555555555p-5);
+    p = std::fma(p, r, 0x1.5555555555555p-3);
+    p = std::fma(p, r, 0x1.0000000000000p-1);
+    p = std::fma(p, r, 1.0);
+    p = std::fma(p, r, 1.0);
+    return std::ldexp(p, (int)n);
+}
+
+static double lpn_log(double x)
+{
+    int e = 0;
+    double m = std::frexp(x, &e);
+    if (m < 0x1.6a09e667f3bcdp-1) {
+        m = m * 2.0;
+        e -= 1;
+    }
+    const double f = m - 1.0;
+    const double s = f / (2.0 + f);
+    const double z = s * s;
+    /* 2 atanh s = 2 s + s R, R = sum_k 2 z^k / (2k + 1); with f = 2 s + s f this is
+     * f - (f^2/2 - s (f^2/2 + R)), which keeps the leading term f exact */
+    double R = 0x1.642c8590b2164p-4;
+    R = std::fma(R, z, 0x1.8618618618618p-4);
+    R = std::fma(R, z, 0x1.af286bca1af28p-4);
+    R = std::fma(R, z, 0x1.e1e1e1e1e1e1ep-4);
+    R = std::fma(R, z, 0x1.1111111111111p-3);
+    R = std::fma(R, z, 0x1.3b13b13b13b14p-3);
+    R = std::fma(R, z, 0x1.745d1745d1746p-3);
+    R = std::fma(R, z, 0x1.c71c71c71c71cp-3);
+    R = std::fma(R, z, 0x1.2492492492492p-2);
+    R = std::fma(R, z, 0x1.999999999999ap-2);
+    R = std::fma(R, z, 0x1.5555555555555p-1);
+    R = R * z;
+    const double hfsq = 0.5 * f * f;
+    const double de = double(e);
+    return de * 0x1.62e42fee00000p-1 - ((hfsq - (s * (hfsq + R) + de * 0x1.a39ef35793c76p-33)) - f);
+}
+
+extern "C" double qjo_lpn_exp10(double y) { return lpn_exp10(y); }
+extern "C" double qjo_lpn_log(double x) { return lpn_log(x); }
+
 /* MKL VSL MCG31m1: x <- a x mod (2^31 - 1), a = 1132489760; the [lo, hi) uniform
  * is fma(double(x), (hi - lo) (1 / m), lo) (pinned by tests/golden/mcg31_mkl.json). */
 static const uint64_t kMcg31A = 1132489760ULL, kMcg31M = 2147483647ULL;
@@ -1177,6 +1245,26 @@ static cplx random_obs_value(const cplx *phi, const cplx *R, int d, std::vector<
     return cplx(rr, ri);
 }
 
+/* calc_random_obs on the STD path (core.cpp:12614-12622, called from calc_chars_std,
+ * core.cpp:694, 724): the random observables random_obs_seed + o of every
+ * trajectory's current state, M x num complex. */
+extern "C" int qjo_random_obs(qjo_ens *e, int num, int seed, int mns, int type, double *out)
+{
+    const int d = e->m.d, M = e->M;
+    for (int o = 0; o < num; o++) {
+        std::vector<cplx> R;
+        random_obs_matrix(d, seed, mns, type, o, R);
+#pragma omp parallel for schedule(static) num_threads(e->threads)
+        for (int tr = 0; tr < M; tr++) {
+            std::vector<cplx> tmp;
+            const cplx v = random_obs_value(&e->psi[(size_t)tr * d], R.data(), d, tmp);
+            out[2 * ((size_t)tr * num + o)] = v.real();
+            out[2 * ((size_t)tr * num + o) + 1] = v.imag();
+        }
+    }
+    return 0;
+}
+
 /* mean and energy of one state as calc_chars_std computes them (the loops of qjo_chars) */
 static void mean_energy(const qjo_ens *e, const cplx *phi, double *mean, double *energy)
 {
@@ -1247,7 +1335,8 @@ static double lpn_delta(const qjo_ens *e, const Lpn &L, int tr, int base, bool f
     if (L.p.lpn_type == -1) return std::fabs(mn[tr] - mn[base]) / double(d);
     if (L.p.lpn_type == -2) return std::fabs(en[tr] - en[base]) / L.max_energy;
     const cplx b = ob[base][L.p.lpn_type], v = ob[tr][L.p.lpn_type];
-    const double tmp = std::pow(b.real() - v.real(), 2) + std::pow(b.imag() - v.imag(), 2);
+    const double dr = b.real() - v.real(), di = b.imag() - v.imag();
+    const double tmp = dr * dr + di * di;   /* pow(x, 2) is the correctly rounded x * x */
     return std::sqrt(tmp);
 }
 
@@ -1259,7 +1348,7 @@ static void lpn_var(qjo_ens *e, Lpn &L, int tr, int base)
     const int d = e->m.d;
     double lo = L.p.lpn_eps_low, hi = L.p.lpn_eps_high, diff = hi - lo;
     int deep = 0;
-    double eps = std::pow(10.0, (lo + hi) * 0.5);
+    double eps = lpn_exp10((lo + hi) * 0.5);
     cplx *phi = &e->psi[(size_t)tr * d];
     const cplx *orig = &e->psi[(size_t)base * d];
     std::vector<double> v(2 * (size_t)d);
@@ -1282,13 +1371,13 @@ static void lpn_var(qjo_ens *e, Lpn &L, int tr, int base)
             hi -= 0.5 * diff;
             diff = hi - lo;
             deep++;
-            eps = std::pow(10.0, (lo + hi) * 0.5);
+            eps = lpn_exp10((lo + hi) * 0.5);
         }
         if (ds < L.p.lpn_delta_s) {
             lo += 0.5 * diff;
             diff = hi - lo;
             deep++;
-            eps = std::pow(10.0, (lo + hi) * 0.5);
+            eps = lpn_exp10((lo + hi) * 0.5);
         }
         if (!(ds > L.p.lpn_delta_s) && !(ds < L.p.lpn_delta_s)) break;   /* equal or NaN */
     }
@@ -1419,7 +1508,7 @@ extern "C" int qjo_lpn_period(qjo_ens *e)
         const bool renorm = L->p.lambda_per_periods != 0 ||
                             df > std::max(L->p.lpn_delta_f_high, L->delta_s[tr] * 10.0) || df < L->p.lpn_delta_f_low;
         if (renorm) {
-            L->lambda[tr] += std::log(df / L->delta_s[tr] + 1.0e-16);
+            L->lambda[tr] += lpn_log(df / L->delta_s[tr] + 1.0e-16);
             L->num_renorms[tr] += 1;
             if (L->p.save_lambdas > 0) {
                 L->lambdas[tr].push_back(df / L->delta_s[tr]);
@@ -1433,7 +1522,7 @@ extern "C" int qjo_lpn_period(qjo_ens *e)
             L->lambda_now[tr] = L->lambda[tr] / curr_time;
         } else {
             lpn_chars_lpn(e, *L, tr);
-            L->lambda_now[tr] = (L->lambda[tr] + std::log(df / L->delta_s[tr] + 1.0e-16)) / curr_time;
+            L->lambda_now[tr] = (L->lambda[tr] + lpn_log(df / L->delta_s[tr] + 1.0e-16)) / curr_time;
         }
     }
     L->period++;

@@ -118,6 +118,9 @@ int qjo_rk_int(qjo_ens *e, int tr, double step);
  * count Box-Muller Gaussians (two uniforms each). */
 void qjo_mcg31_uniform(unsigned seed, int k, int n, int count, double lo, double hi, double *out);
 void qjo_mcg31_gaussian(unsigned seed, int k, int n, int count, double *out);
+/* Random observables seed + o (o < num) of every trajectory's current state
+ * (calc_random_obs of the STD path), num_traj x num complex. */
+int qjo_random_obs(qjo_ens *e, int num, int seed, int mns, int type, double *out);
 /* Random observable obs (d x d complex) of init_random_obs. */
 int qjo_random_obs_matrix(int d, int seed, int mns, int type, int obs, double *out);
 
@@ -137,6 +140,10 @@ int qjo_lpn_get(qjo_ens *e, double *lambda_now, double *lambda, int *num_renorms
                 double *energy_lpn, double *delta_s, double *mean, double *energy);
 int qjo_lpn_random_obs(qjo_ens *e, double *obs, double *obs_lpn);
 int qjo_lpn_lambdas(qjo_ens *e, int tr, double *ratio, double *ds, double *df, int cap);
+/* 10^y and ln x as the LPN bookkeeping evaluates them (Cody-Waite reduction and
+ * Taylor / atanh series, the same code as the library's device side). */
+double qjo_lpn_exp10(double y);
+double qjo_lpn_log(double x);
 
 #ifdef __cplusplus
 }

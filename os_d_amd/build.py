@@ -20,7 +20,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 LIB_SOURCES = ["qjx_kernels.hip", "qjx_step.hip", "qjx_step_f64.hip", "qjx_lindblad.hip", "qjx_rk.hip", "qjx_engine.cpp",
-               "qjx_config.cpp", "qjx_pool.cpp", "qjx_lpn.cpp", "qjx_noalloc.hip"]
+               "qjx_config.cpp", "qjx_pool.cpp", "qjx_lpn.cpp", "qjx_noalloc.hip", "qjx_obs.hip"]
 # The throughput probe keeps MFMA accumulators in ordinary VGPRs: with the default
 # AGPR form the compiler copies each FP64 accumulator chain through v_accvgpr moves
 # every iteration and the probe would measure the moves, not the matrix core.

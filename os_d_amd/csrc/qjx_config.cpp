@@ -414,6 +414,9 @@ extern "C" int qjx_run_config(const char *config_path, const char *params_path, 
         dump_adr_avg = 0, dump_phi = 0, dump_phi_evo = 0, jumps_lim = 0, dnt = 0;
     bool deep = false;
     int deep_steps = 0, sub_steps = 0;   // StdDeep: sub-periods per half period, steps per sub-period
+    int ro_num = 0, ro_seed = 0, ro_mns = 0, ro_type = 0;   // random observables of the STD path
+    std::vector<std::vector<std::vector<double>>> ro_evo;   // [obs][trajectory][(re, im) per dump]
+    std::vector<double> ro;
     std::vector<int> dps;
     std::vector<double> norm_evo, mean_evo, disp_evo, m2_evo, energy_evo, adr, adr_avg, psi;
     std::vector<double> norm, mean_start, mean, disp, m2, energy;
@@ -465,8 +468,16 @@ extern "C" int qjx_run_config(const char *config_path, const char *params_path, 
         const double drv_phase = p.get("dimer_drv_phase");
         if (deep) deep_steps = int(p.get("deep_num_steps"));
         if (!p.missing.empty()) CFG_FAIL(QJX_ERR_ARG, "params.txt lacks key " + p.missing);
-        if (num_random_obs != 0 && c.task_id != 7)
-            CFG_FAIL(QJX_ERR_ARG, "random observables (num_random_obs > 0) are implemented for task_id 7 only");
+        // random observables (init_random_obs, newdel.cpp:3652-3722) are read only
+        // when there are any, as the reference reads their keys
+        if (num_random_obs < 0) CFG_FAIL(QJX_ERR_ARG, "num_random_obs must be >= 0");
+        if (num_random_obs > 0 && c.task_id != 7) {
+            ro_seed = int(p.get("random_obs_seed"));
+            ro_mns = int(p.get("random_obs_mns"));
+            ro_type = int(p.get("random_obs_type"));
+            if (!p.missing.empty()) CFG_FAIL(QJX_ERR_ARG, "params.txt lacks key " + p.missing);
+            ro_num = num_random_obs;
+        }
         if (deep) {
             // StdDeep (task 4): a dump after each of the 2 deep_num_steps sub-periods of
             // every period; dump_type / dump_num do not apply (init_dump_periods_deep,
@@ -537,10 +548,25 @@ extern "C" int qjx_run_config(const char *config_path, const char *params_path, 
         adr_avg.assign(d, 0.0);
         nj.assign(M, 0);
 
+        if (ro_num > 0) {
+            ro_evo.assign(ro_num, std::vector<std::vector<double>>(M));
+            ro.assign((size_t)2 * M * ro_num, 0.0);
+        }
         auto chars = [&](int is_start, int dump_id) -> int {
             int r = qjx_chars(ctx, is_start, norm.data(), mean.data(), disp.data(), m2.data(), energy.data(),
                               adr_avg.data(), need_adr_single ? adr.data() : nullptr);
             if (r < 0) return r;
+            // calc_random_obs (core.cpp:694, 724) and its evo_chars_std record (core.cpp:811-815)
+            if (ro_num > 0) {
+                r = qjx_random_obs(ctx, ro_num, ro_seed, ro_mns, ro_type, ro.data());
+                if (r < 0) return r;
+                for (int tr = 0; tr < M; tr++)
+                    for (int o = 0; o < ro_num; o++) {
+                        const size_t k = 2 * ((size_t)tr * ro_num + o);
+                        ro_evo[o][tr].push_back(ro[k]);
+                        ro_evo[o][tr].push_back(ro[k + 1]);
+                    }
+            }
             if (is_start) mean_start = mean;
             for (int tr = 0; tr < M; tr++) {
                 const size_t i = (size_t)tr * dnt + dump_id;   // evo_chars_std (core.cpp:795-816)
@@ -661,6 +687,8 @@ extern "C" int qjx_run_config(const char *config_path, const char *params_path, 
             IO_CHECK(save_2d_inv(path + "dispersion_evo" + sfx, disp_evo, dnt, M));
             IO_CHECK(save_2d_inv(path + "m2_evo" + sfx, m2_evo, dnt, M));
             IO_CHECK(save_2d_inv(path + "energy_evo" + sfx, energy_evo, dnt, M));
+            for (int o = 0; o < ro_num; o++)
+                IO_CHECK(save_2d_complex(path + "random_obs_evo_" + std::to_string(ro_seed + o) + sfx, ro_evo[o]));
             if (jump > 0) {
                 for (int tr = 0; tr < M; tr++) {
                     const int n = qjx_get_jumps(ctx, tr, nullptr, nullptr, nullptr, nullptr, 0);

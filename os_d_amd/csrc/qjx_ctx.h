@@ -58,6 +58,13 @@ struct qjx_ctx {
 
     bool profiling = false;        // per-kernel timing of the iterations
 
+    // random observables of the STD path (qjx_random_obs): the matrices of the last
+    // (num, seed, mns, type) and scratch for their evaluation
+    double2 *robs_R = nullptr;
+    int robs_num = 0, robs_seed = 0, robs_mns = 0, robs_type = 0;
+    double2 *robs_scratch = nullptr;
+    size_t robs_scratch_n = 0;
+
     // Runge-Kutta propagation (prop_id 1, qjx_ctx_create_rk): no propagator tree;
     // every trajectory takes rk_ns RK4 steps per period in the RK kernel.
     int prop = 0;                  // 0 quantum-jump tree, 1 Runge-Kutta

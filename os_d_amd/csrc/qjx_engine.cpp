@@ -583,6 +583,8 @@ extern "C" void qjx_ctx_destroy(qjx_ctx *c)
     dev_free(c->d_Hoff);
     dev_free(c->d_rkbands);
     dev_free(c->d_rktab);
+    if (c->robs_R) (void)hipFree(c->robs_R);
+    if (c->robs_scratch) (void)hipFree(c->robs_scratch);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

@@ -104,6 +104,7 @@ SIGNATURES = [
     ("qjx_chars", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, _D, _D, _D, _D]),
     ("qjx_density", ctypes.c_int, [ctypes.c_void_p, _D, _D]),
     ("qjx_density_noalloc", ctypes.c_int, [ctypes.c_void_p, _D, _D]),
+    ("qjx_random_obs", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 4 + [_D]),
     ("qjx_get_state", ctypes.c_int, [ctypes.c_void_p, _D, _D, _D, _P(ctypes.c_int),
                                      _P(ctypes.c_uint64)]),
     ("qjx_set_state", ctypes.c_int, [ctypes.c_void_p, _D, _D, _D, _P(ctypes.c_uint64)]),
@@ -257,6 +258,12 @@ class Engine:
         p = np.zeros(1)
         _check(self.lib, self.lib.qjx_density(self.h, None if r is None else _ptr(r.view(np.float64)), _ptr(p)))
         return r, float(p[0])
+
+    def random_obs(self, num, seed=1, mns=1000000, typ=2):
+        """Random observables of every trajectory's current state (M x num complex)."""
+        out = np.zeros((self.M, num), dtype=np.complex128)
+        _check(self.lib, self.lib.qjx_random_obs(self.h, num, seed, mns, typ, _ptr(out.view(np.float64))))
+        return out
 
     def density_noalloc(self, rho=True):
         """density() through the allocation-free entry point (fixed device scratch)."""

@@ -1,7 +1,7 @@
 """The LPN task (task_id 7) on the GPU against the CPU restatement.
 
-Propagation runs on the device, the per-pair bookkeeping on the host in the
-restatement's arithmetic (reference: LpnMultExperimentBehaviour, experiment.cpp:
+Propagation and the per-pair bookkeeping both run on the device, the bookkeeping in
+the restatement's arithmetic (reference: LpnMultExperimentBehaviour, experiment.cpp:
 137-286; var_trajectory_lpn, lambda_lpn(_now), experiment.cpp:2278-2393, 3065-3134).
 With the RK path the propagation is bit-identical, so every LPN quantity must be
 too; with the FP64 quantum-jump path the GEMM's summation order differs from the
@@ -92,4 +92,31 @@ def test_lpn_qj_fp64_to_rounding(oracle_mod):
     np.testing.assert_allclose(gg["lambda"], go["lambda"], rtol=1e-7, atol=1e-9)
     np.testing.assert_allclose(gg["mean_lpn"], go["mean_lpn"], rtol=1e-9)
     L.close()
+    e.close()
+
+
+@pytest.mark.parametrize("N,M,num,typ", [(10, 40, 2, 2), (63, 130, 1, 1), (200, 70, 1, 0)])
+def test_random_obs_equal_oracle_bitwise(oracle_mod, N, M, num, typ):
+    """qjx_random_obs (the STD path's calc_random_obs) on the device equals the
+    oracle's get_random_obs of the same states bit for bit: the matrices R = X X^dag
+    and the products <phi|R|phi> / ||phi||^2 run in the reference's order."""
+    from os_d_amd import qjx
+    print("case: random obs N=%d M=%d num=%d type=%d" % (N, M, num, typ), flush=True)
+    m = oracle_mod.model(N, gamma=0.2, D=1)
+    o = oracle_mod.Ensemble(m, M, jump=0, build_propagators=False)
+    rng = np.random.default_rng(N)
+    psi = rng.normal(size=(M, N + 1)) + 1j * rng.normal(size=(M, N + 1))
+    psi *= rng.uniform(0.3, 1.0, size=(M, 1)) / np.linalg.norm(psi, axis=1, keepdims=True)
+    o.set_state(psi=psi)
+    e = qjx.Engine(N, ex_deep=2, steps_per_period=4, precision=qjx.FP64, diss_gamma=0.2)
+    e.ensemble_init(M, jump=0)
+    e.set_state(psi=psi)
+    want = o.random_obs(num, seed=3, typ=typ)
+    got = e.random_obs(num, seed=3, typ=typ)
+    assert np.array_equal(got, want)
+    # the matrices agree with the LPN object's (the same init_random_obs)
+    for k in range(num):
+        R = oracle_mod.random_obs_matrix(N + 1, 3, 1000000, typ, k)
+        v = psi[5] / np.linalg.norm(psi[5])
+        np.testing.assert_allclose(got[5, k], np.vdot(v, R @ v), rtol=1e-12)
     e.close()

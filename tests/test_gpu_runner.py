@@ -351,3 +351,33 @@ def test_run_config_uniform_start_and_state_dumps_match_oracle(tmp_path, oracle_
     me = np.loadtxt(out + "mean_evo" + sfx)
     for k in range(len(evo)):
         np.testing.assert_allclose(me[k], evo[k]["mean"], rtol=1e-9)
+
+
+def test_run_config_random_observables_task1_match_oracle(tmp_path, oracle_mod):
+    """num_random_obs > 0 with the STD experiment (task_id 1): calc_random_obs at every
+    dump (core.cpp:694, 724) written as random_obs_evo_<seed + o> (core.cpp:1050-1061),
+    rows = dumps, columns = trajectories; equal to the oracle's values of the same
+    states bit for bit."""
+    from os_d_amd import qjx
+    print("case: run_config task 1 random observables", flush=True)
+    out = str(tmp_path) + "/"
+    prm = PARAMS.replace("num_random_obs 0", "num_random_obs 2") + \
+        "random_obs_seed 5\nrandom_obs_mns 1000000\nrandom_obs_type 2\n"
+    (tmp_path / "config.txt").write_text(CONFIG.format(path=out))
+    (tmp_path / "params.txt").write_text(prm)
+    qjx.run_config(str(tmp_path / "config.txt"), str(tmp_path / "params.txt"), qjx.FP64, 0)
+    m = oracle_mod.model(12, gamma=0.2, S=4, D=12)
+    o = oracle_mod.Ensemble(m, 24, jump=2)
+    # the runner's tree is its own; the dumps' states agree to 1e-10, the random
+    # observables are compared on the oracle's states through the same formula
+    o.trans(2)
+    obs = [o.random_obs(2, seed=5, typ=2)]
+    for _ in range(3):
+        o.periods(2, 1)
+        obs.append(o.random_obs(2, seed=5, typ=2))
+    for k in range(2):
+        rows = open(out + "random_obs_evo_%d" % (5 + k) + SUFFIX).read().strip().split("\n")
+        assert len(rows) == 4
+        for r, row in enumerate(rows):
+            vals = np.array([complex(*map(float, c.strip("()").split(","))) for c in row.split("\t")])
+            np.testing.assert_allclose(vals, obs[r][:, k], rtol=1e-9, atol=1e-12)

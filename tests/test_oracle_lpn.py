@@ -140,3 +140,33 @@ def test_lpn_threshold_renormalisation():
         r, ds, df = e.lpn_lambdas(tr)
         assert len(r) == g["num_renorms"][tr] <= P
         assert ((df > np.maximum(5e-2, 10 * ds)) | (df < 2e-3)).all()
+
+
+def test_lpn_exp10_and_log_within_two_ulp_of_the_c_library(oracle_mod):
+    """10^y (the perturbation size, y = log10 eps in [lpn_eps_low, lpn_eps_high]) and
+    ln x (the growth log(df / ds + 1e-16)) are evaluated by the oracle's own
+    Cody-Waite + series code, shared with the device side; against the C library's
+    pow and log they stay within 2 units in the last place."""
+    import math
+    L = oracle_mod.lib()
+    rng = np.random.default_rng(7)
+    ys = np.concatenate([rng.uniform(-100.0, 100.0, 40000), np.arange(-100.0, 100.5, 0.5),
+                         rng.uniform(-1e-3, 1e-3, 2000)])
+    worst = 0.0
+    for y in ys:
+        got, want = L.qjo_lpn_exp10(float(y)), math.pow(10.0, float(y))
+        worst = max(worst, abs(got - want) / math.ulp(want))
+    assert worst <= 2.0, worst
+    xs = np.concatenate([10.0 ** rng.uniform(-300, 300, 40000), rng.uniform(0.5, 2.0, 20000),
+                         1.0 + rng.uniform(-1e-6, 1e-6, 2000), [1e-16, 1.0, 2.0, 0.5, 1e-16 + 1.0]])
+    worst = 0.0
+    for x in xs:
+        got, want = L.qjo_lpn_log(float(x)), math.log(float(x))
+        if want == 0.0:
+            assert got == 0.0
+            continue
+        worst = max(worst, abs(got - want) / math.ulp(want))
+    assert worst <= 2.0, worst
+    # exact points: integer powers of ten that are doubles, and log 1
+    for k in range(0, 23):
+        assert L.qjo_lpn_exp10(float(k)) == 10.0 ** k, k
