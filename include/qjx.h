@@ -245,6 +245,21 @@ int qjx_batch_set_profiling(qjx_batch *batch, int on);
  * rho starts as |e_0><e_0|; rho is d x d complex, row-major interleaved. */
 typedef struct qjx_lindblad qjx_lindblad;
 int qjx_lindblad_create(const qjx_model_desc *model, int steps_per_period, qjx_lindblad **out);
+/* The drive of CQdiss_os_dimer's right-hand side (calcVectValue_real,
+ * CQdiss/calcODE.cpp:100-116), evaluated at every RK4 stage time t = p T + s h
+ * (+ h/2, + h): E + A sin(omega t + phase) for drv_type 1, E + A sign(sin(omega t +
+ * phase)) otherwise (a zero sine counts as positive).  per_stage = 0 instead holds the
+ * square drive constant over each step, steps aligned with the half periods (what
+ * qjx_lindblad_create does; drv_type 0 and phase 0 only).  Time starts at 0 at
+ * creation and at every qjx_lindblad_set_rho; each qjx_lindblad_run_periods call
+ * continues it. */
+typedef struct qjx_lindblad_drive {
+    int drv_type;           /* dimer_drv_type                                       */
+    double drv_phase;       /* dimer_drv_phase                                      */
+    int per_stage;          /* 1: the drive at every stage time (CQdiss); 0: per step */
+} qjx_lindblad_drive;
+int qjx_lindblad_create_drive(const qjx_model_desc *model, int steps_per_period, const qjx_lindblad_drive *drive,
+                              qjx_lindblad **out);
 void qjx_lindblad_destroy(qjx_lindblad *lb);
 int qjx_lindblad_set_rho(qjx_lindblad *lb, const double *rho);
 int qjx_lindblad_get_rho(qjx_lindblad *lb, double *rho);
@@ -254,6 +269,9 @@ int qjx_lindblad_run_periods(qjx_lindblad *lb, int n_periods);
 int qjx_lindblad_observables(qjx_lindblad *lb, double *mean, double *m2, double *energy,
                              double *purity, double *adr);
 int qjx_lindblad_dim(const qjx_lindblad *lb);
+/* rho written as CQdiss_os_dimer writes its final rho (data.cpp:40-46,
+ * save_sparse_complex_mtx utils.cpp:266-310): "row col re im" per entry, 1-based. */
+int qjx_lindblad_save_rho(qjx_lindblad *lb, const char *path);
 
 /* ------------------------------------------------------------------------ */
 /* LPN task (task_id 7): growth rates of observable distances between paired  */

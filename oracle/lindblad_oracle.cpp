@@ -10,10 +10,16 @@
  * 1454-1498).  The reference's Lindblad program (CQdiss_os_dimer/CQdiss/calcODE.cpp:
  * 75-118, 249-389) integrates the same equation in the SU(N) f-basis with RK4; a fixed
  * linear change of variables commutes with RK4, so the rho-space integration is the same
- * scheme.  One deliberate difference: CQdiss evaluates sign(sin(omega t)) at every RK4
- * stage time, so at the two switch times per period one stage takes the value of the
- * neighbouring half period (an O(h) error); here the drive is held constant over each
- * step, and steps are aligned with the half periods, so the scheme is RK4-accurate.
+ * scheme.  Two drive rules:
+ *   - CQdiss's (calcVectValue_real, calcODE.cpp:100-116): at every RK4 stage time
+ *     t = p T + s h (+ h/2, + h), E(t) = E + A sin(omega t + phase) (dimer_drv_type 1)
+ *     or E + A sign(sin(omega t + phase)) (type 0, a zero sine counting as positive);
+ *     at each switch of the square drive one stage then takes the neighbouring half
+ *     period's value (an O(h) error, as in the reference).  The sine is the Cody-Waite
+ *     + Taylor code of the trajectory oracle (qjo_rk_sin), within 2 ulp of libm.  CQdiss
+ *     fixes T = 2 pi whatever the frequency; here T = 2 pi / omega (equal at omega = 1).
+ *   - qjl_evolve's: the square drive held constant over each step, steps aligned with
+ *     the half periods, so the scheme is RK4-accurate.
  *
  * H and L are tridiagonal and L^dag L is pentadiagonal, so each right-hand side
  * costs O(d^2).
@@ -134,8 +140,11 @@ void rhs(const Band &B, double E, const cplx *rho, cplx *out, std::vector<cplx> 
 extern "C" {
 
 /* Evolve rho (d x d interleaved complex, row-major) over n_periods periods with
- * steps_per_period RK4 steps (even; each half period gets half of them). */
-int qjl_evolve(const qjo_model *m, int steps_per_period, int n_periods, double *rho_io)
+ * steps_per_period RK4 steps, from period period0 on (its time is period0 T).
+ * per_stage = 1: CQdiss's drive at every stage time (drv_type, drv_phase); 0: the
+ * square drive constant over each step (each half period gets half of the steps). */
+int qjl_evolve_drive(const qjo_model *m, int steps_per_period, int period0, int n_periods, int drv_type,
+                     double drv_phase, int per_stage, double *rho_io)
 {
     if (!m || steps_per_period < 2 || (steps_per_period % 2) || n_periods < 0) return -1;
     Band B;
@@ -146,19 +155,30 @@ int qjl_evolve(const qjo_model *m, int steps_per_period, int n_periods, double *
     std::vector<cplx> k1(n), k2(n), k3(n), k4(n), v(n), tmp;
     const double T = 2.0 * 3.1415926535897932384626433832795 / m->drv_freq;
     const double h = T / double(steps_per_period);
-    for (int p = 0; p < n_periods; p++)
+    auto drive = [&](int s, double t) {
+        if (!per_stage) return s < steps_per_period / 2 ? m->prm_E + m->drv_ampl : m->prm_E - m->drv_ampl;
+        const double sn = qjo_rk_sin(m->drv_freq * t + drv_phase);
+        const double f = drv_type == 1 ? sn : (sn < 0.0 ? -1.0 : 1.0);
+        return m->prm_E + m->drv_ampl * f;
+    };
+    for (int p = period0; p < period0 + n_periods; p++)
         for (int s = 0; s < steps_per_period; s++) {
-            const double E = s < steps_per_period / 2 ? m->prm_E + m->drv_ampl : m->prm_E - m->drv_ampl;
-            rhs(B, E, rho, k1.data(), tmp);
+            const double time = double(p) * T + double(s) * h;   /* calcODE.cpp:346 */
+            rhs(B, drive(s, time), rho, k1.data(), tmp);
             for (size_t i = 0; i < n; i++) v[i] = rho[i] + 0.5 * h * k1[i];
-            rhs(B, E, v.data(), k2.data(), tmp);
+            rhs(B, drive(s, time + h / 2.0), v.data(), k2.data(), tmp);
             for (size_t i = 0; i < n; i++) v[i] = rho[i] + 0.5 * h * k2[i];
-            rhs(B, E, v.data(), k3.data(), tmp);
+            rhs(B, drive(s, time + h / 2.0), v.data(), k3.data(), tmp);
             for (size_t i = 0; i < n; i++) v[i] = rho[i] + h * k3[i];
-            rhs(B, E, v.data(), k4.data(), tmp);
+            rhs(B, drive(s, time + h), v.data(), k4.data(), tmp);
             for (size_t i = 0; i < n; i++) rho[i] += (h / 6.0) * (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]);
         }
     return 0;
+}
+
+int qjl_evolve(const qjo_model *m, int steps_per_period, int n_periods, double *rho_io)
+{
+    return qjl_evolve_drive(m, steps_per_period, 0, n_periods, 0, 0.0, 0, rho_io);
 }
 
 /* The Lindblad right-hand side itself (for pinning against a dense Liouvillian). */

@@ -110,6 +110,8 @@ def lib():
         L.qjo_lpn_log.restype = ctypes.c_double
         L.qjo_lpn_log.argtypes = [ctypes.c_double]
         L.qjl_evolve.argtypes = [_P(Model), ctypes.c_int, ctypes.c_int, _D]
+        L.qjl_evolve_drive.argtypes = [_P(Model), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_double, ctypes.c_int, _D]
         L.qjl_rhs.argtypes = [_P(Model), ctypes.c_double, _D, _D]
         _lib = L
     return _lib
@@ -149,6 +151,17 @@ def lindblad_evolve(m, rho, n_periods, steps_per_period=2000):
     """rho-space RK4 of the dimer Lindblad equation over n_periods periods."""
     r = np.ascontiguousarray(rho, dtype=np.complex128).copy()
     rc = lib().qjl_evolve(ctypes.byref(m), steps_per_period, n_periods, _p(r.view(np.float64)))
+    assert rc == 0
+    return r
+
+
+def lindblad_evolve_drive(m, rho, n_periods, steps_per_period=2000, drv_type=0, drv_phase=0.0, per_stage=1,
+                          period0=0):
+    """rho-space RK4 with CQdiss's per-stage drive (per_stage=1) or the per-step
+    square drive (per_stage=0), from period period0 on."""
+    r = np.ascontiguousarray(rho, dtype=np.complex128).copy()
+    rc = lib().qjl_evolve_drive(ctypes.byref(m), steps_per_period, period0, n_periods, drv_type, drv_phase,
+                                per_stage, _p(r.view(np.float64)))
     assert rc == 0
     return r
 

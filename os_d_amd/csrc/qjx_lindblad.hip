@@ -13,13 +13,22 @@
 //
 // Each stage kernel evaluates the right-hand side with exactly the operations, in
 // exactly the order, of the CPU restatement used as the oracle, with floating-point
-// contraction off, so both produce the same bits.  The drive is held constant over
-// each step and steps are aligned with the half periods (CQdiss evaluates
-// sign(sin(omega t)) per stage, taking one stage from the neighbouring half period
-// at each switch).
+// contraction off, so both produce the same bits.
+//
+// The drive E(t) follows CQdiss's calcVectValue_real (calcODE.cpp:100-116): at every
+// stage time t (time, time + h/2, time + h/2, time + h with time = p T + s h) it is
+// E + A sin(omega t + phase) (dimer_drv_type 1) or E + A sign(sin(omega t + phase))
+// (type 0, a zero sine counting as positive), so at each switch of the square drive
+// one stage takes the neighbouring half period's value, as there.  The sine is the
+// Cody-Waite + Taylor code the CPU restatement uses.  The other rule holds the
+// square drive constant over each step, steps aligned with the half periods.
+// Stage kernels read the step counter from device memory, so one captured graph
+// holds many consecutive steps.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <fstream>
+#include <iomanip>
 #include <string>
 #include <vector>
 
@@ -38,6 +47,13 @@ struct LBands {
     const double *hd, *hdrv, *ho, *ld, *lo, *q0, *q1, *q2;
     double g;
     int d;
+};
+
+// E(t): the drive at the stage times of step g = *step + j (qjx_lindblad_drive)
+struct LDrive {
+    const long long *step;    // steps done since the integration's t = 0
+    double T, h, E0, A, freq, phase, Eplus, Eminus;
+    int S, type, per_stage;
 };
 
 namespace {
@@ -92,12 +108,65 @@ __device__ inline c2 rhs_at(const LBands &B, double E, const c2 *x, int i, int j
     return cadd(mi, diss);
 }
 
+// sin x: quadrant k = nearest integer to x 2/pi, r = x - k pi/2 with pi/2 split into
+// 33 + 33 + 53 significant bits (Cody & Waite 1980), then the Taylor series of sin r
+// or cos r on |r| <= pi/4 by Horner: the operations of the CPU restatement.
+__device__ inline double lb_sin(double x)
+{
+    const double k = rint(x * 0x1.45f306dc9c883p-1);
+    double r = x - k * 0x1.921fb54400000p+0;
+    r = r - k * 0x1.0b4611a600000p-34;
+    r = r - k * 0x1.3198a2e037073p-69;
+    const double z = r * r;
+    double ps = -0x1.761b41316381ap-75;
+    ps = fma(ps, z, 0x1.71b8ef6dcf572p-66);
+    ps = fma(ps, z, -0x1.2f49b46814157p-57);
+    ps = fma(ps, z, 0x1.952c77030ad4ap-49);
+    ps = fma(ps, z, -0x1.ae7f3e733b81fp-41);
+    ps = fma(ps, z, 0x1.6124613a86d09p-33);
+    ps = fma(ps, z, -0x1.ae64567f544e4p-26);
+    ps = fma(ps, z, 0x1.71de3a556c734p-19);
+    ps = fma(ps, z, -0x1.a01a01a01a01ap-13);
+    ps = fma(ps, z, 0x1.1111111111111p-7);
+    ps = fma(ps, z, -0x1.5555555555555p-3);
+    double pc = 0x1.f2cf01972f578p-80;
+    pc = fma(pc, z, -0x1.0ce396db7f853p-70);
+    pc = fma(pc, z, 0x1.e542ba4020225p-62);
+    pc = fma(pc, z, -0x1.6827863b97d97p-53);
+    pc = fma(pc, z, 0x1.ae7f3e733b81fp-45);
+    pc = fma(pc, z, -0x1.93974a8c07c9dp-37);
+    pc = fma(pc, z, 0x1.1eed8eff8d898p-29);
+    pc = fma(pc, z, -0x1.27e4fb7789f5cp-22);
+    pc = fma(pc, z, 0x1.a01a01a01a01ap-16);
+    pc = fma(pc, z, -0x1.6c16c16c16c17p-10);
+    pc = fma(pc, z, 0x1.5555555555555p-5);
+    pc = fma(pc, z, -0x1.0000000000000p-1);
+    const double s = fma(r * z, ps, r);
+    const double c = fma(z, pc, 1.0);
+    const int q = (int)((long long)k & 3);
+    return q == 0 ? s : q == 1 ? c : q == 2 ? -s : -c;
+}
+
+// E at stage `stage` (0..3) of step *D.step + j
+__device__ inline double stage_drive(const LDrive &D, int j, int stage)
+{
+    const long long g = *D.step + j;
+    const long long p = g / D.S, s = g % D.S;
+    if (!D.per_stage) return s < D.S / 2 ? D.Eplus : D.Eminus;
+    const double time = double(p) * D.T + double(s) * D.h;
+    const double t = stage == 0 ? time : stage == 3 ? time + D.h : time + D.h / 2.0;
+    const double sn = lb_sin(D.freq * t + D.phase);
+    const double f = D.type == 1 ? sn : (sn < 0.0 ? -1.0 : 1.0);
+    return D.E0 + D.A * f;
+}
+
 // stage s of one RK4 step: k = f(x); s < 3: v = rho + c k (k kept for the final
 // combination); s = 3: rho += h6 (((k1 + 2 k2) + 2 k3) + k)
-__global__ __launch_bounds__(256) void rk4_stage_kernel(LBands B, double E, const c2 *x, c2 *rho, c2 *kout,
-                                                        c2 *vout, double c, const c2 *k1, const c2 *k2,
+__global__ __launch_bounds__(256) void rk4_stage_kernel(LBands B, LDrive D, int j, int stage, const c2 *x, c2 *rho,
+                                                        c2 *kout, c2 *vout, double c, const c2 *k1, const c2 *k2,
                                                         const c2 *k3, double h6, int last)
 {
+    const double E = stage_drive(D, j, stage);
     const size_t n = (size_t)B.d * B.d;
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
         const int i = (int)(e / B.d), j = (int)(e % B.d);
@@ -111,6 +180,9 @@ __global__ __launch_bounds__(256) void rk4_stage_kernel(LBands B, double E, cons
         }
     }
 }
+
+// the step counter advances by n (one thread)
+__global__ void lb_advance_kernel(long long *step, int n) { *step += n; }
 
 // <k>, <(k - <k>)^2>, <H + E0 H_drv>, adr_k = rho_kk (fixed-order sums, one workgroup)
 __global__ __launch_bounds__(256) void lb_obs_kernel(LBands B, double E0, const c2 *rho, double *out, double *adr)
@@ -162,15 +234,22 @@ using qjx::set_last_error;
 
 struct qjx_lindblad {
     int d = 0, steps = 0;
-    double T = 0.0, h = 0.0, E0 = 0.0, E1 = 0.0, Estat = 0.0;
+    double T = 0.0, h = 0.0, Estat = 0.0;
     hipStream_t stream = nullptr;
     double *bands = nullptr;          // hd, hdrv, ho, ld, lo, q0, q1, q2 (d each)
     double2 *buf = nullptr;           // rho, va, vb, k1, k2, k3 (d x d each)
     double *obs = nullptr;            // 3 observables + purity + adr[d]
+    long long *d_step = nullptr;      // RK4 steps since t = 0 (device)
     qjx::LBands B{};
-    hipGraphExec_t exec[2] = {nullptr, nullptr};   // one RK4 step with E0 / E1
-    hipGraph_t graph[2] = {nullptr, nullptr};
+    qjx::LDrive D{};
+    bool graphs = true;
+    hipGraphExec_t exec = nullptr;    // kChunk consecutive RK4 steps and the counter advance
+    hipGraph_t graph = nullptr;
 };
+
+namespace {
+constexpr int kChunk = 50;            // RK4 steps per graph launch
+}
 
 namespace {
 
@@ -223,16 +302,25 @@ void build_bands(const qjx_model_desc *m, int d, std::vector<double> &h)
         }
 }
 
-void launch_step(qjx_lindblad *l, double E, hipStream_t s)
+// RK4 step *d_step + j (the stage kernels read the drive's step from the counter)
+void launch_step(qjx_lindblad *l, int j, hipStream_t s)
 {
     const size_t n = nsq(l);
     double2 *rho = l->buf, *va = rho + n, *vb = va + n, *k1 = vb + n, *k2 = k1 + n, *k3 = k2 + n;
     const int grid = (int)std::min<size_t>((n + 255) / 256, 8192);
     const double h = l->h, c2h = 0.5 * h, h6 = h / 6.0;
-    qjx::rk4_stage_kernel<<<grid, 256, 0, s>>>(l->B, E, rho, rho, k1, va, c2h, nullptr, nullptr, nullptr, h6, 0);
-    qjx::rk4_stage_kernel<<<grid, 256, 0, s>>>(l->B, E, va, rho, k2, vb, c2h, nullptr, nullptr, nullptr, h6, 0);
-    qjx::rk4_stage_kernel<<<grid, 256, 0, s>>>(l->B, E, vb, rho, k3, va, h, nullptr, nullptr, nullptr, h6, 0);
-    qjx::rk4_stage_kernel<<<grid, 256, 0, s>>>(l->B, E, va, rho, nullptr, nullptr, 0.0, k1, k2, k3, h6, 1);
+    const qjx::LDrive &D = l->D;
+    qjx::rk4_stage_kernel<<<grid, 256, 0, s>>>(l->B, D, j, 0, rho, rho, k1, va, c2h, nullptr, nullptr, nullptr, h6, 0);
+    qjx::rk4_stage_kernel<<<grid, 256, 0, s>>>(l->B, D, j, 1, va, rho, k2, vb, c2h, nullptr, nullptr, nullptr, h6, 0);
+    qjx::rk4_stage_kernel<<<grid, 256, 0, s>>>(l->B, D, j, 2, vb, rho, k3, va, h, nullptr, nullptr, nullptr, h6, 0);
+    qjx::rk4_stage_kernel<<<grid, 256, 0, s>>>(l->B, D, j, 3, va, rho, nullptr, nullptr, 0.0, k1, k2, k3, h6, 1);
+}
+
+// n consecutive RK4 steps, then the counter advances by n
+void launch_steps(qjx_lindblad *l, int n, hipStream_t s)
+{
+    for (int j = 0; j < n; j++) launch_step(l, j, s);
+    qjx::lb_advance_kernel<<<1, 1, 0, s>>>(l->d_step, n);
 }
 
 }  // namespace
@@ -241,10 +329,9 @@ extern "C" void qjx_lindblad_destroy(qjx_lindblad *l)
 {
     if (!l) return;
     if (l->stream) (void)hipStreamSynchronize(l->stream);
-    for (int b = 0; b < 2; b++) {
-        if (l->exec[b]) (void)hipGraphExecDestroy(l->exec[b]);
-        if (l->graph[b]) (void)hipGraphDestroy(l->graph[b]);
-    }
+    if (l->exec) (void)hipGraphExecDestroy(l->exec);
+    if (l->graph) (void)hipGraphDestroy(l->graph);
+    if (l->d_step) (void)hipFree(l->d_step);
     if (l->bands) (void)hipFree(l->bands);
     if (l->buf) (void)hipFree(l->buf);
     if (l->obs) (void)hipFree(l->obs);
@@ -254,19 +341,28 @@ extern "C" void qjx_lindblad_destroy(qjx_lindblad *l)
 
 extern "C" int qjx_lindblad_create(const qjx_model_desc *m, int steps_per_period, qjx_lindblad **out)
 {
-    if (!m || !out) return lb_fail(QJX_ERR_ARG, "null argument");
+    const qjx_lindblad_drive aligned{0, 0.0, 0};
+    return qjx_lindblad_create_drive(m, steps_per_period, &aligned, out);
+}
+
+extern "C" int qjx_lindblad_create_drive(const qjx_model_desc *m, int steps_per_period,
+                                         const qjx_lindblad_drive *drive, qjx_lindblad **out)
+{
+    if (!m || !drive || !out) return lb_fail(QJX_ERR_ARG, "null argument");
     *out = nullptr;
     if (m->N < 1 || m->N + 1 > 4096) return lb_fail(QJX_ERR_ARG, "N must be in [1, 4095]");
     if (steps_per_period < 2 || (steps_per_period % 2)) return lb_fail(QJX_ERR_ARG, "steps_per_period must be even and >= 2");
     if (!(m->drv_freq > 0.0)) return lb_fail(QJX_ERR_ARG, "dimer_drv_freq must be > 0");
+    if (!drive->per_stage && (drive->drv_type == 1 || drive->drv_phase != 0.0))
+        return lb_fail(QJX_ERR_ARG, "a drive held constant per step is the square drive without phase");
     qjx_lindblad *l = new qjx_lindblad;
     l->d = m->N + 1;
     l->steps = steps_per_period;
     l->T = 2.0 * 3.1415926535897932384626433832795 / m->drv_freq;
     l->h = l->T / double(steps_per_period);
-    l->E0 = m->prm_E + m->drv_ampl;
-    l->E1 = m->prm_E - m->drv_ampl;
     l->Estat = m->prm_E;
+    if (const char *env = std::getenv("QJX_GRAPH"))
+        if (std::string(env) == "0") l->graphs = false;
     auto bail = [&](int code) {
         qjx_lindblad_destroy(l);
         return code;
@@ -276,7 +372,8 @@ extern "C" int qjx_lindblad_create(const qjx_model_desc *m, int steps_per_period
     const int d = l->d;
     const size_t n = nsq(l);
     if (hipStreamCreate(&l->stream) != hipSuccess) return bail(lb_fail(QJX_ERR_HIP, "hipStreamCreate failed"));
-    if (hipMalloc((void **)&l->bands, hb.size() * sizeof(double)) != hipSuccess ||
+    if (hipMalloc((void **)&l->d_step, sizeof(long long)) != hipSuccess ||
+        hipMalloc((void **)&l->bands, hb.size() * sizeof(double)) != hipSuccess ||
         hipMalloc((void **)&l->buf, 6 * n * sizeof(double2)) != hipSuccess ||
         hipMalloc((void **)&l->obs, ((size_t)4 + d) * sizeof(double)) != hipSuccess)
         return bail(lb_fail(QJX_ERR_NOMEM, "Lindblad buffers do not fit in device memory"));
@@ -285,6 +382,22 @@ extern "C" int qjx_lindblad_create(const qjx_model_desc *m, int steps_per_period
     double *bp = l->bands;
     l->B = qjx::LBands{bp, bp + d, bp + 2 * d, bp + 3 * d, bp + 4 * d, bp + 5 * d, bp + 6 * d, bp + 7 * d,
                        m->diss_gamma / double(d - 1), d};
+    qjx::LDrive &D = l->D;
+    D.step = l->d_step;
+    D.T = l->T;
+    D.h = l->h;
+    D.E0 = m->prm_E;
+    D.A = m->drv_ampl;
+    D.freq = m->drv_freq;
+    D.phase = drive->drv_phase;
+    D.Eplus = m->prm_E + m->drv_ampl;
+    D.Eminus = m->prm_E - m->drv_ampl;
+    D.S = steps_per_period;
+    D.type = drive->drv_type;
+    D.per_stage = drive->per_stage;
+    const long long zero = 0;
+    if (hipMemcpy(l->d_step, &zero, sizeof zero, hipMemcpyHostToDevice) != hipSuccess)
+        return bail(lb_fail(QJX_ERR_HIP, "counter upload failed"));
     // start in the pure state e_0 (start_type 0, start_state 0)
     std::vector<double2> r0(n, make_double2(0.0, 0.0));
     r0[0] = make_double2(1.0, 0.0);
@@ -297,7 +410,10 @@ extern "C" int qjx_lindblad_create(const qjx_model_desc *m, int steps_per_period
 extern "C" int qjx_lindblad_set_rho(qjx_lindblad *l, const double *rho)
 {
     if (!l || !rho) return lb_fail(QJX_ERR_ARG, "null argument");
+    LB_TRY(hipStreamSynchronize(l->stream));
     LB_TRY(hipMemcpy(rho_of(l), rho, nsq(l) * sizeof(double2), hipMemcpyHostToDevice));
+    const long long zero = 0;      // a new initial state starts at t = 0
+    LB_TRY(hipMemcpy(l->d_step, &zero, sizeof zero, hipMemcpyHostToDevice));
     return QJX_OK;
 }
 
@@ -314,37 +430,65 @@ extern "C" int qjx_lindblad_run_periods(qjx_lindblad *l, int n_periods)
     if (!l) return lb_fail(QJX_ERR_ARG, "null argument");
     if (n_periods < 0) return lb_fail(QJX_ERR_ARG, "negative period count");
     if (n_periods == 0) return QJX_OK;
-    // one captured RK4 step per half period's drive value, replayed; each launch of
-    // the capture is checked, so a rejected launch is reported as itself
-    for (int b = 0; b < 2; b++) {
-        if (l->exec[b]) continue;
-        LB_TRY(hipGetLastError());                 // an earlier launch error, as itself
-        LB_TRY(hipStreamBeginCapture(l->stream, hipStreamCaptureModeThreadLocal));
-        launch_step(l, b == 0 ? l->E0 : l->E1, l->stream);
-        const hipError_t le = hipGetLastError();
-        hipGraph_t g = nullptr;
-        const hipError_t ee = hipStreamEndCapture(l->stream, &g);   // always ends the capture
-        if (le == hipSuccess && ee == hipSuccess) {
-            const hipError_t ie = hipGraphInstantiate(&l->exec[b], g, nullptr, nullptr, 0);
+    // kChunk consecutive RK4 steps per graph launch (the stage kernels read the step
+    // from the device counter), the remainder launched eagerly; every launch checked
+    const long long total = (long long)n_periods * l->steps;
+    long long done = 0;
+    if (l->graphs && total >= kChunk) {
+        if (!l->exec) {
+            LB_TRY(hipGetLastError());             // an earlier launch error, as itself
+            LB_TRY(hipStreamBeginCapture(l->stream, hipStreamCaptureModeThreadLocal));
+            launch_steps(l, kChunk, l->stream);
+            const hipError_t le = hipGetLastError();
+            hipGraph_t g = nullptr;
+            const hipError_t ee = hipStreamEndCapture(l->stream, &g);   // always ends the capture
+            if (le != hipSuccess || ee != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                (void)hipGetLastError();
+                return lb_fail(QJX_ERR_HIP, std::string("capture of the RK4 steps failed: ") +
+                                                hipGetErrorString(le != hipSuccess ? le : ee));
+            }
+            const hipError_t ie = hipGraphInstantiate(&l->exec, g, nullptr, nullptr, 0);
             if (ie != hipSuccess) {
                 (void)hipGraphDestroy(g);
-                l->exec[b] = nullptr;
+                l->exec = nullptr;
                 return lb_fail(ie == hipErrorOutOfMemory ? QJX_ERR_NOMEM : QJX_ERR_HIP,
                                std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
             }
-            l->graph[b] = g;
-            continue;
+            l->graph = g;
         }
-        if (g) (void)hipGraphDestroy(g);
-        (void)hipGetLastError();
-        return lb_fail(QJX_ERR_HIP, std::string("capture of the RK4 step failed: ") +
-                                        hipGetErrorString(le != hipSuccess ? le : ee));
+        for (; done + kChunk <= total; done += kChunk) {
+            LB_TRY(hipGraphLaunch(l->exec, l->stream));
+            // bound the launches in flight: wait for the previous period's work
+            if ((done / kChunk) % 200 == 199) LB_TRY(hipStreamSynchronize(l->stream));
+        }
     }
-    const int half = l->steps / 2;
-    for (int p = 0; p < n_periods; p++)
-        for (int b = 0; b < 2; b++)
-            for (int s = 0; s < half; s++) LB_TRY(hipGraphLaunch(l->exec[b], l->stream));
+    for (; done < total; done += kChunk) {
+        const int n = (int)std::min<long long>(kChunk, total - done);
+        launch_steps(l, n, l->stream);
+        LB_TRY(hipGetLastError());
+    }
     LB_TRY(hipStreamSynchronize(l->stream));
+    return QJX_OK;
+}
+
+// CQdiss's rho output (data.cpp:40-46 with save_sparse_complex_mtx, utils.cpp:266-310):
+// every entry, one line each, "row col re im" with 1-based indices, setprecision(16)
+// scientific.
+extern "C" int qjx_lindblad_save_rho(qjx_lindblad *l, const char *path)
+{
+    if (!l || !path) return lb_fail(QJX_ERR_ARG, "null argument");
+    std::vector<double2> r(nsq(l));
+    LB_TRY(hipStreamSynchronize(l->stream));
+    LB_TRY(hipMemcpy(r.data(), rho_of(l), r.size() * sizeof(double2), hipMemcpyDeviceToHost));
+    std::ofstream o(path);
+    if (!o.is_open()) return lb_fail(QJX_ERR_IO, std::string("unable to open file ") + path);
+    o << std::setprecision(16) << std::scientific;
+    const int d = l->d;
+    for (int i = 0; i < d; i++)
+        for (int j = 0; j < d; j++)
+            o << i + 1 << " " << j + 1 << " " << r[(size_t)i * d + j].x << " " << r[(size_t)i * d + j].y << std::endl;
+    if (!o.good()) return lb_fail(QJX_ERR_IO, std::string("unable to write file ") + path);
     return QJX_OK;
 }
 

@@ -80,6 +80,10 @@ class JobResult(ctypes.Structure):
                 ("norm_avg", ctypes.c_double)]
 
 
+class LindbladDrive(ctypes.Structure):
+    _fields_ = [("drv_type", ctypes.c_int), ("drv_phase", ctypes.c_double), ("per_stage", ctypes.c_int)]
+
+
 class PoolUsage(ctypes.Structure):
     _fields_ = [("used_bytes", ctypes.c_int64), ("budget_bytes", ctypes.c_int64), ("trajectories", ctypes.c_int64),
                 ("capacity", ctypes.c_int64), ("queued", ctypes.c_int32), ("building", ctypes.c_int32),
@@ -134,6 +138,9 @@ SIGNATURES = [
     ("qjx_lindblad_run_periods", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("qjx_lindblad_observables", ctypes.c_int, [ctypes.c_void_p, _D, _D, _D, _D, _D]),
     ("qjx_lindblad_dim", ctypes.c_int, [ctypes.c_void_p]),
+    ("qjx_lindblad_create_drive", ctypes.c_int, [_P(ModelDesc), ctypes.c_int, _P(LindbladDrive),
+                                                 _P(ctypes.c_void_p)]),
+    ("qjx_lindblad_save_rho", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
     ("qjx_lpn_create", ctypes.c_int, [ctypes.c_void_p, _P(LpnDesc), _P(ctypes.c_void_p)]),
     ("qjx_lpn_destroy", None, [ctypes.c_void_p]),
     ("qjx_lpn_trans", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
@@ -377,11 +384,16 @@ class Lindblad:
     """The dimer's Lindblad equation integrated in rho space on the GPU (FP64 RK4)."""
 
     def __init__(self, N, steps_per_period=2000, prm_E=1.0, prm_U=0.5, prm_J=1.0, drv_ampl=1.5, drv_freq=1.0,
-                 diss_gamma=0.1):
+                 diss_gamma=0.1, drv_type=0, drv_phase=0.0, per_stage=0):
+        """per_stage=1: the drive at every RK4 stage time as CQdiss evaluates it
+        (drv_type 1 sine, else the sign of the sine); per_stage=0: the square drive
+        held constant over each step."""
         self.lib = load_library()
         m = ModelDesc(N, prm_E, prm_U, prm_J, drv_ampl, drv_freq, diss_gamma, 1, 4, 0)
         h = ctypes.c_void_p()
-        _check(self.lib, self.lib.qjx_lindblad_create(ctypes.byref(m), steps_per_period, ctypes.byref(h)))
+        dr = LindbladDrive(drv_type, drv_phase, per_stage)
+        _check(self.lib, self.lib.qjx_lindblad_create_drive(ctypes.byref(m), steps_per_period, ctypes.byref(dr),
+                                                            ctypes.byref(h)))
         self.h = h
         self.d = N + 1
 
@@ -407,6 +419,9 @@ class Lindblad:
 
     def run_periods(self, n):
         _check(self.lib, self.lib.qjx_lindblad_run_periods(self.h, n))
+
+    def save_rho(self, path):
+        _check(self.lib, self.lib.qjx_lindblad_save_rho(self.h, path.encode()))
 
     def observables(self):
         v = np.zeros(4)
