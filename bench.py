@@ -181,7 +181,8 @@ def observables_leg(qjx, N, M, S, D, periods, forms):
     return out
 
 
-def drop_in_leg(qjx, N, M, S, D, trans_periods, periods, cpu_budget_s, cpu_traj, label, cpu_periods=1):
+def drop_in_leg(qjx, N, M, S, D, trans_periods, periods, cpu_budget_s, cpu_traj, label, cpu_periods=1,
+                physics_periods=0):
     """The plain FP64 drop-in on one configuration: M trajectories of the dimer at N
     on the tree (S, D), `trans_periods` transient periods, then `periods` timed
     periods on graph replay; the CPU oracle (OpenMP over trajectories, one zgemv per
@@ -194,16 +195,37 @@ def drop_in_leg(qjx, N, M, S, D, trans_periods, periods, cpu_budget_s, cpu_traj,
     if cpu_budget_s > 0 and trans_periods > 0:
         start = e.get_state()
     s0 = e.stats()
-    t0 = time.perf_counter()
-    e.run_periods(periods, 0)
-    dt = time.perf_counter() - t0
+    physics = None
+    if physics_periods:
+        # the ensemble against its Lindblad oracle (RK4, 1e4 steps per period, CQdiss's
+        # per-stage drive) after physics_periods of the timed periods (untimed check)
+        t0 = time.perf_counter()
+        e.run_periods(physics_periods, 0)
+        dt = time.perf_counter() - t0
+        lb = qjx.Lindblad(N, steps_per_period=10000, diss_gamma=0.1, per_stage=1, prm_E=1.0, prm_U=0.5, prm_J=1.0,
+                          drv_ampl=1.5, drv_freq=1.0)
+        lb.run_periods(trans_periods + physics_periods)
+        ob = lb.observables()
+        lb.close()
+        c = e.chars()
+        physics = {"periods": trans_periods + physics_periods, "trajectories": M}
+        for k in ("mean", "energy"):
+            v, sig = float(c[k].mean()), float(c[k].std() / M ** 0.5)
+            physics[k] = {"ensemble": v, "lindblad": ob[k], "sigma": sig, "error_in_sigma": (v - ob[k]) / sig}
+        t0 = time.perf_counter()
+        e.run_periods(periods - physics_periods, 0)
+        dt += time.perf_counter() - t0
+    else:
+        t0 = time.perf_counter()
+        e.run_periods(periods, 0)
+        dt = time.perf_counter() - t0
     s1 = e.stats()
     d = _stats_delta(s0, s1)
     res = {"config": label, "N": N, "dim": N + 1, "trajectories": M, "steps_per_period": S, "ex_deep": D,
            "transient_periods": trans_periods, "timed_periods": periods, "seconds": dt, "transient_s": trans_s,
            "tp_per_s": M * periods / dt, "matvecs_per_traj_period": d["matvecs"] / (M * periods),
            "jumps_per_traj_period": d["jumps"] / (M * periods), "iterations": d["iterations"],
-           "dense_equivalent_tflops": d["gemm_flops"] / dt / 1e12}
+           "dense_equivalent_tflops": d["gemm_flops"] / dt / 1e12, "lindblad_check": physics}
     if cpu_budget_s > 0:
         res["cpu_baseline"] = cpu_baseline(e, N, S, D, cpu_budget_s, ntraj=cpu_traj, start=start,
                                            max_steps=cpu_periods * S)
@@ -375,7 +397,7 @@ def lindblad(qjx, N=100, steps=10000, periods=2, cpu_steps=200):
     import numpy as np
     from oracle import oracle as orc
     params = dict(prm_E=1.0, prm_U=0.5, prm_J=1.0, drv_ampl=1.5, drv_freq=1.0)
-    lb = qjx.Lindblad(N, steps_per_period=steps, diss_gamma=0.1, **params)
+    lb = qjx.Lindblad(N, steps_per_period=steps, diss_gamma=0.1, per_stage=1, **params)
     lb.run_periods(1)                       # warm-up: graph capture, caches
     t0 = time.perf_counter()
     lb.run_periods(periods)
@@ -389,7 +411,7 @@ def lindblad(qjx, N=100, steps=10000, periods=2, cpu_steps=200):
     rho = np.zeros((N + 1, N + 1), complex)
     rho[0, 0] = 1.0
     t0 = time.perf_counter()
-    orc.lindblad_evolve(m, rho, 1, steps_per_period=cpu_steps)
+    orc.lindblad_evolve_drive(m, rho, 1, steps_per_period=cpu_steps, per_stage=1)
     dtc = time.perf_counter() - t0
     cpu_steps_per_s = cpu_steps / dtc
     gpu_steps_per_s = periods * steps / dt
@@ -671,7 +693,8 @@ def main():
         # the plain CPU configuration: N=100, 1000 trajectories x 100 periods, S=200
         say("c1 d=101")
         c1 = drop_in_leg(qjx, 100, 1000, 200, 16, 0, 100, min(cpu_s, 10.0), 0,
-                         "N=100 (d=101), 1000 trajectories x 100 periods, S=200, ex_deep 16, FP64", cpu_periods=100)
+                         "N=100 (d=101), 1000 trajectories x 100 periods, S=200, ex_deep 16, FP64", cpu_periods=100,
+                         physics_periods=10)
     if "c2" in legs:
         # the FP64 drop-in at N=500 on the reference's tree (4 top-level steps per period)
         say("c2 d=501")
