@@ -141,74 +141,123 @@ __global__ __launch_bounds__(kPlanThreads) void plan_kernel(RunArgs a)
 // contiguous ranges, one per XCD (workgroup w runs on XCD w % 8), so the row tiles
 // of a column tile are claimed on one XCD and share its L2; a workgroup whose range
 // is exhausted takes tiles from the other XCDs' ranges.  Lane 0 of wave 0 claims
-// with an atomic; wave 0 then finds the tile's item in the plan prefix with a
-// two-level ballot search (two dependent loads instead of log2(items)) and
-// broadcasts through LDS.
+// with an atomic (tile_claim, issued early: the GEMM kernels claim their next tile
+// before their epilogue's stores and wait for the answer after them); wave 0 then
+// finds the tile's item in the plan prefix with a two-level ballot search (two
+// dependent loads instead of log2(items)), places the tile in its item and leaves
+// the result in LDS for tile_read.
 struct TileRef {
     int e, g, rt, col0, ncol;
 };
 
 static_assert(kPlanItems <= 64 * 64, "two-level tile search");
 
-template <int BN>
-__device__ inline bool tile_next(const RunArgs &a, const int *pre, int nitems, int total, int *s_claim,
-                                 TileRef &r)
+// Lane 0: the next index of this workgroup's XCD range.  A claim past the range's
+// end is harmless (the counters restart every iteration): tile_resolve then takes a
+// tile from another range.
+__device__ inline int tile_claim(const RunArgs &a)
 {
-    if (threadIdx.x < 64) {
-        const int lane = threadIdx.x;
-        int t = -1;
-        if (lane == 0) {
-            const int x0 = blockIdx.x & 7;
-            for (int i = 0; i < 8 && t < 0; i++) {
-                const int x = (x0 + i) & 7;
-                const int lo = (int)(((long long)total * x) >> 3), hi = (int)(((long long)total * (x + 1)) >> 3);
-                if (hi <= lo) continue;
-                if (a.plan->xnext[x] >= hi - lo) continue;           // cheap pre-check
-                const int c = atomicAdd(&a.plan->xnext[x], 1);
-                if (c < hi - lo) t = lo + c;
+    return threadIdx.x == 0 ? atomicAdd(&a.plan->xnext[blockIdx.x & 7], 1) : 0;
+}
+
+// Wave 0: the tile of claim c0 (lane 0's tile_claim result) -> s_claim[0..4] =
+// ensemble, group, row tile, first column, columns; s_claim[0] = -1 when no tile is
+// left.
+template <int BN>
+__device__ inline void tile_resolve(const RunArgs &a, const int *pre, int nitems, int total, int c0, int *s_claim)
+{
+    const int lane = threadIdx.x;
+    int t = -1;
+    if (lane == 0) {
+        const int x0 = blockIdx.x & 7;
+        for (int i = 0; i < 8 && t < 0; i++) {
+            const int x = (x0 + i) & 7;
+            const int lo = (int)(((long long)total * x) >> 3), hi = (int)(((long long)total * (x + 1)) >> 3);
+            if (i == 0) {
+                if (c0 < hi - lo) t = lo + c0;
+                continue;
             }
-        }
-        t = __shfl(t, 0, 64);
-        int item = 0, base = 0;
-        if (t >= 0) {
-            // the largest item with pre[item] <= t (pre non-decreasing, pre[0] = 0,
-            // pre[nitems] = total > t)
-            const int i1 = lane * 64;
-            const unsigned long long m1 = __ballot(i1 < nitems && gld<int>(pre + i1) <= t);
-            const int b = (63 - __clzll(m1)) * 64;
-            const int i2 = b + lane;
-            const int v2 = i2 < nitems ? gld<int>(pre + i2) : 0x7fffffff;
-            const unsigned long long m2 = __ballot(v2 <= t);
-            const int w = 63 - __clzll(m2);
-            item = b + w;
-            base = __shfl(v2, w, 64);
-        }
-        if (lane == 0) {
-            s_claim[0] = t;
-            s_claim[1] = item;
-            s_claim[2] = base;
+            if (hi <= lo) continue;
+            if (a.plan->xnext[x] >= hi - lo) continue;           // cheap pre-check
+            const int c = atomicAdd(&a.plan->xnext[x], 1);
+            if (c < hi - lo) t = lo + c;
         }
     }
-    __syncthreads();
-    const int t = s_claim[0], item = s_claim[1];
-    const int local = t - s_claim[2];
-    __syncthreads();
-    if (t < 0) return false;
-    r.e = item / kMaxGroups;
-    r.g = item % kMaxGroups;
-    const Ens &E = a.ens[r.e];
+    t = __shfl(t, 0, 64);
+    if (t < 0) {
+        if (lane == 0) s_claim[0] = -1;
+        return;
+    }
+    // the largest item with pre[item] <= t (pre non-decreasing, pre[0] = 0,
+    // pre[nitems] = total > t)
+    const int i1 = lane * 64;
+    const unsigned long long m1 = __ballot(i1 < nitems && gld<int>(pre + i1) <= t);
+    const int b = (63 - __clzll(m1)) * 64;
+    const int i2 = b + lane;
+    const int v2 = i2 < nitems ? gld<int>(pre + i2) : 0x7fffffff;
+    const unsigned long long m2 = __ballot(v2 <= t);
+    const int w = 63 - __clzll(m2);
+    const int item = b + w;
+    const int local = t - __shfl(v2, w, 64);
+    const int e = item / kMaxGroups, g = item % kMaxGroups;
+    const Ens &E = a.ens[e];
     // tiles of an item: super-columns of a.sc column tiles, each visited row tile
     // by row tile, so the row tile's propagator band is reused from L2 by the
     // super-column's column tiles and the column tiles' states by the row tiles
-    const int cnt = gld<int>(E.cnt[a.parity] + r.g);
+    const int cnt = gld<int>(E.cnt[a.parity] + g);
     const int nct = (cnt + BN - 1) / BN;
     const int per = E.R * a.sc;
     const int sc = local / per, rem = local - sc * per;
     const int cw = min(a.sc, nct - sc * a.sc);
-    r.rt = rem / cw;
-    r.col0 = (sc * a.sc + rem % cw) * BN;
-    r.ncol = min(BN, cnt - r.col0);
+    const int col0 = (sc * a.sc + rem % cw) * BN;
+    if (lane == 0) {
+        s_claim[0] = e;
+        s_claim[1] = g;
+        s_claim[2] = rem / cw;
+        s_claim[3] = col0;
+        s_claim[4] = min(BN, cnt - col0);
+    }
+}
+
+// Every thread, after a barrier that follows tile_resolve: the claimed tile (held
+// in scalar registers), false when none is left.
+__device__ inline bool tile_read(const int *s_claim, TileRef &r)
+{
+    r.e = __builtin_amdgcn_readfirstlane(s_claim[0]);
+    if (r.e < 0) return false;
+    r.g = __builtin_amdgcn_readfirstlane(s_claim[1]);
+    r.rt = __builtin_amdgcn_readfirstlane(s_claim[2]);
+    r.col0 = __builtin_amdgcn_readfirstlane(s_claim[3]);
+    r.ncol = __builtin_amdgcn_readfirstlane(s_claim[4]);
     return true;
+}
+
+// Claim and read in one step (kernels that do not overlap the claim with work).
+template <int BN>
+__device__ inline bool tile_next(const RunArgs &a, const int *pre, int nitems, int total, int *s_claim,
+                                 TileRef &r)
+{
+    if (threadIdx.x < 64) tile_resolve<BN>(a, pre, nitems, total, tile_claim(a), s_claim);
+    __syncthreads();
+    const bool ok = tile_read(s_claim, r);
+    __syncthreads();
+    return ok;
+}
+
+// The two state slots of a tile's ensemble (wave-uniform: read once per tile)
+// and the column of trajectory tr in slot cur.
+struct Slots {
+    char *p0, *p1;
+    template <typename T>
+    __device__ T *col(int cur, int tr, int L) const
+    {
+        return reinterpret_cast<T *>(cur ? p1 : p0) + (size_t)tr * L;
+    }
+};
+
+__device__ inline Slots tile_slots(const Ens &E)
+{
+    return Slots{static_cast<char *>(E.slot[0]), static_cast<char *>(E.slot[1])};
 }
 
 template <int BN>
@@ -323,12 +372,15 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
     const int wm = wid / NWN, wn = wid % NWN;
     const int h = lane >> 5, li = lane & 31;
     const int cp_k = (tid & 7) * 4, cp_r = tid >> 3;
-    __shared__ int s_claim[3];
+    __shared__ int s_claim[5];
     TileRef r;
-    while (tile_next<BN>(a, s_tpre, nitems, total, s_claim, r)) {
+    if (wid == 0) tile_resolve<BN>(a, s_tpre, nitems, total, tile_claim(a), s_claim);
+    __syncthreads();
+    while (tile_read(s_claim, r)) {
         load_cols<BN>(a, r, s_tr, s_cur);
         __syncthreads();
         const Ens &E = a.ens[r.e];
+        const Slots sl = tile_slots(E);
         const int L = E.L, r0 = r.rt * MF_BM;
         const float *U = reinterpret_cast<const float *>(E.props) + (size_t)r.g * L * L;
         const float *arow = U + (size_t)(r0 + cp_r) * L + cp_k;
@@ -337,7 +389,7 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
             const int c = cp_r + (T / 8) * q;
-            bcol[q] = reinterpret_cast<const float *>(E.slot[s_cur[c]]) + (size_t)s_tr[c] * L + cp_k;
+            bcol[q] = sl.col<const float>(s_cur[c], s_tr[c], L) + cp_k;
         }
         floatx16 acc[2][NJ];
 #pragma unroll
@@ -431,13 +483,16 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
             }
             if (kb < nk) iter(st1, kb);
         }
+        // the next tile is claimed now and resolved after this tile's stores
+        int cn = 0;
+        if (wid == 0) cn = tile_claim(a);
         double ns[NJ];
 #pragma unroll
         for (int j = 0; j < NJ; j++) {
             ns[j] = 0.0;
             const int c = wn * (BN / NWN) + j * 32 + li;
             const bool ok = c < r.ncol;
-            float *y = reinterpret_cast<float *>(E.slot[s_cur[c] ^ 1]) + (size_t)s_tr[c] * L;
+            float *y = sl.col<float>(s_cur[c] ^ 1, s_tr[c], L);
 #pragma unroll
             for (int i = 0; i < 2; i++)
 #pragma unroll
@@ -449,6 +504,7 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
                     for (int e = 0; e < 4; e++) ns[j] += (double)v[e] * (double)v[e];
                 }
         }
+        if (wid == 0) tile_resolve<BN>(a, s_tpre, nitems, total, cn, s_claim);
 #pragma unroll
         for (int j = 0; j < NJ; j++) ns[j] += __shfl_xor(ns[j], 32, 64);
         if (h == 0) {
@@ -506,12 +562,15 @@ __global__ __launch_bounds__(128 * NWN) void gemm_bf16x3(RunArgs a)
     const int h = lane >> 5, li = lane & 31;
     const int ca_k = (tid & 3) * 8, ca_r = tid >> 2;
     const int cb_k = (tid & 7) * 4, cb_r = tid >> 3;
-    __shared__ int s_claim[3];
+    __shared__ int s_claim[5];
     TileRef r;
-    while (tile_next<BN>(a, s_tpre, nitems, total, s_claim, r)) {
+    if (wid == 0) tile_resolve<BN>(a, s_tpre, nitems, total, tile_claim(a), s_claim);
+    __syncthreads();
+    while (tile_read(s_claim, r)) {
         load_cols<BN>(a, r, s_tr, s_cur);
         __syncthreads();
         const Ens &E = a.ens[r.e];
+        const Slots sl = tile_slots(E);
         const int L = E.L, r0 = r.rt * MF_BM;
         const size_t LL = (size_t)L * L;
         const __bf16 *Uh = reinterpret_cast<const __bf16 *>(E.props) + (size_t)r.g * LL;
@@ -520,7 +579,7 @@ __global__ __launch_bounds__(128 * NWN) void gemm_bf16x3(RunArgs a)
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
             const int c = cb_r + (T / 8) * q;
-            bcol[q] = reinterpret_cast<const float *>(E.slot[s_cur[c]]) + (size_t)s_tr[c] * L + cb_k;
+            bcol[q] = sl.col<const float>(s_cur[c], s_tr[c], L) + cb_k;
         }
         floatx16 acc[2][NJ];
 #pragma unroll
@@ -650,13 +709,16 @@ __global__ __launch_bounds__(128 * NWN) void gemm_bf16x3(RunArgs a)
             }
             if (kb < nk) iter(st1, kb);
         }
+        // the next tile is claimed now and resolved after this tile's stores
+        int cn = 0;
+        if (wid == 0) cn = tile_claim(a);
         double ns[NJ];
 #pragma unroll
         for (int j = 0; j < NJ; j++) {
             ns[j] = 0.0;
             const int c = wn * (BN / NWN) + j * 32 + li;
             const bool ok = c < r.ncol;
-            float *y = reinterpret_cast<float *>(E.slot[s_cur[c] ^ 1]) + (size_t)s_tr[c] * L;
+            float *y = sl.col<float>(s_cur[c] ^ 1, s_tr[c], L);
 #pragma unroll
             for (int i = 0; i < 2; i++)
 #pragma unroll
@@ -668,6 +730,7 @@ __global__ __launch_bounds__(128 * NWN) void gemm_bf16x3(RunArgs a)
                     for (int e = 0; e < 4; e++) ns[j] += (double)v[e] * (double)v[e];
                 }
         }
+        if (wid == 0) tile_resolve<BN>(a, s_tpre, nitems, total, cn, s_claim);
 #pragma unroll
         for (int j = 0; j < NJ; j++) ns[j] += __shfl_xor(ns[j], 32, 64);
         if (h == 0) {
@@ -693,7 +756,7 @@ __global__ __launch_bounds__(128 * NWN) void gemm_bf16x3(RunArgs a)
 constexpr int F64_PITCH = MF_BK + 2;
 
 template <int BN>
-__global__ __launch_bounds__(512) void gemm_f64(RunArgs a)
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(BN == 64 ? 4 : 1))) void gemm_f64(RunArgs a)
 {
     constexpr int NJ = BN / 32;          // 16-column MFMA tiles per wave
     constexpr int NQ = BN / 32;          // B-tile rows (columns of X) copied per thread
@@ -709,12 +772,15 @@ __global__ __launch_bounds__(512) void gemm_f64(RunArgs a)
     const int wm = wid >> 1, wn = wid & 1;
     const int qq = lane >> 4, li = lane & 15;
     const int cp_k = (tid & 15) * 2, cp_r = tid >> 4;
-    __shared__ int s_claim[3];
+    __shared__ int s_claim[5];
     TileRef r;
-    while (tile_next<BN>(a, s_tpre, nitems, total, s_claim, r)) {
+    if (wid == 0) tile_resolve<BN>(a, s_tpre, nitems, total, tile_claim(a), s_claim);
+    __syncthreads();
+    while (tile_read(s_claim, r)) {
         load_cols<BN>(a, r, s_tr, s_cur);
         __syncthreads();
         const Ens &E = a.ens[r.e];
+        const Slots sl = tile_slots(E);
         const int L = E.L, r0 = r.rt * MF_BM;
         const double *U = reinterpret_cast<const double *>(E.props) + (size_t)r.g * L * L;
         const double *arow = U + (size_t)(r0 + cp_r) * L + cp_k;
@@ -723,7 +789,7 @@ __global__ __launch_bounds__(512) void gemm_f64(RunArgs a)
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
             const int c = cp_r + 32 * q;
-            bcol[q] = reinterpret_cast<const double *>(E.slot[s_cur[c]]) + (size_t)s_tr[c] * L + cp_k;
+            bcol[q] = sl.col<const double>(s_cur[c], s_tr[c], L) + cp_k;
         }
         doublex4 acc[2][NJ];
 #pragma unroll
@@ -792,13 +858,16 @@ __global__ __launch_bounds__(512) void gemm_f64(RunArgs a)
                 __syncthreads();
             }
         }
+        // the next tile is claimed now and resolved after this tile's stores
+        int cn = 0;
+        if (wid == 0) cn = tile_claim(a);
         double ns[NJ];
 #pragma unroll
         for (int j = 0; j < NJ; j++) {
             ns[j] = 0.0;
             const int c = wn * (BN / 2) + j * 16 + li;
             const bool ok = c < r.ncol;
-            double *y = reinterpret_cast<double *>(E.slot[s_cur[c] ^ 1]) + (size_t)s_tr[c] * L;
+            double *y = sl.col<double>(s_cur[c] ^ 1, s_tr[c], L);
 #pragma unroll
             for (int i = 0; i < 2; i++) {
                 const int row = r0 + wm * 32 + i * 16 + qq;
@@ -809,6 +878,7 @@ __global__ __launch_bounds__(512) void gemm_f64(RunArgs a)
                 }
             }
         }
+        if (wid == 0) tile_resolve<BN>(a, s_tpre, nitems, total, cn, s_claim);
 #pragma unroll
         for (int j = 0; j < NJ; j++) {
             ns[j] += __shfl_xor(ns[j], 16, 64);
@@ -860,15 +930,16 @@ __global__ __launch_bounds__(256) void gemm_fma(RunArgs a)
     const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
     const int a_i = tid / (FM_BK / 4), a_k = (tid % (FM_BK / 4)) * 4;
     const int b_j = tid / (FM_BK / 4), b_k = (tid % (FM_BK / 4)) * 4;
-    __shared__ int s_claim[3];
+    __shared__ int s_claim[5];
     TileRef r;
     while (tile_next<FM_BN>(a, s_tpre, nitems, total, s_claim, r)) {
         load_cols<FM_BN>(a, r, s_tr, s_cur);
         __syncthreads();
         const Ens &E = a.ens[r.e];
+        const Slots sl = tile_slots(E);
         const int L = E.L, r0 = r.rt * FM_BM;
         const T *U = reinterpret_cast<const T *>(E.props) + (size_t)r.g * L * L;
-        const T *xcol = reinterpret_cast<const T *>(E.slot[s_cur[b_j]]) + (size_t)s_tr[b_j] * L;
+        const T *xcol = sl.col<const T>(s_cur[b_j], s_tr[b_j], L);
         T acc[4][4];
 #pragma unroll
         for (int i = 0; i < 4; i++)
@@ -900,7 +971,7 @@ __global__ __launch_bounds__(256) void gemm_fma(RunArgs a)
             const int c = tx + 16 * j;
             double ns = 0.0;
             if (c < r.ncol) {
-                T *y = reinterpret_cast<T *>(E.slot[s_cur[c] ^ 1]) + (size_t)s_tr[c] * L + r0 + ty * 4;
+                T *y = sl.col<T>(s_cur[c] ^ 1, s_tr[c], L) + r0 + ty * 4;
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
                     y[i] = acc[i][j];
@@ -927,7 +998,8 @@ __global__ __launch_bounds__(256) void gemm_fma(RunArgs a)
 // and t += dt_parent).  Level 0 finishes after S/2 steps and closes the half period;
 // a level l >= 1 finishes after 2 steps.
 // ---------------------------------------------------------------------------
-__device__ inline void complete_step(TrajState &s, int l, const double *dt, int S)
+template <typename DT>
+__device__ inline void complete_step(TrajState &s, int l, DT dt, int S)
 {
     for (;;) {
         if (l == 0) {
@@ -946,8 +1018,8 @@ __device__ inline void complete_step(TrajState &s, int l, const double *dt, int 
         }
         s.bits &= ~bit;
         l -= 1;
-        s.t -= dt[l];
-        s.t += dt[l];
+        s.t -= dt(l);
+        s.t += dt(l);
     }
     s.lvl = l;
 }
@@ -964,6 +1036,7 @@ __global__ __launch_bounds__(256) void decide_kernel(RunArgs a)
     __shared__ int s_cnt[kWin + 3];      // [kWin] jumps, [kWin+1..2] jumps per window ensemble
     __shared__ int s_base[kWin + 1];
     __shared__ int s_next;
+    __shared__ double s_dt[2][kMaxGroups / 2];   // step sizes of the window's ensembles
     const int nitems = *a.nens * kMaxGroups;
     const Plan *pl = a.plan;
     const int total = pl->ent_pre[nitems];
@@ -976,19 +1049,45 @@ __global__ __launch_bounds__(256) void decide_kernel(RunArgs a)
     const int p = a.parity;
     const int idx = base0 + threadIdx.x;
     const int e_lo = plan_search(s_pre, nitems, base0) / kMaxGroups;
+    if (threadIdx.x < kMaxGroups) {
+        // read while the entries' states are in flight; used after the barrier below
+        const int w = threadIdx.x / (kMaxGroups / 2), l = threadIdx.x % (kMaxGroups / 2);
+        if (e_lo + w < *a.nens) {
+            const Ens &W = a.ens[e_lo + w];
+            if (l < W.D) s_dt[w][l] = W.dt[l];
+        }
+    }
     int e = -1, tr = -1, key = -1, local = -1, direct = -1, jlocal = -1, jsrc = 0, ncur = 0;
     double n = 0.0;
+    TrajState s;
     if (idx < total) {
         const int item = plan_search(s_pre, nitems, idx);
         e = item / kMaxGroups;
         const int g = item % kMaxGroups;
         const Ens &E = a.ens[e];
         tr = E.list[p][(size_t)g * E.M + (idx - s_pre[item])] >> 1;
-        TrajState s = E.st[tr];
-        const double *np = E.normpart + (size_t)tr * E.R;     // R contiguous partials
-        for (int rt = 0; rt < E.R; rt++) n += np[rt];
+        s = E.st[tr];
+        // the R partials (contiguous) are loaded together, then added in ascending
+        // row-tile order
+        const double *np = E.normpart + (size_t)tr * E.R;
+        for (int r0 = 0; r0 < E.R; r0 += 16) {
+            double v[16];
+#pragma unroll
+            for (int q = 0; q < 16; q++) v[q] = r0 + q < E.R ? np[r0 + q] : 0.0;
+#pragma unroll
+            for (int q = 0; q < 16; q++)
+                if (r0 + q < E.R) n += v[q];
+        }
+    }
+    __syncthreads();
+    if (idx < total) {
+        const Ens &E = a.ens[e];
+        const bool win = e - e_lo < 2;
+        const double *wdt = s_dt[win ? e - e_lo : 0];
+        const double *gdt = E.dt;
+        auto dt_at = [&](int l) { return win ? wdt[l] : gdt[l]; };
         const int lvl = s.lvl;
-        const double dt = E.dt[lvl];
+        const double dt = dt_at(lvl);
         bool alive = true;
         if (n > 1.0 + E.norm_tol) {
             // experiment.cpp:1247-1251 throws: record the first offender, stop it.
@@ -1030,13 +1129,13 @@ __global__ __launch_bounds__(256) void decide_kernel(RunArgs a)
                     s.cur ^= 1;
                 }
                 s.t += dt;
-                complete_step(s, lvl, E.dt, E.S);
+                complete_step(s, lvl, dt_at, E.S);
             } else if (crossed) {
                 s.lvl = lvl + 1;          // bisect from the pre-step state
             } else {
                 s.cur ^= 1;
                 s.t += dt;
-                complete_step(s, lvl, E.dt, E.S);
+                complete_step(s, lvl, dt_at, E.S);
             }
         }
         E.st[tr] = s;
@@ -1085,14 +1184,16 @@ __global__ __launch_bounds__(256) void decide_kernel(RunArgs a)
 // workgroup per queued jump (persistent):
 //   phi <- phi / ||phi||;  v = L phi;  phi <- v / sqrt(||v||^2)      (FP64)
 // L is tridiagonal: L_kk = Ldiag[k], L_{k+1,k} = -Loff[k], L_{k,k+1} = +Loff[k];
-// v_k sums its three terms in ascending column order.
+// v_k sums its three terms in ascending column order.  Thread t owns the k = t + 256 m
+// (m < PER, PER = ceil(d / 256) rounded up to 4 or 8) and issues every global load
+// of its elements -- state and operator entries -- before it uses the first, so a
+// jump waits for memory once.
 // ---------------------------------------------------------------------------
 constexpr int kJumpThreads = 256;
-constexpr int kJumpPer = kMaxDim / kJumpThreads;
 
 // LDS holds the normalised state: 2 x (jdim + 2) doubles, jdim = the largest d
 // of the table (dynamic, so small models keep more workgroups resident).
-template <typename T>
+template <typename T, int PER>
 __global__ __launch_bounds__(kJumpThreads) void jump_kernel(RunArgs a, int jdim)
 {
     extern __shared__ double jump_lds[];
@@ -1104,35 +1205,55 @@ __global__ __launch_bounds__(kJumpThreads) void jump_kernel(RunArgs a, int jdim)
         const JumpEnt je = a.jl[j];
         const Ens &E = a.ens[je.ens];
         const int d = E.d, dp = E.dp;
+        const double *Ldiag = E.Ldiag, *Loff = E.Loff;
         const double sn = sqrt(je.n);
         const T *x = reinterpret_cast<const T *>(E.slot[je.src]) + (size_t)je.tr * E.L;
         T *y = reinterpret_cast<T *>(E.slot[je.src ^ 1]) + (size_t)je.tr * E.L;
-        for (int k = tid; k < d; k += kJumpThreads) {
-            pre[k + 1] = (double)x[k] / sn;
-            pim[k + 1] = (double)x[dp + k] / sn;
+        T xr[PER], xi[PER];
+        double ld[PER], lm[PER], lp[PER];    // L_kk, Loff[k - 1], Loff[k]
+#pragma unroll
+        for (int m = 0; m < PER; m++) {
+            const int k = tid + m * kJumpThreads;
+            xr[m] = xi[m] = T(0);
+            ld[m] = lm[m] = lp[m] = 0.0;
+            if (k < d) {
+                xr[m] = x[k];
+                xi[m] = x[dp + k];
+                ld[m] = Ldiag[k];
+                if (k > 0) lm[m] = Loff[k - 1];
+                if (k + 1 < d) lp[m] = Loff[k];
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < PER; m++) {
+            const int k = tid + m * kJumpThreads;
+            if (k < d) {
+                pre[k + 1] = (double)xr[m] / sn;
+                pim[k + 1] = (double)xi[m] / sn;
+            }
         }
         if (tid == 0) {
             pre[0] = pim[0] = 0.0;
             pre[d + 1] = pim[d + 1] = 0.0;
         }
         __syncthreads();
-        double vre[kJumpPer], vim[kJumpPer];
+        double vre[PER], vim[PER];
         double p = 0.0;
 #pragma unroll
-        for (int m = 0; m < kJumpPer; m++) {
+        for (int m = 0; m < PER; m++) {
             const int k = tid + m * kJumpThreads;
             vre[m] = vim[m] = 0.0;
             if (k < d) {
                 double rr = 0.0, ii = 0.0;
                 if (k > 0) {
-                    const double l = -E.Loff[k - 1];
+                    const double l = -lm[m];
                     rr += l * pre[k];
                     ii += l * pim[k];
                 }
-                rr += E.Ldiag[k] * pre[k + 1];
-                ii += E.Ldiag[k] * pim[k + 1];
+                rr += ld[m] * pre[k + 1];
+                ii += ld[m] * pim[k + 1];
                 if (k + 1 < d) {
-                    const double l = E.Loff[k];
+                    const double l = lp[m];
                     rr += l * pre[k + 2];
                     ii += l * pim[k + 2];
                 }
@@ -1149,7 +1270,7 @@ __global__ __launch_bounds__(kJumpThreads) void jump_kernel(RunArgs a, int jdim)
         for (int w = 0; w < kJumpThreads / 64; w++) tot += red[w];
         const double sp = sqrt(tot / 1.0);
 #pragma unroll
-        for (int m = 0; m < kJumpPer; m++) {
+        for (int m = 0; m < PER; m++) {
             const int k = tid + m * kJumpThreads;
             if (k < d) {
                 y[k] = (T)(vre[m] / sp);
@@ -1217,8 +1338,14 @@ void launch_tail(int prec, const RunArgs &a, const StepGrid &g, hipStream_t s)
 {
     decide_kernel<<<g.decide, 256, 0, s>>>(a);
     const size_t lds = 2 * (size_t)(g.jdim + 2) * sizeof(double);
-    if (prec == kFP64) jump_kernel<double><<<g.jump, kJumpThreads, lds, s>>>(a, g.jdim);
-    else jump_kernel<float><<<g.jump, kJumpThreads, lds, s>>>(a, g.jdim);
+    if (g.jdim <= 4 * kJumpThreads) {
+        if (prec == kFP64) jump_kernel<double, 4><<<g.jump, kJumpThreads, lds, s>>>(a, g.jdim);
+        else jump_kernel<float, 4><<<g.jump, kJumpThreads, lds, s>>>(a, g.jdim);
+    } else {
+        static_assert(kMaxDim <= 8 * kJumpThreads, "jump layout");
+        if (prec == kFP64) jump_kernel<double, 8><<<g.jump, kJumpThreads, lds, s>>>(a, g.jdim);
+        else jump_kernel<float, 8><<<g.jump, kJumpThreads, lds, s>>>(a, g.jdim);
+    }
 }
 
 int gemm_blocks_per_cu(int prec, int gemm_kind, int bn, int waves)
