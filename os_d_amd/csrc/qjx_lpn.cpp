@@ -86,10 +86,13 @@ struct qjx_lpn {
         return QJX_OK;
     }
 
-    int get(const double *src, double *dst, size_t n)
+    // device -> host in the context's stream order
+    template <typename T>
+    int get(const T *src, T *dst, size_t n)
     {
         if (!dst) return QJX_OK;
-        if (hipMemcpy(dst, src, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        if (hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
             return lpn_fail(QJX_ERR_HIP, "LPN readback failed");
         return QJX_OK;
     }
@@ -163,10 +166,12 @@ extern "C" int qjx_lpn_create(qjx_ctx *c, const qjx_lpn_desc *desc, qjx_lpn **ou
     uint64_t stride = 1;
     for (int tr = 0; tr < M; tr++) qjx::mcg31_stream(777, tr, M, &x0[tr], &stride);
     a.var_stride = stride;
-    if (hipMemcpy(Hd, c->Hdiag.data(), d * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(Ho, hoff.data(), d * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(Hv, c->Hdrv.data(), d * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(vx, x0.data(), M * sizeof(uint64_t), hipMemcpyHostToDevice) != hipSuccess)
+    // uploads in the context's stream order, after the buffers' clearing (the host
+    // vectors live until the sync below)
+    if (hipMemcpyAsync(Hd, c->Hdiag.data(), d * sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipMemcpyAsync(Ho, hoff.data(), d * sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipMemcpyAsync(Hv, c->Hdrv.data(), d * sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipMemcpyAsync(vx, x0.data(), M * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream) != hipSuccess)
         return bail(lpn_fail(QJX_ERR_HIP, "LPN upload failed"));
     a.Hdiag = Hd;
     a.Hoff = Ho;
@@ -227,11 +232,10 @@ extern "C" int qjx_lpn_run_periods(qjx_lpn *L, int n)
             ratio.resize(M);
             ds.resize(M);
             df.resize(M);
-            if (hipMemcpy(flag.data(), L->a.rec_flag, M * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
-                hipMemcpy(ratio.data(), L->a.rec_ratio, M * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
-                hipMemcpy(ds.data(), L->a.rec_ds, M * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
-                hipMemcpy(df.data(), L->a.rec_df, M * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
-                return lpn_fail(QJX_ERR_HIP, "LPN readback failed");
+            if ((rc = L->get<int>(L->a.rec_flag, flag.data(), M)) ||
+                (rc = L->get<double>(L->a.rec_ratio, ratio.data(), M)) ||
+                (rc = L->get<double>(L->a.rec_ds, ds.data(), M)) || (rc = L->get<double>(L->a.rec_df, df.data(), M)))
+                return rc;
             for (int tr = half; tr < M; tr++)
                 if (flag[tr]) {
                     L->lambdas[tr].push_back(ratio[tr]);
@@ -254,9 +258,7 @@ extern "C" int qjx_lpn_get(qjx_lpn *L, double *lambda_now, double *lambda, int *
         (rc = L->get(L->a.mean_lpn, mean_lpn, M)) || (rc = L->get(L->a.energy_lpn, energy_lpn, M)) ||
         (rc = L->get(L->a.delta_s, delta_s, M)))
         return rc;
-    if (num_renorms && hipMemcpy(num_renorms, L->a.num_renorms, M * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
-        return lpn_fail(QJX_ERR_HIP, "LPN readback failed");
-    return QJX_OK;
+    return L->get<int>(L->a.num_renorms, num_renorms, M);
 }
 
 extern "C" int qjx_lpn_random_obs(qjx_lpn *L, double *obs, double *obs_lpn)

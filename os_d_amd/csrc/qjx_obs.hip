@@ -190,7 +190,7 @@ template <typename T>
 __global__ __launch_bounds__(64) void robs_pn_kernel(const T *slot0, const T *slot1, const TrajState *st, int tr0,
                                                      int n, int d, int dp, int L, double2 *pn)
 {
-    __shared__ double inv_n;
+    __shared__ double s_nrm;   // ||phi||
     const int c = blockIdx.x;
     const int tr = tr0 + c;
     const T *col = (st[tr].cur ? slot1 : slot0) + (size_t)tr * L;
@@ -200,10 +200,10 @@ __global__ __launch_bounds__(64) void robs_pn_kernel(const T *slot0, const T *sl
             const double2 v = state_at(col, dp, k);
             s += v.x * v.x + v.y * v.y;
         }
-        inv_n = sqrt(s);
+        s_nrm = sqrt(s);
     }
     __syncthreads();
-    const double nn = inv_n;
+    const double nn = s_nrm;
     for (int k = threadIdx.x; k < d; k += 64) {
         const double2 v = state_at(col, dp, k);
         pn[(size_t)k * n + c] = make_double2(v.x / nn, v.y / nn);
