@@ -14,9 +14,10 @@ headline   : one transient period (FP32, from e_0) gives the common starting sta
              captured iteration graph, as the product runs.  Per-kernel figures
              come from a separate pass with HIP events on every launch (20 FP32
              steps: the roofline; 4 steps for the other forms).
-roofline   : the FP32 step GEMM (dominant kernel); algorithmic flops = the K blocks
-             inside the band range of each 32-row fragment of the propagators
-             (2 x 32 x 32 per column, fragment and block; the rest of the dense
+roofline   : the FP32 step GEMM (dominant kernel); algorithmic flops = the matrix-core
+             flops it executes: 3 products x 2 x 32 x 16 per (column, 32-row
+             fragment, K block of 16 complex columns) inside the fragments' band
+             ranges (Gauss's three-product complex multiply; the rest of the dense
              8 d^2 per application are exact zeros that are skipped) per launch of
              the event pass, over its average launch duration.
 cpu_baseline: the FP64 CPU oracle (OpenMP over trajectories, one dense zgemv per
@@ -238,8 +239,7 @@ def gemm_kernel_name(prec, bn, waves):
     """Name of the step GEMM kernel the launch shape selects (as a kernel trace shows it)."""
     if prec == 0:
         return "qjx::gemm_f64<%d>" % bn
-    name = "qjx::gemm_f32" if prec == 1 else "qjx::gemm_bf16x3"
-    return "%s<%d, %d>" % (name, bn, 4 if (bn == 256 or waves == 8) else 2)
+    return "qjx::gemm_cp<%s, %d, %d>" % ("true" if prec == 2 else "false", bn, 4 if (bn == 256 or waves == 8) else 2)
 
 
 def observables(e, N, M):
@@ -769,8 +769,9 @@ def main():
             "kernel": "%s (grouped step GEMM, gathered columns)" % r32["gemm_kernel"],
             "launches": launches, "avg_launch_ms": avg_ms, "flops_per_launch": flops_per_launch,
             "columns_per_launch": cols_per_launch,
-            "flops_note": "2 x 32 x 32 per (column, 32-row fragment, K block) inside the fragments' band ranges; "
-                          "launches of the %d-step HIP-event pass after the timed period" % args.roofline_steps,
+            "flops_note": "3 products x 2 x 32 x 16 per (column, 32-row fragment, K block of 16 complex columns) "
+                          "inside the fragments' band ranges (three-product complex multiply); launches of the "
+                          "%d-step HIP-event pass after the timed period" % args.roofline_steps,
             "dense_equivalent_flops_per_launch": 8.0 * (N + 1) ** 2 * cols_per_launch,
         }
         line["fp32"] = r32

@@ -26,8 +26,10 @@ struct qjx_ctx {
     std::vector<double> Hqj[2];    // dense interleaved complex d x d
 
     // device model
-    void *props = nullptr;         // 2*D * L*L elements
+    void *props = nullptr;         // 2*D * prop_elems elements
     void *props_lo = nullptr;      // BF16x3: low halves
+    bool cp = false;               // complex-pair layout (dp x L per propagator), else block-real (L x L)
+    size_t prop_elems = 0;         // elements per propagator
     int *krange = nullptr;         // [G][R][2] K-block ranges of the propagators' row tiles
     unsigned *kfrag = nullptr;     // [G][R][2] the same per 32-row fragment (one byte each)
     bool dense = false;            // QJX_DENSE=1: every K block (no band table)

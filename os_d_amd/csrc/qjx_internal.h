@@ -6,9 +6,12 @@
 //     tile multiple; the padding stays zero).  TrajState.cur says which slot holds the
 //     current state; a propagation step reads that slot and writes the other one, so
 //     accepting a step is a bit flip and rejecting it (bisection descent) costs nothing.
-//   * propagators: 2 branches x D levels of U = exp(-i H_eff,b dt_i) in real block form
-//     [[Re U, -Im U], [Im U, Re U]] (2dp x 2dp), row-major, so a row segment of U and a
-//     column segment of the state are both contiguous.
+//   * propagators: 2 branches x D levels of U = exp(-i H_eff,b dt_i), row-major, so a
+//     row segment of U and a column segment of the state are both contiguous.  The
+//     FP64 form (and the vector-FMA GEMM) stores the real block form
+//     [[Re U, -Im U], [Im U, Re U]] (2dp x 2dp); the FP32 and 3xBF16 forms on the
+//     matrix cores store complex-pair rows [Re U_i,: | Im U_i,:] (dp x 2dp), which hold
+//     every entry once (complex-pair GEMM tiles, qjx_step.hip).
 //   * per-trajectory scalar state (TrajState), group lists for the grouped GEMM and
 //     per-row-tile partial norms.
 #pragma once
@@ -50,15 +53,21 @@ constexpr int kMaxDim = 2048;                        // largest Hilbert dimensio
 constexpr int kMaxGroups = 64;                       // 2 branches x up to 32 levels
 constexpr int kMaxEns = 64;                          // ensembles sharing one iteration
 constexpr int kPlanItems = kMaxEns * kMaxGroups;     // (ensemble, group) items
+// Complex-pair GEMM tiles: 64 complex rows (Re and Im outputs: 128 real rows, as a
+// block-real tile) and K blocks of 16 complex columns; band tables of this layout
+// count K blocks of 16 and two 32-row fragments per tile.
+constexpr int kCpRows = 64;
+constexpr int kCpBlock = 16;
 
 // Device-resident descriptor of one ensemble (one dimer model and its
 // trajectories).  A table of these drives every step kernel, so ensembles of
 // different Hilbert dimension share each launch.
 struct Ens {
-    const void *props;        // 2*D matrices, each L*L elements (L = 2*dp), row-major
+    const void *props;        // 2*D matrices, row-major: L*L elements (L = 2*dp), or dp*L complex-pair
     const void *props_lo;     // BF16x3 only: low halves
-    const int *krange;        // [G][R][2]: K blocks (of 32) [lo, hi) holding every nonzero of
-                              // row tile rt within each half of the operand (exact zeros skipped)
+    const int *krange;        // [G][R][2]: K blocks (of 32; of kCpBlock complex columns in the
+                              // complex-pair layout) [lo, hi) holding every nonzero of row tile
+                              // rt within each half of the operand (exact zeros skipped)
     const unsigned *kfrag;    // [G][R][2]: the same for each 32-row fragment f of the tile,
                               // byte f of word 0 (lo) and of word 1 (hi)
     void *slot[2];            // state slots, M*L elements each
@@ -71,8 +80,8 @@ struct Ens {
     const double *Loff;       // [d-1] L_{k+1,k} = -Loff[k], L_{k,k+1} = +Loff[k]
     JumpLog log;
     int *err;                 // [4]: code, trajectory, bits of the offending norm
-    unsigned long long *counters;  // [0] matvecs, [1] jumps, [2] GEMM work in (column x K block
-                                   // of one 32-row fragment) units
+    unsigned long long *counters;  // [0] matvecs, [1] jumps, [2] matrix-core flops the GEMM
+                                   // executes, in units of 1024
     uint64_t stride;          // MCG59 leap-frog multiplier a^mns
     double norm_tol;          // allowed excess of ||psi||^2 over 1 (0 for FP64)
     int d, dp, L, D, S, M, G, R;
@@ -105,6 +114,7 @@ struct RunArgs {
     int parity;               // lists read this iteration: list[parity]
     int bn;                   // GEMM column tile
     int sc;                   // column tiles per super-column (tile order, L2 reuse of U)
+    int rc;                   // row tiles per GEMM work unit (complex-pair GEMM; 1 otherwise)
 };
 
 }  // namespace qjx

@@ -11,6 +11,10 @@ constexpr int kGemmFma = 0;      // LDS-tiled vector FMA, 64 x 64 tiles
 constexpr int kGemmMfma = 1;     // matrix cores, 128 x 128 tiles
 // rows of the block-real state per GEMM row tile
 inline int gemm_bm(int kind) { return kind == kGemmMfma ? 128 : 64; }
+// Whether the propagators are stored as complex-pair rows [Re U | Im U] (dp x 2dp)
+// for the complex-pair GEMM (the FP32 and 3xBF16 forms on the matrix cores); the
+// FP64 form and the vector-FMA GEMM use the block-real operand.
+bool cp_layout(int prec, int gemm_kind);
 
 // Per-trajectory initial state: psi = e_{start_state} (start_type 0) or the
 // uniform vector (otherwise); RNG state x_{seed+tr} of the MCG59 leap-frog stream.
@@ -75,7 +79,9 @@ void launch_density(int prec, const void *slot0, const void *slot1, const TrajSt
 // K-block ranges [lo, hi) per 128-row tile of a packed operand (out: [L/128][2]) and
 // per 32-row fragment (frag: [L/128][2], byte f of word 0 / 1 = fragment f's lo /
 // hi); dense = every block.
-void launch_band(int prec, const void *P, const void *Plo, int L, int dp, bool dense, int *out,
+// Complex-pair layout (cp): the same per 64-row tile of P (dp x L) in K blocks of 16
+// complex columns, two 32-row fragments per tile (bytes 0 and 1; out: [dp/64][2]).
+void launch_band(int prec, bool cp, const void *P, const void *Plo, int L, int dp, bool dense, int *out,
                  unsigned *frag, hipStream_t s);
 
 // Dense complex FP64 helpers for the propagator build (row-major n x n, interleaved).
@@ -84,8 +90,9 @@ void launch_zgemm(int n, const double *A, const double *B, double *C, hipStream_
 void launch_zaxpby_eye(int n, const double *A, double alpha, double beta, double *C, hipStream_t s);
 // C += alpha * A
 void launch_zaxpy(int n, const double *A, double alpha, double *C, hipStream_t s);
-// Write U (n x n complex FP64) as the block-real k-major 2dp x 2dp operand.
-void launch_pack_propagator(int prec, int n, int dp, const double *U, void *dst, void *dst_lo,
+// Write U (n x n complex FP64) as the block-real 2dp x 2dp operand, or (cp) as
+// complex-pair rows [Re U_i,: | Im U_i,:] (dp x 2dp).
+void launch_pack_propagator(int prec, bool cp, int n, int dp, const double *U, void *dst, void *dst_lo,
                             hipStream_t s);
 
 }  // namespace qjx

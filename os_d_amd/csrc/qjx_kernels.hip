@@ -402,10 +402,39 @@ __global__ void pack_propagator_bf16x3_kernel(int n, int dp, const double *U, __
     }
 }
 
-void launch_pack_propagator(int prec, int n, int dp, const double *U, void *dst, void *dst_lo,
+// Complex-pair rows: element (i, k) of the dp x 2dp operand [Re U | Im U] at
+// i * 2dp + k, zero beyond the n x n block; the 3 x BF16 form splits each FP32-rounded
+// element into hi = bf16(u) and lo = bf16(u - hi).
+template <bool SPLIT>
+__global__ void pack_propagator_cp_kernel(int n, int dp, const double *U, void *dst, void *dst_lo)
+{
+    const int L = 2 * dp;
+    const size_t total = (size_t)dp * L;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(e / L), k = (int)(e % L);
+        const int kk = k < dp ? k : k - dp;
+        double v = 0.0;
+        if (i < n && kk < n) v = U[2 * ((size_t)i * n + kk) + (k < dp ? 0 : 1)];
+        const float x = (float)v;
+        if (SPLIT) {
+            const __bf16 h = (__bf16)x;
+            static_cast<__bf16 *>(dst)[e] = h;
+            static_cast<__bf16 *>(dst_lo)[e] = (__bf16)(x - (float)h);
+        } else {
+            static_cast<float *>(dst)[e] = x;
+        }
+    }
+}
+
+bool cp_layout(int prec, int gemm_kind) { return prec != kFP64 && gemm_kind == kGemmMfma; }
+
+void launch_pack_propagator(int prec, bool cp, int n, int dp, const double *U, void *dst, void *dst_lo,
                             hipStream_t s)
 {
-    if (prec == kFP64) pack_propagator_kernel<double><<<2048, 256, 0, s>>>(n, dp, U, (double *)dst);
+    if (cp && prec == kBF16x3) pack_propagator_cp_kernel<true><<<2048, 256, 0, s>>>(n, dp, U, dst, dst_lo);
+    else if (cp) pack_propagator_cp_kernel<false><<<2048, 256, 0, s>>>(n, dp, U, dst, nullptr);
+    else if (prec == kFP64) pack_propagator_kernel<double><<<2048, 256, 0, s>>>(n, dp, U, (double *)dst);
     else if (prec == kBF16x3)
         pack_propagator_bf16x3_kernel<<<2048, 256, 0, s>>>(n, dp, U, (__bf16 *)dst, (__bf16 *)dst_lo);
     else pack_propagator_kernel<float><<<2048, 256, 0, s>>>(n, dp, U, (float *)dst);
@@ -633,10 +662,82 @@ __global__ void band_dense_kernel(int R, int dp, int *out, unsigned *frag)
     frag[2 * rt + 1] = (unsigned)(dp / 32) * 0x01010101u;
 }
 
-void launch_band(int prec, const void *P, const void *Plo, int L, int dp, bool dense, int *out, unsigned *frag,
-                 hipStream_t s)
+// Complex-pair layout: tile rt holds complex rows [64 rt, 64 rt + 64) of P (dp x L),
+// fragment f its rows 64 rt + 32 f ..; K blocks count kCpBlock complex columns, and a
+// column's Re and Im entries (k and dp + k) fall in the same block.
+template <typename T>
+__global__ __launch_bounds__(256) void band_cp_kernel(const T *P, const T *Plo, int dp, int *out, unsigned *frag)
+{
+    __shared__ int s_lo[2], s_hi[2];
+    if (threadIdx.x < 2) {
+        s_lo[threadIdx.x] = dp;
+        s_hi[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    const int L = 2 * dp, r0 = blockIdx.x * kCpRows;
+    for (int f = 0; f < 2; f++) {
+        int lo = dp, hi = 0;
+        for (int e = threadIdx.x; e < 32 * L; e += blockDim.x) {
+            const int r = r0 + 32 * f + e / L, k = e % L;
+            if (nonzero_at(P, Plo, (size_t)r * L + k)) {
+                const int kk = k < dp ? k : k - dp;
+                lo = min(lo, kk);
+                hi = max(hi, kk + 1);
+            }
+        }
+        atomicMin(&s_lo[f], lo);
+        atomicMax(&s_hi[f], hi);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int lo = dp, hi = 0;
+        for (int f = 0; f < 2; f++) {
+            if (s_hi[f] > s_lo[f]) {
+                lo = min(lo, s_lo[f]);
+                hi = max(hi, s_hi[f]);
+            }
+        }
+        // an all-zero row tile keeps one block (of exact zeros, with empty fragment
+        // ranges), so that every row tile of a work unit has a block to end on
+        const bool any = hi > lo;
+        const int kb0 = any ? lo / kCpBlock : 0, kb1 = any ? (hi + kCpBlock - 1) / kCpBlock : 1;
+        out[2 * blockIdx.x] = kb0;
+        out[2 * blockIdx.x + 1] = kb1;
+        unsigned w0 = 0, w1 = 0;
+        for (int f = 0; f < 2; f++) {
+            const bool fa = s_hi[f] > s_lo[f];        // an all-zero fragment gets an empty range
+            const unsigned b0 = fa ? (unsigned)(s_lo[f] / kCpBlock) : (unsigned)kb0;
+            const unsigned b1 = fa ? (unsigned)((s_hi[f] + kCpBlock - 1) / kCpBlock) : (unsigned)kb0;
+            w0 |= b0 << (8 * f);
+            w1 |= b1 << (8 * f);
+        }
+        frag[2 * blockIdx.x] = w0;
+        frag[2 * blockIdx.x + 1] = w1;
+    }
+}
+
+__global__ void band_dense_cp_kernel(int R, int dp, int *out, unsigned *frag)
+{
+    const int rt = blockIdx.x * blockDim.x + threadIdx.x;
+    if (rt >= R) return;
+    out[2 * rt] = 0;
+    out[2 * rt + 1] = dp / kCpBlock;
+    frag[2 * rt] = 0;
+    frag[2 * rt + 1] = (unsigned)(dp / kCpBlock) * 0x0101u;
+}
+
+void launch_band(int prec, bool cp, const void *P, const void *Plo, int L, int dp, bool dense, int *out,
+                 unsigned *frag, hipStream_t s)
 {
     const int R = L / 128;
+    if (cp) {
+        const int Rc = dp / kCpRows;
+        if (dense) band_dense_cp_kernel<<<(Rc + 63) / 64, 64, 0, s>>>(Rc, dp, out, frag);
+        else if (prec == kBF16x3)
+            band_cp_kernel<__bf16><<<Rc, 256, 0, s>>>((const __bf16 *)P, (const __bf16 *)Plo, dp, out, frag);
+        else band_cp_kernel<float><<<Rc, 256, 0, s>>>((const float *)P, nullptr, dp, out, frag);
+        return;
+    }
     if (dense) band_dense_kernel<<<(R + 63) / 64, 64, 0, s>>>(R, dp, out, frag);
     else if (prec == kFP64) band_kernel<double><<<R, 256, 0, s>>>((const double *)P, nullptr, L, dp, out, frag);
     else if (prec == kBF16x3)

@@ -25,6 +25,7 @@ struct Runner {
     int bn = 128;                     // GEMM column tile width (64 or 128)
     int wg_per_cu = 1;                // persistent GEMM workgroups per compute unit
     int sc = 1;                       // column tiles per super-column of the tile order
+    int rc = 1;                       // row tiles per GEMM work unit (complex-pair GEMM)
     bool autotune = true;             // pick (bn, wg_per_cu) by timing the live table
     bool tuned = false;
     int nens = 0;

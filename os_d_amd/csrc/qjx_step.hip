@@ -94,7 +94,7 @@ __global__ __launch_bounds__(kPlanThreads) void plan_kernel(RunArgs a)
             const Ens &E = a.ens[e];
             const int c = g < E.G ? E.cnt[p][g] : 0;
             el[q] = c;
-            tl[q] = ((c + a.bn - 1) / a.bn) * E.R;
+            tl[q] = ((c + a.bn - 1) / a.bn) * ((E.R + a.rc - 1) / a.rc);   // work units: column tiles x row chunks
             E.cnt[p ^ 1][g] = 0;      // the decide kernel of this iteration appends here
         }
         tsum += tl[q];
@@ -161,8 +161,8 @@ __device__ inline int tile_claim(const RunArgs &a)
 }
 
 // Wave 0: the tile of claim c0 (lane 0's tile_claim result) -> s_claim[0..4] =
-// ensemble, group, row tile, first column, columns; s_claim[0] = -1 when no tile is
-// left.
+// ensemble, group, row tile (row chunk of a.rc row tiles), first column, columns;
+// s_claim[0] = -1 when no tile is left.
 template <int BN>
 __device__ inline void tile_resolve(const RunArgs &a, const int *pre, int nitems, int total, int c0, int *s_claim)
 {
@@ -202,11 +202,12 @@ __device__ inline void tile_resolve(const RunArgs &a, const int *pre, int nitems
     const int e = item / kMaxGroups, g = item % kMaxGroups;
     const Ens &E = a.ens[e];
     // tiles of an item: super-columns of a.sc column tiles, each visited row tile
-    // by row tile, so the row tile's propagator band is reused from L2 by the
-    // super-column's column tiles and the column tiles' states by the row tiles
+    // by row tile (row chunk by row chunk: a.rc row tiles per unit), so the row
+    // tile's propagator band is reused from L2 by the super-column's column tiles
+    // and the column tiles' states by the row tiles
     const int cnt = gld<int>(E.cnt[a.parity] + g);
     const int nct = (cnt + BN - 1) / BN;
-    const int per = E.R * a.sc;
+    const int per = ((E.R + a.rc - 1) / a.rc) * a.sc;
     const int sc = local / per, rem = local - sc * per;
     const int cw = min(a.sc, nct - sc * a.sc);
     const int col0 = (sc * a.sc + rem % cw) * BN;
@@ -275,18 +276,15 @@ __device__ inline void load_cols(const RunArgs &a, const TileRef &r, int *s_tr, 
 }
 
 // ---------------------------------------------------------------------------
-// GEMM tile, FP32: v_mfma_f32_32x32x2f32
-//   A (32 x 2): lane l holds A[l % 32][l / 32]; B (2 x 32): B[l / 32][l % 32]
-//   D (32 x 32): lane l, register r holds D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32]
-// 2 x NWN waves, wave tile 64 x BN/NWN, WG tile 128 x BN (BN = 64 or 128 with 4
-// waves, 256 with 8; picked by the launch-shape tuner), K block 32; within a K block, MFMA step s (0..15)
-// takes k = s + 16 h for lane half h, so each lane reads its 16 fragment values per
-// block with four 16-byte LDS reads.  The k order is the same for every column and
-// the row split over waves does not depend on BN, so results do not depend on
-// where a column lands or on the tile width.
+// Matrix-core operand layouts (checked by the probe kernel below):
+//   v_mfma_f32_32x32x2f32: A (32 x 2): lane l holds A[l % 32][l / 32]; B (2 x 32):
+//     B[l / 32][l % 32]; D (32 x 32): lane l, register r holds
+//     D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32]
+//   v_mfma_f32_32x32x16_bf16: lane l holds 8 consecutive k of row / column l % 32
+//     (lane half h = l / 32 selects which 8); D as above
+// Block-real tiles (FP64 form): 128 rows of the 2dp x 2dp operand, K blocks of 32.
 // ---------------------------------------------------------------------------
 constexpr int MF_BM = 128, MF_BK = 32;
-constexpr int F32_PITCH = MF_BK + 4;
 
 // The K blocks a tile multiplies: [kb0, kb0 + nb) of the operand's first half,
 // then the same blocks of its second half (offset dp).  The blocks outside hold
@@ -306,17 +304,6 @@ __device__ inline KRange tile_krange(const Ens &E, const TileRef &r)
     k.kb0 = gld<int>(p);
     k.nb = gld<int>(p + 1) - k.kb0;
     k.hb = E.dp / MF_BK;
-    return k;
-}
-
-// the same, held in scalar registers: every lane of the workgroup reads the same
-// table entry, so the K loop's bounds and block offsets are wave-uniform
-__device__ inline KRange tile_krange_uniform(const Ens &E, const TileRef &r)
-{
-    KRange k = tile_krange(E, r);
-    k.kb0 = __builtin_amdgcn_readfirstlane(k.kb0);
-    k.nb = __builtin_amdgcn_readfirstlane(k.nb);
-    k.hb = __builtin_amdgcn_readfirstlane(k.hb);
     return k;
 }
 
@@ -348,189 +335,9 @@ __device__ inline FragRange tile_frags(const Ens &E, const TileRef &r)
     return f;
 }
 
-template <bool V>
-using Use = std::integral_constant<bool, V>;
-
 #if !defined(QJX_STEP_F64)
-template <int BN, int NWN>
-__global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) void gemm_f32(RunArgs a)
-{
-    constexpr int T = 128 * NWN;         // 2 x NWN waves
-    constexpr int NJ = BN / (32 * NWN);  // 32-column MFMA tiles per wave
-    constexpr int QA = MF_BM / (T / 8);  // A-tile rows copied per thread
-    constexpr int NQ = BN / (T / 8);     // B-tile rows (columns of X) copied per thread
-    static_assert(NJ >= 1 && QA >= 1 && NQ >= 1, "tile shape");
-    __shared__ int s_tr[BN];
-    __shared__ int s_cur[BN];
-    __shared__ __attribute__((aligned(16))) float As[2][MF_BM * F32_PITCH];   // double-buffered
-    __shared__ __attribute__((aligned(16))) float Bs[2][BN * F32_PITCH];
-    __shared__ double s_norm[2][BN];
-    const int *s_tpre = a.plan->tile_pre;     // read through L2: one search per tile
-    const int nitems = *a.nens * kMaxGroups;
-    const int total = s_tpre[nitems];
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int wm = wid / NWN, wn = wid % NWN;
-    const int h = lane >> 5, li = lane & 31;
-    const int cp_k = (tid & 7) * 4, cp_r = tid >> 3;
-    __shared__ int s_claim[5];
-    TileRef r;
-    if (wid == 0) tile_resolve<BN>(a, s_tpre, nitems, total, tile_claim(a), s_claim);
-    __syncthreads();
-    while (tile_read(s_claim, r)) {
-        load_cols<BN>(a, r, s_tr, s_cur);
-        __syncthreads();
-        const Ens &E = a.ens[r.e];
-        const Slots sl = tile_slots(E);
-        const int L = E.L, r0 = r.rt * MF_BM;
-        const float *U = reinterpret_cast<const float *>(E.props) + (size_t)r.g * L * L;
-        const float *arow = U + (size_t)(r0 + cp_r) * L + cp_k;
-        const size_t astep = (size_t)(T / 8) * L;
-        const float *bcol[NQ];
-#pragma unroll
-        for (int q = 0; q < NQ; q++) {
-            const int c = cp_r + (T / 8) * q;
-            bcol[q] = sl.col<const float>(s_cur[c], s_tr[c], L) + cp_k;
-        }
-        floatx16 acc[2][NJ];
-#pragma unroll
-        for (int i = 0; i < 2; i++)
-#pragma unroll
-            for (int j = 0; j < NJ; j++)
-#pragma unroll
-                for (int e = 0; e < 16; e++) acc[i][j][e] = 0.f;
-        // two register stages, as in the 3xBF16 kernel: block kb is multiplied out of
-        // LDS while block kb + 1 waits in registers and block kb + 2 is in flight
-        struct Stage {
-            floatx4 a[QA], b[NQ];
-        } st0, st1;
-        auto gload = [&](Stage &sg, int k0) {
-#pragma unroll
-            for (int q = 0; q < QA; q++) sg.a[q] = gld<floatx4>(arow + q * astep + k0);
-#pragma unroll
-            for (int q = 0; q < NQ; q++) sg.b[q] = gld<floatx4>(bcol[q] + k0);
-        };
-        auto lstore = [&](const Stage &sg, int buf) {
-#pragma unroll
-            for (int q = 0; q < QA; q++)
-                *reinterpret_cast<floatx4 *>(&As[buf][(cp_r + (T / 8) * q) * F32_PITCH + cp_k]) = sg.a[q];
-#pragma unroll
-            for (int q = 0; q < NQ; q++)
-                *reinterpret_cast<floatx4 *>(&Bs[buf][(cp_r + (T / 8) * q) * F32_PITCH + cp_k]) = sg.b[q];
-        };
-        // fragments in four k slices of 4 MFMA steps each (steps 4v..4v+3 read one
-        // 16-byte LDS word per operand and lane): fewer live registers, same order.
-        // U0 / U1: whether the wave's fragment i = 0 / 1 multiplies this block.
-        auto mma_sel = [&](int buf, auto U0, auto U1) {
-            constexpr bool u[2] = {decltype(U0)::value, decltype(U1)::value};
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                floatx4 fa[2], fb[NJ];
-#pragma unroll
-                for (int i = 0; i < 2; i++)
-                    if (u[i])
-                        fa[i] = *reinterpret_cast<const floatx4 *>(
-                            &As[buf][(wm * 64 + i * 32 + li) * F32_PITCH + 16 * h + 4 * v]);
-#pragma unroll
-                for (int j = 0; j < NJ; j++)
-                    fb[j] = *reinterpret_cast<const floatx4 *>(
-                        &Bs[buf][(wn * (BN / NWN) + j * 32 + li) * F32_PITCH + 16 * h + 4 * v]);
-#pragma unroll
-                for (int e = 0; e < 4; e++)
-#pragma unroll
-                    for (int i = 0; i < 2; i++)
-                        if (u[i])
-#pragma unroll
-                            for (int j = 0; j < NJ; j++)
-                                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[i][e], fb[j][e], acc[i][j], 0,
-                                                                                  0, 0);
-            }
-        };
-        const KRange kr = tile_krange_uniform(E, r);
-        const int nk = kr.count();
-        const FragRange fr = tile_frags(E, r);
-        auto mma = [&](int buf, int kb) {
-            const int b = kr.half_block(kb);
-            const bool a0 = fr.active(2 * wm, b), a1 = fr.active(2 * wm + 1, b);
-            if (a0 && a1) mma_sel(buf, Use<true>(), Use<true>());
-            else if (a0) mma_sel(buf, Use<true>(), Use<false>());
-            else if (a1) mma_sel(buf, Use<false>(), Use<true>());
-        };
-        if (nk > 0) {
-            // refills past the last block reload the last block (in bounds, never
-            // stored), so every iteration issues the same loads and the wait before
-            // each stage's LDS store covers only that stage
-            auto blk = [&](int kb) { return kr.at(min(kb, nk - 1)); };
-            // issued stage by stage (not interleaved), as the loop body expects them
-            gload(st0, blk(0));
-            __builtin_amdgcn_sched_barrier(0);
-            gload(st1, blk(1));
-            __builtin_amdgcn_sched_barrier(0);
-            lstore(st0, 0);
-            gload(st0, blk(2));
-            __syncthreads();
-            auto iter = [&](Stage &nxt, int kb) {
-                const int buf = kb & 1;
-                __builtin_amdgcn_sched_barrier(0);
-                lstore(nxt, buf ^ 1);
-                gload(nxt, blk(kb + 3));
-                mma(buf, kb);
-                __syncthreads();
-            };
-            int kb = 0;
-            for (; kb + 1 < nk; kb += 2) {
-                iter(st1, kb);
-                iter(st0, kb + 1);
-            }
-            if (kb < nk) iter(st1, kb);
-        }
-        // the next tile is claimed now and resolved after this tile's stores
-        int cn = 0;
-        if (wid == 0) cn = tile_claim(a);
-        double ns[NJ];
-#pragma unroll
-        for (int j = 0; j < NJ; j++) {
-            ns[j] = 0.0;
-            const int c = wn * (BN / NWN) + j * 32 + li;
-            const bool ok = c < r.ncol;
-            float *y = sl.col<float>(s_cur[c] ^ 1, s_tr[c], L);
-#pragma unroll
-            for (int i = 0; i < 2; i++)
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int row = r0 + wm * 64 + i * 32 + 8 * q + 4 * h;
-                    floatx4 v = {acc[i][j][4 * q], acc[i][j][4 * q + 1], acc[i][j][4 * q + 2], acc[i][j][4 * q + 3]};
-                    if (ok) gst<floatx4>(y + row, v);
-#pragma unroll
-                    for (int e = 0; e < 4; e++) ns[j] += (double)v[e] * (double)v[e];
-                }
-        }
-        if (wid == 0) tile_resolve<BN>(a, s_tpre, nitems, total, cn, s_claim);
-#pragma unroll
-        for (int j = 0; j < NJ; j++) ns[j] += __shfl_xor(ns[j], 32, 64);
-        if (h == 0) {
-#pragma unroll
-            for (int j = 0; j < NJ; j++) s_norm[wm][wn * (BN / NWN) + j * 32 + li] = ns[j];
-        }
-        __syncthreads();
-        if (tid < r.ncol) gst<double>(E.normpart + (size_t)s_tr[tid] * E.R + r.rt, s_norm[0][tid] + s_norm[1][tid]);
-        if (tid == 0) atomicAdd(&E.counters[2], (unsigned long long)r.ncol * fr.work());
-        __syncthreads();
-    }
-}
-
-// ---------------------------------------------------------------------------
-// GEMM tile, 3 x BF16 split product, FP32 accumulate: x = hi + lo (hi = bf16(x),
-// lo = bf16(x - hi)) for both operands, U X ~= Uhi Xhi + Uhi Xlo + Ulo Xhi.
-// v_mfma_f32_32x32x16_bf16: lane l holds 8 consecutive k of row / column l % 32
-// (lane half h = l / 32 selects which 8); D as in the FP32 form.  The propagator
-// is split when packed, the state tile when staged into LDS.  LDS rows are 64 bytes
-// (32 bf16) with the 16-byte chunk index XOR-swizzled by (row / 4) % 4, which keeps
-// the fragment reads conflict-free; tiles are double-buffered.  2 x NWN waves per
-// workgroup (NWN = 2 or 4: 4 waves of 64 x BN/2 or 8 of 64 x BN/4; BN = 256 takes 8
-// waves of 64 x 64): with short band K ranges the fixed cost of a tile (claim,
-// lookup, column gather, first loads, epilogue) weighs, and a 256-column tile
-// halves it per column.
-// ---------------------------------------------------------------------------
+// 3 x BF16 LDS rows: 32 bf16 (64 bytes) with the 16-byte chunk index XOR-swizzled by
+// (row / 4) % 4, which keeps the fragment reads conflict-free.
 constexpr int BF_ROW = MF_BK;          // bf16 per LDS row, no padding
 
 __device__ inline int bf_off(int row, int k)   // element offset of (row, k), k < 32
@@ -539,29 +346,94 @@ __device__ inline int bf_off(int row, int k)   // element offset of (row, k), k 
     return row * BF_ROW + chunk * 8 + (k & 7);
 }
 
-template <int BN, int NWN>
-__global__ __launch_bounds__(128 * NWN) void gemm_bf16x3(RunArgs a)
+
+// ---------------------------------------------------------------------------
+// Complex-pair GEMM (FP32 and 3xBF16 forms).  These forms store a propagator as
+// dp rows [Re U_i,0 .. Re U_i,dp-1 | Im U_i,0 .. Im U_i,dp-1] (row-major dp x 2dp).
+// The block-real operand holds every entry of U twice and one of its tiles yields
+// either Re y or Im y; a complex-pair row tile is kCpRows = 64 complex rows and
+// writes both Re y and Im y of its rows, so each entry of U is staged once per
+// column tile and its K range is that of 64 rows instead of 128.  A K block is
+// kCpBlock = 16 complex columns; an LDS row holds [Re 0..15 | Im 0..15] of one
+// propagator row or one state column.  Wave (wm, wn) of the 2 x NWN waves takes the
+// row tile's complex rows 32 wm .. 32 wm + 31 (band fragment wm) and BN / NWN
+// columns.  The FP64 partial norm of a row tile covers Re and Im of its 64 rows; R =
+// dp / 64 equals the block-real tile count 2 dp / 128, so the decide kernel is shared.
+//
+// Work unit: one column tile and a chunk of a.rc consecutive row tiles, walked as
+// one stream of K blocks.  The register stages and LDS buffers run on across the
+// row tiles' boundaries, so the next row tile's first blocks are already loaded
+// while the current one finishes, and the column gather, the claim and the first
+// loads are paid once per chunk instead of once per row tile.  At a row tile's last
+// block its outputs and norm partials are written and the accumulators restart.
+//
+// FP32 (BF = false): Gauss's three-product complex multiply on
+// v_mfma_f32_32x32x2f32.  With U = A + iB and x = a + ib:
+//   k1 = (A + B) a,   k2 = A (b - a),   k3 = B (a + b),
+//   Re y = k1 - k3,   Im y = k1 + k2,
+// three real products per complex product instead of four; A + B, b - a and a + b
+// are formed in registers from the fragments read out of LDS.  MFMA step s (0..7)
+// of a K block takes k = s + 8 h for lane half h, so a lane reads its 8 values of
+// each operand part with two 16-byte LDS reads (rows padded to CP_PITCH floats).
+//
+// 3xBF16 (BF = true): split products, x = hi + lo (hi = bf16(x), lo = bf16(x - hi))
+// for both operands and U X ~= Ulo Xhi + Uhi Xlo + Uhi Xhi (the propagator is split
+// when packed, the state tile when staged into LDS), FP32 accumulation, with four
+// real products per complex product,
+// Re y = A a - B b and Im y = B a + A b; -B is B with its sign bits flipped in
+// registers.  Each accumulator takes its four small cross terms before its two
+// hi x hi terms.  v_mfma_f32_32x32x16_bf16 takes one part (Re or Im: 16 k) of a K
+// block, lane half h holding k = 8 h .. 8 h + 7; LDS rows are 64 bytes
+// (Re 16 | Im 16 bf16), XOR-swizzled (bf_off).
+//
+// The k order, and the row split over waves, do not depend on BN or on the chunk,
+// so results do not depend on where a column lands, on the tile width or on rc.
+// ---------------------------------------------------------------------------
+constexpr int CP_PITCH = 2 * kCpBlock + 4;   // floats per FP32 LDS row (conflict-free 16-byte reads)
+constexpr int kCpChunkMax = 8;               // row tiles per work unit at most (bytes of a 64-bit word)
+
+typedef unsigned uintx4 __attribute__((ext_vector_type(4)));
+
+// -x of a bf16 vector: the sign bits flipped (exact)
+__device__ inline bf16x8 bf16_neg(bf16x8 v)
 {
-    constexpr int T = 128 * NWN;         // 2 x NWN waves
-    constexpr int NJ = BN / (32 * NWN);  // 32-column MFMA tiles per wave
-    constexpr int QA = MF_BM / (T / 4);  // A-tile rows copied per thread
-    constexpr int NQ = BN / (T / 8);     // B-tile rows (columns of X) copied per thread
-    static_assert(NJ >= 1 && QA >= 1 && NQ >= 1, "tile shape");
+    uintx4 u = __builtin_bit_cast(uintx4, v);
+    u ^= 0x80008000u;
+    return __builtin_bit_cast(bf16x8, u);
+}
+
+__device__ inline int byte_at(uint64_t w, int i) { return (int)((w >> (8 * i)) & 0xffu); }
+
+template <bool BF, int BN, int NWN>
+__global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) void gemm_cp(RunArgs a)
+{
+    constexpr int T = 128 * NWN;             // 2 x NWN waves
+    constexpr int NJ = BN / (32 * NWN);      // 32-column MFMA tiles per wave
+    constexpr int RP = T / 8;                // LDS rows filled per pass (8 threads per row)
+    constexpr int NQ = BN / RP;              // state columns copied per thread
+    // propagator loads per thread: FP32 rows of 8 x 16 bytes; BF16 16-byte chunks of
+    // the hi and lo arrays (64 rows x 4 chunks each)
+    constexpr int NA = BF ? 2 * kCpRows * 4 / T : kCpRows / RP;
+    constexpr int NACC = BF ? 2 : 3;         // accumulator sets per column tile
+    static_assert(NJ >= 1 && NA >= 1 && NQ >= 1, "tile shape");
+    static_assert(!BF || NA * T == 2 * kCpRows * 4, "tile shape");
+    using AV = std::conditional_t<BF, bf16x8, floatx4>;
+    // LDS: two buffers of the propagator tile and the state tile
+    constexpr int A_ELEMS = BF ? 2 * kCpRows * BF_ROW : kCpRows * CP_PITCH;   // per buffer (hi, lo)
+    constexpr int B_ELEMS = BF ? 2 * BN * BF_ROW : BN * CP_PITCH;
+    using LT = std::conditional_t<BF, __bf16, float>;
+    __shared__ __attribute__((aligned(16))) LT s_a[2][A_ELEMS];
+    __shared__ __attribute__((aligned(16))) LT s_b[2][B_ELEMS];
     __shared__ int s_tr[BN];
     __shared__ int s_cur[BN];
-    __shared__ __attribute__((aligned(16))) __bf16 Ah[2][MF_BM * BF_ROW];
-    __shared__ __attribute__((aligned(16))) __bf16 Al[2][MF_BM * BF_ROW];
-    __shared__ __attribute__((aligned(16))) __bf16 Bh[2][BN * BF_ROW];
-    __shared__ __attribute__((aligned(16))) __bf16 Bl[2][BN * BF_ROW];
-    __shared__ double s_norm[2][BN];
-    const int *s_tpre = a.plan->tile_pre;
+    __shared__ double s_norm[2][2][BN];       // [row tile parity][wm][column]
+    const int *s_tpre = a.plan->tile_pre;     // read through L2: one search per unit
     const int nitems = *a.nens * kMaxGroups;
     const int total = s_tpre[nitems];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wm = wid / NWN, wn = wid % NWN;
     const int h = lane >> 5, li = lane & 31;
-    const int ca_k = (tid & 3) * 8, ca_r = tid >> 2;
-    const int cb_k = (tid & 7) * 4, cb_r = tid >> 3;
+    const int cc = tid & 7, cr = tid >> 3;    // 16-byte chunk of a state LDS row; row
     __shared__ int s_claim[5];
     TileRef r;
     if (wid == 0) tile_resolve<BN>(a, s_tpre, nitems, total, tile_claim(a), s_claim);
@@ -571,175 +443,299 @@ __global__ __launch_bounds__(128 * NWN) void gemm_bf16x3(RunArgs a)
         __syncthreads();
         const Ens &E = a.ens[r.e];
         const Slots sl = tile_slots(E);
-        const int L = E.L, r0 = r.rt * MF_BM;
-        const size_t LL = (size_t)L * L;
-        const __bf16 *Uh = reinterpret_cast<const __bf16 *>(E.props) + (size_t)r.g * LL;
-        const __bf16 *Ul = reinterpret_cast<const __bf16 *>(E.props_lo) + (size_t)r.g * LL;
+        const int L = E.L, dp = E.dp, R = E.R;
+        const int rt0 = r.rt * a.rc;                 // first row tile of the unit
+        const int nrt = min(a.rc, R - rt0);          // row tiles in the unit
+        // band of each row tile of the unit, one byte per row tile (wave-uniform):
+        // first K block, block count, and the two fragments' ranges
+        uint64_t bk0 = 0, bnb = 0, bf0 = 0, bf1 = 0, bh0 = 0, bh1 = 0;
+        unsigned work = 0;
+#pragma unroll
+        for (int i = 0; i < kCpChunkMax; i++) {
+            if (i < nrt) {
+                const size_t o = 2 * ((size_t)r.g * R + rt0 + i);
+                const int k0 = __builtin_amdgcn_readfirstlane(gld<int>(E.krange + o));
+                const int k1 = __builtin_amdgcn_readfirstlane(gld<int>(E.krange + o + 1));
+                const unsigned lo = __builtin_amdgcn_readfirstlane(gld<unsigned>(E.kfrag + o));
+                const unsigned hi = __builtin_amdgcn_readfirstlane(gld<unsigned>(E.kfrag + o + 1));
+                bk0 |= (uint64_t)k0 << (8 * i);
+                bnb |= (uint64_t)(k1 - k0) << (8 * i);
+                bf0 |= (uint64_t)(lo & 0xffu) << (8 * i);
+                bf1 |= (uint64_t)((lo >> 8) & 0xffu) << (8 * i);
+                bh0 |= (uint64_t)(hi & 0xffu) << (8 * i);
+                bh1 |= (uint64_t)((hi >> 8) & 0xffu) << (8 * i);
+                work += ((hi & 0xffu) - (lo & 0xffu)) + (((hi >> 8) & 0xffu) - ((lo >> 8) & 0xffu));
+            }
+        }
+        int nbt = 0;                                 // K blocks of the unit
+#pragma unroll
+        for (int i = 0; i < kCpChunkMax; i++)
+            if (i < nrt) nbt += byte_at(bnb, i);
+        // this thread's load addresses at the unit's first row tile, block offset 0
+        const int koff = cc < 4 ? 4 * cc : dp + 4 * (cc - 4);   // Re part (chunks 0..3) or Im part
+        const LT *aptr[NA];
+        int aoff[NA];
+        {
+            const size_t PL = (size_t)dp * L;
+            const LT *U = reinterpret_cast<const LT *>(E.props) + (size_t)r.g * PL;
+#pragma unroll
+            for (int q = 0; q < NA; q++) {
+                if (BF) {
+                    // load index e = tid + T q: array e / 256 (hi, lo; wave-uniform), row
+                    // (e % 256) / 4, 8-element chunk e % 4 (0, 1: Re part; 2, 3: Im part)
+                    const LT *Ul = reinterpret_cast<const LT *>(E.props_lo) + (size_t)r.g * PL;
+                    const int e = tid + T * q, row = (e & 255) >> 2, ch = e & 3;
+                    const int gk = ch < 2 ? 8 * ch : dp + 8 * (ch - 2);
+                    aptr[q] = ((e >> 8) ? Ul : U) + (size_t)(rt0 * kCpRows + row) * L + gk;
+                    aoff[q] = (e >> 8) * kCpRows * BF_ROW + bf_off(row, 8 * ch);
+                } else {
+                    const int row = cr + RP * q;
+                    aptr[q] = U + (size_t)(rt0 * kCpRows + row) * L + koff;
+                    aoff[q] = row * CP_PITCH + 4 * cc;
+                }
+            }
+        }
         const float *bcol[NQ];
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
-            const int c = cb_r + (T / 8) * q;
-            bcol[q] = sl.col<const float>(s_cur[c], s_tr[c], L) + cb_k;
+            const int c = cr + RP * q;
+            bcol[q] = sl.col<const float>(s_cur[c], s_tr[c], L) + koff;
         }
-        floatx16 acc[2][NJ];
+        floatx16 acc[NACC][NJ];
+        auto zero_acc = [&]() {
 #pragma unroll
-        for (int i = 0; i < 2; i++)
+            for (int p = 0; p < NACC; p++)
 #pragma unroll
-            for (int j = 0; j < NJ; j++)
+                for (int j = 0; j < NJ; j++)
 #pragma unroll
-                for (int e = 0; e < 16; e++) acc[i][j][e] = 0.f;
-        // two register stages: while block kb is multiplied out of LDS, block kb + 1
-        // waits in registers for the other LDS buffer and block kb + 2 is in flight
+                    for (int e = 0; e < 16; e++) acc[p][j][e] = 0.f;
+        };
+        zero_acc();
         struct Stage {
-            bf16x8 ah[QA], al[QA];
+            AV a[NA];
             floatx4 b[NQ];
         } st0, st1;
-        auto gload = [&](Stage &sg, int k0) {
+        // load cursor (wave-uniform): row tile lrt of the unit, block lkb of its band;
+        // past the unit's last block it stays there (reloads are in bounds, never stored)
+        int lrt = 0, lkb = 0;
+        auto gload = [&](Stage &sg) {
+            const int k0 = (byte_at(bk0, lrt) + lkb) * kCpBlock;
+            const size_t ao = (size_t)lrt * kCpRows * L + k0;
 #pragma unroll
-            for (int q = 0; q < QA; q++) {
-                const size_t o = (size_t)(r0 + ca_r + (T / 4) * q) * L + k0 + ca_k;
-                sg.ah[q] = gld<bf16x8>(Uh + o);
-                sg.al[q] = gld<bf16x8>(Ul + o);
-            }
+            for (int q = 0; q < NA; q++) sg.a[q] = gld<AV>(aptr[q] + ao);
 #pragma unroll
             for (int q = 0; q < NQ; q++) sg.b[q] = gld<floatx4>(bcol[q] + k0);
+            if (lkb + 1 < byte_at(bnb, lrt)) {
+                lkb++;
+            } else if (lrt + 1 < nrt) {
+                lrt++;
+                lkb = 0;
+            }
         };
         auto lstore = [&](const Stage &sg, int buf) {
 #pragma unroll
-            for (int q = 0; q < QA; q++) {
-                const int rr = ca_r + (T / 4) * q;
-                *reinterpret_cast<bf16x8 *>(&Ah[buf][bf_off(rr, ca_k)]) = sg.ah[q];
-                *reinterpret_cast<bf16x8 *>(&Al[buf][bf_off(rr, ca_k)]) = sg.al[q];
-            }
+            for (int q = 0; q < NA; q++) *reinterpret_cast<AV *>(&s_a[buf][aoff[q]]) = sg.a[q];
 #pragma unroll
             for (int q = 0; q < NQ; q++) {
-                const int rr = cb_r + (T / 8) * q;
-                bf16x4 h4, l4;
+                const int rr = cr + RP * q;
+                if (BF) {
+                    bf16x4 h4, l4;
 #pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    const float x = sg.b[q][e];
-                    const __bf16 hb = (__bf16)x;
-                    h4[e] = hb;
-                    l4[e] = (__bf16)(x - (float)hb);
+                    for (int e = 0; e < 4; e++) {
+                        const float x = sg.b[q][e];
+                        const __bf16 hb = (__bf16)x;
+                        h4[e] = hb;
+                        l4[e] = (__bf16)(x - (float)hb);
+                    }
+                    __bf16 *bb = reinterpret_cast<__bf16 *>(s_b[buf]);
+                    *reinterpret_cast<bf16x4 *>(&bb[bf_off(rr, 4 * cc)]) = h4;
+                    *reinterpret_cast<bf16x4 *>(&bb[BN * BF_ROW + bf_off(rr, 4 * cc)]) = l4;
+                } else {
+                    *reinterpret_cast<floatx4 *>(reinterpret_cast<float *>(s_b[buf]) + rr * CP_PITCH + 4 * cc) =
+                        sg.b[q];
                 }
-                *reinterpret_cast<bf16x4 *>(&Bh[buf][bf_off(rr, cb_k)]) = h4;
-                *reinterpret_cast<bf16x4 *>(&Bl[buf][bf_off(rr, cb_k)]) = l4;
             }
         };
-        // U0 / U1: whether the wave's fragment i = 0 / 1 multiplies this block
-        auto mma_sel = [&](int buf, auto U0, auto U1) {
-            constexpr bool u[2] = {decltype(U0)::value, decltype(U1)::value};
+        auto mma_block = [&](int buf) {
+            if constexpr (BF) {
+                const __bf16 *sa = reinterpret_cast<const __bf16 *>(s_a[buf]);
+                const __bf16 *sb = reinterpret_cast<const __bf16 *>(s_b[buf]);
+                const int ra = wm * 32 + li;
+                const int oR = bf_off(ra, 8 * h), oI = bf_off(ra, kCpBlock + 8 * h);
+                const bf16x8 rh = *reinterpret_cast<const bf16x8 *>(&sa[oR]);
+                const bf16x8 rl = *reinterpret_cast<const bf16x8 *>(&sa[kCpRows * BF_ROW + oR]);
+                const bf16x8 ih = *reinterpret_cast<const bf16x8 *>(&sa[oI]);
+                const bf16x8 il = *reinterpret_cast<const bf16x8 *>(&sa[kCpRows * BF_ROW + oI]);
+                bf16x8 ah[NJ], al[NJ], bh[NJ], bl[NJ];
 #pragma unroll
-            for (int c = 0; c < MF_BK / 16; c++) {
-                bf16x8 fah[2], fal[2], fbh[NJ], fbl[NJ];
+                for (int j = 0; j < NJ; j++) {
+                    const int col = wn * (BN / NWN) + j * 32 + li;
+                    const int oa = bf_off(col, 8 * h), ob = bf_off(col, kCpBlock + 8 * h);
+                    ah[j] = *reinterpret_cast<const bf16x8 *>(&sb[oa]);
+                    al[j] = *reinterpret_cast<const bf16x8 *>(&sb[BN * BF_ROW + oa]);
+                    bh[j] = *reinterpret_cast<const bf16x8 *>(&sb[ob]);
+                    bl[j] = *reinterpret_cast<const bf16x8 *>(&sb[BN * BF_ROW + ob]);
+                }
+                const bf16x8 nih = bf16_neg(ih), nil = bf16_neg(il);
+                // the Re and Im accumulators of each column tile go round in turn, so
+                // consecutive MFMAs never depend on each other
 #pragma unroll
-                for (int i = 0; i < 2; i++) {
-                    if (!u[i]) continue;
-                    const int oa = bf_off(wm * 64 + i * 32 + li, 16 * c + 8 * h);
-                    fah[i] = *reinterpret_cast<const bf16x8 *>(&Ah[buf][oa]);
-                    fal[i] = *reinterpret_cast<const bf16x8 *>(&Al[buf][oa]);
+                for (int j = 0; j < NJ; j++) {
+                    acc[0][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rl, ah[j], acc[0][j], 0, 0, 0);
+                    acc[1][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(il, ah[j], acc[1][j], 0, 0, 0);
                 }
 #pragma unroll
                 for (int j = 0; j < NJ; j++) {
-                    const int ob = bf_off(wn * (BN / NWN) + j * 32 + li, 16 * c + 8 * h);
-                    fbh[j] = *reinterpret_cast<const bf16x8 *>(&Bh[buf][ob]);
-                    fbl[j] = *reinterpret_cast<const bf16x8 *>(&Bl[buf][ob]);
+                    acc[0][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh, al[j], acc[0][j], 0, 0, 0);
+                    acc[1][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ih, al[j], acc[1][j], 0, 0, 0);
                 }
-                // the three products go round the accumulators in turn, so
-                // consecutive MFMAs never depend on each other
 #pragma unroll
-                for (int i = 0; i < 2; i++)
-                    if (u[i])
+                for (int j = 0; j < NJ; j++) {
+                    acc[0][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nil, bh[j], acc[0][j], 0, 0, 0);
+                    acc[1][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rl, bh[j], acc[1][j], 0, 0, 0);
+                }
 #pragma unroll
-                        for (int j = 0; j < NJ; j++)
-                            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fal[i], fbh[j], acc[i][j], 0, 0, 0);
+                for (int j = 0; j < NJ; j++) {
+                    acc[0][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nih, bl[j], acc[0][j], 0, 0, 0);
+                    acc[1][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh, bl[j], acc[1][j], 0, 0, 0);
+                }
 #pragma unroll
-                for (int i = 0; i < 2; i++)
-                    if (u[i])
+                for (int j = 0; j < NJ; j++) {
+                    acc[0][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh, ah[j], acc[0][j], 0, 0, 0);
+                    acc[1][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ih, ah[j], acc[1][j], 0, 0, 0);
+                }
 #pragma unroll
-                        for (int j = 0; j < NJ; j++)
-                            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fah[i], fbl[j], acc[i][j], 0, 0, 0);
+                for (int j = 0; j < NJ; j++) {
+                    acc[0][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nih, bh[j], acc[0][j], 0, 0, 0);
+                    acc[1][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh, bh[j], acc[1][j], 0, 0, 0);
+                }
+            } else {
+                const float *sa = reinterpret_cast<const float *>(s_a[buf]);
+                const float *sb = reinterpret_cast<const float *>(s_b[buf]);
 #pragma unroll
-                for (int i = 0; i < 2; i++)
-                    if (u[i])
+                for (int v = 0; v < 2; v++) {
+                    const float *pa = &sa[(wm * 32 + li) * CP_PITCH + 8 * h + 4 * v];
+                    const floatx4 fre = *reinterpret_cast<const floatx4 *>(pa);
+                    const floatx4 fim = *reinterpret_cast<const floatx4 *>(pa + kCpBlock);
+                    floatx4 xa[NJ], xd[NJ], xs[NJ];
 #pragma unroll
-                        for (int j = 0; j < NJ; j++)
-                            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fah[i], fbh[j], acc[i][j], 0, 0, 0);
+                    for (int j = 0; j < NJ; j++) {
+                        const float *pb = &sb[(wn * (BN / NWN) + j * 32 + li) * CP_PITCH + 8 * h + 4 * v];
+                        xa[j] = *reinterpret_cast<const floatx4 *>(pb);
+                        const floatx4 xb = *reinterpret_cast<const floatx4 *>(pb + kCpBlock);
+                        xd[j] = xb - xa[j];
+                        xs[j] = xa[j] + xb;
+                    }
+                    const floatx4 fs = fre + fim;
+#pragma unroll
+                    for (int e = 0; e < 4; e++)
+#pragma unroll
+                        for (int j = 0; j < NJ; j++) {
+                            acc[0][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fs[e], xa[j][e], acc[0][j], 0, 0, 0);
+                            acc[1][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fre[e], xd[j][e], acc[1][j], 0, 0, 0);
+                            acc[2][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fim[e], xs[j][e], acc[2][j], 0, 0, 0);
+                        }
+                }
             }
         };
-        const KRange kr = tile_krange_uniform(E, r);
-        const int nk = kr.count();
-        const FragRange fr = tile_frags(E, r);
-        auto mma = [&](int buf, int kb) {
-            const int b = kr.half_block(kb);
-            const bool a0 = fr.active(2 * wm, b), a1 = fr.active(2 * wm + 1, b);
-            if (a0 && a1) mma_sel(buf, Use<true>(), Use<true>());
-            else if (a0) mma_sel(buf, Use<true>(), Use<false>());
-            else if (a1) mma_sel(buf, Use<false>(), Use<true>());
-        };
-        if (nk > 0) {
-            // refills past the last block reload the last block (in bounds, never
-            // stored), so every iteration issues the same loads and the waits before
-            // each stage's LDS store cover only that stage
-            auto blk = [&](int kb) { return kr.at(min(kb, nk - 1)); };
-            // issued stage by stage (not interleaved), as the loop body expects them
-            gload(st0, blk(0));
-            __builtin_amdgcn_sched_barrier(0);
-            gload(st1, blk(1));
-            __builtin_amdgcn_sched_barrier(0);
-            lstore(st0, 0);
-            gload(st0, blk(2));
-            __syncthreads();
-            // iteration kb: store block kb + 1 (its LDS buffer was last read before the
-            // previous barrier), refill its registers with block kb + 3, multiply block kb
-            auto iter = [&](Stage &nxt, int kb) {
-                const int buf = kb & 1;
-                // keep the next stage's conversions out of this iteration: hoisted,
-                // they would wait for loads that are meant to stay in flight
-                __builtin_amdgcn_sched_barrier(0);
-                lstore(nxt, buf ^ 1);
-                gload(nxt, blk(kb + 3));
-                mma(buf, kb);
-                __syncthreads();
-            };
-            int kb = 0;
-            for (; kb + 1 < nk; kb += 2) {
-                iter(st1, kb);
-                iter(st0, kb + 1);
-            }
-            if (kb < nk) iter(st1, kb);
-        }
-        // the next tile is claimed now and resolved after this tile's stores
-        int cn = 0;
-        if (wid == 0) cn = tile_claim(a);
-        double ns[NJ];
+        // outputs and norm partial of row tile i of the unit (after its last block)
+        auto epilogue = [&](int i) {
+            const int r0 = (rt0 + i) * kCpRows;
+            double ns[NJ];
 #pragma unroll
-        for (int j = 0; j < NJ; j++) {
-            ns[j] = 0.0;
-            const int c = wn * (BN / NWN) + j * 32 + li;
-            const bool ok = c < r.ncol;
-            float *y = sl.col<float>(s_cur[c] ^ 1, s_tr[c], L);
-#pragma unroll
-            for (int i = 0; i < 2; i++)
+            for (int j = 0; j < NJ; j++) {
+                ns[j] = 0.0;
+                const int c = wn * (BN / NWN) + j * 32 + li;
+                const bool ok = c < r.ncol;
+                float *y = sl.col<float>(s_cur[c] ^ 1, s_tr[c], L);
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    const int row = r0 + wm * 64 + i * 32 + 8 * q + 4 * h;
-                    floatx4 v = {acc[i][j][4 * q], acc[i][j][4 * q + 1], acc[i][j][4 * q + 2], acc[i][j][4 * q + 3]};
-                    if (ok) gst<floatx4>(y + row, v);
+                    const int row = r0 + wm * 32 + 8 * q + 4 * h;
+                    floatx4 re, im;
 #pragma unroll
-                    for (int e = 0; e < 4; e++) ns[j] += (double)v[e] * (double)v[e];
+                    for (int e = 0; e < 4; e++) {
+                        if (BF) {
+                            re[e] = acc[0][j][4 * q + e];
+                            im[e] = acc[1][j][4 * q + e];
+                        } else {
+                            re[e] = acc[0][j][4 * q + e] - acc[NACC - 1][j][4 * q + e];
+                            im[e] = acc[0][j][4 * q + e] + acc[1][j][4 * q + e];
+                        }
+                    }
+                    if (ok) {
+                        gst<floatx4>(y + row, re);
+                        gst<floatx4>(y + dp + row, im);
+                    }
+#pragma unroll
+                    for (int e = 0; e < 4; e++) ns[j] += (double)re[e] * (double)re[e];
+#pragma unroll
+                    for (int e = 0; e < 4; e++) ns[j] += (double)im[e] * (double)im[e];
                 }
-        }
-        if (wid == 0) tile_resolve<BN>(a, s_tpre, nitems, total, cn, s_claim);
+            }
 #pragma unroll
-        for (int j = 0; j < NJ; j++) ns[j] += __shfl_xor(ns[j], 32, 64);
-        if (h == 0) {
+            for (int j = 0; j < NJ; j++) ns[j] += __shfl_xor(ns[j], 32, 64);
+            if (h == 0) {
 #pragma unroll
-            for (int j = 0; j < NJ; j++) s_norm[wm][wn * (BN / NWN) + j * 32 + li] = ns[j];
-        }
+                for (int j = 0; j < NJ; j++) s_norm[i & 1][wm][wn * (BN / NWN) + j * 32 + li] = ns[j];
+            }
+            zero_acc();
+        };
+        // the norm partial of row tile i, after a barrier that follows its epilogue
+        auto write_norm = [&](int i) {
+            if (tid < r.ncol)
+                gst<double>(E.normpart + (size_t)s_tr[tid] * R + rt0 + i, s_norm[i & 1][0][tid] + s_norm[i & 1][1][tid]);
+        };
+        // issued stage by stage (not interleaved), as the loop body expects them
+        gload(st0);
+        __builtin_amdgcn_sched_barrier(0);
+        gload(st1);
+        __builtin_amdgcn_sched_barrier(0);
+        lstore(st0, 0);
+        gload(st0);
         __syncthreads();
-        if (tid < r.ncol) gst<double>(E.normpart + (size_t)s_tr[tid] * E.R + r.rt, s_norm[0][tid] + s_norm[1][tid]);
-        if (tid == 0) atomicAdd(&E.counters[2], (unsigned long long)r.ncol * fr.work());
+        // MMA cursor (wave-uniform): row tile mrt, block mkb of its band; pend = row
+        // tile whose norm partial waits in s_norm
+        int mrt = 0, mkb = 0, pend = -1, cn = 0;
+        // iteration f: store block f + 1 (its LDS buffer was last read before the
+        // previous barrier), refill its registers with block f + 3, multiply block f
+        auto iter = [&](Stage &nxt, int f) {
+            const int buf = f & 1;
+            // the next unit is claimed during the last block and resolved after it
+            if (f == nbt - 1 && wid == 0) cn = tile_claim(a);
+            // keep the next stage's conversions out of this iteration: hoisted, they
+            // would wait for loads that are meant to stay in flight
+            __builtin_amdgcn_sched_barrier(0);
+            lstore(nxt, buf ^ 1);
+            gload(nxt);
+            const int b = byte_at(bk0, mrt) + mkb;
+            const int fl = wm ? byte_at(bf1, mrt) : byte_at(bf0, mrt);
+            const int fh = wm ? byte_at(bh1, mrt) : byte_at(bh0, mrt);
+            // the wave's fragment multiplies only the blocks inside its band range
+            if (b >= fl && b < fh) mma_block(buf);
+            __syncthreads();
+            if (pend >= 0) {
+                write_norm(pend);
+                pend = -1;
+            }
+            if (mkb + 1 == byte_at(bnb, mrt)) {
+                epilogue(mrt);
+                pend = mrt;
+                mrt++;
+                mkb = 0;
+            } else {
+                mkb++;
+            }
+        };
+        int f = 0;
+        for (; f + 1 < nbt; f += 2) {
+            iter(st1, f);
+            iter(st0, f + 1);
+        }
+        if (f < nbt) iter(st1, f);
+        if (wid == 0) tile_resolve<BN>(a, s_tpre, nitems, total, cn, s_claim);
+        __syncthreads();
+        write_norm(pend);
+        if (tid == 0) atomicAdd(&E.counters[2], (unsigned long long)(BF ? 4 : 3) * r.ncol * work);
         __syncthreads();
     }
 }
@@ -892,7 +888,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(BN == 64 ? 
         if (tid < r.ncol)
             gst<double>(E.normpart + (size_t)s_tr[tid] * E.R + r.rt,
                         ((s_norm[0][tid] + s_norm[1][tid]) + s_norm[2][tid]) + s_norm[3][tid]);
-        if (tid == 0) atomicAdd(&E.counters[2], (unsigned long long)r.ncol * fr.work());
+        if (tid == 0) atomicAdd(&E.counters[2], 2ull * r.ncol * fr.work());   // 2048 flops per unit of work()
         __syncthreads();
     }
 }
@@ -1323,14 +1319,14 @@ void launch_gemm(int prec, int gemm_kind, const RunArgs &a, const StepGrid &g, h
     } else if (prec == kFP64) {
         launch_gemm_f64(a, g, s);
     } else if (prec == kBF16x3) {
-        if (a.bn == 256) gemm_bf16x3<256, 4><<<g.gemm, 512, 0, s>>>(a);
-        else if (g.waves == 8) gemm_bf16x3<128, 4><<<g.gemm, 512, 0, s>>>(a);
-        else if (a.bn == 64) gemm_bf16x3<64, 2><<<g.gemm, 256, 0, s>>>(a);
-        else gemm_bf16x3<128, 2><<<g.gemm, 256, 0, s>>>(a);
+        if (a.bn == 256) gemm_cp<true, 256, 4><<<g.gemm, 512, 0, s>>>(a);
+        else if (g.waves == 8) gemm_cp<true, 128, 4><<<g.gemm, 512, 0, s>>>(a);
+        else if (a.bn == 64) gemm_cp<true, 64, 2><<<g.gemm, 256, 0, s>>>(a);
+        else gemm_cp<true, 128, 2><<<g.gemm, 256, 0, s>>>(a);
     } else {
-        if (a.bn == 256) gemm_f32<256, 4><<<g.gemm, 512, 0, s>>>(a);
-        else if (a.bn == 64) gemm_f32<64, 2><<<g.gemm, 256, 0, s>>>(a);
-        else gemm_f32<128, 2><<<g.gemm, 256, 0, s>>>(a);
+        if (a.bn == 256) gemm_cp<false, 256, 4><<<g.gemm, 512, 0, s>>>(a);
+        else if (a.bn == 64) gemm_cp<false, 64, 2><<<g.gemm, 256, 0, s>>>(a);
+        else gemm_cp<false, 128, 2><<<g.gemm, 256, 0, s>>>(a);
     }
 }
 
@@ -1359,14 +1355,14 @@ int gemm_blocks_per_cu(int prec, int gemm_kind, int bn, int waves)
     else if (prec == kFP64)
         e = gemm_f64_occupancy(&n, bn);
     else if (prec == kBF16x3)
-        e = bn == 256 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_bf16x3<256, 4>, 512, 0)
-            : waves == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_bf16x3<128, 4>, 512, 0)
-            : narrow   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_bf16x3<64, 2>, 256, 0)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_bf16x3<128, 2>, 256, 0);
+        e = bn == 256 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_cp<true, 256, 4>, 512, 0)
+            : waves == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_cp<true, 128, 4>, 512, 0)
+            : narrow   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_cp<true, 64, 2>, 256, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_cp<true, 128, 2>, 256, 0);
     else
-        e = bn == 256 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_f32<256, 4>, 512, 0)
-            : narrow  ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_f32<64, 2>, 256, 0)
-                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_f32<128, 2>, 256, 0);
+        e = bn == 256 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_cp<false, 256, 4>, 512, 0)
+            : narrow  ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_cp<false, 64, 2>, 256, 0)
+                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_cp<false, 128, 2>, 256, 0);
     return (e == hipSuccess && n > 0) ? n : 1;
 }
 
