@@ -1536,7 +1536,7 @@ int runner_autotune(Runner *r)
         const int occ = gemm_blocks_per_cu(r->prec, r->gemm_kind, 128, 8);
         cand.push_back({128, occ, 8, 0.0, 0});
     }
-    if (r->prec != kFP64) {                // 256-column tiles (8 waves) for the FP32 kernels
+    if (r->prec != kFP64) {                // 256-column tiles (8 waves) for the FP32 forms
         const int occ = gemm_blocks_per_cu(r->prec, r->gemm_kind, 256, 8);
         cand.push_back({256, occ, 8, 0.0, 0});
     }

@@ -411,12 +411,11 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
     constexpr int NJ = BN / (32 * NWN);      // 32-column MFMA tiles per wave
     constexpr int RP = T / 8;                // LDS rows filled per pass (8 threads per row)
     constexpr int NQ = BN / RP;              // state columns copied per thread
-    // propagator loads per thread: FP32 rows of 8 x 16 bytes; BF16 16-byte chunks of
-    // the hi and lo arrays (64 rows x 4 chunks each)
-    constexpr int NA = BF ? 2 * kCpRows * 4 / T : kCpRows / RP;
+    // propagator loads of a K block: 512 16-byte chunks (FP32: 64 rows x 8; BF16: 64
+    // rows x 4 of the hi and of the lo array), NA per thread
+    constexpr int NA = 2 * kCpRows * 4 / T;
     constexpr int NACC = BF ? 2 : 3;         // accumulator sets per column tile
-    static_assert(NJ >= 1 && NA >= 1 && NQ >= 1, "tile shape");
-    static_assert(!BF || NA * T == 2 * kCpRows * 4, "tile shape");
+    static_assert(NJ >= 1 && NA >= 1 && NQ >= 1 && NA * T == 2 * kCpRows * 4, "tile shape");
     using AV = std::conditional_t<BF, bf16x8, floatx4>;
     // LDS: two buffers of the propagator tile and the state tile
     constexpr int A_ELEMS = BF ? 2 * kCpRows * BF_ROW : kCpRows * CP_PITCH;   // per buffer (hi, lo)
@@ -696,22 +695,26 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
         // MMA cursor (wave-uniform): row tile mrt, block mkb of its band; pend = row
         // tile whose norm partial waits in s_norm
         int mrt = 0, mkb = 0, pend = -1, cn = 0;
-        // iteration f: store block f + 1 (its LDS buffer was last read before the
-        // previous barrier), refill its registers with block f + 3, multiply block f
+        // iteration f: multiply block f, then store block f + 1 into the other LDS
+        // buffer (last read before the previous barrier) and refill its registers with
+        // block f + 3.  Storing after the matrix products lets the LDS stores drain
+        // while the products run, and the next block's fragment reads then find no
+        // stores ahead of them (3xBF16 1.5% faster than storing first; FP32 the same)
         auto iter = [&](Stage &nxt, int f) {
             const int buf = f & 1;
             // the next unit is claimed during the last block and resolved after it
             if (f == nbt - 1 && wid == 0) cn = tile_claim(a);
-            // keep the next stage's conversions out of this iteration: hoisted, they
-            // would wait for loads that are meant to stay in flight
+            // keep the next stage's conversions out of this iteration's products:
+            // hoisted, they would wait for loads that are meant to stay in flight
             __builtin_amdgcn_sched_barrier(0);
-            lstore(nxt, buf ^ 1);
-            gload(nxt);
             const int b = byte_at(bk0, mrt) + mkb;
             const int fl = wm ? byte_at(bf1, mrt) : byte_at(bf0, mrt);
             const int fh = wm ? byte_at(bh1, mrt) : byte_at(bh0, mrt);
             // the wave's fragment multiplies only the blocks inside its band range
             if (b >= fl && b < fh) mma_block(buf);
+            __builtin_amdgcn_sched_barrier(0);
+            lstore(nxt, buf ^ 1);
+            gload(nxt);
             __syncthreads();
             if (pend >= 0) {
                 write_norm(pend);

@@ -95,7 +95,7 @@ def test_autotuner_picks_a_shape_with_identical_results(monkeypatch):
     e.run_steps(12, 0)
     s = e.stats()
     _say("tuned: bn %d grid %d" % (s["gemm_bn"], s["gemm_grid"]))
-    assert s["gemm_bn"] in (64, 128)
+    assert s["gemm_bn"] in (64, 128, 256)
     assert s["gemm_grid"] > 0 and s["gemm_grid"] % 8 == 0
     _same_state(e.get_state(), ref)
     e.close()
