@@ -201,6 +201,9 @@ int qjx_set_profiling(qjx_ctx *ctx, int on);
  * FP32 result), all row-major, computed by one wavefront. */
 int qjx_mfma_probe(const float *A32, const float *B32, float *D32, const double *A64,
                    const double *B64, double *D64, const float *Abf, const float *Bbf, float *Dbf);
+/* The same for the FP32 16x16x4 matrix-core product of the FP32 step GEMM (no
+ * reference counterpart): D = A B, A 16x4, B 4x16, D 16x16, row-major, FP32. */
+int qjx_mfma_probe_f32x16(const float *A, const float *B, float *D);
 /* Measured ceilings of the current device (no reference counterpart): FP32, FP64
  * and BF16 matrix-core rates from back-to-back independent MFMA chains on every
  * compute unit, the FP64 vector FMA rate, and the HBM bandwidth of a 2 GiB device

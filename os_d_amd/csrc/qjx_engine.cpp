@@ -211,7 +211,7 @@ static void band_of(qjx_ctx *c, int b, int lvl)
     const int g = b * c->D + lvl;
     qjx::launch_band(c->prec, c->cp, prop_ptr(c, b, lvl), prop_lo_ptr(c, b, lvl), c->L, c->dp,
                      c->dense || c->gemm_kind != qjx::kGemmMfma, c->krange + (size_t)2 * g * (c->L / 128),
-                     c->kfrag + (size_t)2 * g * (c->L / 128), c->stream);
+                     c->kfrag + (size_t)qjx::kFragWords * g * (c->L / 128), c->stream);
 }
 
 extern "C" const char *qjx_last_error(void) { return g_err.c_str(); }
@@ -303,7 +303,7 @@ int qjx::ctx_create_kind(const qjx_model_desc *m, int gemm_kind, qjx_ctx **out)
     if (c->prec == QJX_BF16X3 && hipMalloc(&c->props_lo, propbytes) != hipSuccess)
         return bail(fail(QJX_ERR_NOMEM, "propagator tree does not fit in device memory"));
     if (hipMalloc((void **)&c->krange, (size_t)2 * c->G * (c->L / 128) * sizeof(int)) != hipSuccess ||
-        hipMalloc((void **)&c->kfrag, (size_t)2 * c->G * (c->L / 128) * sizeof(unsigned)) != hipSuccess)
+        hipMalloc((void **)&c->kfrag, (size_t)qjx::kFragWords * c->G * (c->L / 128) * sizeof(unsigned)) != hipSuccess)
         return bail(fail(QJX_ERR_NOMEM, "band table does not fit in device memory"));
     if (const char *env = std::getenv("QJX_DENSE")) c->dense = std::string(env) == "1";
 
@@ -1022,7 +1022,7 @@ extern "C" int qjx_get_stats(qjx_ctx *c, qjx_stats *out)
     out->iterations = c->runner ? c->runner->iterations : 0;
     out->jumps = (int64_t)h[1];
     out->gemm_flops = 8.0 * double(c->d) * double(c->d) * double(out->matvecs);
-    out->gemm_band_flops = double(h[2]) * 1024.0;
+    out->gemm_band_flops = double(h[2]) * 128.0;
     out->gemm_ms = c->runner ? c->runner->gemm_ms : 0.0;
     out->decide_ms = c->runner ? c->runner->tail_ms + c->runner->plan_ms : 0.0;
     out->gemm_launches = c->runner ? c->runner->gemm_launches : 0;
@@ -1036,6 +1036,24 @@ extern "C" int qjx_get_stats(qjx_ctx *c, qjx_stats *out)
         out->gemm_ms = c->rk_ms;
         out->gemm_launches = c->rk_launches;
     }
+    return QJX_OK;
+}
+
+extern "C" int qjx_mfma_probe_f32x16(const float *A, const float *B, float *D)
+{
+    if (!A || !B || !D) return fail(QJX_ERR_ARG, "null argument");
+    float *buf = nullptr;   // A 64 | B 64 | D 256 floats
+    HIP_TRY(hipMalloc((void **)&buf, (64 + 64 + 256) * sizeof(float)));
+    hipError_t e = hipMemcpy(buf, A, 64 * sizeof(float), hipMemcpyHostToDevice);
+    e = e == hipSuccess ? hipMemcpy(buf + 64, B, 64 * sizeof(float), hipMemcpyHostToDevice) : e;
+    if (e == hipSuccess) {
+        qjx::launch_mfma_probe16(buf, buf + 64, buf + 128, nullptr);
+        e = hipGetLastError();
+    }
+    e = e == hipSuccess ? hipDeviceSynchronize() : e;
+    e = e == hipSuccess ? hipMemcpy(D, buf + 128, 256 * sizeof(float), hipMemcpyDeviceToHost) : e;
+    (void)hipFree(buf);
+    if (e != hipSuccess) return fail(QJX_ERR_HIP, hipGetErrorString(e));
     return QJX_OK;
 }
 
@@ -1185,7 +1203,7 @@ extern "C" int qjx_batch_get_stats(qjx_batch *b, qjx_stats *out)
         out->matvecs += (int64_t)h[0];
         out->jumps += (int64_t)h[1];
         out->gemm_flops += 8.0 * double(c->d) * double(c->d) * double(h[0]);
-        out->gemm_band_flops += double(h[2]) * 1024.0;
+        out->gemm_band_flops += double(h[2]) * 128.0;
     }
     const qjx::Runner *r = b->runner;
     out->iterations = r->iterations;

@@ -58,6 +58,13 @@ constexpr int kPlanItems = kMaxEns * kMaxGroups;     // (ensemble, group) items
 // count K blocks of 16 and two 32-row fragments per tile.
 constexpr int kCpRows = 64;
 constexpr int kCpBlock = 16;
+// Forms of the complex-pair GEMM: 3xBF16 (32-row fragments, K blocks of 16) and FP32
+// (16-row fragments, each with its K window in units of 4 complex columns)
+constexpr int kCpBF16x3 = 1, kCpF32x16 = 2;
+// band-table words per row tile (kfrag): block-real and 3xBF16 tables use two (one
+// byte per fragment in each), the FP32 complex-pair table four (one per 16-row
+// fragment: first | past-the-end unit of 4 complex columns << 16)
+constexpr int kFragWords = 4;
 
 // Device-resident descriptor of one ensemble (one dimer model and its
 // trajectories).  A table of these drives every step kernel, so ensembles of
@@ -68,8 +75,9 @@ struct Ens {
     const int *krange;        // [G][R][2]: K blocks (of 32; of kCpBlock complex columns in the
                               // complex-pair layout) [lo, hi) holding every nonzero of row tile
                               // rt within each half of the operand (exact zeros skipped)
-    const unsigned *kfrag;    // [G][R][2]: the same for each 32-row fragment f of the tile,
-                              // byte f of word 0 (lo) and of word 1 (hi)
+    const unsigned *kfrag;    // [G][R][kFragWords]: the same for each 32-row fragment f of the
+                              // tile, byte f of word 0 (lo) and of word 1 (hi); FP32 complex-pair:
+                              // word f = window of 16-row fragment f in units of 4 columns
     void *slot[2];            // state slots, M*L elements each
     TrajState *st;
     int *list[2];             // per parity: [G * M] entries (trajectory << 1) | current slot
@@ -81,7 +89,7 @@ struct Ens {
     JumpLog log;
     int *err;                 // [4]: code, trajectory, bits of the offending norm
     unsigned long long *counters;  // [0] matvecs, [1] jumps, [2] matrix-core flops the GEMM
-                                   // executes, in units of 1024
+                                   // executes, in units of 128
     uint64_t stride;          // MCG59 leap-frog multiplier a^mns
     double norm_tol;          // allowed excess of ||psi||^2 over 1 (0 for FP64)
     int d, dp, L, D, S, M, G, R;

@@ -51,6 +51,9 @@ void launch_mfma_probe(const float *A32, const float *B32, float *D32, const dou
                        const double *B64, double *D64, const float *Abf, const float *Bbf, float *Dbf,
                        hipStream_t s);
 
+// one wave: D = A (16x4) B (4x16) on v_mfma_f32_16x16x4f32, row-major
+void launch_mfma_probe16(const float *A, const float *B, float *D, hipStream_t s);
+
 // Observables of every trajectory (core.cpp:665-725): norm, mean, m2, energy;
 // dispersion against mean_start; optional |psi_k|^2 / norm per trajectory.
 void launch_chars(int prec, const void *slot0, const void *slot1, const TrajState *st, int M,
@@ -77,10 +80,13 @@ void launch_density(int prec, const void *slot0, const void *slot1, const TrajSt
                     hipStream_t s);
 
 // K-block ranges [lo, hi) per 128-row tile of a packed operand (out: [L/128][2]) and
-// per 32-row fragment (frag: [L/128][2], byte f of word 0 / 1 = fragment f's lo /
-// hi); dense = every block.
+// per 32-row fragment (frag: byte f of word 0 / 1 = fragment f's lo / hi); dense =
+// every block.
 // Complex-pair layout (cp): the same per 64-row tile of P (dp x L) in K blocks of 16
-// complex columns, two 32-row fragments per tile (bytes 0 and 1; out: [dp/64][2]).
+// complex columns (out: [dp/64][2]); 3xBF16: two 32-row fragments per tile (bytes 0
+// and 1 of frag words 0 and 1); FP32: four 16-row fragments, word f = window
+// [lo, hi) in units of 4 complex columns, lo | hi << 16.  frag has kFragWords words
+// per tile.
 void launch_band(int prec, bool cp, const void *P, const void *Plo, int L, int dp, bool dense, int *out,
                  unsigned *frag, hipStream_t s);
 

@@ -647,8 +647,8 @@ __global__ __launch_bounds__(256) void band_kernel(const T *P, const T *Plo, int
             w0 |= b0 << (8 * f);
             w1 |= b1 << (8 * f);
         }
-        frag[2 * blockIdx.x] = w0;
-        frag[2 * blockIdx.x + 1] = w1;
+        frag[kFragWords * blockIdx.x] = w0;
+        frag[kFragWords * blockIdx.x + 1] = w1;
     }
 }
 
@@ -658,11 +658,11 @@ __global__ void band_dense_kernel(int R, int dp, int *out, unsigned *frag)
     if (rt >= R) return;
     out[2 * rt] = 0;
     out[2 * rt + 1] = dp / 32;
-    frag[2 * rt] = 0;
-    frag[2 * rt + 1] = (unsigned)(dp / 32) * 0x01010101u;
+    frag[kFragWords * rt] = 0;
+    frag[kFragWords * rt + 1] = (unsigned)(dp / 32) * 0x01010101u;
 }
 
-// Complex-pair layout: tile rt holds complex rows [64 rt, 64 rt + 64) of P (dp x L),
+// Complex-pair layout, 3xBF16: tile rt holds complex rows [64 rt, 64 rt + 64) of P (dp x L),
 // fragment f its rows 64 rt + 32 f ..; K blocks count kCpBlock complex columns, and a
 // column's Re and Im entries (k and dp + k) fall in the same block.
 template <typename T>
@@ -711,8 +711,8 @@ __global__ __launch_bounds__(256) void band_cp_kernel(const T *P, const T *Plo, 
             w0 |= b0 << (8 * f);
             w1 |= b1 << (8 * f);
         }
-        frag[2 * blockIdx.x] = w0;
-        frag[2 * blockIdx.x + 1] = w1;
+        frag[kFragWords * blockIdx.x] = w0;
+        frag[kFragWords * blockIdx.x + 1] = w1;
     }
 }
 
@@ -722,20 +722,81 @@ __global__ void band_dense_cp_kernel(int R, int dp, int *out, unsigned *frag)
     if (rt >= R) return;
     out[2 * rt] = 0;
     out[2 * rt + 1] = dp / kCpBlock;
-    frag[2 * rt] = 0;
-    frag[2 * rt + 1] = (unsigned)(dp / kCpBlock) * 0x0101u;
+    frag[kFragWords * rt] = 0;
+    frag[kFragWords * rt + 1] = (unsigned)(dp / kCpBlock) * 0x0101u;
+}
+
+// FP32 complex-pair layout: the tile's K blocks as band_cp_kernel, and for each of
+// its four 16-row fragments the window [lo, hi) of its nonzero complex columns in
+// units of 4 (word f = lo | hi << 16; an all-zero fragment gets an empty window).
+__global__ __launch_bounds__(256) void band_cp16_kernel(const float *P, int dp, int *out, unsigned *frag)
+{
+    __shared__ int s_lo[4], s_hi[4];
+    if (threadIdx.x < 4) {
+        s_lo[threadIdx.x] = dp;
+        s_hi[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    const int L = 2 * dp, r0 = blockIdx.x * kCpRows;
+    for (int f = 0; f < 4; f++) {
+        int lo = dp, hi = 0;
+        for (int e = threadIdx.x; e < 16 * L; e += blockDim.x) {
+            const int r = r0 + 16 * f + e / L, k = e % L;
+            if (P[(size_t)r * L + k] != 0.f) {
+                const int kk = k < dp ? k : k - dp;
+                lo = min(lo, kk);
+                hi = max(hi, kk + 1);
+            }
+        }
+        atomicMin(&s_lo[f], lo);
+        atomicMax(&s_hi[f], hi);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int lo = dp, hi = 0;
+        for (int f = 0; f < 4; f++) {
+            if (s_hi[f] > s_lo[f]) {
+                lo = min(lo, s_lo[f]);
+                hi = max(hi, s_hi[f]);
+            }
+        }
+        // an all-zero row tile keeps one block (of exact zeros, with empty windows)
+        const bool any = hi > lo;
+        const int kb0 = any ? lo / kCpBlock : 0, kb1 = any ? (hi + kCpBlock - 1) / kCpBlock : 1;
+        out[2 * blockIdx.x] = kb0;
+        out[2 * blockIdx.x + 1] = kb1;
+        for (int f = 0; f < 4; f++) {
+            const bool fa = s_hi[f] > s_lo[f];
+            const unsigned w0 = fa ? (unsigned)(s_lo[f] / 4) : (unsigned)(4 * kb0);
+            const unsigned w1 = fa ? (unsigned)((s_hi[f] + 3) / 4) : (unsigned)(4 * kb0);
+            frag[kFragWords * blockIdx.x + f] = w0 | (w1 << 16);
+        }
+    }
+}
+
+__global__ void band_dense_cp16_kernel(int R, int dp, int *out, unsigned *frag)
+{
+    const int rt = blockIdx.x * blockDim.x + threadIdx.x;
+    if (rt >= R) return;
+    out[2 * rt] = 0;
+    out[2 * rt + 1] = dp / kCpBlock;
+    for (int f = 0; f < 4; f++) frag[kFragWords * rt + f] = (unsigned)(dp / 4) << 16;
 }
 
 void launch_band(int prec, bool cp, const void *P, const void *Plo, int L, int dp, bool dense, int *out,
                  unsigned *frag, hipStream_t s)
 {
+    if (cp && prec != kBF16x3) {
+        const int Rc = dp / kCpRows;
+        if (dense) band_dense_cp16_kernel<<<(Rc + 63) / 64, 64, 0, s>>>(Rc, dp, out, frag);
+        else band_cp16_kernel<<<Rc, 256, 0, s>>>((const float *)P, dp, out, frag);
+        return;
+    }
     const int R = L / 128;
-    if (cp) {
+    if (cp) {   // 3xBF16
         const int Rc = dp / kCpRows;
         if (dense) band_dense_cp_kernel<<<(Rc + 63) / 64, 64, 0, s>>>(Rc, dp, out, frag);
-        else if (prec == kBF16x3)
-            band_cp_kernel<__bf16><<<Rc, 256, 0, s>>>((const __bf16 *)P, (const __bf16 *)Plo, dp, out, frag);
-        else band_cp_kernel<float><<<Rc, 256, 0, s>>>((const float *)P, nullptr, dp, out, frag);
+        else band_cp_kernel<__bf16><<<Rc, 256, 0, s>>>((const __bf16 *)P, (const __bf16 *)Plo, dp, out, frag);
         return;
     }
     if (dense) band_dense_kernel<<<(R + 63) / 64, 64, 0, s>>>(R, dp, out, frag);

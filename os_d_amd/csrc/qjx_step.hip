@@ -328,7 +328,7 @@ struct FragRange {
 
 __device__ inline FragRange tile_frags(const Ens &E, const TileRef &r)
 {
-    const unsigned *p = E.kfrag + 2 * ((size_t)r.g * E.R + r.rt);
+    const unsigned *p = E.kfrag + kFragWords * ((size_t)r.g * E.R + r.rt);
     FragRange f;
     f.lo = __builtin_amdgcn_readfirstlane(gld<unsigned>(p));
     f.hi = __builtin_amdgcn_readfirstlane(gld<unsigned>(p + 1));
@@ -356,37 +356,45 @@ __device__ inline int bf_off(int row, int k)   // element offset of (row, k), k 
 // column tile and its K range is that of 64 rows instead of 128.  A K block is
 // kCpBlock = 16 complex columns; an LDS row holds [Re 0..15 | Im 0..15] of one
 // propagator row or one state column.  Wave (wm, wn) of the 2 x NWN waves takes the
-// row tile's complex rows 32 wm .. 32 wm + 31 (band fragment wm) and BN / NWN
-// columns.  The FP64 partial norm of a row tile covers Re and Im of its 64 rows; R =
-// dp / 64 equals the block-real tile count 2 dp / 128, so the decide kernel is shared.
+// row tile's complex rows 32 wm .. 32 wm + 31 and BN / NWN columns.  The FP64 partial
+// norm of a row tile covers Re and Im of its 64 rows; R = dp / 64 equals the
+// block-real tile count 2 dp / 128, so the decide kernel is shared.
 //
 // Work unit: one column tile and up to a.rc row tiles (strided: unit c takes row
-// tiles c, c + nch, ...), walked as one stream of K blocks.  The register stages and LDS buffers run on across the
-// row tiles' boundaries, so the next row tile's first blocks are already loaded
-// while the current one finishes, and the column gather, the claim and the first
-// loads are paid once per chunk instead of once per row tile.  At a row tile's last
-// block its outputs and norm partials are written and the accumulators restart.
+// tiles c, c + nch, ...), walked as one stream of K blocks.  The register stages and
+// LDS buffers run on across the row tiles' boundaries, so the next row tile's first
+// blocks are already loaded while the current one finishes, and the column gather,
+// the claim and the first loads are paid once per unit instead of once per row
+// tile.  At a row tile's last block its outputs and norm partials are written and
+// the accumulators restart.
 //
-// FP32 (BF = false): Gauss's three-product complex multiply on
-// v_mfma_f32_32x32x2f32.  With U = A + iB and x = a + ib:
+// FP32 (FM = kCpF32x16): Gauss's three-product complex multiply on
+// v_mfma_f32_16x16x4f32.  With U = A + iB and x = a + ib:
 //   k1 = (A + B) a,   k2 = A (b - a),   k3 = B (a + b),
 //   Re y = k1 - k3,   Im y = k1 + k2,
 // three real products per complex product instead of four; A + B, b - a and a + b
-// are formed in registers from the fragments read out of LDS.  MFMA step s (0..7)
-// of a K block takes k = s + 8 h for lane half h, so a lane reads its 8 values of
-// each operand part with two 16-byte LDS reads (rows padded to CP_PITCH floats).
+// are formed in registers from the fragments read out of LDS.  A wave's rows are
+// two 16-row fragments, and each fragment multiplies only the 4-column k-steps
+// inside its own band window: a 16-row fragment's window is narrower than a 32-row
+// one's, and at 4 instead of 16 columns its rounding wastes less, so the form
+// executes about a quarter fewer products than on 32-row fragments with 16-column
+// blocks.  Within each 16-column part of an LDS row, column k sits at position
+// 4 (k % 4) + k / 4, so lane group g of the 16x16x4 product reads k = g, g + 4,
+// g + 8, g + 12 (one per k-step) with one 16-byte read, and k-step s covers the
+// contiguous columns 4 s .. 4 s + 3.
 //
-// 3xBF16 (BF = true): split products, x = hi + lo (hi = bf16(x), lo = bf16(x - hi))
-// for both operands and U X ~= Ulo Xhi + Uhi Xlo + Uhi Xhi (the propagator is split
-// when packed, the state tile when staged into LDS), FP32 accumulation, with four
-// real products per complex product,
+// 3xBF16 (FM = kCpBF16x3): split products, x = hi + lo (hi = bf16(x), lo =
+// bf16(x - hi)) for both operands and U X ~= Ulo Xhi + Uhi Xlo + Uhi Xhi (the
+// propagator is split when packed, the state tile when staged into LDS), FP32
+// accumulation, with four real products per complex product,
 // Re y = A a - B b and Im y = B a + A b; -B is B with its sign bits flipped in
 // registers.  Each accumulator takes its four small cross terms before its two
 // hi x hi terms.  v_mfma_f32_32x32x16_bf16 takes one part (Re or Im: 16 k) of a K
 // block, lane half h holding k = 8 h .. 8 h + 7; LDS rows are 64 bytes
-// (Re 16 | Im 16 bf16), XOR-swizzled (bf_off).
+// (Re 16 | Im 16 bf16), XOR-swizzled (bf_off).  A wave's 32 rows are one band
+// fragment, skipped in the K blocks outside its range.
 //
-// The k order, and the row split over waves, do not depend on BN or on the chunk,
+// The k order, and the row split over waves, do not depend on BN or on the unit,
 // so results do not depend on where a column lands, on the tile width or on rc.
 // ---------------------------------------------------------------------------
 constexpr int CP_PITCH = 2 * kCpBlock + 4;   // floats per FP32 LDS row (conflict-free 16-byte reads)
@@ -415,11 +423,15 @@ __device__ inline T *uniform_ptr(T *p)
     return reinterpret_cast<T *>(((uint64_t)hi << 32) | lo);
 }
 
-template <bool BF, int BN, int NWN>
+template <int FM, int BN, int NWN>
 __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) void gemm_cp(RunArgs a)
 {
+    static_assert(FM == kCpBF16x3 || FM == kCpF32x16, "form");
+    constexpr bool BF = FM == kCpBF16x3;     // 3xBF16: 32x32x16 products, 32-row fragments
+    constexpr bool F16 = FM == kCpF32x16;    // FP32: 16x16x4 products, 16-row fragments
     constexpr int T = 128 * NWN;             // 2 x NWN waves
-    constexpr int NJ = BN / (32 * NWN);      // 32-column MFMA tiles per wave
+    constexpr int NJ = BN / (32 * NWN);      // 32-column MFMA tiles per wave (16-column: 2 NJ)
+    constexpr int NJ16 = 2 * NJ;
     constexpr int RP = T / 8;                // LDS rows filled per pass (8 threads per row)
     constexpr int NQ = BN / RP;              // state columns copied per thread
     // propagator loads of a K block: 512 16-byte chunks (FP32: 64 rows x 8; BF16: 64
@@ -437,12 +449,14 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
     __shared__ int s_tr[BN];
     __shared__ int s_cur[BN];
     __shared__ double s_norm[2][2][BN];       // [row tile parity][wm][column]
+    __shared__ unsigned s_fw[kCpChunkMax][4];  // 16-row fragment windows of the unit's row tiles
     const int *s_tpre = a.plan->tile_pre;     // read through L2: one search per unit
     const int nitems = *a.nens * kMaxGroups;
     const int total = s_tpre[nitems];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wm = wid / NWN, wn = wid % NWN;
-    const int h = lane >> 5, li = lane & 31;
+    const int h = lane >> 5, li = lane & 31;  // 32x32 products: lane half, row / column
+    const int q4 = lane >> 4, l16 = lane & 15;  // 16x16 products: lane group, row / column
     const int cc = tid & 7, cr = tid >> 3;    // 16-byte chunk of a state LDS row; row
     __shared__ int s_claim[5];
     TileRef r;
@@ -467,24 +481,34 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
         // prefetched ahead of it
         double *const normpart = uniform_ptr(E.normpart);
         // band of each row tile of the unit, one byte per row tile (wave-uniform):
-        // first K block, block count, and the two fragments' ranges
+        // first K block, block count, and (3xBF16) the two 32-row fragments' block
+        // ranges; the FP32 form's 16-row fragment windows go to LDS
         uint64_t bk0 = 0, bnb = 0, bf0 = 0, bf1 = 0, bh0 = 0, bh1 = 0;
         unsigned work = 0;
 #pragma unroll
         for (int i = 0; i < kCpChunkMax; i++) {
             if (i < nrt) {
-                const size_t o = 2 * ((size_t)r.g * R + rt0 + i * nch);
-                const int k0 = __builtin_amdgcn_readfirstlane(gld<int>(E.krange + o));
-                const int k1 = __builtin_amdgcn_readfirstlane(gld<int>(E.krange + o + 1));
-                const unsigned lo = __builtin_amdgcn_readfirstlane(gld<unsigned>(E.kfrag + o));
-                const unsigned hi = __builtin_amdgcn_readfirstlane(gld<unsigned>(E.kfrag + o + 1));
+                const size_t t = (size_t)r.g * R + rt0 + i * nch;
+                const int k0 = __builtin_amdgcn_readfirstlane(gld<int>(E.krange + 2 * t));
+                const int k1 = __builtin_amdgcn_readfirstlane(gld<int>(E.krange + 2 * t + 1));
                 bk0 |= (uint64_t)k0 << (8 * i);
                 bnb |= (uint64_t)(k1 - k0) << (8 * i);
-                bf0 |= (uint64_t)(lo & 0xffu) << (8 * i);
-                bf1 |= (uint64_t)((lo >> 8) & 0xffu) << (8 * i);
-                bh0 |= (uint64_t)(hi & 0xffu) << (8 * i);
-                bh1 |= (uint64_t)((hi >> 8) & 0xffu) << (8 * i);
-                work += ((hi & 0xffu) - (lo & 0xffu)) + (((hi >> 8) & 0xffu) - ((lo >> 8) & 0xffu));
+                if (F16) {
+#pragma unroll
+                    for (int f = 0; f < 4; f++) {
+                        const unsigned w = __builtin_amdgcn_readfirstlane(gld<unsigned>(E.kfrag + kFragWords * t + f));
+                        work += (w >> 16) - (w & 0xffffu);
+                        if (tid == 0) s_fw[i][f] = w;
+                    }
+                } else {
+                    const unsigned lo = __builtin_amdgcn_readfirstlane(gld<unsigned>(E.kfrag + kFragWords * t));
+                    const unsigned hi = __builtin_amdgcn_readfirstlane(gld<unsigned>(E.kfrag + kFragWords * t + 1));
+                    bf0 |= (uint64_t)(lo & 0xffu) << (8 * i);
+                    bf1 |= (uint64_t)((lo >> 8) & 0xffu) << (8 * i);
+                    bh0 |= (uint64_t)(hi & 0xffu) << (8 * i);
+                    bh1 |= (uint64_t)((hi >> 8) & 0xffu) << (8 * i);
+                    work += ((hi & 0xffu) - (lo & 0xffu)) + (((hi >> 8) & 0xffu) - ((lo >> 8) & 0xffu));
+                }
             }
         }
         int nbt = 0;                                 // K blocks of the unit
@@ -493,6 +517,11 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
             if (i < nrt) nbt += byte_at(bnb, i);
         // this thread's load addresses at the unit's first row tile, block offset 0
         const int koff = cc < 4 ? 4 * cc : dp + 4 * (cc - 4);   // Re part (chunks 0..3) or Im part
+        // LDS position of this thread's chunk (FP32): contiguous, or for 16-row
+        // fragments the chunk's element e (k = 4 (cc % 4) + e) at 4 e + cc % 4 of its
+        // part, so that lane group g of a 16x16x4 product reads k = g, g + 4, g + 8,
+        // g + 12 (one per product step) with one 16-byte read
+        const int lpos = F16 ? (cc >> 2) * kCpBlock + (cc & 3) : 4 * cc;
         const LT *aptr[NA];
         int aoff[NA];
         {
@@ -511,7 +540,7 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
                 } else {
                     const int row = cr + RP * q;
                     aptr[q] = U + (size_t)(rt0 * kCpRows + row) * L + koff;
-                    aoff[q] = row * CP_PITCH + 4 * cc;
+                    aoff[q] = row * CP_PITCH + lpos;
                 }
             }
         }
@@ -521,6 +550,8 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
             const int c = cr + RP * q;
             bcol[q] = sl.col<const float>(s_cur[c], s_tr[c], L) + koff;
         }
+        // accumulators: 32x32 products (floatx16 per 32-column tile) or 16x16 products
+        // (floatx4 per 16-row fragment and 16-column tile), the same registers
         floatx16 acc[NACC][NJ];
         auto zero_acc = [&]() {
 #pragma unroll
@@ -529,6 +560,21 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
                 for (int j = 0; j < NJ; j++)
 #pragma unroll
                     for (int e = 0; e < 16; e++) acc[p][j][e] = 0.f;
+        };
+        // the 16x16 accumulator of product p, fragment f (of the wave's two) and
+        // 16-column tile j, as a view of the same registers
+        auto acc16 = [&](int p, int f, int j) -> floatx4 {
+            const floatx16 &v = acc[p][(f * NJ16 + j) >> 2];
+            const int o = 4 * ((f * NJ16 + j) & 3);
+            return floatx4{v[o], v[o + 1], v[o + 2], v[o + 3]};
+        };
+        auto acc16_set = [&](int p, int f, int j, floatx4 x) {
+            floatx16 &v = acc[p][(f * NJ16 + j) >> 2];
+            const int o = 4 * ((f * NJ16 + j) & 3);
+            v[o] = x[0];
+            v[o + 1] = x[1];
+            v[o + 2] = x[2];
+            v[o + 3] = x[3];
         };
         zero_acc();
         struct Stage {
@@ -552,13 +598,27 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
                 lkb = 0;
             }
         };
-        auto lstore = [&](const Stage &sg, int buf) {
+        // one FP32 chunk into LDS: 16 bytes, or (F16) its four elements 4 apart
+        auto st_chunk = [&](float *dst, floatx4 v) {
+            if (F16) {
 #pragma unroll
-            for (int q = 0; q < NA; q++) *reinterpret_cast<AV *>(&s_a[buf][aoff[q]]) = sg.a[q];
+                for (int e = 0; e < 4; e++) dst[4 * e] = v[e];
+            } else {
+                *reinterpret_cast<floatx4 *>(dst) = v;
+            }
+        };
+        auto lstore = [&](const Stage &sg, int buf) {
+            if constexpr (BF) {
+#pragma unroll
+                for (int q = 0; q < NA; q++) *reinterpret_cast<AV *>(&s_a[buf][aoff[q]]) = sg.a[q];
+            } else {
+#pragma unroll
+                for (int q = 0; q < NA; q++) st_chunk(&s_a[buf][aoff[q]], sg.a[q]);
+            }
 #pragma unroll
             for (int q = 0; q < NQ; q++) {
                 const int rr = cr + RP * q;
-                if (BF) {
+                if constexpr (BF) {
                     bf16x4 h4, l4;
 #pragma unroll
                     for (int e = 0; e < 4; e++) {
@@ -571,14 +631,23 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
                     *reinterpret_cast<bf16x4 *>(&bb[bf_off(rr, 4 * cc)]) = h4;
                     *reinterpret_cast<bf16x4 *>(&bb[BN * BF_ROW + bf_off(rr, 4 * cc)]) = l4;
                 } else {
-                    *reinterpret_cast<floatx4 *>(reinterpret_cast<float *>(s_b[buf]) + rr * CP_PITCH + 4 * cc) =
-                        sg.b[q];
+                    st_chunk(reinterpret_cast<float *>(s_b[buf]) + rr * CP_PITCH + lpos, sg.b[q]);
                 }
             }
         };
+        // the 16-row fragments' windows of the current row tile (F16), in units of 4
+        // complex columns: the wave's fragments are 2 wm and 2 wm + 1
+        unsigned fw0 = 0, fw1 = 0;
+        auto load_fw = [&](int i) {
+            if (F16) {
+                fw0 = __builtin_amdgcn_readfirstlane(s_fw[i][2 * wm]);
+                fw1 = __builtin_amdgcn_readfirstlane(s_fw[i][2 * wm + 1]);
+            }
+        };
         // mid(): the next stage's LDS stores and loads, issued between the block's
-        // matrix products so that they overlap the products instead of following them
-        auto mma_block = [&](int buf, auto &&mid) {
+        // matrix products so that they overlap the products instead of following them.
+        // b: the block's index in its half of the operand (F16: the k-step windows).
+        auto mma_block = [&](int buf, int b, auto &&mid) {
             if constexpr (BF) {
                 const __bf16 *sa = reinterpret_cast<const __bf16 *>(s_a[buf]);
                 const __bf16 *sb = reinterpret_cast<const __bf16 *>(s_b[buf]);
@@ -633,74 +702,120 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
                     acc[1][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh, bh[j], acc[1][j], 0, 0, 0);
                 }
             } else {
+                // k-step s (0..3) of fragment f runs if 4 b + s lies in the fragment's
+                // window [lo, hi) (units of 4 complex columns)
+                const int s00 = min(max((int)(fw0 & 0xffffu) - 4 * b, 0), 4);
+                const int s01 = min(max((int)(fw0 >> 16) - 4 * b, 0), 4);
+                const int s10 = min(max((int)(fw1 & 0xffffu) - 4 * b, 0), 4);
+                const int s11 = min(max((int)(fw1 >> 16) - 4 * b, 0), 4);
                 const float *sa = reinterpret_cast<const float *>(s_a[buf]);
                 const float *sb = reinterpret_cast<const float *>(s_b[buf]);
+                floatx4 fre[2], fim[2], xa[NJ16], xb[NJ16];
 #pragma unroll
-                for (int v = 0; v < 2; v++) {
-                    const float *pa = &sa[(wm * 32 + li) * CP_PITCH + 8 * h + 4 * v];
-                    const floatx4 fre = *reinterpret_cast<const floatx4 *>(pa);
-                    const floatx4 fim = *reinterpret_cast<const floatx4 *>(pa + kCpBlock);
-                    floatx4 xa[NJ], xd[NJ], xs[NJ];
+                for (int f = 0; f < 2; f++) {
+                    const float *pa = &sa[(wm * 32 + f * 16 + l16) * CP_PITCH + 4 * q4];
+                    fre[f] = *reinterpret_cast<const floatx4 *>(pa);
+                    fim[f] = *reinterpret_cast<const floatx4 *>(pa + kCpBlock);
+                }
 #pragma unroll
-                    for (int j = 0; j < NJ; j++) {
-                        const float *pb = &sb[(wn * (BN / NWN) + j * 32 + li) * CP_PITCH + 8 * h + 4 * v];
-                        xa[j] = *reinterpret_cast<const floatx4 *>(pb);
-                        const floatx4 xb = *reinterpret_cast<const floatx4 *>(pb + kCpBlock);
-                        xd[j] = xb - xa[j];
-                        xs[j] = xa[j] + xb;
-                    }
-                    const floatx4 fs = fre + fim;
+                for (int j = 0; j < NJ16; j++) {
+                    const float *pb = &sb[(wn * (BN / NWN) + j * 16 + l16) * CP_PITCH + 4 * q4];
+                    xa[j] = *reinterpret_cast<const floatx4 *>(pb);
+                    xb[j] = *reinterpret_cast<const floatx4 *>(pb + kCpBlock);
+                }
 #pragma unroll
-                    for (int e = 0; e < 4; e++)
+                for (int s = 0; s < 4; s++) {
+                    if (s == 2) mid();
 #pragma unroll
-                        for (int j = 0; j < NJ; j++) {
-                            acc[0][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fs[e], xa[j][e], acc[0][j], 0, 0, 0);
-                            acc[1][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fre[e], xd[j][e], acc[1][j], 0, 0, 0);
-                            acc[2][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fim[e], xs[j][e], acc[2][j], 0, 0, 0);
+                    for (int f = 0; f < 2; f++) {
+                        if (s < (f ? s10 : s00) || s >= (f ? s11 : s01)) continue;
+                        const float as = fre[f][s] + fim[f][s];
+#pragma unroll
+                        for (int j = 0; j < NJ16; j++) {
+                            const float bd = xb[j][s] - xa[j][s], bs = xa[j][s] + xb[j][s];
+                            acc16_set(0, f, j, __builtin_amdgcn_mfma_f32_16x16x4f32(as, xa[j][s], acc16(0, f, j), 0, 0, 0));
+                            acc16_set(1, f, j, __builtin_amdgcn_mfma_f32_16x16x4f32(fre[f][s], bd, acc16(1, f, j), 0, 0, 0));
+                            acc16_set(2, f, j, __builtin_amdgcn_mfma_f32_16x16x4f32(fim[f][s], bs, acc16(2, f, j), 0, 0, 0));
                         }
-                    if (v == 0) mid();
+                    }
                 }
             }
         };
         // outputs and norm partial of row tile i of the unit (after its last block)
         auto epilogue = [&](int i) {
             const int r0 = (rt0 + i * nch) * kCpRows;
-            double ns[NJ];
+            if constexpr (F16) {
+                // 16x16 products: lane (g, c) of fragment f holds rows 4 g .. 4 g + 3 of the
+                // fragment in column c of its 16-column tile
+                double ns[NJ16];
 #pragma unroll
-            for (int j = 0; j < NJ; j++) {
-                ns[j] = 0.0;
-                const int c = wn * (BN / NWN) + j * 32 + li;
-                const bool ok = c < r.ncol;
-                float *y = sl.col<float>(s_cur[c] ^ 1, s_tr[c], L);
+                for (int j = 0; j < NJ16; j++) {
+                    ns[j] = 0.0;
+                    const int c = wn * (BN / NWN) + j * 16 + l16;
+                    const bool ok = c < r.ncol;
+                    float *y = sl.col<float>(s_cur[c] ^ 1, s_tr[c], L);
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int row = r0 + wm * 32 + 8 * q + 4 * h;
-                    floatx4 re, im;
+                    for (int f = 0; f < 2; f++) {
+                        const int row = r0 + wm * 32 + f * 16 + 4 * q4;
+                        const floatx4 k1 = acc16(0, f, j), k2 = acc16(1, f, j), k3 = acc16(2, f, j);
+                        floatx4 re, im;
 #pragma unroll
-                    for (int e = 0; e < 4; e++) {
-                        if (BF) {
+                        for (int e = 0; e < 4; e++) {
+                            re[e] = k1[e] - k3[e];
+                            im[e] = k1[e] + k2[e];
+                        }
+                        if (ok) {
+                            gst<floatx4>(y + row, re);
+                            gst<floatx4>(y + dp + row, im);
+                        }
+#pragma unroll
+                        for (int e = 0; e < 4; e++) ns[j] += (double)re[e] * (double)re[e];
+#pragma unroll
+                        for (int e = 0; e < 4; e++) ns[j] += (double)im[e] * (double)im[e];
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < NJ16; j++) {
+                    ns[j] += __shfl_xor(ns[j], 16, 64);
+                    ns[j] += __shfl_xor(ns[j], 32, 64);
+                }
+                if (q4 == 0) {
+#pragma unroll
+                    for (int j = 0; j < NJ16; j++) s_norm[i & 1][wm][wn * (BN / NWN) + j * 16 + l16] = ns[j];
+                }
+            } else {
+                double ns[NJ];
+#pragma unroll
+                for (int j = 0; j < NJ; j++) {
+                    ns[j] = 0.0;
+                    const int c = wn * (BN / NWN) + j * 32 + li;
+                    const bool ok = c < r.ncol;
+                    float *y = sl.col<float>(s_cur[c] ^ 1, s_tr[c], L);
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const int row = r0 + wm * 32 + 8 * q + 4 * h;
+                        floatx4 re, im;
+#pragma unroll
+                        for (int e = 0; e < 4; e++) {
                             re[e] = acc[0][j][4 * q + e];
                             im[e] = acc[1][j][4 * q + e];
-                        } else {
-                            re[e] = acc[0][j][4 * q + e] - acc[NACC - 1][j][4 * q + e];
-                            im[e] = acc[0][j][4 * q + e] + acc[1][j][4 * q + e];
                         }
-                    }
-                    if (ok) {
-                        gst<floatx4>(y + row, re);
-                        gst<floatx4>(y + dp + row, im);
-                    }
+                        if (ok) {
+                            gst<floatx4>(y + row, re);
+                            gst<floatx4>(y + dp + row, im);
+                        }
 #pragma unroll
-                    for (int e = 0; e < 4; e++) ns[j] += (double)re[e] * (double)re[e];
+                        for (int e = 0; e < 4; e++) ns[j] += (double)re[e] * (double)re[e];
 #pragma unroll
-                    for (int e = 0; e < 4; e++) ns[j] += (double)im[e] * (double)im[e];
+                        for (int e = 0; e < 4; e++) ns[j] += (double)im[e] * (double)im[e];
+                    }
                 }
-            }
 #pragma unroll
-            for (int j = 0; j < NJ; j++) ns[j] += __shfl_xor(ns[j], 32, 64);
-            if (h == 0) {
+                for (int j = 0; j < NJ; j++) ns[j] += __shfl_xor(ns[j], 32, 64);
+                if (h == 0) {
 #pragma unroll
-                for (int j = 0; j < NJ; j++) s_norm[i & 1][wm][wn * (BN / NWN) + j * 32 + li] = ns[j];
+                    for (int j = 0; j < NJ; j++) s_norm[i & 1][wm][wn * (BN / NWN) + j * 32 + li] = ns[j];
+                }
             }
             zero_acc();
         };
@@ -717,6 +832,7 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
         lstore(st0, 0);
         gload(st0);
         __syncthreads();
+        load_fw(0);
         // MMA cursor (wave-uniform): row tile mrt, block mkb of its band; pend = row
         // tile whose norm partial waits in s_norm
         int mrt = 0, mkb = 0, pend = -1, cn = 0;
@@ -734,16 +850,23 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
             // hoisted, they would wait for loads that are meant to stay in flight
             __builtin_amdgcn_sched_barrier(0);
             const int b = byte_at(bk0, mrt) + mkb;
-            const int fl = wm ? byte_at(bf1, mrt) : byte_at(bf0, mrt);
-            const int fh = wm ? byte_at(bh1, mrt) : byte_at(bh0, mrt);
-            // the wave's fragment multiplies only the blocks inside its band range
             auto mid = [&]() {
                 __builtin_amdgcn_sched_barrier(0);
                 lstore(nxt, buf ^ 1);
                 gload(nxt);
                 __builtin_amdgcn_sched_barrier(0);
             };
-            if (b >= fl && b < fh) mma_block(buf, mid);
+            // the wave's fragments multiply only the blocks inside their band ranges
+            bool act;
+            if constexpr (F16) {
+                act = (4 * b + 4 > (int)(fw0 & 0xffffu) && 4 * b < (int)(fw0 >> 16)) ||
+                      (4 * b + 4 > (int)(fw1 & 0xffffu) && 4 * b < (int)(fw1 >> 16));
+            } else {
+                const int fl = wm ? byte_at(bf1, mrt) : byte_at(bf0, mrt);
+                const int fh = wm ? byte_at(bh1, mrt) : byte_at(bh0, mrt);
+                act = b >= fl && b < fh;
+            }
+            if (act) mma_block(buf, b, mid);
             else mid();
             __syncthreads();
             if (pend >= 0) {
@@ -755,6 +878,7 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
                 pend = mrt;
                 mrt++;
                 mkb = 0;
+                if (mrt < nrt) load_fw(mrt);
             } else {
                 mkb++;
             }
@@ -768,7 +892,10 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
         if (wid == 0) tile_resolve<BN>(a, s_tpre, nitems, total, cn, s_claim);
         __syncthreads();
         write_norm(pend);
-        if (tid == 0) atomicAdd(&E.counters[2], (unsigned long long)(BF ? 4 : 3) * r.ncol * work);
+        // executed matrix-core flops in units of 128: FP32 3 products x 2 x 16 x 4 per
+        // (16-row fragment, 4-column k-step); 3xBF16 (FP32-equivalent) 4 products x 2 x
+        // 32 x 16 per (32-row fragment, K block)
+        if (tid == 0) atomicAdd(&E.counters[2], (unsigned long long)(BF ? 32 : 3) * r.ncol * work);
         __syncthreads();
     }
 }
@@ -921,7 +1048,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(BN == 64 ? 
         if (tid < r.ncol)
             gst<double>(E.normpart + (size_t)s_tr[tid] * E.R + r.rt,
                         ((s_norm[0][tid] + s_norm[1][tid]) + s_norm[2][tid]) + s_norm[3][tid]);
-        if (tid == 0) atomicAdd(&E.counters[2], 2ull * r.ncol * fr.work());   // 2048 flops per unit of work()
+        if (tid == 0) atomicAdd(&E.counters[2], 16ull * r.ncol * fr.work());   // 2048 flops per unit of work()
         __syncthreads();
     }
 }
@@ -1352,14 +1479,14 @@ void launch_gemm(int prec, int gemm_kind, const RunArgs &a, const StepGrid &g, h
     } else if (prec == kFP64) {
         launch_gemm_f64(a, g, s);
     } else if (prec == kBF16x3) {
-        if (a.bn == 256) gemm_cp<true, 256, 4><<<g.gemm, 512, 0, s>>>(a);
-        else if (g.waves == 8) gemm_cp<true, 128, 4><<<g.gemm, 512, 0, s>>>(a);
-        else if (a.bn == 64) gemm_cp<true, 64, 2><<<g.gemm, 256, 0, s>>>(a);
-        else gemm_cp<true, 128, 2><<<g.gemm, 256, 0, s>>>(a);
+        if (a.bn == 256) gemm_cp<kCpBF16x3, 256, 4><<<g.gemm, 512, 0, s>>>(a);
+        else if (g.waves == 8) gemm_cp<kCpBF16x3, 128, 4><<<g.gemm, 512, 0, s>>>(a);
+        else if (a.bn == 64) gemm_cp<kCpBF16x3, 64, 2><<<g.gemm, 256, 0, s>>>(a);
+        else gemm_cp<kCpBF16x3, 128, 2><<<g.gemm, 256, 0, s>>>(a);
     } else {
-        if (a.bn == 256) gemm_cp<false, 256, 4><<<g.gemm, 512, 0, s>>>(a);
-        else if (a.bn == 64) gemm_cp<false, 64, 2><<<g.gemm, 256, 0, s>>>(a);
-        else gemm_cp<false, 128, 2><<<g.gemm, 256, 0, s>>>(a);
+        if (a.bn == 256) gemm_cp<kCpF32x16, 256, 4><<<g.gemm, 512, 0, s>>>(a);
+        else if (a.bn == 64) gemm_cp<kCpF32x16, 64, 2><<<g.gemm, 256, 0, s>>>(a);
+        else gemm_cp<kCpF32x16, 128, 2><<<g.gemm, 256, 0, s>>>(a);
     }
 }
 
@@ -1388,14 +1515,14 @@ int gemm_blocks_per_cu(int prec, int gemm_kind, int bn, int waves)
     else if (prec == kFP64)
         e = gemm_f64_occupancy(&n, bn);
     else if (prec == kBF16x3)
-        e = bn == 256 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_cp<true, 256, 4>, 512, 0)
-            : waves == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_cp<true, 128, 4>, 512, 0)
-            : narrow   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_cp<true, 64, 2>, 256, 0)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_cp<true, 128, 2>, 256, 0);
+        e = bn == 256 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_cp<kCpBF16x3, 256, 4>, 512, 0)
+            : waves == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_cp<kCpBF16x3, 128, 4>, 512, 0)
+            : narrow   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_cp<kCpBF16x3, 64, 2>, 256, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_cp<kCpBF16x3, 128, 2>, 256, 0);
     else
-        e = bn == 256 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_cp<false, 256, 4>, 512, 0)
-            : narrow  ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_cp<false, 64, 2>, 256, 0)
-                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_cp<false, 128, 2>, 256, 0);
+        e = bn == 256 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_cp<kCpF32x16, 256, 4>, 512, 0)
+            : narrow  ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_cp<kCpF32x16, 64, 2>, 256, 0)
+                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm_cp<kCpF32x16, 128, 2>, 256, 0);
     return (e == hipSuccess && n > 0) ? n : 1;
 }
 
@@ -1421,6 +1548,21 @@ __global__ void mfma_probe_kernel(const float *A32, const float *B32, float *D32
     floatx16 cb = {};
     cb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, cb, 0, 0, 0);
     for (int r = 0; r < 16; r++) Dbf[(8 * (r / 4) + 4 * (l / 32) + r % 4) * 32 + l % 32] = cb[r];
+}
+
+// v_mfma_f32_16x16x4f32: A (16 x 4): lane l holds A[l % 16][l / 16]; B (4 x 16):
+// B[l / 16][l % 16]; D (16 x 16): lane l, register r holds D[4 (l / 16) + r][l % 16]
+__global__ void mfma_probe16_kernel(const float *A, const float *B, float *D)
+{
+    const int l = threadIdx.x;
+    floatx4 c = {0.f, 0.f, 0.f, 0.f};
+    c = __builtin_amdgcn_mfma_f32_16x16x4f32(A[(l % 16) * 4 + l / 16], B[(l / 16) * 16 + l % 16], c, 0, 0, 0);
+    for (int r = 0; r < 4; r++) D[(4 * (l / 16) + r) * 16 + l % 16] = c[r];
+}
+
+void launch_mfma_probe16(const float *A, const float *B, float *D, hipStream_t s)
+{
+    mfma_probe16_kernel<<<1, 64, 0, s>>>(A, B, D);
 }
 
 void launch_mfma_probe(const float *A32, const float *B32, float *D32, const double *A64,

@@ -122,6 +122,7 @@ SIGNATURES = [
     ("qjx_dim", ctypes.c_int, [ctypes.c_void_p]),
     ("qjx_peak_probe", ctypes.c_int, [_D, _D, _D, _D, _D]),
     ("qjx_set_profiling", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("qjx_mfma_probe_f32x16", ctypes.c_int, [_P(ctypes.c_float), _P(ctypes.c_float), _P(ctypes.c_float)]),
     ("qjx_mfma_probe", ctypes.c_int, [_P(ctypes.c_float), _P(ctypes.c_float), _P(ctypes.c_float), _D, _D,
                                       _D, _P(ctypes.c_float), _P(ctypes.c_float), _P(ctypes.c_float)]),
     ("qjx_run_config", ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]),
@@ -509,6 +510,15 @@ def mfma_probe(A32, B32, A64, B64, Abf, Bbf):
                                    _ptr(Abf, ctypes.c_float), _ptr(Bbf, ctypes.c_float),
                                    _ptr(Dbf, ctypes.c_float)))
     return D32, D64, Dbf
+
+
+def mfma_probe_f32x16(A, B):
+    """The 16x16x4 FP32 matrix-core product of one wavefront: D = A (16x4) B (4x16)."""
+    lib = load_library()
+    A, B = (np.ascontiguousarray(x, dtype=np.float32) for x in (A, B))
+    D = np.zeros((16, 16), dtype=np.float32)
+    _check(lib, lib.qjx_mfma_probe_f32x16(_ptr(A, ctypes.c_float), _ptr(B, ctypes.c_float), _ptr(D, ctypes.c_float)))
+    return D
 
 
 def run_config(config_path, params_path, precision=FP64, steps_per_period=0):

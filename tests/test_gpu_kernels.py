@@ -29,6 +29,10 @@ def test_mfma_layout_probe():
     np.testing.assert_allclose(D32, A32.astype(np.float64) @ B32.astype(np.float64), rtol=1e-5, atol=1e-5)
     np.testing.assert_allclose(D64, A64 @ B64, rtol=1e-13, atol=1e-13)
     np.testing.assert_array_equal(Dbf, Abf @ Bbf)
+    # the FP32 step GEMM's 16x16x4 product; small integers keep every sum exact
+    A16 = rng.integers(-8, 9, (16, 4)).astype(np.float32)
+    B16 = rng.integers(-8, 9, (4, 16)).astype(np.float32)
+    np.testing.assert_array_equal(qjx.mfma_probe_f32x16(A16, B16), A16 @ B16)
 
 
 @pytest.mark.parametrize("prec", [0, 1])
