@@ -15,11 +15,12 @@ headline   : one transient period (FP32, from e_0) gives the common starting sta
              come from a separate pass with HIP events on every launch (20 FP32
              steps: the roofline; 4 steps for the other forms).
 roofline   : the FP32 step GEMM (dominant kernel); algorithmic flops = the matrix-core
-             flops it executes: 3 products x 2 x 32 x 16 per (column, 32-row
-             fragment, K block of 16 complex columns) inside the fragments' band
-             ranges (Gauss's three-product complex multiply; the rest of the dense
+             flops it executes: 3 products x 2 x 16 x 4 per (column, 16-row
+             fragment, k-step of 4 complex columns) inside the fragments' band
+             windows (Gauss's three-product complex multiply; the rest of the dense
              8 d^2 per application are exact zeros that are skipped) per launch of
-             the event pass, over its average launch duration.
+             the event pass, over its average launch duration.  hbm_gbps / hbm_frac:
+             the committed FETCH+WRITE bytes per column over the same launches.
 cpu_baseline: the FP64 CPU oracle (OpenMP over trajectories, one dense zgemv per
              step as the reference) on a bounded sample started from the same
              post-transient states, with the same propagator tree.
@@ -239,7 +240,8 @@ def gemm_kernel_name(prec, bn, waves):
     """Name of the step GEMM kernel the launch shape selects (as a kernel trace shows it)."""
     if prec == 0:
         return "qjx::gemm_f64<%d>" % bn
-    return "qjx::gemm_cp<%s, %d, %d>" % ("true" if prec == 2 else "false", bn, 4 if (bn == 256 or waves == 8) else 2)
+    # complex-pair GEMM forms: 1 = 3xBF16, 2 = FP32 (16-row fragments)
+    return "qjx::gemm_cp<%d, %d, %d>" % (1 if prec == 2 else 2, bn, 4 if (bn == 256 or waves == 8) else 2)
 
 
 def observables(e, N, M):
@@ -769,9 +771,11 @@ def main():
             "kernel": "%s (grouped step GEMM, gathered columns)" % r32["gemm_kernel"],
             "launches": launches, "avg_launch_ms": avg_ms, "flops_per_launch": flops_per_launch,
             "columns_per_launch": cols_per_launch,
-            "flops_note": "3 products x 2 x 32 x 16 per (column, 32-row fragment, K block of 16 complex columns) "
-                          "inside the fragments' band ranges (three-product complex multiply); launches of the "
+            "flops_note": "3 products x 2 x 16 x 4 per (column, 16-row fragment, k-step of 4 complex columns) "
+                          "inside the fragments' band windows (three-product complex multiply); launches of the "
                           "%d-step HIP-event pass after the timed period" % args.roofline_steps,
+            "hbm_gbps": (per_col * cols_per_launch / (avg_ms * 1e-3) / 1e9) if (per_col and N == 1000 and avg_ms > 0) else None,
+            "hbm_frac": (per_col * cols_per_launch / (avg_ms * 1e-3) / 8e12) if (per_col and N == 1000 and avg_ms > 0) else None,
             "dense_equivalent_flops_per_launch": 8.0 * (N + 1) ** 2 * cols_per_launch,
         }
         line["fp32"] = r32
