@@ -143,9 +143,11 @@ __global__ __launch_bounds__(kPlanThreads) void plan_kernel(RunArgs a)
 // is exhausted takes tiles from the other XCDs' ranges.  Lane 0 of wave 0 claims
 // with an atomic (tile_claim, issued early: the GEMM kernels claim their next tile
 // before their epilogue's stores and wait for the answer after them); wave 0 then
-// finds the tile's item in the plan prefix with a two-level ballot search (two
-// dependent loads instead of log2(items)), places the tile in its item and leaves
-// the result in LDS for tile_read.
+// finds the tile's item in the plan prefix with a two-level ballot search, places
+// the tile in its item and leaves the result in LDS for tile_read.  The search's
+// first level is loaded once per launch (search_l1: the prefix does not change
+// during one), and the second level comes with the item's column count (the entry
+// prefix) and its ensemble's row-tile count: one dependent load round per claim.
 struct TileRef {
     int e, g, rt, col0, ncol;
 };
@@ -160,11 +162,20 @@ __device__ inline int tile_claim(const RunArgs &a)
     return threadIdx.x == 0 ? atomicAdd(&a.plan->xnext[blockIdx.x & 7], 1) : 0;
 }
 
+// Wave 0, once per launch: lane l's value of the search's first level, pre[64 l]
+// (INT_MAX past the items).
+__device__ inline int search_l1(const int *pre, int nitems)
+{
+    const int i1 = (threadIdx.x & 63) * 64;
+    return i1 < nitems ? gld<int>(pre + i1) : 0x7fffffff;
+}
+
 // Wave 0: the tile of claim c0 (lane 0's tile_claim result) -> s_claim[0..4] =
 // ensemble, group, row tile (row chunk of a.rc row tiles), first column, columns;
-// s_claim[0] = -1 when no tile is left.
+// s_claim[0] = -1 when no tile is left.  v1: this lane's search_l1.
 template <int BN>
-__device__ inline void tile_resolve(const RunArgs &a, const int *pre, int nitems, int total, int c0, int *s_claim)
+__device__ inline void tile_resolve(const RunArgs &a, const int *pre, int nitems, int total, int c0, int v1,
+                                    int *s_claim)
 {
     const int lane = threadIdx.x;
     int t = -1;
@@ -189,25 +200,33 @@ __device__ inline void tile_resolve(const RunArgs &a, const int *pre, int nitems
         return;
     }
     // the largest item with pre[item] <= t (pre non-decreasing, pre[0] = 0,
-    // pre[nitems] = total > t)
-    const int i1 = lane * 64;
-    const unsigned long long m1 = __ballot(i1 < nitems && gld<int>(pre + i1) <= t);
+    // pre[nitems] = total > t).  The 64 items from b are one ensemble's groups
+    // (kMaxGroups = 64), so its row-tile count loads with the second level.
+    static_assert(kMaxGroups == 64, "one ensemble per second-level range");
+    const unsigned long long m1 = __ballot(v1 <= t);
     const int b = (63 - __clzll(m1)) * 64;
     const int i2 = b + lane;
-    const int v2 = i2 < nitems ? gld<int>(pre + i2) : 0x7fffffff;
+    const int *ent = a.plan->ent_pre;
+    const int R = a.ens[b / kMaxGroups].R;
+    int v2 = 0x7fffffff, n0 = 0, n1 = 0;
+    if (i2 < nitems) {
+        v2 = gld<int>(pre + i2);
+        n0 = gld<int>(ent + i2);
+        n1 = gld<int>(ent + i2 + 1);
+    }
     const unsigned long long m2 = __ballot(v2 <= t);
     const int w = 63 - __clzll(m2);
     const int item = b + w;
     const int local = t - __shfl(v2, w, 64);
     const int e = item / kMaxGroups, g = item % kMaxGroups;
-    const Ens &E = a.ens[e];
     // tiles of an item: super-columns of a.sc column tiles, each visited row tile
     // by row tile (row chunk by row chunk: a.rc row tiles per unit), so the row
     // tile's propagator band is reused from L2 by the super-column's column tiles
-    // and the column tiles' states by the row tiles
-    const int cnt = gld<int>(E.cnt[a.parity] + g);
+    // and the column tiles' states by the row tiles.  The item's column count is
+    // its list's entry count (entry prefix).
+    const int cnt = __shfl(n1 - n0, w, 64);
     const int nct = (cnt + BN - 1) / BN;
-    const int per = ((E.R + a.rc - 1) / a.rc) * a.sc;
+    const int per = ((R + a.rc - 1) / a.rc) * a.sc;
     const int sc = local / per, rem = local - sc * per;
     const int cw = min(a.sc, nct - sc * a.sc);
     const int col0 = (sc * a.sc + rem % cw) * BN;
@@ -238,7 +257,7 @@ template <int BN>
 __device__ inline bool tile_next(const RunArgs &a, const int *pre, int nitems, int total, int *s_claim,
                                  TileRef &r)
 {
-    if (threadIdx.x < 64) tile_resolve<BN>(a, pre, nitems, total, tile_claim(a), s_claim);
+    if (threadIdx.x < 64) tile_resolve<BN>(a, pre, nitems, total, tile_claim(a), search_l1(pre, nitems), s_claim);
     __syncthreads();
     const bool ok = tile_read(s_claim, r);
     __syncthreads();
@@ -397,7 +416,8 @@ __device__ inline int bf_off(int row, int k)   // element offset of (row, k), k 
 // The k order, and the row split over waves, do not depend on BN or on the unit,
 // so results do not depend on where a column lands, on the tile width or on rc.
 // ---------------------------------------------------------------------------
-constexpr int CP_PITCH = 2 * kCpBlock + 4;   // floats per FP32 LDS row (conflict-free 16-byte reads)
+// floats per FP32 LDS row: 8-byte reads of 16 consecutive rows fall in distinct banks
+constexpr int CP_PITCH = 2 * kCpBlock + 2;
 constexpr int kCpChunkMax = 8;               // row tiles per work unit at most (bytes of a 64-bit word)
 
 typedef unsigned uintx4 __attribute__((ext_vector_type(4)));
@@ -444,8 +464,11 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
     constexpr int A_ELEMS = BF ? 2 * kCpRows * BF_ROW : kCpRows * CP_PITCH;   // per buffer (hi, lo)
     constexpr int B_ELEMS = BF ? 2 * BN * BF_ROW : BN * CP_PITCH;
     using LT = std::conditional_t<BF, __bf16, float>;
-    __shared__ __attribute__((aligned(16))) LT s_a[2][A_ELEMS];
-    __shared__ __attribute__((aligned(16))) LT s_b[2][B_ELEMS];
+    // LDS buffers: FP32 keeps three (a block is stored two iterations before its
+    // products, so the next block's fragments can be read during this one's)
+    constexpr int NBUF = F16 ? 3 : 2;
+    __shared__ __attribute__((aligned(16))) LT s_a[NBUF][A_ELEMS];
+    __shared__ __attribute__((aligned(16))) LT s_b[NBUF][B_ELEMS];
     __shared__ int s_tr[BN];
     __shared__ int s_cur[BN];
     __shared__ double s_norm[2][2][BN];       // [row tile parity][wm][column]
@@ -460,11 +483,10 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
     const int cc = tid & 7, cr = tid >> 3;    // 16-byte chunk of a state LDS row; row
     __shared__ int s_claim[5];
     TileRef r;
-    if (wid == 0) tile_resolve<BN>(a, s_tpre, nitems, total, tile_claim(a), s_claim);
+    const int v1 = wid == 0 ? search_l1(s_tpre, nitems) : 0;
+    if (wid == 0) tile_resolve<BN>(a, s_tpre, nitems, total, tile_claim(a), v1, s_claim);
     __syncthreads();
     while (tile_read(s_claim, r)) {
-        load_cols<BN>(a, r, s_tr, s_cur);
-        __syncthreads();
         const Ens &E = a.ens[r.e];
         const Slots sl = tile_slots(E);
         const int L = E.L, dp = E.dp, R = E.R;
@@ -511,6 +533,10 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
                 }
             }
         }
+        // the unit's columns, gathered once the band table's loads are issued, so that
+        // the two wait together
+        load_cols<BN>(a, r, s_tr, s_cur);
+        __syncthreads();
         int nbt = 0;                                 // K blocks of the unit
 #pragma unroll
         for (int i = 0; i < kCpChunkMax; i++)
@@ -644,9 +670,57 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
                 fw1 = __builtin_amdgcn_readfirstlane(s_fw[i][2 * wm + 1]);
             }
         };
+        // FP32 fragments of a block in two halves, k-steps 0, 1 (lo) and 2, 3 (hi): the
+        // wave's two 16-row fragments (Re and Im of U) and its 16-column tiles (a and b
+        // of x).  A block's hi half is read as its products start and its lo half during
+        // the previous block's second half, so every LDS read overlaps matrix products.
+        float2 fr_lo[2], fi_lo[2], fr_hi[2], fi_hi[2];
+        float2 xa_lo[NJ16], xb_lo[NJ16], xa_hi[NJ16], xb_hi[NJ16];
+        auto read_half = [&](int buf, int half, float2 (&fr)[2], float2 (&fi)[2], float2 (&xa)[NJ16],
+                             float2 (&xb)[NJ16]) {
+            const float *sa = reinterpret_cast<const float *>(s_a[buf]);
+            const float *sb = reinterpret_cast<const float *>(s_b[buf]);
+#pragma unroll
+            for (int f = 0; f < 2; f++) {
+                const float *pa = &sa[(wm * 32 + f * 16 + l16) * CP_PITCH + 4 * q4 + 2 * half];
+                fr[f] = *reinterpret_cast<const float2 *>(pa);
+                fi[f] = *reinterpret_cast<const float2 *>(pa + kCpBlock);
+            }
+#pragma unroll
+            for (int j = 0; j < NJ16; j++) {
+                const float *pb = &sb[(wn * (BN / NWN) + j * 16 + l16) * CP_PITCH + 4 * q4 + 2 * half];
+                xa[j] = *reinterpret_cast<const float2 *>(pb);
+                xb[j] = *reinterpret_cast<const float2 *>(pb + kCpBlock);
+            }
+        };
+        // k-steps 2 half, 2 half + 1 of block b: step s of fragment f runs if 4 b + s
+        // lies in the fragment's window [lo, hi) (units of 4 complex columns)
+        auto f16_steps = [&](int b, int half, const float2 (&fr)[2], const float2 (&fi)[2],
+                             const float2 (&xa)[NJ16], const float2 (&xb)[NJ16]) {
+            const int s00 = min(max((int)(fw0 & 0xffffu) - 4 * b, 0), 4);
+            const int s01 = min(max((int)(fw0 >> 16) - 4 * b, 0), 4);
+            const int s10 = min(max((int)(fw1 & 0xffffu) - 4 * b, 0), 4);
+            const int s11 = min(max((int)(fw1 >> 16) - 4 * b, 0), 4);
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+                const int s = 2 * half + t;
+#pragma unroll
+                for (int f = 0; f < 2; f++) {
+                    if (s < (f ? s10 : s00) || s >= (f ? s11 : s01)) continue;
+                    const float as = fr[f][t] + fi[f][t];
+#pragma unroll
+                    for (int j = 0; j < NJ16; j++) {
+                        const float bd = xb[j][t] - xa[j][t], bs = xa[j][t] + xb[j][t];
+                        acc16_set(0, f, j, __builtin_amdgcn_mfma_f32_16x16x4f32(as, xa[j][t], acc16(0, f, j), 0, 0, 0));
+                        acc16_set(1, f, j, __builtin_amdgcn_mfma_f32_16x16x4f32(fr[f][t], bd, acc16(1, f, j), 0, 0, 0));
+                        acc16_set(2, f, j, __builtin_amdgcn_mfma_f32_16x16x4f32(fi[f][t], bs, acc16(2, f, j), 0, 0, 0));
+                    }
+                }
+            }
+        };
         // mid(): the next stage's LDS stores and loads, issued between the block's
         // matrix products so that they overlap the products instead of following them.
-        // b: the block's index in its half of the operand (F16: the k-step windows).
+        // (3xBF16; the FP32 form's block runs in iter below.)
         auto mma_block = [&](int buf, int b, auto &&mid) {
             if constexpr (BF) {
                 const __bf16 *sa = reinterpret_cast<const __bf16 *>(s_a[buf]);
@@ -700,44 +774,6 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
                 for (int j = 0; j < NJ; j++) {
                     acc[0][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nih, bh[j], acc[0][j], 0, 0, 0);
                     acc[1][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh, bh[j], acc[1][j], 0, 0, 0);
-                }
-            } else {
-                // k-step s (0..3) of fragment f runs if 4 b + s lies in the fragment's
-                // window [lo, hi) (units of 4 complex columns)
-                const int s00 = min(max((int)(fw0 & 0xffffu) - 4 * b, 0), 4);
-                const int s01 = min(max((int)(fw0 >> 16) - 4 * b, 0), 4);
-                const int s10 = min(max((int)(fw1 & 0xffffu) - 4 * b, 0), 4);
-                const int s11 = min(max((int)(fw1 >> 16) - 4 * b, 0), 4);
-                const float *sa = reinterpret_cast<const float *>(s_a[buf]);
-                const float *sb = reinterpret_cast<const float *>(s_b[buf]);
-                floatx4 fre[2], fim[2], xa[NJ16], xb[NJ16];
-#pragma unroll
-                for (int f = 0; f < 2; f++) {
-                    const float *pa = &sa[(wm * 32 + f * 16 + l16) * CP_PITCH + 4 * q4];
-                    fre[f] = *reinterpret_cast<const floatx4 *>(pa);
-                    fim[f] = *reinterpret_cast<const floatx4 *>(pa + kCpBlock);
-                }
-#pragma unroll
-                for (int j = 0; j < NJ16; j++) {
-                    const float *pb = &sb[(wn * (BN / NWN) + j * 16 + l16) * CP_PITCH + 4 * q4];
-                    xa[j] = *reinterpret_cast<const floatx4 *>(pb);
-                    xb[j] = *reinterpret_cast<const floatx4 *>(pb + kCpBlock);
-                }
-#pragma unroll
-                for (int s = 0; s < 4; s++) {
-                    if (s == 2) mid();
-#pragma unroll
-                    for (int f = 0; f < 2; f++) {
-                        if (s < (f ? s10 : s00) || s >= (f ? s11 : s01)) continue;
-                        const float as = fre[f][s] + fim[f][s];
-#pragma unroll
-                        for (int j = 0; j < NJ16; j++) {
-                            const float bd = xb[j][s] - xa[j][s], bs = xa[j][s] + xb[j][s];
-                            acc16_set(0, f, j, __builtin_amdgcn_mfma_f32_16x16x4f32(as, xa[j][s], acc16(0, f, j), 0, 0, 0));
-                            acc16_set(1, f, j, __builtin_amdgcn_mfma_f32_16x16x4f32(fre[f][s], bd, acc16(1, f, j), 0, 0, 0));
-                            acc16_set(2, f, j, __builtin_amdgcn_mfma_f32_16x16x4f32(fim[f][s], bs, acc16(2, f, j), 0, 0, 0));
-                        }
-                    }
                 }
             }
         };
@@ -824,14 +860,22 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
             if (tid < r.ncol)
                 gst<double>(normpart + (size_t)s_tr[tid] * R + rt0 + i * nch, s_norm[i & 1][0][tid] + s_norm[i & 1][1][tid]);
         };
-        // issued stage by stage (not interleaved), as the loop body expects them
+        // issued stage by stage (not interleaved), as the loop body expects them.
+        // 3xBF16: block 0 in LDS, blocks 1 and 2 in registers; FP32: blocks 0 and 1
+        // in LDS (and block 0's lo fragments read), blocks 2 and 3 in registers.
         gload(st0);
         __builtin_amdgcn_sched_barrier(0);
         gload(st1);
         __builtin_amdgcn_sched_barrier(0);
         lstore(st0, 0);
         gload(st0);
+        if constexpr (F16) {
+            __builtin_amdgcn_sched_barrier(0);
+            lstore(st1, 1);
+            gload(st1);
+        }
         __syncthreads();
+        if constexpr (F16) read_half(0, 0, fr_lo, fi_lo, xa_lo, xb_lo);
         load_fw(0);
         // MMA cursor (wave-uniform): row tile mrt, block mkb of its band; pend = row
         // tile whose norm partial waits in s_norm
@@ -841,19 +885,30 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
         // refill its registers with block f + 3.  The stores drain while the second
         // half of the products runs, and the next block's fragment reads find no stores
         // ahead of them (against storing before the products: 3xBF16 1.8% and FP32
-        // 1.6% faster; against storing after them: FP32 1.6% faster)
+        // 1.6% faster; against storing after them: FP32 1.6% faster).  FP32: buffer
+        // f % 3 holds block f; half-way, block f + 2 is stored into buffer (f + 2) % 3
+        // (block f - 1's, whose reads ended before the previous barrier), its registers
+        // take block f + 4, and block f + 1's lo fragments are read (stored one
+        // iteration ago, visible since the previous barrier).
         auto iter = [&](Stage &nxt, int f) {
-            const int buf = f & 1;
-            // the next unit is claimed during the last block and resolved after it
-            if (f == nbt - 1 && wid == 0) cn = tile_claim(a);
+            const int buf = F16 ? f % 3 : f & 1;
+            // the next unit is claimed three blocks before this one's end (its atomic has
+            // returned by then) and resolved after the last block
+            if (f == (nbt > 3 ? nbt - 3 : 0) && wid == 0) cn = tile_claim(a);
             // keep the next stage's conversions out of this iteration's products:
             // hoisted, they would wait for loads that are meant to stay in flight
             __builtin_amdgcn_sched_barrier(0);
             const int b = byte_at(bk0, mrt) + mkb;
             auto mid = [&]() {
                 __builtin_amdgcn_sched_barrier(0);
-                lstore(nxt, buf ^ 1);
-                gload(nxt);
+                if constexpr (F16) {
+                    lstore(nxt, buf == 0 ? 2 : buf - 1);
+                    gload(nxt);
+                    read_half(buf == 2 ? 0 : buf + 1, 0, fr_lo, fi_lo, xa_lo, xb_lo);
+                } else {
+                    lstore(nxt, buf ^ 1);
+                    gload(nxt);
+                }
                 __builtin_amdgcn_sched_barrier(0);
             };
             // the wave's fragments multiply only the blocks inside their band ranges
@@ -866,8 +921,16 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
                 const int fh = wm ? byte_at(bh1, mrt) : byte_at(bh0, mrt);
                 act = b >= fl && b < fh;
             }
-            if (act) mma_block(buf, b, mid);
-            else mid();
+            if constexpr (F16) {
+                if (act) read_half(buf, 1, fr_hi, fi_hi, xa_hi, xb_hi);
+                __builtin_amdgcn_sched_barrier(0);
+                if (act) f16_steps(b, 0, fr_lo, fi_lo, xa_lo, xb_lo);
+                mid();
+                if (act) f16_steps(b, 1, fr_hi, fi_hi, xa_hi, xb_hi);
+            } else {
+                if (act) mma_block(buf, b, mid);
+                else mid();
+            }
             __syncthreads();
             if (pend >= 0) {
                 write_norm(pend);
@@ -883,13 +946,15 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
                 mkb++;
             }
         };
+        // the stage holding the block stored next: 3xBF16 block f + 1, FP32 block f + 2
+        Stage &se = F16 ? st0 : st1, &so = F16 ? st1 : st0;
         int f = 0;
         for (; f + 1 < nbt; f += 2) {
-            iter(st1, f);
-            iter(st0, f + 1);
+            iter(se, f);
+            iter(so, f + 1);
         }
-        if (f < nbt) iter(st1, f);
-        if (wid == 0) tile_resolve<BN>(a, s_tpre, nitems, total, cn, s_claim);
+        if (f < nbt) iter(se, f);
+        if (wid == 0) tile_resolve<BN>(a, s_tpre, nitems, total, cn, v1, s_claim);
         __syncthreads();
         write_norm(pend);
         // executed matrix-core flops in units of 128: FP32 3 products x 2 x 16 x 4 per
@@ -930,7 +995,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(BN == 64 ? 
     const int cp_k = (tid & 15) * 2, cp_r = tid >> 4;
     __shared__ int s_claim[5];
     TileRef r;
-    if (wid == 0) tile_resolve<BN>(a, s_tpre, nitems, total, tile_claim(a), s_claim);
+    const int v1 = wid == 0 ? search_l1(s_tpre, nitems) : 0;
+    if (wid == 0) tile_resolve<BN>(a, s_tpre, nitems, total, tile_claim(a), v1, s_claim);
     __syncthreads();
     while (tile_read(s_claim, r)) {
         load_cols<BN>(a, r, s_tr, s_cur);
@@ -1034,7 +1100,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(BN == 64 ? 
                 }
             }
         }
-        if (wid == 0) tile_resolve<BN>(a, s_tpre, nitems, total, cn, s_claim);
+        if (wid == 0) tile_resolve<BN>(a, s_tpre, nitems, total, cn, v1, s_claim);
 #pragma unroll
         for (int j = 0; j < NJ; j++) {
             ns[j] += __shfl_xor(ns[j], 16, 64);
