@@ -1272,13 +1272,18 @@ int runner_create(int prec, int gemm_kind, hipStream_t stream, int capacity, Run
     // four (the same results: the tile order does not enter any sum)
     r->sc = 2;
     if (const char *env = std::getenv("QJX_SC")) r->sc = std::max(1, std::min(16, std::atoi(env)));
-    // complex-pair GEMM: chunks of two row tiles per work unit.  On the d = 1001
-    // workload (1e5 trajectories) FP32 runs 4.8% and 3xBF16 2.7% faster than with
-    // one row tile per unit; chunks of 4 and 8 are slower than 2 (the same results:
-    // the chunking does not enter any sum)
+    // complex-pair GEMM: chunks of two row tiles per work unit to start with.  On the
+    // d = 1001 workload (1e5 trajectories) FP32 runs 4.8% and 3xBF16 2.7% faster than
+    // with one row tile per unit, and chunks of 4 and 8 are slower than 2; small tables
+    // run faster with one, so the tuner tries both (the same results: the chunking
+    // does not enter any sum)
     r->rc = cp_layout(prec, gemm_kind) ? 2 : 1;
+    r->tune_rc = cp_layout(prec, gemm_kind);
     if (const char *env = std::getenv("QJX_RC"))
-        if (cp_layout(prec, gemm_kind)) r->rc = std::max(1, std::min(8, std::atoi(env)));
+        if (cp_layout(prec, gemm_kind)) {
+            r->rc = std::max(1, std::min(8, std::atoi(env)));
+            r->tune_rc = false;
+        }
     r->grid.decide = std::max(1, (capacity + 255) / 256);   // one entry per thread
     r->grid.jump = std::max(1, std::min(capacity, 2048));
     if (hipMalloc((void **)&r->d_ens, kMaxEns * sizeof(Ens)) != hipSuccess ||
@@ -1360,11 +1365,13 @@ static RunArgs run_args(Runner *r, int parity)
     return a;
 }
 
-void runner_apply_shape(Runner *r, int bn, int wg_per_cu, int waves)
+void runner_apply_shape(Runner *r, int bn, int wg_per_cu, int waves, int rc)
 {
     int g = r->cus * std::max(1, wg_per_cu);
     g = std::max(8, g / 8 * 8);          // a multiple of the 8 XCDs
-    if (r->bn == bn && r->grid.gemm == g && r->grid.waves == waves) return;
+    if (rc <= 0) rc = r->rc;
+    if (r->bn == bn && r->grid.gemm == g && r->grid.waves == waves && r->rc == rc) return;
+    r->rc = rc;
     r->bn = bn;
     r->wg_per_cu = std::max(1, wg_per_cu);
     r->grid.gemm = g;
@@ -1391,7 +1398,7 @@ struct ShapeKey {
 };
 static std::mutex g_shape_mu;
 struct Shape {
-    int bn, wpc, waves;
+    int bn, wpc, waves, rc;
 };
 static std::map<ShapeKey, Shape> g_shapes;
 
@@ -1534,35 +1541,43 @@ int runner_autotune(Runner *r)
         std::lock_guard<std::mutex> lk(g_shape_mu);
         auto it = g_shapes.find(key);
         if (it != g_shapes.end()) {
-            runner_apply_shape(r, it->second.bn, it->second.wpc, it->second.waves);
+            runner_apply_shape(r, it->second.bn, it->second.wpc, it->second.waves, it->second.rc);
             r->tuned = true;
             return QJX_OK;
         }
     }
     struct Cand {
-        int bn, wpc, waves;
+        int bn, wpc, waves, rc;
         double ms;
         unsigned long long cols;
     };
     std::vector<Cand> cand;
+    // row tiles per complex-pair work unit: two suit large tables (d = 1001, 1e5
+    // trajectories: 7% faster than one), one suits small ones, whose few units leave
+    // most workgroups idle (d = 101 to 501 with 1000 trajectories: 26-38% faster)
+    std::vector<int> rcs{r->rc};
+    if (r->tune_rc) rcs = {2, 1};
+    auto add = [&](int bn, int wpc, int waves) {
+        for (int rc : rcs) cand.push_back({bn, wpc, waves, rc, 0.0, 0});
+    };
     for (int bn : {128, 64}) {
         const int occ = gemm_blocks_per_cu(r->prec, r->gemm_kind, bn, 4);
-        cand.push_back({bn, occ, 4, 0.0, 0});
-        if (occ > 1) cand.push_back({bn, occ / 2, 4, 0.0, 0});
+        add(bn, occ, 4);
+        if (occ > 1) add(bn, occ / 2, 4);
     }
     if (r->prec == kBF16x3) {
         const int occ = gemm_blocks_per_cu(r->prec, r->gemm_kind, 128, 8);
-        cand.push_back({128, occ, 8, 0.0, 0});
+        add(128, occ, 8);
     }
     if (r->prec != kFP64) {                // 256-column tiles (8 waves) for the FP32 forms
         const int occ = gemm_blocks_per_cu(r->prec, r->gemm_kind, 256, 8);
-        cand.push_back({256, occ, 8, 0.0, 0});
+        add(256, occ, 8);
     }
-    const int saved_bn = r->bn, saved_wpc = r->wg_per_cu, saved_waves = r->grid.waves;
+    const int saved_bn = r->bn, saved_wpc = r->wg_per_cu, saved_waves = r->grid.waves, saved_rc = r->rc;
     constexpr int kIters = 4, kRounds = 2;
     for (int round = 0; round < kRounds; round++) {
         for (Cand &c : cand) {
-            runner_apply_shape(r, c.bn, c.wpc, c.waves);
+            runner_apply_shape(r, c.bn, c.wpc, c.waves, c.rc);
             double ms = 0.0;
             unsigned long long cols = 0;
             int rc = timed_iterations(r, kIters, &ms, &cols);
@@ -1573,7 +1588,7 @@ int runner_autotune(Runner *r)
             rc = runner_next_total(r, &total);
             if (rc) return rc;
             if (total == 0) {      // the table ran out of work: try again next time
-                runner_apply_shape(r, saved_bn, saved_wpc, saved_waves);
+                runner_apply_shape(r, saved_bn, saved_wpc, saved_waves, saved_rc);
                 return QJX_OK;
             }
         }
@@ -1582,13 +1597,13 @@ int runner_autotune(Runner *r)
     for (const Cand &c : cand)
         if (c.cols > 0 && (!best || c.ms / c.cols < best->ms / best->cols)) best = &c;
     if (!best) {
-        runner_apply_shape(r, saved_bn, saved_wpc, saved_waves);
+        runner_apply_shape(r, saved_bn, saved_wpc, saved_waves, saved_rc);
         return QJX_OK;
     }
-    runner_apply_shape(r, best->bn, best->wpc, best->waves);
+    runner_apply_shape(r, best->bn, best->wpc, best->waves, best->rc);
     r->tuned = true;
     std::lock_guard<std::mutex> lk(g_shape_mu);
-    g_shapes[key] = {best->bn, best->wpc, best->waves};
+    g_shapes[key] = {best->bn, best->wpc, best->waves, best->rc};
     return QJX_OK;
 }
 

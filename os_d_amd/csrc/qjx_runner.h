@@ -26,6 +26,7 @@ struct Runner {
     int wg_per_cu = 1;                // persistent GEMM workgroups per compute unit
     int sc = 1;                       // column tiles per super-column of the tile order
     int rc = 1;                       // row tiles per GEMM work unit (complex-pair GEMM)
+    bool tune_rc = false;             // the tuner also picks rc (1 or 2)
     bool autotune = true;             // pick (bn, wg_per_cu) by timing the live table
     bool tuned = false;
     int nens = 0;
@@ -56,9 +57,9 @@ int runner_run(Runner *r, long long *iterations);
 // on the table's own iterations (they are real steps; results do not depend on
 // the shape).  No-op once tuned, for the FMA kernel, or with autotune off.
 int runner_autotune(Runner *r);
-// Use tile width bn, wg_per_cu workgroups per CU and `waves` waves per GEMM
-// workgroup (drops a captured graph).
-void runner_apply_shape(Runner *r, int bn, int wg_per_cu, int waves);
+// Use tile width bn, wg_per_cu workgroups per CU, `waves` waves per GEMM workgroup
+// and (rc > 0) rc row tiles per work unit (drops a captured graph when any changes).
+void runner_apply_shape(Runner *r, int bn, int wg_per_cu, int waves, int rc = 0);
 
 void set_last_error(const std::string &msg);
 

@@ -82,6 +82,22 @@ def test_tile_width_does_not_change_results(prec, monkeypatch):
         _same_state(res["64"], res[bn])
 
 
+@pytest.mark.parametrize("prec", [1, 2])
+def test_row_tiles_per_unit_do_not_change_results(prec, monkeypatch):
+    # the complex-pair GEMM's work units (1, 2 or 3 row tiles, which the tuner picks
+    # from) must not enter any sum
+    _say("case: row tiles per unit 1, 2, 3 prec=%d" % prec)
+    res = {}
+    for rc in ("1", "2", "3"):
+        monkeypatch.setenv("QJX_RC", rc)
+        e = _engine(prec, N=300, M=500, seed=5, drv_ampl=1.5, diss_gamma=0.1)
+        e.run_steps(5, 0)
+        res[rc] = e.get_state()
+        e.close()
+    for rc in ("2", "3"):
+        _same_state(res["1"], res[rc])
+
+
 def test_autotuner_picks_a_shape_with_identical_results(monkeypatch):
     _say("case: autotuner")
     monkeypatch.setenv("QJX_BN", "128")
