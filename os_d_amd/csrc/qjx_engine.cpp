@@ -1253,7 +1253,8 @@ int runner_create(int prec, int gemm_kind, hipStream_t stream, int capacity, Run
         if (std::string(env) == "0") r->autotune = false;
     if (const char *env = std::getenv("QJX_BN")) {
         const std::string v(env);
-        if (gemm_kind != kGemmFma && (v == "64" || v == "128" || (v == "256" && prec != kFP64))) {
+        if (gemm_kind != kGemmFma &&
+            (v == "64" || v == "128" || (v == "256" && prec != kFP64) || (v == "32" && prec == kFP64))) {
             bn = std::stoi(v);
             r->autotune = false;
         }
@@ -1422,6 +1423,19 @@ static int read_cols(Runner *r, unsigned long long *cols)
     return QJX_OK;
 }
 
+// Matrix-core work the table's GEMMs have executed so far (units of 128 flops)
+static int read_work(Runner *r, unsigned long long *work)
+{
+    unsigned long long sum = 0, v = 0;
+    for (const Ens &E : r->h_ens) {
+        HIP_TRY(hipMemcpyAsync(&v, E.counters + 2, sizeof v, hipMemcpyDeviceToHost, r->stream));
+        HIP_TRY(hipStreamSynchronize(r->stream));
+        sum += v;
+    }
+    *work = sum;
+    return QJX_OK;
+}
+
 // Optional events recorded around the launches of one iteration.
 struct IterEvents {
     hipEvent_t before_plan = nullptr, before_gemm = nullptr, after_gemm = nullptr, after_tail = nullptr;
@@ -1499,16 +1513,17 @@ static int capture_iterations(Runner *r)
 
 // `iters` iterations with the GEMM bracketed by events: device ms of the GEMM
 // launches and list entries processed.
-static int timed_iterations(Runner *r, int iters, double *ms, unsigned long long *cols)
+static int timed_iterations(Runner *r, int iters, double *ms, unsigned long long *cols, unsigned long long *work)
 {
     if (r->ev.size() < 2 * (size_t)iters) {
         const size_t old = r->ev.size();
         r->ev.resize(2 * (size_t)iters);
         for (size_t i = old; i < r->ev.size(); i++) HIP_TRY(hipEventCreate(&r->ev[i]));
     }
-    unsigned long long c0 = 0, c1 = 0;
+    unsigned long long c0 = 0, c1 = 0, w0 = 0, w1 = 0;
     int rc = read_cols(r, &c0);
     if (rc) return rc;
+    if ((rc = read_work(r, &w0))) return rc;
     for (int k = 0; k < iters; k++) {
         IterEvents ev;
         ev.before_gemm = r->ev[2 * k];
@@ -1517,6 +1532,7 @@ static int timed_iterations(Runner *r, int iters, double *ms, unsigned long long
     }
     rc = read_cols(r, &c1);
     if (rc) return rc;
+    if ((rc = read_work(r, &w1))) return rc;
     double t = 0.0;
     for (int k = 0; k < iters; k++) {
         float g = 0.f;
@@ -1525,6 +1541,7 @@ static int timed_iterations(Runner *r, int iters, double *ms, unsigned long long
     }
     *ms = t;
     *cols = c1 - c0;
+    *work = w1 - w0;
     r->iterations += iters;
     if (r->profiling) {
         r->gemm_ms += t;
@@ -1549,7 +1566,7 @@ int runner_autotune(Runner *r)
     struct Cand {
         int bn, wpc, waves, rc;
         double ms;
-        unsigned long long cols;
+        unsigned long long cols, work;
     };
     std::vector<Cand> cand;
     // row tiles per complex-pair work unit: two suit large tables (d = 1001, 1e5
@@ -1558,9 +1575,12 @@ int runner_autotune(Runner *r)
     std::vector<int> rcs{r->rc};
     if (r->tune_rc) rcs = {2, 1};
     auto add = [&](int bn, int wpc, int waves) {
-        for (int rc : rcs) cand.push_back({bn, wpc, waves, rc, 0.0, 0});
+        for (int rc : rcs) cand.push_back({bn, wpc, waves, rc, 0.0, 0, 0});
     };
-    for (int bn : {128, 64}) {
+    // FP64: also 32-column tiles, whose smaller tiles give small tables more workgroups
+    std::vector<int> bns{128, 64};
+    if (r->prec == kFP64) bns.push_back(32);
+    for (int bn : bns) {
         const int occ = gemm_blocks_per_cu(r->prec, r->gemm_kind, bn, 4);
         add(bn, occ, 4);
         if (occ > 1) add(bn, occ / 2, 4);
@@ -1574,16 +1594,17 @@ int runner_autotune(Runner *r)
         add(256, occ, 8);
     }
     const int saved_bn = r->bn, saved_wpc = r->wg_per_cu, saved_waves = r->grid.waves, saved_rc = r->rc;
-    constexpr int kIters = 4, kRounds = 2;
+    constexpr int kIters = 4, kRounds = 3;
     for (int round = 0; round < kRounds; round++) {
         for (Cand &c : cand) {
             runner_apply_shape(r, c.bn, c.wpc, c.waves, c.rc);
             double ms = 0.0;
-            unsigned long long cols = 0;
-            int rc = timed_iterations(r, kIters, &ms, &cols);
+            unsigned long long cols = 0, work = 0;
+            int rc = timed_iterations(r, kIters, &ms, &cols, &work);
             if (rc) return rc;
             c.ms += ms;
             c.cols += cols;
+            c.work += work;
             int total = 0;
             rc = runner_next_total(r, &total);
             if (rc) return rc;
@@ -1593,9 +1614,14 @@ int runner_autotune(Runner *r)
             }
         }
     }
+    // GEMM time per executed matrix-core flop: a column of a wide-band group costs
+    // several of a deep level's, and each candidate's trial iterations hold a
+    // different mix of groups, so time per column would favour whichever candidate
+    // drew the narrow ones (time per column when no work was counted)
+    auto cost = [](const Cand &c) { return c.work > 0 ? c.ms / (double)c.work : c.ms / (double)c.cols; };
     const Cand *best = nullptr;
     for (const Cand &c : cand)
-        if (c.cols > 0 && (!best || c.ms / c.cols < best->ms / best->cols)) best = &c;
+        if (c.cols > 0 && (!best || cost(c) < cost(*best))) best = &c;
     if (!best) {
         runner_apply_shape(r, saved_bn, saved_wpc, saved_waves, saved_rc);
         return QJX_OK;

@@ -971,13 +971,13 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
 // GEMM tile, FP64: v_mfma_f64_16x16x4f64
 //   A (16 x 4): lane l holds A[l % 16][l / 16]; B (4 x 16): B[l / 16][l % 16]
 //   D (16 x 16): lane l, register r holds D[(l / 16) + 4 r][l % 16]
-// 8 waves as 4 x 2, wave tile 32 x BN/2, WG tile 128 x BN (BN = 128 or 64); within
+// 8 waves as 4 x 2, wave tile 32 x BN/2, WG tile 128 x BN (BN = 128, 64 or 32); within
 // a K block of 32, MFMA step s (0..7) takes k = s + 8 q for lane quarter q.
 // ---------------------------------------------------------------------------
 constexpr int F64_PITCH = MF_BK + 2;
 
 template <int BN>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(BN == 64 ? 4 : 1))) void gemm_f64(RunArgs a)
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(BN <= 64 ? 4 : 1))) void gemm_f64(RunArgs a)
 {
     constexpr int NJ = BN / 32;          // 16-column MFMA tiles per wave
     constexpr int NQ = BN / 32;          // B-tile rows (columns of X) copied per thread
@@ -1121,14 +1121,16 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(BN == 64 ? 
 
 void launch_gemm_f64(const RunArgs &a, const StepGrid &g, hipStream_t s)
 {
-    if (a.bn == 64) gemm_f64<64><<<g.gemm, 512, 0, s>>>(a);
+    if (a.bn == 32) gemm_f64<32><<<g.gemm, 512, 0, s>>>(a);
+    else if (a.bn == 64) gemm_f64<64><<<g.gemm, 512, 0, s>>>(a);
     else gemm_f64<128><<<g.gemm, 512, 0, s>>>(a);
 }
 
 hipError_t gemm_f64_occupancy(int *n, int bn)
 {
-    return bn == 64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(n, gemm_f64<64>, 512, 0)
-                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(n, gemm_f64<128>, 512, 0);
+    return bn == 32   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(n, gemm_f64<32>, 512, 0)
+           : bn == 64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(n, gemm_f64<64>, 512, 0)
+                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(n, gemm_f64<128>, 512, 0);
 }
 #endif  // QJX_STEP_F64
 #if !defined(QJX_STEP_F64)

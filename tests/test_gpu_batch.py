@@ -3,8 +3,9 @@
 * A batch advances several ensembles of different Hilbert dimension in the same
   grouped GEMMs; each member's trajectories must end bitwise identical to a run
   of that ensemble alone (states, etas, times, jump counts, RNG states).
-* The step GEMM's column tile width (64 or 128) and persistent grid are picked
-  by the launch-shape tuner; every shape must give bitwise identical results.
+* The step GEMM's column tile width (32 to 256), persistent grid and row tiles
+  per work unit are picked by the launch-shape tuner; every shape must give
+  bitwise identical results.
 """
 import os
 import sys
@@ -67,9 +68,9 @@ def test_batch_member_equals_alone(prec):
 
 @pytest.mark.parametrize("prec", [1, 0, 2])
 def test_tile_width_does_not_change_results(prec, monkeypatch):
-    _say("case: tile width 64 vs 128 prec=%d" % prec)
+    _say("case: tile widths %s prec=%d" % ("32/64/128" if prec == 0 else "64/128/256", prec))
     res = {}
-    widths = ("64", "128", "256") if prec != 0 else ("64", "128")
+    widths = ("64", "128", "256") if prec != 0 else ("32", "64", "128")
     for bn in widths:
         monkeypatch.setenv("QJX_BN", bn)
         e = _engine(prec, N=150, M=300, seed=11, drv_ampl=1.5, diss_gamma=0.1)
@@ -79,7 +80,7 @@ def test_tile_width_does_not_change_results(prec, monkeypatch):
         res[bn] = e.get_state()
         e.close()
     for bn in widths[1:]:
-        _same_state(res["64"], res[bn])
+        _same_state(res[widths[0]], res[bn])
 
 
 @pytest.mark.parametrize("prec", [1, 2])
