@@ -416,8 +416,6 @@ __device__ inline int bf_off(int row, int k)   // element offset of (row, k), k 
 // The k order, and the row split over waves, do not depend on BN or on the unit,
 // so results do not depend on where a column lands, on the tile width or on rc.
 // ---------------------------------------------------------------------------
-// floats per FP32 LDS row: 8-byte reads of 16 consecutive rows fall in distinct banks
-constexpr int CP_PITCH = 2 * kCpBlock + 2;
 constexpr int kCpChunkMax = 8;               // row tiles per work unit at most (bytes of a 64-bit word)
 
 typedef unsigned uintx4 __attribute__((ext_vector_type(4)));
@@ -449,6 +447,13 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
     static_assert(FM == kCpBF16x3 || FM == kCpF32x16, "form");
     constexpr bool BF = FM == kCpBF16x3;     // 3xBF16: 32x32x16 products, 32-row fragments
     constexpr bool F16 = FM == kCpF32x16;    // FP32: 16x16x4 products, 16-row fragments
+    // FP32 with 8 waves (BN = 256): three LDS buffers and each block's fragments read
+    // ahead in two 8-byte halves (RA), 34-float LDS rows, where 8-byte reads of 16
+    // consecutive rows fall in distinct banks.  The 4-wave FP32 kernels keep two
+    // buffers (their LDS then fits two workgroups per CU) and read a block's fragments
+    // as it starts, 16 bytes at a time, from 36-float rows.
+    constexpr bool RA = F16 && NWN == 4;
+    constexpr int CP_PITCH = RA ? 2 * kCpBlock + 2 : 2 * kCpBlock + 4;
     constexpr int T = 128 * NWN;             // 2 x NWN waves
     constexpr int NJ = BN / (32 * NWN);      // 32-column MFMA tiles per wave (16-column: 2 NJ)
     constexpr int NJ16 = 2 * NJ;
@@ -464,9 +469,9 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
     constexpr int A_ELEMS = BF ? 2 * kCpRows * BF_ROW : kCpRows * CP_PITCH;   // per buffer (hi, lo)
     constexpr int B_ELEMS = BF ? 2 * BN * BF_ROW : BN * CP_PITCH;
     using LT = std::conditional_t<BF, __bf16, float>;
-    // LDS buffers: FP32 keeps three (a block is stored two iterations before its
+    // LDS buffers: three with read-ahead (a block is stored two iterations before its
     // products, so the next block's fragments can be read during this one's)
-    constexpr int NBUF = F16 ? 3 : 2;
+    constexpr int NBUF = RA ? 3 : 2;
     __shared__ __attribute__((aligned(16))) LT s_a[NBUF][A_ELEMS];
     __shared__ __attribute__((aligned(16))) LT s_b[NBUF][B_ELEMS];
     __shared__ int s_tr[BN];
@@ -670,10 +675,11 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
                 fw1 = __builtin_amdgcn_readfirstlane(s_fw[i][2 * wm + 1]);
             }
         };
-        // FP32 fragments of a block in two halves, k-steps 0, 1 (lo) and 2, 3 (hi): the
-        // wave's two 16-row fragments (Re and Im of U) and its 16-column tiles (a and b
-        // of x).  A block's hi half is read as its products start and its lo half during
-        // the previous block's second half, so every LDS read overlaps matrix products.
+        // FP32 with read-ahead: a block's fragments in two halves, k-steps 0, 1 (lo) and
+        // 2, 3 (hi): the wave's two 16-row fragments (Re and Im of U) and its 16-column
+        // tiles (a and b of x).  A block's hi half is read as its products start and its
+        // lo half during the previous block's second half, so every LDS read overlaps
+        // matrix products.
         float2 fr_lo[2], fi_lo[2], fr_hi[2], fi_hi[2];
         float2 xa_lo[NJ16], xb_lo[NJ16], xa_hi[NJ16], xb_hi[NJ16];
         auto read_half = [&](int buf, int half, float2 (&fr)[2], float2 (&fi)[2], float2 (&xa)[NJ16],
@@ -720,7 +726,8 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
         };
         // mid(): the next stage's LDS stores and loads, issued between the block's
         // matrix products so that they overlap the products instead of following them.
-        // (3xBF16; the FP32 form's block runs in iter below.)
+        // b: the block's index in its half of the operand (FP32: the k-step windows).
+        // (3xBF16 and FP32 without read-ahead; with it the block runs in iter below.)
         auto mma_block = [&](int buf, int b, auto &&mid) {
             if constexpr (BF) {
                 const __bf16 *sa = reinterpret_cast<const __bf16 *>(s_a[buf]);
@@ -774,6 +781,44 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
                 for (int j = 0; j < NJ; j++) {
                     acc[0][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nih, bh[j], acc[0][j], 0, 0, 0);
                     acc[1][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh, bh[j], acc[1][j], 0, 0, 0);
+                }
+            } else {
+                // k-step s (0..3) of fragment f runs if 4 b + s lies in the fragment's
+                // window [lo, hi) (units of 4 complex columns)
+                const int s00 = min(max((int)(fw0 & 0xffffu) - 4 * b, 0), 4);
+                const int s01 = min(max((int)(fw0 >> 16) - 4 * b, 0), 4);
+                const int s10 = min(max((int)(fw1 & 0xffffu) - 4 * b, 0), 4);
+                const int s11 = min(max((int)(fw1 >> 16) - 4 * b, 0), 4);
+                const float *sa = reinterpret_cast<const float *>(s_a[buf]);
+                const float *sb = reinterpret_cast<const float *>(s_b[buf]);
+                floatx4 fre[2], fim[2], xa[NJ16], xb[NJ16];
+#pragma unroll
+                for (int f = 0; f < 2; f++) {
+                    const float *pa = &sa[(wm * 32 + f * 16 + l16) * CP_PITCH + 4 * q4];
+                    fre[f] = *reinterpret_cast<const floatx4 *>(pa);
+                    fim[f] = *reinterpret_cast<const floatx4 *>(pa + kCpBlock);
+                }
+#pragma unroll
+                for (int j = 0; j < NJ16; j++) {
+                    const float *pb = &sb[(wn * (BN / NWN) + j * 16 + l16) * CP_PITCH + 4 * q4];
+                    xa[j] = *reinterpret_cast<const floatx4 *>(pb);
+                    xb[j] = *reinterpret_cast<const floatx4 *>(pb + kCpBlock);
+                }
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    if (s == 2) mid();
+#pragma unroll
+                    for (int f = 0; f < 2; f++) {
+                        if (s < (f ? s10 : s00) || s >= (f ? s11 : s01)) continue;
+                        const float as = fre[f][s] + fim[f][s];
+#pragma unroll
+                        for (int j = 0; j < NJ16; j++) {
+                            const float bd = xb[j][s] - xa[j][s], bs = xa[j][s] + xb[j][s];
+                            acc16_set(0, f, j, __builtin_amdgcn_mfma_f32_16x16x4f32(as, xa[j][s], acc16(0, f, j), 0, 0, 0));
+                            acc16_set(1, f, j, __builtin_amdgcn_mfma_f32_16x16x4f32(fre[f][s], bd, acc16(1, f, j), 0, 0, 0));
+                            acc16_set(2, f, j, __builtin_amdgcn_mfma_f32_16x16x4f32(fim[f][s], bs, acc16(2, f, j), 0, 0, 0));
+                        }
+                    }
                 }
             }
         };
@@ -861,21 +906,21 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
                 gst<double>(normpart + (size_t)s_tr[tid] * R + rt0 + i * nch, s_norm[i & 1][0][tid] + s_norm[i & 1][1][tid]);
         };
         // issued stage by stage (not interleaved), as the loop body expects them.
-        // 3xBF16: block 0 in LDS, blocks 1 and 2 in registers; FP32: blocks 0 and 1
-        // in LDS (and block 0's lo fragments read), blocks 2 and 3 in registers.
+        // Two buffers: block 0 in LDS, blocks 1 and 2 in registers; read-ahead: blocks 0
+        // and 1 in LDS (and block 0's lo fragments read), blocks 2 and 3 in registers.
         gload(st0);
         __builtin_amdgcn_sched_barrier(0);
         gload(st1);
         __builtin_amdgcn_sched_barrier(0);
         lstore(st0, 0);
         gload(st0);
-        if constexpr (F16) {
+        if constexpr (RA) {
             __builtin_amdgcn_sched_barrier(0);
             lstore(st1, 1);
             gload(st1);
         }
         __syncthreads();
-        if constexpr (F16) read_half(0, 0, fr_lo, fi_lo, xa_lo, xb_lo);
+        if constexpr (RA) read_half(0, 0, fr_lo, fi_lo, xa_lo, xb_lo);
         load_fw(0);
         // MMA cursor (wave-uniform): row tile mrt, block mkb of its band; pend = row
         // tile whose norm partial waits in s_norm
@@ -885,13 +930,13 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
         // refill its registers with block f + 3.  The stores drain while the second
         // half of the products runs, and the next block's fragment reads find no stores
         // ahead of them (against storing before the products: 3xBF16 1.8% and FP32
-        // 1.6% faster; against storing after them: FP32 1.6% faster).  FP32: buffer
+        // 1.6% faster; against storing after them: FP32 1.6% faster).  Read-ahead: buffer
         // f % 3 holds block f; half-way, block f + 2 is stored into buffer (f + 2) % 3
         // (block f - 1's, whose reads ended before the previous barrier), its registers
         // take block f + 4, and block f + 1's lo fragments are read (stored one
         // iteration ago, visible since the previous barrier).
         auto iter = [&](Stage &nxt, int f) {
-            const int buf = F16 ? f % 3 : f & 1;
+            const int buf = RA ? f % 3 : f & 1;
             // the next unit is claimed three blocks before this one's end (its atomic has
             // returned by then) and resolved after the last block
             if (f == (nbt > 3 ? nbt - 3 : 0) && wid == 0) cn = tile_claim(a);
@@ -901,7 +946,7 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
             const int b = byte_at(bk0, mrt) + mkb;
             auto mid = [&]() {
                 __builtin_amdgcn_sched_barrier(0);
-                if constexpr (F16) {
+                if constexpr (RA) {
                     lstore(nxt, buf == 0 ? 2 : buf - 1);
                     gload(nxt);
                     read_half(buf == 2 ? 0 : buf + 1, 0, fr_lo, fi_lo, xa_lo, xb_lo);
@@ -921,7 +966,7 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
                 const int fh = wm ? byte_at(bh1, mrt) : byte_at(bh0, mrt);
                 act = b >= fl && b < fh;
             }
-            if constexpr (F16) {
+            if constexpr (RA) {
                 if (act) read_half(buf, 1, fr_hi, fi_hi, xa_hi, xb_hi);
                 __builtin_amdgcn_sched_barrier(0);
                 if (act) f16_steps(b, 0, fr_lo, fi_lo, xa_lo, xb_lo);
@@ -946,8 +991,8 @@ __global__ __launch_bounds__(128 * NWN) __attribute__((amdgpu_waves_per_eu(2))) 
                 mkb++;
             }
         };
-        // the stage holding the block stored next: 3xBF16 block f + 1, FP32 block f + 2
-        Stage &se = F16 ? st0 : st1, &so = F16 ? st1 : st0;
+        // the stage holding the block stored next: block f + 1, with read-ahead f + 2
+        Stage &se = RA ? st0 : st1, &so = RA ? st1 : st0;
         int f = 0;
         for (; f + 1 < nbt; f += 2) {
             iter(se, f);
