@@ -13,7 +13,9 @@
 //     A job is admitted while its HBM footprint fits the budget and its
 //     trajectories fit the lane's capacity.
 //   * Two lanes, each its own table and runner: short jobs (expected device work
-//     below a threshold) and long ones.  An iteration's time is set by the largest
+//     below a threshold) and long ones.  The short lane takes in a bounded number
+//     of trajectories (QJX_POOL_SHORT_TRAJ, default 1e5) so that its iterations
+//     stay short.  An iteration's time is set by the largest
 //     jobs in its table, and a trajectory needs one iteration per matrix-vector
 //     step of its bisection chain, so a short job sharing iterations with hour-long
 //     jobs would wait on their GEMMs at every step.  In its own lane it iterates at
@@ -110,6 +112,7 @@ struct qjx_pool {
     } lane[kLanes];
     double short_s = 2.0;             // expected seconds below which a job is short
     double reserve = 0.25;            // budget share long jobs may not take
+    long long short_cap = 100000;     // trajectories the short lane takes in (a larger job: alone)
     std::vector<Job *> jobs;          // by id
     std::vector<int> queue;           // waiting for admission
     std::vector<std::thread> builders;
@@ -184,6 +187,7 @@ extern "C" int qjx_pool_create(int precision, double hbm_budget_gib, int max_tra
     p->t0 = std::chrono::steady_clock::now();
     p->short_s = env_double("QJX_POOL_SHORT_S", 2.0);
     p->reserve = std::min(0.9, std::max(0.0, env_double("QJX_POOL_RESERVE", 0.25)));
+    p->short_cap = std::max(1LL, (long long)env_double("QJX_POOL_SHORT_TRAJ", 1e5));
     size_t freeb = 0, totalb = 0;
     if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) {
         delete p;
@@ -351,9 +355,15 @@ static void admit_queued(qjx_pool *p)
     for (size_t k = 0; k < p->queue.size(); k++) {
         Job *j = p->jobs[p->queue[k]];
         const int l = j->lane;
+        // the short lane's trajectories are capped: its iteration time, which every
+        // short job pays once per step of its bisection chain, grows with them (a job
+        // above the cap runs when the lane is empty)
+        const long long lane_cap = l == kShort ? std::min<long long>(p->capacity, p->short_cap) : p->capacity;
+        const bool room = p->lane[l].in_table + j->e.num_trajectories <= lane_cap ||
+                          (p->lane[l].in_table == 0 && tables[l] == 0);
         const bool fits = p->used + j->footprint <= p->budget &&
                           (l == kShort || p->used_long + j->footprint <= long_cap) &&
-                          p->lane[l].in_table + j->e.num_trajectories <= p->capacity &&
+                          room && p->lane[l].in_table + j->e.num_trajectories <= p->capacity &&
                           tables[l] < qjx::kMaxEns;
         if (!fits) {
             still.push_back(j->id);
