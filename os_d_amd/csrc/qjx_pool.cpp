@@ -15,12 +15,12 @@
 //   * Two lanes, each its own table and runner: short jobs (expected device work
 //     below a threshold) and long ones.  The short lane takes in a bounded number
 //     of trajectories (QJX_POOL_SHORT_TRAJ, default 1e5) so that its iterations
-//     stay short.  An iteration's time is set by the largest
+//     stay short, and gets 3/4 of the device time when both lanes have work
+//     (QJX_POOL_SHORT_SHARE).  An iteration's time is set by the largest
 //     jobs in its table, and a trajectory needs one iteration per matrix-vector
 //     step of its bisection chain, so a short job sharing iterations with hour-long
 //     jobs would wait on their GEMMs at every step.  In its own lane it iterates at
-//     its own pace; when both lanes have work each cycle gives them about equal
-//     device time.  Long jobs may not take the budget share reserved for short ones.
+//     its own pace.  Long jobs may not take the budget share reserved for short ones.
 //   * Models are built by several builder threads, each on the job's own stream,
 //     while the lanes iterate.
 // A trajectory's arithmetic does not depend on which other jobs share the
@@ -113,6 +113,7 @@ struct qjx_pool {
     double short_s = 2.0;             // expected seconds below which a job is short
     double reserve = 0.25;            // budget share long jobs may not take
     long long short_cap = 100000;     // trajectories the short lane takes in (a larger job: alone)
+    double short_share = 0.75;        // device time of the short lane when both lanes have work
     std::vector<Job *> jobs;          // by id
     std::vector<int> queue;           // waiting for admission
     std::vector<std::thread> builders;
@@ -188,6 +189,7 @@ extern "C" int qjx_pool_create(int precision, double hbm_budget_gib, int max_tra
     p->short_s = env_double("QJX_POOL_SHORT_S", 2.0);
     p->reserve = std::min(0.9, std::max(0.0, env_double("QJX_POOL_RESERVE", 0.25)));
     p->short_cap = std::max(1LL, (long long)env_double("QJX_POOL_SHORT_TRAJ", 1e5));
+    p->short_share = std::min(0.95, std::max(0.05, env_double("QJX_POOL_SHORT_SHARE", 0.75)));
     size_t freeb = 0, totalb = 0;
     if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) {
         delete p;
@@ -434,9 +436,10 @@ extern "C" int qjx_pool_step(qjx_pool *p, int *running, int *queued)
     p->cv.notify_all();
     if (busy[kShort] || busy[kLong]) {
         if (hipStreamSynchronize(p->stream) != hipSuccess) return pool_fail(QJX_ERR_HIP, "pool iteration failed");
-        // Equal device time for the two lanes when both have work: the long lane's
-        // chunk stays short enough to keep cycles near 0.2 s, and the short lane's
-        // chunk is sized to last about as long.
+        // When both lanes have work the short lane gets short_share of the device time
+        // (default 3/4: its jobs can finish within minutes, the long lane's take hours):
+        // the long lane's chunk stays short enough to keep cycles near 0.2 s, and the
+        // short lane's chunk is sized to last share / (1 - share) times as long.
         float ms[kLanes] = {0.f, 0.f};
         for (int l = 0; l < kLanes; l++)
             if (busy[l] && hipEventElapsedTime(&ms[l], p->lane[l].ev0, p->lane[l].ev1) != hipSuccess)
@@ -449,7 +452,8 @@ extern "C" int qjx_pool_step(qjx_pool *p, int *running, int *queued)
         }
         if (busy[kShort] && ms[kShort] > 0.f) {
             const double per = ms[kShort] / S.iters;
-            const double target = busy[kLong] ? std::max((double)ms[kLong], 2.0) : 16.0 * per;
+            const double ratio = p->short_share / (1.0 - p->short_share);
+            const double target = busy[kLong] ? std::max((double)ms[kLong] * ratio, 2.0) : 16.0 * per;
             S.iters = std::min(1024, std::max(16, even(target / std::max(per, 1e-3))));
         }
         // a job is finished when its lists for the next iteration are empty
