@@ -110,7 +110,7 @@ struct qjx_pool {
         int iters = 16;               // iterations of the lane's next chunk
         hipEvent_t ev0 = nullptr, ev1 = nullptr;
     } lane[kLanes];
-    double short_s = 2.0;             // expected seconds below which a job is short
+    double short_s = 0.5;             // expected seconds below which a job is short
     double reserve = 0.25;            // budget share long jobs may not take
     long long short_cap = 100000;     // trajectories the short lane takes in (a larger job: alone)
     double short_share = 0.75;        // device time of the short lane when both lanes have work
@@ -186,7 +186,7 @@ extern "C" int qjx_pool_create(int precision, double hbm_budget_gib, int max_tra
     p->prec = precision;
     p->capacity = max_trajectories;
     p->t0 = std::chrono::steady_clock::now();
-    p->short_s = env_double("QJX_POOL_SHORT_S", 2.0);
+    p->short_s = env_double("QJX_POOL_SHORT_S", 0.5);
     p->reserve = std::min(0.9, std::max(0.0, env_double("QJX_POOL_RESERVE", 0.25)));
     p->short_cap = std::max(1LL, (long long)env_double("QJX_POOL_SHORT_TRAJ", 1e5));
     p->short_share = std::min(0.95, std::max(0.05, env_double("QJX_POOL_SHORT_SHARE", 0.75)));
