@@ -1688,6 +1688,20 @@ int runner_next_total(Runner *r, int *total)
     return QJX_OK;
 }
 
+// Iterations a runner runs before it tunes its launch shape (chunks of 8, 16, ...,
+// so tuning starts at the first chunk boundary past this count).  On the d = 1001
+// workload (1e5 trajectories, about 90 iterations per top-level step) tuning after
+// the first 8 iterations picked 128-column tiles for FP32, 3% slower over a period
+// than the 256-column tiles it picks after 512.  QJX_TUNE_AFTER overrides it.
+static long long tune_after()
+{
+    static const long long n = [] {
+        const char *env = std::getenv("QJX_TUNE_AFTER");
+        return env ? std::max(0LL, std::atoll(env)) : 512LL;
+    }();
+    return n;
+}
+
 int runner_run(Runner *r, long long *iterations)
 {
     int chunk = 8;
@@ -1700,8 +1714,10 @@ int runner_run(Runner *r, long long *iterations)
         rc = runner_next_total(r, &total);
         if (rc) return rc;
         if (total == 0) break;
-        if (!r->tuned && r->autotune) {
-            // shape the GEMM once the bisection has spread the table over its groups
+        if (!r->tuned && r->autotune && r->iterations >= tune_after()) {
+            // shape the GEMM once the bisection has spread the table over its groups:
+            // the first iterations from a fresh start hold mostly shallow levels (wide
+            // bands), where other shapes win than in the run's steady mix of deep ones
             const long long before = r->iterations;
             rc = runner_autotune(r);
             if (rc) return rc;

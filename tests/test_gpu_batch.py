@@ -108,6 +108,8 @@ def test_autotuner_picks_a_shape_with_identical_results(monkeypatch):
     e.close()
     monkeypatch.delenv("QJX_BN")
     monkeypatch.delenv("QJX_TUNE", raising=False)
+    # tune right after the first chunk (a run this short ends before the default 512)
+    monkeypatch.setenv("QJX_TUNE_AFTER", "0")
     e = _engine(1, N=200, M=2000, seed=2, drv_ampl=1.5, diss_gamma=0.2)
     e.run_steps(12, 0)
     s = e.stats()
