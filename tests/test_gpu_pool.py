@@ -75,6 +75,32 @@ def test_admission_by_budget_queues_and_completes():
     p.close()
 
 
+def test_short_lane_cap(monkeypatch):
+    """The short lane takes in at most QJX_POOL_SHORT_TRAJ trajectories; a short job
+    above the cap runs when the lane is empty.  Every job completes."""
+    from os_d_amd import qjx
+    _say("case: short-lane trajectory cap")
+    monkeypatch.setenv("QJX_POOL_SHORT_TRAJ", "1000")
+    p = qjx.Pool(1, max_trajectories=1 << 16)
+    sizes = [600, 600, 1500, 600, 600]
+    ids = [p.submit(N=50, num_trajectories=m, n_periods=1, seed=s, steps_per_period=4) for s, m in enumerate(sizes)]
+    peak = 0
+    for _ in range(100000):
+        r, q = p.step()
+        u = p.usage()
+        admitted = u["running"] + u["building"]
+        peak = max(peak, u["trajectories"])
+        # within the cap, or the large job alone
+        assert u["trajectories"] <= 1000 or (u["trajectories"] == 1500 and admitted == 1), u
+        if r == 0 and q == 0 and u["building"] == 0:
+            break
+    res = [p.result(j) for j in ids]
+    _say("peak admitted trajectories %d" % peak)
+    assert all(x["status"] == 2 for x in res)
+    assert peak == 1500
+    p.close()
+
+
 def test_graph_replay_equals_eager(monkeypatch):
     from os_d_amd import qjx
     _say("case: graph vs eager")
