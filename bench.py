@@ -68,10 +68,12 @@ def _stats_delta(s0, s1):
 def headline(qjx, N, M, S, D, trans_periods, warmup, steps, prof_steps, roofline_steps, forms, cpu_budget_s):
     """The precision-form workload, every form timed the same way.
 
-    The FP32 engine runs `trans_periods` transient periods from e_0 (the reference's
+    An FP32 engine runs `trans_periods` transient periods from e_0 (the reference's
     trans_process: no logging, then t := 0).  Its states, etas and RNG states are the
     common starting point of every form, so all of them time the same regime: each
-    form takes `warmup` steps, then `steps` top-level steps are timed on the captured
+    form, on an engine of its own started from that state (the launch shape the
+    transient run tuned is cached, so no tuner trial runs in the timed region), takes
+    `warmup` steps, then `steps` top-level steps are timed on the captured
     iteration graph (as the product runs), then `prof_steps` more steps run with HIP
     events on every launch for the per-kernel figures (`roofline_steps` for FP32,
     whose launches give the roofline).  Observables are taken after the timed steps,
@@ -84,16 +86,14 @@ def headline(qjx, N, M, S, D, trans_periods, warmup, steps, prof_steps, roofline
     out["transient_s"] = time.perf_counter() - t0
     snap = e32.get_state()
     snap.pop("njumps")
+    e32.close()
     names = {qjx.FP32: "fp32", qjx.BF16X3: "bf16x3", qjx.FP64: "fp64"}
     for prec in (qjx.FP32, qjx.BF16X3, qjx.FP64):
         if names[prec] not in forms:
             continue
-        if prec == qjx.FP32:
-            e = e32
-        else:
-            e = make_engine(qjx, prec, N, S, D, M)
-            e.trans(0)
-            e.set_state(**snap)
+        e = make_engine(qjx, prec, N, S, D, M)
+        e.trans(0)
+        e.set_state(**snap)
         if warmup:
             e.run_steps(warmup, 0)
         s0 = e.stats()
@@ -125,8 +125,9 @@ def headline(qjx, N, M, S, D, trans_periods, warmup, steps, prof_steps, roofline
             "gemm_kernel": gemm_kernel_name(prec, p1["gemm_bn"], p1["gemm_waves"]),
             "observables": obs,
             "rng": st["rng"],
-            # the engine's whole life (transient, tuner trials, warmup, timed and
+            # the timed engine's whole life (tuner trials if it tuned, warmup, timed and
             # profiled steps): what a kernel trace or counter pass of the run sees
+            # with no transient period
             "engine_totals": {"matvecs": p1["matvecs"], "gemm_band_flops": p1["gemm_band_flops"],
                               "iterations": p1["iterations"]},
         }

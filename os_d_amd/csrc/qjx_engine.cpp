@@ -1550,16 +1550,18 @@ static int timed_iterations(Runner *r, int iters, double *ms, unsigned long long
     return QJX_OK;
 }
 
-int runner_autotune(Runner *r)
+int runner_autotune(Runner *r, bool again)
 {
-    if (r->tuned || !r->autotune || r->nens == 0) return QJX_OK;
+    if (!r->autotune || r->nens == 0) return QJX_OK;
+    if (r->tuned && (!again || r->cached)) return QJX_OK;
     const ShapeKey key = shape_key(r);
-    {
+    if (!r->tuned) {
         std::lock_guard<std::mutex> lk(g_shape_mu);
         auto it = g_shapes.find(key);
         if (it != g_shapes.end()) {
             runner_apply_shape(r, it->second.bn, it->second.wpc, it->second.waves, it->second.rc);
             r->tuned = true;
+            r->cached = true;
             return QJX_OK;
         }
     }
@@ -1622,12 +1624,19 @@ int runner_autotune(Runner *r)
     const Cand *best = nullptr;
     for (const Cand &c : cand)
         if (c.cols > 0 && (!best || cost(c) < cost(*best))) best = &c;
+    if (std::getenv("QJX_TUNE_VERBOSE")) {
+        for (const Cand &c : cand)
+            std::fprintf(stderr, "qjx tune: bn %d wpc %d waves %d rc %d: %.3f ms, %llu columns, %.4g ms per Gflop%s\n",
+                         c.bn, c.wpc, c.waves, c.rc, c.ms, c.cols, c.work > 0 ? c.ms / (c.work * 128.0 * 1e-9) : 0.0,
+                         &c == best ? "  <- picked" : "");
+    }
     if (!best) {
         runner_apply_shape(r, saved_bn, saved_wpc, saved_waves, saved_rc);
         return QJX_OK;
     }
     runner_apply_shape(r, best->bn, best->wpc, best->waves, best->rc);
     r->tuned = true;
+    r->tunes++;
     std::lock_guard<std::mutex> lk(g_shape_mu);
     g_shapes[key] = {best->bn, best->wpc, best->waves, best->rc};
     return QJX_OK;
@@ -1689,10 +1698,12 @@ int runner_next_total(Runner *r, int *total)
 }
 
 // Iterations a runner runs before it tunes its launch shape (chunks of 8, 16, ...,
-// so tuning starts at the first chunk boundary past this count).  On the d = 1001
-// workload (1e5 trajectories, about 90 iterations per top-level step) tuning after
-// the first 8 iterations picked 128-column tiles for FP32, 3% slower over a period
-// than the 256-column tiles it picks after 512.  QJX_TUNE_AFTER overrides it.
+// so tuning starts at the first chunk boundary past this count), and measures
+// again at 8 and 64 times that count, when the bisection has spread the table over
+// its groups.  On the d = 1001 workload (1e5 trajectories) every trajectory is
+// still at level 0 after 512 iterations from the start state: there the 128- and
+// 256-column FP32 tiles tie, while over a period of the run's later mix the
+// 256-column tiles are 3.5% faster.  QJX_TUNE_AFTER overrides the count.
 static long long tune_after()
 {
     static const long long n = [] {
@@ -1714,12 +1725,18 @@ int runner_run(Runner *r, long long *iterations)
         rc = runner_next_total(r, &total);
         if (rc) return rc;
         if (total == 0) break;
-        if (!r->tuned && r->autotune && r->iterations >= tune_after()) {
-            // shape the GEMM once the bisection has spread the table over its groups:
-            // the first iterations from a fresh start hold mostly shallow levels (wide
-            // bands), where other shapes win than in the run's steady mix of deep ones
+        long long traj = 0;
+        for (const Ens &E : r->h_ens) traj += E.M;
+        if (r->autotune && !r->cached && r->tunes < 3 && r->iterations >= tune_after() << (3 * r->tunes) &&
+            2LL * total >= traj) {
+            // shape the GEMM, and shape it again as the bisection spreads the table
+            // over its groups: the first iterations from a fresh start are level-0
+            // steps of the whole table (wide bands), where other shapes win than in
+            // the run's later mix of deep levels.  Never in the tail of a call, when
+            // most trajectories have reached their target: the few left are not the
+            // run's mix
             const long long before = r->iterations;
-            rc = runner_autotune(r);
+            rc = runner_autotune(r, r->tunes > 0);
             if (rc) return rc;
             done += r->iterations - before;
             rc = runner_next_total(r, &total);

@@ -29,6 +29,8 @@ struct Runner {
     bool tune_rc = false;             // the tuner also picks rc (1 or 2)
     bool autotune = true;             // pick (bn, wg_per_cu) by timing the live table
     bool tuned = false;
+    bool cached = false;              // the shape came from the process-wide cache
+    int tunes = 0;                    // shapes this runner has measured
     int nens = 0;
     std::vector<Ens> h_ens;           // host copy of the table
     bool graphs = true;               // replay a captured pair of iterations
@@ -55,8 +57,10 @@ int runner_run(Runner *r, long long *iterations);
 // Launch-shape tuner: pick the GEMM column tile width and the persistent grid
 // for the current table, from a process-wide cache or by timing the candidates
 // on the table's own iterations (they are real steps; results do not depend on
-// the shape).  No-op once tuned, for the FMA kernel, or with autotune off.
-int runner_autotune(Runner *r);
+// the shape).  No-op once tuned, for the FMA kernel, or with autotune off; with
+// `again`, a runner that measured its shape measures again (the table's mix of
+// groups has changed since) and updates the cache.
+int runner_autotune(Runner *r, bool again = false);
 // Use tile width bn, wg_per_cu workgroups per CU, `waves` waves per GEMM workgroup
 // and (rc > 0) rc row tiles per work unit (drops a captured graph when any changes).
 void runner_apply_shape(Runner *r, int bn, int wg_per_cu, int waves, int rc = 0);
