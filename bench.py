@@ -707,7 +707,7 @@ def main():
     batch = None
     if "batch" in legs and args.batch_steps > 0:
         say("batched uneven ensembles")
-        batch = batched(qjx, S, D, 1, args.batch_steps, qjx.FP32, bool(args.batch_alone))
+        batch = batched(qjx, S, D, 2, args.batch_steps, qjx.FP32, bool(args.batch_alone))
     lind = None
     if "lindblad" in legs:
         say("lindblad")
