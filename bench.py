@@ -151,7 +151,7 @@ def observables_leg(qjx, N, M, S, D, periods, forms):
     share of trajectories whose jump count still equals the FP64 one (equal final
     RNG states).  Once a trajectory's crossing test falls the other way in one form
     its later jumps differ, and the forms' ensembles become independent samples:
-    then the error is judged against the combined standard error."""
+    then the error is compared with the combined standard error."""
     names = {qjx.FP32: "fp32", qjx.BF16X3: "bf16x3", qjx.FP64: "fp64"}
     per, rng, secs = {}, {}, {}
     for prec in (qjx.FP64, qjx.FP32, qjx.BF16X3):
@@ -258,7 +258,7 @@ def observables(e, N, M):
 def observable_errors(form, ref):
     """Error of a reduced-precision run's observables against the FP64 run (same
     workload, seeds and steps; independent jump sequences, so the mean's error is
-    judged against the combined standard error)."""
+    compared with the combined standard error)."""
     if not form or not ref:
         return None
     a, b = form["observables"], ref["observables"]
