@@ -1306,22 +1306,41 @@ __global__ __launch_bounds__(256) void decide_kernel(RunArgs a)
     __shared__ int s_base[kWin + 1];
     __shared__ int s_next;
     __shared__ double s_dt[2][kMaxGroups / 2];   // step sizes of the window's ensembles
-    const int nitems = *a.nens * kMaxGroups;
+    static_assert(kPlanItems >= 256, "first prefix block");
     const Plan *pl = a.plan;
-    const int total = pl->ent_pre[nitems];
-    const int base0 = blockIdx.x * blockDim.x;
-    if (base0 >= total) return;                       // whole workgroup idle
-    for (int i = threadIdx.x; i <= nitems; i += blockDim.x) s_pre[i] = pl->ent_pre[i];
+    const int p = a.parity;
+    // issued together, so that they wait for memory once: the table size, the first
+    // 256 prefix entries (all of them up to three ensembles; the array holds
+    // kPlanItems + 1) and ensemble 0's arrays, which most entries use
+    const int nens = *a.nens;
+    s_pre[threadIdx.x] = pl->ent_pre[threadIdx.x];
+    const Ens &E0 = a.ens[0];
+    const int *const list0 = E0.list[p];
+    TrajState *const st0 = E0.st;
+    const double *const np0 = E0.normpart;
+    const int M0 = E0.M, R0 = E0.R;
+    // what the crossing test reads of the ensemble, held from the same loads
+    struct Fields {
+        const double *dt;
+        double norm_tol;
+        uint64_t stride;
+        int D, S, jump, is_obs, logcap;
+    };
+    const Fields F0 = {E0.dt, E0.norm_tol, E0.stride, E0.D, E0.S, E0.jump, E0.is_obs, E0.log.cap};
+    const int nitems = nens * kMaxGroups;
+    for (int i = threadIdx.x + blockDim.x; i <= nitems; i += blockDim.x) s_pre[i] = pl->ent_pre[i];
     for (int i = threadIdx.x; i < kWin + 3; i += blockDim.x) s_cnt[i] = 0;
     if (threadIdx.x == 0) s_next = 0;
     __syncthreads();
-    const int p = a.parity;
+    const int total = s_pre[nitems];
+    const int base0 = blockIdx.x * blockDim.x;
+    if (base0 >= total) return;                       // whole workgroup idle
     const int idx = base0 + threadIdx.x;
     const int e_lo = plan_search(s_pre, nitems, base0) / kMaxGroups;
     if (threadIdx.x < kMaxGroups) {
         // read while the entries' states are in flight; used after the barrier below
         const int w = threadIdx.x / (kMaxGroups / 2), l = threadIdx.x % (kMaxGroups / 2);
-        if (e_lo + w < *a.nens) {
+        if (e_lo + w < nens) {
             const Ens &W = a.ens[e_lo + w];
             if (l < W.D) s_dt[w][l] = W.dt[l];
         }
@@ -1329,23 +1348,36 @@ __global__ __launch_bounds__(256) void decide_kernel(RunArgs a)
     int e = -1, tr = -1, key = -1, local = -1, direct = -1, jlocal = -1, jsrc = 0, ncur = 0;
     double n = 0.0;
     TrajState s;
+    Fields F = F0;
+    const int *list = list0;
+    TrajState *st = st0;
+    const double *npart = np0;
+    int M = M0, R = R0;
     if (idx < total) {
         const int item = plan_search(s_pre, nitems, idx);
         e = item / kMaxGroups;
         const int g = item % kMaxGroups;
-        const Ens &E = a.ens[e];
-        tr = E.list[p][(size_t)g * E.M + (idx - s_pre[item])] >> 1;
-        s = E.st[tr];
+        if (e != 0) {
+            const Ens &E = a.ens[e];
+            list = E.list[p];
+            st = E.st;
+            npart = E.normpart;
+            M = E.M;
+            R = E.R;
+            F = {E.dt, E.norm_tol, E.stride, E.D, E.S, E.jump, E.is_obs, E.log.cap};
+        }
+        tr = list[(size_t)g * M + (idx - s_pre[item])] >> 1;
+        s = st[tr];
         // the R partials (contiguous) are loaded together, then added in ascending
         // row-tile order
-        const double *np = E.normpart + (size_t)tr * E.R;
-        for (int r0 = 0; r0 < E.R; r0 += 16) {
+        const double *np = npart + (size_t)tr * R;
+        for (int r0 = 0; r0 < R; r0 += 16) {
             double v[16];
 #pragma unroll
-            for (int q = 0; q < 16; q++) v[q] = r0 + q < E.R ? np[r0 + q] : 0.0;
+            for (int q = 0; q < 16; q++) v[q] = r0 + q < R ? np[r0 + q] : 0.0;
 #pragma unroll
             for (int q = 0; q < 16; q++)
-                if (r0 + q < E.R) n += v[q];
+                if (r0 + q < R) n += v[q];
         }
     }
     __syncthreads();
@@ -1353,12 +1385,12 @@ __global__ __launch_bounds__(256) void decide_kernel(RunArgs a)
         const Ens &E = a.ens[e];
         const bool win = e - e_lo < 2;
         const double *wdt = s_dt[win ? e - e_lo : 0];
-        const double *gdt = E.dt;
+        const double *gdt = F.dt;
         auto dt_at = [&](int l) { return win ? wdt[l] : gdt[l]; };
         const int lvl = s.lvl;
         const double dt = dt_at(lvl);
         bool alive = true;
-        if (n > 1.0 + E.norm_tol) {
+        if (n > 1.0 + F.norm_tol) {
             // experiment.cpp:1247-1251 throws: record the first offender, stop it.
             if (atomicCAS(&E.err[0], 0, 1) == 0) {
                 E.err[1] = tr;
@@ -1369,10 +1401,10 @@ __global__ __launch_bounds__(256) void decide_kernel(RunArgs a)
             alive = false;
         } else {
             const bool crossed = !(n > s.eta);              // experiment.cpp:1253-1260
-            if (lvl == E.D - 1) {
+            if (lvl == F.D - 1) {
                 if (crossed) {
-                    if (E.jump > 0 && E.is_obs == 1) {
-                        if (s.logn < E.log.cap) {
+                    if (F.jump > 0 && F.is_obs == 1) {
+                        if (s.logn < F.logcap) {
                             const size_t o = (size_t)tr * E.log.cap + s.logn;
                             E.log.time[o] = s.t + dt;
                             E.log.norm[o] = n;
@@ -1384,9 +1416,9 @@ __global__ __launch_bounds__(256) void decide_kernel(RunArgs a)
                     }
                     // recovery draws `ran` (one channel, so channel 0 is chosen),
                     // then eta is redrawn while it is 0 (qj_proc.cpp:310-314).
-                    (void)mcg_next(s.rng, E.stride);
-                    double ev = mcg_next(s.rng, E.stride);
-                    while (ev == 0.0) ev = mcg_next(s.rng, E.stride);
+                    (void)mcg_next(s.rng, F.stride);
+                    double ev = mcg_next(s.rng, F.stride);
+                    while (ev == 0.0) ev = mcg_next(s.rng, F.stride);
                     s.eta = ev;
                     // the jump kernel reads the step just taken (slot cur ^ 1) and
                     // writes the jumped state over the pre-step state (slot cur)
@@ -1398,19 +1430,19 @@ __global__ __launch_bounds__(256) void decide_kernel(RunArgs a)
                     s.cur ^= 1;
                 }
                 s.t += dt;
-                complete_step(s, lvl, dt_at, E.S);
+                complete_step(s, lvl, dt_at, F.S);
             } else if (crossed) {
                 s.lvl = lvl + 1;          // bisect from the pre-step state
             } else {
                 s.cur ^= 1;
                 s.t += dt;
-                complete_step(s, lvl, dt_at, E.S);
+                complete_step(s, lvl, dt_at, F.S);
             }
         }
-        E.st[tr] = s;
+        st[tr] = s;
         ncur = s.cur;            // the jump kernel writes into this slot, too
         if (alive && s.pos < s.target) {
-            const int gn = s.b * E.D + s.lvl;
+            const int gn = s.b * F.D + s.lvl;
             key = (e - e_lo) * kMaxGroups + gn;
             atomicAdd(&s_next, 1);
             if (e - e_lo < 2) local = atomicAdd(&s_cnt[key], 1);
@@ -1418,7 +1450,6 @@ __global__ __launch_bounds__(256) void decide_kernel(RunArgs a)
         }
     }
     __syncthreads();
-    const int nens = *a.nens;
     for (int t = threadIdx.x; t < kWin; t += blockDim.x) {
         const int c = s_cnt[t];
         const int ee = e_lo + t / kMaxGroups;
