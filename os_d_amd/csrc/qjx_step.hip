@@ -80,27 +80,32 @@ __global__ __launch_bounds__(kPlanThreads) void plan_kernel(RunArgs a)
     __shared__ int s_t[kPlanThreads];
     __shared__ int s_e[kPlanThreads];
     const int tid = threadIdx.x;
-    const int nens = *a.nens;
-    const int nitems = nens * kMaxGroups;
     const int p = a.parity;
-    int tl[4], el[4], tsum = 0, esum = 0;
     const int item0 = 4 * tid;
     const int e = item0 / kMaxGroups;
+    static_assert(kPlanItems / kMaxGroups <= kMaxEns, "plan ensembles");
+    // the thread's ensemble record is loaded with the table size (the table holds
+    // kMaxEns records; those past the size are not used)
+    const Ens &E = a.ens[e];
+    const int G = E.G, R = E.R;
+    int *const cnt_cur = E.cnt[p], *const cnt_next = E.cnt[p ^ 1];
+    const int nens = *a.nens;
+    const int nitems = nens * kMaxGroups;
+    int tl[4], el[4], tsum = 0, esum = 0;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         tl[q] = el[q] = 0;
         if (item0 + q < nitems) {
             const int g = (item0 + q) % kMaxGroups;
-            const Ens &E = a.ens[e];
-            const int c = g < E.G ? E.cnt[p][g] : 0;
+            const int c = g < G ? cnt_cur[g] : 0;
             el[q] = c;
-            tl[q] = ((c + a.bn - 1) / a.bn) * ((E.R + a.rc - 1) / a.rc);   // work units: column tiles x row chunks
-            E.cnt[p ^ 1][g] = 0;      // the decide kernel of this iteration appends here
+            tl[q] = ((c + a.bn - 1) / a.bn) * ((R + a.rc - 1) / a.rc);   // work units: column tiles x row chunks
+            cnt_next[g] = 0;          // the decide kernel of this iteration appends here
         }
         tsum += tl[q];
         esum += el[q];
     }
-    if (esum > 0) atomicAdd(&a.ens[e].counters[0], (unsigned long long)esum);
+    if (esum > 0) atomicAdd(&E.counters[0], (unsigned long long)esum);
     s_t[tid] = tsum;
     s_e[tid] = esum;
     __syncthreads();
@@ -1499,16 +1504,34 @@ __global__ __launch_bounds__(kJumpThreads) void jump_kernel(RunArgs a, int jdim)
     extern __shared__ double jump_lds[];
     double *pre = jump_lds, *pim = jump_lds + (jdim + 2);
     __shared__ double red[kJumpThreads / 64];
+    // issued together, so that a workgroup's first jump waits for memory once more
+    // before its state loads: the jump count, the workgroup's first entry (in bounds:
+    // the grid is never larger than the list's capacity) and ensemble 0's record
     const int njump = *a.jcnt;
+    JumpEnt je = a.jl[blockIdx.x];
+    const Ens &E0 = a.ens[0];
+    const int d0 = E0.d, dp0 = E0.dp, L0 = E0.L;
+    const double *const Ld0 = E0.Ldiag, *const Lo0 = E0.Loff;
+    char *const sl00 = static_cast<char *>(E0.slot[0]), *const sl01 = static_cast<char *>(E0.slot[1]);
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     for (int j = blockIdx.x; j < njump; j += gridDim.x) {
-        const JumpEnt je = a.jl[j];
-        const Ens &E = a.ens[je.ens];
-        const int d = E.d, dp = E.dp;
-        const double *Ldiag = E.Ldiag, *Loff = E.Loff;
+        if (j != (int)blockIdx.x) je = a.jl[j];
+        int d = d0, dp = dp0, L = L0;
+        const double *Ldiag = Ld0, *Loff = Lo0;
+        char *s0 = sl00, *s1 = sl01;
+        if (je.ens != 0) {
+            const Ens &E = a.ens[je.ens];
+            d = E.d;
+            dp = E.dp;
+            L = E.L;
+            Ldiag = E.Ldiag;
+            Loff = E.Loff;
+            s0 = static_cast<char *>(E.slot[0]);
+            s1 = static_cast<char *>(E.slot[1]);
+        }
         const double sn = sqrt(je.n);
-        const T *x = reinterpret_cast<const T *>(E.slot[je.src]) + (size_t)je.tr * E.L;
-        T *y = reinterpret_cast<T *>(E.slot[je.src ^ 1]) + (size_t)je.tr * E.L;
+        const T *x = reinterpret_cast<const T *>(je.src ? s1 : s0) + (size_t)je.tr * L;
+        T *y = reinterpret_cast<T *>(je.src ? s0 : s1) + (size_t)je.tr * L;
         T xr[PER], xi[PER];
         double ld[PER], lm[PER], lp[PER];    // L_kk, Loff[k - 1], Loff[k]
 #pragma unroll
