@@ -1376,13 +1376,29 @@ __global__ __launch_bounds__(256) void decide_kernel(RunArgs a)
         // the R partials (contiguous) are loaded together, then added in ascending
         // row-tile order
         const double *np = npart + (size_t)tr * R;
-        for (int r0 = 0; r0 < R; r0 += 16) {
-            double v[16];
+        if ((R & 1) == 0) {
+            // even R: the partials start 16-byte aligned and come in pairs
+            for (int r0 = 0; r0 < R; r0 += 16) {
+                double2 v[8];
 #pragma unroll
-            for (int q = 0; q < 16; q++) v[q] = r0 + q < R ? np[r0 + q] : 0.0;
+                for (int q = 0; q < 8; q++)
+                    v[q] = r0 + 2 * q < R ? *reinterpret_cast<const double2 *>(np + r0 + 2 * q) : make_double2(0.0, 0.0);
 #pragma unroll
-            for (int q = 0; q < 16; q++)
-                if (r0 + q < R) n += v[q];
+                for (int q = 0; q < 8; q++)
+                    if (r0 + 2 * q < R) {
+                        n += v[q].x;
+                        n += v[q].y;
+                    }
+            }
+        } else {
+            for (int r0 = 0; r0 < R; r0 += 16) {
+                double v[16];
+#pragma unroll
+                for (int q = 0; q < 16; q++) v[q] = r0 + q < R ? np[r0 + q] : 0.0;
+#pragma unroll
+                for (int q = 0; q < 16; q++)
+                    if (r0 + q < R) n += v[q];
+            }
         }
     }
     __syncthreads();
