@@ -22,10 +22,14 @@ def _say(msg):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("N,steps,periods", [(20, 200, 2), (64, 2000, 1)])
-def test_gpu_lindblad_matches_oracle_bitwise(oracle_mod, N, steps, periods):
+# the step kernel's 8 x 8 and 16 x 16 tiles: d = 21 and 65 end in partial tiles (65:
+# a one-row tile), d = 32 is whole tiles of both sizes
+@pytest.mark.parametrize("tile", ["8", "16"])
+@pytest.mark.parametrize("N,steps,periods", [(20, 200, 2), (64, 2000, 1), (31, 400, 1)])
+def test_gpu_lindblad_matches_oracle_bitwise(oracle_mod, N, steps, periods, tile, monkeypatch):
     from os_d_amd import qjx
-    _say("case: lindblad N=%d steps=%d periods=%d" % (N, steps, periods))
+    monkeypatch.setenv("QJX_LB_TILE", tile)
+    _say("case: lindblad N=%d steps=%d periods=%d tile=%s" % (N, steps, periods, tile))
     m = oracle_mod.model(N, gamma=0.3, **PARAMS)
     lb = qjx.Lindblad(N, steps_per_period=steps, diss_gamma=0.3, **PARAMS)
     rho0 = np.zeros((N + 1, N + 1), complex)
@@ -37,6 +41,23 @@ def test_gpu_lindblad_matches_oracle_bitwise(oracle_mod, N, steps, periods):
     diff = np.abs(got - ref).max()
     _say("max |rho_gpu - rho_oracle| = %.3e" % diff)
     assert np.array_equal(got, ref)
+    lb.close()
+
+
+def test_gpu_lindblad_eager_matches_oracle_bitwise(oracle_mod, monkeypatch):
+    """Eager launches (no graph): the step kernels alternate rho between two buffers
+    launch by launch; rho after the period equals the oracle's bit for bit."""
+    from os_d_amd import qjx
+    monkeypatch.setenv("QJX_GRAPH", "0")
+    N, steps = 40, 200
+    _say("case: lindblad eager N=%d steps=%d" % (N, steps))
+    m = oracle_mod.model(N, gamma=0.3, **PARAMS)
+    lb = qjx.Lindblad(N, steps_per_period=steps, diss_gamma=0.3, **PARAMS)
+    rho0 = np.zeros((N + 1, N + 1), complex)
+    rho0[0, 0] = 1.0
+    lb.run_periods(1)
+    ref = oracle_mod.lindblad_evolve(m, rho0, 1, steps_per_period=steps)
+    assert np.array_equal(lb.get_rho(), ref)
     lb.close()
 
 
