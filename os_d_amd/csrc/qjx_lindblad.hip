@@ -183,95 +183,109 @@ __device__ inline double stage_drive(const LDrive &D, int j, int stage)
     return D.E0 + D.A * f;
 }
 
-// One RK4 step, all four stages in one launch: workgroup (ti, tj) owns the T x T
-// tile of rho at (i0, j0) = (T ti, T tj).  A stage's right-hand side reads its
-// argument up to two entries away, so stage s (0..3) is evaluated on the tile
-// widened by 2 (3 - s) on every side, and the step reads rho on the tile widened
-// by 8.  Everything between the read of rho and the write of the new tile stays in
-// LDS, the band coefficients of the window's rows and columns included; the
-// neighbouring tiles' stage values are recomputed, not exchanged (2x the stencil
-// work at T = 16, 3.4x at T = 8, which gives small d four times the workgroups).
-// Per entry:
+// K RK4 steps (K = 1 or 2), all their stages in one launch: workgroup (ti, tj) owns
+// the T x T tile of rho at (i0, j0) = (T ti, T tj).  A stage's right-hand side
+// reads its argument up to two entries away, so a step shrinks the window it can
+// evaluate by 8 on every side: the launch reads rho on the tile widened by 8K, and
+// step s (0..K-1) evaluates its stages on the window widened by 8(K - s) - 2,
+// - 4, - 6 and its new rho on the window widened by 8(K - 1 - s), over the old rho
+// in LDS (each entry's update reads the old rho at that entry only).  Everything
+// between the read of rho and the write of the new tile stays in LDS, the band
+// coefficients of the window's rows and columns included; the neighbouring tiles'
+// values are recomputed, not exchanged (2x the stencil work at T = 16 and K = 1,
+// 3.4x at T = 8 and K = 1, which gives small d four times the workgroups).  Per
+// entry and step:
 //   k1 = f(rho), v1 = rho + (h/2) k1;  k2 = f(v1), v2 = rho + (h/2) k2;
 //   k3 = f(v2), v3 = rho + h k3;       k4 = f(v3);
 //   rho' = rho + (h/6) (((k1 + 2 k2) + 2 k3) + k4)
 // the operations of the CPU restatement in its order.  rho is read from `in` and
-// written to `out` (other workgroups still read `in`'s halo).  NT threads: one
-// evaluation each per stage.
-constexpr int kHalo = 8;                     // 2 per stage
-
+// the tile written to `out` (other workgroups still read `in`'s halo).  NT threads.
 // a stage argument held in LDS over a square window: entry (r, c) at
 // buf[(r - r0) * w + (c - c0)]
 struct LdsView {
-    const c2 *buf;
+    c2 *buf;
     int r0, c0, w;
-    __device__ c2 operator()(int r, int c) const { return buf[(r - r0) * w + (c - c0)]; }
+    __device__ c2 &operator()(int r, int c) const { return buf[(r - r0) * w + (c - c0)]; }
 };
 
-template <int T, int NT>
+template <int T, int NT, int K>
 __global__ __launch_bounds__(NT) void rk4_step_kernel(LBands B, LDrive D, int j, const c2 *in, c2 *out, double h)
 {
-    constexpr int W0 = T + 2 * kHalo;        // rho and the bands
-    constexpr int W1 = T + 12;               // v1 (and v3, on the inner T + 4)
-    constexpr int W2 = T + 8;                // v2
-    static_assert(W1 * W1 <= NT && 2 * kBands * W0 <= NT, "one evaluation per thread");
+    constexpr int H = 8 * K;                 // halo of the launch
+    constexpr int W0 = T + 2 * H;            // rho and the bands
+    constexpr int WA = W0 - 4;               // v1 (and v3) of the first step
+    constexpr int WB = W0 - 8;               // v2 of the first step
+    constexpr int WF = W0 - 16;              // the first step's new rho and combination
+    static_assert(2 * kBands * W0 <= NT, "band load");
     __shared__ c2 s_rho[W0 * W0];
-    __shared__ c2 s_a[W1 * W1];
-    __shared__ c2 s_b[W2 * W2];
-    __shared__ c2 s_acc[T * T];
+    __shared__ c2 s_a[WA * WA];
+    __shared__ c2 s_b[WB * WB];
+    __shared__ c2 s_acc[WF * WF];
     __shared__ double s_brow[kBands * W0], s_bcol[kBands * W0];
     const int d = B.d, t = threadIdx.x;
     const int i0 = blockIdx.y * T, j0 = blockIdx.x * T;
     const double c2h = 0.5 * h, h6 = h / 6.0;
     // rho on the widened tile (entries outside 0..d-1 are never read)
     for (int q = t; q < W0 * W0; q += NT) {
-        const int r = i0 - kHalo + q / W0, c = j0 - kHalo + q % W0;
+        const int r = i0 - H + q / W0, c = j0 - H + q % W0;
         s_rho[q] = (r >= 0 && r < d && c >= 0 && c < d) ? in[(size_t)r * d + c] : make_double2(0.0, 0.0);
     }
     // the bands at the window's row and column indices
     if (t < 2 * kBands * W0) {
         const int col = t >= kBands * W0, u = t % (kBands * W0);
-        const int b = u / W0, k = (col ? j0 : i0) - kHalo + u % W0;
+        const int b = u / W0, k = (col ? j0 : i0) - H + u % W0;
         const double *src[kBands] = {B.hd, B.hdrv, B.ho, B.ld, B.lo, B.q0, B.q1, B.q2};
         (col ? s_bcol : s_brow)[u] = (k >= 0 && k < d) ? src[b][k] : 0.0;
     }
-    const BandView R{s_brow, i0 - kHalo, W0}, C{s_bcol, j0 - kHalo, W0};
-    const LdsView rho{s_rho, i0 - kHalo, j0 - kHalo, W0};
-    const LdsView v1{s_a, i0 - 6, j0 - 6, W1};
-    const LdsView v2{s_b, i0 - 4, j0 - 4, W2};
-    const LdsView v3{s_a, i0 - 2, j0 - 2, T + 4};
-    double E[4];
+    const BandView R{s_brow, i0 - H, W0}, C{s_bcol, j0 - H, W0};
+    const LdsView rho{s_rho, i0 - H, j0 - H, W0};
+    double E[K][4];
 #pragma unroll
-    for (int st = 0; st < 4; st++) E[st] = stage_drive(D, j, st);
+    for (int s = 0; s < K; s++)
+#pragma unroll
+        for (int st = 0; st < 4; st++) E[s][st] = stage_drive(D, j + s, st);
     __syncthreads();
-    // stages 0..2: v = rho + c k over the window widened by w on each side, and the
-    // running combination on the tile itself
-    auto stage = [&](const auto &x, double Es, double c, c2 *vout, int w, int st) {
-        const int n = T + 2 * w;
-        for (int q = t; q < n * n; q += NT) {
-            const int r = i0 - w + q / n, cc = j0 - w + q % n;
-            if (r < 0 || r >= d || cc < 0 || cc >= d) continue;
-            const c2 k = rhs_at(R, C, d, B.g, Es, x, r, cc);
-            vout[q] = cadd(rho(r, cc), rmul(c, k));
-            const int ti = r - i0, tj = cc - j0;
-            if (ti >= 0 && ti < T && tj >= 0 && tj < T) {
-                c2 &acc = s_acc[ti * T + tj];
-                acc = st == 0 ? k : cadd(acc, rmul(2.0, k));
+#pragma unroll
+    for (int s = 0; s < K; s++) {
+        const int g = 8 * (K - s);           // this step's input window: the tile widened by g
+        const int wf = T + 2 * (g - 8);      // its output window
+        const LdsView acc{s_acc, i0 - (g - 8), j0 - (g - 8), wf};
+        // stages 0..2: v = rho + c k over the tile widened by w, and the running
+        // combination on the output window
+        auto stage = [&](const LdsView &x, double Es, double c, const LdsView &v, int w, int st) {
+            const int n = T + 2 * w;
+            for (int q = t; q < n * n; q += NT) {
+                const int r = i0 - w + q / n, cc = j0 - w + q % n;
+                if (r < 0 || r >= d || cc < 0 || cc >= d) continue;
+                const c2 k = rhs_at(R, C, d, B.g, Es, x, r, cc);
+                v(r, cc) = cadd(rho(r, cc), rmul(c, k));
+                const int ai = r - (i0 - (g - 8)), aj = cc - (j0 - (g - 8));
+                if (ai >= 0 && ai < wf && aj >= 0 && aj < wf) {
+                    c2 &a = acc(r, cc);
+                    a = st == 0 ? k : cadd(a, rmul(2.0, k));
+                }
             }
+        };
+        const LdsView v1{s_a, i0 - (g - 2), j0 - (g - 2), T + 2 * (g - 2)};
+        const LdsView v2{s_b, i0 - (g - 4), j0 - (g - 4), T + 2 * (g - 4)};
+        const LdsView v3{s_a, i0 - (g - 6), j0 - (g - 6), T + 2 * (g - 6)};
+        stage(rho, E[s][0], c2h, v1, g - 2, 0);
+        __syncthreads();
+        stage(v1, E[s][1], c2h, v2, g - 4, 1);
+        __syncthreads();
+        stage(v2, E[s][2], h, v3, g - 6, 2);     // v3 over v1's buffer (v1 is no longer read)
+        __syncthreads();
+        const int w = g - 8;
+        for (int q = t; q < wf * wf; q += NT) {
+            const int r = i0 - w + q / wf, cc = j0 - w + q % wf;
+            if (r < 0 || r >= d || cc < 0 || cc >= d) continue;
+            const c2 k = rhs_at(R, C, d, B.g, E[s][3], v3, r, cc);
+            const c2 sum = cadd(acc(r, cc), k);
+            const c2 nr = cadd(rho(r, cc), rmul(h6, sum));
+            if (s == K - 1) out[(size_t)r * d + cc] = nr;
+            else rho(r, cc) = nr;
         }
-    };
-    stage(rho, E[0], c2h, s_a, 6, 0);
-    __syncthreads();
-    stage(v1, E[1], c2h, s_b, 4, 1);
-    __syncthreads();
-    stage(v2, E[2], h, s_a, 2, 2);           // v3 over v1's buffer (v1 is no longer read)
-    __syncthreads();
-    for (int q = t; q < T * T; q += NT) {
-        const int r = i0 + q / T, cc = j0 + q % T;
-        if (r >= d || cc >= d) continue;
-        const c2 k = rhs_at(R, C, d, B.g, E[3], v3, r, cc);
-        const c2 sum = cadd(s_acc[q], k);
-        out[(size_t)r * d + cc] = cadd(rho(r, cc), rmul(h6, sum));
+        if (s < K - 1) __syncthreads();
     }
 }
 
@@ -338,13 +352,14 @@ struct qjx_lindblad {
     qjx::LDrive D{};
     bool graphs = true;
     int tile = 16;                    // step kernel tile edge: 8 or 16
+    int fuse = 1;                     // RK4 steps per launch with 8 x 8 tiles: 1 or 2
     hipGraphExec_t exec = nullptr;    // kChunk consecutive RK4 steps and the counter advance
     hipGraph_t graph = nullptr;
 };
 
 namespace {
-constexpr int kChunk = 50;            // RK4 steps per graph launch
-static_assert(kChunk % 2 == 0, "a chunk leaves rho in its own buffer");
+constexpr int kChunk = 48;            // RK4 steps per graph launch
+static_assert(kChunk % 4 == 0, "a chunk has an even number of launches (one or two steps each)");
 }
 
 namespace {
@@ -398,23 +413,31 @@ void build_bands(const qjx_model_desc *m, int d, std::vector<double> &h)
         }
 }
 
-// n consecutive RK4 steps (n even), then the counter advances by n.  The step
-// kernel reads the drive's step from the counter; rho alternates between the two
-// buffers and is back in the first after the even count.
-void launch_steps(qjx_lindblad *l, int n, hipStream_t s)
+// n consecutive RK4 steps, then the counter advances by n.  The step kernel reads
+// the drive's step from the counter.  Each launch writes rho to the other buffer;
+// after an odd number of launches rho is copied back, so it always ends in its own
+// buffer (a captured chunk has an even number of launches and no copy).
+int launch_steps(qjx_lindblad *l, int n, hipStream_t s)
 {
     const size_t sq = nsq(l);
     double2 *rho = l->buf, *alt = rho + sq;
     const int T = l->tile;
     const int tiles = (l->d + T - 1) / T;
     const dim3 grid(tiles, tiles);
-    for (int j = 0; j < n; j++) {
-        const double2 *in = (j & 1) ? alt : rho;
-        double2 *out = (j & 1) ? rho : alt;
-        if (T == 8) qjx::rk4_step_kernel<8, 512><<<grid, 512, 0, s>>>(l->B, l->D, j, in, out, l->h);
-        else qjx::rk4_step_kernel<16, 1024><<<grid, 1024, 0, s>>>(l->B, l->D, j, in, out, l->h);
+    int launches = 0;
+    for (int j = 0; j < n;) {
+        const double2 *in = (launches & 1) ? alt : rho;
+        double2 *out = (launches & 1) ? rho : alt;
+        const int k = (T == 8 && l->fuse == 2 && n - j >= 2) ? 2 : 1;
+        if (k == 2) qjx::rk4_step_kernel<8, 1024, 2><<<grid, 1024, 0, s>>>(l->B, l->D, j, in, out, l->h);
+        else if (T == 8) qjx::rk4_step_kernel<8, 512, 1><<<grid, 512, 0, s>>>(l->B, l->D, j, in, out, l->h);
+        else qjx::rk4_step_kernel<16, 1024, 1><<<grid, 1024, 0, s>>>(l->B, l->D, j, in, out, l->h);
+        j += k;
+        launches++;
     }
+    if (launches & 1) LB_TRY(hipMemcpyAsync(rho, alt, sq * sizeof(double2), hipMemcpyDeviceToDevice, s));
     qjx::lb_advance_kernel<<<1, 1, 0, s>>>(l->d_step, n);
+    return QJX_OK;
 }
 
 }  // namespace
@@ -464,6 +487,11 @@ extern "C" int qjx_lindblad_create_drive(const qjx_model_desc *m, int steps_per_
         const std::string v(env);
         if (v == "8" || v == "16") l->tile = std::stoi(v);
     }
+    // QJX_LB_FUSE=2: with 8 x 8 tiles, two RK4 steps per launch (halo 16).  At d = 101
+    // it measured 14% slower than one step per launch: the recomputed halo work
+    // (5.3x) costs more than the launch and the load of rho it saves
+    if (const char *env = std::getenv("QJX_LB_FUSE"))
+        if (std::string(env) == "2") l->fuse = 2;
     auto bail = [&](int code) {
         qjx_lindblad_destroy(l);
         return code;
@@ -531,7 +559,7 @@ extern "C" int qjx_lindblad_run_periods(qjx_lindblad *l, int n_periods)
     if (!l) return lb_fail(QJX_ERR_ARG, "null argument");
     if (n_periods < 0) return lb_fail(QJX_ERR_ARG, "negative period count");
     if (n_periods == 0) return QJX_OK;
-    // kChunk consecutive RK4 steps per graph launch (the stage kernels read the step
+    // kChunk consecutive RK4 steps per graph launch (the step kernel reads the step
     // from the device counter), the remainder launched eagerly; every launch checked
     const long long total = (long long)n_periods * l->steps;
     long long done = 0;
@@ -539,8 +567,8 @@ extern "C" int qjx_lindblad_run_periods(qjx_lindblad *l, int n_periods)
         if (!l->exec) {
             LB_TRY(hipGetLastError());             // an earlier launch error, as itself
             LB_TRY(hipStreamBeginCapture(l->stream, hipStreamCaptureModeThreadLocal));
-            launch_steps(l, kChunk, l->stream);
-            const hipError_t le = hipGetLastError();
+            const int lr = launch_steps(l, kChunk, l->stream);   // an even number of launches: no copy
+            const hipError_t le = lr == QJX_OK ? hipGetLastError() : hipErrorUnknown;
             hipGraph_t g = nullptr;
             const hipError_t ee = hipStreamEndCapture(l->stream, &g);   // always ends the capture
             if (le != hipSuccess || ee != hipSuccess) {
@@ -566,7 +594,7 @@ extern "C" int qjx_lindblad_run_periods(qjx_lindblad *l, int n_periods)
     }
     for (; done < total; done += kChunk) {
         const int n = (int)std::min<long long>(kChunk, total - done);
-        launch_steps(l, n, l->stream);
+        if (const int rc = launch_steps(l, n, l->stream)) return rc;
         LB_TRY(hipGetLastError());
     }
     LB_TRY(hipStreamSynchronize(l->stream));

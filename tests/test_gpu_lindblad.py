@@ -22,14 +22,16 @@ def _say(msg):
     sys.stdout.flush()
 
 
-# the step kernel's 8 x 8 and 16 x 16 tiles: d = 21 and 65 end in partial tiles (65:
-# a one-row tile), d = 32 is whole tiles of both sizes
-@pytest.mark.parametrize("tile", ["8", "16"])
+# the step kernel's shapes (tile edge x RK4 steps per launch): d = 21 and 65 end in
+# partial tiles (65: a one-row tile), d = 32 is whole tiles of both sizes
+@pytest.mark.parametrize("shape", ["8x2", "8x1", "16x1"])
 @pytest.mark.parametrize("N,steps,periods", [(20, 200, 2), (64, 2000, 1), (31, 400, 1)])
-def test_gpu_lindblad_matches_oracle_bitwise(oracle_mod, N, steps, periods, tile, monkeypatch):
+def test_gpu_lindblad_matches_oracle_bitwise(oracle_mod, N, steps, periods, shape, monkeypatch):
     from os_d_amd import qjx
+    tile, fuse = shape.split("x")
     monkeypatch.setenv("QJX_LB_TILE", tile)
-    _say("case: lindblad N=%d steps=%d periods=%d tile=%s" % (N, steps, periods, tile))
+    monkeypatch.setenv("QJX_LB_FUSE", fuse)
+    _say("case: lindblad N=%d steps=%d periods=%d shape=%s" % (N, steps, periods, shape))
     m = oracle_mod.model(N, gamma=0.3, **PARAMS)
     lb = qjx.Lindblad(N, steps_per_period=steps, diss_gamma=0.3, **PARAMS)
     rho0 = np.zeros((N + 1, N + 1), complex)
@@ -46,10 +48,12 @@ def test_gpu_lindblad_matches_oracle_bitwise(oracle_mod, N, steps, periods, tile
 
 def test_gpu_lindblad_eager_matches_oracle_bitwise(oracle_mod, monkeypatch):
     """Eager launches (no graph): the step kernels alternate rho between two buffers
-    launch by launch; rho after the period equals the oracle's bit for bit."""
+    launch by launch, and a sequence with an odd number of launches (the last 10 of
+    202 steps, two per launch) copies it back; rho after the period equals the
+    oracle's bit for bit."""
     from os_d_amd import qjx
     monkeypatch.setenv("QJX_GRAPH", "0")
-    N, steps = 40, 200
+    N, steps = 40, 202
     _say("case: lindblad eager N=%d steps=%d" % (N, steps))
     m = oracle_mod.model(N, gamma=0.3, **PARAMS)
     lb = qjx.Lindblad(N, steps_per_period=steps, diss_gamma=0.3, **PARAMS)
