@@ -1550,21 +1550,27 @@ static int timed_iterations(Runner *r, int iters, double *ms, unsigned long long
     return QJX_OK;
 }
 
+// A shape this process has already tuned for the runner's table shape and form:
+// applied, and the runner does not tune.
+static bool apply_cached_shape(Runner *r)
+{
+    if (!r->autotune || r->tuned || r->nens == 0) return false;
+    const ShapeKey key = shape_key(r);
+    std::lock_guard<std::mutex> lk(g_shape_mu);
+    auto it = g_shapes.find(key);
+    if (it == g_shapes.end()) return false;
+    runner_apply_shape(r, it->second.bn, it->second.wpc, it->second.waves, it->second.rc);
+    r->tuned = true;
+    r->cached = true;
+    return true;
+}
+
 int runner_autotune(Runner *r, bool again)
 {
     if (!r->autotune || r->nens == 0) return QJX_OK;
     if (r->tuned && (!again || r->cached)) return QJX_OK;
+    if (apply_cached_shape(r)) return QJX_OK;
     const ShapeKey key = shape_key(r);
-    if (!r->tuned) {
-        std::lock_guard<std::mutex> lk(g_shape_mu);
-        auto it = g_shapes.find(key);
-        if (it != g_shapes.end()) {
-            runner_apply_shape(r, it->second.bn, it->second.wpc, it->second.waves, it->second.rc);
-            r->tuned = true;
-            r->cached = true;
-            return QJX_OK;
-        }
-    }
     struct Cand {
         int bn, wpc, waves, rc;
         double ms;
@@ -1717,6 +1723,9 @@ int runner_run(Runner *r, long long *iterations)
 {
     int chunk = 8;
     long long done = 0;
+    // a table shape tuned before in this process runs its shape from the first
+    // iteration on
+    (void)apply_cached_shape(r);
     for (;;) {
         int rc = runner_chunk(r, chunk);
         if (rc) return rc;
